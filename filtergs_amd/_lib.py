@@ -1,0 +1,180 @@
+"""ctypes binding of libfiltergs_amd.so (the gfx950 C ABI, include/filtergs_amd.h).
+
+torch supplies device memory and the current HIP stream; every pointer handed
+to the library is ``tensor.data_ptr()`` of a contiguous float32/int tensor on
+the GPU.  There is no CPU fallback: when the library (or a GPU) is missing the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfiltergs_amd.so")
+_lib = None
+
+c_void_p = ctypes.c_void_p
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_float = ctypes.c_float
+c_size_t = ctypes.c_size_t
+
+FGS_VARIANT_WODILATE = 0
+FGS_VARIANT_DILATE = 1
+FGS_GRAD_REFERENCE = 0
+FGS_GRAD_EXACT = 1
+
+
+class RasterSettings(ctypes.Structure):
+    """fgs_raster_settings."""
+    _fields_ = [
+        ("image_height", c_int32),
+        ("image_width", c_int32),
+        ("tanfovx", c_float),
+        ("tanfovy", c_float),
+        ("bg", c_void_p),
+        ("scale_modifier", c_float),
+        ("viewmatrix", c_void_p),
+        ("projmatrix", c_void_p),
+        ("sh_degree", c_int32),
+        ("campos", c_void_p),
+        ("prefiltered", c_int32),
+        ("debug", c_int32),
+        ("variant", c_int32),
+        ("use_filter", c_int32),
+        ("ogfs_epsilon_scale", c_float),
+        ("ogfs_epsilon_max", c_float),
+        ("ogfs_energy_floor", c_float),
+        ("ogfs_radius_ratio_min", c_float),
+        ("ogfs_enable_stats", c_int32),
+        ("grad_mode", c_int32),
+    ]
+
+
+class Gaussians(ctypes.Structure):
+    """fgs_gaussians."""
+    _fields_ = [
+        ("P", c_int32),
+        ("M", c_int32),
+        ("means3D", c_void_p),
+        ("shs", c_void_p),
+        ("colors_precomp", c_void_p),
+        ("opacities", c_void_p),
+        ("scales", c_void_p),
+        ("rotations", c_void_p),
+        ("cov3D_precomp", c_void_p),
+    ]
+
+
+_SIGS = {
+    "fgs_geom_bytes": (c_size_t, [c_int32]),
+    "fgs_image_bytes": (c_size_t, [c_int32, c_int32]),
+    "fgs_binning_bytes": (c_size_t, [c_int64, c_int32, c_int32]),
+    "fgs_forward_geometry": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fgs_forward_render": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
+    "fgs_read_sorted": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]),
+    "fgs_image_ranges": (c_void_p, [c_void_p, c_int32, c_int32]),
+    "fgs_geom_offsets": (None, [c_int32, c_void_p]),
+    "fgs_backward_ws_bytes": (c_size_t, [c_int32]),
+    "fgs_backward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fgs_mark_visible": (c_int32, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fgs_compute_radius": (c_int32, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_float, c_float, c_int32, c_int32, c_void_p, c_void_p]),
+    "fgs_compute_radius_v2": (c_int32, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_float, c_float, c_float, c_float, c_void_p,
+                                        c_void_p]),
+    "fgs_lod_ws_bytes": (c_size_t, [c_int32]),
+    "fgs_compute_visibility_radius": (c_int32, [c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                c_void_p, c_void_p, c_void_p, c_int32, c_float,
+                                                c_float, c_float, c_float, c_float, c_float,
+                                                c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                c_void_p]),
+    "fgs_visible_flag": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_float, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+    "fgs_gather_attributes": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p]),
+    "fgs_profile_enable": (c_int32, [c_int32]),
+    "fgs_profile_read": (c_int32, [c_void_p, c_void_p, c_int32, c_int32]),
+    "fgs_last_error": (ctypes.c_char_p, []),
+    "fgs_abi_version": (c_int32, []),
+    "fgs_ogfs_log_lambda": (c_float, [c_float, c_float]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+class FGSError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the shared library (no GPU needed) and bind every entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FGSError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
+            "or make -C filtergs_amd/csrc)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        msg = load().fgs_last_error().decode(errors="replace")
+        raise FGSError(f"{name} failed (code {rc}): {msg}")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL).  The tensor must be contiguous."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor passed to the C ABI must be contiguous")
+    return t.data_ptr() if t.numel() > 0 else None
+
+
+def stream_handle(device=None):
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_gpu(*tensors):
+    if not torch.cuda.is_available():
+        raise FGSError("filtergs_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    for t in tensors:
+        if t is not None and torch.is_tensor(t) and t.numel() > 0 and not t.is_cuda:
+            raise ValueError("all tensors must live on the GPU")
+
+
+STAGES = ("preprocess", "depth_sort", "tile_offsets", "duplicate", "tile_sort", "ranges",
+          "blend_fwd", "blend_bwd", "preprocess_bwd")
+
+
+def profile_enable(on=True):
+    call("fgs_profile_enable", int(bool(on)))
+
+
+def profile_read(reset=True):
+    """{stage: (total_ms, calls)} accumulated since the last reset."""
+    n = len(STAGES)
+    ms = (ctypes.c_double * n)()
+    calls = (ctypes.c_uint64 * n)()
+    load().fgs_profile_read(ms, calls, n, int(bool(reset)))
+    return {STAGES[k]: (float(ms[k]), int(calls[k])) for k in range(n)}

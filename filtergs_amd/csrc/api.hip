@@ -1,0 +1,626 @@
+// api.hip -- the extern "C" entry points of libfiltergs_amd.so.
+//
+// Host orchestration of rasterizer_impl.cu:254-676 (forward / backward) and
+// of the FilterGS/cuda host functions, without allocation: every buffer is
+// supplied by the caller and carved here into 256-byte aligned sub-arrays.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/filtergs_amd.h"
+#include "fgs_device.h"
+#include "fgs_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// Optional per-stage timing with HIP events recorded on the launch stream.
+enum Stage { ST_PREPROCESS, ST_DEPTH_SORT, ST_OFFSETS, ST_DUPLICATE, ST_TILE_SORT, ST_RANGES,
+             ST_BLEND_FWD, ST_BLEND_BWD, ST_PREPROCESS_BWD, ST_COUNT };
+bool g_prof = false;
+hipEvent_t g_ev[ST_COUNT][2];
+bool g_ev_ready = false;
+double g_prof_ms[ST_COUNT];
+uint64_t g_prof_calls[ST_COUNT];
+bool g_pending[ST_COUNT];
+
+void prof_collect()
+{
+    for (int k = 0; k < ST_COUNT; ++k) {
+        if (!g_pending[k]) continue;
+        float ms = 0.f;
+        if (hipEventSynchronize(g_ev[k][1]) == hipSuccess &&
+            hipEventElapsedTime(&ms, g_ev[k][0], g_ev[k][1]) == hipSuccess) {
+            g_prof_ms[k] += ms;
+            g_prof_calls[k] += 1;
+        }
+        g_pending[k] = false;
+    }
+}
+
+struct ProfScope {
+    int k;
+    hipStream_t st;
+    ProfScope(int stage, hipStream_t s) : k(stage), st(s)
+    {
+        if (!g_prof) return;
+        if (g_pending[k]) prof_collect();
+        (void)hipEventRecord(g_ev[k][0], st);
+    }
+    ~ProfScope()
+    {
+        if (!g_prof) return;
+        (void)hipEventRecord(g_ev[k][1], st);
+        g_pending[k] = true;
+    }
+};
+
+int fail(int code, const char* fmt, const char* detail = "")
+{
+    char buf[512];
+    snprintf(buf, sizeof(buf), fmt, detail);
+    g_last_error = buf;
+    return code;
+}
+
+int check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        char buf[512];
+        snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+        g_last_error = buf;
+        return FGS_E_LAUNCH;
+    }
+    return FGS_OK;
+}
+
+int sync_check(const fgs_raster_settings* s, hipStream_t st, const char* what)
+{
+    if (!s->debug) return FGS_OK;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipGetLastError();
+    return check(e, what);
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Sequential carving of a caller buffer.
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t count)
+    {
+        T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+        off += align256(count * sizeof(T));
+        return p;
+    }
+};
+
+struct Geom {
+    float2* means2D;
+    float* depths;
+    float4* conic_opacity;
+    float* rgb;
+    uint32_t* tiles;
+    uint64_t* vkeysA;
+    uint64_t* vkeysB;
+    uint32_t* offs;
+    uint32_t* hist;
+    uint32_t* partials;
+};
+
+size_t scan_words_for_hist(size_t n)
+{
+    // partials needed by the scan of an n-element radix histogram table
+    return fgs::scan_partials_words(fgs::radix_hist_words(n));
+}
+
+Geom carve_geom(void* buf, int P, size_t* total)
+{
+    Carver c{ static_cast<char*>(buf) };
+    Geom g;
+    const size_t n = (size_t)(P > 0 ? P : 1);
+    g.means2D = c.take<float2>(n);
+    g.depths = c.take<float>(n);
+    g.conic_opacity = c.take<float4>(n);
+    g.rgb = c.take<float>(3 * n);
+    g.tiles = c.take<uint32_t>(n);
+    g.vkeysA = c.take<uint64_t>(n);
+    g.vkeysB = c.take<uint64_t>(n);
+    g.offs = c.take<uint32_t>(n);
+    g.hist = c.take<uint32_t>(fgs::radix_hist_words(n));
+    size_t pw = fgs::scan_partials_words(n);
+    const size_t ph = scan_words_for_hist(n);
+    g.partials = c.take<uint32_t>(pw > ph ? pw : ph);
+    if (total) *total = c.off;
+    return g;
+}
+
+struct Binning {
+    uint64_t* pairsA;
+    uint64_t* pairsB;
+    uint32_t* hist;
+    uint32_t* partials;
+};
+
+Binning carve_binning(void* buf, int64_t K, size_t* total)
+{
+    Carver c{ static_cast<char*>(buf) };
+    Binning b;
+    const size_t n = (size_t)(K > 0 ? K : 1);
+    b.pairsA = c.take<uint64_t>(n);
+    b.pairsB = c.take<uint64_t>(n);
+    b.hist = c.take<uint32_t>(fgs::radix_hist_words(n));
+    b.partials = c.take<uint32_t>(scan_words_for_hist(n));
+    if (total) *total = c.off;
+    return b;
+}
+
+struct Image {
+    float* final_T;
+    uint32_t* n_contrib;
+    uint2* ranges;
+};
+
+Image carve_image(void* buf, int W, int H, size_t* total)
+{
+    Carver c{ static_cast<char*>(buf) };
+    Image im;
+    const size_t npix = (size_t)W * (size_t)H;
+    const size_t T = (size_t)((W + 15) / 16) * (size_t)((H + 15) / 16);
+    im.final_T = c.take<float>(npix > 0 ? npix : 1);
+    im.n_contrib = c.take<uint32_t>(npix > 0 ? npix : 1);
+    im.ranges = c.take<uint2>(T > 0 ? T : 1);
+    if (total) *total = c.off;
+    return im;
+}
+
+int bit_length(uint32_t n)
+{
+    int b = 0;
+    while (n) {
+        ++b;
+        n >>= 1;
+    }
+    return b;
+}
+
+float ogfs_log_lambda(float eps_scale, float eps_max)
+{
+    // forward.cu:346-351, evaluated once per launch on the host
+    const float epsilon_pix = 1.0f / 255.0f;
+    const float eps_base = fmaxf(0.0f, 1.0f - epsilon_pix);
+    const float lambda_raw = 1.0f - (float)pow((double)eps_base, 256.0);
+    float lambda_g = epsilon_pix + eps_scale * (lambda_raw - epsilon_pix);
+    lambda_g = fminf(fmaxf(lambda_g, epsilon_pix), eps_max);
+    return (float)log((double)lambda_g);
+}
+
+int validate(const fgs_raster_settings* s, const fgs_gaussians* g)
+{
+    if (!s || !g) return fail(FGS_E_INVALID, "null settings or gaussians%s");
+    if (g->P < 0) return fail(FGS_E_INVALID, "negative P%s");
+    if (s->image_width <= 0 || s->image_height <= 0)
+        return fail(FGS_E_INVALID, "image size must be positive%s");
+    if (!s->viewmatrix || !s->projmatrix || !s->bg || !s->campos)
+        return fail(FGS_E_INVALID, "null camera pointer%s");
+    if (g->P > 0 && (!g->means3D || !g->opacities))
+        return fail(FGS_E_INVALID, "null means3D or opacities%s");
+    if ((g->shs == nullptr) == (g->colors_precomp == nullptr) && g->P > 0)
+        return fail(FGS_E_INVALID, "provide exactly one of shs / colors_precomp%s");
+    if (g->P > 0 && g->shs && (g->M < (s->sh_degree + 1) * (s->sh_degree + 1) || s->sh_degree > 3 || s->sh_degree < 0))
+        return fail(FGS_E_INVALID, "sh_degree must be in [0,3] with M >= (deg+1)^2%s");
+    const bool sr = g->scales && g->rotations;
+    if (g->P > 0 && (sr == (g->cov3D_precomp != nullptr)))
+        return fail(FGS_E_INVALID, "provide exactly one of scales+rotations / cov3D_precomp%s");
+    return FGS_OK;
+}
+
+fgs::PreArgs make_pre_args(const fgs_raster_settings* s, const fgs_gaussians* g)
+{
+    fgs::PreArgs a{};
+    a.P = g->P;
+    a.D = s->sh_degree;
+    a.M = g->M;
+    a.W = s->image_width;
+    a.H = s->image_height;
+    a.gx = (a.W + 15) / 16;
+    a.gy = (a.H + 15) / 16;
+    a.tanx = s->tanfovx;
+    a.tany = s->tanfovy;
+    a.fy = (float)a.H / (2.0f * s->tanfovy);   // rasterizer_impl.cu:283-284
+    a.fx = (float)a.W / (2.0f * s->tanfovx);
+    a.mod = s->scale_modifier;
+    a.log_lambda = ogfs_log_lambda(s->ogfs_epsilon_scale, s->ogfs_epsilon_max);
+    a.energy_floor = s->ogfs_energy_floor;
+    a.ratio_min = s->ogfs_radius_ratio_min;
+    a.variant = s->variant;
+    a.use_filter = s->use_filter;
+    a.prefiltered = s->prefiltered;
+    a.grad_mode = s->grad_mode;
+    a.view = s->viewmatrix;
+    a.proj = s->projmatrix;
+    a.campos = s->campos;
+    a.means3D = g->means3D;
+    a.scales = g->cov3D_precomp ? nullptr : g->scales;
+    a.rotations = g->cov3D_precomp ? nullptr : g->rotations;
+    a.opacities = g->opacities;
+    a.shs = g->colors_precomp ? nullptr : g->shs;
+    a.colors_precomp = g->colors_precomp;
+    a.cov3D_precomp = g->cov3D_precomp;
+    return a;
+}
+
+__global__ void k_read_sorted(const uint64_t* pairs, const float* depths, size_t K, uint64_t* keys,
+                              uint32_t* vals)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const uint64_t p = pairs[k];
+    const uint32_t g = (uint32_t)p;
+    if (keys) keys[k] = (p & 0xFFFFFFFF00000000ull) | __float_as_uint(depths[g]);
+    if (vals) vals[k] = g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgs_profile_enable(int32_t on)
+{
+    if (on && !g_ev_ready) {
+        for (int k = 0; k < ST_COUNT; ++k)
+            for (int j = 0; j < 2; ++j)
+                if (hipEventCreate(&g_ev[k][j]) != hipSuccess)
+                    return fail(FGS_E_LAUNCH, "hipEventCreate failed%s");
+        g_ev_ready = true;
+    }
+    if (!on && g_prof) prof_collect();
+    g_prof = on != 0;
+    return FGS_OK;
+}
+
+int fgs_profile_read(double* ms, uint64_t* calls, int32_t n, int32_t reset)
+{
+    prof_collect();
+    for (int k = 0; k < n && k < ST_COUNT; ++k) {
+        if (ms) ms[k] = g_prof_ms[k];
+        if (calls) calls[k] = g_prof_calls[k];
+    }
+    if (reset)
+        for (int k = 0; k < ST_COUNT; ++k) {
+            g_prof_ms[k] = 0.0;
+            g_prof_calls[k] = 0;
+        }
+    return ST_COUNT;
+}
+
+const char* fgs_last_error(void) { return g_last_error.c_str(); }
+int fgs_abi_version(void) { return FGS_ABI_VERSION; }
+float fgs_ogfs_log_lambda(float eps_scale, float eps_max) { return ogfs_log_lambda(eps_scale, eps_max); }
+
+size_t fgs_geom_bytes(int32_t P)
+{
+    size_t t = 0;
+    carve_geom(nullptr, P, &t);
+    return t;
+}
+
+size_t fgs_image_bytes(int32_t W, int32_t H)
+{
+    size_t t = 0;
+    carve_image(nullptr, W, H, &t);
+    return t;
+}
+
+size_t fgs_binning_bytes(int64_t K, int32_t W, int32_t H)
+{
+    (void)W;
+    (void)H;
+    size_t t = 0;
+    carve_binning(nullptr, K, &t);
+    return t;
+}
+
+int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, void* geom,
+                         int32_t* radii, uint64_t* counts, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    if (!counts || (g->P > 0 && (!geom || !radii))) return fail(FGS_E_INVALID, "null output%s");
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = check(fgs::launch_fill_u32(counts, 0u, 8, st), "counts clear"))) return rc;
+    if (g->P == 0) return sync_check(s, st, "geometry");
+    const fgs::PreArgs a = make_pre_args(s, g);
+    Geom gm = carve_geom(geom, g->P, nullptr);
+    fgs::PreOut o{};
+    o.radii = radii;
+    o.means2D = gm.means2D;
+    o.depths = gm.depths;
+    o.conic_opacity = gm.conic_opacity;
+    o.rgb = gm.rgb;
+    o.tiles = gm.tiles;
+    o.vkeys = gm.vkeysA;
+    o.counters = (unsigned long long*)counts;
+    o.error_flag = (int*)(counts + 2);
+    {
+        ProfScope ps(ST_PREPROCESS, st);
+        if ((rc = check(fgs::launch_preprocess_fwd(a, o, st), "preprocess"))) return rc;
+    }
+    if ((rc = sync_check(s, st, "preprocess"))) return rc;
+    // depth sort of the per-Gaussian keys on bits 32..63 (4 passes -> back in A)
+    {
+        ProfScope ps(ST_DEPTH_SORT, st);
+        if ((rc = check(fgs::radix_sort_u64(gm.vkeysA, gm.vkeysB, (size_t)g->P, 32, 4, gm.hist,
+                                            gm.partials, st), "depth sort")))
+            return rc;
+    }
+    {
+        ProfScope ps(ST_OFFSETS, st);
+        if ((rc = check(fgs::launch_scan_tiles_sorted(gm.vkeysA, gm.tiles, gm.offs, g->P, gm.partials, st),
+                        "tile offsets")))
+            return rc;
+    }
+    return sync_check(s, st, "geometry");
+}
+
+int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                       void* geom, int64_t K, void* binning, void* image, float* out_color,
+                       int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                       float* kv_alpha_sum, uint32_t* kv_alpha_count, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    if (K < 0 || K > 0x7FFFFFFFll) return fail(FGS_E_CAPACITY, "kv pair count out of range%s");
+    if (!image || !out_color || !out_point_id || !out_weight_pixel)
+        return fail(FGS_E_INVALID, "null output%s");
+    if ((kv_alpha_sum == nullptr) != (kv_alpha_count == nullptr))
+        return fail(FGS_E_INVALID, "kv_alpha_sum and kv_alpha_count go together%s");
+    hipStream_t st = (hipStream_t)stream;
+    const int W = s->image_width, H = s->image_height;
+    const int gx = (W + 15) / 16, gy = (H + 15) / 16, T = gx * gy;
+    Image im = carve_image(image, W, H, nullptr);
+    Geom gm = carve_geom(geom, g->P, nullptr);
+    fgs::BlendArgs b{};
+    b.ranges = im.ranges;
+    b.W = W;
+    b.H = H;
+    b.gx = gx;
+    b.gy = gy;
+    b.means2D = gm.means2D;
+    b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+    b.conic_opacity = gm.conic_opacity;
+    b.bg = s->bg;
+    if (K > 0) {
+        if (!binning || !geom || !radii) return fail(FGS_E_INVALID, "null binning/geometry buffer%s");
+        Binning bn = carve_binning(binning, K, nullptr);
+        const int passes = (bit_length((uint32_t)T) + 7) / 8;
+        uint64_t* first = (passes % 2 == 0) ? bn.pairsA : bn.pairsB;
+        uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
+        {
+            ProfScope ps(ST_DUPLICATE, st);
+            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.means2D, radii, g->P, gx, gy,
+                                                  first, st), "duplicate")))
+                return rc;
+        }
+        {
+            ProfScope ps(ST_TILE_SORT, st);
+            if ((rc = check(fgs::radix_sort_u64(first, second, (size_t)K, 32, passes, bn.hist,
+                                                bn.partials, st), "tile sort")))
+                return rc;
+        }
+        if ((rc = sync_check(s, st, "binning"))) return rc;
+        b.pairs = bn.pairsA;
+    } else {
+        b.pairs = nullptr;
+    }
+    {
+        ProfScope ps(ST_RANGES, st);
+        if ((rc = check(fgs::launch_ranges(b.pairs, (size_t)K, im.ranges, T, st), "ranges"))) return rc;
+    }
+    {
+        ProfScope ps(ST_BLEND_FWD, st);
+        if ((rc = check(fgs::launch_blend_fwd(b, im.final_T, im.n_contrib, out_color, out_point_id,
+                                              out_weight_pixel, out_point_weight, kv_alpha_sum,
+                                              kv_alpha_count, st), "blend")))
+            return rc;
+    }
+    return sync_check(s, st, "render");
+}
+
+int fgs_read_sorted(const void* geom, int32_t P, int64_t K, const void* binning, uint64_t* keys,
+                    uint32_t* vals, void* stream)
+{
+    if (K <= 0) return FGS_OK;
+    if (!geom || !binning) return fail(FGS_E_INVALID, "null buffer%s");
+    Geom gm = carve_geom(const_cast<void*>(geom), P, nullptr);
+    Binning bn = carve_binning(const_cast<void*>(binning), K, nullptr);
+    hipLaunchKernelGGL(k_read_sorted, dim3((unsigned)((K + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, bn.pairsA, gm.depths, (size_t)K, keys, vals);
+    return check(hipGetLastError(), "read_sorted");
+}
+
+void fgs_geom_offsets(int32_t P, size_t* out5)
+{
+    Geom g = carve_geom(nullptr, P, nullptr);
+    out5[0] = (size_t)(uintptr_t)g.means2D;
+    out5[1] = (size_t)(uintptr_t)g.depths;
+    out5[2] = (size_t)(uintptr_t)g.conic_opacity;
+    out5[3] = (size_t)(uintptr_t)g.rgb;
+    out5[4] = (size_t)(uintptr_t)g.tiles;
+}
+
+const uint32_t* fgs_image_ranges(const void* image, int32_t W, int32_t H)
+{
+    return reinterpret_cast<const uint32_t*>(carve_image(const_cast<void*>(image), W, H, nullptr).ranges);
+}
+
+size_t fgs_backward_ws_bytes(int32_t P)
+{
+    return align256((size_t)(P > 0 ? P : 1) * 3 * sizeof(float));
+}
+
+int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                 const void* geom, int64_t K, const void* binning, const void* image,
+                 const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity,
+                 float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh, float* dL_dscales,
+                 float* dL_drot, void* ws, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    if (g->P == 0) return FGS_OK;
+    if (!geom || !image || !radii || !dL_dpix || !dL_dmeans2D || !dL_dcolors || !dL_dopacity ||
+        !dL_dmeans3D || !dL_dcov3D || !ws)
+        return fail(FGS_E_INVALID, "null buffer%s");
+    if (g->shs && !g->colors_precomp && !dL_dsh) return fail(FGS_E_INVALID, "dL_dsh required%s");
+    if (g->scales && !g->cov3D_precomp && (!dL_dscales || !dL_drot))
+        return fail(FGS_E_INVALID, "dL_dscales / dL_drot required%s");
+    hipStream_t st = (hipStream_t)stream;
+    const int W = s->image_width, H = s->image_height;
+    const int gx = (W + 15) / 16, gy = (H + 15) / 16;
+    Geom gm = carve_geom(const_cast<void*>(geom), g->P, nullptr);
+    Image im = carve_image(const_cast<void*>(image), W, H, nullptr);
+    float* dconic = static_cast<float*>(ws);
+    if ((rc = check(fgs::launch_fill_u32(dconic, 0u, (size_t)g->P * 3, st), "ws clear"))) return rc;
+    if (K > 0) {
+        if (!binning) return fail(FGS_E_INVALID, "null binning buffer%s");
+        Binning bn = carve_binning(const_cast<void*>(binning), K, nullptr);
+        fgs::BlendArgs b{};
+        b.ranges = im.ranges;
+        b.pairs = bn.pairsA;
+        b.W = W;
+        b.H = H;
+        b.gx = gx;
+        b.gy = gy;
+        b.means2D = gm.means2D;
+        b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+        b.conic_opacity = gm.conic_opacity;
+        b.bg = s->bg;
+        ProfScope ps(ST_BLEND_BWD, st);
+        if ((rc = check(fgs::launch_blend_bwd(b, im.final_T, im.n_contrib, dL_dpix, dL_dmeans2D,
+                                              dconic, dL_dopacity, dL_dcolors, s->grad_mode == 1, st),
+                        "blend backward")))
+            return rc;
+        if ((rc = sync_check(s, st, "blend backward"))) return rc;
+    }
+    const fgs::PreArgs a = make_pre_args(s, g);
+    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors };
+    fgs::PreBwdOut o{};
+    o.dL_dopacity = dL_dopacity;
+    o.dL_dmeans3D = dL_dmeans3D;
+    o.dL_dcov3D = dL_dcov3D;
+    o.dL_dsh = a.shs ? dL_dsh : nullptr;
+    o.dL_dscales = a.scales ? dL_dscales : nullptr;
+    o.dL_drot = a.scales ? dL_drot : nullptr;
+    {
+        ProfScope ps(ST_PREPROCESS_BWD, st);
+        if ((rc = check(fgs::launch_preprocess_bwd(a, in, o, st), "preprocess backward"))) return rc;
+    }
+    return sync_check(s, st, "backward");
+}
+
+int fgs_mark_visible(int32_t P, const float* means3D, const float* viewmatrix,
+                     const float* projmatrix, uint8_t* present, void* stream)
+{
+    (void)projmatrix;
+    if (P < 0 || (P > 0 && (!means3D || !viewmatrix || !present)))
+        return fail(FGS_E_INVALID, "mark_visible: bad arguments%s");
+    return check(fgs::launch_mark_visible(P, means3D, viewmatrix, present, (hipStream_t)stream),
+                 "mark_visible");
+}
+
+int fgs_compute_radius(int32_t P, const float* means3D, const float* scales,
+                       const float* rotations, const float* viewmatrix, const float* projmatrix,
+                       float tanfovx, float tanfovy, int32_t H, int32_t W, float* radii,
+                       void* stream)
+{
+    if (P < 0 || (P > 0 && (!means3D || !scales || !rotations || !viewmatrix || !projmatrix || !radii)))
+        return fail(FGS_E_INVALID, "compute_radius: bad arguments%s");
+    return check(fgs::launch_radius_aa(P, means3D, scales, rotations, viewmatrix, projmatrix, tanfovx,
+                                       tanfovy, H, W, radii, (hipStream_t)stream),
+                 "compute_radius");
+}
+
+int fgs_compute_radius_v2(int32_t P, const float* means3D, const float* scales,
+                          const float* rotations, const float* projmatrix,
+                          const float* viewmatrix, float focal_x, float focal_y, float tan_fovx,
+                          float tan_fovy, float* radii, void* stream)
+{
+    if (P < 0 || (P > 0 && (!means3D || !scales || !rotations || !viewmatrix || !projmatrix || !radii)))
+        return fail(FGS_E_INVALID, "compute_radius_v2: bad arguments%s");
+    return check(fgs::launch_radius_v2(P, means3D, scales, rotations, projmatrix, viewmatrix, focal_x,
+                                       focal_y, tan_fovx, tan_fovy, radii, (hipStream_t)stream),
+                 "compute_radius_v2");
+}
+
+size_t fgs_lod_ws_bytes(int32_t N) { return align256((size_t)(N > 0 ? N : 1)); }
+
+int fgs_compute_visibility_radius(int32_t N, const float* means3D, const float* log_scales,
+                                  const float* rotations, const float* projmatrix,
+                                  const float* viewmatrix, const int32_t* tree_node_index,
+                                  const int32_t* ancestor_path, int32_t L, float focal_x,
+                                  float focal_y, float tan_fovx, float tan_fovy, float padding,
+                                  float radius_max, float radius_filter_threshold, uint8_t* mask,
+                                  float* radius2d, float* depth, void* ws, void* stream)
+{
+    if (N < 0 || L < 0) return fail(FGS_E_INVALID, "compute_visibility_radius: negative size%s");
+    if (N == 0) return FGS_OK;
+    if (!means3D || !log_scales || !rotations || !projmatrix || !viewmatrix || !tree_node_index ||
+        (L > 0 && !ancestor_path) || !mask || !radius2d || !depth || !ws)
+        return fail(FGS_E_INVALID, "compute_visibility_radius: null pointer%s");
+    fgs::LodArgs a{};
+    a.N = N;
+    a.L = L;
+    a.means3d = means3D;
+    a.log_scales = log_scales;
+    a.rotations = rotations;
+    a.proj = projmatrix;
+    a.view = viewmatrix;
+    a.node_index = tree_node_index;
+    a.ancestor_path = ancestor_path;
+    a.fx = focal_x;
+    a.fy = focal_y;
+    a.tanx = tan_fovx;
+    a.tany = tan_fovy;
+    a.padding = padding;
+    a.radius_max = radius_max;
+    a.radius_filter_threshold = radius_filter_threshold;
+    a.mask = mask;
+    a.radius2d = radius2d;
+    a.depth = depth;
+    a.cull_flag = static_cast<uint8_t*>(ws);
+    return check(fgs::launch_lod_cut(a, (hipStream_t)stream), "compute_visibility_radius");
+}
+
+int fgs_visible_flag(int32_t B, int32_t N, const float* xyz, const float* proj_matrix,
+                     float padding, uint8_t* valid, float* depth, float* p_proj, void* stream)
+{
+    if (B < 0 || N < 0) return fail(FGS_E_INVALID, "visible_flag: negative size%s");
+    if ((size_t)B * N > 0 && (!xyz || !proj_matrix || !valid || !depth || !p_proj))
+        return fail(FGS_E_INVALID, "visible_flag: null pointer%s");
+    return check(fgs::launch_visible_flag(B, N, xyz, proj_matrix, padding, valid, depth, p_proj,
+                                          (hipStream_t)stream),
+                 "visible_flag");
+}
+
+int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const float* xyz,
+                          const float* scaling, const float* rotation, const float* opacity,
+                          const float* colors, const float* shs, int32_t sh_floats,
+                          float* xyz_out, float* scaling_out, float* rotation_out,
+                          float* opacity_out, float* colors_out, float* shs_out, int32_t* bad_index,
+                          void* stream)
+{
+    if (N < 0 || n_src < 0) return fail(FGS_E_INVALID, "gather: negative size%s");
+    if (N > 0 && (!index || !xyz || !scaling || !rotation || !opacity || !colors || !xyz_out ||
+                  !scaling_out || !rotation_out || !opacity_out || !colors_out || !bad_index))
+        return fail(FGS_E_INVALID, "gather: null pointer%s");
+    return check(fgs::launch_gather(N, index, n_src, xyz, scaling, rotation, opacity, colors, shs,
+                                     sh_floats, xyz_out, scaling_out, rotation_out, opacity_out,
+                                     colors_out, shs_out, bad_index, (hipStream_t)stream),
+                 "gather_attributes");
+}
+
+}  // extern "C"

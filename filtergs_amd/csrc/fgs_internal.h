@@ -1,0 +1,130 @@
+// fgs_internal.h -- argument records and launchers shared between the
+// kernel translation units and the C-ABI layer.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fgs {
+
+// Per-launch geometry arguments of the preprocess kernels (forward and
+// backward).  All pointers are device pointers.
+struct PreArgs {
+    int P, D, M, W, H, gx, gy;
+    float tanx, tany, fx, fy, mod;
+    float log_lambda, energy_floor, ratio_min;
+    int variant;      // 0 wodilate, 1 dilate
+    int use_filter;
+    int prefiltered;
+    int grad_mode;    // 0 reference, 1 exact
+    const float* view;
+    const float* proj;
+    const float* campos;
+    const float* means3D;
+    const float* scales;
+    const float* rotations;
+    const float* opacities;
+    const float* shs;
+    const float* colors_precomp;
+    const float* cov3D_precomp;
+};
+
+// Outputs of the forward preprocess (SoA).
+struct PreOut {
+    int* radii;
+    float2* means2D;
+    float* depths;
+    float4* conic_opacity;
+    float* rgb;
+    uint32_t* tiles;
+    uint64_t* vkeys;                    // (depth_bits << 32 | i), culled: (~0 << 32 | i)
+    unsigned long long* counters;       // [0] visible count, [1] kv pairs
+    int* error_flag;                    // bit 0: prefiltered point culled
+};
+
+struct PreBwdIn {
+    const int* radii;
+    const float* dL_dmean2D;   // [P,3] (x, y used)
+    const float* dL_dconic;    // [P,3]
+    const float* dL_dcolors;   // [P,3]
+};
+struct PreBwdOut {
+    float* dL_dopacity;        // [P], modified in exact mode
+    float* dL_dmeans3D;        // [P,3]
+    float* dL_dcov3D;          // [P,6]
+    float* dL_dsh;             // [P,M,3] or null
+    float* dL_dscales;         // [P,3] or null
+    float* dL_drot;            // [P,4] or null
+};
+
+struct BlendArgs {
+    const uint2* ranges;
+    const uint64_t* pairs;     // sorted (tile << 32 | gaussian)
+    int W, H, gx, gy;
+    const float2* means2D;
+    const float* colors;       // [P,3]
+    const float4* conic_opacity;
+    const float* bg;           // device [3]
+};
+
+// launchers (return hipError_t of the launch)
+hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t s);
+hipError_t launch_preprocess_bwd(const PreArgs& a, const PreBwdIn& in, const PreBwdOut& o,
+                                 hipStream_t s);
+hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s);
+
+// scans / sort / binning
+size_t scan_partials_words(size_t n);
+hipError_t launch_scan_tiles_sorted(const uint64_t* vkeys, const uint32_t* tiles, uint32_t* offs,
+                                    int P, uint32_t* partials, hipStream_t s);
+size_t radix_hist_words(size_t n);
+hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
+                          uint32_t* hist, uint32_t* partials, hipStream_t s);
+hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const float2* means2D,
+                            const int* radii, int P, int gx, int gy, uint64_t* pairs,
+                            hipStream_t s);
+hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T, hipStream_t s);
+
+// blend
+hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib,
+                            float* out_color, int* out_point_id, float* out_weight_pixel,
+                            float* point_weight, float* kv_sum, uint32_t* kv_cnt, hipStream_t s);
+hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint32_t* n_contrib,
+                            const float* dL_dpix, float* dL_dmean2D, float* dL_dconic,
+                            float* dL_dopacity, float* dL_dcolors, int exact, hipStream_t s);
+
+// LoD cut and per-point utilities
+struct LodArgs {
+    int N, L;
+    const float* means3d;
+    const float* log_scales;
+    const float* rotations;
+    const float* proj;
+    const float* view;
+    const int32_t* node_index;
+    const int32_t* ancestor_path;
+    float fx, fy, tanx, tany, padding, radius_max, radius_filter_threshold;
+    uint8_t* mask;
+    float* radius2d;
+    float* depth;
+    uint8_t* cull_flag;        // scratch [N]: internal & visible & 0 < r <= radius_max
+};
+hipError_t launch_lod_cut(const LodArgs& a, hipStream_t s);
+hipError_t launch_radius_v2(int P, const float* means3d, const float* scales,
+                            const float* rotations, const float* proj, const float* view,
+                            float fx, float fy, float tanx, float tany, float* radii,
+                            hipStream_t s);
+hipError_t launch_radius_aa(int P, const float* means3d, const float* scales,
+                            const float* rotations, const float* view, const float* proj,
+                            float tanx, float tany, int H, int W, float* radii, hipStream_t s);
+hipError_t launch_mark_visible(int P, const float* means3d, const float* view, uint8_t* present,
+                               hipStream_t s);
+hipError_t launch_visible_flag(int B, int N, const float* xyz, const float* proj, float padding,
+                               uint8_t* valid, float* depth, float* p_proj, hipStream_t s);
+hipError_t launch_gather(int64_t N, const int64_t* index, int64_t n_src, const float* xyz,
+                         const float* scaling, const float* rotation, const float* opacity,
+                         const float* colors, const float* shs, int sh_floats, float* xyz_o,
+                         float* scaling_o, float* rotation_o, float* opacity_o, float* colors_o,
+                         float* shs_o, int* bad_index, hipStream_t s);
+
+}  // namespace fgs

@@ -1,0 +1,464 @@
+// preprocess.hip -- per-Gaussian forward and backward preprocessing.
+//
+// Forward: antialias/cuda_rasterizer/forward.cu:266-401 (preprocessCUDA)
+//   with computeCov3D (:151-185), computeCov2D (:102-146),
+//   computeColorFromSH (:48-99), in_frustum (auxiliary.h:139-164),
+//   ndc2Pix (auxiliary.h:41-44) and getRect (auxiliary.h:46-56).
+// Backward: antialias/cuda_rasterizer/backward.cu:144-274 (computeCov2DCUDA)
+//   and :346-396 (preprocessCUDA), fused into one kernel.
+//
+// This file is compiled with -ffp-contract=off: the radius, rectangle and
+// depth key are integer results of FP32 arithmetic and must not depend on
+// FMA contraction.
+#include "fgs_device.h"
+#include "fgs_internal.h"
+
+namespace fgs {
+
+__constant__ float kSH_C0 = 0.28209479177387814f;
+__constant__ float kSH_C1 = 0.4886025119029199f;
+__constant__ float kSH_C2[5] = { 1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f,
+                                 -1.0925484305920792f, 0.5462742152960396f };
+__constant__ float kSH_C3[7] = { -0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+                                 0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
+                                 -0.5900435899266435f };
+
+// SH colour of one channel, before the +0.5 shift.  sh points at the
+// Gaussian's coefficients, channel stride 3.
+__device__ __forceinline__ float sh_eval(int deg, const float* sh, int c, float x, float y, float z)
+{
+#define SH(k) sh[(k) * 3 + c]
+    float res = kSH_C0 * SH(0);
+    if (deg > 0) {
+        res = res - kSH_C1 * y * SH(1) + kSH_C1 * z * SH(2) - kSH_C1 * x * SH(3);
+        if (deg > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z;
+            const float xy = x * y, yz = y * z, xz = x * z;
+            res = res + kSH_C2[0] * xy * SH(4) + kSH_C2[1] * yz * SH(5) +
+                  kSH_C2[2] * (2.0f * zz - xx - yy) * SH(6) + kSH_C2[3] * xz * SH(7) +
+                  kSH_C2[4] * (xx - yy) * SH(8);
+            if (deg > 2) {
+                res = res + kSH_C3[0] * y * (3.0f * xx - yy) * SH(9) + kSH_C3[1] * xy * z * SH(10) +
+                      kSH_C3[2] * y * (4.0f * zz - xx - yy) * SH(11) +
+                      kSH_C3[3] * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * SH(12) +
+                      kSH_C3[4] * x * (4.0f * zz - xx - yy) * SH(13) +
+                      kSH_C3[5] * z * (xx - yy) * SH(14) + kSH_C3[6] * x * (xx - 3.0f * yy) * SH(15);
+            }
+        }
+    }
+#undef SH
+    return res;
+}
+
+__device__ __forceinline__ void view_dir(const float* campos, float px, float py, float pz,
+                                         float* d, float* dor)
+{
+    dor[0] = px - campos[0];
+    dor[1] = py - campos[1];
+    dor[2] = pz - campos[2];
+    const float len = sqrtf(dor[0] * dor[0] + dor[1] * dor[1] + dor[2] * dor[2]);
+    d[0] = dor[0] / len;
+    d[1] = dor[1] / len;
+    d[2] = dor[2] / len;
+}
+
+__global__ __launch_bounds__(256) void k_preprocess_fwd(PreArgs a, PreOut o)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_range = i < a.P;
+    int radius_px = 0;
+    uint32_t tiles = 0;
+    float mx = 0.f, my = 0.f, depth = 0.f;
+    float4 con = make_float4(0.f, 0.f, 0.f, 0.f);
+    float col[3] = { 0.f, 0.f, 0.f };
+    bool write_rgb = false;
+
+    if (in_range) {
+        const float px = a.means3D[3 * i], py = a.means3D[3 * i + 1], pz = a.means3D[3 * i + 2];
+        float hom[4], view[3];
+        xf4(a.proj, px, py, pz, hom);
+        const float p_w = 1.0f / (hom[3] + 0.0000001f);
+        const float projx = hom[0] * p_w, projy = hom[1] * p_w;
+        xf3(a.view, px, py, pz, view);
+        bool alive = !(view[2] <= 0.2f);
+        if (!alive && a.prefiltered) atomicOr(o.error_flag, 1);
+        float c3[6];
+        Cov2D cv;
+        float ca = 0.f, cb = 0.f, cc = 0.f, det = 0.f;
+        if (alive) {
+            if (a.cov3D_precomp) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) c3[k] = a.cov3D_precomp[6 * i + k];
+            } else {
+                cov3d(a.mod * a.scales[3 * i], a.mod * a.scales[3 * i + 1], a.mod * a.scales[3 * i + 2],
+                      a.rotations[4 * i], a.rotations[4 * i + 1], a.rotations[4 * i + 2],
+                      a.rotations[4 * i + 3], c3);
+            }
+            cov2d(a.view, px, py, pz, a.fx, a.fy, a.tanx, a.tany, c3, cv);
+            ca = cv.a; cb = cv.b; cc = cv.c;
+            if (a.variant == 1) {
+                ca = (float)((double)ca + 0.3);
+                cc = (float)((double)cc + 0.3);
+            } else if (a.use_filter) {
+                ca = fmaxf(ca, 0.3f);
+                cc = fmaxf(cc, 0.3f);
+            }
+            det = ca * cc - cb * cb;
+            alive = det != 0.0f;
+        }
+        float opacity = 0.f;
+        if (alive) {
+            const float det_inv = 1.0f / det;
+            con.x = cc * det_inv;
+            con.y = -cb * det_inv;
+            con.z = ca * det_inv;
+            if (a.variant == 1) {
+                opacity = a.opacities[i];
+                const float mid = 0.5f * (ca + cc);
+                const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
+                const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
+                radius_px = sat_int(ceilf(3.0f * sqrtf(fmaxf(l1, l2))));
+                if (radius_px <= 0) alive = false;
+            } else {
+                opacity = fmaxf(a.opacities[i], 1e-6f);
+                const float log_opacity = logf_cr(opacity);
+                const float energy_clamped = fmaxf(log_opacity - a.log_lambda, a.energy_floor);
+                const float mid = 0.5f * (ca + cc);
+                const float disc = fmaxf(0.0f, mid * mid - det);
+                const float sqrt_disc = sqrtf(disc);
+                const float l1 = mid + sqrt_disc, l2 = mid - sqrt_disc;
+                const float max_lambda = fmaxf(fmaxf(l1, l2), 1e-6f);
+                const float radius_base = 3.0f * sqrtf(max_lambda);
+                float radius_major = sqrtf(2.0f * energy_clamped * max_lambda);
+                radius_major = fminf(radius_major, radius_base);
+                radius_major = fmaxf(radius_major, a.ratio_min * radius_base);
+                const int rp = sat_int(ceilf(radius_major + 1.0f));
+                radius_px = rp > 1 ? rp : 1;
+            }
+            con.w = opacity;
+        }
+        if (alive) {
+            mx = (float)((((double)projx + 1.0) * (double)a.W - 1.0) * 0.5);
+            my = (float)((((double)projy + 1.0) * (double)a.H - 1.0) * 0.5);
+            int rc[4];
+            get_rect(mx, my, radius_px, a.gx, a.gy, rc);
+            tiles = (uint32_t)((rc[2] - rc[0]) * (rc[3] - rc[1]));
+            alive = tiles != 0;
+        }
+        if (alive) {
+            depth = view[2];
+            if (!a.colors_precomp) {
+                float d[3], dor[3];
+                view_dir(a.campos, px, py, pz, d, dor);
+                const float* sh = a.shs + (size_t)i * a.M * 3;
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    const float v = sh_eval(a.D, sh, ch, d[0], d[1], d[2]) + 0.5f;
+                    col[ch] = v < 0.0f ? 0.0f : v;
+                }
+                write_rgb = true;
+            }
+        } else {
+            radius_px = 0;
+            tiles = 0;
+            mx = my = 0.f;
+            con = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        o.radii[i] = radius_px;
+        o.tiles[i] = tiles;
+        o.means2D[i] = make_float2(mx, my);
+        o.depths[i] = depth;
+        o.conic_opacity[i] = con;
+        if (write_rgb || !a.colors_precomp) {
+            o.rgb[3 * i + 0] = col[0];
+            o.rgb[3 * i + 1] = col[1];
+            o.rgb[3 * i + 2] = col[2];
+        }
+        const uint32_t hi = tiles ? __float_as_uint(depth) : 0xFFFFFFFFu;
+        o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
+    }
+    // wave-level totals: visible count and kv pairs
+    const uint32_t vis = (in_range && tiles) ? 1u : 0u;
+    const uint32_t nvis = wave_sum_u32(vis);
+    const uint32_t ntiles = wave_sum_u32(tiles);
+    if (lane_id() == 0 && nvis) {
+        atomicAdd(&o.counters[0], (unsigned long long)nvis);
+        atomicAdd(&o.counters[1], (unsigned long long)ntiles);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backward: backward.cu:144-396 fused (cov2D chain, projection chain, SH, cov3D).
+// Every output entry of every Gaussian is written (zeros where radius == 0).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_preprocess_bwd(PreArgs a, PreBwdIn in, PreBwdOut o)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.P) return;
+    const int M = a.M;
+    const bool alive = in.radii[i] > 0;
+    if (!alive) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o.dL_dcov3D[6 * i + k] = 0.f;
+        if (o.dL_dsh)
+            for (int k = 0; k < M * 3; ++k) o.dL_dsh[(size_t)i * M * 3 + k] = 0.f;
+        if (o.dL_dscales)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) o.dL_dscales[3 * i + k] = 0.f;
+        if (o.dL_drot)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o.dL_drot[4 * i + k] = 0.f;
+        return;
+    }
+    const bool exact = a.grad_mode == 1;
+    const float mx = a.means3D[3 * i], my = a.means3D[3 * i + 1], mz = a.means3D[3 * i + 2];
+    float c3[6];
+    float sv[3] = { 0.f, 0.f, 0.f };
+    float qr = 0.f, qx = 0.f, qy = 0.f, qz = 0.f;
+    if (a.cov3D_precomp) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c3[k] = a.cov3D_precomp[6 * i + k];
+    } else {
+        sv[0] = a.mod * a.scales[3 * i];
+        sv[1] = a.mod * a.scales[3 * i + 1];
+        sv[2] = a.mod * a.scales[3 * i + 2];
+        qr = a.rotations[4 * i]; qx = a.rotations[4 * i + 1];
+        qy = a.rotations[4 * i + 2]; qz = a.rotations[4 * i + 3];
+        cov3d(sv[0], sv[1], sv[2], qr, qx, qy, qz, c3);
+    }
+    Cov2D cv;
+    cov2d(a.view, mx, my, mz, a.fx, a.fy, a.tanx, a.tany, c3, cv);
+    const float x_grad_mul = (cv.txtz < -cv.limx || cv.txtz > cv.limx) ? 0.f : 1.f;
+    const float y_grad_mul = (cv.tytz < -cv.limy || cv.tytz > cv.limy) ? 0.f : 1.f;
+    float ca = cv.a, cb = cv.b, cc = cv.c;
+    float mask_a = 1.f, mask_c = 1.f;
+    if (!exact || a.variant == 1) {
+        ca = (float)((double)ca + 0.3);   // backward.cu:197-199
+        cc = (float)((double)cc + 0.3);
+    } else if (a.use_filter) {
+        mask_a = ca > 0.3f ? 1.f : 0.f;
+        mask_c = cc > 0.3f ? 1.f : 0.f;
+        ca = fmaxf(ca, 0.3f);
+        cc = fmaxf(cc, 0.3f);
+    }
+    const float dcx = in.dL_dconic[3 * i], dcy = in.dL_dconic[3 * i + 1], dcz = in.dL_dconic[3 * i + 2];
+    const float denom = ca * cc - cb * cb;
+    const float denom2inv = exact ? 1.0f / (denom * denom) : 1.0f / ((denom * denom) + 0.0000001f);
+    float dL_da = 0.f, dL_db = 0.f, dL_dc = 0.f;
+    float dcov[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+    const float (*T)[3] = cv.T;
+    if (denom2inv != 0.f) {
+        dL_da = denom2inv * (-cc * cc * dcx + 2.f * cb * cc * dcy + (denom - ca * cc) * dcz);
+        dL_dc = denom2inv * (-ca * ca * dcz + 2.f * ca * cb * dcy + (denom - ca * cc) * dcx);
+        dL_db = denom2inv * 2.f * (cb * cc * dcx - (denom + 2.f * cb * cb) * dcy + ca * cb * dcz);
+        dL_da *= mask_a;
+        dL_dc *= mask_c;
+        dcov[0] = T[0][0] * T[0][0] * dL_da + T[0][0] * T[1][0] * dL_db + T[1][0] * T[1][0] * dL_dc;
+        dcov[3] = T[0][1] * T[0][1] * dL_da + T[0][1] * T[1][1] * dL_db + T[1][1] * T[1][1] * dL_dc;
+        dcov[5] = T[0][2] * T[0][2] * dL_da + T[0][2] * T[1][2] * dL_db + T[1][2] * T[1][2] * dL_dc;
+        dcov[1] = 2.f * T[0][0] * T[0][1] * dL_da + (T[0][0] * T[1][1] + T[0][1] * T[1][0]) * dL_db + 2.f * T[1][0] * T[1][1] * dL_dc;
+        dcov[2] = 2.f * T[0][0] * T[0][2] * dL_da + (T[0][0] * T[1][2] + T[0][2] * T[1][0]) * dL_db + 2.f * T[1][0] * T[1][2] * dL_dc;
+        dcov[4] = 2.f * T[0][2] * T[0][1] * dL_da + (T[0][1] * T[1][2] + T[0][2] * T[1][1]) * dL_db + 2.f * T[1][1] * T[1][2] * dL_dc;
+    }
+    const float Vk[3][3] = { { c3[0], c3[1], c3[2] }, { c3[1], c3[3], c3[4] }, { c3[2], c3[4], c3[5] } };
+    float dT[2][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float t0 = T[0][0] * Vk[k][0] + T[0][1] * Vk[k][1] + T[0][2] * Vk[k][2];
+        const float t1 = T[1][0] * Vk[k][0] + T[1][1] * Vk[k][1] + T[1][2] * Vk[k][2];
+        dT[0][k] = 2.f * t0 * dL_da + t1 * dL_db;
+        dT[1][k] = 2.f * t1 * dL_dc + t0 * dL_db;
+    }
+    const float (*W)[3] = cv.W;
+    const float dL_dJ00 = W[0][0] * dT[0][0] + W[0][1] * dT[0][1] + W[0][2] * dT[0][2];
+    const float dL_dJ02 = W[2][0] * dT[0][0] + W[2][1] * dT[0][1] + W[2][2] * dT[0][2];
+    const float dL_dJ11 = W[1][0] * dT[1][0] + W[1][1] * dT[1][1] + W[1][2] * dT[1][2];
+    const float dL_dJ12 = W[2][0] * dT[1][0] + W[2][1] * dT[1][1] + W[2][2] * dT[1][2];
+    const float tz = 1.f / cv.t[2];
+    const float tz2 = tz * tz, tz3 = tz2 * tz;
+    const float dL_dtx = x_grad_mul * -a.fx * tz2 * dL_dJ02;
+    const float dL_dty = y_grad_mul * -a.fy * tz2 * dL_dJ12;
+    const float dL_dtz = -a.fx * tz2 * dL_dJ00 - a.fy * tz2 * dL_dJ11 + (2.f * a.fx * cv.t[0]) * tz3 * dL_dJ02 +
+                         (2.f * a.fy * cv.t[1]) * tz3 * dL_dJ12;
+    const float* V = a.view;
+    float dm[3];
+    dm[0] = V[0] * dL_dtx + V[1] * dL_dty + V[2] * dL_dtz;
+    dm[1] = V[4] * dL_dtx + V[5] * dL_dty + V[6] * dL_dtz;
+    dm[2] = V[8] * dL_dtx + V[9] * dL_dty + V[10] * dL_dtz;
+
+    // projection of the mean (backward.cu:370-387)
+    const float* Pm = a.proj;
+    float hom[4];
+    xf4(Pm, mx, my, mz, hom);
+    const float m_w = 1.0f / (hom[3] + 0.0000001f);
+    const float mul1 = hom[0] * m_w * m_w;
+    const float mul2 = hom[1] * m_w * m_w;
+    const float g2x = in.dL_dmean2D[3 * i], g2y = in.dL_dmean2D[3 * i + 1];
+    dm[0] += (Pm[0] * m_w - Pm[3] * mul1) * g2x + (Pm[1] * m_w - Pm[3] * mul2) * g2y;
+    dm[1] += (Pm[4] * m_w - Pm[7] * mul1) * g2x + (Pm[5] * m_w - Pm[7] * mul2) * g2y;
+    dm[2] += (Pm[8] * m_w - Pm[11] * mul1) * g2x + (Pm[9] * m_w - Pm[11] * mul2) * g2y;
+
+    // SH backward (backward.cu:20-139)
+    if (a.shs) {
+        const float* sh = a.shs + (size_t)i * M * 3;
+        float* dsh = o.dL_dsh + (size_t)i * M * 3;
+        const int deg = a.D;
+        float d[3], dor[3];
+        view_dir(a.campos, mx, my, mz, d, dor);
+        const float x = d[0], y = d[1], z = d[2];
+        float dRGB[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float v = sh_eval(deg, sh, ch, x, y, z) + 0.5f;
+            dRGB[ch] = v < 0.f ? 0.f : in.dL_dcolors[3 * i + ch];
+        }
+        float ddx[3] = { 0.f, 0.f, 0.f }, ddy[3] = { 0.f, 0.f, 0.f }, ddz[3] = { 0.f, 0.f, 0.f };
+        const int used = (deg + 1) * (deg + 1);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+#define SHv(k) sh[(k) * 3 + ch]
+#define DSH(k) dsh[(k) * 3 + ch]
+            const float g = dRGB[ch];
+            DSH(0) = kSH_C0 * g;
+            if (deg > 0) {
+                const float C1 = kSH_C1;
+                DSH(1) = -C1 * y * g;
+                DSH(2) = C1 * z * g;
+                DSH(3) = -C1 * x * g;
+                ddx[ch] = -C1 * SHv(3);
+                ddy[ch] = -C1 * SHv(1);
+                ddz[ch] = C1 * SHv(2);
+                if (deg > 1) {
+                    const float xx = x * x, yy = y * y, zz = z * z;
+                    const float xy = x * y, yz = y * z, xz = x * z;
+                    DSH(4) = kSH_C2[0] * xy * g;
+                    DSH(5) = kSH_C2[1] * yz * g;
+                    DSH(6) = kSH_C2[2] * (2.f * zz - xx - yy) * g;
+                    DSH(7) = kSH_C2[3] * xz * g;
+                    DSH(8) = kSH_C2[4] * (xx - yy) * g;
+                    ddx[ch] += kSH_C2[0] * y * SHv(4) + kSH_C2[2] * 2.f * -x * SHv(6) + kSH_C2[3] * z * SHv(7) + kSH_C2[4] * 2.f * x * SHv(8);
+                    ddy[ch] += kSH_C2[0] * x * SHv(4) + kSH_C2[1] * z * SHv(5) + kSH_C2[2] * 2.f * -y * SHv(6) + kSH_C2[4] * 2.f * -y * SHv(8);
+                    ddz[ch] += kSH_C2[1] * y * SHv(5) + kSH_C2[2] * 2.f * 2.f * z * SHv(6) + kSH_C2[3] * x * SHv(7);
+                    if (deg > 2) {
+                        DSH(9) = kSH_C3[0] * y * (3.f * xx - yy) * g;
+                        DSH(10) = kSH_C3[1] * xy * z * g;
+                        DSH(11) = kSH_C3[2] * y * (4.f * zz - xx - yy) * g;
+                        DSH(12) = kSH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy) * g;
+                        DSH(13) = kSH_C3[4] * x * (4.f * zz - xx - yy) * g;
+                        DSH(14) = kSH_C3[5] * z * (xx - yy) * g;
+                        DSH(15) = kSH_C3[6] * x * (xx - 3.f * yy) * g;
+                        ddx[ch] += (kSH_C3[0] * SHv(9) * 3.f * 2.f * xy + kSH_C3[1] * SHv(10) * yz +
+                                    kSH_C3[2] * SHv(11) * -2.f * xy + kSH_C3[3] * SHv(12) * -3.f * 2.f * xz +
+                                    kSH_C3[4] * SHv(13) * (-3.f * xx + 4.f * zz - yy) +
+                                    kSH_C3[5] * SHv(14) * 2.f * xz + kSH_C3[6] * SHv(15) * 3.f * (xx - yy));
+                        ddy[ch] += (kSH_C3[0] * SHv(9) * 3.f * (xx - yy) + kSH_C3[1] * SHv(10) * xz +
+                                    kSH_C3[2] * SHv(11) * (-3.f * yy + 4.f * zz - xx) +
+                                    kSH_C3[3] * SHv(12) * -3.f * 2.f * yz + kSH_C3[4] * SHv(13) * -2.f * xy +
+                                    kSH_C3[5] * SHv(14) * -2.f * yz + kSH_C3[6] * SHv(15) * -3.f * 2.f * xy);
+                        ddz[ch] += (kSH_C3[1] * SHv(10) * xy + kSH_C3[2] * SHv(11) * 4.f * 2.f * yz +
+                                    kSH_C3[3] * SHv(12) * 3.f * (2.f * zz - xx - yy) +
+                                    kSH_C3[4] * SHv(13) * 4.f * 2.f * xz + kSH_C3[5] * SHv(14) * (xx - yy));
+                    }
+                }
+            }
+#undef SHv
+#undef DSH
+        }
+        for (int k = used; k < M; ++k) {
+            dsh[3 * k] = 0.f;
+            dsh[3 * k + 1] = 0.f;
+            dsh[3 * k + 2] = 0.f;
+        }
+        const float ddir0 = ddx[0] * dRGB[0] + ddx[1] * dRGB[1] + ddx[2] * dRGB[2];
+        const float ddir1 = ddy[0] * dRGB[0] + ddy[1] * dRGB[1] + ddy[2] * dRGB[2];
+        const float ddir2 = ddz[0] * dRGB[0] + ddz[1] * dRGB[1] + ddz[2] * dRGB[2];
+        const float sum2 = dor[0] * dor[0] + dor[1] * dor[1] + dor[2] * dor[2];
+        const float invsum32 = 1.0f / sqrtf(sum2 * sum2 * sum2);
+        dm[0] += ((sum2 - dor[0] * dor[0]) * ddir0 - dor[1] * dor[0] * ddir1 - dor[2] * dor[0] * ddir2) * invsum32;
+        dm[1] += (-dor[0] * dor[1] * ddir0 + (sum2 - dor[1] * dor[1]) * ddir1 - dor[2] * dor[1] * ddir2) * invsum32;
+        dm[2] += (-dor[0] * dor[2] * ddir0 - dor[1] * dor[2] * ddir1 + (sum2 - dor[2] * dor[2]) * ddir2) * invsum32;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = dm[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o.dL_dcov3D[6 * i + k] = dcov[k];
+
+    // cov3D backward (backward.cu:278-341)
+    if (!a.cov3D_precomp && o.dL_dscales) {
+        float R[3][3];
+        quat_rot(qr, qx, qy, qz, R);
+        float Mm[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Mm[j][q] = sv[q] * R[j][q];
+        const float dS[3][3] = { { dcov[0], 0.5f * dcov[1], 0.5f * dcov[2] },
+                                 { 0.5f * dcov[1], dcov[3], 0.5f * dcov[4] },
+                                 { 0.5f * dcov[2], 0.5f * dcov[4], dcov[5] } };
+        float dM[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                dM[j][q] = (2.f * Mm[0][q]) * dS[j][0] + (2.f * Mm[1][q]) * dS[j][1] + (2.f * Mm[2][q]) * dS[j][2];
+        const float smul = exact ? a.mod : 1.f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            o.dL_dscales[3 * i + q] = (R[0][q] * dM[0][q] + R[1][q] * dM[1][q] + R[2][q] * dM[2][q]) * smul;
+        float dMt[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) dMt[j][q] = dM[q][j] * sv[j];
+        const float r = qr, x = qx, y = qy, z = qz;
+        float4 dq;
+        dq.x = 2.f * z * (dMt[0][1] - dMt[1][0]) + 2.f * y * (dMt[2][0] - dMt[0][2]) + 2.f * x * (dMt[1][2] - dMt[2][1]);
+        dq.y = 2.f * y * (dMt[1][0] + dMt[0][1]) + 2.f * z * (dMt[2][0] + dMt[0][2]) + 2.f * r * (dMt[1][2] - dMt[2][1]) - 4.f * x * (dMt[2][2] + dMt[1][1]);
+        dq.z = 2.f * x * (dMt[1][0] + dMt[0][1]) + 2.f * r * (dMt[2][0] - dMt[0][2]) + 2.f * z * (dMt[1][2] + dMt[2][1]) - 4.f * y * (dMt[2][2] + dMt[0][0]);
+        dq.w = 2.f * r * (dMt[0][1] - dMt[1][0]) + 2.f * x * (dMt[2][0] + dMt[0][2]) + 2.f * y * (dMt[1][2] + dMt[2][1]) - 4.f * z * (dMt[1][1] + dMt[0][0]);
+        o.dL_drot[4 * i + 0] = dq.x;
+        o.dL_drot[4 * i + 1] = dq.y;
+        o.dL_drot[4 * i + 2] = dq.z;
+        o.dL_drot[4 * i + 3] = dq.w;
+    } else {
+        if (o.dL_dscales)
+            for (int k = 0; k < 3; ++k) o.dL_dscales[3 * i + k] = 0.f;
+        if (o.dL_drot)
+            for (int k = 0; k < 4; ++k) o.dL_drot[4 * i + k] = 0.f;
+    }
+    if (exact && a.variant == 0 && a.opacities[i] < 1e-6f) o.dL_dopacity[i] = 0.f;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s)
+{
+    if (n_words == 0) return hipSuccess;
+    size_t blocks = (n_words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)p, v, n_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t s)
+{
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_fwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a, o);
+    return hipGetLastError();
+}
+
+hipError_t launch_preprocess_bwd(const PreArgs& a, const PreBwdIn& in, const PreBwdOut& o,
+                                 hipStream_t s)
+{
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_bwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a, in, o);
+    return hipGetLastError();
+}
+
+}  // namespace fgs

@@ -1,0 +1,293 @@
+// sort.hip -- tile binning: scans, wave64 LSD radix sort, duplicate-with-keys
+// and tile ranges.
+//
+// The reference (rasterizer_impl.cu:113-147, :363, :427-445, :172-194) sorts
+// K = sum(tiles_touched) 64-bit keys (tile << 32 | depth bits) on 32 +
+// bit_length(T) bits.  The result is the order (tile, depth bits, Gaussian
+// index).  Here the same order is produced in two smaller sorts:
+//   1. the P per-Gaussian keys (depth bits << 32 | index) are sorted on their
+//      upper 32 bits (culled Gaussians carry 0xFFFFFFFF and land last);
+//   2. duplicate_keys emits (tile << 32 | index) in that depth order;
+//   3. a stable LSD sort on the tile bits only (bit_length(T) bits) keeps the
+//      depth order inside every tile.
+// Stable LSD passes use 8-bit digits: a per-block digit histogram, a
+// device-wide exclusive scan of the digit-major histogram table, and a
+// scatter that ranks equal digits inside each 256-element slab with wave64
+// ballots (match masks) and an LDS prefix over the four waves.
+#include "fgs_device.h"
+#include "fgs_internal.h"
+
+namespace fgs {
+
+constexpr int kThreads = 256;
+constexpr int kIpt = 16;                      // items per thread
+constexpr int kChunk = kThreads * kIpt;       // 4096 items per block
+constexpr int kRadix = 256;
+
+static inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of one u32 per thread (256 threads = 4 waves).
+// Returns the exclusive prefix; *total receives the block sum.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t* total)
+{
+    const int lane = lane_id();
+    const int wave = threadIdx.x / FGS_WAVE;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < FGS_WAVE; off <<= 1) {
+        const uint32_t n = __shfl_up(incl, off, FGS_WAVE);
+        if (lane >= off) incl += n;
+    }
+    if (lane == FGS_WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) {
+        const uint32_t t = s_wave[w];
+        if (w < wave) wave_off += t;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return wave_off + incl - v;
+}
+
+// Loader for the exclusive scan of tiles_touched in depth-sorted order.
+struct TilesSortedLoader {
+    const uint64_t* vkeys;
+    const uint32_t* tiles;
+    __device__ uint32_t operator()(size_t s) const { return tiles[(uint32_t)vkeys[s]]; }
+};
+struct PlainLoader {
+    const uint32_t* in;
+    __device__ uint32_t operator()(size_t s) const { return in[s]; }
+};
+
+template <typename Loader>
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, uint32_t* partials)
+{
+    __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j)
+        if (base + j < n) acc += ld(base + j);
+    uint32_t total;
+    block_exclusive_scan(acc, s_wave, &total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// Single workgroup: exclusive scan of partials[0..nb) in place.
+__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, size_t nb)
+{
+    __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    uint32_t carry = 0;
+    for (size_t base = 0; base < nb; base += kThreads) {
+        const size_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? partials[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, s_wave, &total);
+        if (i < nb) partials[i] = carry + ex;
+        carry += total;
+    }
+}
+
+template <typename Loader>
+__global__ __launch_bounds__(kThreads) void k_scan_down(Loader ld, size_t n, const uint32_t* partials,
+                                                        uint32_t* out)
+{
+    __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
+    uint32_t v[kIpt];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        v[j] = (base + j < n) ? ld(base + j) : 0u;
+        acc += v[j];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan(acc, s_wave, &total) + partials[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+size_t scan_partials_words(size_t n) { return div_up(n > 0 ? n : 1, kChunk); }
+
+template <typename Loader>
+static hipError_t exclusive_scan(Loader ld, size_t n, uint32_t* out, uint32_t* partials,
+                                 hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const size_t nb = div_up(n, kChunk);
+    hipLaunchKernelGGL(k_scan_reduce<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kThreads), 0, s, partials, nb);
+    hipLaunchKernelGGL(k_scan_down<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_tiles_sorted(const uint64_t* vkeys, const uint32_t* tiles, uint32_t* offs,
+                                    int P, uint32_t* partials, hipStream_t s)
+{
+    return exclusive_scan(TilesSortedLoader{ vkeys, tiles }, (size_t)P, offs, partials, s);
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of u64 keys on 8-bit digits.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_rs_count(const uint64_t* __restrict__ in, size_t n,
+                                                       int shift, uint32_t* __restrict__ hist,
+                                                       size_t nb)
+{
+    __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
+    const int wave = threadIdx.x / FGS_WAVE;
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kChunk;
+#pragma unroll 4
+    for (int j = 0; j < kIpt; ++j) {
+        const size_t e = base + (size_t)j * kThreads + threadIdx.x;
+        if (e < n) {
+            const uint32_t d = (uint32_t)(in[e] >> shift) & (kRadix - 1);
+            atomicAdd(&s_h[wave][d], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t c = s_h[0][threadIdx.x] + s_h[1][threadIdx.x] + s_h[2][threadIdx.x] + s_h[3][threadIdx.x];
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restrict__ in,
+                                                         uint64_t* __restrict__ out, size_t n,
+                                                         int shift, const uint32_t* __restrict__ offs,
+                                                         size_t nb)
+{
+    __shared__ uint32_t s_cnt[kThreads / FGS_WAVE][kRadix];
+    __shared__ uint32_t s_off[kRadix];
+    const int wave = threadIdx.x / FGS_WAVE;
+    const int lane = lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
+#pragma unroll
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
+    s_off[threadIdx.x] = offs[(size_t)threadIdx.x * nb + blockIdx.x];
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kChunk;
+    for (int j = 0; j < kIpt; ++j) {
+        const size_t e = base + (size_t)j * kThreads + threadIdx.x;
+        const bool valid = e < n;
+        const uint64_t key = valid ? in[e] : 0ull;
+        const uint32_t d = (uint32_t)(key >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(peers & lt_mask);
+        if (valid && rank == 0) s_cnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        {
+            const int t = threadIdx.x;
+            uint32_t run = s_off[t];
+#pragma unroll
+            for (int w = 0; w < kThreads / FGS_WAVE; ++w) {
+                const uint32_t c = s_cnt[w][t];
+                s_cnt[w][t] = run;
+                run += c;
+            }
+            s_off[t] = run;
+        }
+        __syncthreads();
+        if (valid) out[s_cnt[wave][d] + rank] = key;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
+        __syncthreads();
+    }
+}
+
+size_t radix_hist_words(size_t n) { return (size_t)kRadix * div_up(n > 0 ? n : 1, kChunk); }
+
+// Sorts n keys on n_passes 8-bit digits starting at bit shift_lo.  The input
+// is in a; after the call the sorted keys are in a if n_passes is even and
+// in b otherwise.
+hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
+                          uint32_t* hist, uint32_t* partials, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const size_t nb = div_up(n, kChunk);
+    uint64_t* src = a;
+    uint64_t* dst = b;
+    for (int p = 0; p < n_passes; ++p) {
+        const int shift = shift_lo + 8 * p;
+        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nb), dim3(kThreads), 0, s, src, n, shift, hist, nb);
+        hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(kThreads), 0, s, src, dst, n, shift, hist, nb);
+        uint64_t* t = src;
+        src = dst;
+        dst = t;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// duplicateWithKeys in depth order (rasterizer_impl.cu:113-147).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ vkeys,
+                                                   const uint32_t* __restrict__ offs,
+                                                   const float2* __restrict__ means2D,
+                                                   const int* __restrict__ radii, int P, int gx,
+                                                   int gy, uint64_t* __restrict__ pairs)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P) return;
+    const uint64_t vk = vkeys[s];
+    if ((uint32_t)(vk >> 32) == 0xFFFFFFFFu) return;
+    const uint32_t g = (uint32_t)vk;
+    const int r = radii[g];
+    if (r <= 0) return;
+    int rc[4];
+    const float2 m = means2D[g];
+    get_rect(m.x, m.y, r, gx, gy, rc);
+    uint32_t off = offs[s];
+    for (int y = rc[1]; y < rc[3]; ++y)
+        for (int x = rc[0]; x < rc[2]; ++x) {
+            pairs[off++] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | g;
+        }
+}
+
+hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const float2* means2D,
+                            const int* radii, int P, int gx, int gy, uint64_t* pairs, hipStream_t s)
+{
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_duplicate, dim3((P + 255) / 256), dim3(256), 0, s, vkeys, offs, means2D,
+                       radii, P, gx, gy, pairs);
+    return hipGetLastError();
+}
+
+// identifyTileRanges (rasterizer_impl.cu:172-194); ranges zeroed beforehand.
+__global__ __launch_bounds__(256) void k_ranges(const uint64_t* __restrict__ sorted, size_t K,
+                                                uint2* __restrict__ ranges)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const uint32_t t = (uint32_t)(sorted[k] >> 32);
+    if (k == 0 || (uint32_t)(sorted[k - 1] >> 32) != t) ranges[t].x = (uint32_t)k;
+    if (k == K - 1 || (uint32_t)(sorted[k + 1] >> 32) != t) ranges[t].y = (uint32_t)(k + 1);
+}
+
+hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T, hipStream_t s)
+{
+    hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
+    if (e != hipSuccess || K == 0) return e;
+    hipLaunchKernelGGL(k_ranges, dim3((unsigned)div_up(K, 256)), dim3(256), 0, s, sorted, K, ranges);
+    return hipGetLastError();
+}
+
+}  // namespace fgs

@@ -1,0 +1,314 @@
+"""Functions with the call signatures of the reference's ``_C`` extension
+(antialias/ext.cpp:17-50, antialias/rasterize_points.cu:36-318), running the
+gfx950 kernels through the C ABI.
+
+``rasterize_gaussians`` returns the same 10-tuple (num_rendered, color,
+point_id, point_weight_pixel, point_weight, radii, geomBuffer, binningBuffer,
+imgBuffer, stats) and ``rasterize_gaussians_backward`` the same 8-tuple.
+Keyword-only extras select the dilate variant and the gradient mode.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import Gaussians, RasterSettings, call, ptr
+
+KPC_BIN_EDGES = (0.01, 0.05, 0.10, 0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45, 0.50)
+MAX_KPC_BINS = 64
+
+
+def _nonempty(t):
+    return t is not None and torch.is_tensor(t) and t.numel() > 0
+
+
+def _f32(t, device):
+    if not _nonempty(t):
+        return None
+    if t.dtype != torch.float32:
+        raise RuntimeError(f"expected a float32 tensor, got {t.dtype}")
+    return t.to(device=device).contiguous()
+
+
+def _settings(bg, viewmatrix, projmatrix, campos, image_height, image_width, tanfovx, tanfovy,
+              scale_modifier, sh_degree, prefiltered, debug, variant, use_filter, eps_scale,
+              eps_max, energy_floor, ratio_min, enable_stats, grad_mode, device):
+    keep = [t.to(device=device, dtype=torch.float32).contiguous()
+            for t in (bg, viewmatrix, projmatrix, campos)]
+    s = RasterSettings()
+    s.image_height = int(image_height)
+    s.image_width = int(image_width)
+    s.tanfovx = float(tanfovx)
+    s.tanfovy = float(tanfovy)
+    s.bg = keep[0].data_ptr()
+    s.scale_modifier = float(scale_modifier)
+    s.viewmatrix = keep[1].data_ptr()
+    s.projmatrix = keep[2].data_ptr()
+    s.sh_degree = int(sh_degree)
+    s.campos = keep[3].data_ptr()
+    s.prefiltered = int(bool(prefiltered))
+    s.debug = int(bool(debug))
+    s.variant = int(variant)
+    s.use_filter = int(bool(use_filter))
+    s.ogfs_epsilon_scale = float(eps_scale)
+    s.ogfs_epsilon_max = float(eps_max)
+    s.ogfs_energy_floor = float(energy_floor)
+    s.ogfs_radius_ratio_min = float(ratio_min)
+    s.ogfs_enable_stats = int(bool(enable_stats))
+    s.grad_mode = int(grad_mode)
+    return s, keep
+
+
+def _gaussians(P, means3D, sh, colors, opacity, scales, rotations, cov3D_precomp, device):
+    t = dict(means3D=_f32(means3D, device), shs=_f32(sh, device), colors=_f32(colors, device),
+             opacity=_f32(opacity, device), scales=_f32(scales, device),
+             rotations=_f32(rotations, device), cov3D=_f32(cov3D_precomp, device))
+    g = Gaussians()
+    g.P = int(P)
+    g.M = int(sh.size(1)) if _nonempty(sh) else 0
+    g.means3D = ptr(t["means3D"])
+    g.shs = ptr(t["shs"])
+    g.colors_precomp = ptr(t["colors"])
+    g.opacities = ptr(t["opacity"])
+    g.scales = ptr(t["scales"])
+    g.rotations = ptr(t["rotations"])
+    g.cov3D_precomp = ptr(t["cov3D"])
+    return g, t
+
+
+def _stats_tensor(P, T, K, kv_sum, kv_cnt, ranges, collect):
+    """RenderStats packed as rasterize_points.cu:138-177 packs them, with the
+    KPC / GTC reductions of rasterizer_impl.cu:479-556 done on the GPU."""
+    import os
+    thresh = 1e-4
+    env = os.environ.get("KPC_USED_THRESH", "")
+    if env:
+        try:
+            v = float(env)
+            thresh = v if v > 0 else 1e-4
+        except ValueError:
+            thresh = 1e-4
+    head = torch.zeros(9 + 2 * MAX_KPC_BINS, dtype=torch.float64)
+    head[4] = float(T)
+    head[5] = float(P)
+    head[3] = float(K)
+    if P > 0:
+        head[1] = float(K) / float(P)
+    if T > 0:
+        head[2] = float(K) / float(T)
+    if not (collect and K > 0):
+        return head
+    dev = kv_sum.device
+    edges = torch.tensor(KPC_BIN_EDGES, dtype=torch.float32, device=dev)
+    used_mask = kv_cnt > 0
+    b = torch.searchsorted(edges, kv_sum.contiguous(), right=False)
+    in_bins = used_mask & (b < len(KPC_BIN_EDGES))
+    hist = torch.bincount(b[in_bins], minlength=len(KPC_BIN_EDGES)).to(torch.float64)
+    used_kv = in_bins.sum()
+    unused_kv = (~used_mask).sum()
+    # per tile sums over [start, end) via an f64 prefix sum
+    rg = ranges.view(T, 2).to(torch.int64)
+    start, end = rg[:, 0], rg[:, 1]
+    n = (end - start).clamp(min=0)
+    cs = torch.zeros(K + 1, dtype=torch.float64, device=dev)
+    cs[1:] = torch.cumsum(kv_sum.to(torch.float64), 0)
+    usedt = (used_mask & (kv_sum >= thresh)).to(torch.int64)
+    cu = torch.zeros(K + 1, dtype=torch.int64, device=dev)
+    cu[1:] = torch.cumsum(usedt, 0)
+    nonempty = n > 0
+    tile_sum = torch.where(nonempty, cs[end] - cs[start], torch.zeros_like(cs[end]))
+    tile_used = torch.where(nonempty, cu[end] - cu[start], torch.zeros_like(cu[end]))
+    nf = n.to(torch.float64)
+    tile_avg = torch.where(nonempty, tile_sum / nf.clamp(min=1), torch.zeros_like(tile_sum))
+    tile_ratio = torch.where(nonempty, tile_used.to(torch.float64) / nf.clamp(min=1),
+                             torch.zeros_like(tile_sum))
+    tile_total = torch.where(nonempty, nf, torch.zeros_like(nf))
+    packed = torch.stack([tile_avg.sum() / T, used_kv.to(torch.float64),
+                          unused_kv.to(torch.float64)]).cpu()
+    head[0] = packed[0]
+    head[6] = packed[1]
+    head[7] = packed[2]
+    head[8] = float(len(KPC_BIN_EDGES))
+    head[9:9 + len(KPC_BIN_EDGES)] = torch.tensor(KPC_BIN_EDGES, dtype=torch.float32).to(torch.float64)
+    head[9 + MAX_KPC_BINS:9 + MAX_KPC_BINS + len(KPC_BIN_EDGES)] = hist.cpu()
+    return torch.cat([head, tile_ratio.cpu(), tile_total.cpu()])
+
+
+def rasterize_gaussians(background, means3D, colors, opacity, scales, rotations, scale_modifier,
+                        cov3D_precomp, viewmatrix, projmatrix, tan_fovx, tan_fovy, image_height,
+                        image_width, sh, degree, campos, prefiltered, use_filter, debug=False,
+                        ogfs_epsilon_scale=0.2, ogfs_epsilon_max=0.2, ogfs_energy_floor=0.05,
+                        ogfs_radius_ratio_min=0.7, ogfs_enable_stats=True, *, variant=0,
+                        grad_mode=0):
+    """RasterizeGaussiansCUDA (antialias/rasterize_points.cu:36-179)."""
+    if means3D.ndim != 2 or means3D.size(1) != 3:
+        raise RuntimeError("means3D must have dimensions (num_points, 3)")
+    _lib.require_gpu(means3D)
+    dev = means3D.device
+    P = int(means3D.size(0))
+    H, W = int(image_height), int(image_width)
+    gx, gy = (W + 15) // 16, (H + 15) // 16
+    T = gx * gy
+    fopt = dict(dtype=torch.float32, device=dev)
+    out_color = torch.zeros((3, H, W), **fopt)
+    out_point_id = torch.zeros((H, W), dtype=torch.int32, device=dev)
+    out_wpix = torch.zeros((H, W), **fopt)
+    radii = torch.zeros((P,), dtype=torch.int32, device=dev)
+    out_pw = torch.zeros((P,), **fopt)
+    u8 = dict(dtype=torch.uint8, device=dev)
+    geom = torch.empty((0,), **u8)
+    binning = torch.empty((0,), **u8)
+    img = torch.empty((0,), **u8)
+    K = 0
+    kv_sum = kv_cnt = ranges = None
+    if P != 0:
+        s, keep_s = _settings(background, viewmatrix, projmatrix, campos, H, W, tan_fovx, tan_fovy,
+                              scale_modifier, degree, prefiltered, debug, variant, use_filter,
+                              ogfs_epsilon_scale, ogfs_epsilon_max, ogfs_energy_floor,
+                              ogfs_radius_ratio_min, ogfs_enable_stats, grad_mode, dev)
+        g, keep_g = _gaussians(P, means3D, sh, colors, opacity, scales, rotations, cov3D_precomp, dev)
+        lib = _lib.load()
+        geom = torch.empty((lib.fgs_geom_bytes(P),), **u8)
+        img = torch.empty((lib.fgs_image_bytes(W, H),), **u8)
+        counts = torch.empty((4,), dtype=torch.int64, device=dev)
+        stream = _lib.stream_handle(dev)
+        call("fgs_forward_geometry", ctypes.byref(s), ctypes.byref(g), geom.data_ptr(),
+             radii.data_ptr(), counts.data_ptr(), stream)
+        ch = counts.cpu()   # the reference's num_rendered readback (rasterizer_impl.cu:367)
+        if int(ch[2]) & 1:
+            raise RuntimeError("Point is filtered although prefiltered is set")
+        K = int(ch[1])
+        if K > 0x7FFFFFFF:
+            raise RuntimeError(f"too many kv pairs ({K})")
+        binning = torch.empty((lib.fgs_binning_bytes(K, W, H),), **u8)
+        collect = bool(ogfs_enable_stats) and K > 0
+        if collect:
+            kv_sum = torch.zeros((K,), **fopt)
+            kv_cnt = torch.zeros((K,), dtype=torch.int32, device=dev)
+        call("fgs_forward_render", ctypes.byref(s), ctypes.byref(g), radii.data_ptr(),
+             geom.data_ptr(), K, binning.data_ptr(), img.data_ptr(), out_color.data_ptr(),
+             out_point_id.data_ptr(), out_wpix.data_ptr(), out_pw.data_ptr(), ptr(kv_sum),
+             ptr(kv_cnt), stream)
+        if collect:
+            rp = lib.fgs_image_ranges(img.data_ptr(), W, H)
+            off = rp - img.data_ptr()
+            ranges = img[off:off + 8 * T].view(torch.int32)
+        del keep_s, keep_g
+    stats = _stats_tensor(P, T, K, kv_sum, kv_cnt, ranges, bool(ogfs_enable_stats) and P != 0)
+    return (K, out_color, out_point_id, out_wpix, out_pw, radii, geom, binning, img, stats)
+
+
+def rasterize_gaussians_backward(background, means3D, radii, colors, scales, rotations,
+                                 scale_modifier, cov3D_precomp, viewmatrix, projmatrix, tan_fovx,
+                                 tan_fovy, dL_dout_color, sh, degree, campos, geomBuffer, R,
+                                 binningBuffer, imageBuffer, debug=False, *, variant=0,
+                                 use_filter=True, grad_mode=0, opacities=None):
+    """RasterizeGaussiansBackwardCUDA (antialias/rasterize_points.cu:181-260)."""
+    dev = means3D.device
+    P = int(means3D.size(0))
+    H, W = int(dL_dout_color.size(1)), int(dL_dout_color.size(2))
+    M = int(sh.size(1)) if _nonempty(sh) else 0
+    fopt = dict(dtype=torch.float32, device=dev)
+    dL_dmeans2D = torch.zeros((P, 3), **fopt)
+    dL_dcolors = torch.zeros((P, 3), **fopt)
+    dL_dopacity = torch.zeros((P, 1), **fopt)
+    has_sr = _nonempty(scales) and not _nonempty(cov3D_precomp)
+    alloc = torch.empty if P > 0 else torch.zeros
+    dL_dmeans3D = alloc((P, 3), **fopt)
+    dL_dcov3D = alloc((P, 6), **fopt)
+    dL_dsh = (alloc if (M > 0 and not _nonempty(colors)) else torch.zeros)((P, M, 3), **fopt)
+    dL_dscales = (alloc if has_sr else torch.zeros)((P, 3), **fopt)
+    dL_drot = (alloc if has_sr else torch.zeros)((P, 4), **fopt)
+    if P != 0:
+        _lib.require_gpu(means3D, dL_dout_color)
+        s, keep_s = _settings(background, viewmatrix, projmatrix, campos, H, W, tan_fovx, tan_fovy,
+                              scale_modifier, degree, False, debug, variant, use_filter, 0.2, 0.2,
+                              0.05, 0.7, False, grad_mode, dev)
+        if opacities is None:
+            if int(grad_mode) == 1:
+                raise RuntimeError("grad_mode 'exact' needs the opacities in the backward")
+            opacities = torch.ones((P,), **fopt)   # never read by the reference equations
+        g, keep_g = _gaussians(P, means3D, sh, colors, opacities.reshape(-1), scales, rotations,
+                               cov3D_precomp, dev)
+        lib = _lib.load()
+        ws = torch.empty((lib.fgs_backward_ws_bytes(P),), dtype=torch.uint8, device=dev)
+        dpix = dL_dout_color.to(dtype=torch.float32).contiguous()
+        call("fgs_backward", ctypes.byref(s), ctypes.byref(g), radii.contiguous().data_ptr(),
+             geomBuffer.data_ptr(), int(R), ptr(binningBuffer), imageBuffer.data_ptr(),
+             dpix.data_ptr(), dL_dmeans2D.data_ptr(), dL_dcolors.data_ptr(), dL_dopacity.data_ptr(),
+             dL_dmeans3D.data_ptr(), dL_dcov3D.data_ptr(), ptr(dL_dsh), ptr(dL_dscales),
+             ptr(dL_drot), ws.data_ptr(), _lib.stream_handle(dev))
+        del keep_s, keep_g
+    return (dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales,
+            dL_drot)
+
+
+def compute_radius(means3D, scales, rotations, viewmatrix, projmatrix, tan_fovx, tan_fovy,
+                   image_height, image_width):
+    """compute_radius (antialias/rasterize_points.cu:263-296): float radii [P]."""
+    P = int(means3D.size(0))
+    radii = torch.zeros((P,), dtype=means3D.dtype, device=means3D.device)
+    if P != 0:
+        _lib.require_gpu(means3D)
+        dev = means3D.device
+        m, sc, ro = (_f32(t, dev) for t in (means3D, scales, rotations))
+        vm = viewmatrix.to(device=dev, dtype=torch.float32).contiguous()
+        pm = projmatrix.to(device=dev, dtype=torch.float32).contiguous()
+        call("fgs_compute_radius", P, ptr(m), ptr(sc), ptr(ro), ptr(vm), ptr(pm), float(tan_fovx),
+             float(tan_fovy), int(image_height), int(image_width), radii.data_ptr(),
+             _lib.stream_handle(dev))
+    return radii
+
+
+def mark_visible(means3D, viewmatrix, projmatrix):
+    """markVisible (antialias/rasterize_points.cu:299-318): bool [P]."""
+    P = int(means3D.size(0))
+    present = torch.zeros((P,), dtype=torch.bool, device=means3D.device)
+    if P != 0:
+        _lib.require_gpu(means3D)
+        dev = means3D.device
+        m = _f32(means3D, dev)
+        vm = viewmatrix.to(device=dev, dtype=torch.float32).contiguous()
+        pm = projmatrix.to(device=dev, dtype=torch.float32).contiguous()
+        call("fgs_mark_visible", P, ptr(m), ptr(vm), ptr(pm), present.data_ptr(),
+             _lib.stream_handle(dev))
+    return present
+
+
+def read_sorted(geomBuffer, P, num_rendered, binningBuffer):
+    """The sorted kv list of the last forward: (keys u64 as int64, values int32)."""
+    K = int(num_rendered)
+    dev = geomBuffer.device
+    keys = torch.empty((K,), dtype=torch.int64, device=dev)
+    vals = torch.empty((K,), dtype=torch.int32, device=dev)
+    if K > 0:
+        call("fgs_read_sorted", geomBuffer.data_ptr(), int(P), K, binningBuffer.data_ptr(),
+             keys.data_ptr(), vals.data_ptr(), _lib.stream_handle(dev))
+    return keys, vals
+
+
+def image_ranges(imageBuffer, image_width, image_height):
+    """Tile ranges [T,2] (int32 view into the image buffer)."""
+    W, H = int(image_width), int(image_height)
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    rp = _lib.load().fgs_image_ranges(imageBuffer.data_ptr(), W, H)
+    off = rp - imageBuffer.data_ptr()
+    return imageBuffer[off:off + 8 * T].view(torch.int32).view(T, 2)
+
+
+def geom_views(geomBuffer, P):
+    """Views of the preprocess outputs inside a geometry buffer:
+    dict(means2D [P,2], depths [P], conic_opacity [P,4], rgb [P,3], tiles [P])."""
+    P = int(P)
+    offs = (ctypes.c_size_t * 5)()
+    _lib.load().fgs_geom_offsets(P, offs)
+
+    def view(i, nbytes_per, dtype, shape):
+        o = int(offs[i])
+        return geomBuffer[o:o + nbytes_per * P].view(dtype).view(shape)
+
+    return dict(means2D=view(0, 8, torch.float32, (P, 2)), depths=view(1, 4, torch.float32, (P,)),
+                conic_opacity=view(2, 16, torch.float32, (P, 4)),
+                rgb=view(3, 12, torch.float32, (P, 3)), tiles=view(4, 4, torch.int32, (P,)))

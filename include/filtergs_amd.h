@@ -1,0 +1,204 @@
+/*
+ * filtergs_amd.h -- C ABI of the MI355X (gfx950) FilterGS rasterizer and
+ * LoD-cut kernels (libfiltergs_amd.so).
+ *
+ * Plain C: device pointers, sizes and a hipStream_t passed as void*.  No
+ * allocation happens inside any entry point: callers size their buffers with
+ * the *_bytes() queries and own all memory.  Every entry point returns 0 on
+ * success and a nonzero FGS_E_* code otherwise; fgs_last_error() gives the
+ * message of the calling thread's last failure.  Launches are asynchronous on
+ * the given stream; no entry point synchronises the host.
+ *
+ * Each function names the reference interface it replaces (paths relative to
+ * the FilterGS repository).
+ */
+#ifndef FILTERGS_AMD_H
+#define FILTERGS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FGS_ABI_VERSION 1
+
+enum {
+    FGS_OK = 0,
+    FGS_E_INVALID = 1,   /* bad argument (shape, size, null pointer)  */
+    FGS_E_LAUNCH = 2,    /* HIP launch / runtime failure              */
+    FGS_E_CAPACITY = 3,  /* caller buffer too small                    */
+};
+
+enum { FGS_VARIANT_WODILATE = 0, FGS_VARIANT_DILATE = 1 };
+enum { FGS_GRAD_REFERENCE = 0, FGS_GRAD_EXACT = 1 };
+
+/* GaussianRasterizationSettings plus the per-call OGFS overrides
+ * (antialias/diff_gaussian_rasterization_wodilate/__init__.py:193-210,
+ *  :245-268).  viewmatrix/projmatrix are device float[16] in column-major
+ * order (the transposed row-vector matrices of LoG/dataset/base.py:20-55);
+ * bg and campos are device float[3]. */
+typedef struct {
+    int32_t image_height;
+    int32_t image_width;
+    float tanfovx;
+    float tanfovy;
+    const float* bg;
+    float scale_modifier;
+    const float* viewmatrix;
+    const float* projmatrix;
+    int32_t sh_degree;
+    const float* campos;
+    int32_t prefiltered;
+    int32_t debug;              /* nonzero: synchronise and check after each stage */
+    int32_t variant;            /* FGS_VARIANT_*                                    */
+    int32_t use_filter;         /* wodilate: max(cov2D_ii, 0.3) when nonzero        */
+    float ogfs_epsilon_scale;
+    float ogfs_epsilon_max;
+    float ogfs_energy_floor;
+    float ogfs_radius_ratio_min;
+    int32_t ogfs_enable_stats;
+    int32_t grad_mode;          /* FGS_GRAD_*                                       */
+} fgs_raster_settings;
+
+/* Per-Gaussian inputs (device, contiguous float32).  Exactly one of shs /
+ * colors_precomp and one of (scales, rotations) / cov3D_precomp is non-null
+ * (the checks of wodilate/__init__.py:270-274). */
+typedef struct {
+    int32_t P;                    /* number of Gaussians                          */
+    int32_t M;                    /* SH coefficients per Gaussian (shs [P,M,3])   */
+    const float* means3D;         /* [P,3]                                        */
+    const float* shs;             /* [P,M,3] or NULL                              */
+    const float* colors_precomp;  /* [P,3] or NULL                                */
+    const float* opacities;       /* [P]                                          */
+    const float* scales;          /* [P,3] or NULL                                */
+    const float* rotations;       /* [P,4] (w,x,y,z) or NULL                      */
+    const float* cov3D_precomp;   /* [P,6] or NULL                                */
+} fgs_gaussians;
+
+/* ---- rasterizer: replaces _C.rasterize_gaussians / RasterizeGaussiansCUDA
+ * (antialias/rasterize_points.cu:36-179, ext.cpp:19-47) and
+ * CudaRasterizer::Rasterizer::forward (cuda_rasterizer/rasterizer_impl.cu:254-563).
+ * The reference sizes its binning buffer after a blocking read of the kv-pair
+ * count; here the forward is split at that point into two calls. */
+
+/* Bytes of the per-Gaussian state (the reference's geomBuffer). */
+size_t fgs_geom_bytes(int32_t P);
+/* Bytes of the per-pixel state (imgBuffer): final T, n_contrib, tile ranges. */
+size_t fgs_image_bytes(int32_t image_width, int32_t image_height);
+/* Bytes of the per-kv-pair state (binningBuffer) for K pairs. */
+size_t fgs_binning_bytes(int64_t K, int32_t image_width, int32_t image_height);
+
+/* Phase 1 -- preprocessCUDA (forward.cu:266-401) + per-Gaussian depth sort.
+ * Writes radii[P] (int32, device) and counts[4] (uint64, device):
+ * counts[0] = visible Gaussians, counts[1] = K (kv pairs), counts[2] =
+ * error bits (bit 0: a point was culled although prefiltered was set). */
+int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, void* geom,
+                         int32_t* radii, uint64_t* counts, void* stream);
+
+/* Phase 2 -- duplicateWithKeys, stable tile sort, identifyTileRanges and
+ * renderCUDA (forward.cu:406-567).  K is counts[1] of phase 1.  Outputs:
+ * out_color [3,H,W], out_point_id [H,W], out_weight_pixel [H,W],
+ * out_point_weight [P] (must be zero on entry).  kv_alpha_sum [K] /
+ * kv_alpha_count [K] may be null; when given they must be zero on entry and
+ * receive the per-kv-pair sum of T*alpha and contribution counts
+ * (the OGFS statistics inputs of rasterizer_impl.cu:450-458, 532-540). */
+int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                       void* geom, int64_t K, void* binning, void* image, float* out_color,
+                       int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                       float* kv_alpha_sum, uint32_t* kv_alpha_count, void* stream);
+
+/* Sorted kv list of phase 2, for inspection: keys[k] = tile << 32 |
+ * float_bits(depth of vals[k]) exactly as the reference's point_list_keys,
+ * vals[k] = point_list.  Either output may be null. */
+int fgs_read_sorted(const void* geom, int32_t P, int64_t K, const void* binning, uint64_t* keys,
+                    uint32_t* vals, void* stream);
+/* Byte offsets inside a geometry buffer of P Gaussians of: means2D (float2),
+ * depths (float), conic_opacity (float4), rgb (float[3]), tiles_touched
+ * (uint32) -- the preprocess outputs, for inspection. */
+void fgs_geom_offsets(int32_t P, size_t* out5);
+/* Device pointer to the tile ranges (uint32 pairs [T,2]) inside an image buffer. */
+const uint32_t* fgs_image_ranges(const void* image, int32_t image_width, int32_t image_height);
+
+/* Backward -- replaces _C.rasterize_gaussians_backward /
+ * RasterizeGaussiansBackwardCUDA (rasterize_points.cu:181-260) and
+ * Rasterizer::backward (rasterizer_impl.cu:582-676): renderCUDA backward
+ * (backward.cu:399-557), computeCov2DCUDA (:144-274) and preprocessCUDA
+ * (:346-396).  dL_dpix [3,H,W].  dL_dmeans2D [P,3], dL_dcolors [P,3],
+ * dL_dopacity [P] must be zero on entry (they are accumulated);
+ * dL_dmeans3D [P,3], dL_dcov3D [P,6], dL_dsh [P,M,3], dL_dscales [P,3],
+ * dL_drot [P,4] are fully written.  ws: fgs_backward_ws_bytes(P) bytes. */
+size_t fgs_backward_ws_bytes(int32_t P);
+int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                 const void* geom, int64_t K, const void* binning, const void* image,
+                 const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity,
+                 float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh, float* dL_dscales,
+                 float* dL_drot, void* ws, void* stream);
+
+/* ---- per-point utilities ------------------------------------------------ */
+/* _C.mark_visible / markVisible (rasterize_points.cu:299-318,
+ * rasterizer_impl.cu:97-109): present[i] = z_view > 0.2. */
+int fgs_mark_visible(int32_t P, const float* means3D, const float* viewmatrix,
+                     const float* projmatrix, uint8_t* present, void* stream);
+
+/* _C.compute_radius (rasterize_points.cu:263-296, forward.cu:187-263). */
+int fgs_compute_radius(int32_t P, const float* means3D, const float* scales,
+                       const float* rotations, const float* viewmatrix, const float* projmatrix,
+                       float tanfovx, float tanfovy, int32_t image_height, int32_t image_width,
+                       float* radii, void* stream);
+
+/* compute_radius_module.compute_radius of FilterGS
+ * (FilterGS/cuda/compute_radius_kernel_v2.cu:180-229, :355-380). */
+int fgs_compute_radius_v2(int32_t P, const float* means3D, const float* scales,
+                          const float* rotations, const float* projmatrix,
+                          const float* viewmatrix, float focal_x, float focal_y, float tan_fovx,
+                          float tan_fovy, float* radii, void* stream);
+
+/* compute_visibility_radius -- the LoD cut
+ * (FilterGS/cuda/compute_radius_kernel_v2.cu:231-353, :382-430;
+ *  FilterGS/cuda/compute_radius.py:23-45).  ancestor_path is int32 [N, L].
+ * mask is bool/uint8 [N]; ws: fgs_lod_ws_bytes(N) bytes. */
+size_t fgs_lod_ws_bytes(int32_t N);
+int fgs_compute_visibility_radius(int32_t N, const float* means3D, const float* log_scales,
+                                  const float* rotations, const float* projmatrix,
+                                  const float* viewmatrix, const int32_t* tree_node_index,
+                                  const int32_t* ancestor_path, int32_t L, float focal_x,
+                                  float focal_y, float tan_fovx, float tan_fovy, float padding,
+                                  float radius_max, float radius_filter_threshold, uint8_t* mask,
+                                  float* radius2d, float* depth, void* ws, void* stream);
+
+/* visible_flag_cuda / visible_flag_cuda_batch
+ * (FilterGS/cuda/visible_flag_kernel.cu:19-81, :165-287): B batches of N
+ * points sharing one projection matrix (B = 1 for the unbatched call). */
+int fgs_visible_flag(int32_t B, int32_t N, const float* xyz, const float* proj_matrix,
+                     float padding, uint8_t* valid, float* depth, float* p_proj, void* stream);
+
+/* gather_attributes_cuda (FilterGS/cuda/gather_attributes_kernel.cu:70-145).
+ * shs / shs_out may be null; sh_floats = coefficients*channels per row.
+ * bad_index (device int, zero on entry) is set when an index is outside
+ * [0, n_src). */
+int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const float* xyz,
+                          const float* scaling, const float* rotation, const float* opacity,
+                          const float* colors, const float* shs, int32_t sh_floats,
+                          float* xyz_out, float* scaling_out, float* rotation_out,
+                          float* opacity_out, float* colors_out, float* shs_out, int32_t* bad_index,
+                          void* stream);
+
+/* ---- misc ---------------------------------------------------------------- */
+/* Per-stage timing with HIP events recorded on the launch stream (off by
+ * default; not for use under graph capture).  Stages, in order: preprocess,
+ * depth sort, tile offsets, duplicate, tile sort, ranges, blend forward,
+ * blend backward, preprocess backward.  fgs_profile_read fills up to n
+ * accumulated milliseconds / call counts and returns the number of stages. */
+int fgs_profile_enable(int32_t on);
+int fgs_profile_read(double* ms, uint64_t* calls, int32_t n, int32_t reset);
+const char* fgs_last_error(void);
+int fgs_abi_version(void);
+/* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */
+float fgs_ogfs_log_lambda(float ogfs_epsilon_scale, float ogfs_epsilon_max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FILTERGS_AMD_H */

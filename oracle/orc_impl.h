@@ -268,6 +268,8 @@ void SFX(orc_preprocess)(const orc_settings* s, int32_t P, int32_t M,
             const R l1 = mid + RSQRT(RMAX(CF(0.1), mid * mid - det));
             const R l2 = mid - RSQRT(RMAX(CF(0.1), mid * mid - det));
             radius_px = orc_sat_int((double)RCEIL(CF(3.0) * RSQRT(RMAX(l1, l2))));
+            if (radius_px <= 0)
+                continue;
         } else {
             /* OGFS energy radius, forward.cu:345-369 */
             opacity = RMAX(opacities[i], CF(1e-6));

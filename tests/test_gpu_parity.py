@@ -1,0 +1,316 @@
+"""GPU parity of the gfx950 kernels against the CPU oracle (through the C ABI).
+
+Bars:
+  * integer results (radius, tiles touched, depth keys, sorted kv list, tile
+    ranges, LoD mask, frustum flags) and the FP32 preprocess outputs that
+    feed them: bit-exact against the oracle's FP32 emulation;
+  * image: |GPU - oracle FP32| <= 2e-5 everywhere, and |GPU - oracle FP64|
+    <= 1e-4 on every pixel whose blend decisions (alpha < 1/255 skips, the
+    T < 1e-4 stop) agree between FP32 and FP64; pixels where FP32 and FP64
+    take a different decision are counted and must stay below 0.1 %;
+  * gradients: max |GPU - oracle FP64| <= 2e-4 * max |oracle| per tensor.
+"""
+import numpy as np
+import pytest
+import torch
+
+from filtergs_amd import ext, workloads
+from filtergs_amd.camera import make_camera
+from tests._scenes import orc_settings, small_scene
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def _cam_tensors(cam):
+    return dict(viewmatrix=_t(cam.world_view_transform), projmatrix=_t(cam.full_proj_transform),
+                campos=_t(cam.center))
+
+
+def gpu_forward(g, cam, variant=0, path="shs", sh_degree=3, bg=(0.0, 0.0, 0.0), stats=False,
+                use_filter=True, grad_mode=0, **ogfs):
+    P = g["means3D"].shape[0]
+    ct = _cam_tensors(cam)
+    sh = _t(g["shs"]) if path == "shs" else torch.Tensor([])
+    colors = _t(g["colors"]) if path == "colors" else torch.Tensor([])
+    if path == "cov3D":
+        sh = _t(g["shs"])
+    cov = _t(g["cov3D"]) if "cov3D" in g and path == "cov3D" else torch.Tensor([])
+    scales = torch.Tensor([]) if path == "cov3D" else _t(g["scales"])
+    rots = torch.Tensor([]) if path == "cov3D" else _t(g["rotations"])
+    out = ext.rasterize_gaussians(
+        _t(bg), _t(g["means3D"]), colors, _t(g["opacities"].reshape(P, 1)), scales, rots, 1.0, cov,
+        ct["viewmatrix"], ct["projmatrix"], cam.tanfovx, cam.tanfovy, cam.height, cam.width, sh,
+        sh_degree, ct["campos"], False, use_filter, False, ogfs_enable_stats=stats,
+        variant=variant, grad_mode=grad_mode, **ogfs)
+    return out
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _check_forward(orc, g, cam, variant, path="shs", bg=(0.1, 0.2, 0.3), f64=True, **kw):
+    out = gpu_forward(g, cam, variant=variant, path=path, bg=bg, **kw)
+    K, color, point_id, wpix, pw, radii, geom, binning, img, _stats = out
+    s = orc_settings(orc, cam, sh_degree=3, variant=variant, bg=bg, **kw)
+    args = dict(scales=None if path == "cov3D" else g["scales"],
+                rotations=None if path == "cov3D" else g["rotations"],
+                shs=g["shs"] if path in ("shs", "cov3D") else None,
+                colors=g["colors"] if path == "colors" else None,
+                cov3D=g.get("cov3D") if path == "cov3D" else None)
+    o32 = orc.render(s, g["means3D"], g["opacities"], mode="f32", nthreads=8,
+                     decision_hash=True, **args)
+    P = g["means3D"].shape[0]
+    pre = o32["pre32"]
+    # integer stage and its FP32 inputs: bit-exact
+    assert np.array_equal(_np(radii), pre["radii"])
+    gv = ext.geom_views(geom, P)
+    vis = pre["radii"] > 0
+    assert np.array_equal(_np(gv["tiles"]).astype(np.uint32), pre["tiles_touched"])
+    assert np.array_equal(_np(gv["means2D"])[vis].view(np.uint32), pre["means2D"][vis].view(np.uint32))
+    assert np.array_equal(_np(gv["depths"])[vis].view(np.uint32), pre["depths"][vis].view(np.uint32))
+    assert np.array_equal(_np(gv["conic_opacity"])[vis].view(np.uint32),
+                          pre["conic_opacity"][vis].view(np.uint32))
+    assert K == len(o32["vals"])
+    keys, vals = ext.read_sorted(geom, P, K, binning)
+    assert np.array_equal(_np(keys).view(np.uint64), o32["keys"])
+    assert np.array_equal(_np(vals).view(np.uint32), o32["vals"])
+    ranges = ext.image_ranges(img, cam.width, cam.height)
+    assert np.array_equal(_np(ranges).view(np.uint32), o32["ranges"])
+    # colours (contraction-free FP32 in both): tight
+    if path != "colors":
+        np.testing.assert_allclose(_np(gv["rgb"])[vis], pre["rgb"][vis], atol=2e-6)
+    # image vs FP32 emulation
+    gimg = _np(color)
+    np.testing.assert_allclose(gimg, o32["color"], atol=2e-5)
+    res = dict(out=out, o32=o32)
+    if f64:
+        o64 = orc.render(s, g["means3D"].astype(np.float64), g["opacities"].astype(np.float64),
+                         mode="f64", nthreads=8, decision_hash=True,
+                         **{k: (v.astype(np.float64) if v is not None else None) for k, v in args.items()})
+        same = (o64["decision_hash"] == o32["decision_hash"])
+        flips = int((~same).sum())
+        d = np.abs(gimg - o64["color"]).max(0)
+        assert d[same].max() <= 1e-4, d[same].max()
+        assert flips <= 1e-3 * same.size, flips
+        res["o64"] = o64
+        res["maxabs_f64"] = float(d[same].max())
+        res["flips"] = flips
+    return res
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_forward_sh_small(orc, variant):
+    g, cam = small_scene(n=6000, W=320, H=240, seed=101 + variant, dist=7.0)
+    _check_forward(orc, g, cam, variant)
+
+
+def test_forward_c0_scene(orc):
+    """Configuration 0 (100k Gaussians, 800x800, SH3, dilate)."""
+    g, cam = workloads.config("c0")
+    _check_forward(orc, g, cam, 1, bg=(0.0, 0.0, 0.0))
+
+
+def test_forward_colors_and_cov3d_paths(orc):
+    g, cam = small_scene(n=4000, W=256, H=200, seed=111, dist=7.0)
+    rng = np.random.default_rng(5)
+    g["colors"] = rng.uniform(0, 1, size=(4000, 3)).astype(np.float32)
+    _check_forward(orc, g, cam, 0, path="colors")
+    s = orc_settings(orc, cam, sh_degree=3, variant=0)
+    pre = orc.preprocess(s, g["means3D"], g["opacities"], g["scales"], g["rotations"], g["shs"])
+    g["cov3D"] = pre["cov3D"]
+    _check_forward(orc, g, cam, 0, path="cov3D")
+
+
+def test_forward_ogfs_settings_and_no_filter(orc):
+    g, cam = small_scene(n=4000, W=256, H=192, seed=121, dist=6.0)
+    _check_forward(orc, g, cam, 0, ogfs_epsilon_scale=1.17, ogfs_epsilon_max=0.2,
+                   ogfs_energy_floor=0.01, ogfs_radius_ratio_min=0.1)
+    _check_forward(orc, g, cam, 0, use_filter=False)
+
+
+def test_odd_sizes_and_depth_ties(orc):
+    """Ragged image (partial tiles), identical depths (stable order by index)."""
+    g, cam = small_scene(n=3000, W=203, H=117, seed=131, dist=6.0)
+    g["means3D"][100:400, 1] = g["means3D"][100, 1]   # same view depth for a slab
+    _check_forward(orc, g, cam, 0)
+
+
+def test_empty_and_all_culled(orc):
+    cam = make_camera(64, 48, center=(0, -5, 0))
+    g = workloads.gaussians(0, 1.0, 3, 0)
+    out = gpu_forward(g, cam)
+    assert out[0] == 0 and out[1].shape == (3, 48, 64)
+    g = workloads.gaussians(500, 1.0, 3, 1)
+    g["means3D"][:, 1] = -10.0        # all behind the camera
+    out = gpu_forward(g, cam, bg=(0.25, 0.5, 0.75))
+    assert out[0] == 0
+    assert (out[5] == 0).all()
+    img = _np(out[1])
+    assert np.allclose(img, np.array([0.25, 0.5, 0.75])[:, None, None])
+
+
+def _grads_gpu(out, g, cam, dL, variant, grad_mode, path="shs", bg=(0.1, 0.2, 0.3)):
+    K, color, point_id, wpix, pw, radii, geom, binning, img, _ = out
+    ct = _cam_tensors(cam)
+    P = g["means3D"].shape[0]
+    sh = _t(g["shs"]) if path == "shs" else torch.Tensor([])
+    colors = _t(g["colors"]) if path == "colors" else torch.Tensor([])
+    return ext.rasterize_gaussians_backward(
+        _t(bg), _t(g["means3D"]), radii, colors, _t(g["scales"]), _t(g["rotations"]), 1.0,
+        torch.Tensor([]), ct["viewmatrix"], ct["projmatrix"], cam.tanfovx, cam.tanfovy, _t(dL), sh,
+        3, ct["campos"], geom, K, binning, img, False, variant=variant, use_filter=True,
+        grad_mode=grad_mode, opacities=_t(g["opacities"].reshape(P)))
+
+
+@pytest.mark.parametrize("variant,grad_mode", [(0, 0), (0, 1), (1, 0)])
+def test_backward_vs_oracle_f64(orc, variant, grad_mode):
+    g, cam = small_scene(n=3000, W=256, H=192, seed=141 + variant, dist=6.0)
+    bg = (0.1, 0.2, 0.3)
+    out = gpu_forward(g, cam, variant=variant, bg=bg, grad_mode=grad_mode)
+    rng = np.random.default_rng(7)
+    dL = rng.normal(size=(3, cam.height, cam.width)).astype(np.float32)
+    gg = _grads_gpu(out, g, cam, dL, variant, grad_mode, bg=bg)
+    names = ["dL_dmean2D", "dL_dcolors", "dL_dopacity", "dL_dmeans3D", "dL_dcov3D", "dL_dsh",
+             "dL_dscales", "dL_drot"]
+    s = orc_settings(orc, cam, sh_degree=3, variant=variant, bg=bg, grad_mode=grad_mode)
+    d64 = {k: g[k].astype(np.float64) for k in g}
+    f = orc.render(s, d64["means3D"], d64["opacities"], d64["scales"], d64["rotations"],
+                   d64["shs"], mode="f64", nthreads=8)
+    ob = orc.backward(s, f, dL.astype(np.float64), d64["means3D"], d64["opacities"],
+                      d64["scales"], d64["rotations"], d64["shs"], mode="f64")
+    ref = dict(dL_dmean2D=ob["dL_dmean2D"], dL_dcolors=ob["dL_dcolors"],
+               dL_dopacity=ob["dL_dopacity"].reshape(-1, 1), dL_dmeans3D=ob["dL_dmeans3D"],
+               dL_dcov3D=ob["dL_dcov3D"], dL_dsh=ob["dL_dsh"], dL_dscales=ob["dL_dscales"],
+               dL_drot=ob["dL_drot"])
+    for name, t in zip(names, gg):
+        a = _np(t).astype(np.float64)
+        r = ref[name].reshape(a.shape)
+        scale = np.abs(r).max()
+        err = np.abs(a - r).max()
+        assert err <= 2e-4 * scale + 1e-9, (name, err, scale)
+
+
+def test_autograd_through_plugin_surface(orc):
+    """GaussianRasterizer forward/backward through torch.autograd."""
+    from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+    g, cam = small_scene(n=2000, W=160, H=128, seed=151, dist=6.0)
+    ct = _cam_tensors(cam)
+    rs = GaussianRasterizationSettings(
+        image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
+        bg=_t([0.0, 0.0, 0.0]), scale_modifier=1.0, viewmatrix=ct["viewmatrix"],
+        projmatrix=ct["projmatrix"], sh_degree=3, campos=ct["campos"], prefiltered=False,
+        debug=False)
+    r = GaussianRasterizer(rs)
+    params = {k: _t(g[k]).requires_grad_(True) for k in ("means3D", "scales", "rotations", "shs")}
+    opac = _t(g["opacities"].reshape(-1, 1)).requires_grad_(True)
+    means2D = torch.zeros_like(params["means3D"], requires_grad=True)
+    color, radii, pid, wpix, pw = r(means3D=params["means3D"], means2D=means2D, opacities=opac,
+                                    shs=params["shs"], scales=params["scales"],
+                                    rotations=params["rotations"])
+    assert r.latest_stats is not None and r.latest_stats["num_gaussians"] == 2000
+    assert "tile_ratio" in r.latest_stats and len(r.latest_stats["tile_ratio"]) == r.latest_stats["num_tiles"]
+    loss = (color * color).sum()
+    loss.backward()
+    for k, p in params.items():
+        assert p.grad is not None and torch.isfinite(p.grad).all(), k
+    assert means2D.grad is not None and means2D.grad.abs().sum() > 0
+    assert opac.grad.shape == opac.shape
+
+
+def test_stats_vs_oracle(orc):
+    g, cam = small_scene(n=3000, W=256, H=192, seed=161, dist=6.0)
+    out = gpu_forward(g, cam, variant=0, stats=True)
+    from filtergs_amd.rasterizer import parse_stats
+    st = parse_stats(out[9])
+    s = orc_settings(orc, cam, sh_degree=3, variant=0)
+    f = orc.render(s, g["means3D"], g["opacities"], g["scales"], g["rotations"], g["shs"],
+                   mode="f32", stats=True)
+    kv, cnt, rg = f["kv_alpha_sum"].astype(np.float64), f["kv_alpha_count"], f["ranges"]
+    T = rg.shape[0]
+    used = cnt > 0
+    edges = np.array([0.01, 0.05, 0.10, 0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45, 0.50], np.float32)
+    b = np.searchsorted(edges, f["kv_alpha_sum"], side="left")
+    inb = used & (b < 11)
+    assert st["num_kv_pairs"] == len(kv)
+    assert st["unused_kv_pairs"] == int((~used).sum())
+    assert abs(st["used_kv_pairs"] - int(inb.sum())) <= 2
+    hist = np.bincount(b[inb], minlength=11)
+    assert np.abs(np.array(st["kpc_hist"]) - hist).sum() <= 4
+    tile_avg = np.array([kv[a:e].mean() if e > a else 0.0 for a, e in rg])
+    assert abs(st["bar_g"] - tile_avg.sum() / T) <= 1e-5
+    np.testing.assert_allclose(st["tile_total_kv"], [e - a for a, e in rg])
+
+
+def test_lod_cut_bitexact(orc):
+    from filtergs_amd import lod
+    from tests.test_oracle_lod import random_tree
+    tree = random_tree(n_roots=200, depth=6, seed=3)
+    cam = make_camera(640, 480, center=(0.0, -9.0, 2.0), fovx_deg=70.0)
+    fx = cam.width / (2 * cam.tanfovx)
+    fy = cam.height / (2 * cam.tanfovy)
+    for rmax, thr in ((5.0, 700.0), (1.5, 40.0)):
+        mask, r2d, depth = lod.compute_visibility_radius(
+            _t(tree["means3D"]), _t(tree["log_scales"]), _t(tree["rotations"]),
+            _t(cam.full_proj_transform), _t(cam.world_view_transform),
+            _t(tree["node_index"], torch.int32), _t(tree["ancestor_path"], torch.int32), fx, fy,
+            cam.tanfovx, cam.tanfovy, rmax, padding=0.1, radius_filter_threshold=thr)
+        om, orr, od = orc.lod_cut(tree["means3D"], tree["log_scales"], tree["rotations"],
+                                  cam.full_proj_transform, cam.world_view_transform,
+                                  tree["node_index"], tree["ancestor_path"], fx, fy, cam.tanfovx,
+                                  cam.tanfovy, 0.1, rmax, thr)
+        assert np.array_equal(_np(mask), om)
+        assert np.array_equal(_np(r2d).view(np.uint32), orr.view(np.uint32))
+        assert np.array_equal(_np(depth).view(np.uint32), od.view(np.uint32))
+        assert 0 < om.sum() < len(om)
+
+
+def test_point_utilities(orc):
+    from filtergs_amd import lod
+    g, cam = small_scene(n=5000, W=320, H=240, seed=171, dist=6.0, half=4.0)
+    m, sc, ro = _t(g["means3D"]), _t(g["scales"]), _t(g["rotations"])
+    vm, pm = _t(cam.world_view_transform), _t(cam.full_proj_transform)
+    vis = ext.mark_visible(m, vm, pm)
+    assert np.array_equal(_np(vis), orc.mark_visible(g["means3D"], cam.world_view_transform,
+                                                     cam.full_proj_transform))
+    ra = ext.compute_radius(m, sc, ro, vm, pm, cam.tanfovx, cam.tanfovy, cam.height, cam.width)
+    assert np.array_equal(_np(ra).view(np.uint32),
+                          orc.radius_aa(g["means3D"], g["scales"], g["rotations"],
+                                        cam.world_view_transform, cam.full_proj_transform,
+                                        cam.tanfovx, cam.tanfovy, cam.height, cam.width).view(np.uint32))
+    fx, fy = cam.width / (2 * cam.tanfovx), cam.height / (2 * cam.tanfovy)
+    rv = lod.compute_radius_module.compute_radius(m, sc, ro, pm, vm, fx, fy, cam.tanfovx, cam.tanfovy)
+    assert np.array_equal(_np(rv).view(np.uint32),
+                          orc.radius_v2(g["means3D"], g["scales"], g["rotations"],
+                                        cam.full_proj_transform, cam.world_view_transform, fx, fy,
+                                        cam.tanfovx, cam.tanfovy).view(np.uint32))
+    valid, depth, pp = lod.visible_flag_cuda_direct(m, pm, padding=0.05)
+    ov, od, op = orc.visible_flag(g["means3D"], cam.full_proj_transform, 0.05)
+    assert np.array_equal(_np(valid), ov)
+    assert np.array_equal(_np(depth).view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(_np(pp).view(np.uint32), op.view(np.uint32))
+    vb, db, pb = lod.visible_flag_cuda_batch_direct(torch.stack([m, m * 0.5]), pm, padding=0.05)
+    assert np.array_equal(_np(vb[0]), ov)
+    idx = torch.tensor([4, 0, 4999, 17], dtype=torch.int64, device=DEV)
+    cols = _t(np.random.default_rng(1).uniform(size=(5000, 3)))
+    shs = _t(g["shs"][:, 1:, :])
+    out = lod.gather_attributes(idx, m, sc, ro, _t(g["opacities"].reshape(-1, 1)), cols, shs)
+    for k, src in (("xyz", m), ("scaling", sc), ("rotation", ro), ("colors", cols), ("shs", shs)):
+        assert torch.equal(out[k], src[idx]), k
+    with pytest.raises(RuntimeError):
+        lod.gather_attributes(torch.tensor([5000], device=DEV), m, sc, ro,
+                              _t(g["opacities"].reshape(-1, 1)), cols)
+
+
+def test_c1_scale_keys_bitexact_and_image(orc):
+    """Configuration 1 (1M Gaussians in a 40^3 box, 1920x1080, SH3): tile keys,
+    sort order and ranges bit-exact; image against FP64."""
+    g, cam = workloads.config("c1")
+    res = _check_forward(orc, g, cam, 0, bg=(0.0, 0.0, 0.0))
+    assert res["out"][0] > 1_000_000
