@@ -344,7 +344,6 @@ int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, v
     o.rgb = gm.rgb;
     o.tiles = gm.tiles;
     o.vkeys = gm.vkeysA;
-    o.counters = (unsigned long long*)counts;
     o.error_flag = (int*)(counts + 2);
     {
         ProfScope ps(ST_PREPROCESS, st);
@@ -360,7 +359,8 @@ int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, v
     }
     {
         ProfScope ps(ST_OFFSETS, st);
-        if ((rc = check(fgs::launch_scan_tiles_sorted(gm.vkeysA, gm.tiles, gm.offs, g->P, gm.partials, st),
+        if ((rc = check(fgs::launch_scan_tiles_sorted(gm.vkeysA, gm.tiles, gm.offs, g->P, gm.partials,
+                                                      (unsigned long long*)counts, st),
                         "tile offsets")))
             return rc;
     }
@@ -445,12 +445,14 @@ int fgs_read_sorted(const void* geom, int32_t P, int64_t K, const void* binning,
 
 void fgs_geom_offsets(int32_t P, size_t* out5)
 {
-    Geom g = carve_geom(nullptr, P, nullptr);
-    out5[0] = (size_t)(uintptr_t)g.means2D;
-    out5[1] = (size_t)(uintptr_t)g.depths;
-    out5[2] = (size_t)(uintptr_t)g.conic_opacity;
-    out5[3] = (size_t)(uintptr_t)g.rgb;
-    out5[4] = (size_t)(uintptr_t)g.tiles;
+    // carve a fictitious buffer at a non-null base and report offsets from it
+    static char probe[256];
+    Geom g = carve_geom(probe, P, nullptr);
+    out5[0] = (size_t)((char*)g.means2D - probe);
+    out5[1] = (size_t)((char*)g.depths - probe);
+    out5[2] = (size_t)((char*)g.conic_opacity - probe);
+    out5[3] = (size_t)((char*)g.rgb - probe);
+    out5[4] = (size_t)((char*)g.tiles - probe);
 }
 
 const uint32_t* fgs_image_ranges(const void* image, int32_t W, int32_t H)

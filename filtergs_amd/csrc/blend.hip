@@ -4,19 +4,52 @@
 // Backward: antialias/cuda_rasterizer/backward.cu:399-557 (renderCUDA).
 //
 // One workgroup per 16x16 tile, one pixel per lane: 256 lanes = four wave64s,
-// each wave covering four pixel rows.  Gaussians of the tile's range are
-// staged in LDS in batches of 256 (position, conic+opacity, colour, id).
+// each wave covering a 16x4 pixel strip.  Gaussians of the tile's range are
+// staged in LDS in batches of 256 together with a conservative bounding box
+// of the region where alpha >= 1/255 (the ellipse q <= 2 ln(255 o) of the
+// conic, slightly inflated); a wave whose strip misses that box skips the
+// Gaussian with one uniform branch -- no lane of the strip could blend it.
 // point_weight (the maximum T*alpha of each Gaussian over all pixels) is
 // reduced in LDS per batch and published with one float-as-uint atomicMax
 // per (tile, Gaussian), so it is exact and deterministic.
-// The backward pre-reduces the nine per-pixel gradient terms of each
-// Gaussian across the wave with a reduce-scatter butterfly (ten shuffles for
-// eight values, six for the ninth), sums the four waves in LDS and issues one
-// atomic add per term per (tile, Gaussian).
+// The backward visits only positions below the tile's largest n_contrib,
+// pre-reduces the nine per-pixel gradient terms of each Gaussian across the
+// wave with a reduce-scatter butterfly (ten shuffles for eight values, six for
+// the ninth), sums the four waves in LDS and issues one atomic add per term
+// per (tile, Gaussian).
 #include "fgs_device.h"
 #include "fgs_internal.h"
 
 namespace fgs {
+
+constexpr float kLog2e = 1.4426950408889634f;
+
+// Conservative half extents of {alpha >= 1/255} for conic (A, B, C) and
+// opacity o, centred on the mean; negative when the Gaussian can never reach
+// alpha >= 1/255; +inf when the conic is not positive definite (no culling).
+__device__ __forceinline__ float2 alpha_extent(float A, float B, float C, float o)
+{
+    const float det = A * C - B * B;
+    if (!(det > 0.f) || !(A > 0.f)) return make_float2(INFINITY, INFINITY);
+    const float l = 255.f * o;
+    if (!(l >= 1.f)) return make_float2(-1.f, -1.f);
+    const float tau = 2.f * __logf(l) * 1.001f + 1e-4f;
+    const float ex = sqrtf(tau * C / det) * 1.001f + 0.01f;
+    const float ey = sqrtf(tau * A / det) * 1.001f + 0.01f;
+    return make_float2(ex, ey);
+}
+
+__device__ __forceinline__ float4 alpha_box(float2 m, float4 co)
+{
+    const float2 e = alpha_extent(co.x, co.y, co.z, co.w);
+    if (e.x < 0.f) return make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    return make_float4(m.x - e.x, m.x + e.x, m.y - e.y, m.y + e.y);
+}
+
+__device__ __forceinline__ bool strip_misses(float4 bb, float x0, float x1, float y0, float y1)
+{
+    return bb.x > x1 || bb.y < x0 || bb.z > y1 || bb.w < y0;
+}
 
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __restrict__ final_T,
                                                          uint32_t* __restrict__ n_contrib,
@@ -30,20 +63,23 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
     const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
+    const int wave = tid / FGS_WAVE;
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
     const int py = ty * FGS_TILE + (tid / FGS_TILE);
     const bool inside = px < a.W && py < a.H;
     const float pfx = (float)px, pfy = (float)py;
+    const float sx0 = (float)(tx * FGS_TILE), sx1 = sx0 + (FGS_TILE - 1);
+    const float sy0 = (float)(ty * FGS_TILE + wave * 4), sy1 = sy0 + 3.f;
     bool done = !inside;
 
     const uint2 range = a.ranges[tile];
     const int n = (int)(range.y - range.x);
     const int rounds = (n + FGS_BLOCK - 1) / FGS_BLOCK;
 
-    __shared__ float2 s_xy[FGS_BLOCK];
-    __shared__ float4 s_co[FGS_BLOCK];
-    __shared__ float s_rgb[3][FGS_BLOCK];
-    __shared__ uint32_t s_id[FGS_BLOCK];
+    __shared__ float4 s_g0[FGS_BLOCK];   // x, y, a', b'  (power in log2 units)
+    __shared__ float4 s_g1[FGS_BLOCK];   // c', opacity
+    __shared__ float4 s_col[FGS_BLOCK];  // r, g, b, id bits
+    __shared__ float4 s_bb[FGS_BLOCK];   // alpha >= 1/255 box
     __shared__ uint32_t s_w[FGS_BLOCK];
     __shared__ float s_kv[FGS_BLOCK];
     __shared__ uint32_t s_kc[FGS_BLOCK];
@@ -61,12 +97,13 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
         const int k = r * FGS_BLOCK + tid;
         if (k < n) {
             const uint32_t g = (uint32_t)a.pairs[range.x + k];
-            s_id[tid] = g;
-            s_xy[tid] = a.means2D[g];
-            s_co[tid] = a.conic_opacity[g];
-            s_rgb[0][tid] = a.colors[3 * g + 0];
-            s_rgb[1][tid] = a.colors[3 * g + 1];
-            s_rgb[2][tid] = a.colors[3 * g + 2];
+            const float2 m = a.means2D[g];
+            const float4 co = a.conic_opacity[g];
+            s_g0[tid] = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+            s_g1[tid] = make_float4(-0.5f * kLog2e * co.z, co.w, 0.f, 0.f);
+            s_col[tid] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
+                                     __uint_as_float(g));
+            s_bb[tid] = alpha_box(m, co);
         }
         s_w[tid] = 0u;
         s_kv[tid] = 0.f;
@@ -75,12 +112,13 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
         const int cnt = todo < FGS_BLOCK ? todo : FGS_BLOCK;
         for (int j = 0; !done && j < cnt; ++j) {
             contributor++;
-            const float2 xy = s_xy[j];
-            const float dx = xy.x - pfx, dy = xy.y - pfy;
-            const float4 co = s_co[j];
-            const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-            if (power > 0.0f) continue;
-            const float alpha = fminf(0.99f, co.w * __expf(power));
+            if (strip_misses(s_bb[j], sx0, sx1, sy0, sy1)) continue;
+            const float4 g0 = s_g0[j];
+            const float4 g1 = s_g1[j];
+            const float dx = g0.x - pfx, dy = g0.y - pfy;
+            const float p2 = g0.z * dx * dx + g1.x * dy * dy + g0.w * dx * dy;
+            if (p2 > 0.0f) continue;
+            const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
             if (alpha < 1.0f / 255.0f) continue;
             const float test_T = T * (1.0f - alpha);
             if (test_T < 0.0001f) {
@@ -88,12 +126,13 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
                 continue;
             }
             const float w = alpha * T;
-            C0 += s_rgb[0][j] * w;
-            C1 += s_rgb[1][j] * w;
-            C2 += s_rgb[2][j] * w;
+            const float4 col = s_col[j];
+            C0 += col.x * w;
+            C1 += col.y * w;
+            C2 += col.z * w;
             if (w > weight_max) {
                 weight_max = w;
-                max_id = (int)s_id[j];
+                max_id = (int)__float_as_uint(col.w);
             }
             atomicMax(&s_w[j], __float_as_uint(w));
             if (kv_sum) {
@@ -105,7 +144,8 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
         }
         __syncthreads();
         if (k < n) {
-            if (s_w[tid] != 0u) atomicMax((uint32_t*)&point_weight[s_id[tid]], s_w[tid]);
+            const uint32_t g = __float_as_uint(s_col[tid].w);
+            if (s_w[tid] != 0u) atomicMax((uint32_t*)&point_weight[g], s_w[tid]);
             if (kv_sum) {
                 // each kv pair belongs to exactly one tile: plain stores
                 kv_sum[range.x + k] = s_kv[tid];
@@ -180,6 +220,16 @@ __device__ __forceinline__ float wave_reduce_scatter8(const float v[8])
     return z;
 }
 
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, FGS_WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
 constexpr int kGradTerms = 9;   // dcolor x3, dmean2D x2, dconic x3, dopacity
 
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* __restrict__ final_Ts,
@@ -199,22 +249,33 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
     const int py = ty * FGS_TILE + (tid / FGS_TILE);
     const bool inside = px < a.W && py < a.H;
     const float pfx = (float)px, pfy = (float)py;
+    const float sx0 = (float)(tx * FGS_TILE), sx1 = sx0 + (FGS_TILE - 1);
+    const float sy0 = (float)(ty * FGS_TILE + wave * 4), sy1 = sy0 + 3.f;
     const int pix = a.W * py + px;
     const int HW = a.W * a.H;
 
     const uint2 range = a.ranges[tile];
-    const int n = (int)(range.y - range.x);
-    const int rounds = (n + FGS_BLOCK - 1) / FGS_BLOCK;
 
-    __shared__ float2 s_xy[FGS_BLOCK];
-    __shared__ float4 s_co[FGS_BLOCK];
-    __shared__ float s_rgb[3][FGS_BLOCK];
-    __shared__ uint32_t s_id[FGS_BLOCK];
+    __shared__ float4 s_g0[FGS_BLOCK];   // x, y, A, B
+    __shared__ float4 s_g1[FGS_BLOCK];   // C, opacity
+    __shared__ float4 s_col[FGS_BLOCK];  // r, g, b, id bits
+    __shared__ float4 s_bb[FGS_BLOCK];
     __shared__ float s_g[FGS_BLOCK / FGS_WAVE][kGradTerms][FGS_BLOCK];
+    __shared__ uint32_t s_wmax[FGS_BLOCK / FGS_WAVE];
 
     const float T_final = inside ? final_Ts[pix] : 0.f;
     float T = T_final;
     const uint32_t last_contributor = inside ? n_contrib[pix] : 0u;
+    // positions >= the largest n_contrib of the tile contribute to no pixel
+    const uint32_t wmax = wave_max_u32(last_contributor);
+    if (lane == 0) s_wmax[wave] = wmax;
+    __syncthreads();
+    uint32_t bmax = 0;
+#pragma unroll
+    for (int w = 0; w < FGS_BLOCK / FGS_WAVE; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
+    const int n = (int)min(bmax, range.y - range.x);
+    const int rounds = (n + FGS_BLOCK - 1) / FGS_BLOCK;
+
     float dpix0 = 0.f, dpix1 = 0.f, dpix2 = 0.f;
     if (inside) {
         dpix0 = dL_dpix[pix];
@@ -227,74 +288,76 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
     float lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
     const float ddelx_dx = 0.5f * (float)a.W;
     const float ddely_dy = 0.5f * (float)a.H;
-    uint32_t contributor = (uint32_t)n;
-    bool done = !inside;
     int todo = n;
 
     for (int r = 0; r < rounds; ++r, todo -= FGS_BLOCK) {
         __syncthreads();
         const int k = r * FGS_BLOCK + tid;
         if (k < n) {
-            const uint32_t g = (uint32_t)a.pairs[range.y - k - 1];
-            s_id[tid] = g;
-            s_xy[tid] = a.means2D[g];
-            s_co[tid] = a.conic_opacity[g];
-            s_rgb[0][tid] = a.colors[3 * g + 0];
-            s_rgb[1][tid] = a.colors[3 * g + 1];
-            s_rgb[2][tid] = a.colors[3 * g + 2];
+            // back to front: position n-1-k of the tile's list
+            const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
+            const float2 m = a.means2D[g];
+            const float4 co = a.conic_opacity[g];
+            s_g0[tid] = make_float4(m.x, m.y, co.x, co.y);
+            s_g1[tid] = make_float4(co.z, co.w, 0.f, 0.f);
+            s_col[tid] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
+                                     __uint_as_float(g));
+            s_bb[tid] = alpha_box(m, co);
         }
 #pragma unroll
-        for (int t = 0; t < kGradTerms; ++t) s_g[wave][t][tid] = 0.f;
+        for (int w = 0; w < FGS_BLOCK / FGS_WAVE; ++w)
+#pragma unroll
+            for (int t = 0; t < kGradTerms; ++t) s_g[w][t][tid] = 0.f;
         __syncthreads();
         const int cnt = todo < FGS_BLOCK ? todo : FGS_BLOCK;
         for (int j = 0; j < cnt; ++j) {
-            // wave-uniform early exit: every lane of this wave is past its range
-            if (__all(done)) break;
+            const uint32_t position = (uint32_t)(n - 1 - (r * FGS_BLOCK + j));
+            if (position >= wmax) continue;                           // wave-uniform
+            if (strip_misses(s_bb[j], sx0, sx1, sy0, sy1)) continue;  // wave-uniform
             float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
             float vop = 0.f;
             bool act = false;
-            if (!done) {
-                contributor--;
-                if (contributor < last_contributor) {
-                    const float2 xy = s_xy[j];
-                    const float dx = xy.x - pfx, dy = xy.y - pfy;
-                    const float4 co = s_co[j];
-                    const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-                    if (power <= 0.0f) {
-                        const float G = __expf(power);
-                        const float alpha_raw = co.w * G;
-                        const float alpha = fminf(0.99f, alpha_raw);
-                        if (alpha >= 1.0f / 255.0f) {
-                            act = true;
-                            T = T / (1.f - alpha);
-                            const float dch = alpha * T;
-                            const float c0 = s_rgb[0][j], c1 = s_rgb[1][j], c2 = s_rgb[2][j];
-                            acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-                            acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-                            acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-                            lc0 = c0; lc1 = c1; lc2 = c2;
-                            float dL_dalpha = (c0 - acc0) * dpix0 + (c1 - acc1) * dpix1 + (c2 - acc2) * dpix2;
-                            v[0] = dch * dpix0;
-                            v[1] = dch * dpix1;
-                            v[2] = dch * dpix2;
-                            dL_dalpha *= T;
-                            last_alpha = alpha;
-                            dL_dalpha += (-T_final / (1.f - alpha)) * bg_dot;
-                            const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
-                            const float dL_dG = co.w * dL_dalpha * slope;
-                            const float gdx = G * dx, gdy = G * dy;
-                            const float dG_ddelx = -gdx * co.x - gdy * co.y;
-                            const float dG_ddely = -gdy * co.z - gdx * co.y;
-                            v[3] = dL_dG * dG_ddelx * ddelx_dx;
-                            v[4] = dL_dG * dG_ddely * ddely_dy;
-                            v[5] = -0.5f * gdx * dx * dL_dG;
-                            v[6] = -0.5f * gdx * dy * dL_dG;
-                            v[7] = -0.5f * gdy * dy * dL_dG;
-                            vop = G * dL_dalpha * slope;
-                        }
+            if (position < last_contributor) {
+                const float4 g0 = s_g0[j];
+                const float4 g1 = s_g1[j];
+                const float dx = g0.x - pfx, dy = g0.y - pfy;
+                const float power = -0.5f * (g0.z * dx * dx + g1.x * dy * dy) - g0.w * dx * dy;
+                if (power <= 0.0f) {
+                    const float G = __builtin_amdgcn_exp2f(power * kLog2e);
+                    const float alpha_raw = g1.y * G;
+                    const float alpha = fminf(0.99f, alpha_raw);
+                    if (alpha >= 1.0f / 255.0f) {
+                        act = true;
+                        const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+                        T = T * inv;
+                        const float dch = alpha * T;
+                        const float4 col = s_col[j];
+                        acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
+                        acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
+                        acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
+                        lc0 = col.x;
+                        lc1 = col.y;
+                        lc2 = col.z;
+                        float dL_dalpha = (col.x - acc0) * dpix0 + (col.y - acc1) * dpix1 + (col.z - acc2) * dpix2;
+                        v[0] = dch * dpix0;
+                        v[1] = dch * dpix1;
+                        v[2] = dch * dpix2;
+                        dL_dalpha *= T;
+                        last_alpha = alpha;
+                        dL_dalpha += (-T_final * inv) * bg_dot;
+                        const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
+                        const float dL_dG = g1.y * dL_dalpha * slope;
+                        const float gdx = G * dx, gdy = G * dy;
+                        const float dG_ddelx = -gdx * g0.z - gdy * g0.w;
+                        const float dG_ddely = -gdy * g1.x - gdx * g0.w;
+                        v[3] = dL_dG * dG_ddelx * ddelx_dx;
+                        v[4] = dL_dG * dG_ddely * ddely_dy;
+                        v[5] = -0.5f * gdx * dx * dL_dG;
+                        v[6] = -0.5f * gdx * dy * dL_dG;
+                        v[7] = -0.5f * gdy * dy * dL_dG;
+                        vop = G * dL_dalpha * slope;
                     }
                 }
-                if (contributor == 0) done = true;
             }
             if (__any(act)) {
                 const float z = wave_reduce_scatter8(v);
@@ -305,7 +368,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
         }
         __syncthreads();
         if (k < n) {
-            const uint32_t g = s_id[tid];
+            const uint32_t g = __float_as_uint(s_col[tid].w);
             float t[kGradTerms];
 #pragma unroll
             for (int q = 0; q < kGradTerms; ++q)

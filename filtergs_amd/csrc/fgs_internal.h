@@ -38,7 +38,6 @@ struct PreOut {
     float* rgb;
     uint32_t* tiles;
     uint64_t* vkeys;                    // (depth_bits << 32 | i), culled: (~0 << 32 | i)
-    unsigned long long* counters;       // [0] visible count, [1] kv pairs
     int* error_flag;                    // bit 0: prefiltered point culled
 };
 
@@ -76,7 +75,8 @@ hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s);
 // scans / sort / binning
 size_t scan_partials_words(size_t n);
 hipError_t launch_scan_tiles_sorted(const uint64_t* vkeys, const uint32_t* tiles, uint32_t* offs,
-                                    int P, uint32_t* partials, hipStream_t s);
+                                    int P, uint32_t* partials, unsigned long long* counters,
+                                    hipStream_t s);
 size_t radix_hist_words(size_t n);
 hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
                           uint32_t* hist, uint32_t* partials, hipStream_t s);

@@ -23,11 +23,12 @@ __constant__ float kSH_C3[7] = { -0.5900435899266435f, 2.890611442640554f, -0.45
                                  0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f,
                                  -0.5900435899266435f };
 
-// SH colour of one channel, before the +0.5 shift.  sh points at the
-// Gaussian's coefficients, channel stride 3.
-__device__ __forceinline__ float sh_eval(int deg, const float* sh, int c, float x, float y, float z)
+// SH colour of one channel, before the +0.5 shift.  sh(f) returns float f of
+// the Gaussian's coefficient row (coefficient k, channel c at f = 3k + c).
+template <typename Sh>
+__device__ __forceinline__ float sh_eval(int deg, const Sh& sh, int c, float x, float y, float z)
 {
-#define SH(k) sh[(k) * 3 + c]
+#define SH(k) sh((k) * 3 + c)
     float res = kSH_C0 * SH(0);
     if (deg > 0) {
         res = res - kSH_C1 * y * SH(1) + kSH_C1 * z * SH(2) - kSH_C1 * x * SH(3);
@@ -50,6 +51,68 @@ __device__ __forceinline__ float sh_eval(int deg, const float* sh, int c, float 
     return res;
 }
 
+// SH rows of a 256-Gaussian block staged in LDS, transposed (float f of the
+// block's row t at s[f * kPitch + t]; the odd pitch keeps both the staging
+// writes and the per-thread reads free of bank conflicts).  Rows are read
+// from HBM as one contiguous, fully coalesced stream.
+constexpr int kPreThreads = 256;
+constexpr int kPitch = kPreThreads + 1;
+constexpr int kMaxShFloats = 48;
+
+struct ShLds {
+    const float* s;
+    int t;
+    __device__ float operator()(int f) const { return s[f * kPitch + t]; }
+};
+
+template <int NU>
+__device__ __forceinline__ void stage_rows_fixed(const float* __restrict__ src, int rs, int nrows, float* s)
+{
+    const int total = nrows * NU;
+    for (int k = threadIdx.x; k < total; k += kPreThreads) {
+        const int t = k / NU;
+        const int f = k - t * NU;
+        s[f * kPitch + t] = src[(size_t)t * rs + f];
+    }
+}
+
+__device__ __forceinline__ void stage_rows(const float* __restrict__ src, int rs, int nu, int nrows, float* s)
+{
+    if (nu == 48) { stage_rows_fixed<48>(src, rs, nrows, s); return; }
+    if (nu == 27) { stage_rows_fixed<27>(src, rs, nrows, s); return; }
+    if (nu == 12) { stage_rows_fixed<12>(src, rs, nrows, s); return; }
+    if (nu == 3) { stage_rows_fixed<3>(src, rs, nrows, s); return; }
+    const int total = nrows * nu;
+    for (int k = threadIdx.x; k < total; k += kPreThreads) {
+        const int t = k / nu;
+        const int f = k - t * nu;
+        s[f * kPitch + t] = src[(size_t)t * rs + f];
+    }
+}
+
+// Inverse of stage_rows for the first nu floats of each row: LDS -> HBM.
+template <int NU>
+__device__ __forceinline__ void unstage_rows_fixed(const float* s, int rs, int nrows, float* __restrict__ dst)
+{
+    const int total = nrows * NU;
+    for (int k = threadIdx.x; k < total; k += kPreThreads) {
+        const int t = k / NU;
+        const int f = k - t * NU;
+        dst[(size_t)t * rs + f] = s[f * kPitch + t];
+    }
+}
+
+__device__ __forceinline__ void unstage_rows(const float* s, int rs, int nu, int nrows, float* __restrict__ dst)
+{
+    if (nu == 48) { unstage_rows_fixed<48>(s, rs, nrows, dst); return; }
+    const int total = nrows * nu;
+    for (int k = threadIdx.x; k < total; k += kPreThreads) {
+        const int t = k / nu;
+        const int f = k - t * nu;
+        dst[(size_t)t * rs + f] = s[f * kPitch + t];
+    }
+}
+
 __device__ __forceinline__ void view_dir(const float* campos, float px, float py, float pz,
                                          float* d, float* dor)
 {
@@ -62,10 +125,18 @@ __device__ __forceinline__ void view_dir(const float* campos, float px, float py
     d[2] = dor[2] / len;
 }
 
-__global__ __launch_bounds__(256) void k_preprocess_fwd(PreArgs a, PreOut o)
+__global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOut o)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float s_sh[kMaxShFloats * kPitch];
+    const int base = blockIdx.x * kPreThreads;
+    const int i = base + threadIdx.x;
     const bool in_range = i < a.P;
+    const int nrows = min(kPreThreads, a.P - base);
+    const int nu = (a.D + 1) * (a.D + 1) * 3;
+    if (!a.colors_precomp) {
+        stage_rows(a.shs + (size_t)base * a.M * 3, a.M * 3, nu, nrows, s_sh);
+        __syncthreads();
+    }
     int radius_px = 0;
     uint32_t tiles = 0;
     float mx = 0.f, my = 0.f, depth = 0.f;
@@ -150,7 +221,7 @@ __global__ __launch_bounds__(256) void k_preprocess_fwd(PreArgs a, PreOut o)
             if (!a.colors_precomp) {
                 float d[3], dor[3];
                 view_dir(a.campos, px, py, pz, d, dor);
-                const float* sh = a.shs + (size_t)i * a.M * 3;
+                const ShLds sh{ s_sh, (int)threadIdx.x };
 #pragma unroll
                 for (int ch = 0; ch < 3; ++ch) {
                     const float v = sh_eval(a.D, sh, ch, d[0], d[1], d[2]) + 0.5f;
@@ -177,33 +248,32 @@ __global__ __launch_bounds__(256) void k_preprocess_fwd(PreArgs a, PreOut o)
         const uint32_t hi = tiles ? __float_as_uint(depth) : 0xFFFFFFFFu;
         o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
     }
-    // wave-level totals: visible count and kv pairs
-    const uint32_t vis = (in_range && tiles) ? 1u : 0u;
-    const uint32_t nvis = wave_sum_u32(vis);
-    const uint32_t ntiles = wave_sum_u32(tiles);
-    if (lane_id() == 0 && nvis) {
-        atomicAdd(&o.counters[0], (unsigned long long)nvis);
-        atomicAdd(&o.counters[1], (unsigned long long)ntiles);
-    }
 }
 
 // ---------------------------------------------------------------------------
 // Backward: backward.cu:144-396 fused (cov2D chain, projection chain, SH, cov3D).
 // Every output entry of every Gaussian is written (zeros where radius == 0).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_preprocess_bwd(PreArgs a, PreBwdIn in, PreBwdOut o)
+struct ShReg {
+    const float* r;
+    __device__ float operator()(int f) const { return r[f]; }
+};
+
+// One Gaussian of the backward.  shr holds its SH row (first nu floats);
+// dsh receives dL/dsh at s_out[f * kPitch + t].
+__device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBwdIn& in,
+                                                   const PreBwdOut& o, int i, const float* shr,
+                                                   float* s_out, int t)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.P) return;
-    const int M = a.M;
     const bool alive = in.radii[i] > 0;
+    const int nu = (a.D + 1) * (a.D + 1) * 3;
     if (!alive) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = 0.f;
 #pragma unroll
         for (int k = 0; k < 6; ++k) o.dL_dcov3D[6 * i + k] = 0.f;
-        if (o.dL_dsh)
-            for (int k = 0; k < M * 3; ++k) o.dL_dsh[(size_t)i * M * 3 + k] = 0.f;
+        if (a.shs)
+            for (int f = 0; f < nu; ++f) s_out[f * kPitch + t] = 0.f;
         if (o.dL_dscales)
 #pragma unroll
             for (int k = 0; k < 3; ++k) o.dL_dscales[3 * i + k] = 0.f;
@@ -302,8 +372,7 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreArgs a, PreBwdIn in, 
 
     // SH backward (backward.cu:20-139)
     if (a.shs) {
-        const float* sh = a.shs + (size_t)i * M * 3;
-        float* dsh = o.dL_dsh + (size_t)i * M * 3;
+        const ShReg sh{ shr };
         const int deg = a.D;
         float d[3], dor[3];
         view_dir(a.campos, mx, my, mz, d, dor);
@@ -315,11 +384,10 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreArgs a, PreBwdIn in, 
             dRGB[ch] = v < 0.f ? 0.f : in.dL_dcolors[3 * i + ch];
         }
         float ddx[3] = { 0.f, 0.f, 0.f }, ddy[3] = { 0.f, 0.f, 0.f }, ddz[3] = { 0.f, 0.f, 0.f };
-        const int used = (deg + 1) * (deg + 1);
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-#define SHv(k) sh[(k) * 3 + ch]
-#define DSH(k) dsh[(k) * 3 + ch]
+#define SHv(k) shr[(k) * 3 + ch]
+#define DSH(k) s_out[((k) * 3 + ch) * kPitch + t]
             const float g = dRGB[ch];
             DSH(0) = kSH_C0 * g;
             if (deg > 0) {
@@ -365,11 +433,6 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreArgs a, PreBwdIn in, 
             }
 #undef SHv
 #undef DSH
-        }
-        for (int k = used; k < M; ++k) {
-            dsh[3 * k] = 0.f;
-            dsh[3 * k + 1] = 0.f;
-            dsh[3 * k + 2] = 0.f;
         }
         const float ddir0 = ddx[0] * dRGB[0] + ddx[1] * dRGB[1] + ddx[2] * dRGB[2];
         const float ddir1 = ddy[0] * dRGB[0] + ddy[1] * dRGB[1] + ddy[2] * dRGB[2];
@@ -431,6 +494,43 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreArgs a, PreBwdIn in, 
     if (exact && a.variant == 0 && a.opacities[i] < 1e-6f) o.dL_dopacity[i] = 0.f;
 }
 
+__global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd(PreArgs a, PreBwdIn in, PreBwdOut o)
+{
+    __shared__ float s_sh[kMaxShFloats * kPitch];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kPreThreads;
+    const int i = base + t;
+    const int nrows = min(kPreThreads, a.P - base);
+    const bool has_sh = a.shs != nullptr;
+    const int nu = (a.D + 1) * (a.D + 1) * 3;
+    const int rs = a.M * 3;
+    float shr[kMaxShFloats];
+#pragma unroll
+    for (int f = 0; f < kMaxShFloats; ++f) shr[f] = 0.f;
+    if (has_sh) {
+        stage_rows(a.shs + (size_t)base * rs, rs, nu, nrows, s_sh);
+        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < kMaxShFloats; ++f)
+            if (f < nu) shr[f] = s_sh[f * kPitch + t];
+        __syncthreads();   // the buffer now collects dL/dsh
+    }
+    if (i < a.P) preprocess_bwd_one(a, in, o, i, shr, s_sh, t);
+    if (has_sh) {
+        __syncthreads();
+        float* dst = o.dL_dsh + (size_t)base * rs;
+        unstage_rows(s_sh, rs, nu, nrows, dst);
+        if (nu < rs) {
+            // coefficients above the active degree get zero gradient
+            const int tail = rs - nu;
+            for (int k = t; k < nrows * tail; k += kPreThreads) {
+                const int r = k / tail;
+                dst[(size_t)r * rs + nu + (k - r * tail)] = 0.f;
+            }
+        }
+    }
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -449,7 +549,7 @@ hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s)
 hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t s)
 {
     if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_preprocess_fwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a, o);
+    hipLaunchKernelGGL(k_preprocess_fwd, dim3((a.P + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, a, o);
     return hipGetLastError();
 }
 
@@ -457,7 +557,7 @@ hipError_t launch_preprocess_bwd(const PreArgs& a, const PreBwdIn& in, const Pre
                                  hipStream_t s)
 {
     if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_preprocess_bwd, dim3((a.P + 255) / 256), dim3(256), 0, s, a, in, o);
+    hipLaunchKernelGGL(k_preprocess_bwd, dim3((a.P + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, a, in, o);
     return hipGetLastError();
 }
 

@@ -66,24 +66,37 @@ struct PlainLoader {
 };
 
 template <typename Loader>
-__global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, uint32_t* partials)
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, uint32_t* partials,
+                                                          unsigned long long* nz_count)
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
     const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
-    uint32_t acc = 0;
+    uint32_t acc = 0, nz = 0;
 #pragma unroll
     for (int j = 0; j < kIpt; ++j)
-        if (base + j < n) acc += ld(base + j);
+        if (base + j < n) {
+            const uint32_t v = ld(base + j);
+            acc += v;
+            nz += v != 0u;
+        }
     uint32_t total;
     block_exclusive_scan(acc, s_wave, &total);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
+    if (nz_count) {
+        uint32_t nz_total;
+        block_exclusive_scan(nz, s_wave, &nz_total);
+        if (threadIdx.x == 0 && nz_total) atomicAdd(nz_count, (unsigned long long)nz_total);
+    }
 }
 
-// Single workgroup: exclusive scan of partials[0..nb) in place.
-__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, size_t nb)
+// Single workgroup: exclusive scan of partials[0..nb) in place; the grand
+// total (64-bit) goes to *total when given.
+__global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, size_t nb,
+                                                            unsigned long long* total_out)
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
     uint32_t carry = 0;
+    unsigned long long carry64 = 0;
     for (size_t base = 0; base < nb; base += kThreads) {
         const size_t i = base + threadIdx.x;
         const uint32_t v = i < nb ? partials[i] : 0u;
@@ -91,7 +104,9 @@ __global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, 
         const uint32_t ex = block_exclusive_scan(v, s_wave, &total);
         if (i < nb) partials[i] = carry + ex;
         carry += total;
+        carry64 += total;
     }
+    if (total_out && threadIdx.x == 0) *total_out = carry64;
 }
 
 template <typename Loader>
@@ -120,20 +135,25 @@ size_t scan_partials_words(size_t n) { return div_up(n > 0 ? n : 1, kChunk); }
 
 template <typename Loader>
 static hipError_t exclusive_scan(Loader ld, size_t n, uint32_t* out, uint32_t* partials,
-                                 hipStream_t s)
+                                 hipStream_t s, unsigned long long* total = nullptr,
+                                 unsigned long long* nz_count = nullptr)
 {
     if (n == 0) return hipSuccess;
     const size_t nb = div_up(n, kChunk);
-    hipLaunchKernelGGL(k_scan_reduce<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kThreads), 0, s, partials, nb);
+    hipLaunchKernelGGL(k_scan_reduce<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials,
+                       nz_count);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kThreads), 0, s, partials, nb, total);
     hipLaunchKernelGGL(k_scan_down<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials, out);
     return hipGetLastError();
 }
 
 hipError_t launch_scan_tiles_sorted(const uint64_t* vkeys, const uint32_t* tiles, uint32_t* offs,
-                                    int P, uint32_t* partials, hipStream_t s)
+                                    int P, uint32_t* partials, unsigned long long* counters,
+                                    hipStream_t s)
 {
-    return exclusive_scan(TilesSortedLoader{ vkeys, tiles }, (size_t)P, offs, partials, s);
+    // counters[0] += Gaussians with tiles > 0, counters[1] = sum of tiles (K)
+    return exclusive_scan(TilesSortedLoader{ vkeys, tiles }, (size_t)P, offs, partials, s,
+                          counters + 1, counters);
 }
 
 // ---------------------------------------------------------------------------
