@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-band", type=int, default=64, help="rows of the CPU baseline sample")
+    ap.add_argument("--cpu-band", type=int, default=256, help="rows of the CPU baseline sample")
     return ap.parse_args()
 
 

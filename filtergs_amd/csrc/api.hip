@@ -105,6 +105,9 @@ struct Geom {
     float4* conic_opacity;
     float* rgb;
     uint32_t* tiles;
+    uint4* recs;
+    uint32_t* tiles_s;
+    uint2* rects_s;
     uint64_t* vkeysA;
     uint64_t* vkeysB;
     uint32_t* offs;
@@ -128,6 +131,9 @@ Geom carve_geom(void* buf, int P, size_t* total)
     g.conic_opacity = c.take<float4>(n);
     g.rgb = c.take<float>(3 * n);
     g.tiles = c.take<uint32_t>(n);
+    g.recs = c.take<uint4>(n);
+    g.tiles_s = c.take<uint32_t>(n);
+    g.rects_s = c.take<uint2>(n);
     g.vkeysA = c.take<uint64_t>(n);
     g.vkeysB = c.take<uint64_t>(n);
     g.offs = c.take<uint32_t>(n);
@@ -205,6 +211,8 @@ int validate(const fgs_raster_settings* s, const fgs_gaussians* g)
     if (g->P < 0) return fail(FGS_E_INVALID, "negative P%s");
     if (s->image_width <= 0 || s->image_height <= 0)
         return fail(FGS_E_INVALID, "image size must be positive%s");
+    if ((s->image_width + 15) / 16 > 65535 || (s->image_height + 15) / 16 > 65535)
+        return fail(FGS_E_INVALID, "image larger than 65535 tiles per side%s");
     if (!s->viewmatrix || !s->projmatrix || !s->bg || !s->campos)
         return fail(FGS_E_INVALID, "null camera pointer%s");
     if (g->P > 0 && (!g->means3D || !g->opacities))
@@ -343,6 +351,7 @@ int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, v
     o.conic_opacity = gm.conic_opacity;
     o.rgb = gm.rgb;
     o.tiles = gm.tiles;
+    o.recs = gm.recs;
     o.vkeys = gm.vkeysA;
     o.error_flag = (int*)(counts + 2);
     {
@@ -359,8 +368,11 @@ int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, v
     }
     {
         ProfScope ps(ST_OFFSETS, st);
-        if ((rc = check(fgs::launch_scan_tiles_sorted(gm.vkeysA, gm.tiles, gm.offs, g->P, gm.partials,
-                                                      (unsigned long long*)counts, st),
+        if ((rc = check(fgs::launch_sorted_records(gm.vkeysA, gm.recs, g->P, gm.tiles_s, gm.rects_s, st),
+                        "sorted records")))
+            return rc;
+        if ((rc = check(fgs::launch_scan_tiles(gm.tiles_s, gm.offs, g->P, gm.partials,
+                                               (unsigned long long*)counts, st),
                         "tile offsets")))
             return rc;
     }
@@ -402,8 +414,8 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
         uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
         {
             ProfScope ps(ST_DUPLICATE, st);
-            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.means2D, radii, g->P, gx, gy,
-                                                  first, st), "duplicate")))
+            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.rects_s, g->P, gx, first, st),
+                            "duplicate")))
                 return rc;
         }
         {

@@ -37,6 +37,7 @@ struct PreOut {
     float4* conic_opacity;
     float* rgb;
     uint32_t* tiles;
+    uint4* recs;                        // packed tile rectangle (x0|y0<<16, x1|y1<<16), tiles, 0
     uint64_t* vkeys;                    // (depth_bits << 32 | i), culled: (~0 << 32 | i)
     int* error_flag;                    // bit 0: prefiltered point culled
 };
@@ -74,15 +75,15 @@ hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s);
 
 // scans / sort / binning
 size_t scan_partials_words(size_t n);
-hipError_t launch_scan_tiles_sorted(const uint64_t* vkeys, const uint32_t* tiles, uint32_t* offs,
-                                    int P, uint32_t* partials, unsigned long long* counters,
-                                    hipStream_t s);
+hipError_t launch_sorted_records(const uint64_t* vkeys, const uint4* recs, int P,
+                                 uint32_t* tiles_s, uint2* rects_s, hipStream_t s);
+hipError_t launch_scan_tiles(const uint32_t* tiles_s, uint32_t* offs, int P, uint32_t* partials,
+                             unsigned long long* counters, hipStream_t s);
 size_t radix_hist_words(size_t n);
 hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
                           uint32_t* hist, uint32_t* partials, hipStream_t s);
-hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const float2* means2D,
-                            const int* radii, int P, int gx, int gy, uint64_t* pairs,
-                            hipStream_t s);
+hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
+                            int P, int gx, uint64_t* pairs, hipStream_t s);
 hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T, hipStream_t s);
 
 // blend

@@ -76,8 +76,28 @@ __device__ __forceinline__ void stage_rows_fixed(const float* __restrict__ src, 
     }
 }
 
+// Contiguous rows (nu == rs, 16-byte aligned block): float4 loads.
+template <int NU>
+__device__ __forceinline__ void stage_rows_vec4(const float* __restrict__ src, int nrows, float* s)
+{
+    static_assert(NU % 4 == 0, "row length must be a multiple of 4 floats");
+    const float4* src4 = reinterpret_cast<const float4*>(src);
+    const int total4 = nrows * (NU / 4);
+    for (int k = threadIdx.x; k < total4; k += kPreThreads) {
+        const float4 v = src4[k];
+        const int t = k / (NU / 4);
+        const int f = (k - t * (NU / 4)) * 4;
+        s[(f + 0) * kPitch + t] = v.x;
+        s[(f + 1) * kPitch + t] = v.y;
+        s[(f + 2) * kPitch + t] = v.z;
+        s[(f + 3) * kPitch + t] = v.w;
+    }
+}
+
 __device__ __forceinline__ void stage_rows(const float* __restrict__ src, int rs, int nu, int nrows, float* s)
 {
+    if (nu == 48 && rs == 48 && ((uintptr_t)src & 15) == 0) { stage_rows_vec4<48>(src, nrows, s); return; }
+    if (nu == 12 && rs == 12 && ((uintptr_t)src & 15) == 0) { stage_rows_vec4<12>(src, nrows, s); return; }
     if (nu == 48) { stage_rows_fixed<48>(src, rs, nrows, s); return; }
     if (nu == 27) { stage_rows_fixed<27>(src, rs, nrows, s); return; }
     if (nu == 12) { stage_rows_fixed<12>(src, rs, nrows, s); return; }
@@ -102,8 +122,22 @@ __device__ __forceinline__ void unstage_rows_fixed(const float* s, int rs, int n
     }
 }
 
+template <int NU>
+__device__ __forceinline__ void unstage_rows_vec4(const float* s, int nrows, float* __restrict__ dst)
+{
+    float4* dst4 = reinterpret_cast<float4*>(dst);
+    const int total4 = nrows * (NU / 4);
+    for (int k = threadIdx.x; k < total4; k += kPreThreads) {
+        const int t = k / (NU / 4);
+        const int f = (k - t * (NU / 4)) * 4;
+        dst4[k] = make_float4(s[(f + 0) * kPitch + t], s[(f + 1) * kPitch + t],
+                              s[(f + 2) * kPitch + t], s[(f + 3) * kPitch + t]);
+    }
+}
+
 __device__ __forceinline__ void unstage_rows(const float* s, int rs, int nu, int nrows, float* __restrict__ dst)
 {
+    if (nu == 48 && rs == 48 && ((uintptr_t)dst & 15) == 0) { unstage_rows_vec4<48>(s, nrows, dst); return; }
     if (nu == 48) { unstage_rows_fixed<48>(s, rs, nrows, dst); return; }
     const int total = nrows * nu;
     for (int k = threadIdx.x; k < total; k += kPreThreads) {
@@ -156,6 +190,7 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOu
         float c3[6];
         Cov2D cv;
         float ca = 0.f, cb = 0.f, cc = 0.f, det = 0.f;
+        int rc[4] = { 0, 0, 0, 0 };
         if (alive) {
             if (a.cov3D_precomp) {
 #pragma unroll
@@ -211,7 +246,6 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOu
         if (alive) {
             mx = (float)((((double)projx + 1.0) * (double)a.W - 1.0) * 0.5);
             my = (float)((((double)projy + 1.0) * (double)a.H - 1.0) * 0.5);
-            int rc[4];
             get_rect(mx, my, radius_px, a.gx, a.gy, rc);
             tiles = (uint32_t)((rc[2] - rc[0]) * (rc[3] - rc[1]));
             alive = tiles != 0;
@@ -237,6 +271,9 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOu
         }
         o.radii[i] = radius_px;
         o.tiles[i] = tiles;
+        o.recs[i] = tiles ? make_uint4((uint32_t)rc[0] | ((uint32_t)rc[1] << 16),
+                                       (uint32_t)rc[2] | ((uint32_t)rc[3] << 16), tiles, 0u)
+                          : make_uint4(0u, 0u, 0u, 0u);
         o.means2D[i] = make_float2(mx, my);
         o.depths[i] = depth;
         o.conic_opacity[i] = con;

@@ -10,6 +10,9 @@
 //   2. duplicate_keys emits (tile << 32 | index) in that depth order;
 //   3. a stable LSD sort on the tile bits only (bit_length(T) bits) keeps the
 //      depth order inside every tile.
+// The tile rectangle of each Gaussian is packed by the preprocess and moved
+// into depth order once, so the offset scan and the duplication read
+// coalesced streams.
 // Stable LSD passes use 8-bit digits: a per-block digit histogram, a
 // device-wide exclusive scan of the digit-major histogram table, and a
 // scatter that ranks equal digits inside each 256-element slab with wave64
@@ -54,12 +57,6 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return wave_off + incl - v;
 }
 
-// Loader for the exclusive scan of tiles_touched in depth-sorted order.
-struct TilesSortedLoader {
-    const uint64_t* vkeys;
-    const uint32_t* tiles;
-    __device__ uint32_t operator()(size_t s) const { return tiles[(uint32_t)vkeys[s]]; }
-};
 struct PlainLoader {
     const uint32_t* in;
     __device__ uint32_t operator()(size_t s) const { return in[s]; }
@@ -147,13 +144,35 @@ static hipError_t exclusive_scan(Loader ld, size_t n, uint32_t* out, uint32_t* p
     return hipGetLastError();
 }
 
-hipError_t launch_scan_tiles_sorted(const uint64_t* vkeys, const uint32_t* tiles, uint32_t* offs,
-                                    int P, uint32_t* partials, unsigned long long* counters,
-                                    hipStream_t s)
+// Per sorted slot: tiles touched and packed tile rectangle of the Gaussian
+// (one 16-byte random read per Gaussian; everything after is coalesced).
+__global__ __launch_bounds__(256) void k_sorted_records(const uint64_t* __restrict__ vkeys,
+                                                        const uint4* __restrict__ recs, int P,
+                                                        uint32_t* __restrict__ tiles_s,
+                                                        uint2* __restrict__ rects_s)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P) return;
+    const uint4 r = recs[(uint32_t)vkeys[s]];
+    tiles_s[s] = r.z;
+    rects_s[s] = make_uint2(r.x, r.y);
+}
+
+hipError_t launch_sorted_records(const uint64_t* vkeys, const uint4* recs, int P,
+                                 uint32_t* tiles_s, uint2* rects_s, hipStream_t s)
+{
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sorted_records, dim3((P + 255) / 256), dim3(256), 0, s, vkeys, recs, P,
+                       tiles_s, rects_s);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_tiles(const uint32_t* tiles_s, uint32_t* offs, int P, uint32_t* partials,
+                             unsigned long long* counters, hipStream_t s)
 {
     // counters[0] += Gaussians with tiles > 0, counters[1] = sum of tiles (K)
-    return exclusive_scan(TilesSortedLoader{ vkeys, tiles }, (size_t)P, offs, partials, s,
-                          counters + 1, counters);
+    return exclusive_scan(PlainLoader{ tiles_s }, (size_t)P, offs, partials, s, counters + 1,
+                          counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -261,33 +280,26 @@ hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ vkeys,
                                                    const uint32_t* __restrict__ offs,
-                                                   const float2* __restrict__ means2D,
-                                                   const int* __restrict__ radii, int P, int gx,
-                                                   int gy, uint64_t* __restrict__ pairs)
+                                                   const uint2* __restrict__ rects_s, int P, int gx,
+                                                   uint64_t* __restrict__ pairs)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P) return;
-    const uint64_t vk = vkeys[s];
-    if ((uint32_t)(vk >> 32) == 0xFFFFFFFFu) return;
-    const uint32_t g = (uint32_t)vk;
-    const int r = radii[g];
-    if (r <= 0) return;
-    int rc[4];
-    const float2 m = means2D[g];
-    get_rect(m.x, m.y, r, gx, gy, rc);
+    const uint2 rc = rects_s[s];
+    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
+    if (x1 <= x0 || y1 <= y0) return;
+    const uint32_t g = (uint32_t)vkeys[s];
     uint32_t off = offs[s];
-    for (int y = rc[1]; y < rc[3]; ++y)
-        for (int x = rc[0]; x < rc[2]; ++x) {
-            pairs[off++] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | g;
-        }
+    for (uint32_t y = y0; y < y1; ++y)
+        for (uint32_t x = x0; x < x1; ++x) pairs[off++] = ((uint64_t)(y * (uint32_t)gx + x) << 32) | g;
 }
 
-hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const float2* means2D,
-                            const int* radii, int P, int gx, int gy, uint64_t* pairs, hipStream_t s)
+hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
+                            int P, int gx, uint64_t* pairs, hipStream_t s)
 {
     if (P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_duplicate, dim3((P + 255) / 256), dim3(256), 0, s, vkeys, offs, means2D,
-                       radii, P, gx, gy, pairs);
+    hipLaunchKernelGGL(k_duplicate, dim3((P + 255) / 256), dim3(256), 0, s, vkeys, offs, rects_s, P,
+                       gx, pairs);
     return hipGetLastError();
 }
 
