@@ -16,6 +16,10 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// Blend work shape (forward pixels per lane, forward batch, backward pixels
+// per lane, backward batch), process-wide; set by fgs_set_work_shape.
+int g_shape[4] = { 1, 256, 1, 256 };
+
 // Optional per-stage timing with HIP events recorded on the launch stream.
 enum Stage { ST_PREPROCESS, ST_DEPTH_SORT, ST_OFFSETS, ST_DUPLICATE, ST_TILE_SORT, ST_RANGES,
              ST_BLEND_FWD, ST_BLEND_BWD, ST_PREPROCESS_BWD, ST_COUNT };
@@ -277,6 +281,25 @@ __global__ void k_read_sorted(const uint64_t* pairs, const float* depths, size_t
 
 extern "C" {
 
+int fgs_set_work_shape(int32_t fwd_pix, int32_t fwd_batch, int32_t bwd_pix, int32_t bwd_batch)
+{
+    const int v[4] = { fwd_pix, fwd_batch, bwd_pix, bwd_batch };
+    for (int k = 0; k < 4; k += 2) {
+        if (v[k] != 1 && v[k] != 2) return fail(FGS_E_INVALID, "pixels per lane must be 1 or 2%s");
+        if (v[k + 1] != 64 && v[k + 1] != 128 && v[k + 1] != 256)
+            return fail(FGS_E_INVALID, "batch must be 64, 128 or 256%s");
+    }
+    for (int k = 0; k < 4; ++k) g_shape[k] = v[k];
+    return FGS_OK;
+}
+
+int fgs_get_work_shape(int32_t* out4)
+{
+    if (!out4) return fail(FGS_E_INVALID, "null output%s");
+    for (int k = 0; k < 4; ++k) out4[k] = g_shape[k];
+    return FGS_OK;
+}
+
 int fgs_profile_enable(int32_t on)
 {
     if (on && !g_ev_ready) {
@@ -406,6 +429,8 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
     b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
     b.conic_opacity = gm.conic_opacity;
     b.bg = s->bg;
+    b.pix_per_thread = g_shape[0];
+    b.batch = g_shape[1];
     if (K > 0) {
         if (!binning || !geom || !radii) return fail(FGS_E_INVALID, "null binning/geometry buffer%s");
         Binning bn = carve_binning(binning, K, nullptr);
@@ -467,6 +492,15 @@ void fgs_geom_offsets(int32_t P, size_t* out5)
     out5[4] = (size_t)((char*)g.tiles - probe);
 }
 
+void fgs_image_offsets(int32_t W, int32_t H, size_t* out3)
+{
+    static char probe[256];
+    Image im = carve_image(probe, W, H, nullptr);
+    out3[0] = (size_t)((char*)im.final_T - probe);
+    out3[1] = (size_t)((char*)im.n_contrib - probe);
+    out3[2] = (size_t)((char*)im.ranges - probe);
+}
+
 const uint32_t* fgs_image_ranges(const void* image, int32_t W, int32_t H)
 {
     return reinterpret_cast<const uint32_t*>(carve_image(const_cast<void*>(image), W, H, nullptr).ranges);
@@ -513,6 +547,8 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
         b.conic_opacity = gm.conic_opacity;
         b.bg = s->bg;
+        b.bwd_pix_per_thread = g_shape[2];
+        b.bwd_batch = g_shape[3];
         ProfScope ps(ST_BLEND_BWD, st);
         if ((rc = check(fgs::launch_blend_bwd(b, im.final_T, im.n_contrib, dL_dpix, dL_dmeans2D,
                                               dconic, dL_dopacity, dL_dcolors, s->grad_mode == 1, st),

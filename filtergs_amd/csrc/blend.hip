@@ -3,20 +3,25 @@
 // Forward: antialias/cuda_rasterizer/forward.cu:406-567 (renderCUDA).
 // Backward: antialias/cuda_rasterizer/backward.cu:399-557 (renderCUDA).
 //
-// One workgroup per 16x16 tile, one pixel per lane: 256 lanes = four wave64s,
-// each wave covering a 16x4 pixel strip.  Gaussians of the tile's range are
-// staged in LDS in batches of 256 together with a conservative bounding box
-// of the region where alpha >= 1/255 (the ellipse q <= 2 ln(255 o) of the
-// conic, slightly inflated); a wave whose strip misses that box skips the
-// Gaussian with one uniform branch -- no lane of the strip could blend it.
-// point_weight (the maximum T*alpha of each Gaussian over all pixels) is
-// reduced in LDS per batch and published with one float-as-uint atomicMax
-// per (tile, Gaussian), so it is exact and deterministic.
-// The backward visits only positions below the tile's largest n_contrib,
-// pre-reduces the nine per-pixel gradient terms of each Gaussian across the
-// wave with a reduce-scatter butterfly (ten shuffles for eight values, six for
-// the ninth), sums the four waves in LDS and issues one atomic add per term
-// per (tile, Gaussian).
+// One workgroup per 16x16 tile.  A lane owns PIX pixels of one column, rows
+// r, r+16/PIX, ... (PIX = 1: 256 lanes = four wave64s; PIX = 2: 128 lanes =
+// two wave64s), so a wave covers PIX bands of 16x4 pixels.  Gaussians of the
+// tile's range are staged in LDS in batches of BATCH together with a
+// conservative box of the region where alpha >= 1/255 (the ellipse
+// q <= 2 ln(255 o) of the conic, slightly inflated); a wave whose bands all
+// miss that box skips the Gaussian with one uniform branch -- none of its
+// pixels could blend it.  PIX and BATCH form the work shape picked by the
+// autotuner; the binning tile stays 16x16 (OGFS radii depend on it).
+//
+// Forward: point_weight (the maximum T*alpha of each Gaussian over all
+// pixels) is reduced in LDS per batch and published with one float-as-uint
+// atomicMax per (tile, Gaussian), so it is exact and deterministic.
+// Backward: only positions below the tile's largest n_contrib are visited;
+// the nine per-pixel gradient terms of each Gaussian are summed over the
+// lane's pixels, reduced across the wave (a DPP reduce-scatter inside 16-lane
+// rows, then two cross-row shuffles), added over the waves with LDS float
+// atomics, and published with one global atomic add per term per
+// (tile, Gaussian).
 #include "fgs_device.h"
 #include "fgs_internal.h"
 
@@ -46,124 +51,175 @@ __device__ __forceinline__ float4 alpha_box(float2 m, float4 co)
     return make_float4(m.x - e.x, m.x + e.x, m.y - e.y, m.y + e.y);
 }
 
-__device__ __forceinline__ bool strip_misses(float4 bb, float x0, float x1, float y0, float y1)
+// True when box bb misses every band of the wave (x range [x0, x1], rows
+// [y0 + k*rstep, y0 + k*rstep + 3] for k < PIX).
+template <int PIX>
+__device__ __forceinline__ bool bands_miss(float4 bb, float x0, float x1, float y0, float rstep)
 {
-    return bb.x > x1 || bb.y < x0 || bb.z > y1 || bb.w < y0;
+    if (bb.x > x1 || bb.y < x0) return true;
+    bool miss = true;
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) {
+        const float b0 = y0 + k * rstep;
+        miss = miss && (bb.z > b0 + 3.f || bb.w < b0);
+    }
+    return miss;
 }
 
-__global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __restrict__ final_T,
-                                                         uint32_t* __restrict__ n_contrib,
-                                                         float* __restrict__ out_color,
-                                                         int* __restrict__ out_point_id,
-                                                         float* __restrict__ out_weight_pixel,
-                                                         float* __restrict__ point_weight,
-                                                         float* __restrict__ kv_sum,
-                                                         uint32_t* __restrict__ kv_cnt)
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
 {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kDppRowMirror = 0x140;      // lane r <-> 15 - r inside each 16-lane row
+constexpr int kDppRowHalfMirror = 0x141;  // lane r <-> 7 - r inside each 8-lane half row
+constexpr int kDppXor2 = 0x4E;            // quad_perm [2,3,0,1]
+constexpr int kDppXor1 = 0xB1;            // quad_perm [1,0,3,2]
+
+// ---------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------
+template <int PIX, int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_fwd(BlendArgs a, float* __restrict__ final_T,
+                                                               uint32_t* __restrict__ n_contrib,
+                                                               float* __restrict__ out_color,
+                                                               int* __restrict__ out_point_id,
+                                                               float* __restrict__ out_weight_pixel,
+                                                               float* __restrict__ point_weight,
+                                                               float* __restrict__ kv_sum,
+                                                               uint32_t* __restrict__ kv_cnt)
+{
+    // Every product and sum of the per-pixel chain is written out (explicit
+    // fmaf, no contraction by the compiler), so all work shapes round alike
+    // and render bit-identical images.
+#pragma clang fp contract(off)
+    constexpr int NT = FGS_BLOCK / PIX;
+    constexpr int RSTEP = FGS_TILE / PIX;
     const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
     const int wave = tid / FGS_WAVE;
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
-    const int py = ty * FGS_TILE + (tid / FGS_TILE);
-    const bool inside = px < a.W && py < a.H;
-    const float pfx = (float)px, pfy = (float)py;
+    const int py0 = ty * FGS_TILE + (tid / FGS_TILE);
+    const float pfx = (float)px;
     const float sx0 = (float)(tx * FGS_TILE), sx1 = sx0 + (FGS_TILE - 1);
-    const float sy0 = (float)(ty * FGS_TILE + wave * 4), sy1 = sy0 + 3.f;
-    bool done = !inside;
+    const float sy0 = (float)(ty * FGS_TILE + wave * 4);
+
+    bool done[PIX];
+    float T[PIX], C0[PIX], C1[PIX], C2[PIX], wmax[PIX];
+    uint32_t last[PIX];
+    int max_id[PIX];
+    int ndone = 0;
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) {
+        done[k] = !(px < a.W && py0 + k * RSTEP < a.H);
+        ndone += done[k];
+        T[k] = 1.f;
+        C0[k] = C1[k] = C2[k] = 0.f;
+        wmax[k] = 0.f;
+        last[k] = 0u;
+        max_id[k] = -1;
+    }
 
     const uint2 range = a.ranges[tile];
     const int n = (int)(range.y - range.x);
-    const int rounds = (n + FGS_BLOCK - 1) / FGS_BLOCK;
+    const int rounds = (n + BATCH - 1) / BATCH;
 
-    __shared__ float4 s_g0[FGS_BLOCK];   // x, y, a', b'  (power in log2 units)
-    __shared__ float4 s_g1[FGS_BLOCK];   // c', opacity
-    __shared__ float4 s_col[FGS_BLOCK];  // r, g, b, id bits
-    __shared__ float4 s_bb[FGS_BLOCK];   // alpha >= 1/255 box
-    __shared__ uint32_t s_w[FGS_BLOCK];
-    __shared__ float s_kv[FGS_BLOCK];
-    __shared__ uint32_t s_kc[FGS_BLOCK];
+    __shared__ float4 s_g0[BATCH];   // x, y, a', b'  (power in log2 units)
+    __shared__ float2 s_g1[BATCH];   // c', opacity
+    __shared__ float4 s_col[BATCH];  // r, g, b, id bits
+    __shared__ float4 s_bb[BATCH];   // alpha >= 1/255 box
+    __shared__ uint32_t s_w[BATCH];
+    __shared__ float s_kv[BATCH];
+    __shared__ uint32_t s_kc[BATCH];
 
-    float T = 1.0f;
-    float C0 = 0.f, C1 = 0.f, C2 = 0.f;
-    uint32_t contributor = 0, last_contributor = 0;
-    float weight_max = 0.f;
-    int max_id = -1;
-    int todo = n;
-
-    for (int r = 0; r < rounds; ++r, todo -= FGS_BLOCK) {
-        const int num_done = __syncthreads_count(done);
-        if (num_done == FGS_BLOCK) break;
-        const int k = r * FGS_BLOCK + tid;
-        if (k < n) {
-            const uint32_t g = (uint32_t)a.pairs[range.x + k];
-            const float2 m = a.means2D[g];
-            const float4 co = a.conic_opacity[g];
-            s_g0[tid] = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
-            s_g1[tid] = make_float4(-0.5f * kLog2e * co.z, co.w, 0.f, 0.f);
-            s_col[tid] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
-                                     __uint_as_float(g));
-            s_bb[tid] = alpha_box(m, co);
+    for (int r = 0; r < rounds; ++r) {
+        const int num_done = __syncthreads_count(ndone == PIX);
+        if (num_done == NT) break;
+        for (int q = tid; q < BATCH; q += NT) {
+            const int k = r * BATCH + q;
+            if (k < n) {
+                const uint32_t g = (uint32_t)a.pairs[range.x + k];
+                const float2 m = a.means2D[g];
+                const float4 co = a.conic_opacity[g];
+                s_g0[q] = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                s_g1[q] = make_float2(-0.5f * kLog2e * co.z, co.w);
+                s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
+                                       __uint_as_float(g));
+                s_bb[q] = alpha_box(m, co);
+            }
+            s_w[q] = 0u;
+            s_kv[q] = 0.f;
+            s_kc[q] = 0u;
         }
-        s_w[tid] = 0u;
-        s_kv[tid] = 0.f;
-        s_kc[tid] = 0u;
         __syncthreads();
-        const int cnt = todo < FGS_BLOCK ? todo : FGS_BLOCK;
-        for (int j = 0; !done && j < cnt; ++j) {
-            contributor++;
-            if (strip_misses(s_bb[j], sx0, sx1, sy0, sy1)) continue;
+        const int cnt = min(BATCH, n - r * BATCH);
+        for (int j = 0; ndone < PIX && j < cnt; ++j) {
+            if (bands_miss<PIX>(s_bb[j], sx0, sx1, sy0, (float)RSTEP)) continue;
             const float4 g0 = s_g0[j];
-            const float4 g1 = s_g1[j];
-            const float dx = g0.x - pfx, dy = g0.y - pfy;
-            const float p2 = g0.z * dx * dx + g1.x * dy * dy + g0.w * dx * dy;
-            if (p2 > 0.0f) continue;
-            const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
-            if (alpha < 1.0f / 255.0f) continue;
-            const float test_T = T * (1.0f - alpha);
-            if (test_T < 0.0001f) {
-                done = true;
-                continue;
+            const float2 g1 = s_g1[j];
+            const float dx = g0.x - pfx;
+            const uint32_t position = (uint32_t)(r * BATCH + j + 1);
+#pragma unroll
+            for (int k = 0; k < PIX; ++k) {
+                if (done[k]) continue;
+                const float dy = g0.y - (float)(py0 + k * RSTEP);
+                const float p2 = fmaf(g0.z * dx, dx, fmaf(g1.x * dy, dy, (g0.w * dx) * dy));
+                if (p2 > 0.0f) continue;
+                const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+                if (alpha < 1.0f / 255.0f) continue;
+                const float test_T = T[k] * (1.0f - alpha);
+                if (test_T < 0.0001f) {
+                    done[k] = true;
+                    ++ndone;
+                    continue;
+                }
+                const float w = alpha * T[k];
+                const float4 col = s_col[j];
+                C0[k] = fmaf(col.x, w, C0[k]);
+                C1[k] = fmaf(col.y, w, C1[k]);
+                C2[k] = fmaf(col.z, w, C2[k]);
+                if (w > wmax[k]) {
+                    wmax[k] = w;
+                    max_id[k] = (int)__float_as_uint(col.w);
+                }
+                atomicMax(&s_w[j], __float_as_uint(w));
+                if (kv_sum) {
+                    atomicAdd(&s_kv[j], w);
+                    atomicAdd(&s_kc[j], 1u);
+                }
+                T[k] = test_T;
+                last[k] = position;
             }
-            const float w = alpha * T;
-            const float4 col = s_col[j];
-            C0 += col.x * w;
-            C1 += col.y * w;
-            C2 += col.z * w;
-            if (w > weight_max) {
-                weight_max = w;
-                max_id = (int)__float_as_uint(col.w);
-            }
-            atomicMax(&s_w[j], __float_as_uint(w));
-            if (kv_sum) {
-                atomicAdd(&s_kv[j], w);
-                atomicAdd(&s_kc[j], 1u);
-            }
-            T = test_T;
-            last_contributor = contributor;
         }
         __syncthreads();
-        if (k < n) {
-            const uint32_t g = __float_as_uint(s_col[tid].w);
-            if (s_w[tid] != 0u) atomicMax((uint32_t*)&point_weight[g], s_w[tid]);
-            if (kv_sum) {
-                // each kv pair belongs to exactly one tile: plain stores
-                kv_sum[range.x + k] = s_kv[tid];
-                kv_cnt[range.x + k] = s_kc[tid];
+        for (int q = tid; q < BATCH; q += NT) {
+            const int k = r * BATCH + q;
+            if (k < n) {
+                const uint32_t g = __float_as_uint(s_col[q].w);
+                if (s_w[q] != 0u) atomicMax((uint32_t*)&point_weight[g], s_w[q]);
+                if (kv_sum) {
+                    // each kv pair belongs to exactly one tile: plain stores
+                    kv_sum[range.x + k] = s_kv[q];
+                    kv_cnt[range.x + k] = s_kc[q];
+                }
             }
         }
     }
 
-    if (inside) {
+    const int HW = a.W * a.H;
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) {
+        const int py = py0 + k * RSTEP;
+        if (!(px < a.W && py < a.H)) continue;
         const int pix = a.W * py + px;
-        const int HW = a.W * a.H;
-        final_T[pix] = T;
-        n_contrib[pix] = last_contributor;
-        out_color[pix] = C0 + T * a.bg[0];
-        out_color[HW + pix] = C1 + T * a.bg[1];
-        out_color[2 * HW + pix] = C2 + T * a.bg[2];
-        out_point_id[pix] = max_id;
-        out_weight_pixel[pix] = weight_max;
+        final_T[pix] = T[k];
+        n_contrib[pix] = last[k];
+        out_color[pix] = fmaf(T[k], a.bg[0], C0[k]);
+        out_color[HW + pix] = fmaf(T[k], a.bg[1], C1[k]);
+        out_color[2 * HW + pix] = fmaf(T[k], a.bg[2], C2[k]);
+        out_point_id[pix] = max_id[k];
+        out_weight_pixel[pix] = wmax[k];
     }
 }
 
@@ -173,51 +229,71 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blend_fwd, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib, out_color,
-                       out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
+#define FGS_FWD(PIX, BATCH)                                                                          \
+    hipLaunchKernelGGL((k_blend_fwd<PIX, BATCH>), dim3(T), dim3(FGS_BLOCK / PIX), 0, s, a, final_T, \
+                       n_contrib, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum,   \
+                       kv_cnt)
+    const int pix = a.pix_per_thread == 2 ? 2 : 1;
+    const int batch = a.batch == 128 ? 128 : (a.batch == 64 ? 64 : 256);
+    if (pix == 1 && batch == 256) FGS_FWD(1, 256);
+    else if (pix == 1 && batch == 128) FGS_FWD(1, 128);
+    else if (pix == 1) FGS_FWD(1, 64);
+    else if (batch == 256) FGS_FWD(2, 256);
+    else if (batch == 128) FGS_FWD(2, 128);
+    else FGS_FWD(2, 64);
+#undef FGS_FWD
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// Backward.
+// Backward
 // ---------------------------------------------------------------------------
 
-// Reduce-scatter of 8 values across the wave: on return, lanes whose low
-// three bits are zero hold the wave total of value (lane >> 3).
+// Reduce-scatter of 8 values across the wave.  Inside every 16-lane row the
+// halving steps pair lanes with DPP permutations (mirror: bit 3, half mirror:
+// bit 2, quad_perm: bits 1 and 0), leaving in lane r the row sum of value
+// 4*bit3 + 2*bit2 + bit1; two shuffles then add the four rows.  On return
+// lane l < 16 holds the wave total of value l >> 1.
 __device__ __forceinline__ float wave_reduce_scatter8(const float v[8])
 {
     const int lane = lane_id();
+    const bool h3 = (lane >> 3) & 1, h2 = (lane >> 2) & 1, h1 = (lane >> 1) & 1;
     float w4[4];
-    {
-        const bool h = (lane >> 5) & 1;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float send = h ? v[k] : v[k + 4];
-            const float keep = h ? v[k + 4] : v[k];
-            w4[k] = keep + __shfl_xor(send, 32, FGS_WAVE);
-        }
+    for (int k = 0; k < 4; ++k) {
+        const float send = h3 ? v[k] : v[k + 4];
+        const float keep = h3 ? v[k + 4] : v[k];
+        w4[k] = keep + dpp<kDppRowMirror>(send);
     }
     float w2[2];
-    {
-        const bool h = (lane >> 4) & 1;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const float send = h ? w4[k] : w4[k + 2];
-            const float keep = h ? w4[k + 2] : w4[k];
-            w2[k] = keep + __shfl_xor(send, 16, FGS_WAVE);
-        }
+    for (int k = 0; k < 2; ++k) {
+        const float send = h2 ? w4[k] : w4[k + 2];
+        const float keep = h2 ? w4[k + 2] : w4[k];
+        w2[k] = keep + dpp<kDppRowHalfMirror>(send);
     }
     float z;
     {
-        const bool h = (lane >> 3) & 1;
-        const float send = h ? w2[0] : w2[1];
-        const float keep = h ? w2[1] : w2[0];
-        z = keep + __shfl_xor(send, 8, FGS_WAVE);
+        const float send = h1 ? w2[0] : w2[1];
+        const float keep = h1 ? w2[1] : w2[0];
+        z = keep + dpp<kDppXor2>(send);
     }
-    z += __shfl_xor(z, 4, FGS_WAVE);
-    z += __shfl_xor(z, 2, FGS_WAVE);
-    z += __shfl_xor(z, 1, FGS_WAVE);
+    z += dpp<kDppXor1>(z);
+    z += __shfl_xor(z, 16, FGS_WAVE);
+    z += __shfl_xor(z, 32, FGS_WAVE);
     return z;
+}
+
+// Wave total of one value (every lane receives it).
+__device__ __forceinline__ float wave_total(float x)
+{
+    x += dpp<0x128>(x);   // row_ror 8
+    x += dpp<0x124>(x);   // row_ror 4
+    x += dpp<0x122>(x);   // row_ror 2
+    x += dpp<0x121>(x);   // row_ror 1
+    x += __shfl_xor(x, 16, FGS_WAVE);
+    x += __shfl_xor(x, 32, FGS_WAVE);
+    return x;
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -232,147 +308,159 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 
 constexpr int kGradTerms = 9;   // dcolor x3, dmean2D x2, dconic x3, dopacity
 
-__global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* __restrict__ final_Ts,
-                                                         const uint32_t* __restrict__ n_contrib,
-                                                         const float* __restrict__ dL_dpix,
-                                                         float* __restrict__ dL_dmean2D,
-                                                         float* __restrict__ dL_dconic,
-                                                         float* __restrict__ dL_dopacity,
-                                                         float* __restrict__ dL_dcolors, int exact)
+template <int PIX, int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_bwd(BlendArgs a, const float* __restrict__ final_Ts,
+                                                               const uint32_t* __restrict__ n_contrib,
+                                                               const float* __restrict__ dL_dpix,
+                                                               float* __restrict__ dL_dmean2D,
+                                                               float* __restrict__ dL_dconic,
+                                                               float* __restrict__ dL_dopacity,
+                                                               float* __restrict__ dL_dcolors, int exact)
 {
+    constexpr int NT = FGS_BLOCK / PIX;
+    constexpr int NW = NT / FGS_WAVE;
+    constexpr int RSTEP = FGS_TILE / PIX;
     const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
     const int wave = tid / FGS_WAVE;
     const int lane = lane_id();
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
-    const int py = ty * FGS_TILE + (tid / FGS_TILE);
-    const bool inside = px < a.W && py < a.H;
-    const float pfx = (float)px, pfy = (float)py;
+    const int py0 = ty * FGS_TILE + (tid / FGS_TILE);
+    const float pfx = (float)px;
     const float sx0 = (float)(tx * FGS_TILE), sx1 = sx0 + (FGS_TILE - 1);
-    const float sy0 = (float)(ty * FGS_TILE + wave * 4), sy1 = sy0 + 3.f;
-    const int pix = a.W * py + px;
+    const float sy0 = (float)(ty * FGS_TILE + wave * 4);
     const int HW = a.W * a.H;
 
     const uint2 range = a.ranges[tile];
 
-    __shared__ float4 s_g0[FGS_BLOCK];   // x, y, A, B
-    __shared__ float4 s_g1[FGS_BLOCK];   // C, opacity
-    __shared__ float4 s_col[FGS_BLOCK];  // r, g, b, id bits
-    __shared__ float4 s_bb[FGS_BLOCK];
-    __shared__ float s_g[FGS_BLOCK / FGS_WAVE][kGradTerms][FGS_BLOCK];
-    __shared__ uint32_t s_wmax[FGS_BLOCK / FGS_WAVE];
+    __shared__ float4 s_g0[BATCH];   // x, y, A, B
+    __shared__ float2 s_g1[BATCH];   // C, opacity
+    __shared__ float4 s_col[BATCH];  // r, g, b, id bits
+    __shared__ float4 s_bb[BATCH];
+    __shared__ float s_g[kGradTerms][BATCH];
+    __shared__ uint32_t s_wmax[NW];
 
-    const float T_final = inside ? final_Ts[pix] : 0.f;
-    float T = T_final;
-    const uint32_t last_contributor = inside ? n_contrib[pix] : 0u;
+    float T_final[PIX], T[PIX], dp0[PIX], dp1[PIX], dp2[PIX], bg_dot[PIX];
+    float acc0[PIX], acc1[PIX], acc2[PIX], last_alpha[PIX], lc0[PIX], lc1[PIX], lc2[PIX];
+    uint32_t last[PIX];
+    uint32_t lmax = 0;
+#pragma unroll
+    for (int k = 0; k < PIX; ++k) {
+        const int py = py0 + k * RSTEP;
+        const bool inside = px < a.W && py < a.H;
+        const int pix = a.W * py + px;
+        T_final[k] = inside ? final_Ts[pix] : 0.f;
+        T[k] = T_final[k];
+        last[k] = inside ? n_contrib[pix] : 0u;
+        lmax = last[k] > lmax ? last[k] : lmax;
+        dp0[k] = inside ? dL_dpix[pix] : 0.f;
+        dp1[k] = inside ? dL_dpix[HW + pix] : 0.f;
+        dp2[k] = inside ? dL_dpix[2 * HW + pix] : 0.f;
+        bg_dot[k] = a.bg[0] * dp0[k] + a.bg[1] * dp1[k] + a.bg[2] * dp2[k];
+        acc0[k] = acc1[k] = acc2[k] = 0.f;
+        last_alpha[k] = 0.f;
+        lc0[k] = lc1[k] = lc2[k] = 0.f;
+    }
     // positions >= the largest n_contrib of the tile contribute to no pixel
-    const uint32_t wmax = wave_max_u32(last_contributor);
+    const uint32_t wmax = wave_max_u32(lmax);
     if (lane == 0) s_wmax[wave] = wmax;
     __syncthreads();
     uint32_t bmax = 0;
 #pragma unroll
-    for (int w = 0; w < FGS_BLOCK / FGS_WAVE; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
+    for (int w = 0; w < NW; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
     const int n = (int)min(bmax, range.y - range.x);
-    const int rounds = (n + FGS_BLOCK - 1) / FGS_BLOCK;
-
-    float dpix0 = 0.f, dpix1 = 0.f, dpix2 = 0.f;
-    if (inside) {
-        dpix0 = dL_dpix[pix];
-        dpix1 = dL_dpix[HW + pix];
-        dpix2 = dL_dpix[2 * HW + pix];
-    }
-    const float bg_dot = a.bg[0] * dpix0 + a.bg[1] * dpix1 + a.bg[2] * dpix2;
-    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-    float last_alpha = 0.f;
-    float lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
+    const int rounds = (n + BATCH - 1) / BATCH;
     const float ddelx_dx = 0.5f * (float)a.W;
     const float ddely_dy = 0.5f * (float)a.H;
-    int todo = n;
 
-    for (int r = 0; r < rounds; ++r, todo -= FGS_BLOCK) {
+    for (int r = 0; r < rounds; ++r) {
         __syncthreads();
-        const int k = r * FGS_BLOCK + tid;
-        if (k < n) {
-            // back to front: position n-1-k of the tile's list
-            const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
-            const float2 m = a.means2D[g];
-            const float4 co = a.conic_opacity[g];
-            s_g0[tid] = make_float4(m.x, m.y, co.x, co.y);
-            s_g1[tid] = make_float4(co.z, co.w, 0.f, 0.f);
-            s_col[tid] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
-                                     __uint_as_float(g));
-            s_bb[tid] = alpha_box(m, co);
+        for (int q = tid; q < BATCH; q += NT) {
+            const int k = r * BATCH + q;
+            if (k < n) {
+                // back to front: position n-1-k of the tile's list
+                const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
+                const float2 m = a.means2D[g];
+                const float4 co = a.conic_opacity[g];
+                s_g0[q] = make_float4(m.x, m.y, co.x, co.y);
+                s_g1[q] = make_float2(co.z, co.w);
+                s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
+                                       __uint_as_float(g));
+                s_bb[q] = alpha_box(m, co);
+            }
+#pragma unroll
+            for (int t = 0; t < kGradTerms; ++t) s_g[t][q] = 0.f;
         }
-#pragma unroll
-        for (int w = 0; w < FGS_BLOCK / FGS_WAVE; ++w)
-#pragma unroll
-            for (int t = 0; t < kGradTerms; ++t) s_g[w][t][tid] = 0.f;
         __syncthreads();
-        const int cnt = todo < FGS_BLOCK ? todo : FGS_BLOCK;
+        const int cnt = min(BATCH, n - r * BATCH);
         for (int j = 0; j < cnt; ++j) {
-            const uint32_t position = (uint32_t)(n - 1 - (r * FGS_BLOCK + j));
-            if (position >= wmax) continue;                           // wave-uniform
-            if (strip_misses(s_bb[j], sx0, sx1, sy0, sy1)) continue;  // wave-uniform
+            const uint32_t position = (uint32_t)(n - 1 - (r * BATCH + j));
+            if (position >= wmax) continue;                                        // wave-uniform
+            if (bands_miss<PIX>(s_bb[j], sx0, sx1, sy0, (float)RSTEP)) continue;  // wave-uniform
             float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
             float vop = 0.f;
             bool act = false;
-            if (position < last_contributor) {
-                const float4 g0 = s_g0[j];
-                const float4 g1 = s_g1[j];
-                const float dx = g0.x - pfx, dy = g0.y - pfy;
+            const float4 g0 = s_g0[j];
+            const float2 g1 = s_g1[j];
+            const float dx = g0.x - pfx;
+#pragma unroll
+            for (int k = 0; k < PIX; ++k) {
+                if (position >= last[k]) continue;
+                const float dy = g0.y - (float)(py0 + k * RSTEP);
                 const float power = -0.5f * (g0.z * dx * dx + g1.x * dy * dy) - g0.w * dx * dy;
-                if (power <= 0.0f) {
-                    const float G = __builtin_amdgcn_exp2f(power * kLog2e);
-                    const float alpha_raw = g1.y * G;
-                    const float alpha = fminf(0.99f, alpha_raw);
-                    if (alpha >= 1.0f / 255.0f) {
-                        act = true;
-                        const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-                        T = T * inv;
-                        const float dch = alpha * T;
-                        const float4 col = s_col[j];
-                        acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-                        acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-                        acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-                        lc0 = col.x;
-                        lc1 = col.y;
-                        lc2 = col.z;
-                        float dL_dalpha = (col.x - acc0) * dpix0 + (col.y - acc1) * dpix1 + (col.z - acc2) * dpix2;
-                        v[0] = dch * dpix0;
-                        v[1] = dch * dpix1;
-                        v[2] = dch * dpix2;
-                        dL_dalpha *= T;
-                        last_alpha = alpha;
-                        dL_dalpha += (-T_final * inv) * bg_dot;
-                        const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
-                        const float dL_dG = g1.y * dL_dalpha * slope;
-                        const float gdx = G * dx, gdy = G * dy;
-                        const float dG_ddelx = -gdx * g0.z - gdy * g0.w;
-                        const float dG_ddely = -gdy * g1.x - gdx * g0.w;
-                        v[3] = dL_dG * dG_ddelx * ddelx_dx;
-                        v[4] = dL_dG * dG_ddely * ddely_dy;
-                        v[5] = -0.5f * gdx * dx * dL_dG;
-                        v[6] = -0.5f * gdx * dy * dL_dG;
-                        v[7] = -0.5f * gdy * dy * dL_dG;
-                        vop = G * dL_dalpha * slope;
-                    }
-                }
+                if (power > 0.0f) continue;
+                const float G = __builtin_amdgcn_exp2f(power * kLog2e);
+                const float alpha_raw = g1.y * G;
+                const float alpha = fminf(0.99f, alpha_raw);
+                if (alpha < 1.0f / 255.0f) continue;
+                act = true;
+                const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+                T[k] = T[k] * inv;
+                const float dch = alpha * T[k];
+                const float4 col = s_col[j];
+                acc0[k] = last_alpha[k] * lc0[k] + (1.f - last_alpha[k]) * acc0[k];
+                acc1[k] = last_alpha[k] * lc1[k] + (1.f - last_alpha[k]) * acc1[k];
+                acc2[k] = last_alpha[k] * lc2[k] + (1.f - last_alpha[k]) * acc2[k];
+                lc0[k] = col.x;
+                lc1[k] = col.y;
+                lc2[k] = col.z;
+                float dL_dalpha = (col.x - acc0[k]) * dp0[k] + (col.y - acc1[k]) * dp1[k] +
+                                  (col.z - acc2[k]) * dp2[k];
+                v[0] += dch * dp0[k];
+                v[1] += dch * dp1[k];
+                v[2] += dch * dp2[k];
+                dL_dalpha *= T[k];
+                last_alpha[k] = alpha;
+                dL_dalpha += (-T_final[k] * inv) * bg_dot[k];
+                const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
+                const float dL_dG = g1.y * dL_dalpha * slope;
+                const float gdx = G * dx, gdy = G * dy;
+                const float dG_ddelx = -gdx * g0.z - gdy * g0.w;
+                const float dG_ddely = -gdy * g1.x - gdx * g0.w;
+                v[3] += dL_dG * dG_ddelx * ddelx_dx;
+                v[4] += dL_dG * dG_ddely * ddely_dy;
+                v[5] += -0.5f * gdx * dx * dL_dG;
+                v[6] += -0.5f * gdx * dy * dL_dG;
+                v[7] += -0.5f * gdy * dy * dL_dG;
+                vop += G * dL_dalpha * slope;
             }
             if (__any(act)) {
+                // the waves meet on the same slot: LDS float atomics
                 const float z = wave_reduce_scatter8(v);
-                const float zop = wave_sum(vop);
-                if ((lane & 7) == 0) s_g[wave][lane >> 3][j] = z;
-                if (lane == 0) s_g[wave][8][j] = zop;
+                const float zop = wave_total(vop);
+                if (lane < 16 && (lane & 1) == 0) atomicAdd(&s_g[lane >> 1][j], z);
+                if (lane == 16) atomicAdd(&s_g[8][j], zop);
             }
         }
         __syncthreads();
-        if (k < n) {
-            const uint32_t g = __float_as_uint(s_col[tid].w);
+        for (int q = tid; q < BATCH; q += NT) {
+            const int k = r * BATCH + q;
+            if (k >= n) continue;
+            const uint32_t g = __float_as_uint(s_col[q].w);
             float t[kGradTerms];
 #pragma unroll
-            for (int q = 0; q < kGradTerms; ++q)
-                t[q] = s_g[0][q][tid] + s_g[1][q][tid] + s_g[2][q][tid] + s_g[3][q][tid];
+            for (int e = 0; e < kGradTerms; ++e) t[e] = s_g[e][q];
             if (t[0] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 0], t[0]);
             if (t[1] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 1], t[1]);
             if (t[2] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 2], t[2]);
@@ -392,8 +480,18 @@ hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blend_bwd, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib, dL_dpix,
-                       dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact);
+#define FGS_BWD(PIX, BATCH)                                                                          \
+    hipLaunchKernelGGL((k_blend_bwd<PIX, BATCH>), dim3(T), dim3(FGS_BLOCK / PIX), 0, s, a, final_T, \
+                       n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact)
+    const int pix = a.bwd_pix_per_thread == 2 ? 2 : 1;
+    const int batch = a.bwd_batch == 128 ? 128 : (a.bwd_batch == 64 ? 64 : 256);
+    if (pix == 1 && batch == 256) FGS_BWD(1, 256);
+    else if (pix == 1 && batch == 128) FGS_BWD(1, 128);
+    else if (pix == 1) FGS_BWD(1, 64);
+    else if (batch == 256) FGS_BWD(2, 256);
+    else if (batch == 128) FGS_BWD(2, 128);
+    else FGS_BWD(2, 64);
+#undef FGS_BWD
     return hipGetLastError();
 }
 

@@ -298,6 +298,23 @@ def image_ranges(imageBuffer, image_width, image_height):
     return imageBuffer[off:off + 8 * T].view(torch.int32).view(T, 2)
 
 
+def image_views(imageBuffer, image_width, image_height):
+    """Views of the per-pixel state inside an image buffer:
+    dict(final_T [H,W] float32, n_contrib [H,W] int32, ranges [T,2] int32)."""
+    W, H = int(image_width), int(image_height)
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    offs = (ctypes.c_size_t * 3)()
+    _lib.load().fgs_image_offsets(W, H, offs)
+
+    def view(i, n, dtype, shape):
+        o = int(offs[i])
+        return imageBuffer[o:o + 4 * n].view(dtype).view(shape)
+
+    return dict(final_T=view(0, W * H, torch.float32, (H, W)),
+                n_contrib=view(1, W * H, torch.int32, (H, W)),
+                ranges=view(2, 2 * T, torch.int32, (T, 2)))
+
+
 def geom_views(geomBuffer, P):
     """Views of the preprocess outputs inside a geometry buffer:
     dict(means2D [P,2], depths [P], conic_opacity [P,4], rgb [P,3], tiles [P])."""

@@ -118,6 +118,10 @@ int fgs_read_sorted(const void* geom, int32_t P, int64_t K, const void* binning,
  * depths (float), conic_opacity (float4), rgb (float[3]), tiles_touched
  * (uint32) -- the preprocess outputs, for inspection. */
 void fgs_geom_offsets(int32_t P, size_t* out5);
+/* Byte offsets inside an image buffer of: final T (float [H*W]), n_contrib
+ * (uint32 [H*W]), tile ranges (uint32 pairs [T,2]) -- the per-pixel state
+ * renderCUDA leaves for the backward (forward.cu:552-553), for inspection. */
+void fgs_image_offsets(int32_t image_width, int32_t image_height, size_t* out3);
 /* Device pointer to the tile ranges (uint32 pairs [T,2]) inside an image buffer. */
 const uint32_t* fgs_image_ranges(const void* image, int32_t image_width, int32_t image_height);
 
@@ -193,6 +197,14 @@ int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const 
  * accumulated milliseconds / call counts and returns the number of stages. */
 int fgs_profile_enable(int32_t on);
 int fgs_profile_read(double* ms, uint64_t* calls, int32_t n, int32_t reset);
+/* Blend work shape, process-wide: pixels per lane (1 or 2) and Gaussians
+ * staged in LDS per batch (64, 128 or 256), for the forward and backward
+ * blend.  Images and integer outputs do not depend on it (the 16x16
+ * binning tile is fixed); gradients change only in float summation order.  No counterpart in the reference, whose renderCUDA
+ * (forward.cu:406, backward.cu:399) fixes one pixel per thread and
+ * BLOCK_SIZE batches; the autotuner picks it. */
+int fgs_set_work_shape(int32_t fwd_pix, int32_t fwd_batch, int32_t bwd_pix, int32_t bwd_batch);
+int fgs_get_work_shape(int32_t* out4);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */
