@@ -106,7 +106,7 @@ _SIGS = {
                                         c_void_p, c_void_p, c_void_p]),
     "fgs_profile_enable": (c_int32, [c_int32]),
     "fgs_profile_read": (c_int32, [c_void_p, c_void_p, c_int32, c_int32]),
-    "fgs_set_work_shape": (c_int32, [c_int32, c_int32, c_int32, c_int32]),
+    "fgs_set_work_shape": (c_int32, [c_int32, c_int32]),
     "fgs_get_work_shape": (c_int32, [c_void_p]),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_abi_version": (c_int32, []),
@@ -183,12 +183,12 @@ def profile_read(reset=True):
     return {STAGES[k]: (float(ms[k]), int(calls[k])) for k in range(n)}
 
 
-def set_work_shape(fwd_pix=1, fwd_batch=256, bwd_pix=1, bwd_batch=256):
-    """Blend work shape: pixels per lane (1, 2) and LDS batch (64, 128, 256)."""
-    call("fgs_set_work_shape", int(fwd_pix), int(fwd_batch), int(bwd_pix), int(bwd_batch))
+def set_work_shape(fwd_batch=256, bwd_batch=128):
+    """Blend work shape: Gaussians per LDS batch (forward 128/256, backward 64/128/256)."""
+    call("fgs_set_work_shape", int(fwd_batch), int(bwd_batch))
 
 
 def get_work_shape():
-    out = (ctypes.c_int32 * 4)()
+    out = (ctypes.c_int32 * 2)()
     call("fgs_get_work_shape", out)
     return tuple(int(v) for v in out)

@@ -16,9 +16,9 @@ namespace {
 
 thread_local std::string g_last_error;
 
-// Blend work shape (forward pixels per lane, forward batch, backward pixels
-// per lane, backward batch), process-wide; set by fgs_set_work_shape.
-int g_shape[4] = { 1, 256, 1, 256 };
+// Blend work shape (Gaussians per LDS batch of the forward and of the
+// backward), process-wide; set by fgs_set_work_shape.
+int g_shape[2] = { 256, 128 };
 
 // Optional per-stage timing with HIP events recorded on the launch stream.
 enum Stage { ST_PREPROCESS, ST_DEPTH_SORT, ST_OFFSETS, ST_DUPLICATE, ST_TILE_SORT, ST_RANGES,
@@ -281,22 +281,22 @@ __global__ void k_read_sorted(const uint64_t* pairs, const float* depths, size_t
 
 extern "C" {
 
-int fgs_set_work_shape(int32_t fwd_pix, int32_t fwd_batch, int32_t bwd_pix, int32_t bwd_batch)
+int fgs_set_work_shape(int32_t fwd_batch, int32_t bwd_batch)
 {
-    const int v[4] = { fwd_pix, fwd_batch, bwd_pix, bwd_batch };
-    for (int k = 0; k < 4; k += 2) {
-        if (v[k] != 1 && v[k] != 2) return fail(FGS_E_INVALID, "pixels per lane must be 1 or 2%s");
-        if (v[k + 1] != 64 && v[k + 1] != 128 && v[k + 1] != 256)
-            return fail(FGS_E_INVALID, "batch must be 64, 128 or 256%s");
-    }
-    for (int k = 0; k < 4; ++k) g_shape[k] = v[k];
+    if (fwd_batch != 128 && fwd_batch != 256)
+        return fail(FGS_E_INVALID, "forward batch must be 128 or 256%s");
+    if (bwd_batch != 64 && bwd_batch != 128 && bwd_batch != 256)
+        return fail(FGS_E_INVALID, "backward batch must be 64, 128 or 256%s");
+    g_shape[0] = fwd_batch;
+    g_shape[1] = bwd_batch;
     return FGS_OK;
 }
 
-int fgs_get_work_shape(int32_t* out4)
+int fgs_get_work_shape(int32_t* out2)
 {
-    if (!out4) return fail(FGS_E_INVALID, "null output%s");
-    for (int k = 0; k < 4; ++k) out4[k] = g_shape[k];
+    if (!out2) return fail(FGS_E_INVALID, "null output%s");
+    out2[0] = g_shape[0];
+    out2[1] = g_shape[1];
     return FGS_OK;
 }
 
@@ -429,8 +429,7 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
     b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
     b.conic_opacity = gm.conic_opacity;
     b.bg = s->bg;
-    b.pix_per_thread = g_shape[0];
-    b.batch = g_shape[1];
+    b.batch = g_shape[0];
     if (K > 0) {
         if (!binning || !geom || !radii) return fail(FGS_E_INVALID, "null binning/geometry buffer%s");
         Binning bn = carve_binning(binning, K, nullptr);
@@ -547,8 +546,7 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
         b.conic_opacity = gm.conic_opacity;
         b.bg = s->bg;
-        b.bwd_pix_per_thread = g_shape[2];
-        b.bwd_batch = g_shape[3];
+        b.bwd_batch = g_shape[1];
         ProfScope ps(ST_BLEND_BWD, st);
         if ((rc = check(fgs::launch_blend_bwd(b, im.final_T, im.n_contrib, dL_dpix, dL_dmeans2D,
                                               dconic, dL_dopacity, dL_dcolors, s->grad_mode == 1, st),

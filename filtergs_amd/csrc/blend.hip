@@ -3,31 +3,30 @@
 // Forward: antialias/cuda_rasterizer/forward.cu:406-567 (renderCUDA).
 // Backward: antialias/cuda_rasterizer/backward.cu:399-557 (renderCUDA).
 //
-// One workgroup per 16x16 tile.  A lane owns PIX pixels of one column, rows
-// r, r+16/PIX, ... (PIX = 1: 256 lanes = four wave64s; PIX = 2: 128 lanes =
-// two wave64s), so a wave covers PIX bands of 16x4 pixels.  Gaussians of the
-// tile's range are staged in LDS in batches of BATCH together with a
-// conservative box of the region where alpha >= 1/255 (the ellipse
-// q <= 2 ln(255 o) of the conic, slightly inflated); a wave whose bands all
-// miss that box skips the Gaussian with one uniform branch -- none of its
-// pixels could blend it.  PIX and BATCH form the work shape picked by the
-// autotuner; the binning tile stays 16x16 (OGFS radii depend on it).
+// One workgroup of 256 lanes (four wave64s) per 16x16 tile, one pixel per
+// lane; wave w owns the 16x4 strip of rows 4w..4w+3.  Gaussians of the tile's
+// range are staged in LDS in batches of BATCH.  While staging, every Gaussian
+// is tested against the four strips with a conservative box of the region
+// where alpha >= 1/255 (the ellipse q <= 2 ln(255 o) of the conic, slightly
+// inflated), and each wave gets a compacted list of the batch positions that
+// can touch its strip (ballot + prefix count, order preserved).  A wave then
+// walks only its own list: no per-Gaussian cull test in the inner loop.
 //
 // Forward: point_weight (the maximum T*alpha of each Gaussian over all
 // pixels) is reduced in LDS per batch and published with one float-as-uint
 // atomicMax per (tile, Gaussian), so it is exact and deterministic.
-// Backward: only positions below the tile's largest n_contrib are visited;
-// the nine per-pixel gradient terms of each Gaussian are summed over the
-// lane's pixels, reduced across the wave (a DPP reduce-scatter inside 16-lane
-// rows, then two cross-row shuffles), added over the waves with LDS float
-// atomics, and published with one global atomic add per term per
-// (tile, Gaussian).
+// Backward: only positions below the strip's largest n_contrib enter the
+// lists; the nine per-pixel gradient terms of each Gaussian are reduced
+// across the wave (a DPP reduce-scatter inside 16-lane rows, then two
+// cross-row shuffles), added over the waves with LDS float atomics, and
+// published with one global atomic add per term per (tile, Gaussian).
 #include "fgs_device.h"
 #include "fgs_internal.h"
 
 namespace fgs {
 
 constexpr float kLog2e = 1.4426950408889634f;
+constexpr int kStrips = FGS_BLOCK / FGS_WAVE;   // 4 waves, one 16x4 strip each
 
 // Conservative half extents of {alpha >= 1/255} for conic (A, B, C) and
 // opacity o, centred on the mean; negative when the Gaussian can never reach
@@ -51,19 +50,54 @@ __device__ __forceinline__ float4 alpha_box(float2 m, float4 co)
     return make_float4(m.x - e.x, m.x + e.x, m.y - e.y, m.y + e.y);
 }
 
-// True when box bb misses every band of the wave (x range [x0, x1], rows
-// [y0 + k*rstep, y0 + k*rstep + 3] for k < PIX).
-template <int PIX>
-__device__ __forceinline__ bool bands_miss(float4 bb, float x0, float x1, float y0, float rstep)
+// Bit w set when box bb (x0, x1, y0, y1) can touch strip w of the tile whose
+// top-left pixel is (tx0, ty0).
+__device__ __forceinline__ uint32_t strip_bits(float4 bb, float tx0, float ty0)
 {
-    if (bb.x > x1 || bb.y < x0) return true;
-    bool miss = true;
+    if (bb.x > tx0 + (FGS_TILE - 1) || bb.y < tx0) return 0u;
+    uint32_t bits = 0u;
 #pragma unroll
-    for (int k = 0; k < PIX; ++k) {
-        const float b0 = y0 + k * rstep;
-        miss = miss && (bb.z > b0 + 3.f || bb.w < b0);
+    for (int w = 0; w < kStrips; ++w) {
+        const float y0 = ty0 + 4.f * w;
+        if (!(bb.z > y0 + 3.f || bb.w < y0)) bits |= 1u << w;
     }
-    return miss;
+    return bits;
+}
+
+// Per-wave compacted lists of batch positions.  Every lane of the block calls
+// it (q = threadIdx.x is the batch slot it staged; bits its strip mask, zero
+// for empty slots).  Two barriers; on return s_list[w][0 .. count(w)) holds
+// the slots whose bit w is set, in increasing order, and the count for the
+// calling wave is returned.
+template <int BATCH>
+__device__ __forceinline__ int build_lists(uint32_t bits, uint16_t (*s_list)[BATCH],
+                                           uint32_t (*s_wcnt)[kStrips])
+{
+    const int tid = threadIdx.x;
+    const int wave = tid / FGS_WAVE;
+    const int lane = lane_id();
+    int rank[kStrips];
+#pragma unroll
+    for (int w = 0; w < kStrips; ++w) {
+        const uint64_t bal = __ballot((bits >> w) & 1u);
+        rank[w] = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+        if (lane == 0) s_wcnt[wave][w] = (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kStrips; ++w) {
+        if ((bits >> w) & 1u) {
+            int off = 0;
+            for (int sw = 0; sw < wave; ++sw) off += (int)s_wcnt[sw][w];
+            s_list[w][off + rank[w]] = (uint16_t)tid;
+        }
+    }
+    int total = 0;
+#pragma unroll
+    for (int sw = 0; sw < kStrips; ++sw) total += (int)s_wcnt[sw][wave];
+    __syncthreads();
+    return total;
 }
 
 template <int CTRL>
@@ -79,47 +113,31 @@ constexpr int kDppXor1 = 0xB1;            // quad_perm [1,0,3,2]
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
-template <int PIX, int BATCH>
-__global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_fwd(BlendArgs a, float* __restrict__ final_T,
-                                                               uint32_t* __restrict__ n_contrib,
-                                                               float* __restrict__ out_color,
-                                                               int* __restrict__ out_point_id,
-                                                               float* __restrict__ out_weight_pixel,
-                                                               float* __restrict__ point_weight,
-                                                               float* __restrict__ kv_sum,
-                                                               uint32_t* __restrict__ kv_cnt)
+template <int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __restrict__ final_T,
+                                                         uint32_t* __restrict__ n_contrib,
+                                                         float* __restrict__ out_color,
+                                                         int* __restrict__ out_point_id,
+                                                         float* __restrict__ out_weight_pixel,
+                                                         float* __restrict__ point_weight,
+                                                         float* __restrict__ kv_sum,
+                                                         uint32_t* __restrict__ kv_cnt)
 {
     // Every product and sum of the per-pixel chain is written out (explicit
     // fmaf, no contraction by the compiler), so all work shapes round alike
     // and render bit-identical images.
 #pragma clang fp contract(off)
-    constexpr int NT = FGS_BLOCK / PIX;
-    constexpr int RSTEP = FGS_TILE / PIX;
+    static_assert(BATCH <= FGS_BLOCK, "one staging lane per batch slot");
     const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
     const int wave = tid / FGS_WAVE;
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
-    const int py0 = ty * FGS_TILE + (tid / FGS_TILE);
-    const float pfx = (float)px;
-    const float sx0 = (float)(tx * FGS_TILE), sx1 = sx0 + (FGS_TILE - 1);
-    const float sy0 = (float)(ty * FGS_TILE + wave * 4);
-
-    bool done[PIX];
-    float T[PIX], C0[PIX], C1[PIX], C2[PIX], wmax[PIX];
-    uint32_t last[PIX];
-    int max_id[PIX];
-    int ndone = 0;
-#pragma unroll
-    for (int k = 0; k < PIX; ++k) {
-        done[k] = !(px < a.W && py0 + k * RSTEP < a.H);
-        ndone += done[k];
-        T[k] = 1.f;
-        C0[k] = C1[k] = C2[k] = 0.f;
-        wmax[k] = 0.f;
-        last[k] = 0u;
-        max_id[k] = -1;
-    }
+    const int py = ty * FGS_TILE + (tid / FGS_TILE);
+    const bool inside = px < a.W && py < a.H;
+    const float pfx = (float)px, pfy = (float)py;
+    const float tx0 = (float)(tx * FGS_TILE), ty0 = (float)(ty * FGS_TILE);
+    bool done = !inside;
 
     const uint2 range = a.ranges[tile];
     const int n = (int)(range.y - range.x);
@@ -128,16 +146,25 @@ __global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_fwd(BlendArgs a, floa
     __shared__ float4 s_g0[BATCH];   // x, y, a', b'  (power in log2 units)
     __shared__ float2 s_g1[BATCH];   // c', opacity
     __shared__ float4 s_col[BATCH];  // r, g, b, id bits
-    __shared__ float4 s_bb[BATCH];   // alpha >= 1/255 box
     __shared__ uint32_t s_w[BATCH];
     __shared__ float s_kv[BATCH];
     __shared__ uint32_t s_kc[BATCH];
+    __shared__ uint16_t s_list[kStrips][BATCH];
+    __shared__ uint32_t s_wcnt[kStrips][kStrips];
+
+    float T = 1.0f;
+    float C0 = 0.f, C1 = 0.f, C2 = 0.f;
+    uint32_t last = 0;
+    float wmax = 0.f;
+    int max_id = -1;
 
     for (int r = 0; r < rounds; ++r) {
-        const int num_done = __syncthreads_count(ndone == PIX);
-        if (num_done == NT) break;
-        for (int q = tid; q < BATCH; q += NT) {
-            const int k = r * BATCH + q;
+        const int num_done = __syncthreads_count(done);
+        if (num_done == FGS_BLOCK) break;
+        const int q = tid;
+        const int k = r * BATCH + q;
+        uint32_t bits = 0u;
+        if (q < BATCH) {
             if (k < n) {
                 const uint32_t g = (uint32_t)a.pairs[range.x + k];
                 const float2 m = a.means2D[g];
@@ -146,80 +173,65 @@ __global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_fwd(BlendArgs a, floa
                 s_g1[q] = make_float2(-0.5f * kLog2e * co.z, co.w);
                 s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
                                        __uint_as_float(g));
-                s_bb[q] = alpha_box(m, co);
+                bits = strip_bits(alpha_box(m, co), tx0, ty0);
             }
             s_w[q] = 0u;
             s_kv[q] = 0.f;
             s_kc[q] = 0u;
         }
-        __syncthreads();
-        const int cnt = min(BATCH, n - r * BATCH);
-        for (int j = 0; ndone < PIX && j < cnt; ++j) {
-            if (bands_miss<PIX>(s_bb[j], sx0, sx1, sy0, (float)RSTEP)) continue;
+        const int cnt = build_lists<BATCH>(bits, s_list, s_wcnt);
+        for (int i = 0; !done && i < cnt; ++i) {
+            const int j = s_list[wave][i];
             const float4 g0 = s_g0[j];
             const float2 g1 = s_g1[j];
-            const float dx = g0.x - pfx;
-            const uint32_t position = (uint32_t)(r * BATCH + j + 1);
-#pragma unroll
-            for (int k = 0; k < PIX; ++k) {
-                if (done[k]) continue;
-                const float dy = g0.y - (float)(py0 + k * RSTEP);
-                const float p2 = fmaf(g0.z * dx, dx, fmaf(g1.x * dy, dy, (g0.w * dx) * dy));
-                if (p2 > 0.0f) continue;
-                const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
-                if (alpha < 1.0f / 255.0f) continue;
-                const float test_T = T[k] * (1.0f - alpha);
-                if (test_T < 0.0001f) {
-                    done[k] = true;
-                    ++ndone;
-                    continue;
-                }
-                const float w = alpha * T[k];
+            const float dx = g0.x - pfx, dy = g0.y - pfy;
+            const float p2 = fmaf(g0.z * dx, dx, fmaf(g1.x * dy, dy, (g0.w * dx) * dy));
+            const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+            const float test_T = T * (1.0f - alpha);
+            const bool pass = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+            const bool stop = pass && test_T < 0.0001f;
+            done = done || stop;
+            if (pass && !stop) {
+                const float w = alpha * T;
                 const float4 col = s_col[j];
-                C0[k] = fmaf(col.x, w, C0[k]);
-                C1[k] = fmaf(col.y, w, C1[k]);
-                C2[k] = fmaf(col.z, w, C2[k]);
-                if (w > wmax[k]) {
-                    wmax[k] = w;
-                    max_id[k] = (int)__float_as_uint(col.w);
+                C0 = fmaf(col.x, w, C0);
+                C1 = fmaf(col.y, w, C1);
+                C2 = fmaf(col.z, w, C2);
+                if (w > wmax) {
+                    wmax = w;
+                    max_id = (int)__float_as_uint(col.w);
                 }
                 atomicMax(&s_w[j], __float_as_uint(w));
                 if (kv_sum) {
                     atomicAdd(&s_kv[j], w);
                     atomicAdd(&s_kc[j], 1u);
                 }
-                T[k] = test_T;
-                last[k] = position;
+                T = test_T;
+                last = (uint32_t)(r * BATCH + j + 1);
             }
         }
         __syncthreads();
-        for (int q = tid; q < BATCH; q += NT) {
-            const int k = r * BATCH + q;
-            if (k < n) {
-                const uint32_t g = __float_as_uint(s_col[q].w);
-                if (s_w[q] != 0u) atomicMax((uint32_t*)&point_weight[g], s_w[q]);
-                if (kv_sum) {
-                    // each kv pair belongs to exactly one tile: plain stores
-                    kv_sum[range.x + k] = s_kv[q];
-                    kv_cnt[range.x + k] = s_kc[q];
-                }
+        if (q < BATCH && k < n) {
+            const uint32_t g = __float_as_uint(s_col[q].w);
+            if (s_w[q] != 0u) atomicMax((uint32_t*)&point_weight[g], s_w[q]);
+            if (kv_sum) {
+                // each kv pair belongs to exactly one tile: plain stores
+                kv_sum[range.x + k] = s_kv[q];
+                kv_cnt[range.x + k] = s_kc[q];
             }
         }
     }
 
-    const int HW = a.W * a.H;
-#pragma unroll
-    for (int k = 0; k < PIX; ++k) {
-        const int py = py0 + k * RSTEP;
-        if (!(px < a.W && py < a.H)) continue;
+    if (inside) {
         const int pix = a.W * py + px;
-        final_T[pix] = T[k];
-        n_contrib[pix] = last[k];
-        out_color[pix] = fmaf(T[k], a.bg[0], C0[k]);
-        out_color[HW + pix] = fmaf(T[k], a.bg[1], C1[k]);
-        out_color[2 * HW + pix] = fmaf(T[k], a.bg[2], C2[k]);
-        out_point_id[pix] = max_id[k];
-        out_weight_pixel[pix] = wmax[k];
+        const int HW = a.W * a.H;
+        final_T[pix] = T;
+        n_contrib[pix] = last;
+        out_color[pix] = fmaf(T, a.bg[0], C0);
+        out_color[HW + pix] = fmaf(T, a.bg[1], C1);
+        out_color[2 * HW + pix] = fmaf(T, a.bg[2], C2);
+        out_point_id[pix] = max_id;
+        out_weight_pixel[pix] = wmax;
     }
 }
 
@@ -229,19 +241,12 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-#define FGS_FWD(PIX, BATCH)                                                                          \
-    hipLaunchKernelGGL((k_blend_fwd<PIX, BATCH>), dim3(T), dim3(FGS_BLOCK / PIX), 0, s, a, final_T, \
-                       n_contrib, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum,   \
-                       kv_cnt)
-    const int pix = a.pix_per_thread == 2 ? 2 : 1;
-    const int batch = a.batch == 128 ? 128 : (a.batch == 64 ? 64 : 256);
-    if (pix == 1 && batch == 256) FGS_FWD(1, 256);
-    else if (pix == 1 && batch == 128) FGS_FWD(1, 128);
-    else if (pix == 1) FGS_FWD(1, 64);
-    else if (batch == 256) FGS_FWD(2, 256);
-    else if (batch == 128) FGS_FWD(2, 128);
-    else FGS_FWD(2, 64);
-#undef FGS_FWD
+if (a.batch == 128)
+        hipLaunchKernelGGL(k_blend_fwd<128>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
+                           out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
+    else
+        hipLaunchKernelGGL(k_blend_fwd<256>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
+                           out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
     return hipGetLastError();
 }
 
@@ -308,67 +313,61 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 
 constexpr int kGradTerms = 9;   // dcolor x3, dmean2D x2, dconic x3, dopacity
 
-template <int PIX, int BATCH>
-__global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_bwd(BlendArgs a, const float* __restrict__ final_Ts,
-                                                               const uint32_t* __restrict__ n_contrib,
-                                                               const float* __restrict__ dL_dpix,
-                                                               float* __restrict__ dL_dmean2D,
-                                                               float* __restrict__ dL_dconic,
-                                                               float* __restrict__ dL_dopacity,
-                                                               float* __restrict__ dL_dcolors, int exact)
+template <int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* __restrict__ final_Ts,
+                                                         const uint32_t* __restrict__ n_contrib,
+                                                         const float* __restrict__ dL_dpix,
+                                                         float* __restrict__ dL_dmean2D,
+                                                         float* __restrict__ dL_dconic,
+                                                         float* __restrict__ dL_dopacity,
+                                                         float* __restrict__ dL_dcolors, int exact)
 {
-    constexpr int NT = FGS_BLOCK / PIX;
-    constexpr int NW = NT / FGS_WAVE;
-    constexpr int RSTEP = FGS_TILE / PIX;
+    static_assert(BATCH <= FGS_BLOCK, "one staging lane per batch slot");
     const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
     const int wave = tid / FGS_WAVE;
     const int lane = lane_id();
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
-    const int py0 = ty * FGS_TILE + (tid / FGS_TILE);
-    const float pfx = (float)px;
-    const float sx0 = (float)(tx * FGS_TILE), sx1 = sx0 + (FGS_TILE - 1);
-    const float sy0 = (float)(ty * FGS_TILE + wave * 4);
+    const int py = ty * FGS_TILE + (tid / FGS_TILE);
+    const bool inside = px < a.W && py < a.H;
+    const float pfx = (float)px, pfy = (float)py;
+    const float tx0 = (float)(tx * FGS_TILE), ty0 = (float)(ty * FGS_TILE);
     const int HW = a.W * a.H;
+    const int pix = a.W * py + px;
 
     const uint2 range = a.ranges[tile];
 
     __shared__ float4 s_g0[BATCH];   // x, y, A, B
     __shared__ float2 s_g1[BATCH];   // C, opacity
     __shared__ float4 s_col[BATCH];  // r, g, b, id bits
-    __shared__ float4 s_bb[BATCH];
     __shared__ float s_g[kGradTerms][BATCH];
-    __shared__ uint32_t s_wmax[NW];
+    __shared__ uint32_t s_wmax[kStrips];
+    __shared__ uint16_t s_list[kStrips][BATCH];
+    __shared__ uint32_t s_wcnt[kStrips][kStrips];
 
-    float T_final[PIX], T[PIX], dp0[PIX], dp1[PIX], dp2[PIX], bg_dot[PIX];
-    float acc0[PIX], acc1[PIX], acc2[PIX], last_alpha[PIX], lc0[PIX], lc1[PIX], lc2[PIX];
-    uint32_t last[PIX];
-    uint32_t lmax = 0;
-#pragma unroll
-    for (int k = 0; k < PIX; ++k) {
-        const int py = py0 + k * RSTEP;
-        const bool inside = px < a.W && py < a.H;
-        const int pix = a.W * py + px;
-        T_final[k] = inside ? final_Ts[pix] : 0.f;
-        T[k] = T_final[k];
-        last[k] = inside ? n_contrib[pix] : 0u;
-        lmax = last[k] > lmax ? last[k] : lmax;
-        dp0[k] = inside ? dL_dpix[pix] : 0.f;
-        dp1[k] = inside ? dL_dpix[HW + pix] : 0.f;
-        dp2[k] = inside ? dL_dpix[2 * HW + pix] : 0.f;
-        bg_dot[k] = a.bg[0] * dp0[k] + a.bg[1] * dp1[k] + a.bg[2] * dp2[k];
-        acc0[k] = acc1[k] = acc2[k] = 0.f;
-        last_alpha[k] = 0.f;
-        lc0[k] = lc1[k] = lc2[k] = 0.f;
-    }
-    // positions >= the largest n_contrib of the tile contribute to no pixel
-    const uint32_t wmax = wave_max_u32(lmax);
+    const float T_final = inside ? final_Ts[pix] : 0.f;
+    float T = T_final;
+    const uint32_t last = inside ? n_contrib[pix] : 0u;
+    const float dp0 = inside ? dL_dpix[pix] : 0.f;
+    const float dp1 = inside ? dL_dpix[HW + pix] : 0.f;
+    const float dp2 = inside ? dL_dpix[2 * HW + pix] : 0.f;
+    const float bg_dot = a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2;
+    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
+    float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
+
+    // positions >= the largest n_contrib of a strip contribute to none of its
+    // pixels; past the largest of the tile to no pixel at all
+    const uint32_t wmax = wave_max_u32(last);
     if (lane == 0) s_wmax[wave] = wmax;
     __syncthreads();
+    uint32_t smax[kStrips];
     uint32_t bmax = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
+    for (int w = 0; w < kStrips; ++w) {
+        smax[w] = s_wmax[w];
+        bmax = smax[w] > bmax ? smax[w] : bmax;
+    }
     const int n = (int)min(bmax, range.y - range.x);
     const int rounds = (n + BATCH - 1) / BATCH;
     const float ddelx_dx = 0.5f * (float)a.W;
@@ -376,8 +375,10 @@ __global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_bwd(BlendArgs a, cons
 
     for (int r = 0; r < rounds; ++r) {
         __syncthreads();
-        for (int q = tid; q < BATCH; q += NT) {
-            const int k = r * BATCH + q;
+        const int q = tid;
+        const int k = r * BATCH + q;
+        uint32_t bits = 0u;
+        if (q < BATCH) {
             if (k < n) {
                 // back to front: position n-1-k of the tile's list
                 const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
@@ -387,63 +388,61 @@ __global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_bwd(BlendArgs a, cons
                 s_g1[q] = make_float2(co.z, co.w);
                 s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
                                        __uint_as_float(g));
-                s_bb[q] = alpha_box(m, co);
+                bits = strip_bits(alpha_box(m, co), tx0, ty0);
+                const uint32_t position = (uint32_t)(n - 1 - k);
+#pragma unroll
+                for (int w = 0; w < kStrips; ++w)
+                    if (position >= smax[w]) bits &= ~(1u << w);
             }
 #pragma unroll
             for (int t = 0; t < kGradTerms; ++t) s_g[t][q] = 0.f;
         }
-        __syncthreads();
-        const int cnt = min(BATCH, n - r * BATCH);
-        for (int j = 0; j < cnt; ++j) {
+        const int cnt = build_lists<BATCH>(bits, s_list, s_wcnt);
+        for (int i = 0; i < cnt; ++i) {
+            const int j = s_list[wave][i];
             const uint32_t position = (uint32_t)(n - 1 - (r * BATCH + j));
-            if (position >= wmax) continue;                                        // wave-uniform
-            if (bands_miss<PIX>(s_bb[j], sx0, sx1, sy0, (float)RSTEP)) continue;  // wave-uniform
             float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
             float vop = 0.f;
             bool act = false;
             const float4 g0 = s_g0[j];
             const float2 g1 = s_g1[j];
-            const float dx = g0.x - pfx;
-#pragma unroll
-            for (int k = 0; k < PIX; ++k) {
-                if (position >= last[k]) continue;
-                const float dy = g0.y - (float)(py0 + k * RSTEP);
+            const float dx = g0.x - pfx, dy = g0.y - pfy;
+            if (position < last) {
                 const float power = -0.5f * (g0.z * dx * dx + g1.x * dy * dy) - g0.w * dx * dy;
-                if (power > 0.0f) continue;
                 const float G = __builtin_amdgcn_exp2f(power * kLog2e);
                 const float alpha_raw = g1.y * G;
                 const float alpha = fminf(0.99f, alpha_raw);
-                if (alpha < 1.0f / 255.0f) continue;
-                act = true;
-                const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-                T[k] = T[k] * inv;
-                const float dch = alpha * T[k];
-                const float4 col = s_col[j];
-                acc0[k] = last_alpha[k] * lc0[k] + (1.f - last_alpha[k]) * acc0[k];
-                acc1[k] = last_alpha[k] * lc1[k] + (1.f - last_alpha[k]) * acc1[k];
-                acc2[k] = last_alpha[k] * lc2[k] + (1.f - last_alpha[k]) * acc2[k];
-                lc0[k] = col.x;
-                lc1[k] = col.y;
-                lc2[k] = col.z;
-                float dL_dalpha = (col.x - acc0[k]) * dp0[k] + (col.y - acc1[k]) * dp1[k] +
-                                  (col.z - acc2[k]) * dp2[k];
-                v[0] += dch * dp0[k];
-                v[1] += dch * dp1[k];
-                v[2] += dch * dp2[k];
-                dL_dalpha *= T[k];
-                last_alpha[k] = alpha;
-                dL_dalpha += (-T_final[k] * inv) * bg_dot[k];
-                const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
-                const float dL_dG = g1.y * dL_dalpha * slope;
-                const float gdx = G * dx, gdy = G * dy;
-                const float dG_ddelx = -gdx * g0.z - gdy * g0.w;
-                const float dG_ddely = -gdy * g1.x - gdx * g0.w;
-                v[3] += dL_dG * dG_ddelx * ddelx_dx;
-                v[4] += dL_dG * dG_ddely * ddely_dy;
-                v[5] += -0.5f * gdx * dx * dL_dG;
-                v[6] += -0.5f * gdx * dy * dL_dG;
-                v[7] += -0.5f * gdy * dy * dL_dG;
-                vop += G * dL_dalpha * slope;
+                if (!(power > 0.0f) && !(alpha < 1.0f / 255.0f)) {
+                    act = true;
+                    const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+                    T = T * inv;
+                    const float dch = alpha * T;
+                    const float4 col = s_col[j];
+                    acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
+                    acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
+                    acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
+                    lc0 = col.x;
+                    lc1 = col.y;
+                    lc2 = col.z;
+                    float dL_dalpha = (col.x - acc0) * dp0 + (col.y - acc1) * dp1 + (col.z - acc2) * dp2;
+                    v[0] = dch * dp0;
+                    v[1] = dch * dp1;
+                    v[2] = dch * dp2;
+                    dL_dalpha *= T;
+                    last_alpha = alpha;
+                    dL_dalpha += (-T_final * inv) * bg_dot;
+                    const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
+                    const float dL_dG = g1.y * dL_dalpha * slope;
+                    const float gdx = G * dx, gdy = G * dy;
+                    const float dG_ddelx = -gdx * g0.z - gdy * g0.w;
+                    const float dG_ddely = -gdy * g1.x - gdx * g0.w;
+                    v[3] = dL_dG * dG_ddelx * ddelx_dx;
+                    v[4] = dL_dG * dG_ddely * ddely_dy;
+                    v[5] = -0.5f * gdx * dx * dL_dG;
+                    v[6] = -0.5f * gdx * dy * dL_dG;
+                    v[7] = -0.5f * gdy * dy * dL_dG;
+                    vop = G * dL_dalpha * slope;
+                }
             }
             if (__any(act)) {
                 // the waves meet on the same slot: LDS float atomics
@@ -454,9 +453,7 @@ __global__ __launch_bounds__(FGS_BLOCK / PIX) void k_blend_bwd(BlendArgs a, cons
             }
         }
         __syncthreads();
-        for (int q = tid; q < BATCH; q += NT) {
-            const int k = r * BATCH + q;
-            if (k >= n) continue;
+        if (q < BATCH && k < n) {
             const uint32_t g = __float_as_uint(s_col[q].w);
             float t[kGradTerms];
 #pragma unroll
@@ -480,17 +477,12 @@ hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-#define FGS_BWD(PIX, BATCH)                                                                          \
-    hipLaunchKernelGGL((k_blend_bwd<PIX, BATCH>), dim3(T), dim3(FGS_BLOCK / PIX), 0, s, a, final_T, \
-                       n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact)
-    const int pix = a.bwd_pix_per_thread == 2 ? 2 : 1;
-    const int batch = a.bwd_batch == 128 ? 128 : (a.bwd_batch == 64 ? 64 : 256);
-    if (pix == 1 && batch == 256) FGS_BWD(1, 256);
-    else if (pix == 1 && batch == 128) FGS_BWD(1, 128);
-    else if (pix == 1) FGS_BWD(1, 64);
-    else if (batch == 256) FGS_BWD(2, 256);
-    else if (batch == 128) FGS_BWD(2, 128);
-    else FGS_BWD(2, 64);
+#define FGS_BWD(BATCH)                                                                            \
+    hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,  \
+                       dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact)
+    if (a.bwd_batch == 256) FGS_BWD(256);
+    else if (a.bwd_batch == 64) FGS_BWD(64);
+    else FGS_BWD(128);
 #undef FGS_BWD
     return hipGetLastError();
 }

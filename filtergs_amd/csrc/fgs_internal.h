@@ -65,9 +65,9 @@ struct BlendArgs {
     const float* colors;       // [P,3]
     const float4* conic_opacity;
     const float* bg;           // device [3]
-    // work shape: pixels per lane (1 or 2) and LDS batch (64, 128 or 256)
-    int pix_per_thread, batch;
-    int bwd_pix_per_thread, bwd_batch;
+    // work shape: Gaussians staged in LDS per batch (forward 128 or 256,
+    // backward 64, 128 or 256)
+    int batch, bwd_batch;
 };
 
 // launchers (return hipError_t of the launch)

@@ -197,14 +197,15 @@ int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const 
  * accumulated milliseconds / call counts and returns the number of stages. */
 int fgs_profile_enable(int32_t on);
 int fgs_profile_read(double* ms, uint64_t* calls, int32_t n, int32_t reset);
-/* Blend work shape, process-wide: pixels per lane (1 or 2) and Gaussians
- * staged in LDS per batch (64, 128 or 256), for the forward and backward
- * blend.  Images and integer outputs do not depend on it (the 16x16
- * binning tile is fixed); gradients change only in float summation order.  No counterpart in the reference, whose renderCUDA
- * (forward.cu:406, backward.cu:399) fixes one pixel per thread and
- * BLOCK_SIZE batches; the autotuner picks it. */
-int fgs_set_work_shape(int32_t fwd_pix, int32_t fwd_batch, int32_t bwd_pix, int32_t bwd_batch);
-int fgs_get_work_shape(int32_t* out4);
+/* Blend work shape, process-wide: Gaussians staged in LDS per batch by the
+ * forward blend (128 or 256; default 256) and the backward blend (64, 128
+ * or 256; default 128).  Images and integer outputs do not depend on it (the
+ * 16x16 binning tile is fixed); gradients change only in float summation
+ * order.  No counterpart in the reference, whose renderCUDA
+ * (forward.cu:406, backward.cu:399) fixes BLOCK_SIZE batches; the
+ * autotuner picks it.  fgs_get_work_shape fills {fwd_batch, bwd_batch}. */
+int fgs_set_work_shape(int32_t fwd_batch, int32_t bwd_batch);
+int fgs_get_work_shape(int32_t* out2);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */

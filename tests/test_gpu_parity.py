@@ -344,10 +344,11 @@ def test_c1_scale_keys_bitexact_and_image(orc):
 
 
 def test_work_shapes_agree():
-    """Every blend work shape (pixels per lane x LDS batch) renders the same
-    image, n_contrib, max-weight ids and point weights bit for bit, and the
-    same gradients up to float summation order.  The scene puts several
-    hundred Gaussians on most tiles so every shape runs several batches."""
+    """Every blend work shape (Gaussians per LDS batch, forward and backward)
+    renders the same image, n_contrib, max-weight ids and point weights bit
+    for bit, and the same gradients up to float summation order.  The scene
+    puts several hundred Gaussians on most tiles so every shape runs several
+    batches."""
     from filtergs_amd import _lib
     g, cam = small_scene(n=40000, W=200, H=136, seed=161, dist=5.0)
     rng = np.random.default_rng(8)
@@ -355,29 +356,28 @@ def test_work_shapes_agree():
     bg = (0.1, 0.2, 0.3)
     base = None
     try:
-        for pix in (1, 2):
-            for batch in (256, 128, 64):
-                _lib.set_work_shape(pix, batch, pix, batch)
-                assert _lib.get_work_shape() == (pix, batch, pix, batch)
-                out = gpu_forward(g, cam, bg=bg)
-                gr = _grads_gpu(out, g, cam, dL, 0, 0, bg=bg)
-                iv = ext.image_views(out[8], cam.width, cam.height)
-                fw = [_np(out[k]) for k in (1, 2, 3, 4)] + [_np(iv[k]) for k in
-                                                            ("final_T", "n_contrib", "ranges")]
-                res = (fw, [_np(t) for t in gr])
-                if base is None:
-                    base = res
-                    assert int(out[0]) > 200 * (cam.width // 16) * (cam.height // 16)
-                    continue
-                for k, (a, b) in enumerate(zip(res[0], base[0])):
-                    assert np.array_equal(a, b), (pix, batch, "forward output", k)
-                for k, (a, b) in enumerate(zip(res[1], base[1])):
-                    # blend terms (mean2D, colors, opacity) differ by atomic
-                    # summation order only; the chain through cov3D / rotation
-                    # has cancellations that amplify it
-                    d = np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-30)
-                    assert d <= (1e-5 if k < 3 else 1e-4), (pix, batch, "gradient", k, d)
+        for fb, bb in ((256, 128), (128, 64), (256, 256), (128, 128)):
+            _lib.set_work_shape(fb, bb)
+            assert _lib.get_work_shape() == (fb, bb)
+            out = gpu_forward(g, cam, bg=bg)
+            gr = _grads_gpu(out, g, cam, dL, 0, 0, bg=bg)
+            iv = ext.image_views(out[8], cam.width, cam.height)
+            fw = [_np(out[k]) for k in (1, 2, 3, 4)] + [_np(iv[k]) for k in
+                                                        ("final_T", "n_contrib", "ranges")]
+            res = (fw, [_np(t) for t in gr])
+            if base is None:
+                base = res
+                assert int(out[0]) > 200 * (cam.width // 16) * (cam.height // 16)
+                continue
+            for k, (a, b) in enumerate(zip(res[0], base[0])):
+                assert np.array_equal(a, b), (fb, bb, "forward output", k)
+            for k, (a, b) in enumerate(zip(res[1], base[1])):
+                # blend terms (mean2D, colors, opacity) differ by atomic
+                # summation order only; the chain through cov3D / rotation
+                # has cancellations that amplify it
+                d = np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-30)
+                assert d <= (1e-5 if k < 3 else 1e-4), (fb, bb, "gradient", k, d)
     finally:
-        _lib.set_work_shape(1, 256, 1, 256)
+        _lib.set_work_shape(256, 128)
     with pytest.raises(Exception):
-        _lib.set_work_shape(3, 256, 1, 256)
+        _lib.set_work_shape(512, 128)
