@@ -14,6 +14,7 @@ import torch
 import torch.nn as nn
 
 from . import ext
+from . import ext as _C  # the reference module exposes its extension as _C
 from .rasterizer import GRAD_MODES, cpu_deep_copy_tuple
 
 

@@ -1,0 +1,84 @@
+"""The import-name shims resolve the names the reference's callers import, with
+the reference's signatures (no GPU needed)."""
+import importlib
+import inspect
+import sys
+
+import pytest
+
+
+@pytest.fixture()
+def shims():
+    from filtergs_amd import compat
+    compat.install()
+    yield
+    compat.uninstall()
+
+
+def _params(fn):
+    return [p.name for p in inspect.signature(fn).parameters.values()]
+
+
+def test_wodilate_names_and_signatures(shims):
+    from diff_gaussian_rasterization_wodilate import (GaussianRasterizationSettings,
+                                                      GaussianRasterizer)
+    from filtergs_amd import rasterizer
+    assert GaussianRasterizer is rasterizer.GaussianRasterizer
+    # diff_gaussian_rasterization_wodilate/__init__.py:193-210
+    fields = list(GaussianRasterizationSettings._fields)
+    assert fields[:17] == ["image_height", "image_width", "tanfovx", "tanfovy", "bg",
+                           "scale_modifier", "viewmatrix", "projmatrix", "sh_degree", "campos",
+                           "prefiltered", "debug", "ogfs_epsilon_scale", "ogfs_epsilon_max",
+                           "ogfs_energy_floor", "ogfs_radius_ratio_min", "ogfs_enable_stats"]
+    d = GaussianRasterizationSettings._field_defaults
+    assert (d["ogfs_epsilon_scale"], d["ogfs_epsilon_max"], d["ogfs_energy_floor"],
+            d["ogfs_radius_ratio_min"], d["ogfs_enable_stats"]) == (0.2, 0.2, 0.05, 0.7, True)
+    # :245-262
+    assert _params(GaussianRasterizer.forward)[:15] == [
+        "self", "means3D", "means2D", "opacities", "shs", "colors_precomp", "scales", "rotations",
+        "cov3D_precomp", "use_filter", "ogfs_epsilon_scale", "ogfs_epsilon_max",
+        "ogfs_energy_floor", "ogfs_radius_ratio_min", "ogfs_enable_stats"]
+    assert _params(GaussianRasterizer.compute_radius) == ["self", "positions", "scales",
+                                                          "rotations"]
+    assert _params(GaussianRasterizer.markVisible) == ["self", "positions"]
+    from diff_gaussian_rasterization_wodilate import _C
+    assert _C.rasterize_gaussians is importlib.import_module("filtergs_amd.ext").rasterize_gaussians
+
+
+def test_dilate_names(shims):
+    from diff_gaussian_rasterization import GaussianRasterizationSettings, GaussianRasterizer
+    from filtergs_amd import dilate
+    assert GaussianRasterizer is dilate.GaussianRasterizer
+    assert "ogfs_epsilon_scale" not in GaussianRasterizationSettings._fields
+    assert _params(GaussianRasterizer.forward)[:9] == [
+        "self", "means3D", "means2D", "opacities", "shs", "colors_precomp", "scales", "rotations",
+        "cov3D_precomp"]
+
+
+def test_filtergs_cuda_names(shims):
+    from FilterGS.cuda.compute_radius import compute_radius_module, compute_visibility_radius
+    from FilterGS.cuda.gather_attributes import gather_attributes
+    from FilterGS.cuda.visible_flag import (visible_flag_cuda, visible_flag_cuda_batch,
+                                            visible_flag_cuda_batch_direct,
+                                            visible_flag_cuda_direct)
+    # FilterGS/cuda/compute_radius.py:23-26
+    assert _params(compute_visibility_radius) == [
+        "means3d", "scales", "rotations", "projmatrix", "viewmatrix", "tree_node_index",
+        "ancestor_path", "focal_x", "focal_y", "tan_fovx", "tan_fovy", "radius_max", "padding",
+        "radius_filter_threshold"]
+    sig = inspect.signature(compute_visibility_radius).parameters
+    assert sig["padding"].default == 0.05 and sig["radius_filter_threshold"].default == 100.0
+    assert callable(compute_radius_module.compute_radius)
+    assert _params(gather_attributes) == ["index", "xyz", "scaling", "rotation", "opacity",
+                                          "colors", "shs"]
+    for fn in (visible_flag_cuda, visible_flag_cuda_direct, visible_flag_cuda_batch,
+               visible_flag_cuda_batch_direct):
+        assert _params(fn)[2:] == ["padding", "use_optimized"]
+
+
+def test_uninstall_removes_aliases(shims):
+    from filtergs_amd import compat
+    compat.uninstall()
+    assert "diff_gaussian_rasterization_wodilate" not in sys.modules
+    assert "FilterGS.cuda.compute_radius" not in sys.modules
+    compat.install()
