@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-band", type=int, default=256, help="rows of the CPU baseline sample")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after timing, run one device-to-device copy of this many MiB (a "
+                         "reference kernel of known traffic for hardware byte counters)")
     return ap.parse_args()
 
 
@@ -128,6 +131,14 @@ def main():
     elapsed = time.perf_counter() - t0
     prof = _lib.profile_read(reset=True)
     _lib.profile_enable(False)
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ms = elapsed * 1000.0 / args.steps
     npix = W * H

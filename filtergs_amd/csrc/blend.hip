@@ -100,15 +100,53 @@ __device__ __forceinline__ int build_lists(uint32_t bits, uint16_t (*s_list)[BAT
     return total;
 }
 
+// DPP read of another lane's value inside a 16-lane row; written as
+// update_dpp with a zero old value so the compiler folds it into the
+// consuming add (v_add_f32_dpp).
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v)
 {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// x[l] + x[l ^ 32] and x[l] + x[l ^ 16] in every lane with the gfx950 lane
+// swaps (v_permlane32_swap / v_permlane16_swap): VALU only, no LDS round trip.
+// Both halves add the pair in the same order, so every lane gets the same bits.
+__device__ __forceinline__ float sum_xor32(float x)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false,
+                                                    false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float sum_xor16(float x)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false,
+                                                    false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
 constexpr int kDppRowMirror = 0x140;      // lane r <-> 15 - r inside each 16-lane row
 constexpr int kDppRowHalfMirror = 0x141;  // lane r <-> 7 - r inside each 8-lane half row
 constexpr int kDppXor2 = 0x4E;            // quad_perm [2,3,0,1]
 constexpr int kDppXor1 = 0xB1;            // quad_perm [1,0,3,2]
+
+// Wave total of one value (every lane receives it).
+__device__ __forceinline__ float wave_total(float x)
+{
+    x += dpp<0x128>(x);   // row_ror 8
+    x += dpp<0x124>(x);   // row_ror 4
+    x += dpp<0x122>(x);   // row_ror 2
+    x += dpp<0x121>(x);   // row_ror 1
+    return sum_xor32(sum_xor16(x));
+}
+
+// Exponent of the Gaussian in log2 units at offset (dx, dy) from its mean,
+// from the staged terms (a', b') = g0.zw and c' = g1.x.  Shared by the
+// forward and the backward so that both take the same alpha decisions.
+__device__ __forceinline__ float exponent2(float4 g0, float2 g1, float dx, float dy)
+{
+#pragma clang fp contract(off)
+    return fmaf(g0.z * dx, dx, fmaf(g1.x * dy, dy, (g0.w * dx) * dy));
+}
 
 // ---------------------------------------------------------------------------
 // Forward
@@ -185,7 +223,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
             const float4 g0 = s_g0[j];
             const float2 g1 = s_g1[j];
             const float dx = g0.x - pfx, dy = g0.y - pfy;
-            const float p2 = fmaf(g0.z * dx, dx, fmaf(g1.x * dy, dy, (g0.w * dx) * dy));
+            const float p2 = exponent2(g0, g1, dx, dy);
             const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
             const float test_T = T * (1.0f - alpha);
             const bool pass = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
@@ -241,7 +279,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-if (a.batch == 128)
+    if (a.batch == 128)
         hipLaunchKernelGGL(k_blend_fwd<128>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
                            out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
     else
@@ -257,7 +295,7 @@ if (a.batch == 128)
 // Reduce-scatter of 8 values across the wave.  Inside every 16-lane row the
 // halving steps pair lanes with DPP permutations (mirror: bit 3, half mirror:
 // bit 2, quad_perm: bits 1 and 0), leaving in lane r the row sum of value
-// 4*bit3 + 2*bit2 + bit1; two shuffles then add the four rows.  On return
+// 4*bit3 + 2*bit2 + bit1; two lane swaps then add the four rows.  On return
 // lane l < 16 holds the wave total of value l >> 1.
 __device__ __forceinline__ float wave_reduce_scatter8(const float v[8])
 {
@@ -284,21 +322,7 @@ __device__ __forceinline__ float wave_reduce_scatter8(const float v[8])
         z = keep + dpp<kDppXor2>(send);
     }
     z += dpp<kDppXor1>(z);
-    z += __shfl_xor(z, 16, FGS_WAVE);
-    z += __shfl_xor(z, 32, FGS_WAVE);
-    return z;
-}
-
-// Wave total of one value (every lane receives it).
-__device__ __forceinline__ float wave_total(float x)
-{
-    x += dpp<0x128>(x);   // row_ror 8
-    x += dpp<0x124>(x);   // row_ror 4
-    x += dpp<0x122>(x);   // row_ror 2
-    x += dpp<0x121>(x);   // row_ror 1
-    x += __shfl_xor(x, 16, FGS_WAVE);
-    x += __shfl_xor(x, 32, FGS_WAVE);
-    return x;
+    return sum_xor32(sum_xor16(z));
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -338,8 +362,9 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
 
     const uint2 range = a.ranges[tile];
 
-    __shared__ float4 s_g0[BATCH];   // x, y, A, B
-    __shared__ float2 s_g1[BATCH];   // C, opacity
+    __shared__ float4 s_g0[BATCH];   // x, y, a', b'  (power in log2 units, as the forward)
+    __shared__ float2 s_g1[BATCH];   // c', opacity
+    __shared__ float4 s_con[BATCH];  // conic A, B, C
     __shared__ float4 s_col[BATCH];  // r, g, b, id bits
     __shared__ float s_g[kGradTerms][BATCH];
     __shared__ uint32_t s_wmax[kStrips];
@@ -353,6 +378,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
     const float dp1 = inside ? dL_dpix[HW + pix] : 0.f;
     const float dp2 = inside ? dL_dpix[2 * HW + pix] : 0.f;
     const float bg_dot = a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2;
+    const float tfb = -T_final * bg_dot;   // dL/dalpha gets -T_final / (1 - alpha) * (bg . dL/dpix)
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
     float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
 
@@ -384,8 +410,9 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
                 const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
                 const float2 m = a.means2D[g];
                 const float4 co = a.conic_opacity[g];
-                s_g0[q] = make_float4(m.x, m.y, co.x, co.y);
-                s_g1[q] = make_float2(co.z, co.w);
+                s_g0[q] = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                s_g1[q] = make_float2(-0.5f * kLog2e * co.z, co.w);
+                s_con[q] = co;
                 s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
                                        __uint_as_float(g));
                 bits = strip_bits(alpha_box(m, co), tx0, ty0);
@@ -401,56 +428,59 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
         for (int i = 0; i < cnt; ++i) {
             const int j = s_list[wave][i];
             const uint32_t position = (uint32_t)(n - 1 - (r * BATCH + j));
-            float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
-            float vop = 0.f;
-            bool act = false;
+            // alpha exactly as the forward computed it (same staged terms,
+            // same operations), so both passes take the same decisions
             const float4 g0 = s_g0[j];
             const float2 g1 = s_g1[j];
             const float dx = g0.x - pfx, dy = g0.y - pfy;
-            if (position < last) {
-                const float power = -0.5f * (g0.z * dx * dx + g1.x * dy * dy) - g0.w * dx * dy;
-                const float G = __builtin_amdgcn_exp2f(power * kLog2e);
-                const float alpha_raw = g1.y * G;
-                const float alpha = fminf(0.99f, alpha_raw);
-                if (!(power > 0.0f) && !(alpha < 1.0f / 255.0f)) {
-                    act = true;
-                    const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
-                    T = T * inv;
-                    const float dch = alpha * T;
-                    const float4 col = s_col[j];
-                    acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-                    acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-                    acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-                    lc0 = col.x;
-                    lc1 = col.y;
-                    lc2 = col.z;
-                    float dL_dalpha = (col.x - acc0) * dp0 + (col.y - acc1) * dp1 + (col.z - acc2) * dp2;
-                    v[0] = dch * dp0;
-                    v[1] = dch * dp1;
-                    v[2] = dch * dp2;
-                    dL_dalpha *= T;
-                    last_alpha = alpha;
-                    dL_dalpha += (-T_final * inv) * bg_dot;
-                    const float slope = (exact && alpha_raw > 0.99f) ? 0.f : 1.f;
-                    const float dL_dG = g1.y * dL_dalpha * slope;
-                    const float gdx = G * dx, gdy = G * dy;
-                    const float dG_ddelx = -gdx * g0.z - gdy * g0.w;
-                    const float dG_ddely = -gdy * g1.x - gdx * g0.w;
-                    v[3] = dL_dG * dG_ddelx * ddelx_dx;
-                    v[4] = dL_dG * dG_ddely * ddely_dy;
-                    v[5] = -0.5f * gdx * dx * dL_dG;
-                    v[6] = -0.5f * gdx * dy * dL_dG;
-                    v[7] = -0.5f * gdy * dy * dL_dG;
-                    vop = G * dL_dalpha * slope;
-                }
+            const float p2 = exponent2(g0, g1, dx, dy);
+            const float G = __builtin_amdgcn_exp2f(fminf(p2, 0.0f));
+            const float alpha_raw = g1.y * G;
+            const float alpha = fminf(0.99f, alpha_raw);
+            const bool act = position < last && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+            if (!__any(act)) continue;   // wave-uniform
+            // every lane computes; inactive lanes contribute zeros and keep
+            // their state (all terms stay finite: G <= 1, alpha <= 0.99)
+            const float4 con = s_con[j];
+            const float4 col = s_col[j];
+            const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+            const float Tn = T * inv;
+            const float a0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
+            const float a1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
+            const float a2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
+            const float dLa = ((col.x - a0) * dp0 + (col.y - a1) * dp1 + (col.z - a2) * dp2) * Tn +
+                              inv * tfb;
+            const bool flows = act && !(exact && alpha_raw > 0.99f);
+            const float dch = act ? alpha * Tn : 0.f;
+            float v[8];
+            v[0] = dch * dp0;
+            v[1] = dch * dp1;
+            v[2] = dch * dp2;
+            // h = dL/dG * G; the constant factors (-W/2, -H/2, -1/2) are
+            // applied once per (tile, Gaussian) when the sums are published
+            const float vop = flows ? G * dLa : 0.f;
+            const float h = g1.y * vop;
+            const float hdx = h * dx, hdy = h * dy;
+            v[3] = hdx * con.x + hdy * con.y;
+            v[4] = hdy * con.z + hdx * con.y;
+            v[5] = hdx * dx;
+            v[6] = hdx * dy;
+            v[7] = hdy * dy;
+            if (act) {
+                T = Tn;
+                acc0 = a0;
+                acc1 = a1;
+                acc2 = a2;
+                lc0 = col.x;
+                lc1 = col.y;
+                lc2 = col.z;
+                last_alpha = alpha;
             }
-            if (__any(act)) {
-                // the waves meet on the same slot: LDS float atomics
-                const float z = wave_reduce_scatter8(v);
-                const float zop = wave_total(vop);
-                if (lane < 16 && (lane & 1) == 0) atomicAdd(&s_g[lane >> 1][j], z);
-                if (lane == 16) atomicAdd(&s_g[8][j], zop);
-            }
+            // the waves meet on the same slot: LDS float atomics
+            const float z = wave_reduce_scatter8(v);
+            const float zop = wave_total(vop);
+            if (lane < 16 && (lane & 1) == 0) atomicAdd(&s_g[lane >> 1][j], z);
+            if (lane == 16) atomicAdd(&s_g[8][j], zop);
         }
         __syncthreads();
         if (q < BATCH && k < n) {
@@ -461,11 +491,11 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
             if (t[0] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 0], t[0]);
             if (t[1] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 1], t[1]);
             if (t[2] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 2], t[2]);
-            if (t[3] != 0.f) atomic_add_f32(&dL_dmean2D[3 * g + 0], t[3]);
-            if (t[4] != 0.f) atomic_add_f32(&dL_dmean2D[3 * g + 1], t[4]);
-            if (t[5] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 0], t[5]);
-            if (t[6] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 1], t[6]);
-            if (t[7] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 2], t[7]);
+            if (t[3] != 0.f) atomic_add_f32(&dL_dmean2D[3 * g + 0], -ddelx_dx * t[3]);
+            if (t[4] != 0.f) atomic_add_f32(&dL_dmean2D[3 * g + 1], -ddely_dy * t[4]);
+            if (t[5] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 0], -0.5f * t[5]);
+            if (t[6] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 1], -0.5f * t[6]);
+            if (t[7] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 2], -0.5f * t[7]);
             if (t[8] != 0.f) atomic_add_f32(&dL_dopacity[g], t[8]);
         }
     }
