@@ -152,6 +152,7 @@ Geom carve_geom(void* buf, int P, size_t* total)
 struct Binning {
     uint64_t* pairsA;
     uint64_t* pairsB;
+    uint8_t* strips;   // per kv pair: bit w = the Gaussian blended into strip w of its tile
     uint32_t* hist;
     uint32_t* partials;
 };
@@ -163,6 +164,7 @@ Binning carve_binning(void* buf, int64_t K, size_t* total)
     const size_t n = (size_t)(K > 0 ? K : 1);
     b.pairsA = c.take<uint64_t>(n);
     b.pairsB = c.take<uint64_t>(n);
+    b.strips = c.take<uint8_t>(n);
     b.hist = c.take<uint32_t>(fgs::radix_hist_words(n));
     b.partials = c.take<uint32_t>(scan_words_for_hist(n));
     if (total) *total = c.off;
@@ -450,6 +452,7 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
         }
         if ((rc = sync_check(s, st, "binning"))) return rc;
         b.pairs = bn.pairsA;
+        b.kv_strips = bn.strips;
     } else {
         b.pairs = nullptr;
     }
@@ -538,6 +541,7 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         fgs::BlendArgs b{};
         b.ranges = im.ranges;
         b.pairs = bn.pairsA;
+        b.kv_strips = bn.strips;
         b.W = W;
         b.H = H;
         b.gx = gx;

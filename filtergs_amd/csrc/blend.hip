@@ -5,21 +5,23 @@
 //
 // One workgroup of 256 lanes (four wave64s) per 16x16 tile, one pixel per
 // lane; wave w owns the 16x4 strip of rows 4w..4w+3.  Gaussians of the tile's
-// range are staged in LDS in batches of BATCH.  While staging, every Gaussian
-// is tested against the four strips with a conservative box of the region
-// where alpha >= 1/255 (the ellipse q <= 2 ln(255 o) of the conic, slightly
-// inflated), and each wave gets a compacted list of the batch positions that
-// can touch its strip (ballot + prefix count, order preserved).  A wave then
-// walks only its own list: no per-Gaussian cull test in the inner loop.
+// range are staged in LDS in batches of BATCH, and each wave walks a compacted
+// list of the batch slots that concern its strip (ballot + prefix count, order
+// preserved), so the inner loop carries no per-Gaussian cull test.
 //
-// Forward: point_weight (the maximum T*alpha of each Gaussian over all
-// pixels) is reduced in LDS per batch and published with one float-as-uint
-// atomicMax per (tile, Gaussian), so it is exact and deterministic.
-// Backward: only positions below the strip's largest n_contrib enter the
-// lists; the nine per-pixel gradient terms of each Gaussian are reduced
-// across the wave (a DPP reduce-scatter inside 16-lane rows, then two
-// cross-row shuffles), added over the waves with LDS float atomics, and
-// published with one global atomic add per term per (tile, Gaussian).
+// Forward: a slot enters strip w's list when a conservative box of the region
+// where alpha >= 1/255 (the ellipse q <= 2 ln(255 o) of the conic, slightly
+// inflated) touches the strip.  Per slot and strip the largest T*alpha is
+// kept in LDS; after the batch it gives point_weight (one float-as-uint
+// atomicMax per (tile, Gaussian): exact and deterministic) and the pair's
+// strip mask -- the strips where the pair actually blended -- stored next to
+// the sorted pairs.
+// Backward: the lists come from those masks, so every listed slot blends into
+// at least one pixel of the strip and pairs that blended nowhere are never
+// loaded.  The nine per-pixel gradient terms of each Gaussian are reduced
+// across the wave (a DPP reduce-scatter inside 16-lane rows, then two lane
+// swaps), added over the waves with LDS float atomics, and published with one
+// global atomic add per term per (tile, Gaussian).
 #include "fgs_device.h"
 #include "fgs_internal.h"
 
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
     __shared__ float4 s_g0[BATCH];   // x, y, a', b'  (power in log2 units)
     __shared__ float2 s_g1[BATCH];   // c', opacity
     __shared__ float4 s_col[BATCH];  // r, g, b, id bits
-    __shared__ uint32_t s_w[BATCH];
+    __shared__ uint32_t s_w[kStrips][BATCH];   // per strip: max T*alpha (float bits)
     __shared__ float s_kv[BATCH];
     __shared__ uint32_t s_kc[BATCH];
     __shared__ uint16_t s_list[kStrips][BATCH];
@@ -213,7 +215,8 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
                                        __uint_as_float(g));
                 bits = strip_bits(alpha_box(m, co), tx0, ty0);
             }
-            s_w[q] = 0u;
+#pragma unroll
+            for (int w = 0; w < kStrips; ++w) s_w[w][q] = 0u;
             s_kv[q] = 0.f;
             s_kc[q] = 0u;
         }
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
                     wmax = w;
                     max_id = (int)__float_as_uint(col.w);
                 }
-                atomicMax(&s_w[j], __float_as_uint(w));
+                atomicMax(&s_w[wave][j], __float_as_uint(w));
                 if (kv_sum) {
                     atomicAdd(&s_kv[j], w);
                     atomicAdd(&s_kc[j], 1u);
@@ -251,7 +254,14 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
         __syncthreads();
         if (q < BATCH && k < n) {
             const uint32_t g = __float_as_uint(s_col[q].w);
-            if (s_w[q] != 0u) atomicMax((uint32_t*)&point_weight[g], s_w[q]);
+            uint32_t m = 0u, used = 0u;
+#pragma unroll
+            for (int w = 0; w < kStrips; ++w) {
+                m = s_w[w][q] > m ? s_w[w][q] : m;
+                used |= (s_w[w][q] != 0u ? 1u : 0u) << w;
+            }
+            if (m != 0u) atomicMax((uint32_t*)&point_weight[g], m);
+            a.kv_strips[range.x + k] = (uint8_t)used;
             if (kv_sum) {
                 // each kv pair belongs to exactly one tile: plain stores
                 kv_sum[range.x + k] = s_kv[q];
@@ -356,7 +366,6 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
     const int py = ty * FGS_TILE + (tid / FGS_TILE);
     const bool inside = px < a.W && py < a.H;
     const float pfx = (float)px, pfy = (float)py;
-    const float tx0 = (float)(tx * FGS_TILE), ty0 = (float)(ty * FGS_TILE);
     const int HW = a.W * a.H;
     const int pix = a.W * py + px;
 
@@ -382,18 +391,13 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
     float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
 
-    // positions >= the largest n_contrib of a strip contribute to none of its
-    // pixels; past the largest of the tile to no pixel at all
+    // positions at or past the largest n_contrib of the tile blend into no pixel
     const uint32_t wmax = wave_max_u32(last);
     if (lane == 0) s_wmax[wave] = wmax;
     __syncthreads();
-    uint32_t smax[kStrips];
     uint32_t bmax = 0;
 #pragma unroll
-    for (int w = 0; w < kStrips; ++w) {
-        smax[w] = s_wmax[w];
-        bmax = smax[w] > bmax ? smax[w] : bmax;
-    }
+    for (int w = 0; w < kStrips; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
     const int n = (int)min(bmax, range.y - range.x);
     const int rounds = (n + BATCH - 1) / BATCH;
     const float ddelx_dx = 0.5f * (float)a.W;
@@ -405,8 +409,11 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
         const int k = r * BATCH + q;
         uint32_t bits = 0u;
         if (q < BATCH) {
-            if (k < n) {
-                // back to front: position n-1-k of the tile's list
+            // back to front: position n-1-k of the tile's list.  Only the
+            // strips where the forward blended this pair take part, and a
+            // pair that blended nowhere is not even loaded.
+            if (k < n) bits = a.kv_strips[range.x + n - 1 - k];
+            if (bits != 0u) {
                 const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
                 const float2 m = a.means2D[g];
                 const float4 co = a.conic_opacity[g];
@@ -415,11 +422,6 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
                 s_con[q] = co;
                 s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
                                        __uint_as_float(g));
-                bits = strip_bits(alpha_box(m, co), tx0, ty0);
-                const uint32_t position = (uint32_t)(n - 1 - k);
-#pragma unroll
-                for (int w = 0; w < kStrips; ++w)
-                    if (position >= smax[w]) bits &= ~(1u << w);
             }
 #pragma unroll
             for (int t = 0; t < kGradTerms; ++t) s_g[t][q] = 0.f;
@@ -441,7 +443,6 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
             if (!__any(act)) continue;   // wave-uniform
             // every lane computes; inactive lanes contribute zeros and keep
             // their state (all terms stay finite: G <= 1, alpha <= 0.99)
-            const float4 con = s_con[j];
             const float4 col = s_col[j];
             const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
             const float Tn = T * inv;
@@ -456,13 +457,15 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
             v[0] = dch * dp0;
             v[1] = dch * dp1;
             v[2] = dch * dp2;
-            // h = dL/dG * G; the constant factors (-W/2, -H/2, -1/2) are
-            // applied once per (tile, Gaussian) when the sums are published
+            // h = dL/dG * G.  dL/dmean2D = -(W/2, H/2) * (A Sx + B Sy, B Sx + C Sy)
+            // and dL/dconic = -1/2 (Sxx, Sxy, Syy) with Sx = sum h dx, ...:
+            // the conic and the constants are applied once per (tile,
+            // Gaussian) when the sums are published
             const float vop = flows ? G * dLa : 0.f;
             const float h = g1.y * vop;
             const float hdx = h * dx, hdy = h * dy;
-            v[3] = hdx * con.x + hdy * con.y;
-            v[4] = hdy * con.z + hdx * con.y;
+            v[3] = hdx;
+            v[4] = hdy;
             v[5] = hdx * dx;
             v[6] = hdx * dy;
             v[7] = hdy * dy;
@@ -483,7 +486,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
             if (lane == 16) atomicAdd(&s_g[8][j], zop);
         }
         __syncthreads();
-        if (q < BATCH && k < n) {
+        if (bits != 0u) {
             const uint32_t g = __float_as_uint(s_col[q].w);
             float t[kGradTerms];
 #pragma unroll
@@ -491,8 +494,11 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
             if (t[0] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 0], t[0]);
             if (t[1] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 1], t[1]);
             if (t[2] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 2], t[2]);
-            if (t[3] != 0.f) atomic_add_f32(&dL_dmean2D[3 * g + 0], -ddelx_dx * t[3]);
-            if (t[4] != 0.f) atomic_add_f32(&dL_dmean2D[3 * g + 1], -ddely_dy * t[4]);
+            if (t[3] != 0.f || t[4] != 0.f) {
+                const float4 con = s_con[q];
+                atomic_add_f32(&dL_dmean2D[3 * g + 0], -ddelx_dx * (con.x * t[3] + con.y * t[4]));
+                atomic_add_f32(&dL_dmean2D[3 * g + 1], -ddely_dy * (con.y * t[3] + con.z * t[4]));
+            }
             if (t[5] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 0], -0.5f * t[5]);
             if (t[6] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 1], -0.5f * t[6]);
             if (t[7] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 2], -0.5f * t[7]);

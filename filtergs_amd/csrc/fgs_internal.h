@@ -60,6 +60,8 @@ struct PreBwdOut {
 struct BlendArgs {
     const uint2* ranges;
     const uint64_t* pairs;     // sorted (tile << 32 | gaussian)
+    uint8_t* kv_strips;        // per kv pair: strips of the tile it blended into
+                               // (written by the forward, read by the backward)
     int W, H, gx, gy;
     const float2* means2D;
     const float* colors;       // [P,3]
