@@ -69,6 +69,34 @@ class Gaussians(ctypes.Structure):
     ]
 
 
+class BatchView(ctypes.Structure):
+    """fgs_batch_view."""
+    _fields_ = [
+        ("image_height", c_int32),
+        ("image_width", c_int32),
+        ("tanfovx", c_float),
+        ("tanfovy", c_float),
+        ("viewmatrix", c_float * 16),
+        ("projmatrix", c_float * 16),
+        ("campos", c_float * 3),
+        ("bg", c_float * 3),
+    ]
+
+
+class BatchSizes(ctypes.Structure):
+    """fgs_batch_sizes."""
+    _fields_ = [
+        ("views", c_int32),
+        ("P", c_int32),
+        ("tiles", c_int64),
+        ("pixels", c_int64),
+        ("plan_bytes", c_size_t),
+        ("geom_bytes", c_size_t),
+        ("image_bytes", c_size_t),
+        ("backward_ws_bytes", c_size_t),
+    ]
+
+
 _SIGS = {
     "fgs_geom_bytes": (c_size_t, [c_int32]),
     "fgs_image_bytes": (c_size_t, [c_int32, c_int32]),
@@ -104,6 +132,16 @@ _SIGS = {
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p]),
+    "fgs_batch_sizes_query": (c_int32, [c_void_p, c_int32, c_int32, c_void_p]),
+    "fgs_batch_binning_bytes": (c_size_t, [c_int64]),
+    "fgs_batch_plan": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
+    "fgs_batch_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "fgs_batch_backward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
     "fgs_profile_enable": (c_int32, [c_int32]),
     "fgs_profile_read": (c_int32, [c_void_p, c_void_p, c_int32, c_int32]),
     "fgs_set_work_shape": (c_int32, [c_int32, c_int32]),

@@ -1,0 +1,270 @@
+"""Batched multi-view rasterization: V cameras over one set of Gaussians per
+call, through the batched C ABI (fgs_batch_*).
+
+The reference renders one camera per ``_C.rasterize_gaussians`` call
+(antialias/rasterize_points.cu:36-260) and a training step loops over its
+cameras.  ``BatchRasterizer`` renders all of them with one launch per stage:
+the Gaussians are read once per step by the preprocess, one depth sort and one
+tile sort cover every view, one blend launch renders every tile of every view,
+and the backward folds the per-view gradients into one set of parameter
+gradients.  The kv-pair buffer has a capacity, so a step needs no host read
+and ``capture()`` can record it in a HIP graph; ``autotune()`` picks the blend
+work shape by timing the step.
+
+Per view the forward outputs equal those of ``ext.rasterize_gaussians`` bit
+for bit (images, radii, point ids, point weights); gradients equal the sum of
+the per-view backward calls up to float summation order.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BatchSizes, BatchView, Gaussians, RasterSettings, call, ptr
+
+
+def _mat16(m) -> list:
+    a = m.detach().cpu().numpy() if torch.is_tensor(m) else np.asarray(m)
+    return [float(x) for x in np.asarray(a, dtype=np.float32).reshape(-1)]
+
+
+def _vec3(v) -> list:
+    a = v.detach().cpu().numpy() if torch.is_tensor(v) else np.asarray(v)
+    return [float(x) for x in np.asarray(a, dtype=np.float32).reshape(-1)[:3]]
+
+
+class BatchRasterizer:
+    """Forward and backward of V views of the same P Gaussians per call.
+
+    ``cameras``: objects with ``width``, ``height``, ``tanfovx``, ``tanfovy``,
+    ``world_view_transform`` and ``full_proj_transform`` (4x4, the transposed
+    row-vector matrices of the plugin settings) and ``center`` (camera
+    position).  ``bg``: one colour for every view or one per view.
+    """
+
+    def __init__(self, cameras: Sequence, P: int, *, bg=(0.0, 0.0, 0.0), sh_degree: int = 3,
+                 variant: int = 0, use_filter: bool = True, scale_modifier: float = 1.0,
+                 ogfs_epsilon_scale: float = 0.2, ogfs_epsilon_max: float = 0.2,
+                 ogfs_energy_floor: float = 0.05, ogfs_radius_ratio_min: float = 0.7,
+                 grad_mode: int = 0, debug: bool = False, device="cuda", k_headroom: float = 1.1):
+        if not torch.cuda.is_available():
+            raise _lib.FGSError("filtergs_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device(device)
+        self.V = len(cameras)
+        self.P = int(P)
+        self.k_headroom = float(k_headroom)
+        bgs = np.asarray(bg, dtype=np.float32).reshape(-1, 3)
+        if bgs.shape[0] == 1:
+            bgs = np.repeat(bgs, self.V, axis=0)
+        views = (BatchView * self.V)()
+        self.shapes = []
+        for v, cam in enumerate(cameras):
+            w = views[v]
+            w.image_height = int(cam.height)
+            w.image_width = int(cam.width)
+            w.tanfovx = float(cam.tanfovx)
+            w.tanfovy = float(cam.tanfovy)
+            w.viewmatrix[:] = _mat16(cam.world_view_transform)
+            w.projmatrix[:] = _mat16(cam.full_proj_transform)
+            w.campos[:] = _vec3(cam.center)
+            w.bg[:] = [float(x) for x in bgs[v]]
+            self.shapes.append((int(cam.height), int(cam.width)))
+        self._views = views
+        lib = _lib.load()
+        self.sizes = BatchSizes()
+        call("fgs_batch_sizes_query", views, self.V, self.P, ctypes.byref(self.sizes))
+        u8 = dict(dtype=torch.uint8, device=self.device)
+        f32 = dict(dtype=torch.float32, device=self.device)
+        self.plan = torch.empty((self.sizes.plan_bytes,), **u8)
+        call("fgs_batch_plan", views, self.V, self.plan.data_ptr(), _lib.stream_handle(self.device))
+        self.geom = torch.empty((self.sizes.geom_bytes,), **u8)
+        self.image = torch.empty((self.sizes.image_bytes,), **u8)
+        self.ws = torch.empty((self.sizes.backward_ws_bytes,), **u8)
+        npix = int(self.sizes.pixels)
+        self.radii = torch.zeros((self.V, self.P), dtype=torch.int32, device=self.device)
+        self.color = torch.zeros((3 * npix,), **f32)
+        self.point_id = torch.zeros((npix,), dtype=torch.int32, device=self.device)
+        self.weight_pixel = torch.zeros((npix,), **f32)
+        self.point_weight = torch.zeros((self.V, self.P), **f32)
+        self.counts = torch.zeros((4,), dtype=torch.int64, device=self.device)
+        self.dL_dmeans2D = torch.zeros((self.V, self.P, 3), **f32)
+        self.dL_dcolors = torch.zeros((self.V, self.P, 3), **f32)
+        self.K_cap = 0
+        self.binning = torch.empty((0,), **u8)
+        self.pix_base = np.concatenate([[0], np.cumsum([h * w for h, w in self.shapes])]).astype(np.int64)
+        s = RasterSettings()
+        s.scale_modifier = float(scale_modifier)
+        s.sh_degree = int(sh_degree)
+        s.debug = int(bool(debug))
+        s.variant = int(variant)
+        s.use_filter = int(bool(use_filter))
+        s.ogfs_epsilon_scale = float(ogfs_epsilon_scale)
+        s.ogfs_epsilon_max = float(ogfs_epsilon_max)
+        s.ogfs_energy_floor = float(ogfs_energy_floor)
+        s.ogfs_radius_ratio_min = float(ogfs_radius_ratio_min)
+        s.ogfs_enable_stats = 0
+        s.grad_mode = int(grad_mode)
+        self.settings = s
+        self.sh_degree = int(sh_degree)
+        self._g = None
+        self._keep = None
+        self._lib = lib
+
+    # -- inputs ------------------------------------------------------------
+    def bind(self, means3D, opacities, scales=None, rotations=None, shs=None, colors_precomp=None,
+             cov3D_precomp=None):
+        """Set the Gaussian tensors the next calls read (kept by reference:
+        a captured graph reads whatever these tensors hold at replay)."""
+        def f(t):
+            if t is None or t.numel() == 0:
+                return None
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("Gaussian tensors must be contiguous float32 on the GPU")
+            return t
+        keep = dict(means3D=f(means3D), opacities=f(opacities.reshape(-1)), scales=f(scales),
+                    rotations=f(rotations), shs=f(shs), colors=f(colors_precomp),
+                    cov3D=f(cov3D_precomp))
+        if int(means3D.shape[0]) != self.P:
+            raise ValueError("means3D has a different number of Gaussians than the plan")
+        if keep["colors"] is not None and keep["colors"].numel() != self.V * self.P * 3:
+            raise ValueError("colors_precomp of a batch is per view: [V, P, 3]")
+        g = Gaussians()
+        g.P = self.P
+        g.M = int(shs.shape[1]) if keep["shs"] is not None else 0
+        g.means3D = ptr(keep["means3D"])
+        g.shs = ptr(keep["shs"])
+        g.colors_precomp = ptr(keep["colors"])
+        g.opacities = ptr(keep["opacities"])
+        g.scales = ptr(keep["scales"])
+        g.rotations = ptr(keep["rotations"])
+        g.cov3D_precomp = ptr(keep["cov3D"])
+        self._g, self._keep = g, keep
+
+    def reserve(self, K_cap: int):
+        """Size the kv-pair buffer for K_cap pairs."""
+        K_cap = int(K_cap)
+        if K_cap > self.K_cap:
+            self.binning = torch.empty((self._lib.fgs_batch_binning_bytes(K_cap),), dtype=torch.uint8,
+                                       device=self.device)
+            self.K_cap = K_cap
+
+    # -- passes ------------------------------------------------------------
+    def _forward_launch(self):
+        self.point_weight.zero_()
+        call("fgs_batch_forward", ctypes.byref(self.settings), ctypes.byref(self._g),
+             ctypes.byref(self.sizes), self.plan.data_ptr(), self.geom.data_ptr(),
+             self.image.data_ptr(), ptr(self.binning), self.K_cap, self.radii.data_ptr(),
+             self.color.data_ptr(), self.point_id.data_ptr(), self.weight_pixel.data_ptr(),
+             self.point_weight.data_ptr(), self.counts.data_ptr(),
+             _lib.stream_handle(self.device))
+
+    def _backward_launch(self, dL_dpix, out):
+        self.dL_dmeans2D.zero_()
+        self.dL_dcolors.zero_()
+        call("fgs_batch_backward", ctypes.byref(self.settings), ctypes.byref(self._g),
+             ctypes.byref(self.sizes), self.plan.data_ptr(), self.radii.data_ptr(),
+             self.geom.data_ptr(), self.image.data_ptr(), ptr(self.binning), self.K_cap,
+             dL_dpix.data_ptr(), self.dL_dmeans2D.data_ptr(), self.dL_dcolors.data_ptr(),
+             out["opacities"].data_ptr(), out["means3D"].data_ptr(), out["cov3D"].data_ptr(),
+             ptr(out["shs"]), ptr(out["scales"]), ptr(out["rotations"]), self.ws.data_ptr(),
+             _lib.stream_handle(self.device))
+
+    def forward(self):
+        """Render every view.  Grows the kv-pair buffer when the views need
+        more pairs than it holds (one host read of the pair count)."""
+        if self._g is None:
+            raise RuntimeError("bind() the Gaussians first")
+        self._forward_launch()
+        ch = self.counts.cpu()
+        if int(ch[2]) & 1:
+            raise RuntimeError("Point is filtered although prefiltered is set")
+        K = int(ch[1])
+        if K > self.K_cap:
+            self.reserve(int(K * self.k_headroom) + 1024)
+            self._forward_launch()
+        return K
+
+    def grad_buffers(self):
+        """Output tensors of the parameter gradients (summed over views)."""
+        f32 = dict(dtype=torch.float32, device=self.device)
+        k = self._keep
+        M = int(k["shs"].shape[1]) if k["shs"] is not None else 0
+        return dict(means3D=torch.empty((self.P, 3), **f32), cov3D=torch.empty((self.P, 6), **f32),
+                    opacities=torch.empty((self.P,), **f32),
+                    shs=torch.empty((self.P, M, 3), **f32) if M else None,
+                    scales=torch.empty((self.P, 3), **f32) if k["cov3D"] is None else None,
+                    rotations=torch.empty((self.P, 4), **f32) if k["cov3D"] is None else None)
+
+    def backward(self, dL_dpix, out=None):
+        """dL_dpix: flat [3 * pixels] (the views' [3,H,W] blocks
+        concatenated) or a list of [3,H,W] tensors.  Returns the parameter
+        gradients summed over views; per-view dL/dmeans2D and dL/dcolors stay
+        in ``dL_dmeans2D`` / ``dL_dcolors`` ([V, P, 3])."""
+        if isinstance(dL_dpix, (list, tuple)):
+            dL_dpix = torch.cat([t.reshape(-1) for t in dL_dpix])
+        dL_dpix = dL_dpix.to(dtype=torch.float32).contiguous()
+        if dL_dpix.numel() != 3 * int(self.sizes.pixels):
+            raise ValueError("dL_dpix must hold 3 * pixels values")
+        out = out if out is not None else self.grad_buffers()
+        self._backward_launch(dL_dpix, out)
+        return out
+
+    # -- views of the outputs ---------------------------------------------
+    def view_color(self, v):
+        h, w = self.shapes[v]
+        b = int(self.pix_base[v])
+        return self.color[3 * b:3 * (b + h * w)].view(3, h, w)
+
+    def view_pixels(self, t, v):
+        h, w = self.shapes[v]
+        b = int(self.pix_base[v])
+        return t[b:b + h * w].view(h, w)
+
+    # -- graph capture and autotuning ---------------------------------------
+    def capture(self, dL_dpix, out=None, warmup: int = 2):
+        """Record forward + backward in a HIP graph.  The capacity must
+        already hold the views' pairs (run forward() first).  Returns
+        (graph, grads); graph.replay() recomputes both from the bound tensors
+        and dL_dpix.  After a replay, ``counts[1] <= K_cap`` must hold."""
+        out = out if out is not None else self.grad_buffers()
+        dL_dpix = dL_dpix.contiguous()
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                self._forward_launch()
+                self._backward_launch(dL_dpix, out)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self._forward_launch()
+            self._backward_launch(dL_dpix, out)
+        return graph, out
+
+    def autotune(self, dL_dpix, shapes=((256, 128), (256, 64), (256, 256), (128, 128), (128, 64)),
+                 iters: int = 3):
+        """Time forward + backward for each blend work shape and keep the
+        fastest (process-wide).  Returns {shape: ms}."""
+        out = self.grad_buffers()
+        res = {}
+        for fb, bb in shapes:
+            _lib.set_work_shape(fb, bb)
+            self._forward_launch()
+            self._backward_launch(dL_dpix, out)
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(iters):
+                self._forward_launch()
+                self._backward_launch(dL_dpix, out)
+            t1.record()
+            t1.synchronize()
+            res[(fb, bb)] = t0.elapsed_time(t1) / iters
+        best = min(res, key=res.get)
+        _lib.set_work_shape(*best)
+        return res

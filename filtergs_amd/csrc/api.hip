@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/filtergs_amd.h"
 #include "fgs_device.h"
@@ -176,6 +177,18 @@ struct Image {
     uint32_t* n_contrib;
     uint2* ranges;
 };
+
+// Per-pixel state of npix pixels and T tiles (a batch concatenates its views).
+Image carve_image_n(void* buf, size_t npix, size_t T, size_t* total)
+{
+    Carver c{ static_cast<char*>(buf) };
+    Image im;
+    im.final_T = c.take<float>(npix > 0 ? npix : 1);
+    im.n_contrib = c.take<uint32_t>(npix > 0 ? npix : 1);
+    im.ranges = c.take<uint2>(T > 0 ? T : 1);
+    if (total) *total = c.off;
+    return im;
+}
 
 Image carve_image(void* buf, int W, int H, size_t* total)
 {
@@ -673,6 +686,278 @@ int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const 
                                      sh_floats, xyz_out, scaling_out, rotation_out, opacity_out,
                                      colors_out, shs_out, bad_index, (hipStream_t)stream),
                  "gather_attributes");
+}
+
+
+/* ---- batched multi-view path ------------------------------------------- */
+
+static int batch_validate(const fgs_batch_view* views, int32_t V, int32_t P)
+{
+    if (!views) return fail(FGS_E_INVALID, "null views%s");
+    if (V <= 0 || V > fgs::kBatchMaxViews) return fail(FGS_E_INVALID, "views must be in [1, 64]%s");
+    if (P < 0 || (uint64_t)P > fgs::kBatchIdxMask + 1)
+        return fail(FGS_E_INVALID, "a batch holds at most 2^26 Gaussians%s");
+    if ((uint64_t)V * (uint64_t)P > 0x7FFFFFFFull)
+        return fail(FGS_E_INVALID, "views * Gaussians must stay below 2^31%s");
+    for (int v = 0; v < V; ++v) {
+        const fgs_batch_view& w = views[v];
+        if (w.image_width <= 0 || w.image_height <= 0)
+            return fail(FGS_E_INVALID, "image size must be positive%s");
+        if ((w.image_width + 15) / 16 > 65535 || (w.image_height + 15) / 16 > 65535)
+            return fail(FGS_E_INVALID, "image larger than 65535 tiles per side%s");
+    }
+    return FGS_OK;
+}
+
+int fgs_batch_sizes_query(const fgs_batch_view* views, int32_t V, int32_t P, fgs_batch_sizes* out)
+{
+    int rc = batch_validate(views, V, P);
+    if (rc) return rc;
+    if (!out) return fail(FGS_E_INVALID, "null output%s");
+    int64_t T = 0, npix = 0;
+    for (int v = 0; v < V; ++v) {
+        const int gx = (views[v].image_width + 15) / 16, gy = (views[v].image_height + 15) / 16;
+        T += (int64_t)gx * gy;
+        npix += (int64_t)views[v].image_width * views[v].image_height;
+    }
+    if (T > 0xFFFFFFFELL) return fail(FGS_E_INVALID, "too many tiles in the batch%s");
+    out->views = V;
+    out->P = P;
+    out->tiles = T;
+    out->pixels = npix;
+    out->plan_bytes = align256(sizeof(fgs::BatchView) * (size_t)V);
+    carve_geom(nullptr, V * P, &out->geom_bytes);
+    carve_image_n(nullptr, (size_t)npix, (size_t)T, &out->image_bytes);
+    out->backward_ws_bytes = align256((size_t)(V * P > 0 ? V * P : 1) * 3 * sizeof(float)) +
+                             align256((size_t)(V * P > 0 ? V * P : 1) * sizeof(float));
+    return FGS_OK;
+}
+
+size_t fgs_batch_binning_bytes(int64_t K_cap)
+{
+    size_t t = 0;
+    carve_binning(nullptr, K_cap, &t);
+    return t;
+}
+
+int fgs_batch_plan(const fgs_batch_view* views, int32_t V, void* plan, void* stream)
+{
+    int rc = batch_validate(views, V, 0);
+    if (rc) return rc;
+    if (!plan) return fail(FGS_E_INVALID, "null plan buffer%s");
+    std::vector<fgs::BatchView> bv((size_t)V);
+    uint32_t tile_base = 0;
+    uint64_t pix_base = 0;
+    for (int v = 0; v < V; ++v) {
+        const fgs_batch_view& w = views[v];
+        fgs::BatchView& b = bv[(size_t)v];
+        memcpy(b.view, w.viewmatrix, sizeof(b.view));
+        memcpy(b.proj, w.projmatrix, sizeof(b.proj));
+        memcpy(b.campos, w.campos, sizeof(b.campos));
+        memcpy(b.bg, w.bg, sizeof(b.bg));
+        b.tanx = w.tanfovx;
+        b.tany = w.tanfovy;
+        b.W = w.image_width;
+        b.H = w.image_height;
+        b.gx = (b.W + 15) / 16;
+        b.gy = (b.H + 15) / 16;
+        b.fy = (float)b.H / (2.0f * w.tanfovy);   // as make_pre_args
+        b.fx = (float)b.W / (2.0f * w.tanfovx);
+        b.tile_base = tile_base;
+        b.pad = 0;
+        b.pix_base = pix_base;
+        tile_base += (uint32_t)(b.gx * b.gy);
+        pix_base += (uint64_t)b.W * (uint64_t)b.H;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = check(hipMemcpyAsync(plan, bv.data(), sizeof(fgs::BatchView) * bv.size(),
+                                   hipMemcpyHostToDevice, st), "plan upload")))
+        return rc;
+    return check(hipStreamSynchronize(st), "plan upload");
+}
+
+static int batch_common(const fgs_raster_settings* s, const fgs_gaussians* g, const fgs_batch_sizes* sz)
+{
+    if (!s || !g || !sz) return fail(FGS_E_INVALID, "null settings, gaussians or sizes%s");
+    if (g->P != sz->P) return fail(FGS_E_INVALID, "gaussians.P differs from the plan's P%s");
+    if (g->P > 0 && (!g->means3D || !g->opacities))
+        return fail(FGS_E_INVALID, "null means3D or opacities%s");
+    if ((g->shs == nullptr) == (g->colors_precomp == nullptr) && g->P > 0)
+        return fail(FGS_E_INVALID, "provide exactly one of shs / colors_precomp%s");
+    if (g->P > 0 && g->shs && (g->M < (s->sh_degree + 1) * (s->sh_degree + 1) || s->sh_degree > 3 || s->sh_degree < 0))
+        return fail(FGS_E_INVALID, "sh_degree must be in [0,3] with M >= (deg+1)^2%s");
+    const bool sr = g->scales && g->rotations;
+    if (g->P > 0 && (sr == (g->cov3D_precomp != nullptr)))
+        return fail(FGS_E_INVALID, "provide exactly one of scales+rotations / cov3D_precomp%s");
+    return FGS_OK;
+}
+
+static fgs::PreArgs batch_pre_args(const fgs_raster_settings* s, const fgs_gaussians* g)
+{
+    fgs_raster_settings c = *s;
+    c.image_width = 16;   // per-view fields come from the plan
+    c.image_height = 16;
+    c.tanfovx = 1.f;
+    c.tanfovy = 1.f;
+    return make_pre_args(&c, g);
+}
+
+int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, const fgs_batch_sizes* sz,
+                      const void* plan, void* geom, void* image, void* binning, int64_t K_cap,
+                      int32_t* radii, float* out_color, int32_t* out_point_id, float* out_weight_pixel,
+                      float* out_point_weight, uint64_t* counts, void* stream)
+{
+    int rc = batch_common(s, g, sz);
+    if (rc) return rc;
+    if (!plan || !counts || !image || !out_color || !out_point_id || !out_weight_pixel)
+        return fail(FGS_E_INVALID, "null plan or output%s");
+    if (K_cap < 0) return fail(FGS_E_INVALID, "negative capacity%s");
+    hipStream_t st = (hipStream_t)stream;
+    const int V = sz->views, P = g->P;
+    const size_t n = (size_t)V * (size_t)P;
+    const int T = (int)sz->tiles;
+    const fgs::BatchView* views = static_cast<const fgs::BatchView*>(plan);
+    if ((rc = check(fgs::launch_fill_u32(counts, 0u, 8, st), "counts clear"))) return rc;
+    Image im = carve_image_n(image, (size_t)sz->pixels, (size_t)T, nullptr);
+    Geom gm = carve_geom(geom, (int)(n > 0 ? n : 1), nullptr);
+    fgs::BlendArgs b{};
+    b.ranges = im.ranges;
+    b.batch = g_shape[0];
+    b.id_stride = (uint32_t)P;
+    if (n > 0) {
+        if (!geom || !radii || (K_cap > 0 && !binning) || !out_point_weight)
+            return fail(FGS_E_INVALID, "null geometry / binning buffer or output%s");
+        const fgs::PreArgs a = batch_pre_args(s, g);
+        fgs::PreOut o{};
+        o.radii = radii;
+        o.means2D = gm.means2D;
+        o.depths = gm.depths;
+        o.conic_opacity = gm.conic_opacity;
+        o.rgb = gm.rgb;
+        o.tiles = gm.tiles;
+        o.recs = gm.recs;
+        o.vkeys = gm.vkeysA;
+        o.error_flag = (int*)(counts + 2);
+        {
+            ProfScope ps(ST_PREPROCESS, st);
+            if ((rc = check(fgs::launch_preprocess_fwd_batch(a, views, V, o, st), "batch preprocess")))
+                return rc;
+        }
+        // (view, depth, index) order: bits 26..63, 5 passes -> result in B
+        {
+            ProfScope ps(ST_DEPTH_SORT, st);
+            if ((rc = check(fgs::radix_sort_u64(gm.vkeysA, gm.vkeysB, n, fgs::kBatchIdxBits, 5, gm.hist,
+                                                gm.partials, st), "batch depth sort")))
+                return rc;
+        }
+        unsigned long long* cnt = (unsigned long long*)counts;
+        {
+            ProfScope ps(ST_OFFSETS, st);
+            if ((rc = check(fgs::launch_sorted_records_batch(gm.vkeysB, gm.recs, n, P, gm.tiles_s,
+                                                             gm.rects_s, st), "batch records")))
+                return rc;
+            if ((rc = check(fgs::launch_scan_tiles_batch(gm.tiles_s, gm.offs, n, gm.partials, cnt, st),
+                            "batch offsets")))
+                return rc;
+        }
+        if (K_cap > 0) {
+            Binning bn = carve_binning(binning, K_cap, nullptr);
+            const int passes = (bit_length((uint32_t)T) + 7) / 8;
+            uint64_t* first = (passes % 2 == 0) ? bn.pairsA : bn.pairsB;
+            uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
+            {
+                ProfScope ps(ST_DUPLICATE, st);
+                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, P, views,
+                                                            first, (size_t)K_cap, cnt + 1, st),
+                                "batch duplicate")))
+                    return rc;
+            }
+            {
+                ProfScope ps(ST_TILE_SORT, st);
+                if ((rc = check(fgs::radix_sort_u64(first, second, (size_t)K_cap, 32, passes, bn.hist,
+                                                    bn.partials, st), "batch tile sort")))
+                    return rc;
+            }
+            b.pairs = bn.pairsA;
+            b.kv_strips = bn.strips;
+        }
+        b.means2D = gm.means2D;
+        b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+        b.conic_opacity = gm.conic_opacity;
+    }
+    {
+        ProfScope ps(ST_RANGES, st);
+        if ((rc = check(fgs::launch_ranges_batch(b.pairs, b.pairs ? (size_t)K_cap : 0, im.ranges, T, st),
+                        "batch ranges")))
+            return rc;
+    }
+    {
+        ProfScope ps(ST_BLEND_FWD, st);
+        if ((rc = check(fgs::launch_blend_fwd_batch(b, views, V, T, im.final_T, im.n_contrib, out_color,
+                                                    out_point_id, out_weight_pixel, out_point_weight,
+                                                    st), "batch blend")))
+            return rc;
+    }
+    return sync_check(s, st, "batch forward");
+}
+
+int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const fgs_batch_sizes* sz,
+                       const void* plan, const int32_t* radii, const void* geom, const void* image,
+                       const void* binning, int64_t K_cap, const float* dL_dpix, float* dL_dmeans2D,
+                       float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
+                       float* dL_dsh, float* dL_dscales, float* dL_drot, void* ws, void* stream)
+{
+    int rc = batch_common(s, g, sz);
+    if (rc) return rc;
+    const int V = sz->views, P = g->P;
+    const size_t n = (size_t)V * (size_t)P;
+    if (n == 0) return FGS_OK;
+    if (!plan || !radii || !geom || !image || (K_cap > 0 && !binning) || !dL_dpix || !dL_dmeans2D || !dL_dcolors ||
+        !dL_dopacity || !dL_dmeans3D || !dL_dcov3D || !ws)
+        return fail(FGS_E_INVALID, "null input, output or workspace%s");
+    if (g->shs && !dL_dsh) return fail(FGS_E_INVALID, "null dL_dsh%s");
+    if (!g->cov3D_precomp && (!dL_dscales || !dL_drot)) return fail(FGS_E_INVALID, "null dL_dscales / dL_drot%s");
+    hipStream_t st = (hipStream_t)stream;
+    const int T = (int)sz->tiles;
+    const fgs::BatchView* views = static_cast<const fgs::BatchView*>(plan);
+    Geom gm = carve_geom(const_cast<void*>(geom), (int)n, nullptr);
+    Image im = carve_image_n(const_cast<void*>(image), (size_t)sz->pixels, (size_t)T, nullptr);
+    float* dconic = static_cast<float*>(ws);
+    float* dop_view = reinterpret_cast<float*>(static_cast<char*>(ws) + align256(n * 3 * sizeof(float)));
+    if ((rc = check(fgs::launch_fill_u32(dconic, 0u, n * 3, st), "ws clear"))) return rc;
+    if ((rc = check(fgs::launch_fill_u32(dop_view, 0u, n, st), "ws clear"))) return rc;
+    if (K_cap > 0) {
+        Binning bn = carve_binning(const_cast<void*>(binning), K_cap, nullptr);
+        fgs::BlendArgs b{};
+        b.ranges = im.ranges;
+        b.pairs = bn.pairsA;
+        b.kv_strips = bn.strips;
+        b.means2D = gm.means2D;
+        b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+        b.conic_opacity = gm.conic_opacity;
+        b.bwd_batch = g_shape[1];
+        ProfScope ps(ST_BLEND_BWD, st);
+        if ((rc = check(fgs::launch_blend_bwd_batch(b, views, V, T, im.final_T, im.n_contrib, dL_dpix,
+                                                    dL_dmeans2D, dconic, dop_view, dL_dcolors,
+                                                    s->grad_mode == 1, st), "batch blend backward")))
+            return rc;
+    }
+    const fgs::PreArgs a = batch_pre_args(s, g);
+    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors };
+    fgs::PreBwdOut o{};
+    o.dL_dopacity = dL_dopacity;
+    o.dL_dmeans3D = dL_dmeans3D;
+    o.dL_dcov3D = dL_dcov3D;
+    o.dL_dsh = g->shs ? dL_dsh : nullptr;
+    o.dL_dscales = g->cov3D_precomp ? nullptr : dL_dscales;
+    o.dL_drot = g->cov3D_precomp ? nullptr : dL_drot;
+    {
+        ProfScope ps(ST_PREPROCESS_BWD, st);
+        if ((rc = check(fgs::launch_preprocess_bwd_batch(a, views, V, in, dop_view, o, st),
+                        "batch preprocess backward")))
+            return rc;
+    }
+    return sync_check(s, st, "batch backward");
 }
 
 }  // extern "C"

@@ -153,22 +153,21 @@ __device__ __forceinline__ float exponent2(float4 g0, float2 g1, float dx, float
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
+// Forward of one tile (one workgroup of FGS_BLOCK lanes).
 template <int BATCH>
-__global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __restrict__ final_T,
-                                                         uint32_t* __restrict__ n_contrib,
-                                                         float* __restrict__ out_color,
-                                                         int* __restrict__ out_point_id,
-                                                         float* __restrict__ out_weight_pixel,
-                                                         float* __restrict__ point_weight,
-                                                         float* __restrict__ kv_sum,
-                                                         uint32_t* __restrict__ kv_cnt)
+__device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, float* __restrict__ final_T,
+                                               uint32_t* __restrict__ n_contrib,
+                                               float* __restrict__ out_color,
+                                               int* __restrict__ out_point_id,
+                                               float* __restrict__ out_weight_pixel,
+                                               float* __restrict__ point_weight,
+                                               float* __restrict__ kv_sum, uint32_t* __restrict__ kv_cnt)
 {
     // Every product and sum of the per-pixel chain is written out (explicit
     // fmaf, no contraction by the compiler), so all work shapes round alike
     // and render bit-identical images.
 #pragma clang fp contract(off)
     static_assert(BATCH <= FGS_BLOCK, "one staging lane per batch slot");
-    const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
     const int wave = tid / FGS_WAVE;
@@ -278,9 +277,77 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* __r
         out_color[pix] = fmaf(T, a.bg[0], C0);
         out_color[HW + pix] = fmaf(T, a.bg[1], C1);
         out_color[2 * HW + pix] = fmaf(T, a.bg[2], C2);
-        out_point_id[pix] = max_id;
+        out_point_id[pix] = max_id >= 0 ? (int)((uint32_t)max_id - a.id_base) : -1;
         out_weight_pixel[pix] = wmax;
     }
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* final_T, uint32_t* n_contrib,
+                                                         float* out_color, int* out_point_id,
+                                                         float* out_weight_pixel, float* point_weight,
+                                                         float* kv_sum, uint32_t* kv_cnt)
+{
+    blend_fwd_tile<BATCH>(a, blockIdx.x, final_T, n_contrib, out_color, out_point_id, out_weight_pixel,
+                          point_weight, kv_sum, kv_cnt);
+}
+
+// View of a batch that owns workgroup b (views sorted by tile_base).
+__device__ __forceinline__ int find_view(const BatchView* views, int V, uint32_t b)
+{
+    int v = 0;
+    while (v + 1 < V && views[v + 1].tile_base <= b) ++v;
+    return v;
+}
+
+// The view-dependent fields of a BlendArgs; per-pixel pointers are offset by
+// the caller with pix_base.
+__device__ __forceinline__ BlendArgs bind_blend(const BlendArgs& common, const BatchView& vw, int v)
+{
+    BlendArgs a = common;
+    a.id_base = (uint32_t)v * common.id_stride;
+    a.ranges = common.ranges + vw.tile_base;
+    a.W = vw.W;
+    a.H = vw.H;
+    a.gx = vw.gx;
+    a.gy = vw.gy;
+    a.bg = vw.bg;
+    return a;
+}
+
+// Every tile of every view of a batch in one launch: workgroup b renders tile
+// b - tile_base of its view.  Per-pixel outputs are the views' [3,H,W] / [H,W]
+// blocks concatenated (view v starts at pixel pix_base).
+template <int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd_batch(BlendArgs common, const BatchView* views,
+                                                               int V, float* final_T,
+                                                               uint32_t* n_contrib, float* out_color,
+                                                               int* out_point_id,
+                                                               float* out_weight_pixel,
+                                                               float* point_weight)
+{
+    const int v = find_view(views, V, blockIdx.x);
+    const BatchView& vw = views[v];
+    const BlendArgs a = bind_blend(common, vw, v);
+    const size_t pb = vw.pix_base;
+    blend_fwd_tile<BATCH>(a, (int)(blockIdx.x - vw.tile_base), final_T + pb, n_contrib + pb,
+                          out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb, point_weight,
+                          nullptr, nullptr);
+}
+
+hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
+                                  float* final_T, uint32_t* n_contrib, float* out_color,
+                                  int* out_point_id, float* out_weight_pixel, float* point_weight,
+                                  hipStream_t s)
+{
+    if (T_total == 0) return hipSuccess;
+    if (common.batch == 128)
+        hipLaunchKernelGGL(k_blend_fwd_batch<128>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
+                           final_T, n_contrib, out_color, out_point_id, out_weight_pixel, point_weight);
+    else
+        hipLaunchKernelGGL(k_blend_fwd_batch<256>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
+                           final_T, n_contrib, out_color, out_point_id, out_weight_pixel, point_weight);
+    return hipGetLastError();
 }
 
 hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib,
@@ -347,17 +414,17 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 
 constexpr int kGradTerms = 9;   // dcolor x3, dmean2D x2, dconic x3, dopacity
 
+// Backward of one tile (one workgroup of FGS_BLOCK lanes).
 template <int BATCH>
-__global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* __restrict__ final_Ts,
-                                                         const uint32_t* __restrict__ n_contrib,
-                                                         const float* __restrict__ dL_dpix,
-                                                         float* __restrict__ dL_dmean2D,
-                                                         float* __restrict__ dL_dconic,
-                                                         float* __restrict__ dL_dopacity,
-                                                         float* __restrict__ dL_dcolors, int exact)
+__device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, const float* __restrict__ final_Ts,
+                                               const uint32_t* __restrict__ n_contrib,
+                                               const float* __restrict__ dL_dpix,
+                                               float* __restrict__ dL_dmean2D,
+                                               float* __restrict__ dL_dconic,
+                                               float* __restrict__ dL_dopacity,
+                                               float* __restrict__ dL_dcolors, int exact)
 {
     static_assert(BATCH <= FGS_BLOCK, "one staging lane per batch slot");
-    const int tile = blockIdx.x;
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int tid = threadIdx.x;
     const int wave = tid / FGS_WAVE;
@@ -505,6 +572,48 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const floa
             if (t[8] != 0.f) atomic_add_f32(&dL_dopacity[g], t[8]);
         }
     }
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* final_Ts,
+                                                         const uint32_t* n_contrib, const float* dL_dpix,
+                                                         float* dL_dmean2D, float* dL_dconic,
+                                                         float* dL_dopacity, float* dL_dcolors, int exact)
+{
+    blend_bwd_tile<BATCH>(a, blockIdx.x, final_Ts, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity,
+                          dL_dcolors, exact);
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd_batch(BlendArgs common, const BatchView* views,
+                                                               int V, const float* final_Ts,
+                                                               const uint32_t* n_contrib,
+                                                               const float* dL_dpix, float* dL_dmean2D,
+                                                               float* dL_dconic, float* dL_dopacity,
+                                                               float* dL_dcolors, int exact)
+{
+    const int v = find_view(views, V, blockIdx.x);
+    const BatchView& vw = views[v];
+    const BlendArgs a = bind_blend(common, vw, v);
+    const size_t pb = vw.pix_base;
+    blend_bwd_tile<BATCH>(a, (int)(blockIdx.x - vw.tile_base), final_Ts + pb, n_contrib + pb,
+                          dL_dpix + 3 * pb, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact);
+}
+
+hipError_t launch_blend_bwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
+                                  const float* final_T, const uint32_t* n_contrib, const float* dL_dpix,
+                                  float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
+                                  float* dL_dcolors, int exact, hipStream_t s)
+{
+    if (T_total == 0) return hipSuccess;
+#define FGS_BWD(BATCH)                                                                             \
+    hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V, \
+                       final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact)
+    if (common.bwd_batch == 256) FGS_BWD(256);
+    else if (common.bwd_batch == 64) FGS_BWD(64);
+    else FGS_BWD(128);
+#undef FGS_BWD
+    return hipGetLastError();
 }
 
 hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint32_t* n_contrib,

@@ -57,6 +57,35 @@ struct PreBwdOut {
     float* dL_drot;            // [P,4] or null
 };
 
+// One view of a batch, as the batched kernels read it from device memory.
+// Matrices are column-major float[16] like fgs_raster_settings'.
+struct BatchView {
+    float view[16];
+    float proj[16];
+    float campos[3];
+    float bg[3];
+    float tanx, tany, fx, fy;
+    int W, H, gx, gy;
+    uint32_t tile_base;   // first tile of the view in the batch's tile numbering
+    uint32_t pad;
+    uint64_t pix_base;    // first pixel of the view in the concatenated per-pixel arrays
+};
+
+__device__ inline void bind_view(PreArgs& a, const BatchView& v)
+{
+    a.view = v.view;
+    a.proj = v.proj;
+    a.campos = v.campos;
+    a.tanx = v.tanx;
+    a.tany = v.tany;
+    a.fx = v.fx;
+    a.fy = v.fy;
+    a.W = v.W;
+    a.H = v.H;
+    a.gx = v.gx;
+    a.gy = v.gy;
+}
+
 struct BlendArgs {
     const uint2* ranges;
     const uint64_t* pairs;     // sorted (tile << 32 | gaussian)
@@ -70,6 +99,9 @@ struct BlendArgs {
     // work shape: Gaussians staged in LDS per batch (forward 128 or 256,
     // backward 64, 128 or 256)
     int batch, bwd_batch;
+    // batched path: kv values are elements v * P + i; point ids report i
+    uint32_t id_stride;        // P of a batch, 0 for one view
+    uint32_t id_base;          // v * P (set per view)
 };
 
 // launchers (return hipError_t of the launch)
@@ -77,6 +109,11 @@ hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t 
 hipError_t launch_preprocess_bwd(const PreArgs& a, const PreBwdIn& in, const PreBwdOut& o,
                                  hipStream_t s);
 hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s);
+hipError_t launch_preprocess_fwd_batch(const PreArgs& a, const BatchView* views, int V,
+                                       const PreOut& o, hipStream_t s);
+hipError_t launch_preprocess_bwd_batch(const PreArgs& a, const BatchView* views, int V,
+                                       const PreBwdIn& in, const float* dL_dopacity_view,
+                                       const PreBwdOut& o, hipStream_t s);
 
 // scans / sort / binning
 size_t scan_partials_words(size_t n);
@@ -91,6 +128,20 @@ hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const u
                             int P, int gx, uint64_t* pairs, hipStream_t s);
 hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T, hipStream_t s);
 
+// batched binning (depth key (v << 58) | (depth_bits << 26) | i)
+constexpr int kBatchViewShift = 58;
+constexpr int kBatchIdxBits = 26;
+constexpr uint64_t kBatchIdxMask = (1ull << kBatchIdxBits) - 1;
+constexpr int kBatchMaxViews = 64;
+hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n, int P,
+                                       uint32_t* tiles_s, uint2* rects_s, hipStream_t s);
+hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n, uint32_t* partials,
+                                   unsigned long long* counters, hipStream_t s);
+hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
+                                  size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
+                                  const unsigned long long* K, hipStream_t s);
+hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, uint2* ranges, int T, hipStream_t s);
+
 // blend
 hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib,
                             float* out_color, int* out_point_id, float* out_weight_pixel,
@@ -98,6 +149,16 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint32_t* n_contrib,
                             const float* dL_dpix, float* dL_dmean2D, float* dL_dconic,
                             float* dL_dopacity, float* dL_dcolors, int exact, hipStream_t s);
+// batched: common.ranges / pairs / kv_strips / means2D / colors / conic_opacity
+// cover the whole batch; per-pixel arrays are the views' blocks concatenated
+hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
+                                  float* final_T, uint32_t* n_contrib, float* out_color,
+                                  int* out_point_id, float* out_weight_pixel, float* point_weight,
+                                  hipStream_t s);
+hipError_t launch_blend_bwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
+                                  const float* final_T, const uint32_t* n_contrib, const float* dL_dpix,
+                                  float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
+                                  float* dL_dcolors, int exact, hipStream_t s);
 
 // LoD cut and per-point utilities
 struct LodArgs {

@@ -159,131 +159,204 @@ __device__ __forceinline__ void view_dir(const float* campos, float px, float py
     d[2] = dor[2] / len;
 }
 
+// Forward preprocess of one Gaussian in one view.  (px, py, pz) is its mean,
+// c3 its 3D covariance (upper triangle) and opacity_in its stored opacity;
+// the view, the image and the OGFS settings come from a.
+struct PreRes {
+    int radius;
+    uint32_t tiles;
+    uint4 rec;                  // packed tile rectangle, tiles, 0
+    float2 mean2d;
+    float depth;
+    float4 con;
+    float col[3];
+};
+
+__device__ __forceinline__ void preprocess_view(const PreArgs& a, float px, float py, float pz,
+                                                const float* c3, float opacity_in, const ShLds& sh,
+                                                int* error_flag, PreRes& r)
+{
+    int radius_px = 0;
+    uint32_t tiles = 0;
+    float mx = 0.f, my = 0.f, depth = 0.f;
+    float4 con = make_float4(0.f, 0.f, 0.f, 0.f);
+    float col[3] = { 0.f, 0.f, 0.f };
+    int rc[4] = { 0, 0, 0, 0 };
+
+    float hom[4], view[3];
+    xf4(a.proj, px, py, pz, hom);
+    const float p_w = 1.0f / (hom[3] + 0.0000001f);
+    const float projx = hom[0] * p_w, projy = hom[1] * p_w;
+    xf3(a.view, px, py, pz, view);
+    bool alive = !(view[2] <= 0.2f);
+    if (!alive && a.prefiltered) atomicOr(error_flag, 1);
+    float ca = 0.f, cb = 0.f, cc = 0.f, det = 0.f;
+    if (alive) {
+        Cov2D cv;
+        cov2d(a.view, px, py, pz, a.fx, a.fy, a.tanx, a.tany, c3, cv);
+        ca = cv.a; cb = cv.b; cc = cv.c;
+        if (a.variant == 1) {
+            ca = (float)((double)ca + 0.3);
+            cc = (float)((double)cc + 0.3);
+        } else if (a.use_filter) {
+            ca = fmaxf(ca, 0.3f);
+            cc = fmaxf(cc, 0.3f);
+        }
+        det = ca * cc - cb * cb;
+        alive = det != 0.0f;
+    }
+    float opacity = 0.f;
+    if (alive) {
+        const float det_inv = 1.0f / det;
+        con.x = cc * det_inv;
+        con.y = -cb * det_inv;
+        con.z = ca * det_inv;
+        if (a.variant == 1) {
+            opacity = opacity_in;
+            const float mid = 0.5f * (ca + cc);
+            const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
+            const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
+            radius_px = sat_int(ceilf(3.0f * sqrtf(fmaxf(l1, l2))));
+            if (radius_px <= 0) alive = false;
+        } else {
+            opacity = fmaxf(opacity_in, 1e-6f);
+            const float log_opacity = logf_cr(opacity);
+            const float energy_clamped = fmaxf(log_opacity - a.log_lambda, a.energy_floor);
+            const float mid = 0.5f * (ca + cc);
+            const float disc = fmaxf(0.0f, mid * mid - det);
+            const float sqrt_disc = sqrtf(disc);
+            const float l1 = mid + sqrt_disc, l2 = mid - sqrt_disc;
+            const float max_lambda = fmaxf(fmaxf(l1, l2), 1e-6f);
+            const float radius_base = 3.0f * sqrtf(max_lambda);
+            float radius_major = sqrtf(2.0f * energy_clamped * max_lambda);
+            radius_major = fminf(radius_major, radius_base);
+            radius_major = fmaxf(radius_major, a.ratio_min * radius_base);
+            const int rp = sat_int(ceilf(radius_major + 1.0f));
+            radius_px = rp > 1 ? rp : 1;
+        }
+        con.w = opacity;
+    }
+    if (alive) {
+        mx = (float)((((double)projx + 1.0) * (double)a.W - 1.0) * 0.5);
+        my = (float)((((double)projy + 1.0) * (double)a.H - 1.0) * 0.5);
+        get_rect(mx, my, radius_px, a.gx, a.gy, rc);
+        tiles = (uint32_t)((rc[2] - rc[0]) * (rc[3] - rc[1]));
+        alive = tiles != 0;
+    }
+    if (alive) {
+        depth = view[2];
+        if (!a.colors_precomp) {
+            float d[3], dor[3];
+            view_dir(a.campos, px, py, pz, d, dor);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const float v = sh_eval(a.D, sh, ch, d[0], d[1], d[2]) + 0.5f;
+                col[ch] = v < 0.0f ? 0.0f : v;
+            }
+        }
+    } else {
+        radius_px = 0;
+        tiles = 0;
+        mx = my = 0.f;
+        con = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    r.radius = radius_px;
+    r.tiles = tiles;
+    r.rec = tiles ? make_uint4((uint32_t)rc[0] | ((uint32_t)rc[1] << 16),
+                               (uint32_t)rc[2] | ((uint32_t)rc[3] << 16), tiles, 0u)
+                  : make_uint4(0u, 0u, 0u, 0u);
+    r.mean2d = make_float2(mx, my);
+    r.depth = depth;
+    r.con = con;
+    r.col[0] = col[0];
+    r.col[1] = col[1];
+    r.col[2] = col[2];
+}
+
+// View-independent inputs of Gaussian i: mean and 3D covariance.
+__device__ __forceinline__ void gaussian_inputs(const PreArgs& a, int i, float* p, float* c3)
+{
+    p[0] = a.means3D[3 * i];
+    p[1] = a.means3D[3 * i + 1];
+    p[2] = a.means3D[3 * i + 2];
+    if (a.cov3D_precomp) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c3[k] = a.cov3D_precomp[6 * i + k];
+    } else {
+        cov3d(a.mod * a.scales[3 * i], a.mod * a.scales[3 * i + 1], a.mod * a.scales[3 * i + 2],
+              a.rotations[4 * i], a.rotations[4 * i + 1], a.rotations[4 * i + 2],
+              a.rotations[4 * i + 3], c3);
+    }
+}
+
+__device__ __forceinline__ void write_pre(const PreOut& o, size_t e, const PreRes& r, bool rgb)
+{
+    o.radii[e] = r.radius;
+    o.tiles[e] = r.tiles;
+    o.recs[e] = r.rec;
+    o.means2D[e] = r.mean2d;
+    o.depths[e] = r.depth;
+    o.conic_opacity[e] = r.con;
+    if (rgb) {
+        o.rgb[3 * e + 0] = r.col[0];
+        o.rgb[3 * e + 1] = r.col[1];
+        o.rgb[3 * e + 2] = r.col[2];
+    }
+}
+
 __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOut o)
 {
     __shared__ float s_sh[kMaxShFloats * kPitch];
     const int base = blockIdx.x * kPreThreads;
     const int i = base + threadIdx.x;
-    const bool in_range = i < a.P;
     const int nrows = min(kPreThreads, a.P - base);
     const int nu = (a.D + 1) * (a.D + 1) * 3;
     if (!a.colors_precomp) {
         stage_rows(a.shs + (size_t)base * a.M * 3, a.M * 3, nu, nrows, s_sh);
         __syncthreads();
     }
-    int radius_px = 0;
-    uint32_t tiles = 0;
-    float mx = 0.f, my = 0.f, depth = 0.f;
-    float4 con = make_float4(0.f, 0.f, 0.f, 0.f);
-    float col[3] = { 0.f, 0.f, 0.f };
-    bool write_rgb = false;
+    if (i >= a.P) return;
+    float p[3], c3[6];
+    gaussian_inputs(a, i, p, c3);
+    PreRes r;
+    preprocess_view(a, p[0], p[1], p[2], c3, a.opacities[i], ShLds{ s_sh, (int)threadIdx.x },
+                    o.error_flag, r);
+    write_pre(o, (size_t)i, r, !a.colors_precomp);
+    const uint32_t hi = r.tiles ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
+    o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
+}
 
-    if (in_range) {
-        const float px = a.means3D[3 * i], py = a.means3D[3 * i + 1], pz = a.means3D[3 * i + 2];
-        float hom[4], view[3];
-        xf4(a.proj, px, py, pz, hom);
-        const float p_w = 1.0f / (hom[3] + 0.0000001f);
-        const float projx = hom[0] * p_w, projy = hom[1] * p_w;
-        xf3(a.view, px, py, pz, view);
-        bool alive = !(view[2] <= 0.2f);
-        if (!alive && a.prefiltered) atomicOr(o.error_flag, 1);
-        float c3[6];
-        Cov2D cv;
-        float ca = 0.f, cb = 0.f, cc = 0.f, det = 0.f;
-        int rc[4] = { 0, 0, 0, 0 };
-        if (alive) {
-            if (a.cov3D_precomp) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) c3[k] = a.cov3D_precomp[6 * i + k];
-            } else {
-                cov3d(a.mod * a.scales[3 * i], a.mod * a.scales[3 * i + 1], a.mod * a.scales[3 * i + 2],
-                      a.rotations[4 * i], a.rotations[4 * i + 1], a.rotations[4 * i + 2],
-                      a.rotations[4 * i + 3], c3);
-            }
-            cov2d(a.view, px, py, pz, a.fx, a.fy, a.tanx, a.tany, c3, cv);
-            ca = cv.a; cb = cv.b; cc = cv.c;
-            if (a.variant == 1) {
-                ca = (float)((double)ca + 0.3);
-                cc = (float)((double)cc + 0.3);
-            } else if (a.use_filter) {
-                ca = fmaxf(ca, 0.3f);
-                cc = fmaxf(cc, 0.3f);
-            }
-            det = ca * cc - cb * cb;
-            alive = det != 0.0f;
-        }
-        float opacity = 0.f;
-        if (alive) {
-            const float det_inv = 1.0f / det;
-            con.x = cc * det_inv;
-            con.y = -cb * det_inv;
-            con.z = ca * det_inv;
-            if (a.variant == 1) {
-                opacity = a.opacities[i];
-                const float mid = 0.5f * (ca + cc);
-                const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
-                const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
-                radius_px = sat_int(ceilf(3.0f * sqrtf(fmaxf(l1, l2))));
-                if (radius_px <= 0) alive = false;
-            } else {
-                opacity = fmaxf(a.opacities[i], 1e-6f);
-                const float log_opacity = logf_cr(opacity);
-                const float energy_clamped = fmaxf(log_opacity - a.log_lambda, a.energy_floor);
-                const float mid = 0.5f * (ca + cc);
-                const float disc = fmaxf(0.0f, mid * mid - det);
-                const float sqrt_disc = sqrtf(disc);
-                const float l1 = mid + sqrt_disc, l2 = mid - sqrt_disc;
-                const float max_lambda = fmaxf(fmaxf(l1, l2), 1e-6f);
-                const float radius_base = 3.0f * sqrtf(max_lambda);
-                float radius_major = sqrtf(2.0f * energy_clamped * max_lambda);
-                radius_major = fminf(radius_major, radius_base);
-                radius_major = fmaxf(radius_major, a.ratio_min * radius_base);
-                const int rp = sat_int(ceilf(radius_major + 1.0f));
-                radius_px = rp > 1 ? rp : 1;
-            }
-            con.w = opacity;
-        }
-        if (alive) {
-            mx = (float)((((double)projx + 1.0) * (double)a.W - 1.0) * 0.5);
-            my = (float)((((double)projy + 1.0) * (double)a.H - 1.0) * 0.5);
-            get_rect(mx, my, radius_px, a.gx, a.gy, rc);
-            tiles = (uint32_t)((rc[2] - rc[0]) * (rc[3] - rc[1]));
-            alive = tiles != 0;
-        }
-        if (alive) {
-            depth = view[2];
-            if (!a.colors_precomp) {
-                float d[3], dor[3];
-                view_dir(a.campos, px, py, pz, d, dor);
-                const ShLds sh{ s_sh, (int)threadIdx.x };
-#pragma unroll
-                for (int ch = 0; ch < 3; ++ch) {
-                    const float v = sh_eval(a.D, sh, ch, d[0], d[1], d[2]) + 0.5f;
-                    col[ch] = v < 0.0f ? 0.0f : v;
-                }
-                write_rgb = true;
-            }
-        } else {
-            radius_px = 0;
-            tiles = 0;
-            mx = my = 0.f;
-            con = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        o.radii[i] = radius_px;
-        o.tiles[i] = tiles;
-        o.recs[i] = tiles ? make_uint4((uint32_t)rc[0] | ((uint32_t)rc[1] << 16),
-                                       (uint32_t)rc[2] | ((uint32_t)rc[3] << 16), tiles, 0u)
-                          : make_uint4(0u, 0u, 0u, 0u);
-        o.means2D[i] = make_float2(mx, my);
-        o.depths[i] = depth;
-        o.conic_opacity[i] = con;
-        if (write_rgb || !a.colors_precomp) {
-            o.rgb[3 * i + 0] = col[0];
-            o.rgb[3 * i + 1] = col[1];
-            o.rgb[3 * i + 2] = col[2];
-        }
-        const uint32_t hi = tiles ? __float_as_uint(depth) : 0xFFFFFFFFu;
-        o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
+// All views of a batch in one pass: the Gaussian's inputs (mean, covariance,
+// opacity, SH row) are read once and reused for every view.  Outputs of view
+// v go to element e = v * P + i of the [V * P] arrays; the depth key is
+// (v << 58) | (depth_bits << 26) | i, so one sort on bits 26..63 orders every
+// view by depth and keeps the views apart.
+__global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a, const BatchView* views,
+                                                                      int V, PreOut o)
+{
+    __shared__ float s_sh[kMaxShFloats * kPitch];
+    const int base = blockIdx.x * kPreThreads;
+    const int i = base + threadIdx.x;
+    const int nrows = min(kPreThreads, a.P - base);
+    const int nu = (a.D + 1) * (a.D + 1) * 3;
+    if (!a.colors_precomp) {
+        stage_rows(a.shs + (size_t)base * a.M * 3, a.M * 3, nu, nrows, s_sh);
+        __syncthreads();
+    }
+    if (i >= a.P) return;
+    float p[3], c3[6];
+    gaussian_inputs(a, i, p, c3);
+    const float opacity_in = a.opacities[i];
+    const ShLds sh{ s_sh, (int)threadIdx.x };
+    for (int v = 0; v < V; ++v) {
+        PreArgs av = a;
+        bind_view(av, views[v]);
+        PreRes r;
+        preprocess_view(av, p[0], p[1], p[2], c3, opacity_in, sh, o.error_flag, r);
+        const size_t e = (size_t)v * a.P + i;
+        write_pre(o, e, r, !a.colors_precomp);
+        const uint64_t hi = r.tiles ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
+        o.vkeys[e] = ((uint64_t)v << 58) | (hi << 26) | (uint32_t)i;
     }
 }
 
@@ -296,45 +369,16 @@ struct ShReg {
     __device__ float operator()(int f) const { return r[f]; }
 };
 
-// One Gaussian of the backward.  shr holds its SH row (first nu floats);
-// dsh receives dL/dsh at s_out[f * kPitch + t].
-__device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBwdIn& in,
-                                                   const PreBwdOut& o, int i, const float* shr,
-                                                   float* s_out, int t)
+// Backward of one Gaussian in one view, accumulated into dm (dL/dmean3D),
+// dcov (dL/dcov3D) and the Gaussian's dL/dsh column s_out[f * kPitch + t].
+// (mx, my, mz) is the mean, c3 the 3D covariance, shr the SH row (first nu
+// floats); the per-view gradients of the blend are read at element e.
+__device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreBwdIn& in, size_t e,
+                                                    float mx, float my, float mz, const float* c3,
+                                                    const float* shr, float* dm, float* dcov,
+                                                    float* s_out, int t)
 {
-    const bool alive = in.radii[i] > 0;
-    const int nu = (a.D + 1) * (a.D + 1) * 3;
-    if (!alive) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = 0.f;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) o.dL_dcov3D[6 * i + k] = 0.f;
-        if (a.shs)
-            for (int f = 0; f < nu; ++f) s_out[f * kPitch + t] = 0.f;
-        if (o.dL_dscales)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) o.dL_dscales[3 * i + k] = 0.f;
-        if (o.dL_drot)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) o.dL_drot[4 * i + k] = 0.f;
-        return;
-    }
     const bool exact = a.grad_mode == 1;
-    const float mx = a.means3D[3 * i], my = a.means3D[3 * i + 1], mz = a.means3D[3 * i + 2];
-    float c3[6];
-    float sv[3] = { 0.f, 0.f, 0.f };
-    float qr = 0.f, qx = 0.f, qy = 0.f, qz = 0.f;
-    if (a.cov3D_precomp) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) c3[k] = a.cov3D_precomp[6 * i + k];
-    } else {
-        sv[0] = a.mod * a.scales[3 * i];
-        sv[1] = a.mod * a.scales[3 * i + 1];
-        sv[2] = a.mod * a.scales[3 * i + 2];
-        qr = a.rotations[4 * i]; qx = a.rotations[4 * i + 1];
-        qy = a.rotations[4 * i + 2]; qz = a.rotations[4 * i + 3];
-        cov3d(sv[0], sv[1], sv[2], qr, qx, qy, qz, c3);
-    }
     Cov2D cv;
     cov2d(a.view, mx, my, mz, a.fx, a.fy, a.tanx, a.tany, c3, cv);
     const float x_grad_mul = (cv.txtz < -cv.limx || cv.txtz > cv.limx) ? 0.f : 1.f;
@@ -350,11 +394,10 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
         ca = fmaxf(ca, 0.3f);
         cc = fmaxf(cc, 0.3f);
     }
-    const float dcx = in.dL_dconic[3 * i], dcy = in.dL_dconic[3 * i + 1], dcz = in.dL_dconic[3 * i + 2];
+    const float dcx = in.dL_dconic[3 * e], dcy = in.dL_dconic[3 * e + 1], dcz = in.dL_dconic[3 * e + 2];
     const float denom = ca * cc - cb * cb;
     const float denom2inv = exact ? 1.0f / (denom * denom) : 1.0f / ((denom * denom) + 0.0000001f);
     float dL_da = 0.f, dL_db = 0.f, dL_dc = 0.f;
-    float dcov[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
     const float (*T)[3] = cv.T;
     if (denom2inv != 0.f) {
         dL_da = denom2inv * (-cc * cc * dcx + 2.f * cb * cc * dcy + (denom - ca * cc) * dcz);
@@ -362,12 +405,12 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
         dL_db = denom2inv * 2.f * (cb * cc * dcx - (denom + 2.f * cb * cb) * dcy + ca * cb * dcz);
         dL_da *= mask_a;
         dL_dc *= mask_c;
-        dcov[0] = T[0][0] * T[0][0] * dL_da + T[0][0] * T[1][0] * dL_db + T[1][0] * T[1][0] * dL_dc;
-        dcov[3] = T[0][1] * T[0][1] * dL_da + T[0][1] * T[1][1] * dL_db + T[1][1] * T[1][1] * dL_dc;
-        dcov[5] = T[0][2] * T[0][2] * dL_da + T[0][2] * T[1][2] * dL_db + T[1][2] * T[1][2] * dL_dc;
-        dcov[1] = 2.f * T[0][0] * T[0][1] * dL_da + (T[0][0] * T[1][1] + T[0][1] * T[1][0]) * dL_db + 2.f * T[1][0] * T[1][1] * dL_dc;
-        dcov[2] = 2.f * T[0][0] * T[0][2] * dL_da + (T[0][0] * T[1][2] + T[0][2] * T[1][0]) * dL_db + 2.f * T[1][0] * T[1][2] * dL_dc;
-        dcov[4] = 2.f * T[0][2] * T[0][1] * dL_da + (T[0][1] * T[1][2] + T[0][2] * T[1][1]) * dL_db + 2.f * T[1][1] * T[1][2] * dL_dc;
+        dcov[0] += T[0][0] * T[0][0] * dL_da + T[0][0] * T[1][0] * dL_db + T[1][0] * T[1][0] * dL_dc;
+        dcov[3] += T[0][1] * T[0][1] * dL_da + T[0][1] * T[1][1] * dL_db + T[1][1] * T[1][1] * dL_dc;
+        dcov[5] += T[0][2] * T[0][2] * dL_da + T[0][2] * T[1][2] * dL_db + T[1][2] * T[1][2] * dL_dc;
+        dcov[1] += 2.f * T[0][0] * T[0][1] * dL_da + (T[0][0] * T[1][1] + T[0][1] * T[1][0]) * dL_db + 2.f * T[1][0] * T[1][1] * dL_dc;
+        dcov[2] += 2.f * T[0][0] * T[0][2] * dL_da + (T[0][0] * T[1][2] + T[0][2] * T[1][0]) * dL_db + 2.f * T[1][0] * T[1][2] * dL_dc;
+        dcov[4] += 2.f * T[0][2] * T[0][1] * dL_da + (T[0][1] * T[1][2] + T[0][2] * T[1][1]) * dL_db + 2.f * T[1][1] * T[1][2] * dL_dc;
     }
     const float Vk[3][3] = { { c3[0], c3[1], c3[2] }, { c3[1], c3[3], c3[4] }, { c3[2], c3[4], c3[5] } };
     float dT[2][3];
@@ -390,10 +433,10 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
     const float dL_dtz = -a.fx * tz2 * dL_dJ00 - a.fy * tz2 * dL_dJ11 + (2.f * a.fx * cv.t[0]) * tz3 * dL_dJ02 +
                          (2.f * a.fy * cv.t[1]) * tz3 * dL_dJ12;
     const float* V = a.view;
-    float dm[3];
-    dm[0] = V[0] * dL_dtx + V[1] * dL_dty + V[2] * dL_dtz;
-    dm[1] = V[4] * dL_dtx + V[5] * dL_dty + V[6] * dL_dtz;
-    dm[2] = V[8] * dL_dtx + V[9] * dL_dty + V[10] * dL_dtz;
+    float dmv[3];
+    dmv[0] = V[0] * dL_dtx + V[1] * dL_dty + V[2] * dL_dtz;
+    dmv[1] = V[4] * dL_dtx + V[5] * dL_dty + V[6] * dL_dtz;
+    dmv[2] = V[8] * dL_dtx + V[9] * dL_dty + V[10] * dL_dtz;
 
     // projection of the mean (backward.cu:370-387)
     const float* Pm = a.proj;
@@ -402,10 +445,10 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
     const float m_w = 1.0f / (hom[3] + 0.0000001f);
     const float mul1 = hom[0] * m_w * m_w;
     const float mul2 = hom[1] * m_w * m_w;
-    const float g2x = in.dL_dmean2D[3 * i], g2y = in.dL_dmean2D[3 * i + 1];
-    dm[0] += (Pm[0] * m_w - Pm[3] * mul1) * g2x + (Pm[1] * m_w - Pm[3] * mul2) * g2y;
-    dm[1] += (Pm[4] * m_w - Pm[7] * mul1) * g2x + (Pm[5] * m_w - Pm[7] * mul2) * g2y;
-    dm[2] += (Pm[8] * m_w - Pm[11] * mul1) * g2x + (Pm[9] * m_w - Pm[11] * mul2) * g2y;
+    const float g2x = in.dL_dmean2D[3 * e], g2y = in.dL_dmean2D[3 * e + 1];
+    dmv[0] += (Pm[0] * m_w - Pm[3] * mul1) * g2x + (Pm[1] * m_w - Pm[3] * mul2) * g2y;
+    dmv[1] += (Pm[4] * m_w - Pm[7] * mul1) * g2x + (Pm[5] * m_w - Pm[7] * mul2) * g2y;
+    dmv[2] += (Pm[8] * m_w - Pm[11] * mul1) * g2x + (Pm[9] * m_w - Pm[11] * mul2) * g2y;
 
     // SH backward (backward.cu:20-139)
     if (a.shs) {
@@ -418,7 +461,7 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             const float v = sh_eval(deg, sh, ch, x, y, z) + 0.5f;
-            dRGB[ch] = v < 0.f ? 0.f : in.dL_dcolors[3 * i + ch];
+            dRGB[ch] = v < 0.f ? 0.f : in.dL_dcolors[3 * e + ch];
         }
         float ddx[3] = { 0.f, 0.f, 0.f }, ddy[3] = { 0.f, 0.f, 0.f }, ddz[3] = { 0.f, 0.f, 0.f };
 #pragma unroll
@@ -426,34 +469,34 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
 #define SHv(k) shr[(k) * 3 + ch]
 #define DSH(k) s_out[((k) * 3 + ch) * kPitch + t]
             const float g = dRGB[ch];
-            DSH(0) = kSH_C0 * g;
+            DSH(0) += kSH_C0 * g;
             if (deg > 0) {
                 const float C1 = kSH_C1;
-                DSH(1) = -C1 * y * g;
-                DSH(2) = C1 * z * g;
-                DSH(3) = -C1 * x * g;
+                DSH(1) += -C1 * y * g;
+                DSH(2) += C1 * z * g;
+                DSH(3) += -C1 * x * g;
                 ddx[ch] = -C1 * SHv(3);
                 ddy[ch] = -C1 * SHv(1);
                 ddz[ch] = C1 * SHv(2);
                 if (deg > 1) {
                     const float xx = x * x, yy = y * y, zz = z * z;
                     const float xy = x * y, yz = y * z, xz = x * z;
-                    DSH(4) = kSH_C2[0] * xy * g;
-                    DSH(5) = kSH_C2[1] * yz * g;
-                    DSH(6) = kSH_C2[2] * (2.f * zz - xx - yy) * g;
-                    DSH(7) = kSH_C2[3] * xz * g;
-                    DSH(8) = kSH_C2[4] * (xx - yy) * g;
+                    DSH(4) += kSH_C2[0] * xy * g;
+                    DSH(5) += kSH_C2[1] * yz * g;
+                    DSH(6) += kSH_C2[2] * (2.f * zz - xx - yy) * g;
+                    DSH(7) += kSH_C2[3] * xz * g;
+                    DSH(8) += kSH_C2[4] * (xx - yy) * g;
                     ddx[ch] += kSH_C2[0] * y * SHv(4) + kSH_C2[2] * 2.f * -x * SHv(6) + kSH_C2[3] * z * SHv(7) + kSH_C2[4] * 2.f * x * SHv(8);
                     ddy[ch] += kSH_C2[0] * x * SHv(4) + kSH_C2[1] * z * SHv(5) + kSH_C2[2] * 2.f * -y * SHv(6) + kSH_C2[4] * 2.f * -y * SHv(8);
                     ddz[ch] += kSH_C2[1] * y * SHv(5) + kSH_C2[2] * 2.f * 2.f * z * SHv(6) + kSH_C2[3] * x * SHv(7);
                     if (deg > 2) {
-                        DSH(9) = kSH_C3[0] * y * (3.f * xx - yy) * g;
-                        DSH(10) = kSH_C3[1] * xy * z * g;
-                        DSH(11) = kSH_C3[2] * y * (4.f * zz - xx - yy) * g;
-                        DSH(12) = kSH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy) * g;
-                        DSH(13) = kSH_C3[4] * x * (4.f * zz - xx - yy) * g;
-                        DSH(14) = kSH_C3[5] * z * (xx - yy) * g;
-                        DSH(15) = kSH_C3[6] * x * (xx - 3.f * yy) * g;
+                        DSH(9) += kSH_C3[0] * y * (3.f * xx - yy) * g;
+                        DSH(10) += kSH_C3[1] * xy * z * g;
+                        DSH(11) += kSH_C3[2] * y * (4.f * zz - xx - yy) * g;
+                        DSH(12) += kSH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy) * g;
+                        DSH(13) += kSH_C3[4] * x * (4.f * zz - xx - yy) * g;
+                        DSH(14) += kSH_C3[5] * z * (xx - yy) * g;
+                        DSH(15) += kSH_C3[6] * x * (xx - 3.f * yy) * g;
                         ddx[ch] += (kSH_C3[0] * SHv(9) * 3.f * 2.f * xy + kSH_C3[1] * SHv(10) * yz +
                                     kSH_C3[2] * SHv(11) * -2.f * xy + kSH_C3[3] * SHv(12) * -3.f * 2.f * xz +
                                     kSH_C3[4] * SHv(13) * (-3.f * xx + 4.f * zz - yy) +
@@ -476,17 +519,33 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
         const float ddir2 = ddz[0] * dRGB[0] + ddz[1] * dRGB[1] + ddz[2] * dRGB[2];
         const float sum2 = dor[0] * dor[0] + dor[1] * dor[1] + dor[2] * dor[2];
         const float invsum32 = 1.0f / sqrtf(sum2 * sum2 * sum2);
-        dm[0] += ((sum2 - dor[0] * dor[0]) * ddir0 - dor[1] * dor[0] * ddir1 - dor[2] * dor[0] * ddir2) * invsum32;
-        dm[1] += (-dor[0] * dor[1] * ddir0 + (sum2 - dor[1] * dor[1]) * ddir1 - dor[2] * dor[1] * ddir2) * invsum32;
-        dm[2] += (-dor[0] * dor[2] * ddir0 - dor[1] * dor[2] * ddir1 + (sum2 - dor[2] * dor[2]) * ddir2) * invsum32;
+        dmv[0] += ((sum2 - dor[0] * dor[0]) * ddir0 - dor[1] * dor[0] * ddir1 - dor[2] * dor[0] * ddir2) * invsum32;
+        dmv[1] += (-dor[0] * dor[1] * ddir0 + (sum2 - dor[1] * dor[1]) * ddir1 - dor[2] * dor[1] * ddir2) * invsum32;
+        dmv[2] += (-dor[0] * dor[2] * ddir0 - dor[1] * dor[2] * ddir1 + (sum2 - dor[2] * dor[2]) * ddir2) * invsum32;
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = dm[k];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) o.dL_dcov3D[6 * i + k] = dcov[k];
+    for (int k = 0; k < 3; ++k) dm[k] += dmv[k];
+}
 
-    // cov3D backward (backward.cu:278-341)
-    if (!a.cov3D_precomp && o.dL_dscales) {
+// Writes dL/dmean3D and dL/dcov3D of Gaussian i and backpropagates dcov to
+// scale and rotation (backward.cu:278-341: dL/dscale is not multiplied by the
+// modifier in reference mode; the quaternion is used unnormalised).
+// A Gaussian rendered in no view gets zeros (alive = false).
+__device__ __forceinline__ void preprocess_bwd_finish(const PreArgs& a, const PreBwdOut& o, int i,
+                                                      bool alive, const float* dm, const float* dcov)
+{
+    const bool exact = a.grad_mode == 1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = alive ? dm[k] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o.dL_dcov3D[6 * i + k] = alive ? dcov[k] : 0.f;
+    if (alive && !a.cov3D_precomp && o.dL_dscales) {
+        float sv[3];
+        sv[0] = a.mod * a.scales[3 * i];
+        sv[1] = a.mod * a.scales[3 * i + 1];
+        sv[2] = a.mod * a.scales[3 * i + 2];
+        const float qr = a.rotations[4 * i], qx = a.rotations[4 * i + 1];
+        const float qy = a.rotations[4 * i + 2], qz = a.rotations[4 * i + 3];
         float R[3][3];
         quat_rot(qr, qx, qy, qz, R);
         float Mm[3][3];
@@ -528,7 +587,42 @@ __device__ __forceinline__ void preprocess_bwd_one(const PreArgs& a, const PreBw
         if (o.dL_drot)
             for (int k = 0; k < 4; ++k) o.dL_drot[4 * i + k] = 0.f;
     }
-    if (exact && a.variant == 0 && a.opacities[i] < 1e-6f) o.dL_dopacity[i] = 0.f;
+}
+
+// SH rows of the block into registers (shr) and the LDS buffer cleared for
+// the dL/dsh accumulation.
+__device__ __forceinline__ void bwd_stage_sh(const PreArgs& a, int base, int nrows, int nu, float* shr,
+                                             float* s_sh, int t)
+{
+    const int rs = a.M * 3;
+#pragma unroll
+    for (int f = 0; f < kMaxShFloats; ++f) shr[f] = 0.f;
+    stage_rows(a.shs + (size_t)base * rs, rs, nu, nrows, s_sh);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < kMaxShFloats; ++f)
+        if (f < nu) shr[f] = s_sh[f * kPitch + t];
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < kMaxShFloats; ++f)
+        if (f < nu) s_sh[f * kPitch + t] = 0.f;   // the buffer now collects dL/dsh
+}
+
+__device__ __forceinline__ void bwd_store_sh(const PreArgs& a, const PreBwdOut& o, int base, int nrows,
+                                             int nu, const float* s_sh, int t)
+{
+    const int rs = a.M * 3;
+    __syncthreads();
+    float* dst = o.dL_dsh + (size_t)base * rs;
+    unstage_rows(s_sh, rs, nu, nrows, dst);
+    if (nu < rs) {
+        // coefficients above the active degree get zero gradient
+        const int tail = rs - nu;
+        for (int k = t; k < nrows * tail; k += kPreThreads) {
+            const int r = k / tail;
+            dst[(size_t)r * rs + nu + (k - r * tail)] = 0.f;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd(PreArgs a, PreBwdIn in, PreBwdOut o)
@@ -540,32 +634,60 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd(PreArgs a, PreBw
     const int nrows = min(kPreThreads, a.P - base);
     const bool has_sh = a.shs != nullptr;
     const int nu = (a.D + 1) * (a.D + 1) * 3;
-    const int rs = a.M * 3;
     float shr[kMaxShFloats];
-#pragma unroll
-    for (int f = 0; f < kMaxShFloats; ++f) shr[f] = 0.f;
-    if (has_sh) {
-        stage_rows(a.shs + (size_t)base * rs, rs, nu, nrows, s_sh);
-        __syncthreads();
-#pragma unroll
-        for (int f = 0; f < kMaxShFloats; ++f)
-            if (f < nu) shr[f] = s_sh[f * kPitch + t];
-        __syncthreads();   // the buffer now collects dL/dsh
-    }
-    if (i < a.P) preprocess_bwd_one(a, in, o, i, shr, s_sh, t);
-    if (has_sh) {
-        __syncthreads();
-        float* dst = o.dL_dsh + (size_t)base * rs;
-        unstage_rows(s_sh, rs, nu, nrows, dst);
-        if (nu < rs) {
-            // coefficients above the active degree get zero gradient
-            const int tail = rs - nu;
-            for (int k = t; k < nrows * tail; k += kPreThreads) {
-                const int r = k / tail;
-                dst[(size_t)r * rs + nu + (k - r * tail)] = 0.f;
-            }
+    if (has_sh) bwd_stage_sh(a, base, nrows, nu, shr, s_sh, t);
+    if (i < a.P) {
+        float dm[3] = { 0.f, 0.f, 0.f }, dcov[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+        const bool alive = in.radii[i] > 0;
+        if (alive) {
+            float m[3], c3[6];
+            gaussian_inputs(a, i, m, c3);
+            preprocess_bwd_view(a, in, (size_t)i, m[0], m[1], m[2], c3, shr, dm, dcov, s_sh, t);
         }
+        preprocess_bwd_finish(a, o, i, alive, dm, dcov);
+        if (alive && a.grad_mode == 1 && a.variant == 0 && a.opacities[i] < 1e-6f) o.dL_dopacity[i] = 0.f;
     }
+    if (has_sh) bwd_store_sh(a, o, base, nrows, nu, s_sh, t);
+}
+
+// All views of a batch: per-view blend gradients at e = v * P + i are folded
+// into one set of parameter gradients (dL/dcov3D is summed over the views
+// before the scale / rotation backward, which is linear in it).  The
+// per-view dL/dopacity of the blend is summed into o.dL_dopacity.
+__global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd_batch(PreArgs a, const BatchView* views,
+                                                                      int V, PreBwdIn in,
+                                                                      const float* dL_dopacity_view,
+                                                                      PreBwdOut o)
+{
+    __shared__ float s_sh[kMaxShFloats * kPitch];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * kPreThreads;
+    const int i = base + t;
+    const int nrows = min(kPreThreads, a.P - base);
+    const bool has_sh = a.shs != nullptr;
+    const int nu = (a.D + 1) * (a.D + 1) * 3;
+    float shr[kMaxShFloats];
+    if (has_sh) bwd_stage_sh(a, base, nrows, nu, shr, s_sh, t);
+    if (i < a.P) {
+        float dm[3] = { 0.f, 0.f, 0.f }, dcov[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+        float m[3], c3[6];
+        gaussian_inputs(a, i, m, c3);
+        float dop = 0.f;
+        bool any = false;
+        for (int v = 0; v < V; ++v) {
+            const size_t e = (size_t)v * a.P + i;
+            if (in.radii[e] <= 0) continue;
+            any = true;
+            PreArgs av = a;
+            bind_view(av, views[v]);
+            preprocess_bwd_view(av, in, e, m[0], m[1], m[2], c3, shr, dm, dcov, s_sh, t);
+            dop += dL_dopacity_view[e];
+        }
+        preprocess_bwd_finish(a, o, i, any, dm, dcov);
+        if (any && a.grad_mode == 1 && a.variant == 0 && a.opacities[i] < 1e-6f) dop = 0.f;
+        o.dL_dopacity[i] = dop;
+    }
+    if (has_sh) bwd_store_sh(a, o, base, nrows, nu, s_sh, t);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
@@ -587,6 +709,25 @@ hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t 
 {
     if (a.P == 0) return hipSuccess;
     hipLaunchKernelGGL(k_preprocess_fwd, dim3((a.P + kPreThreads - 1) / kPreThreads), dim3(kPreThreads), 0, s, a, o);
+    return hipGetLastError();
+}
+
+hipError_t launch_preprocess_fwd_batch(const PreArgs& a, const BatchView* views, int V,
+                                       const PreOut& o, hipStream_t s)
+{
+    if (a.P == 0 || V == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_fwd_batch, dim3((a.P + kPreThreads - 1) / kPreThreads),
+                       dim3(kPreThreads), 0, s, a, views, V, o);
+    return hipGetLastError();
+}
+
+hipError_t launch_preprocess_bwd_batch(const PreArgs& a, const BatchView* views, int V,
+                                       const PreBwdIn& in, const float* dL_dopacity_view,
+                                       const PreBwdOut& o, hipStream_t s)
+{
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_bwd_batch, dim3((a.P + kPreThreads - 1) / kPreThreads),
+                       dim3(kPreThreads), 0, s, a, views, V, in, dL_dopacity_view, o);
     return hipGetLastError();
 }
 

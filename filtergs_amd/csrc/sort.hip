@@ -322,4 +322,109 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Batched binning.  Depth keys of a batch are (v << 58) | (depth_bits << 26) | i
+// (element e = v * P + i); kv pairs carry the batch-wide tile number
+// tile_base[v] + y * gx + x and the element e.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t batch_elem(uint64_t key, int P)
+{
+    return (uint32_t)(key >> kBatchViewShift) * (uint32_t)P + (uint32_t)(key & kBatchIdxMask);
+}
+
+__global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __restrict__ vkeys,
+                                                              const uint4* __restrict__ recs, size_t n,
+                                                              int P, uint32_t* __restrict__ tiles_s,
+                                                              uint2* __restrict__ rects_s)
+{
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint4 r = recs[batch_elem(vkeys[s], P)];
+    tiles_s[s] = r.z;
+    rects_s[s] = make_uint2(r.x, r.y);
+}
+
+hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n, int P,
+                                       uint32_t* tiles_s, uint2* rects_s, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sorted_records_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys,
+                       recs, n, P, tiles_s, rects_s);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n, uint32_t* partials,
+                                   unsigned long long* counters, hipStream_t s)
+{
+    return exclusive_scan(PlainLoader{ tiles_s }, n, offs, partials, s, counters + 1, counters);
+}
+
+// Pairs past the capacity are dropped (the caller compares K with it).
+__global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restrict__ vkeys,
+                                                         const uint32_t* __restrict__ offs,
+                                                         const uint2* __restrict__ rects_s, size_t n,
+                                                         int P, const BatchView* __restrict__ views,
+                                                         uint64_t* __restrict__ pairs, size_t cap)
+{
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint2 rc = rects_s[s];
+    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
+    if (x1 <= x0 || y1 <= y0) return;
+    const uint64_t key = vkeys[s];
+    const uint32_t v = (uint32_t)(key >> kBatchViewShift);
+    const uint32_t e = batch_elem(key, P);
+    const uint32_t gx = (uint32_t)views[v].gx, tb = views[v].tile_base;
+    size_t off = offs[s];
+    for (uint32_t y = y0; y < y1; ++y)
+        for (uint32_t x = x0; x < x1; ++x, ++off)
+            if (off < cap) pairs[off] = ((uint64_t)(tb + y * gx + x) << 32) | e;
+}
+
+// Slots [K, cap) get all-ones keys, which sort after every real tile.
+__global__ __launch_bounds__(256) void k_fill_tail(uint64_t* __restrict__ pairs,
+                                                   const unsigned long long* __restrict__ K, size_t cap)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t k0 = (size_t)*K;
+    for (size_t k = k0 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < cap; k += stride)
+        pairs[k] = ~0ull;
+}
+
+hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
+                                  size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
+                                  const unsigned long long* K, hipStream_t s)
+{
+    if (cap == 0) return hipSuccess;
+    size_t blocks = div_up(cap, 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill_tail, dim3((unsigned)blocks), dim3(256), 0, s, pairs, K, cap);
+    if (n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys, offs,
+                       rects_s, n, P, views, pairs, cap);
+    return hipGetLastError();
+}
+
+// identifyTileRanges over the batch's tiles; keys of tile >= T (the tail) are
+// skipped.
+__global__ __launch_bounds__(256) void k_ranges_bounded(const uint64_t* __restrict__ sorted, size_t n,
+                                                        uint32_t T, uint2* __restrict__ ranges)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = (uint32_t)(sorted[k] >> 32);
+    if (t >= T) return;
+    if (k == 0 || (uint32_t)(sorted[k - 1] >> 32) != t) ranges[t].x = (uint32_t)k;
+    if (k == n - 1 || (uint32_t)(sorted[k + 1] >> 32) != t) ranges[t].y = (uint32_t)(k + 1);
+}
+
+hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, uint2* ranges, int T, hipStream_t s)
+{
+    hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
+    if (e != hipSuccess || cap == 0) return e;
+    hipLaunchKernelGGL(k_ranges_bounded, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted, cap,
+                       (uint32_t)T, ranges);
+    return hipGetLastError();
+}
+
 }  // namespace fgs

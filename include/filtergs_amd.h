@@ -140,6 +140,77 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
                  float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh, float* dL_dscales,
                  float* dL_drot, void* ws, void* stream);
 
+/* ---- batched multi-view path --------------------------------------------
+ * No single counterpart in the reference: it replaces V sequential
+ * _C.rasterize_gaussians / _C.rasterize_gaussians_backward calls
+ * (rasterize_points.cu:36-260) over the same Gaussians, e.g. the per-camera
+ * loop of a training step.  One launch per stage covers every view:
+ *   - preprocess reads each Gaussian once and projects it into all views;
+ *   - one depth sort orders all views ((view, depth, index) keys);
+ *   - duplicate / tile sort / ranges run over the batch's tile numbering
+ *     (view v owns tiles [tile_base_v, tile_base_v + T_v));
+ *   - one blend launch renders every tile of every view;
+ *   - the preprocess backward folds the per-view gradients into one set of
+ *     parameter gradients.
+ * Per view the results equal those of the plain path: identical images,
+ * radii, n_contrib and kv order; gradients up to float summation order.
+ * The kv-pair buffer has a caller-chosen capacity K_cap, so a step issues no
+ * host read and can be captured in a graph; counts[1] reports the true K,
+ * and a step with K > K_cap must be redone with a larger buffer. */
+
+/* One camera of a batch (host memory).  Matrices are column-major as in
+ * fgs_raster_settings; campos and bg by value. */
+typedef struct {
+    int32_t image_height;
+    int32_t image_width;
+    float tanfovx;
+    float tanfovy;
+    float viewmatrix[16];
+    float projmatrix[16];
+    float campos[3];
+    float bg[3];
+} fgs_batch_view;
+
+typedef struct {
+    int32_t views;
+    int32_t P;
+    int64_t tiles;            /* tiles over all views                        */
+    int64_t pixels;           /* pixels over all views                       */
+    size_t plan_bytes;        /* device bytes of the plan (per-view records) */
+    size_t geom_bytes;        /* device bytes of the batch geometry buffer   */
+    size_t image_bytes;       /* device bytes of the batch image buffer      */
+    size_t backward_ws_bytes; /* device bytes of the backward workspace      */
+} fgs_batch_sizes;
+
+/* Sizes for V views of P Gaussians (V <= 64, P <= 2^26, V*P < 2^31). */
+int fgs_batch_sizes_query(const fgs_batch_view* views, int32_t V, int32_t P, fgs_batch_sizes* out);
+size_t fgs_batch_binning_bytes(int64_t K_cap);
+/* Uploads the per-view records into plan (plan_bytes, device).  Synchronises
+ * the stream: call it when the cameras change, outside graph capture. */
+int fgs_batch_plan(const fgs_batch_view* views, int32_t V, void* plan, void* stream);
+/* Forward of every view.  s supplies the settings shared by the views
+ * (scale_modifier, sh_degree, prefiltered, debug, variant, use_filter, OGFS
+ * fields, grad_mode); its image size, tan-fov, matrices, bg and campos are
+ * ignored.  g->colors_precomp, when given, is per view: [V,P,3].
+ * Outputs, views concatenated in plan order: radii [V,P]; out_color the
+ * views' [3,H,W] blocks; out_point_id / out_weight_pixel the views' [H,W]
+ * blocks; out_point_weight [V,P] (zero on entry).  counts[4] (device u64):
+ * visible (Gaussian, view) pairs, K, error bits, 0. */
+int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, const fgs_batch_sizes* sz,
+                      const void* plan, void* geom, void* image, void* binning, int64_t K_cap,
+                      int32_t* radii, float* out_color, int32_t* out_point_id, float* out_weight_pixel,
+                      float* out_point_weight, uint64_t* counts, void* stream);
+/* Backward of every view.  dL_dpix: the views' [3,H,W] blocks.  Per view
+ * (zero on entry, accumulated): dL_dmeans2D [V,P,3], dL_dcolors [V,P,3].
+ * Summed over views (fully written): dL_dopacity [P], dL_dmeans3D [P,3],
+ * dL_dcov3D [P,6], dL_dsh [P,M,3], dL_dscales [P,3], dL_drot [P,4].
+ * ws: backward_ws_bytes. */
+int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const fgs_batch_sizes* sz,
+                       const void* plan, const int32_t* radii, const void* geom, const void* image,
+                       const void* binning, int64_t K_cap, const float* dL_dpix, float* dL_dmeans2D,
+                       float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
+                       float* dL_dsh, float* dL_dscales, float* dL_drot, void* ws, void* stream);
+
 /* ---- per-point utilities ------------------------------------------------ */
 /* _C.mark_visible / markVisible (rasterize_points.cu:299-318,
  * rasterizer_impl.cu:97-109): present[i] = z_view > 0.2. */
