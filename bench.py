@@ -1,15 +1,21 @@
-"""Benchmark: fwd+bwd of the rasterizer for 3M Gaussians at 1920x1080 on one MI355X.
+"""Benchmark: fwd+bwd of the rasterizer for 3M Gaussians on one MI355X,
+batched multi-view vs per-view.
 
-One step = one training-style pass of one view: forward (preprocess, depth
-sort, duplicate, tile sort, ranges, blend) + backward (blend backward,
-preprocess backward) through the C ABI, for a fixed seeded synthetic scene
-(3M Gaussians in a 40^3 box, SH degree 3, wodilate + OGFS defaults) and a
+One step = one training-style pass over a batch of 16 cameras (orbit poses at
+radius 8-30 around a seeded 3M-Gaussian scene in a 40^3 box, resolutions drawn
+from 960x540 / 1920x1080 / 2560x1440 / 3840x2160, SH degree 3, wodilate + OGFS
+defaults): forward (preprocess, depth sort, duplicate, tile sort, ranges,
+blend) and backward (blend backward, preprocess backward) of every view, with
 random dL/dimage.  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line (value = Mpix/s of fwd+bwd, higher is better) with a
-roofline entry for the dominant kernel (HIP events recorded on the launch
-stream around every kernel of the timed steps) and a CPU baseline (the oracle
-on a bounded sample of the same view, timed on this host).
+``value`` is the batched path (one launch per stage for all views, the step
+replayed from a HIP graph, work shape picked by the autotuner) in Mpix/s.
+Beside it the line reports the same 16 views rendered by 16 sequential plain
+per-view calls, and the plain per-view path on one 1920x1080 view of the same
+scene with its per-stage times.  The roofline entry covers the dominant
+kernel of the batched step (HIP events on the launch stream around every
+stage of an eager batched step); the CPU baseline is the oracle on a bounded
+sample of the 1080p view, timed on this host.
 """
 import argparse
 import json
@@ -23,17 +29,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+HBM_STAGES = ("preprocess", "depth_sort", "tile_offsets", "duplicate", "tile_sort", "ranges",
+              "preprocess_bwd")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--gaussians", type=int, default=3_000_000)
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--views", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=20, help="steps of the plain 1080p leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-band", type=int, default=256, help="rows of the CPU baseline sample")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB (a "
@@ -41,23 +50,28 @@ def parse():
     return ap.parse_args()
 
 
-def stage_bytes(P, Pv, K, T, npix, M, passes_tile):
-    """Algorithmic HBM bytes per launch of each stage (per view)."""
+def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
+    """Algorithmic HBM bytes per launch of each stage.  P Gaussians, V views,
+    Pv visible (Gaussian, view) pairs, K kv pairs, T tiles and npix pixels over
+    all views.  The batched preprocess reads the Gaussians once for all views
+    and the batched depth sort makes 5 passes over V*P keys."""
+    params = P * (44 + 12 * M)
+    grads_out = P * (92 + 12 * M)
     return {
-        "preprocess": P * (44 + 12 * M) + P * 8 + Pv * 40,
-        "depth_sort": 4 * 24 * P,
-        "tile_offsets": 8 * P,
+        "preprocess": params + V * P * 8 + Pv * 40,
+        "depth_sort": (4 if V == 1 else 5) * 24 * V * P,
+        "tile_offsets": 8 * V * P,
         "duplicate": 20 * Pv + 8 * K,
         "tile_sort": passes_tile * 24 * K,
         "ranges": 8 * K + 8 * T,
-        "blend_fwd": 40 * K + 28 * npix + 4 * P,
+        "blend_fwd": 40 * K + 28 * npix + 4 * V * P,
         "blend_bwd": 40 * K + 20 * npix + 36 * Pv,
-        "preprocess_bwd": Pv * (44 + 12 * M + 36) + P * (92 + 12 * M),
+        "preprocess_bwd": (Pv * (44 + 12 * M + 36) if V == 1 else params + 36 * Pv) + grads_out,
     }
 
 
 def cpu_baseline(g, cam, dL, band):
-    """Oracle (FP32 emulation) fwd+bwd on a horizontal band of the same view:
+    """Oracle (FP32 emulation) fwd+bwd on a horizontal band of the 1080p view:
     all Gaussians are preprocessed, pixels of the central `band` rows are
     blended forward and backward.  Mpix/s = band pixels / wall time."""
     from filtergs_amd.camera import Camera
@@ -65,7 +79,6 @@ def cpu_baseline(g, cam, dL, band):
     y0 = (cam.height - band) // 2
     sub = Camera(cam.width, band, cam.fx, cam.fy, cam.cx, cam.cy - y0, cam.R, cam.T, cam.znear,
                  cam.zfar)
-    # same rays: the band keeps fy and shifts the principal point; tanfovy follows H
     s = orc.settings(band, cam.width, sub.tanfovx, sub.tanfovy, sub.world_view_transform,
                      sub.full_proj_transform, sub.center, sh_degree=3)
     cores = min(os.cpu_count() or 1, 16)
@@ -77,7 +90,7 @@ def cpu_baseline(g, cam, dL, band):
     dt = time.perf_counter() - t0
     return {"value": round(cam.width * band / dt / 1e6, 4), "unit": "Mpix/s", "cores": cores,
             "kind": "port",
-            "sample": f"oracle FP32 fwd+bwd of rows {y0}..{y0 + band} of the {cam.width}x{cam.height} "
+            "sample": f"oracle FP32 fwd+bwd of rows {y0}..{y0 + band} of one {cam.width}x{cam.height} "
                       f"view, all {g['means3D'].shape[0]} Gaussians preprocessed "
                       f"(preprocess/blend-forward on {cores} threads, backward 1 thread), {dt:.1f} s"}
 
@@ -90,11 +103,9 @@ def main():
     import torch
 
     from filtergs_amd import _lib, ext, workloads
-    W, H, P = args.width, args.height, args.gaussians
+    from filtergs_amd.batch import BatchRasterizer
+    P = args.gaussians
     g = workloads.gaussians(P, 20.0, 3, seed=0)
-    cam = workloads.camera_c1(W, H)
-    rng = np.random.default_rng(1)
-    dL = rng.normal(size=(3, H, W)).astype(np.float32)
     dev = "cuda"
 
     def t(a, dt=torch.float32):
@@ -102,35 +113,110 @@ def main():
 
     means, scales, rots = t(g["means3D"]), t(g["scales"]), t(g["rotations"])
     opac, shs = t(g["opacities"].reshape(-1, 1)), t(g["shs"])
-    vm, pm, cp = t(cam.world_view_transform), t(cam.full_proj_transform), t(cam.center)
-    bg = t([0.0, 0.0, 0.0])
-    dLt = t(dL)
     empty = torch.Tensor([])
+    M = 16
 
-    def step():
+    def plain_step(cam, cams_t, dLt, bg):
+        vm, pm, cp = cams_t
         out = ext.rasterize_gaussians(bg, means, empty, opac, scales, rots, 1.0, empty, vm, pm,
-                                      cam.tanfovx, cam.tanfovy, H, W, shs, 3, cp, False, True,
-                                      False, ogfs_enable_stats=False)
+                                      cam.tanfovx, cam.tanfovy, cam.height, cam.width, shs, 3, cp,
+                                      False, True, False, ogfs_enable_stats=False)
         K, color, pid, wpix, pw, radii, geom, binning, img, _ = out
-        ext.rasterize_gaussians_backward(bg, means, radii, empty, scales, rots, 1.0, empty, vm,
-                                         pm, cam.tanfovx, cam.tanfovy, dLt, shs, 3, cp, geom, K,
+        ext.rasterize_gaussians_backward(bg, means, radii, empty, scales, rots, 1.0, empty, vm, pm,
+                                         cam.tanfovx, cam.tanfovy, dLt, shs, 3, cp, geom, K,
                                          binning, img, False)
         return K, radii
 
-    for _ in range(args.warmup):
-        K, radii = step()
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    def stages_of(prof, nbytes):
+        out = {}
+        for name, (tot, calls) in prof.items():
+            if calls:
+                avg = tot / calls
+                out[name] = {"ms": round(avg, 4), "GB/s": round(nbytes[name] / avg / 1e6, 1)}
+        return out
+
+    def hbm_frac(st, nbytes):
+        b = sum(nbytes[k] for k in HBM_STAGES if k in st)
+        ms = sum(st[k]["ms"] for k in HBM_STAGES if k in st)
+        return {"GB/s": round(b / ms / 1e6, 1), "frac": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+
+    bg0 = t([0.0, 0.0, 0.0])
+
+    # ---- plain per-view path, one 1920x1080 view (stage breakdown) --------
+    cam1 = workloads.camera_c1(1920, 1080)
+    ct1 = (t(cam1.world_view_transform), t(cam1.full_proj_transform), t(cam1.center))
+    dL1 = np.random.default_rng(1).normal(size=(3, 1080, 1920)).astype(np.float32)
+    dL1t = t(dL1)
+    for _ in range(max(1, args.warmup)):
+        K1, radii1 = plain_step(cam1, ct1, dL1t, bg0)
     torch.cuda.synchronize()
-    Pv = int((radii > 0).sum().item())
+    Pv1 = int((radii1 > 0).sum().item())
     _lib.profile_enable(True)
     _lib.profile_read(reset=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        K, radii = step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = _lib.profile_read(reset=True)
+    ms1 = timed(lambda: plain_step(cam1, ct1, dL1t, bg0), args.plain_steps) * 1e3
+    prof1 = _lib.profile_read(reset=True)
     _lib.profile_enable(False)
+    T1 = 120 * 68
+    nb1 = stage_bytes(P, Pv1, K1, T1, 1920 * 1080, M, (T1.bit_length() + 7) // 8)
+    st1 = stages_of(prof1, nb1)
+
+    # ---- the views: 16 sequential plain calls ------------------------------
+    cams = workloads.orbit_cameras(args.views, seed=3, radius_range=(8.0, 30.0))
+    npix = sum(c.width * c.height for c in cams)
+    rng = np.random.default_rng(2)
+    dLs = [t(rng.normal(size=(3, c.height, c.width)).astype(np.float32)) for c in cams]
+    cts = [(t(c.world_view_transform), t(c.full_proj_transform), t(c.center)) for c in cams]
+
+    def seq():
+        for c, ct, d in zip(cams, cts, dLs):
+            plain_step(c, ct, d, bg0)
+
+    seq()
+    ms_seq = timed(seq, max(2, args.steps // 2)) * 1e3
+
+    # ---- batched: autotune, eager stage profile, graph replay --------------
+    br = BatchRasterizer(cams, P, bg=(0.0, 0.0, 0.0))
+    br.bind(means, opac.reshape(-1), scales, rots, shs=shs)
+    K = br.forward()
+    dL_flat = torch.cat([d.reshape(-1) for d in dLs])
+    grads = br.backward(dL_flat)
+    tuned = None if args.no_autotune else br.autotune(dL_flat)
+    Pv = int((br.radii > 0).sum().item())
+
+    def eager():
+        br._forward_launch()
+        br._backward_launch(dL_flat, grads)
+
+    _lib.profile_enable(True)
+    _lib.profile_read(reset=True)
+    ms_eager = timed(eager, 3) * 1e3
+    profb = _lib.profile_read(reset=True)
+    _lib.profile_enable(False)
+    graph, _ = br.capture(dL_flat, out=grads)
+    for _ in range(args.warmup):
+        graph.replay()
+    ms_graph = timed(graph.replay, args.steps) * 1e3
+    if int(br.counts[1].item()) > br.K_cap:
+        raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
+    T = int(br.sizes.tiles)
+    nbb = stage_bytes(P, Pv, K, T, npix, M, (T.bit_length() + 7) // 8, V=args.views)
+    stb = stages_of(profb, nbb)
+    dominant = max(stb, key=lambda k: stb[k]["ms"])
+    achieved = nbb[dominant] / stb[dominant]["ms"] / 1e6
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device=dev)
@@ -140,48 +226,41 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    ms = elapsed * 1000.0 / args.steps
-    npix = W * H
-    T = ((W + 15) // 16) * ((H + 15) // 16)
-    passes_tile = (T.bit_length() + 7) // 8
-    nbytes = stage_bytes(P, Pv, K, T, npix, 16, passes_tile)
-    stages = {}
-    for name, (tot, calls) in prof.items():
-        if calls:
-            avg = tot / calls
-            stages[name] = {"ms": round(avg, 4), "GB/s": round(nbytes[name] / avg / 1e6, 1)}
-    dominant = max(stages, key=lambda k: stages[k]["ms"])
-    avg_ms = stages[dominant]["ms"]
-    achieved = nbytes[dominant] / avg_ms / 1e6
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+    res = sorted({f"{c.width}x{c.height}" for c in cams})
     line = {
         "metric": "fwd+bwd ms/view & Mpix/s, 3M Gaussians @1080p; batched vs per-view; % HBM peak",
-        "value": round(npix / ms / 1000.0, 3),
+        "value": round(npix / ms_graph / 1000.0, 3),
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(ms_graph, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded 3M-Gaussian scene, random dL/dimage)",
-        "config": {"workload": f"per-view fwd+bwd, {P} Gaussians in a 40^3 box, {W}x{H}, SH3, "
-                               "wodilate+OGFS defaults, stats off",
-                   "gaussians": P, "width": W, "height": H, "visible": Pv, "kv_pairs": K,
-                   "path": "plain per-view"},
+        "data": "synthetic (seeded 3M-Gaussian scene in a 40^3 box, random dL/dimage)",
+        "config": {"workload": f"batched fwd+bwd of {args.views} orbit cameras (radius 8-30, "
+                               f"resolutions {'/'.join(res)}) over {P} Gaussians, SH3, "
+                               "wodilate+OGFS defaults, one launch per stage, graph replay",
+                   "gaussians": P, "views": args.views, "pixels": npix, "kv_pairs": K,
+                   "visible_pairs": Pv, "work_shape": list(_lib.get_work_shape())},
+        "batched": {"graph_ms": round(ms_graph, 3), "eager_ms": round(ms_eager, 3),
+                    "ms_per_view": round(ms_graph / args.views, 3),
+                    "autotune_ms": {f"{a}/{b}": round(v, 3) for (a, b), v in (tuned or {}).items()},
+                    "stages": stb, "hbm_stages": hbm_frac(stb, nbb)},
+        "sequential_plain": {"ms": round(ms_seq, 3), "Mpix/s": round(npix / ms_seq / 1000.0, 3),
+                             "batched_speedup": round(ms_seq / ms_graph, 3)},
+        "per_view_plain_1080p": {"ms_per_view": round(ms1, 3),
+                                 "Mpix/s": round(1920 * 1080 / ms1 / 1000.0, 3),
+                                 "visible": Pv1, "kv_pairs": K1, "stages": st1,
+                                 "hbm_stages": hbm_frac(st1, nb1)},
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "stages": stages,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(g, cam, dL, args.cpu_band)
+        line["cpu_baseline"] = cpu_baseline(g, cam1, dL1, args.cpu_band)
     print(json.dumps(line))
 
 
