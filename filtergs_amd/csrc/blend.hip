@@ -30,38 +30,49 @@ namespace fgs {
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr int kStrips = FGS_BLOCK / FGS_WAVE;   // 4 waves, one 16x4 strip each
 
-// Conservative half extents of {alpha >= 1/255} for conic (A, B, C) and
-// opacity o, centred on the mean; negative when the Gaussian can never reach
-// alpha >= 1/255; +inf when the conic is not positive definite (no culling).
-__device__ __forceinline__ float2 alpha_extent(float A, float B, float C, float o)
+// Minimum over the rectangle [dx0, dx1] x [dy0, dy1] (offsets from the mean)
+// of q(d) = A dx^2 + 2 B dx dy + C dy^2 for a positive definite conic: zero
+// when the mean lies inside, else attained on an edge, where q is a 1-D
+// convex quadratic minimised at its clamped stationary point.
+__device__ __forceinline__ float rect_min_q(float A, float B, float C, float rA, float rC, float dx0,
+                                            float dx1, float dy0, float dy1)
 {
+    if (dx0 <= 0.f && dx1 >= 0.f && dy0 <= 0.f && dy1 >= 0.f) return 0.f;
+    float qmin = INFINITY;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const float dx = e ? dx1 : dx0;
+        const float dy = fminf(fmaxf(-B * dx * rC, dy0), dy1);
+        qmin = fminf(qmin, A * dx * dx + 2.f * B * dx * dy + C * dy * dy);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const float dy = e ? dy1 : dy0;
+        const float dx = fminf(fmaxf(-B * dy * rA, dx0), dx1);
+        qmin = fminf(qmin, A * dx * dx + 2.f * B * dx * dy + C * dy * dy);
+    }
+    return qmin;
+}
+
+// Strips of the tile whose top-left pixel is (tx0, ty0) that the region where
+// alpha >= 1/255 -- q(p - mean) <= 2 ln(255 o), slightly inflated -- can
+// reach: the exact minimum of q over each strip's pixel rectangle.  A conic
+// that is not positive definite keeps every strip.
+__device__ __forceinline__ uint32_t strip_bits_ellipse(float2 m, float4 co, float tx0, float ty0)
+{
+    const float A = co.x, B = co.y, C = co.z;
     const float det = A * C - B * B;
-    if (!(det > 0.f) || !(A > 0.f)) return make_float2(INFINITY, INFINITY);
-    const float l = 255.f * o;
-    if (!(l >= 1.f)) return make_float2(-1.f, -1.f);
+    if (!(det > 0.f) || !(A > 0.f)) return (1u << kStrips) - 1u;
+    const float l = 255.f * co.w;
+    if (!(l >= 1.f)) return 0u;
     const float tau = 2.f * __logf(l) * 1.001f + 1e-4f;
-    const float ex = sqrtf(tau * C / det) * 1.001f + 0.01f;
-    const float ey = sqrtf(tau * A / det) * 1.001f + 0.01f;
-    return make_float2(ex, ey);
-}
-
-__device__ __forceinline__ float4 alpha_box(float2 m, float4 co)
-{
-    const float2 e = alpha_extent(co.x, co.y, co.z, co.w);
-    if (e.x < 0.f) return make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
-    return make_float4(m.x - e.x, m.x + e.x, m.y - e.y, m.y + e.y);
-}
-
-// Bit w set when box bb (x0, x1, y0, y1) can touch strip w of the tile whose
-// top-left pixel is (tx0, ty0).
-__device__ __forceinline__ uint32_t strip_bits(float4 bb, float tx0, float ty0)
-{
-    if (bb.x > tx0 + (FGS_TILE - 1) || bb.y < tx0) return 0u;
+    const float rA = 1.f / A, rC = 1.f / C;
+    const float dx0 = tx0 - m.x, dx1 = tx0 + (FGS_TILE - 1) - m.x;
     uint32_t bits = 0u;
 #pragma unroll
     for (int w = 0; w < kStrips; ++w) {
-        const float y0 = ty0 + 4.f * w;
-        if (!(bb.z > y0 + 3.f || bb.w < y0)) bits |= 1u << w;
+        const float dy0 = ty0 + 4.f * w - m.y;
+        if (rect_min_q(A, B, C, rA, rC, dx0, dx1, dy0, dy0 + 3.f) <= tau) bits |= 1u << w;
     }
     return bits;
 }
@@ -212,7 +223,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                 s_g1[q] = make_float2(-0.5f * kLog2e * co.z, co.w);
                 s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
                                        __uint_as_float(g));
-                bits = strip_bits(alpha_box(m, co), tx0, ty0);
+                bits = strip_bits_ellipse(m, co, tx0, ty0);
             }
 #pragma unroll
             for (int w = 0; w < kStrips; ++w) s_w[w][q] = 0u;
