@@ -89,6 +89,7 @@ class BatchSizes(ctypes.Structure):
         ("views", c_int32),
         ("P", c_int32),
         ("tiles", c_int64),
+        ("max_view_tiles", c_int64),
         ("pixels", c_int64),
         ("plan_bytes", c_size_t),
         ("geom_bytes", c_size_t),

@@ -714,16 +714,18 @@ int fgs_batch_sizes_query(const fgs_batch_view* views, int32_t V, int32_t P, fgs
     int rc = batch_validate(views, V, P);
     if (rc) return rc;
     if (!out) return fail(FGS_E_INVALID, "null output%s");
-    int64_t T = 0, npix = 0;
+    int64_t T = 0, npix = 0, Tmax = 0;
     for (int v = 0; v < V; ++v) {
         const int gx = (views[v].image_width + 15) / 16, gy = (views[v].image_height + 15) / 16;
         T += (int64_t)gx * gy;
+        Tmax = (int64_t)gx * gy > Tmax ? (int64_t)gx * gy : Tmax;
         npix += (int64_t)views[v].image_width * views[v].image_height;
     }
     if (T > 0xFFFFFFFELL) return fail(FGS_E_INVALID, "too many tiles in the batch%s");
     out->views = V;
     out->P = P;
     out->tiles = T;
+    out->max_view_tiles = Tmax;
     out->pixels = npix;
     out->plan_bytes = align256(sizeof(fgs::BatchView) * (size_t)V);
     carve_geom(nullptr, V * P, &out->geom_bytes);
@@ -862,7 +864,8 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         }
         if (K_cap > 0) {
             Binning bn = carve_binning(binning, K_cap, nullptr);
-            const int passes = (bit_length((uint32_t)T) + 7) / 8;
+            // pairs carry the view's own tile number: the largest view sets the passes
+            const int passes = (bit_length((uint32_t)sz->max_view_tiles) + 7) / 8;
             uint64_t* first = (passes % 2 == 0) ? bn.pairsA : bn.pairsB;
             uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
             {
@@ -887,7 +890,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     }
     {
         ProfScope ps(ST_RANGES, st);
-        if ((rc = check(fgs::launch_ranges_batch(b.pairs, b.pairs ? (size_t)K_cap : 0, im.ranges, T, st),
+        if ((rc = check(fgs::launch_ranges_batch(b.pairs, b.pairs ? (size_t)K_cap : 0, P, views, im.ranges, T, st),
                         "batch ranges")))
             return rc;
     }

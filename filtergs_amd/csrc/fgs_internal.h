@@ -140,7 +140,8 @@ hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                   size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
                                   const unsigned long long* K, hipStream_t s);
-hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, uint2* ranges, int T, hipStream_t s);
+hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, int P, const BatchView* views,
+                               uint2* ranges, int T, hipStream_t s);
 
 // blend
 hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib,

@@ -278,20 +278,60 @@ hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int 
 // ---------------------------------------------------------------------------
 // duplicateWithKeys in depth order (rasterizer_impl.cu:113-147).
 // ---------------------------------------------------------------------------
+// Rectangles of up to kDupSolo tiles are written by their own lane; larger
+// ones by the whole wave, one at a time, lanes striding over the tiles so the
+// stores coalesce and one huge splat does not hold up its wave.
+constexpr uint32_t kDupSolo = 16;
+
+// Writes the kv pairs (tile_base + y * gx + x) << 32 | val of rectangle rc
+// (packed x0 | y0 << 16, x1 | y1 << 16) at off; every argument is the
+// calling lane's own.  Every lane of the wave calls it (valid = false for
+// lanes without an element); pairs at or past cap are dropped.
+__device__ __forceinline__ void dup_rect(bool valid, uint2 rc, size_t off, uint32_t val, uint32_t gx,
+                                         uint32_t tile_base, uint64_t* __restrict__ pairs, size_t cap)
+{
+    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
+    const uint32_t w = (valid && x1 > x0 && y1 > y0) ? x1 - x0 : 0u;
+    const uint32_t n = w ? w * (y1 - y0) : 0u;
+    if (n > 0 && n <= kDupSolo) {
+        for (uint32_t y = y0; y < y1; ++y)
+            for (uint32_t x = x0; x < x1; ++x, ++off)
+                if (off < cap) pairs[off] = ((uint64_t)(tile_base + y * gx + x) << 32) | val;
+    }
+    uint64_t big = __ballot(n > kDupSolo);
+    const uint32_t lane = (uint32_t)lane_id();
+    while (big) {
+        const int src = __ffsll((long long)big) - 1;
+        big &= big - 1;
+        const uint32_t bx0 = __builtin_amdgcn_readlane(x0, src);
+        const uint32_t by0 = __builtin_amdgcn_readlane(y0, src);
+        const uint32_t bw = __builtin_amdgcn_readlane(w, src);
+        const uint32_t bn = __builtin_amdgcn_readlane(n, src);
+        const uint32_t bval = __builtin_amdgcn_readlane(val, src);
+        const uint32_t bgx = __builtin_amdgcn_readlane(gx, src);
+        const uint32_t btb = __builtin_amdgcn_readlane(tile_base, src);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)off, src);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(off >> 32), src);
+        const size_t boff = ((size_t)hi << 32) | lo;
+        for (uint32_t k = lane; k < bn; k += FGS_WAVE) {
+            const uint32_t dy = k / bw;
+            const uint32_t x = bx0 + (k - dy * bw), y = by0 + dy;
+            if (boff + k < cap) pairs[boff + k] = ((uint64_t)(btb + y * bgx + x) << 32) | bval;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ vkeys,
                                                    const uint32_t* __restrict__ offs,
                                                    const uint2* __restrict__ rects_s, int P, int gx,
                                                    uint64_t* __restrict__ pairs)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P) return;
-    const uint2 rc = rects_s[s];
-    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
-    if (x1 <= x0 || y1 <= y0) return;
-    const uint32_t g = (uint32_t)vkeys[s];
-    uint32_t off = offs[s];
-    for (uint32_t y = y0; y < y1; ++y)
-        for (uint32_t x = x0; x < x1; ++x) pairs[off++] = ((uint64_t)(y * (uint32_t)gx + x) << 32) | g;
+    const bool valid = s < P;
+    const uint2 rc = valid ? rects_s[s] : make_uint2(0u, 0u);
+    const uint32_t g = valid ? (uint32_t)vkeys[s] : 0u;
+    const size_t off = valid ? offs[s] : 0u;
+    dup_rect(valid, rc, off, g, (uint32_t)gx, 0u, pairs, ~(size_t)0);
 }
 
 hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
@@ -324,8 +364,12 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
 
 // ---------------------------------------------------------------------------
 // Batched binning.  Depth keys of a batch are (v << 58) | (depth_bits << 26) | i
-// (element e = v * P + i); kv pairs carry the batch-wide tile number
-// tile_base[v] + y * gx + x and the element e.
+// (element e = v * P + i), so one sort orders every view by depth with the
+// views in sequence.  kv pairs carry the view's own tile number and e: a
+// stable sort on the tile bits alone then leaves, for every tile number t,
+// the pairs of view 0, view 1, ... one after the other, each in depth order,
+// so (view, t) ranges stay contiguous and the sort needs only the bits of the
+// largest view's tile count.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t batch_elem(uint64_t key, int P)
 {
@@ -367,18 +411,13 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
                                                          uint64_t* __restrict__ pairs, size_t cap)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const uint2 rc = rects_s[s];
-    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
-    if (x1 <= x0 || y1 <= y0) return;
-    const uint64_t key = vkeys[s];
+    const bool valid = s < n;
+    const uint2 rc = valid ? rects_s[s] : make_uint2(0u, 0u);
+    const uint64_t key = valid ? vkeys[s] : 0ull;
     const uint32_t v = (uint32_t)(key >> kBatchViewShift);
-    const uint32_t e = batch_elem(key, P);
-    const uint32_t gx = (uint32_t)views[v].gx, tb = views[v].tile_base;
-    size_t off = offs[s];
-    for (uint32_t y = y0; y < y1; ++y)
-        for (uint32_t x = x0; x < x1; ++x, ++off)
-            if (off < cap) pairs[off] = ((uint64_t)(tb + y * gx + x) << 32) | e;
+    const uint32_t gx = valid ? (uint32_t)views[v].gx : 0u;
+    const size_t off = valid ? offs[s] : 0u;
+    dup_rect(valid, rc, off, batch_elem(key, P), gx, 0u, pairs, cap);
 }
 
 // Slots [K, cap) get all-ones keys, which sort after every real tile.
@@ -405,25 +444,36 @@ hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, c
     return hipGetLastError();
 }
 
-// identifyTileRanges over the batch's tiles; keys of tile >= T (the tail) are
-// skipped.
-__global__ __launch_bounds__(256) void k_ranges_bounded(const uint64_t* __restrict__ sorted, size_t n,
-                                                        uint32_t T, uint2* __restrict__ ranges)
+// identifyTileRanges over the batch: a range runs while (tile, view) stays
+// the same; it is stored at the view's tile_base + tile.  The all-ones tail
+// keys are skipped.
+__device__ __forceinline__ uint64_t tile_view(uint64_t key, uint32_t P)
+{
+    return ((key >> 32) << 32) | ((uint32_t)key / P);
+}
+
+__global__ __launch_bounds__(256) void k_ranges_batch(const uint64_t* __restrict__ sorted, size_t n,
+                                                      uint32_t P, const BatchView* __restrict__ views,
+                                                      uint2* __restrict__ ranges)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t t = (uint32_t)(sorted[k] >> 32);
-    if (t >= T) return;
-    if (k == 0 || (uint32_t)(sorted[k - 1] >> 32) != t) ranges[t].x = (uint32_t)k;
-    if (k == n - 1 || (uint32_t)(sorted[k + 1] >> 32) != t) ranges[t].y = (uint32_t)(k + 1);
+    const uint64_t key = sorted[k];
+    if (key == ~0ull) return;
+    const uint64_t tv = tile_view(key, P);
+    const uint32_t gt = views[(uint32_t)tv].tile_base + (uint32_t)(key >> 32);
+    if (k == 0 || tile_view(sorted[k - 1], P) != tv) ranges[gt].x = (uint32_t)k;
+    if (k == n - 1 || sorted[k + 1] == ~0ull || tile_view(sorted[k + 1], P) != tv)
+        ranges[gt].y = (uint32_t)(k + 1);
 }
 
-hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, uint2* ranges, int T, hipStream_t s)
+hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, int P, const BatchView* views,
+                               uint2* ranges, int T, hipStream_t s)
 {
     hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
     if (e != hipSuccess || cap == 0) return e;
-    hipLaunchKernelGGL(k_ranges_bounded, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted, cap,
-                       (uint32_t)T, ranges);
+    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted, cap,
+                       (uint32_t)P, views, ranges);
     return hipGetLastError();
 }
 

@@ -147,7 +147,10 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
  * loop of a training step.  One launch per stage covers every view:
  *   - preprocess reads each Gaussian once and projects it into all views;
  *   - one depth sort orders all views ((view, depth, index) keys);
- *   - duplicate / tile sort / ranges run over the batch's tile numbering
+ *   - duplicate / tile sort / ranges run over all views at once; pairs keep
+ *     their view's own tile numbers, so the stable tile sort needs only the
+ *     bits of the largest view's tile count and leaves each (view, tile)
+ *     range contiguous; ranges are stored in the batch's tile numbering
  *     (view v owns tiles [tile_base_v, tile_base_v + T_v));
  *   - one blend launch renders every tile of every view;
  *   - the preprocess backward folds the per-view gradients into one set of
@@ -175,6 +178,7 @@ typedef struct {
     int32_t views;
     int32_t P;
     int64_t tiles;            /* tiles over all views                        */
+    int64_t max_view_tiles;   /* tiles of the largest view                   */
     int64_t pixels;           /* pixels over all views                       */
     size_t plan_bytes;        /* device bytes of the plan (per-view records) */
     size_t geom_bytes;        /* device bytes of the batch geometry buffer   */
