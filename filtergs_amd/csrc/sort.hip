@@ -383,7 +383,10 @@ __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const uint4 r = recs[batch_elem(vkeys[s], P)];
+    const uint64_t key = vkeys[s];
+    // culled elements (all-ones depth field) have no rectangle stored
+    const bool culled = ((key >> kBatchIdxBits) & 0xFFFFFFFFull) == 0xFFFFFFFFull;
+    const uint4 r = culled ? make_uint4(0u, 0u, 0u, 0u) : recs[batch_elem(key, P)];
     tiles_s[s] = r.z;
     rects_s[s] = make_uint2(r.x, r.y);
 }
