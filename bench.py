@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=20, help="steps of the plain 1080p leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-lod", action="store_true", help="skip the urban LoD leg")
     ap.add_argument("--cpu-band", type=int, default=256, help="rows of the CPU baseline sample")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many MiB (a "
@@ -68,6 +69,88 @@ def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
         "blend_bwd": 40 * K + 20 * npix + 36 * Pv,
         "preprocess_bwd": (Pv * (44 + 12 * M + 36) if V == 1 else params + 36 * Pv) + grads_out,
     }
+
+
+def lod_leg(steps, warmup):
+    """Urban-scale LoD: a 22.4M-node quadtree (12 levels, 2x2 roots over 2 km x
+    2 km, leaves on a seeded height field, SH degree 1), the LoD cut of each of
+    8 oblique aerial 1920x1080 views (radius_max 1.5 px, padding 0.1,
+    threshold 700), the attribute gather of the union of the cuts, then the
+    batched fwd+bwd of the 8 views over the gathered Gaussians."""
+    import torch
+
+    from filtergs_amd import lod, workloads
+    from filtergs_amd.batch import BatchRasterizer
+    dev = "cuda"
+    tree = workloads.lod_tree(levels=12, roots_side=2, extent=2000.0, sh_degree=1, seed=0, device=dev)
+    cams = workloads.aerial_cameras(8)
+    N = int(tree["means3D"].shape[0])
+    L = int(tree["ancestor_path"].shape[1])
+    n_leaf = int((tree["tree_node_index"] < 0).sum().item())
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def cut(cam, rmax):
+        f = cam.width / (2 * cam.tanfovx)
+        return lod.compute_visibility_radius(
+            tree["means3D"], tree["log_scales"], tree["rotations"],
+            torch.as_tensor(cam.full_proj_transform, device=dev),
+            torch.as_tensor(cam.world_view_transform, device=dev), tree["tree_node_index"],
+            tree["ancestor_path"], f, cam.height / (2 * cam.tanfovy), cam.tanfovx, cam.tanfovy,
+            rmax, padding=0.1, radius_filter_threshold=700.0)
+
+    res = {}
+    for rmax in (5.0, 1.5):
+        cut(cams[0], rmax)
+        torch.cuda.synchronize()
+        ev[0].record()
+        masks = [cut(c, rmax)[0] for c in cams]
+        ev[1].record()
+        ev[1].synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / len(cams)
+        union = torch.stack(masks).any(0)
+        nbytes = N * (44 + 4 + 9) + n_leaf * 4 * L
+        res[rmax] = {"ms_per_view": round(ms, 3), "GB/s": round(nbytes / ms / 1e6, 1),
+                     "kept_per_view": [int(m.sum().item()) for m in masks],
+                     "union": int(union.sum().item())}
+    union = torch.stack([cut(c, 1.5)[0] for c in cams]).any(0)
+    idx = torch.nonzero(union).reshape(-1)
+    torch.cuda.synchronize()
+    ev[0].record()
+    ga = lod.gather_attributes(idx, tree["means3D"], tree["log_scales"], tree["rotations"],
+                               tree["opacities"], tree["means3D"], tree["shs"])
+    ev[1].record()
+    ev[1].synchronize()
+    ms_gather = ev[0].elapsed_time(ev[1])
+    P = int(idx.numel())
+    means = ga["xyz"].contiguous()
+    scales = ga["scaling"].exp().contiguous()
+    rots = torch.nn.functional.normalize(ga["rotation"], dim=1).contiguous()
+    opac = ga["opacity"].reshape(-1).contiguous()
+    shs = ga["shs"].contiguous()
+    del tree
+    br = BatchRasterizer(cams, P, bg=(0.0, 0.0, 0.0), sh_degree=1)
+    br.bind(means, opac, scales, rots, shs=shs)
+    K = br.forward()
+    npix = int(br.sizes.pixels)
+    dL = torch.randn((3 * npix,), device=dev, generator=torch.Generator(device=dev).manual_seed(4))
+    grads = br.backward(dL)
+    graph, _ = br.capture(dL, out=grads)
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(steps):
+        graph.replay()
+    ev[1].record()
+    ev[1].synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    if int(br.counts[1].item()) > br.K_cap:
+        raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
+    return {"nodes": N, "leaves": n_leaf, "levels": L,
+            "cut_radius_max_1.5": res[1.5], "cut_radius_max_5.0": res[5.0],
+            "gather_ms": round(ms_gather, 3), "gaussians_rendered": P, "kv_pairs": K,
+            "batched_fwd_bwd_ms": round(ms, 3), "Mpix/s": round(npix / ms / 1000.0, 3),
+            "views": len(cams), "pixels": npix}
 
 
 def cpu_baseline(g, cam, dL, band):
@@ -259,6 +342,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    if not args.no_lod:
+        del br, graph, grads, means, scales, rots, opac, shs
+        torch.cuda.empty_cache()
+        line["lod_c4"] = lod_leg(max(2, args.steps // 2), args.warmup)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(g, cam1, dL1, args.cpu_band)
     print(json.dumps(line))

@@ -68,3 +68,102 @@ def config(name: str, seed: int = 0):
     if name == "c2b":
         return gaussians(3_000_000, 20.0, 3, seed), camera_c1()
     raise KeyError(name)
+
+
+def terrain_height(x, y, seed: int = 0):
+    """Seeded 2.5D height field in metres (torch tensors or numpy arrays):
+    a sum of a few rotated sinusoids of 1 km to 20 m wavelength."""
+    rng = np.random.default_rng(seed + 1000)
+    h = 0.0
+    for amp, wl in ((45.0, 1100.0), (18.0, 420.0), (6.0, 130.0), (2.0, 37.0), (0.6, 11.0)):
+        th = float(rng.uniform(0, math.pi))
+        ph = float(rng.uniform(0, 2 * math.pi))
+        k = 2 * math.pi / wl
+        u = x * math.cos(th) + y * math.sin(th)
+        v = -x * math.sin(th) + y * math.cos(th)
+        h = h + amp * (u * k + ph).sin() * (0.7 * v * k).cos() if hasattr(u, "sin") else \
+            h + amp * np.sin(u * k + ph) * np.cos(0.7 * v * k)
+    return h
+
+
+def lod_tree(levels: int = 12, roots_side: int = 2, extent: float = 2000.0, sh_degree: int = 1,
+             seed: int = 0, device="cpu"):
+    """Urban-scale LoD tree: a quadtree over an `extent` x `extent` area with
+    roots_side^2 roots and `levels` levels (level k is a grid of
+    (roots_side * 2^k)^2 nodes, every internal node has 4 children, child
+    scale = parent scale / 2).  Leaves sit on the seeded 2.5D height field
+    with jitter inside their cell; an internal node is the mean of its
+    children.  Nodes are numbered level by level, row-major inside a level.
+
+    Returns torch tensors on `device`: means3D [N,3], log_scales [N,3],
+    rotations [N,4] (raw), opacities [N,1], shs [N,M,3], tree_node_index [N]
+    (own index for internal nodes, -1 for leaves), ancestor_path [N,levels]
+    (ancestor at depth d in column d, -1 beyond the node's depth: the
+    apps/ancestor.py format), level [N].
+    """
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    dev = torch.device(device)
+    sides = [roots_side << k for k in range(levels)]
+    counts = [s * s for s in sides]
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    N = int(offs[-1])
+    L = levels
+    # leaves
+    sl = sides[-1]
+    cell = extent / sl
+    iy, ix = torch.meshgrid(torch.arange(sl, dtype=torch.float32), torch.arange(sl, dtype=torch.float32),
+                            indexing="ij")
+    jit = (torch.rand((sl, sl, 2), generator=g) - 0.5) * 0.8 * cell
+    x = (ix + 0.5) * cell - extent / 2 + jit[..., 0]
+    y = (iy + 0.5) * cell - extent / 2 + jit[..., 1]
+    z = terrain_height(x, y, seed) + torch.randn((sl, sl), generator=g) * 0.3
+    level_means = [torch.stack([x, y, z], -1)]
+    for k in range(levels - 2, -1, -1):
+        m = level_means[0]
+        s = sides[k]
+        level_means.insert(0, m.view(s, 2, s, 2, 3).mean(dim=(1, 3)))
+    means = torch.cat([m.reshape(-1, 3) for m in level_means])
+    log_scales = torch.cat([torch.full((counts[k], 3), math.log(0.5 * extent / sides[k])) for k in range(levels)])
+    log_scales = log_scales + torch.randn((N, 3), generator=g) * 0.15
+    q = torch.randn((N, 4), generator=g)
+    opac = torch.sigmoid(torch.randn((N, 1), generator=g) + 1.0)
+    M = (sh_degree + 1) ** 2
+    shs = torch.randn((N, M, 3), generator=g) * 0.05
+    shs[:, 0, :] = torch.randn((N, 3), generator=g) * 0.5
+    # structure
+    level = torch.cat([torch.full((counts[k],), k, dtype=torch.int32) for k in range(levels)])
+    idx = torch.arange(N, dtype=torch.int64)
+    node_index = torch.where(level < levels - 1, idx.to(torch.int32), torch.full_like(level, -1))
+    anc = torch.full((N, L), -1, dtype=torch.int32)
+    for k in range(1, levels):
+        s = sides[k]
+        local = torch.arange(counts[k], dtype=torch.int64)
+        ly, lx = local // s, local % s
+        for d in range(k):
+            sh = k - d
+            ad = (ly >> sh) * sides[d] + (lx >> sh) + int(offs[d])
+            anc[int(offs[k]):int(offs[k + 1]), d] = ad.to(torch.int32)
+    out = dict(means3D=means, log_scales=log_scales, rotations=q, opacities=opac, shs=shs,
+               tree_node_index=node_index, ancestor_path=anc, level=level)
+    return {k: v.contiguous().to(dev) for k, v in out.items()}
+
+
+def aerial_cameras(n_views: int = 8, extent: float = 2000.0, seed: int = 5,
+                   altitude_range=(150.0, 400.0), width: int = 1920, height: int = 1080):
+    """Oblique aerial views over the LoD terrain: positions uniform over the
+    central 60 % of the area at altitude U(altitude_range), each looking at a
+    ground point 150-400 m ahead, FoVx 60 deg, far plane 5 km."""
+    rng = np.random.default_rng(seed)
+    cams = []
+    for _ in range(n_views):
+        cx, cy = rng.uniform(-0.3 * extent, 0.3 * extent, 2)
+        alt = float(rng.uniform(*altitude_range))
+        th = float(rng.uniform(0, 2 * math.pi))
+        d = float(rng.uniform(150.0, 400.0))
+        tx, ty = cx + d * math.cos(th), cy + d * math.sin(th)
+        ground = float(terrain_height(np.array(cx), np.array(cy)))
+        cams.append(make_camera(width, height, center=(float(cx), float(cy), ground + alt),
+                                target=(tx, ty, float(terrain_height(np.array(tx), np.array(ty)))),
+                                fovx_deg=60.0, up=(0.0, 0.0, 1.0), znear=0.5, zfar=5000.0))
+    return cams
