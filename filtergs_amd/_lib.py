@@ -133,6 +133,10 @@ _SIGS = {
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p]),
+    "fgs_activate_gather": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p]),
     "fgs_batch_sizes_query": (c_int32, [c_void_p, c_int32, c_int32, c_void_p]),
     "fgs_batch_binning_bytes": (c_size_t, [c_int64]),
     "fgs_batch_plan": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),

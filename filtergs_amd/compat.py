@@ -8,7 +8,9 @@ import (LoG/render/renderer.py:1, FilterGS/filtergs.py:3-5):
 * ``diff_gaussian_rasterization`` (and its ``._C``) -> ``filtergs_amd.dilate`` /
   ``filtergs_amd.ext``;
 * ``FilterGS.cuda.compute_radius``, ``FilterGS.cuda.visible_flag``,
-  ``FilterGS.cuda.gather_attributes`` -> ``filtergs_amd.lod``.
+  ``FilterGS.cuda.gather_attributes`` -> ``filtergs_amd.lod``;
+* ``FilterGS`` / ``FilterGS.filtergs.FilterGS`` -> ``filtergs_amd.filtergs``
+  when no FilterGS package is importable.
 
 A ``FilterGS`` package that is importable keeps its own top-level module; only
 its ``cuda`` subpackage is taken over, which also stops the reference's
@@ -21,7 +23,7 @@ import importlib.util
 import sys
 import types
 
-from . import dilate, ext, lod, rasterizer
+from . import dilate, ext, filtergs, lod, rasterizer
 
 _FILTERGS_CUDA = ("compute_radius", "visible_flag", "gather_attributes")
 
@@ -43,9 +45,12 @@ def install(replace: bool = True) -> None:
     put("diff_gaussian_rasterization._C", ext)
 
     if "FilterGS" not in sys.modules and importlib.util.find_spec("FilterGS") is None:
+        # no FilterGS package here: provide this package's FilterGS class
         pkg = types.ModuleType("FilterGS")
         pkg.__path__ = []
+        pkg.FilterGS = filtergs.FilterGS
         put("FilterGS", pkg)
+        put("FilterGS.filtergs", filtergs)
     cuda = types.ModuleType("FilterGS.cuda")
     cuda.__path__ = []
     for sub in _FILTERGS_CUDA:
@@ -57,7 +62,7 @@ def install(replace: bool = True) -> None:
 
 def uninstall() -> None:
     """Drop the aliases that point at this package's modules."""
-    ours = {rasterizer, dilate, ext, lod}
+    ours = {rasterizer, dilate, ext, lod, filtergs}
     for name in list(sys.modules):
         mod = sys.modules[name]
         if mod in ours and not name.startswith("filtergs_amd"):

@@ -688,6 +688,43 @@ int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const 
                  "gather_attributes");
 }
 
+int fgs_activate_gather(int64_t N, const int64_t* index, int64_t n_src, const float* xyz,
+                        const float* log_scaling, const float* rotation, const float* opacity_logit,
+                        const float* features_dc, const float* features_rest, int32_t rest_coeffs,
+                        int32_t active_sh_degree, const float* campos, float* xyz_out,
+                        float* scaling_out, float* rotation_out, float* opacity_out, float* colors_out,
+                        int32_t* bad_index, void* stream)
+{
+    if (N < 0 || n_src < 0) return fail(FGS_E_INVALID, "activate: negative size%s");
+    if (active_sh_degree < 0 || active_sh_degree > 4)
+        return fail(FGS_E_INVALID, "activate: SH degree must be in [0, 4]%s");
+    const int need = (active_sh_degree + 1) * (active_sh_degree + 1) - 1;
+    if (active_sh_degree > 0 && (rest_coeffs < need || !features_rest || !campos))
+        return fail(FGS_E_INVALID, "activate: SH rest coefficients / camera centre missing for the degree%s");
+    if (N > 0 && (!index || !xyz || !log_scaling || !rotation || !opacity_logit || !features_dc ||
+                  !xyz_out || !scaling_out || !rotation_out || !opacity_out || !colors_out || !bad_index))
+        return fail(FGS_E_INVALID, "activate: null pointer%s");
+    fgs::ActivateArgs a{};
+    a.N = N;
+    a.n_src = n_src;
+    a.index = index;
+    a.xyz = xyz;
+    a.log_scale = log_scaling;
+    a.rot = rotation;
+    a.opacity_logit = opacity_logit;
+    a.dc = features_dc;
+    a.rest = features_rest;
+    a.rest_floats = rest_coeffs * 3;
+    a.deg = active_sh_degree;
+    a.campos = campos;
+    a.xyz_o = xyz_out;
+    a.scale_o = scaling_out;
+    a.rot_o = rotation_out;
+    a.opacity_o = opacity_out;
+    a.colors_o = colors_out;
+    a.bad_index = bad_index;
+    return check(fgs::launch_activate_gather(a, (hipStream_t)stream), "activate_gather");
+}
 
 /* ---- batched multi-view path ------------------------------------------- */
 

@@ -178,6 +178,28 @@ struct LodArgs {
     uint8_t* cull_flag;        // scratch [N]: internal & visible & 0 < r <= radius_max
 };
 hipError_t launch_lod_cut(const LodArgs& a, hipStream_t s);
+
+// gather + activation of FilterGS.get_all
+struct ActivateArgs {
+    int64_t N, n_src;
+    const int64_t* index;
+    const float* xyz;            // [n_src,3]
+    const float* log_scale;      // [n_src,3]
+    const float* rot;            // [n_src,4] raw
+    const float* opacity_logit;  // [n_src]
+    const float* dc;             // [n_src,3] SH DC coefficient
+    const float* rest;           // [n_src, rest_floats] coefficients 1.. (or null when deg == 0)
+    int rest_floats;
+    int deg;                     // active SH degree, 0..4
+    const float* campos;         // device [3]
+    float* xyz_o;
+    float* scale_o;
+    float* rot_o;
+    float* opacity_o;
+    float* colors_o;
+    int* bad_index;
+};
+hipError_t launch_activate_gather(const ActivateArgs& a, hipStream_t s);
 hipError_t launch_radius_v2(int P, const float* means3d, const float* scales,
                             const float* rotations, const float* proj, const float* view,
                             float fx, float fy, float tanx, float tany, float* radii,

@@ -1,0 +1,265 @@
+"""The FilterGS caller API (FilterGS/filtergs.py:9-359) on the gfx950 kernels.
+
+``FilterGS`` keeps the reference's buffers (xyz, log-scale ``scaling``, raw
+``rotation``, logit ``opacity``, SH DC ``colors``, SH rest ``shs``) and its three
+stages:
+
+* ``prepare`` -- the LoD cut of the tree (``compute_visibility_radius``), or
+  without a tree the NDC frustum test plus the 3-sigma radius of the raw
+  parameters (the reference computes that radius from the log-scales; see
+  ``radius_from_log_scales``);
+* ``get_all`` -- gather of the visible rows and their activation (exp, sigmoid,
+  normalise, SH2RGB(dc) + SH without DC toward the camera) in one fused kernel,
+  ``fgs_activate_gather``, instead of a gather plus five element-wise passes;
+* ``render`` -- the finite / positive filter and the wodilate rasterizer with
+  precomputed colours.
+"""
+from __future__ import annotations
+
+import time
+
+import torch
+import torch.nn as nn
+
+from . import _lib, lod
+from ._lib import call, ptr
+
+SH_C0 = 0.28209479177387814
+
+
+class FilterGS(nn.Module):
+    def __init__(self, sh_degree=1, xyz_scale=1.):
+        super().__init__()
+        self.xyz_scale = xyz_scale
+        self.max_sh_degree = sh_degree
+        self.active_sh_degree = 0
+        self.keys = ['xyz', 'features_dc', 'features_rest', 'scaling', 'rotation', 'opacity']
+        self.visibility_flag = None
+        self.sh_c0 = SH_C0
+        # FilterGS/filtergs.py:261-278 feeds the raw log-scales to the no-tree
+        # radius; keep that unless set to False
+        self.radius_from_log_scales = True
+
+    # -- construction (FilterGS/filtergs.py:304-359) -----------------------
+    def register_by_pointcloud(self, xyz, colors, scales, init_opacity, **init_ply):
+        print(f'[{self.__class__.__name__}] {self.log_radius(scales)}')
+        scales = torch.clamp(scales, min=scales.mean() / 4, max=scales.mean() * 4)
+        print(f'[{self.__class__.__name__}] -> {self.log_radius(scales)}')
+        scaling = torch.log(scales)[:, None].repeat(1, 3)
+        colors = (colors - 0.5) / SH_C0
+        shs = None
+        if self.max_sh_degree > 0:
+            shs = torch.zeros((colors.shape[0], (self.max_sh_degree + 1) ** 2 - 1, 3),
+                              dtype=torch.float32)
+        opacity = torch.ones_like(xyz[:, :1]) * init_opacity
+        opacity = torch.log(opacity / (1 - opacity))
+        rotation = self.init_rotation(xyz.shape[0], xyz.device)
+        if 'height' in init_ply:
+            local_min, local_max = xyz.min(dim=0), xyz.max(dim=0)
+            xyz_g, colors_g, scaling_g, opacity_g = self.create_from_ground(
+                local_min, local_max, init_ply['init_step'], init_ply['height'],
+                init_ply['ground_opacity'])
+            rotation_g = self.init_rotation(xyz_g.shape[0], xyz.device)
+            print(f'[{self.__class__.__name__}] add {xyz_g.shape[0]} ground points')
+            xyz = torch.cat([xyz, xyz_g], dim=0)
+            opacity = torch.cat([opacity, opacity_g], dim=0)
+            colors = torch.cat([colors, colors_g], dim=0)
+            scaling = torch.cat([scaling, torch.log(scaling_g)], dim=0)
+            rotation = torch.cat([rotation, rotation_g], dim=0)
+            if shs is not None:
+                shs = torch.cat([shs, torch.zeros((xyz_g.shape[0], *shs.shape[1:]),
+                                                  dtype=torch.float32)], dim=0)
+        self.register_buffer('scaling', scaling)
+        self.register_buffer('colors', colors)
+        self.register_buffer('xyz', xyz)
+        self.register_buffer('opacity', opacity)
+        self.register_buffer('rotation', rotation)
+        self.keys.extend(['scaling', 'colors', 'xyz', 'opacity', 'rotation'])
+        if shs is not None:
+            self.register_buffer('shs', shs)
+            self.keys.append('shs')
+
+    @staticmethod
+    def create_from_ground(local_min, local_max, init_step, height, init_opacity=0.9, padding=0.05):
+        x = torch.arange(local_min[0][0] - padding, local_max[0][0] + padding, init_step)
+        y = torch.arange(local_min[0][1] - padding, local_max[0][1] + padding, init_step)
+        x, y = torch.meshgrid(x, y, indexing="ij")
+        xyz = torch.stack((x, y), axis=-1).reshape(-1, 2)
+        xyz = torch.cat([xyz, torch.zeros((xyz.shape[0], 1)) + height], dim=1)
+        colors = torch.zeros_like(xyz) + 0.5
+        scaling = torch.zeros_like(xyz) + init_step
+        scaling[:, 2] = init_step * 0.1
+        opacity = torch.zeros((xyz.shape[0], 1)) + init_opacity
+        return xyz, colors, scaling, opacity
+
+    def init_rotation(self, num_points, device):
+        rot = torch.zeros((num_points, 4), dtype=torch.float32, device=device)
+        rot[:, 0] = 1.
+        return rot
+
+    def log_radius(self, scales):
+        return f'scales: [{scales.min().item():.4f}~{scales.mean().item():.4f}~{scales.max().item():.4f}]'
+
+    # -- stage 1 -------------------------------------------------------------
+    def prepare(self, rasterizer, camera, tree=None, tree_node_index_gpu=None, ancestor_path_gpu=None,
+                radius_max=2.0, padding=0.5, radius_filter_threshold=700.0):
+        """Visibility of the candidates: the LoD cut when a tree is given."""
+        device = self.xyz.device
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        if tree_node_index_gpu is not None and ancestor_path_gpu is not None:
+            node_index_gpu = tree_node_index_gpu
+        elif tree is not None and self.xyz.is_cuda:
+            node_index_gpu = tree.node_index.to(device=device, dtype=torch.int32)
+            ancestor_path_gpu = tree.ancestor_path.to(device=device, dtype=torch.int32)
+        else:
+            node_index_gpu, ancestor_path_gpu = None, None
+        ev[1].record()
+        all_mask, all_radius2d, all_depth = self._compute_visibility_radius_cuda(
+            rasterizer, camera, node_index_gpu, ancestor_path_gpu, radius_max, padding,
+            radius_filter_threshold)
+        ev[2].record()
+        index_all = torch.nonzero(all_mask, as_tuple=True)[0]
+        radius2d = torch.masked_select(all_radius2d, all_mask)
+        depth_visible = torch.masked_select(all_depth, all_mask) if all_depth.numel() > 0 else all_depth
+        self.visibility_flag = {
+            'index': index_all, 'flag': all_mask, 'radius2d': radius2d, 'depth': depth_visible,
+            'raw_inputs': {
+                'visible_index': index_all, 'node_index_gpu': node_index_gpu,
+                'ancestor_path_gpu': ancestor_path_gpu,
+                'base_attrs': {'xyz': self.xyz, 'scaling': self.scaling, 'rotation': self.rotation,
+                               'opacity': self.opacity, 'colors': self.colors,
+                               'shs': getattr(self, 'shs', None)}}}
+        ev[3].record()
+        ev[3].synchronize()
+        self.last_prepare_timing = {
+            "T_tree_ms": float(ev[0].elapsed_time(ev[1])),
+            "T_vis_ms": float(ev[1].elapsed_time(ev[2])),
+            "T_pack_ms": float(ev[2].elapsed_time(ev[3])),
+            "T_prepare_total_ms": float(ev[0].elapsed_time(ev[3])),
+            "vis_fallback": False,
+            "num_visible": int(index_all.numel()),
+        }
+        return len(index_all)
+
+    # -- stage 2 -------------------------------------------------------------
+    def get_all(self, camera):
+        """Gather + activation of the visible rows (one fused kernel)."""
+        if self.visibility_flag is None:
+            raise RuntimeError("Must call prepare() before get_all()")
+        idx = self.visibility_flag['index']
+        device = self.xyz.device
+        _lib.require_gpu(self.xyz)
+        idx = idx.to(device=device, dtype=torch.int64).contiguous()
+        N = int(idx.numel())
+        n_src = int(self.xyz.shape[0])
+        f32 = dict(dtype=torch.float32, device=device)
+        out = dict(xyz=torch.empty((N, 3), **f32), scaling=torch.empty((N, 3), **f32),
+                   rotation=torch.empty((N, 4), **f32), opacity=torch.empty((N, 1), **f32),
+                   colors=torch.empty((N, 3), **f32))
+        deg = int(self.active_sh_degree)
+        shs = getattr(self, 'shs', None)
+        rest = shs.to(**f32).contiguous() if (deg > 0 and shs is not None) else None
+        rest_coeffs = int(rest.shape[1]) if rest is not None else 0
+        campos = None
+        if deg > 0 and camera is not None:
+            campos = torch.as_tensor(camera['camera_center']).to(**f32).reshape(3).contiguous()
+        else:
+            deg = 0
+        bad = torch.zeros((1,), dtype=torch.int32, device=device)
+        src = [t.to(**f32).contiguous() for t in (self.xyz, self.scaling, self.rotation,
+                                                  self.opacity, self.colors)]
+        if N > 0:
+            call("fgs_activate_gather", N, idx.data_ptr(), n_src, *[ptr(t) for t in src], ptr(rest),
+                 rest_coeffs, deg, ptr(campos), out['xyz'].data_ptr(), out['scaling'].data_ptr(),
+                 out['rotation'].data_ptr(), out['opacity'].data_ptr(), out['colors'].data_ptr(),
+                 bad.data_ptr(), _lib.stream_handle(device))
+            if int(bad.item()):
+                raise RuntimeError("gather_attributes: index out of range")
+        return out
+
+    # -- stage 3 -------------------------------------------------------------
+    def render(self, camera, rasterizer, background=None, enable_stats=True):
+        if self.visibility_flag is None:
+            raise RuntimeError("Must call prepare() before render()")
+        activated = self.get_all(camera)
+        xyz, opacity, colors = activated['xyz'], activated['opacity'], activated['colors']
+        scales, rotations = activated['scaling'], activated['rotation']
+        if opacity.dim() == 1:
+            opacity = opacity.unsqueeze(-1)
+        finite = (torch.isfinite(xyz).all(dim=-1) & torch.isfinite(scales).all(dim=-1)
+                  & torch.isfinite(rotations).all(dim=-1) & torch.isfinite(opacity).all(dim=-1)
+                  & torch.isfinite(colors).all(dim=-1))
+        finite = finite & (scales.min(dim=-1).values > 0.0) & (opacity[:, 0] > 0.0)
+        if not torch.all(finite):
+            xyz, opacity, colors = xyz[finite], opacity[finite], colors[finite]
+            scales, rotations = scales[finite], rotations[finite]
+        if xyz.shape[0] == 0:
+            rs = rasterizer.raster_settings
+            h, w = int(rs.image_height), int(rs.image_width)
+            if background is None:
+                bg = torch.zeros(3, dtype=xyz.dtype, device=xyz.device)
+            else:
+                bg = background.to(device=xyz.device, dtype=xyz.dtype).reshape(-1)
+                if bg.numel() >= 3:
+                    bg = bg[:3]
+                elif bg.numel() == 1:
+                    bg = bg.repeat(3)
+                else:
+                    bg = torch.zeros(3, dtype=xyz.dtype, device=xyz.device)
+            return {"render": bg[:, None, None].expand(3, h, w).contiguous(),
+                    "timing": {"T_raster_ms": 0.0}}
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        out = rasterizer(means3D=xyz, means2D=torch.zeros_like(xyz), shs=None, colors_precomp=colors,
+                         opacities=opacity, scales=scales, rotations=rotations,
+                         ogfs_enable_stats=bool(enable_stats))
+        ev1.record()
+        ev1.synchronize()
+        return {"render": out[0][:3], "timing": {"T_raster_ms": float(ev0.elapsed_time(ev1))}}
+
+    # -- visibility (FilterGS/filtergs.py:225-302) ---------------------------
+    def _visible_flag_by_camera_cuda(self, xyz, camera, padding=0.05):
+        return lod.visible_flag_cuda_direct(xyz, camera['full_proj_transform'], padding=padding)
+
+    @staticmethod
+    def _visible_flag_by_camera_torch(xyz, camera, padding=0.05):
+        """The torch frustum test of the reference (NDC depth in (0, 1)),
+        computed from the visible_flag kernel's projection."""
+        _, _, p_proj = lod.visible_flag_cuda_direct(xyz, camera['full_proj_transform'], padding=padding)
+        depth = p_proj[:, 2]
+        valid = ((depth > 0.0) & (depth < 1.0) & (p_proj[:, 0] > -1.0 - padding)
+                 & (p_proj[:, 0] < 1.0 + padding) & (p_proj[:, 1] > -1.0 - padding)
+                 & (p_proj[:, 1] < 1.0 + padding))
+        return valid, depth, p_proj
+
+    def _compute_visibility_radius_cuda(self, rasterizer, camera, tree_node_index_gpu,
+                                        ancestor_path_gpu, radius_max, padding,
+                                        radius_filter_threshold=100.0):
+        rs = rasterizer.raster_settings
+        device = self.xyz.device
+        tanfovx, tanfovy = float(rs.tanfovx), float(rs.tanfovy)
+        focal_x = float(rs.image_width) / (2.0 * tanfovx)
+        focal_y = float(rs.image_height) / (2.0 * tanfovy)
+        proj = rs.projmatrix.to(device=device, dtype=torch.float32).contiguous()
+        view = rs.viewmatrix.to(device=device, dtype=torch.float32).contiguous()
+        scaling = self.scaling.detach().to(device=device).contiguous()
+        rotation = self.rotation.detach().to(device=device).contiguous()
+        if tree_node_index_gpu is None or ancestor_path_gpu is None:
+            valid, depth, _ = self._visible_flag_by_camera_torch(self.xyz, camera, padding=padding)
+            sc = scaling if self.radius_from_log_scales else scaling.exp()
+            radius2d = lod.compute_radius_module.compute_radius(
+                self.xyz.detach().contiguous(), sc, rotation, proj, view, focal_x, focal_y, tanfovx,
+                tanfovy)
+            mask = valid
+            if radius_filter_threshold > 0:
+                mask = mask & (radius2d <= radius_filter_threshold)
+            return mask, radius2d, depth
+        return lod.compute_visibility_radius(
+            self.xyz.detach().contiguous(), scaling, rotation, proj, view, tree_node_index_gpu,
+            ancestor_path_gpu, focal_x, focal_y, tanfovx, tanfovy, float(radius_max), padding,
+            radius_filter_threshold)
+
+
+__all__ = ["FilterGS"]

@@ -92,8 +92,8 @@ def lod_tree(levels: int = 12, roots_side: int = 2, extent: float = 2000.0, sh_d
     roots_side^2 roots and `levels` levels (level k is a grid of
     (roots_side * 2^k)^2 nodes, every internal node has 4 children, child
     scale = parent scale / 2).  Leaves sit on the seeded 2.5D height field
-    with jitter inside their cell; an internal node is the mean of its
-    children.  Nodes are numbered level by level, row-major inside a level.
+    (scaled with the area: its 2 km form times extent / 2 km) with jitter
+    inside their cell; an internal node is the mean of its children.  Nodes are numbered level by level, row-major inside a level.
 
     Returns torch tensors on `device`: means3D [N,3], log_scales [N,3],
     rotations [N,4] (raw), opacities [N,1], shs [N,M,3], tree_node_index [N]
@@ -117,7 +117,8 @@ def lod_tree(levels: int = 12, roots_side: int = 2, extent: float = 2000.0, sh_d
     jit = (torch.rand((sl, sl, 2), generator=g) - 0.5) * 0.8 * cell
     x = (ix + 0.5) * cell - extent / 2 + jit[..., 0]
     y = (iy + 0.5) * cell - extent / 2 + jit[..., 1]
-    z = terrain_height(x, y, seed) + torch.randn((sl, sl), generator=g) * 0.3
+    k = extent / 2000.0   # the height field scales with the area
+    z = terrain_height(x / k, y / k, seed) * k + torch.randn((sl, sl), generator=g) * 0.3 * k
     level_means = [torch.stack([x, y, z], -1)]
     for k in range(levels - 2, -1, -1):
         m = level_means[0]

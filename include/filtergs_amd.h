@@ -264,6 +264,21 @@ int fgs_gather_attributes(int64_t N, const int64_t* index, int64_t n_src, const 
                           float* opacity_out, float* colors_out, float* shs_out, int32_t* bad_index,
                           void* stream);
 
+/* FilterGS.get_all (FilterGS/filtergs.py:98-134): gather_attributes of the
+ * rows in index, then Activation.activate_root_return
+ * (LoG/model/activation.py:36-43) -- scaling = exp, opacity = sigmoid,
+ * rotation = q / max(|q|, 1e-12), colors = SH2RGB(dc) + eval_sh_wobase
+ * (LoG/model/sh_utils.py:31-68, no clamp) toward campos (device float[3])
+ * for active_sh_degree > 0 -- fused into one pass.  features_rest is
+ * [n_src, rest_coeffs, 3] (coefficients 1..); bad_index (device int, zero on
+ * entry) is set when an index is outside [0, n_src). */
+int fgs_activate_gather(int64_t N, const int64_t* index, int64_t n_src, const float* xyz,
+                        const float* log_scaling, const float* rotation, const float* opacity_logit,
+                        const float* features_dc, const float* features_rest, int32_t rest_coeffs,
+                        int32_t active_sh_degree, const float* campos, float* xyz_out,
+                        float* scaling_out, float* rotation_out, float* opacity_out, float* colors_out,
+                        int32_t* bad_index, void* stream);
+
 /* ---- misc ---------------------------------------------------------------- */
 /* Per-stage timing with HIP events recorded on the launch stream (off by
  * default; not for use under graph capture).  Stages, in order: preprocess,

@@ -338,3 +338,63 @@ def backward(s: OrcSettings, fwd, dL_dpix, means3D, opacities, scales=None, rota
     out["dL_dcolors"] = gb["dL_dcolors"]
     out["dL_dconic"] = gb["dL_dconic"]
     return out
+
+
+# ---------------------------------------------------------------------------
+# FilterGS.get_all: gather + Activation.activate_root_return, FP64 numpy.
+# ---------------------------------------------------------------------------
+_SH_C0 = 0.28209479177387814
+_SH_C1 = 0.4886025119029199
+_SH_C2 = (1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792,
+          0.5462742152960396)
+_SH_C3 = (-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154,
+          -0.4570457994644658, 1.445305721320277, -0.5900435899266435)
+_SH_C4 = (2.5033429417967046, -1.7701307697799304, 0.9461746957575601, -0.6690465435572892,
+          0.10578554691520431, -0.6690465435572892, 0.47308734787878004, -1.7701307697799304,
+          0.6258357354491761)
+
+
+def sh_wobase_basis(dirs, degree):
+    """Basis functions 1 .. (degree+1)^2-1 of eval_sh_wobase
+    (LoG/model/sh_utils.py:31-68) at unit directions [n,3] -> [n, K]."""
+    x, y, z = dirs[:, 0], dirs[:, 1], dirs[:, 2]
+    cols = [-_SH_C1 * y, _SH_C1 * z, -_SH_C1 * x]
+    if degree > 1:
+        xx, yy, zz, xy, yz, xz = x * x, y * y, z * z, x * y, y * z, x * z
+        cols += [_SH_C2[0] * xy, _SH_C2[1] * yz, _SH_C2[2] * (2 * zz - xx - yy), _SH_C2[3] * xz,
+                 _SH_C2[4] * (xx - yy)]
+        if degree > 2:
+            cols += [_SH_C3[0] * y * (3 * xx - yy), _SH_C3[1] * xy * z,
+                     _SH_C3[2] * y * (4 * zz - xx - yy), _SH_C3[3] * z * (2 * zz - 3 * xx - 3 * yy),
+                     _SH_C3[4] * x * (4 * zz - xx - yy), _SH_C3[5] * z * (xx - yy),
+                     _SH_C3[6] * x * (xx - 3 * yy)]
+            if degree > 3:
+                cols += [_SH_C4[0] * xy * (xx - yy), _SH_C4[1] * yz * (3 * xx - yy),
+                         _SH_C4[2] * xy * (7 * zz - 1), _SH_C4[3] * yz * (7 * zz - 3),
+                         _SH_C4[4] * (zz * (35 * zz - 30) + 3), _SH_C4[5] * xz * (7 * zz - 3),
+                         _SH_C4[6] * (xx - yy) * (7 * zz - 1), _SH_C4[7] * xz * (xx - 3 * yy),
+                         _SH_C4[8] * (xx * (xx - 3 * yy) - yy * (3 * xx - yy))]
+    return np.stack(cols, 1)
+
+
+def activate_gather(index, xyz, log_scaling, rotation, opacity_logit, features_dc,
+                    features_rest, active_sh_degree, campos):
+    """FilterGS.get_all (FilterGS/filtergs.py:98-134): row gather, then
+    scaling = exp, opacity = sigmoid, rotation = F.normalize (eps 1e-12),
+    colors = SH2RGB(dc) + eval_sh_wobase(dir, rest) with dir the normalised
+    xyz - campos, no clamp (LoG/model/activation.py:27-43)."""
+    idx = np.asarray(index, np.int64)
+    f = lambda a: np.asarray(a, np.float64)[idx]
+    xyz_g = f(xyz)
+    q = f(rotation)
+    qn = np.maximum(np.linalg.norm(q, axis=1, keepdims=True), 1e-12)
+    colors = f(features_dc).reshape(-1, 3) * _SH_C0 + 0.5
+    if active_sh_degree > 0:
+        d = xyz_g - np.asarray(campos, np.float64)[None]
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        B = sh_wobase_basis(d, active_sh_degree)
+        rest = f(features_rest).reshape(len(idx), -1, 3)[:, :B.shape[1], :]
+        colors = colors + np.einsum("nk,nkc->nc", B, rest)
+    return dict(xyz=xyz_g, scaling=np.exp(f(log_scaling)),
+                opacity=1.0 / (1.0 + np.exp(-f(opacity_logit).reshape(-1))),
+                rotation=q / qn, colors=colors)

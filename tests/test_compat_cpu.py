@@ -82,3 +82,39 @@ def test_uninstall_removes_aliases(shims):
     assert "diff_gaussian_rasterization_wodilate" not in sys.modules
     assert "FilterGS.cuda.compute_radius" not in sys.modules
     compat.install()
+
+
+def test_filtergs_class_api(shims):
+    """FilterGS/filtergs.py:9-359: constructor, stages and helpers."""
+    from filtergs_amd.filtergs import FilterGS
+    pkg = sys.modules.get("FilterGS")
+    if pkg is not None and getattr(pkg, "__file__", None) is None:   # the shim's package
+        from FilterGS import FilterGS as F2
+        from FilterGS.filtergs import FilterGS as F3
+        assert F2 is FilterGS and F3 is FilterGS
+    assert _params(FilterGS.__init__) == ["self", "sh_degree", "xyz_scale"]
+    assert _params(FilterGS.prepare) == ["self", "rasterizer", "camera", "tree",
+                                         "tree_node_index_gpu", "ancestor_path_gpu", "radius_max",
+                                         "padding", "radius_filter_threshold"]
+    sig = inspect.signature(FilterGS.prepare).parameters
+    assert (sig["radius_max"].default, sig["padding"].default,
+            sig["radius_filter_threshold"].default) == (2.0, 0.5, 700.0)
+    assert _params(FilterGS.get_all) == ["self", "camera"]
+    assert _params(FilterGS.render) == ["self", "camera", "rasterizer", "background", "enable_stats"]
+    assert _params(FilterGS.register_by_pointcloud)[:5] == ["self", "xyz", "colors", "scales",
+                                                            "init_opacity"]
+
+
+def test_filtergs_register_by_pointcloud_buffers():
+    """register_by_pointcloud stores the inverse activations (filtergs.py:304-348)."""
+    import torch
+    from filtergs_amd.filtergs import FilterGS
+    fg = FilterGS(sh_degree=2)
+    xyz = torch.zeros((4, 3))
+    colors = torch.full((4, 3), 0.75)
+    scales = torch.tensor([1.0, 1.0, 1.0, 1.0])
+    fg.register_by_pointcloud(xyz, colors, scales, 0.1)
+    assert torch.allclose(fg.scaling, torch.zeros((4, 3)))
+    assert torch.allclose(fg.colors, torch.full((4, 3), 0.25 / 0.28209479177387814))
+    assert torch.allclose(torch.sigmoid(fg.opacity), torch.full((4, 1), 0.1))
+    assert fg.shs.shape == (4, 8, 3) and fg.rotation[:, 0].eq(1).all()

@@ -1,0 +1,111 @@
+"""The FilterGS caller API on the GPU: LoD-cut index bit-exact, fused
+gather + activation against the oracle's FP64 activation, the no-tree
+fallback's mask bit-exact against the oracle's frustum and radius, and the
+render against the rasterizer fed the oracle's activated attributes."""
+import numpy as np
+import pytest
+import torch
+
+from filtergs_amd import workloads
+from filtergs_amd.camera import make_camera
+from filtergs_amd.filtergs import FilterGS
+from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def _scene(sh_degree=2):
+    tree = workloads.lod_tree(levels=7, roots_side=2, extent=40.0, sh_degree=sh_degree, seed=3)
+    fg = FilterGS(sh_degree=sh_degree)
+    fg.register_buffer("xyz", tree["means3D"].to(DEV))
+    fg.register_buffer("scaling", tree["log_scales"].to(DEV))
+    fg.register_buffer("rotation", tree["rotations"].to(DEV))
+    op = tree["opacities"].clamp(1e-4, 1 - 1e-4)
+    fg.register_buffer("opacity", torch.log(op / (1 - op)).to(DEV))
+    fg.register_buffer("colors", tree["shs"][:, 0, :].contiguous().to(DEV))
+    fg.register_buffer("shs", tree["shs"][:, 1:, :].contiguous().to(DEV))
+    return tree, fg
+
+
+def _raster(cam, bg=(0.0, 0.0, 0.0)):
+    rs = GaussianRasterizationSettings(
+        image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
+        bg=_t(bg), scale_modifier=1.0, viewmatrix=_t(cam.world_view_transform),
+        projmatrix=_t(cam.full_proj_transform), sh_degree=0, campos=_t(cam.center),
+        prefiltered=False, debug=False)
+    return GaussianRasterizer(rs)
+
+
+def _cam():
+    return make_camera(320, 200, center=(3.0, -4.0, 16.0), target=(0.0, 0.0, 0.0), fovx_deg=70.0,
+                       znear=0.05, zfar=200.0)
+
+
+def test_filtergs_lod_pipeline(orc):
+    tree, fg = _scene()
+    cam = _cam()
+    r = _raster(cam)
+    camera = {"camera_center": _t(cam.center), "full_proj_transform": _t(cam.full_proj_transform)}
+    n = fg.prepare(r, camera, tree_node_index_gpu=tree["tree_node_index"].to(DEV),
+                   ancestor_path_gpu=tree["ancestor_path"].to(DEV), radius_max=5.0, padding=0.1,
+                   radius_filter_threshold=700.0)
+    f = cam.width / (2 * cam.tanfovx)
+    np_t = {k: v.numpy() for k, v in tree.items()}
+    mask, _, _ = orc.lod_cut(np_t["means3D"], np_t["log_scales"], np_t["rotations"],
+                             cam.full_proj_transform, cam.world_view_transform,
+                             np_t["tree_node_index"], np_t["ancestor_path"], f,
+                             cam.height / (2 * cam.tanfovy), cam.tanfovx, cam.tanfovy, 0.1, 5.0,
+                             700.0)
+    idx = np.nonzero(mask)[0]
+    assert n == idx.size > 100
+    assert np.array_equal(fg.visibility_flag["index"].cpu().numpy(), idx)
+
+    fg.active_sh_degree = 2
+    act = fg.get_all(camera)
+    op = np_t["opacities"].clip(1e-4, 1 - 1e-4).astype(np.float64)
+    ref = orc.activate_gather(idx, np_t["means3D"], np_t["log_scales"], np_t["rotations"],
+                              fg.opacity.cpu().numpy().reshape(-1), np_t["shs"][:, 0, :],
+                              np_t["shs"][:, 1:, :], 2, cam.center)
+    assert np.array_equal(act["xyz"].cpu().numpy(), np_t["means3D"][idx])
+    for k, tol in (("scaling", 2e-6), ("rotation", 2e-6), ("colors", 2e-6)):
+        np.testing.assert_allclose(act[k].cpu().numpy(), ref[k], rtol=tol, atol=1e-6, err_msg=k)
+    np.testing.assert_allclose(act["opacity"].cpu().numpy().reshape(-1), ref["opacity"], rtol=2e-6,
+                               atol=1e-7)
+    np.testing.assert_allclose(ref["opacity"], op[idx, 0], rtol=1e-5)
+
+    out = fg.render(camera, r)
+    img = out["render"]
+    # the same render from the oracle's activation (rounded to float32)
+    ref_img = r(means3D=_t(ref["xyz"]), means2D=torch.zeros((idx.size, 3), device=DEV), shs=None,
+                colors_precomp=_t(ref["colors"]), opacities=_t(ref["opacity"].reshape(-1, 1)),
+                scales=_t(ref["scaling"]), rotations=_t(ref["rotation"]), ogfs_enable_stats=False)[0]
+    d = (img - ref_img).abs()
+    assert float(d.max()) < 0.02
+    assert float((d > 1e-4).float().mean()) < 1e-3
+    assert out["timing"]["T_raster_ms"] >= 0.0
+
+
+def test_filtergs_no_tree_fallback(orc):
+    tree, fg = _scene(sh_degree=1)
+    cam = _cam()
+    r = _raster(cam)
+    camera = {"camera_center": _t(cam.center), "full_proj_transform": _t(cam.full_proj_transform)}
+    n = fg.prepare(r, camera, padding=0.1, radius_filter_threshold=40.0)
+    np_t = {k: v.numpy() for k, v in tree.items()}
+    f = cam.width / (2 * cam.tanfovx)
+    _, _, pp = orc.visible_flag(np_t["means3D"], cam.full_proj_transform, 0.1)
+    vis = ((pp[:, 2] > 0) & (pp[:, 2] < 1) & (pp[:, 0] > -1.1) & (pp[:, 0] < 1.1)
+           & (pp[:, 1] > -1.1) & (pp[:, 1] < 1.1))
+    # the reference feeds the raw log-scales to the radius (FilterGS/filtergs.py:261-278)
+    rad = orc.radius_v2(np_t["means3D"], np_t["log_scales"], np_t["rotations"],
+                        cam.full_proj_transform, cam.world_view_transform, f,
+                        cam.height / (2 * cam.tanfovy), cam.tanfovx, cam.tanfovy)
+    expect = vis & (rad <= 40.0)
+    assert n == int(expect.sum()) > 100
+    assert np.array_equal(fg.visibility_flag["flag"].cpu().numpy(), expect)
