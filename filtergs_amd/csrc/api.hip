@@ -153,7 +153,9 @@ Geom carve_geom(void* buf, int P, size_t* total)
 struct Binning {
     uint64_t* pairsA;
     uint64_t* pairsB;
-    uint8_t* strips;   // per kv pair: bit w = the Gaussian blended into strip w of its tile
+    uint8_t* strips;   // per visible-list entry: bit w = it blended into strip w of its tile
+    float* part;       // backward: nine gradient sums per kv pair (pre-sort position)
+    uint8_t* mark;     // backward: the pair's part entry is written
     uint32_t* hist;
     uint32_t* partials;
 };
@@ -166,6 +168,8 @@ Binning carve_binning(void* buf, int64_t K, size_t* total)
     b.pairsA = c.take<uint64_t>(n);
     b.pairsB = c.take<uint64_t>(n);
     b.strips = c.take<uint8_t>(n);
+    b.part = c.take<float>(9 * n);
+    b.mark = c.take<uint8_t>(n);
     b.hist = c.take<uint32_t>(fgs::radix_hist_words(n));
     b.partials = c.take<uint32_t>(scan_words_for_hist(n));
     if (total) *total = c.off;
@@ -176,6 +180,8 @@ struct Image {
     float* final_T;
     uint32_t* n_contrib;
     uint2* ranges;
+    uint32_t* vis_last;   // per pixel: end of its blends in the tile's visible list
+    uint32_t* tile_vis;   // per tile: length of the visible list
 };
 
 // Per-pixel state of npix pixels and T tiles (a batch concatenates its views).
@@ -186,21 +192,17 @@ Image carve_image_n(void* buf, size_t npix, size_t T, size_t* total)
     im.final_T = c.take<float>(npix > 0 ? npix : 1);
     im.n_contrib = c.take<uint32_t>(npix > 0 ? npix : 1);
     im.ranges = c.take<uint2>(T > 0 ? T : 1);
+    im.vis_last = c.take<uint32_t>(npix > 0 ? npix : 1);
+    im.tile_vis = c.take<uint32_t>(T > 0 ? T : 1);
     if (total) *total = c.off;
     return im;
 }
 
 Image carve_image(void* buf, int W, int H, size_t* total)
 {
-    Carver c{ static_cast<char*>(buf) };
-    Image im;
     const size_t npix = (size_t)W * (size_t)H;
     const size_t T = (size_t)((W + 15) / 16) * (size_t)((H + 15) / 16);
-    im.final_T = c.take<float>(npix > 0 ? npix : 1);
-    im.n_contrib = c.take<uint32_t>(npix > 0 ? npix : 1);
-    im.ranges = c.take<uint2>(T > 0 ? T : 1);
-    if (total) *total = c.off;
-    return im;
+    return carve_image_n(buf, npix, T, total);
 }
 
 int bit_length(uint32_t n)
@@ -436,6 +438,7 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
     Geom gm = carve_geom(geom, g->P, nullptr);
     fgs::BlendArgs b{};
     b.ranges = im.ranges;
+    b.tile_vis = im.tile_vis;
     b.W = W;
     b.H = H;
     b.gx = gx;
@@ -453,7 +456,8 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
         uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
         {
             ProfScope ps(ST_DUPLICATE, st);
-            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.rects_s, g->P, gx, first, st),
+            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.rects_s, g->P, gx, first, gm.recs,
+                                                  st),
                             "duplicate")))
                 return rc;
         }
@@ -465,6 +469,7 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
         }
         if ((rc = sync_check(s, st, "binning"))) return rc;
         b.pairs = bn.pairsA;
+        b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);   // free once sorted
         b.kv_strips = bn.strips;
     } else {
         b.pairs = nullptr;
@@ -475,7 +480,7 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
     }
     {
         ProfScope ps(ST_BLEND_FWD, st);
-        if ((rc = check(fgs::launch_blend_fwd(b, im.final_T, im.n_contrib, out_color, out_point_id,
+        if ((rc = check(fgs::launch_blend_fwd(b, im.final_T, im.n_contrib, im.vis_last, out_color, out_point_id,
                                               out_weight_pixel, out_point_weight, kv_alpha_sum,
                                               kv_alpha_count, st), "blend")))
             return rc;
@@ -553,8 +558,14 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         Binning bn = carve_binning(const_cast<void*>(binning), K, nullptr);
         fgs::BlendArgs b{};
         b.ranges = im.ranges;
+        b.tile_vis = im.tile_vis;
         b.pairs = bn.pairsA;
+        b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);
         b.kv_strips = bn.strips;
+        b.recs = gm.recs;
+        b.part = bn.part;
+        b.mark = bn.mark;
+        b.part_cap = (size_t)K;
         b.W = W;
         b.H = H;
         b.gx = gx;
@@ -565,7 +576,8 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         b.bg = s->bg;
         b.bwd_batch = g_shape[1];
         ProfScope ps(ST_BLEND_BWD, st);
-        if ((rc = check(fgs::launch_blend_bwd(b, im.final_T, im.n_contrib, dL_dpix, dL_dmeans2D,
+        const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, 0 };
+        if ((rc = check(fgs::launch_blend_bwd(b, gg, im.final_T, im.vis_last, dL_dpix, dL_dmeans2D,
                                               dconic, dL_dopacity, dL_dcolors, s->grad_mode == 1, st),
                         "blend backward")))
             return rc;
@@ -861,6 +873,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     Geom gm = carve_geom(geom, (int)(n > 0 ? n : 1), nullptr);
     fgs::BlendArgs b{};
     b.ranges = im.ranges;
+    b.tile_vis = im.tile_vis;
     b.batch = g_shape[0];
     b.id_stride = (uint32_t)P;
     if (n > 0) {
@@ -908,7 +921,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
             {
                 ProfScope ps(ST_DUPLICATE, st);
                 if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, P, views,
-                                                            first, (size_t)K_cap, cnt + 1, st),
+                                                            first, (size_t)K_cap, cnt + 1, gm.recs, st),
                                 "batch duplicate")))
                     return rc;
             }
@@ -919,6 +932,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
                     return rc;
             }
             b.pairs = bn.pairsA;
+            b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);   // free once sorted
             b.kv_strips = bn.strips;
         }
         b.means2D = gm.means2D;
@@ -933,7 +947,8 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     }
     {
         ProfScope ps(ST_BLEND_FWD, st);
-        if ((rc = check(fgs::launch_blend_fwd_batch(b, views, V, T, im.final_T, im.n_contrib, out_color,
+        if ((rc = check(fgs::launch_blend_fwd_batch(b, views, V, T, im.final_T, im.n_contrib, im.vis_last,
+                                                    out_color,
                                                     out_point_id, out_weight_pixel, out_point_weight,
                                                     st), "batch blend")))
             return rc;
@@ -970,14 +985,21 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         Binning bn = carve_binning(const_cast<void*>(binning), K_cap, nullptr);
         fgs::BlendArgs b{};
         b.ranges = im.ranges;
+        b.tile_vis = im.tile_vis;
         b.pairs = bn.pairsA;
+        b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);
         b.kv_strips = bn.strips;
+        b.recs = gm.recs;
+        b.part = bn.part;
+        b.mark = bn.mark;
+        b.part_cap = (size_t)K_cap;
         b.means2D = gm.means2D;
         b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
         b.conic_opacity = gm.conic_opacity;
         b.bwd_batch = g_shape[1];
         ProfScope ps(ST_BLEND_BWD, st);
-        if ((rc = check(fgs::launch_blend_bwd_batch(b, views, V, T, im.final_T, im.n_contrib, dL_dpix,
+        const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P };
+        if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,
                                                     dL_dmeans2D, dconic, dop_view, dL_dcolors,
                                                     s->grad_mode == 1, st), "batch blend backward")))
             return rc;

@@ -80,10 +80,10 @@ __device__ __forceinline__ uint32_t strip_bits_ellipse(float2 m, float4 co, floa
 // Per-wave compacted lists of batch positions.  Every lane of the block calls
 // it (q = threadIdx.x is the batch slot it staged; bits its strip mask, zero
 // for empty slots).  Two barriers; on return s_list[w][0 .. count(w)) holds
-// the slots whose bit w is set, in increasing order, and the count for the
-// calling wave is returned.
-template <int BATCH>
-__device__ __forceinline__ int build_lists(uint32_t bits, uint16_t (*s_list)[BATCH],
+// the slots whose bit w is set (as slot << SHIFT), in increasing order, and
+// the count for the calling wave is returned (wave-uniform, in an SGPR).
+template <int BATCH, int SHIFT, typename L>
+__device__ __forceinline__ int build_lists(uint32_t bits, L (*s_list)[BATCH],
                                            uint32_t (*s_wcnt)[kStrips])
 {
     const int tid = threadIdx.x;
@@ -103,14 +103,14 @@ __device__ __forceinline__ int build_lists(uint32_t bits, uint16_t (*s_list)[BAT
         if ((bits >> w) & 1u) {
             int off = 0;
             for (int sw = 0; sw < wave; ++sw) off += (int)s_wcnt[sw][w];
-            s_list[w][off + rank[w]] = (uint16_t)tid;
+            s_list[w][off + rank[w]] = (L)(tid << SHIFT);
         }
     }
     int total = 0;
 #pragma unroll
     for (int sw = 0; sw < kStrips; ++sw) total += (int)s_wcnt[sw][wave];
     __syncthreads();
-    return total;
+    return __builtin_amdgcn_readfirstlane(total);
 }
 
 // DPP read of another lane's value inside a 16-lane row; written as
@@ -164,10 +164,24 @@ __device__ __forceinline__ float exponent2(float4 g0, float2 g1, float dx, float
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
+// One staged Gaussian of the forward, 64 bytes.  The per-wave lists hold slot
+// byte offsets, so the loop reads a slot at fixed offsets from one address.
+struct __align__(16) FwdSlot {
+    float4 g0;         // x, y, a', b'  (power in log2 units)
+    float4 col;        // r, g, b, id bits
+    float2 g1;         // c', opacity
+    uint32_t w[kStrips];   // per strip: max T*alpha (float bits)
+    float kv;          // sum of T*alpha (statistics)
+    uint32_t kc;       // contributions (statistics)
+};
+static_assert(sizeof(FwdSlot) == 64, "forward slot is 64 bytes");
+constexpr int kFwdSlotShift = 6;
+
 // Forward of one tile (one workgroup of FGS_BLOCK lanes).
 template <int BATCH>
 __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, float* __restrict__ final_T,
                                                uint32_t* __restrict__ n_contrib,
+                                               uint32_t* __restrict__ vis_last,
                                                float* __restrict__ out_color,
                                                int* __restrict__ out_point_id,
                                                float* __restrict__ out_weight_pixel,
@@ -185,7 +199,10 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
     const int py = ty * FGS_TILE + (tid / FGS_TILE);
     const bool inside = px < a.W && py < a.H;
-    const float pfx = (float)px, pfy = (float)py;
+    float pfx = (float)px, pfy = (float)py;
+    // keep the pixel position in registers (the compiler would otherwise
+    // convert it again in every iteration)
+    asm volatile("" : "+v"(pfx), "+v"(pfy));
     const float tx0 = (float)(tx * FGS_TILE), ty0 = (float)(ty * FGS_TILE);
     bool done = !inside;
 
@@ -193,23 +210,30 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     const int n = (int)(range.y - range.x);
     const int rounds = (n + BATCH - 1) / BATCH;
 
-    __shared__ float4 s_g0[BATCH];   // x, y, a', b'  (power in log2 units)
-    __shared__ float2 s_g1[BATCH];   // c', opacity
-    __shared__ float4 s_col[BATCH];  // r, g, b, id bits
-    __shared__ uint32_t s_w[kStrips][BATCH];   // per strip: max T*alpha (float bits)
-    __shared__ float s_kv[BATCH];
-    __shared__ uint32_t s_kc[BATCH];
-    __shared__ uint16_t s_list[kStrips][BATCH];
+    __shared__ FwdSlot s_slot[BATCH];
+    __shared__ uint32_t s_list[kStrips][BATCH];
     __shared__ uint32_t s_wcnt[kStrips][kStrips];
+    __shared__ uint32_t s_vcnt[kStrips];   // visible slots per wave in this round
+    __shared__ uint32_t s_vidx[BATCH];     // visible-list index of each visible slot
 
     float T = 1.0f;
     float C0 = 0.f, C1 = 0.f, C2 = 0.f;
     uint32_t last = 0;
     float wmax = 0.f;
     int max_id = -1;
+    // The tile's visible list (Gaussians that blend into at least one pixel)
+    // is appended round by round; the pixel remembers the slot of its last
+    // blend in the round and turns it into a list index after the round.
+    uint32_t vis_n = 0;        // entries appended so far (block-uniform)
+    uint32_t vlast = 0;        // 1 + list index of the pixel's last blend
+    int last_slot = -1;
 
     for (int r = 0; r < rounds; ++r) {
         const int num_done = __syncthreads_count(done);
+        if (last_slot >= 0) {
+            vlast = s_vidx[last_slot] + 1u;
+            last_slot = -1;
+        }
         if (num_done == FGS_BLOCK) break;
         const int q = tid;
         const int k = r * BATCH + q;
@@ -219,22 +243,27 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                 const uint32_t g = (uint32_t)a.pairs[range.x + k];
                 const float2 m = a.means2D[g];
                 const float4 co = a.conic_opacity[g];
-                s_g0[q] = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
-                s_g1[q] = make_float2(-0.5f * kLog2e * co.z, co.w);
-                s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
-                                       __uint_as_float(g));
+                s_slot[q].g0 = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                s_slot[q].g1 = make_float2(-0.5f * kLog2e * co.z, co.w);
+                s_slot[q].col = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
+                                            __uint_as_float(g));
                 bits = strip_bits_ellipse(m, co, tx0, ty0);
             }
 #pragma unroll
-            for (int w = 0; w < kStrips; ++w) s_w[w][q] = 0u;
-            s_kv[q] = 0.f;
-            s_kc[q] = 0u;
+            for (int w = 0; w < kStrips; ++w) s_slot[q].w[w] = 0u;
+            s_slot[q].kv = 0.f;
+            s_slot[q].kc = 0u;
         }
-        const int cnt = build_lists<BATCH>(bits, s_list, s_wcnt);
+        const int cnt = build_lists<BATCH, kFwdSlotShift>(bits, s_list, s_wcnt);
+        char* const slots = reinterpret_cast<char*>(s_slot);
+        // the next entry of the list is read one iteration ahead
+        uint32_t next = s_list[wave][0];
         for (int i = 0; !done && i < cnt; ++i) {
-            const int j = s_list[wave][i];
-            const float4 g0 = s_g0[j];
-            const float2 g1 = s_g1[j];
+            FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
+            next = s_list[wave][min(i + 1, BATCH - 1)];
+            const float4 g0 = sp->g0;
+            const float2 g1 = sp->g1;
+            const float4 col = sp->col;
             const float dx = g0.x - pfx, dy = g0.y - pfy;
             const float p2 = exponent2(g0, g1, dx, dy);
             const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
@@ -244,7 +273,6 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
             done = done || stop;
             if (pass && !stop) {
                 const float w = alpha * T;
-                const float4 col = s_col[j];
                 C0 = fmaf(col.x, w, C0);
                 C1 = fmaf(col.y, w, C1);
                 C2 = fmaf(col.z, w, C2);
@@ -252,39 +280,63 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                     wmax = w;
                     max_id = (int)__float_as_uint(col.w);
                 }
-                atomicMax(&s_w[wave][j], __float_as_uint(w));
+                atomicMax(&sp->w[wave], __float_as_uint(w));
                 if (kv_sum) {
-                    atomicAdd(&s_kv[j], w);
-                    atomicAdd(&s_kc[j], 1u);
+                    atomicAdd(&sp->kv, w);
+                    atomicAdd(&sp->kc, 1u);
                 }
                 T = test_T;
-                last = (uint32_t)(r * BATCH + j + 1);
+                last_slot = (int)(sp - s_slot);
+                last = (uint32_t)(r * BATCH + last_slot + 1);
             }
         }
         __syncthreads();
+        uint32_t used = 0u, g = 0u;
         if (q < BATCH && k < n) {
-            const uint32_t g = __float_as_uint(s_col[q].w);
-            uint32_t m = 0u, used = 0u;
+            g = __float_as_uint(s_slot[q].col.w);
+            uint32_t m = 0u;
 #pragma unroll
             for (int w = 0; w < kStrips; ++w) {
-                m = s_w[w][q] > m ? s_w[w][q] : m;
-                used |= (s_w[w][q] != 0u ? 1u : 0u) << w;
+                const uint32_t sw = s_slot[q].w[w];
+                m = sw > m ? sw : m;
+                used |= (sw != 0u ? 1u : 0u) << w;
             }
             if (m != 0u) atomicMax((uint32_t*)&point_weight[g], m);
-            a.kv_strips[range.x + k] = (uint8_t)used;
             if (kv_sum) {
                 // each kv pair belongs to exactly one tile: plain stores
-                kv_sum[range.x + k] = s_kv[q];
-                kv_cnt[range.x + k] = s_kc[q];
+                kv_sum[range.x + k] = s_slot[q].kv;
+                kv_cnt[range.x + k] = s_slot[q].kc;
             }
         }
+        // append the round's visible slots in slot order
+        const uint64_t vb = __ballot(used != 0u);
+        if (lane_id() == 0) s_vcnt[wave] = (uint32_t)__popcll(vb);
+        __syncthreads();
+        uint32_t off = vis_n;
+#pragma unroll
+        for (int w = 0; w < kStrips; ++w) {
+            const uint32_t c = s_vcnt[w];
+            off += w < wave ? c : 0u;
+            vis_n += c;
+        }
+        if (used != 0u) {
+            const uint32_t idx = off + __builtin_amdgcn_mbcnt_hi((uint32_t)(vb >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)vb, 0u));
+            a.vis_ids[range.x + idx] = g;
+            a.kv_strips[range.x + idx] = (uint8_t)used;
+            s_vidx[q] = idx;
+        }
     }
+    __syncthreads();
+    if (last_slot >= 0) vlast = s_vidx[last_slot] + 1u;
+    if (tid == 0) a.tile_vis[tile] = vis_n;
 
     if (inside) {
         const int pix = a.W * py + px;
         const int HW = a.W * a.H;
         final_T[pix] = T;
         n_contrib[pix] = last;
+        vis_last[pix] = vlast;
         out_color[pix] = fmaf(T, a.bg[0], C0);
         out_color[HW + pix] = fmaf(T, a.bg[1], C1);
         out_color[2 * HW + pix] = fmaf(T, a.bg[2], C2);
@@ -295,12 +347,13 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
 
 template <int BATCH>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* final_T, uint32_t* n_contrib,
-                                                         float* out_color, int* out_point_id,
-                                                         float* out_weight_pixel, float* point_weight,
-                                                         float* kv_sum, uint32_t* kv_cnt)
+                                                         uint32_t* vis_last, float* out_color,
+                                                         int* out_point_id, float* out_weight_pixel,
+                                                         float* point_weight, float* kv_sum,
+                                                         uint32_t* kv_cnt)
 {
-    blend_fwd_tile<BATCH>(a, blockIdx.x, final_T, n_contrib, out_color, out_point_id, out_weight_pixel,
-                          point_weight, kv_sum, kv_cnt);
+    blend_fwd_tile<BATCH>(a, blockIdx.x, final_T, n_contrib, vis_last, out_color, out_point_id,
+                          out_weight_pixel, point_weight, kv_sum, kv_cnt);
 }
 
 // View of a batch that owns workgroup b (views sorted by tile_base).
@@ -318,6 +371,7 @@ __device__ __forceinline__ BlendArgs bind_blend(const BlendArgs& common, const B
     BlendArgs a = common;
     a.id_base = (uint32_t)v * common.id_stride;
     a.ranges = common.ranges + vw.tile_base;
+    a.tile_vis = common.tile_vis + vw.tile_base;
     a.W = vw.W;
     a.H = vw.H;
     a.gx = vw.gx;
@@ -332,8 +386,8 @@ __device__ __forceinline__ BlendArgs bind_blend(const BlendArgs& common, const B
 template <int BATCH>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd_batch(BlendArgs common, const BatchView* views,
                                                                int V, float* final_T,
-                                                               uint32_t* n_contrib, float* out_color,
-                                                               int* out_point_id,
+                                                               uint32_t* n_contrib, uint32_t* vis_last,
+                                                               float* out_color, int* out_point_id,
                                                                float* out_weight_pixel,
                                                                float* point_weight)
 {
@@ -342,26 +396,28 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd_batch(BlendArgs common,
     const BlendArgs a = bind_blend(common, vw, v);
     const size_t pb = vw.pix_base;
     blend_fwd_tile<BATCH>(a, (int)(blockIdx.x - vw.tile_base), final_T + pb, n_contrib + pb,
-                          out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb, point_weight,
-                          nullptr, nullptr);
+                          vis_last + pb, out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb,
+                          point_weight, nullptr, nullptr);
 }
 
 hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
-                                  float* final_T, uint32_t* n_contrib, float* out_color,
-                                  int* out_point_id, float* out_weight_pixel, float* point_weight,
-                                  hipStream_t s)
+                                  float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
+                                  float* out_color, int* out_point_id, float* out_weight_pixel,
+                                  float* point_weight, hipStream_t s)
 {
     if (T_total == 0) return hipSuccess;
     if (common.batch == 128)
         hipLaunchKernelGGL(k_blend_fwd_batch<128>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
-                           final_T, n_contrib, out_color, out_point_id, out_weight_pixel, point_weight);
+                           final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
+                           point_weight);
     else
         hipLaunchKernelGGL(k_blend_fwd_batch<256>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
-                           final_T, n_contrib, out_color, out_point_id, out_weight_pixel, point_weight);
+                           final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
+                           point_weight);
     return hipGetLastError();
 }
 
-hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib,
+hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
                             float* out_color, int* out_point_id, float* out_weight_pixel,
                             float* point_weight, float* kv_sum, uint32_t* kv_cnt, hipStream_t s)
 {
@@ -369,10 +425,10 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
     if (T == 0) return hipSuccess;
     if (a.batch == 128)
         hipLaunchKernelGGL(k_blend_fwd<128>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
-                           out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
+                           vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
     else
         hipLaunchKernelGGL(k_blend_fwd<256>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
-                           out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
+                           vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
     return hipGetLastError();
 }
 
@@ -380,37 +436,67 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 // Backward
 // ---------------------------------------------------------------------------
 
-// Reduce-scatter of 8 values across the wave.  Inside every 16-lane row the
-// halving steps pair lanes with DPP permutations (mirror: bit 3, half mirror:
-// bit 2, quad_perm: bits 1 and 0), leaving in lane r the row sum of value
-// 4*bit3 + 2*bit2 + bit1; two lane swaps then add the four rows.  On return
-// lane l < 16 holds the wave total of value l >> 1.
-__device__ __forceinline__ float wave_reduce_scatter8(const float v[8])
+// Lane swaps of the gfx950 (v_permlane32_swap / v_permlane16_swap) on a
+// pair of registers: one half (row) of a is exchanged with the other half
+// (row) of b, so a + b afterwards holds a's pair sums in one half (every
+// other row) and b's in the other.
+__device__ __forceinline__ void swap_halves(uint32_t& a, uint32_t& b)
 {
-    const int lane = lane_id();
-    const bool h3 = (lane >> 3) & 1, h2 = (lane >> 2) & 1, h1 = (lane >> 1) & 1;
-    float w4[4];
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+}
+__device__ __forceinline__ void swap_rows(uint32_t& a, uint32_t& b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+}
+__device__ __forceinline__ float fold_halves(float a, float b)
+{
+    uint32_t x = __float_as_uint(a), y = __float_as_uint(b);
+    swap_halves(x, y);
+    return __uint_as_float(x) + __uint_as_float(y);
+}
+__device__ __forceinline__ float fold_rows(float a, float b)
+{
+    uint32_t x = __float_as_uint(a), y = __float_as_uint(b);
+    swap_rows(x, y);
+    return __uint_as_float(x) + __uint_as_float(y);
+}
+
+// Reduce-scatter of 8 values across the wave: four pairs folded across the
+// wave halves, two across row pairs, then inside each 16-lane row the two
+// remaining values split by lane bit 3 (mirror) and three DPP steps add the
+// 8-lane groups.  Every lane ends with the wave total of one value; which
+// value is fixed per lane and given by scatter_term().
+__device__ __forceinline__ float wave_reduce_scatter8(const float v[8], bool h3)
+{
+    float u[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float send = h3 ? v[k] : v[k + 4];
-        const float keep = h3 ? v[k + 4] : v[k];
-        w4[k] = keep + dpp<kDppRowMirror>(send);
-    }
-    float w2[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const float send = h2 ? w4[k] : w4[k + 2];
-        const float keep = h2 ? w4[k + 2] : w4[k];
-        w2[k] = keep + dpp<kDppRowHalfMirror>(send);
-    }
-    float z;
-    {
-        const float send = h1 ? w2[0] : w2[1];
-        const float keep = h1 ? w2[1] : w2[0];
-        z = keep + dpp<kDppXor2>(send);
-    }
+    for (int k = 0; k < 4; ++k) u[k] = fold_halves(v[k], v[k + 4]);
+    const float w0 = fold_rows(u[0], u[2]);
+    const float w1 = fold_rows(u[1], u[3]);
+    float z = (h3 ? w1 : w0) + dpp<kDppRowMirror>(h3 ? w0 : w1);
+    z += dpp<kDppRowHalfMirror>(z);
+    z += dpp<kDppXor2>(z);
     z += dpp<kDppXor1>(z);
-    return sum_xor32(sum_xor16(z));
+    return z;
+}
+
+// The value index whose total wave_reduce_scatter8 leaves in the calling
+// lane: the same exchanges applied to the indices, keeping the first operand
+// of every fold.
+__device__ __forceinline__ uint32_t scatter_term(bool h3)
+{
+    uint32_t id[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) id[k] = (uint32_t)k;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) swap_halves(id[k], id[k + 4]);
+    swap_rows(id[0], id[2]);
+    swap_rows(id[1], id[3]);
+    return h3 ? id[1] : id[0];
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
@@ -423,17 +509,38 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
     return v;
 }
 
-constexpr int kGradTerms = 9;   // dcolor x3, dmean2D x2, dconic x3, dopacity
+// Per-slot sums of the backward: dcolor x3, (Sx, Sy) for dmean2D,
+// (Sxx, Sxy, Syy) for dconic, dopacity.
+constexpr int kGradTerms = 9;
+
+// One staged Gaussian of the backward, 64 bytes.
+struct __align__(16) BwdSlot {
+    float4 g0;    // x, y, a', b'  (power in log2 units, as the forward)
+    float4 col;   // r, g, b, id bits
+    float4 con;   // conic A, B, C
+    float2 g1;    // c', opacity
+    float2 pad;
+};
+static_assert(sizeof(BwdSlot) == 64, "backward slot is 64 bytes");
 
 // Backward of one tile (one workgroup of FGS_BLOCK lanes).
+//
+// Per pixel the walk goes back to front from T_final like the reference
+// (antialias/cuda_rasterizer/backward.cu:441-557): T is recovered by dividing by
+// (1 - alpha), the colour accumulated behind the current Gaussian is carried
+// already dotted with dL/dpix (the reference's per-channel sum
+// (c - accum_rec) . dL/dpix is linear in accum_rec), and per (pixel,
+// Gaussian) the nine sums below are formed:
+//   dcolor_c += alpha T dL/dpix_c
+//   v = G dL/dalpha  ->  dopacity += v,  Sx += v dx, Sy += v dy,
+//                        Sxx += v dx dx, Sxy += v dx dy, Syy += v dy dy
+// The opacity, the conic, -(W/2, H/2) and -1/2 of
+//   dmean2D = -(W/2, H/2) o (A Sx + B Sy, B Sx + C Sy),  dconic = -1/2 o (Sxx, Sxy, Syy)
+// are applied once per (tile, Gaussian) when the sums are published.
 template <int BATCH>
 __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, const float* __restrict__ final_Ts,
-                                               const uint32_t* __restrict__ n_contrib,
-                                               const float* __restrict__ dL_dpix,
-                                               float* __restrict__ dL_dmean2D,
-                                               float* __restrict__ dL_dconic,
-                                               float* __restrict__ dL_dopacity,
-                                               float* __restrict__ dL_dcolors, int exact)
+                                               const uint32_t* __restrict__ vis_last,
+                                               const float* __restrict__ dL_dpix, int exact)
 {
     static_assert(BATCH <= FGS_BLOCK, "one staging lane per batch slot");
     const int tx = tile % a.gx, ty = tile / a.gx;
@@ -443,43 +550,49 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     const int px = tx * FGS_TILE + (tid & (FGS_TILE - 1));
     const int py = ty * FGS_TILE + (tid / FGS_TILE);
     const bool inside = px < a.W && py < a.H;
-    const float pfx = (float)px, pfy = (float)py;
+    float pfx = (float)px, pfy = (float)py;
+    asm volatile("" : "+v"(pfx), "+v"(pfy));
     const int HW = a.W * a.H;
     const int pix = a.W * py + px;
 
     const uint2 range = a.ranges[tile];
 
-    __shared__ float4 s_g0[BATCH];   // x, y, a', b'  (power in log2 units, as the forward)
-    __shared__ float2 s_g1[BATCH];   // c', opacity
-    __shared__ float4 s_con[BATCH];  // conic A, B, C
-    __shared__ float4 s_col[BATCH];  // r, g, b, id bits
-    __shared__ float s_g[kGradTerms][BATCH];
+    __shared__ BwdSlot s_slot[BATCH];
+    __shared__ float s_g[BATCH][kGradTerms];
     __shared__ uint32_t s_wmax[kStrips];
-    __shared__ uint16_t s_list[kStrips][BATCH];
+    __shared__ uint32_t s_list[kStrips][BATCH];
     __shared__ uint32_t s_wcnt[kStrips][kStrips];
 
     const float T_final = inside ? final_Ts[pix] : 0.f;
     float T = T_final;
-    const uint32_t last = inside ? n_contrib[pix] : 0u;
+    const uint32_t last = inside ? vis_last[pix] : 0u;   // visible-list positions below it
     const float dp0 = inside ? dL_dpix[pix] : 0.f;
     const float dp1 = inside ? dL_dpix[HW + pix] : 0.f;
     const float dp2 = inside ? dL_dpix[2 * HW + pix] : 0.f;
     const float bg_dot = a.bg[0] * dp0 + a.bg[1] * dp1 + a.bg[2] * dp2;
     const float tfb = -T_final * bg_dot;   // dL/dalpha gets -T_final / (1 - alpha) * (bg . dL/dpix)
-    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-    float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
+    // colour accumulated behind the current Gaussian and the colour of the
+    // last blended one, both dotted with dL/dpix
+    float acc_d = 0.f, last_d = 0.f, last_alpha = 0.f;
 
-    // positions at or past the largest n_contrib of the tile blend into no pixel
+    // positions at or past the largest vis_last of the tile blend into no pixel
     const uint32_t wmax = wave_max_u32(last);
     if (lane == 0) s_wmax[wave] = wmax;
     __syncthreads();
     uint32_t bmax = 0;
 #pragma unroll
     for (int w = 0; w < kStrips; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
-    const int n = (int)min(bmax, range.y - range.x);
+    const int n = (int)min(bmax, a.tile_vis[tile]);
     const int rounds = (n + BATCH - 1) / BATCH;
     const float ddelx_dx = 0.5f * (float)a.W;
     const float ddely_dy = 0.5f * (float)a.H;
+
+    // lane roles in the reduction: lanes 0, 8, ..., 56 publish the eight
+    // scattered sums, lane 1 the dopacity total
+    const bool h3 = (lane >> 3) & 1;
+    const bool pub_op = lane == 1;
+    const bool pub = (lane & 7) == 0 || pub_op;
+    const uint32_t term = pub_op ? 8u : scatter_term(h3);
 
     for (int r = 0; r < rounds; ++r) {
         __syncthreads();
@@ -487,160 +600,248 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         const int k = r * BATCH + q;
         uint32_t bits = 0u;
         if (q < BATCH) {
-            // back to front: position n-1-k of the tile's list.  Only the
-            // strips where the forward blended this pair take part, and a
-            // pair that blended nowhere is not even loaded.
-            if (k < n) bits = a.kv_strips[range.x + n - 1 - k];
-            if (bits != 0u) {
-                const uint32_t g = (uint32_t)a.pairs[range.x + n - 1 - k];
+            // back to front: position n-1-k of the tile's visible list (the
+            // Gaussians the forward blended somewhere in the tile).  Only the
+            // strips where it blended take part.
+            if (k < n) {
+                bits = a.kv_strips[range.x + n - 1 - k];
+                const uint32_t g = a.vis_ids[range.x + n - 1 - k];
                 const float2 m = a.means2D[g];
                 const float4 co = a.conic_opacity[g];
-                s_g0[q] = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
-                s_g1[q] = make_float2(-0.5f * kLog2e * co.z, co.w);
-                s_con[q] = co;
-                s_col[q] = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
-                                       __uint_as_float(g));
+                s_slot[q].g0 = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                s_slot[q].g1 = make_float2(-0.5f * kLog2e * co.z, co.w);
+                s_slot[q].con = co;
+                s_slot[q].col = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1],
+                                            a.colors[3 * g + 2], __uint_as_float(g));
             }
 #pragma unroll
-            for (int t = 0; t < kGradTerms; ++t) s_g[t][q] = 0.f;
+            for (int t = 0; t < kGradTerms; ++t) s_g[q][t] = 0.f;
         }
-        const int cnt = build_lists<BATCH>(bits, s_list, s_wcnt);
+        const int cnt = build_lists<BATCH, 0>(bits, s_list, s_wcnt);
+        // slot j holds visible position n-1-(r*BATCH+j), which blended into
+        // this pixel iff it lies below vis_last
+        const int thr = n - 1 - r * BATCH - (int)last;
         for (int i = 0; i < cnt; ++i) {
-            const int j = s_list[wave][i];
-            const uint32_t position = (uint32_t)(n - 1 - (r * BATCH + j));
+            const int j = (int)s_list[wave][i];
+            const BwdSlot* const sp = &s_slot[j];
             // alpha exactly as the forward computed it (same staged terms,
             // same operations), so both passes take the same decisions
-            const float4 g0 = s_g0[j];
-            const float2 g1 = s_g1[j];
+            const float4 g0 = sp->g0;
+            const float2 g1 = sp->g1;
             const float dx = g0.x - pfx, dy = g0.y - pfy;
             const float p2 = exponent2(g0, g1, dx, dy);
             const float G = __builtin_amdgcn_exp2f(fminf(p2, 0.0f));
             const float alpha_raw = g1.y * G;
             const float alpha = fminf(0.99f, alpha_raw);
-            const bool act = position < last && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+            const bool act = j > thr && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
             if (!__any(act)) continue;   // wave-uniform
             // every lane computes; inactive lanes contribute zeros and keep
             // their state (all terms stay finite: G <= 1, alpha <= 0.99)
-            const float4 col = s_col[j];
+            const float4 col = sp->col;
             const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
             const float Tn = T * inv;
-            const float a0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-            const float a1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-            const float a2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-            const float dLa = ((col.x - a0) * dp0 + (col.y - a1) * dp1 + (col.z - a2) * dp2) * Tn +
-                              inv * tfb;
+            const float c_d = col.x * dp0 + col.y * dp1 + col.z * dp2;
+            const float acc_n = acc_d + last_alpha * (last_d - acc_d);
+            const float dLa = (c_d - acc_n) * Tn + inv * tfb;
             const bool flows = act && !(exact && alpha_raw > 0.99f);
             const float dch = act ? alpha * Tn : 0.f;
+            const float vop = flows ? G * dLa : 0.f;
             float v[8];
             v[0] = dch * dp0;
             v[1] = dch * dp1;
             v[2] = dch * dp2;
-            // h = dL/dG * G.  dL/dmean2D = -(W/2, H/2) * (A Sx + B Sy, B Sx + C Sy)
-            // and dL/dconic = -1/2 (Sxx, Sxy, Syy) with Sx = sum h dx, ...:
-            // the conic and the constants are applied once per (tile,
-            // Gaussian) when the sums are published
-            const float vop = flows ? G * dLa : 0.f;
-            const float h = g1.y * vop;
-            const float hdx = h * dx, hdy = h * dy;
-            v[3] = hdx;
-            v[4] = hdy;
-            v[5] = hdx * dx;
-            v[6] = hdx * dy;
-            v[7] = hdy * dy;
+            const float vdx = vop * dx, vdy = vop * dy;
+            v[3] = vdx;
+            v[4] = vdy;
+            v[5] = vdx * dx;
+            v[6] = vdx * dy;
+            v[7] = vdy * dy;
             if (act) {
                 T = Tn;
-                acc0 = a0;
-                acc1 = a1;
-                acc2 = a2;
-                lc0 = col.x;
-                lc1 = col.y;
-                lc2 = col.z;
+                acc_d = acc_n;
+                last_d = c_d;
                 last_alpha = alpha;
             }
             // the waves meet on the same slot: LDS float atomics
-            const float z = wave_reduce_scatter8(v);
+            const float z = wave_reduce_scatter8(v, h3);
             const float zop = wave_total(vop);
-            if (lane < 16 && (lane & 1) == 0) atomicAdd(&s_g[lane >> 1][j], z);
-            if (lane == 16) atomicAdd(&s_g[8][j], zop);
+            if (pub) atomicAdd(&s_g[j][term], pub_op ? zop : z);
         }
         __syncthreads();
         if (bits != 0u) {
-            const uint32_t g = __float_as_uint(s_col[q].w);
+            // the pair's gradient in the layout of the gather: dcolor x3,
+            // dmean2D x2, dconic x3, dopacity, stored at its pre-sort position
+            const uint32_t g = __float_as_uint(s_slot[q].col.w);
+            const float o = s_slot[q].g1.y;
             float t[kGradTerms];
 #pragma unroll
-            for (int e = 0; e < kGradTerms; ++e) t[e] = s_g[e][q];
-            if (t[0] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 0], t[0]);
-            if (t[1] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 1], t[1]);
-            if (t[2] != 0.f) atomic_add_f32(&dL_dcolors[3 * g + 2], t[2]);
-            if (t[3] != 0.f || t[4] != 0.f) {
-                const float4 con = s_con[q];
-                atomic_add_f32(&dL_dmean2D[3 * g + 0], -ddelx_dx * (con.x * t[3] + con.y * t[4]));
-                atomic_add_f32(&dL_dmean2D[3 * g + 1], -ddely_dy * (con.y * t[3] + con.z * t[4]));
+            for (int e = 0; e < kGradTerms; ++e) t[e] = s_g[q][e];
+            const float4 con = s_slot[q].con;
+            const float sx = o * t[3], sy = o * t[4];
+            const float ho = -0.5f * o;
+            const uint4 rc = a.recs[g];
+            const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
+            const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
+            if (p < a.part_cap) {
+                float* d = a.part + (size_t)kGradTerms * p;
+                d[0] = t[0];
+                d[1] = t[1];
+                d[2] = t[2];
+                d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
+                d[4] = -ddely_dy * (con.y * sx + con.z * sy);
+                d[5] = ho * t[5];
+                d[6] = ho * t[6];
+                d[7] = ho * t[7];
+                d[8] = t[8];
+                a.mark[p] = 1;
             }
-            if (t[5] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 0], -0.5f * t[5]);
-            if (t[6] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 1], -0.5f * t[6]);
-            if (t[7] != 0.f) atomic_add_f32(&dL_dconic[3 * g + 2], -0.5f * t[7]);
-            if (t[8] != 0.f) atomic_add_f32(&dL_dopacity[g], t[8]);
         }
     }
 }
 
+// Gather of the per-pair gradients: slot s of the depth order sums the
+// marked entries of its Gaussian's pairs [offs[s], offs[s] + tiles_s[s]) and
+// stores the nine totals (plain stores: every Gaussian has one slot).
+// Segments of up to kGatherSolo pairs are summed by their own lane, longer
+// ones by the whole wave.
+constexpr uint32_t kGatherSolo = 16;
+
+__device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* __restrict__ dL_dmean2D,
+                                            float* __restrict__ dL_dconic, float* __restrict__ dL_dopacity,
+                                            float* __restrict__ dL_dcolors)
+{
+    dL_dcolors[3 * (size_t)g + 0] = t[0];
+    dL_dcolors[3 * (size_t)g + 1] = t[1];
+    dL_dcolors[3 * (size_t)g + 2] = t[2];
+    dL_dmean2D[3 * (size_t)g + 0] = t[3];
+    dL_dmean2D[3 * (size_t)g + 1] = t[4];
+    dL_dconic[3 * (size_t)g + 0] = t[5];
+    dL_dconic[3 * (size_t)g + 1] = t[6];
+    dL_dconic[3 * (size_t)g + 2] = t[7];
+    dL_dopacity[g] = t[8];
+}
+
+__global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float* __restrict__ part,
+                                                     const uint8_t* __restrict__ mark, size_t cap,
+                                                     float* __restrict__ dL_dmean2D,
+                                                     float* __restrict__ dL_dconic,
+                                                     float* __restrict__ dL_dopacity,
+                                                     float* __restrict__ dL_dcolors)
+{
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = s < gg.n;
+    uint32_t g = 0u, n = 0u, off = 0u;
+    if (valid) {
+        const uint64_t key = gg.vkeys[s];
+        g = gg.batch_P ? batch_elem(key, gg.batch_P) : (uint32_t)key;
+        n = gg.tiles_s[s];
+        off = gg.offs[s];
+    }
+    if (n > 0u && n <= kGatherSolo) {
+        float t[kGradTerms] = {};
+        bool any = false;
+        for (uint32_t k = 0; k < n; ++k) {
+            const size_t p = (size_t)off + k;
+            if (p < cap && mark[p]) {
+                any = true;
+#pragma unroll
+                for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
+            }
+        }
+        if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+    }
+    uint64_t big = __ballot(n > kGatherSolo);
+    const uint32_t lane = (uint32_t)lane_id();
+    while (big) {
+        const int src = __ffsll((long long)big) - 1;
+        big &= big - 1;
+        const uint32_t bg = __builtin_amdgcn_readlane(g, src);
+        const uint32_t bn = __builtin_amdgcn_readlane(n, src);
+        const uint32_t boff = __builtin_amdgcn_readlane(off, src);
+        float t[kGradTerms] = {};
+        for (uint32_t k = lane; k < bn; k += FGS_WAVE) {
+            const size_t p = (size_t)boff + k;
+            if (p < cap && mark[p]) {
+#pragma unroll
+                for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kGradTerms; ++e) t[e] = wave_total(t[e]);
+        if (lane == 0) store_grads(bg, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+    }
+}
+
+// Clear the marks, blend, gather.
+template <typename Launch>
+hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float* dL_dmean2D,
+                                float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s,
+                                Launch&& launch)
+{
+    hipError_t e = launch_fill_u32(a.mark, 0u, (a.part_cap + 3) / 4, s);
+    if (e != hipSuccess) return e;
+    launch();
+    if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
+    hipLaunchKernelGGL(k_grad_gather, dim3((unsigned)((gg.n + 255) / 256)), dim3(256), 0, s, gg, a.part,
+                       a.mark, a.part_cap, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+    return hipGetLastError();
+}
+
 template <int BATCH>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* final_Ts,
-                                                         const uint32_t* n_contrib, const float* dL_dpix,
-                                                         float* dL_dmean2D, float* dL_dconic,
-                                                         float* dL_dopacity, float* dL_dcolors, int exact)
+                                                         const uint32_t* vis_last, const float* dL_dpix,
+                                                         int exact)
 {
-    blend_bwd_tile<BATCH>(a, blockIdx.x, final_Ts, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity,
-                          dL_dcolors, exact);
+    blend_bwd_tile<BATCH>(a, blockIdx.x, final_Ts, vis_last, dL_dpix, exact);
 }
 
 template <int BATCH>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd_batch(BlendArgs common, const BatchView* views,
                                                                int V, const float* final_Ts,
-                                                               const uint32_t* n_contrib,
-                                                               const float* dL_dpix, float* dL_dmean2D,
-                                                               float* dL_dconic, float* dL_dopacity,
-                                                               float* dL_dcolors, int exact)
+                                                               const uint32_t* vis_last,
+                                                               const float* dL_dpix, int exact)
 {
     const int v = find_view(views, V, blockIdx.x);
     const BatchView& vw = views[v];
     const BlendArgs a = bind_blend(common, vw, v);
     const size_t pb = vw.pix_base;
-    blend_bwd_tile<BATCH>(a, (int)(blockIdx.x - vw.tile_base), final_Ts + pb, n_contrib + pb,
-                          dL_dpix + 3 * pb, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact);
+    blend_bwd_tile<BATCH>(a, (int)(blockIdx.x - vw.tile_base), final_Ts + pb, vis_last + pb,
+                          dL_dpix + 3 * pb, exact);
 }
 
-hipError_t launch_blend_bwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
-                                  const float* final_T, const uint32_t* n_contrib, const float* dL_dpix,
-                                  float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
-                                  float* dL_dcolors, int exact, hipStream_t s)
+hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg, const BatchView* views,
+                                  int V, int T_total, const float* final_T, const uint32_t* vis_last,
+                                  const float* dL_dpix, float* dL_dmean2D, float* dL_dconic,
+                                  float* dL_dopacity, float* dL_dcolors, int exact, hipStream_t s)
 {
     if (T_total == 0) return hipSuccess;
-#define FGS_BWD(BATCH)                                                                             \
+    return blend_bwd_and_gather(common, gg, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, s, [&] {
+#define FGS_BWD(BATCH)                                                                               \
     hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V, \
-                       final_T, n_contrib, dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact)
-    if (common.bwd_batch == 256) FGS_BWD(256);
-    else if (common.bwd_batch == 64) FGS_BWD(64);
-    else FGS_BWD(128);
+                       final_T, vis_last, dL_dpix, exact)
+        if (common.bwd_batch == 256) FGS_BWD(256);
+        else if (common.bwd_batch == 64) FGS_BWD(64);
+        else FGS_BWD(128);
 #undef FGS_BWD
-    return hipGetLastError();
+    });
 }
 
-hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint32_t* n_contrib,
-                            const float* dL_dpix, float* dL_dmean2D, float* dL_dconic,
-                            float* dL_dopacity, float* dL_dcolors, int exact, hipStream_t s)
+hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const float* final_T,
+                            const uint32_t* vis_last, const float* dL_dpix, float* dL_dmean2D,
+                            float* dL_dconic, float* dL_dopacity, float* dL_dcolors, int exact,
+                            hipStream_t s)
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-#define FGS_BWD(BATCH)                                                                            \
-    hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,  \
-                       dL_dpix, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, exact)
-    if (a.bwd_batch == 256) FGS_BWD(256);
-    else if (a.bwd_batch == 64) FGS_BWD(64);
-    else FGS_BWD(128);
+    return blend_bwd_and_gather(a, gg, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, s, [&] {
+#define FGS_BWD(BATCH)                                                                              \
+    hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, vis_last, dL_dpix, \
+                       exact)
+        if (a.bwd_batch == 256) FGS_BWD(256);
+        else if (a.bwd_batch == 64) FGS_BWD(64);
+        else FGS_BWD(128);
 #undef FGS_BWD
-    return hipGetLastError();
+    });
 }
 
 }  // namespace fgs

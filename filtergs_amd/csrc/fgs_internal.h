@@ -89,8 +89,22 @@ __device__ inline void bind_view(PreArgs& a, const BatchView& v)
 struct BlendArgs {
     const uint2* ranges;
     const uint64_t* pairs;     // sorted (tile << 32 | gaussian)
-    uint8_t* kv_strips;        // per kv pair: strips of the tile it blended into
-                               // (written by the forward, read by the backward)
+    // Written by the forward, read by the backward: per tile, the kv values
+    // that blended into at least one pixel, in blend order, from the tile's
+    // first kv position (vis_ids[ranges[t].x + i]); the strips of the tile
+    // each of them blended into (kv_strips, same positions); their count
+    // (tile_vis[t]).
+    uint32_t* vis_ids;
+    uint8_t* kv_strips;
+    uint32_t* tile_vis;
+    // Backward: the nine gradient sums of a (tile, Gaussian) pair are stored
+    // at part[9 p ..] with mark[p] = 1, p = the pair's position before the
+    // tile sort (recs[g].w, the Gaussian's first pair, plus the row-major
+    // offset of the tile in its rectangle); p < part_cap.
+    const uint4* recs;
+    float* part;
+    uint8_t* mark;
+    size_t part_cap;
     int W, H, gx, gy;
     const float2* means2D;
     const float* colors;       // [P,3]
@@ -124,8 +138,9 @@ hipError_t launch_scan_tiles(const uint32_t* tiles_s, uint32_t* offs, int P, uin
 size_t radix_hist_words(size_t n);
 hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
                           uint32_t* hist, uint32_t* partials, hipStream_t s);
+// duplicate also records in recs[g].w the position of the Gaussian's first pair
 hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                            int P, int gx, uint64_t* pairs, hipStream_t s);
+                            int P, int gx, uint64_t* pairs, uint4* recs, hipStream_t s);
 hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T, hipStream_t s);
 
 // batched binning (depth key (v << 58) | (depth_bits << 26) | i)
@@ -133,31 +148,50 @@ constexpr int kBatchViewShift = 58;
 constexpr int kBatchIdxBits = 26;
 constexpr uint64_t kBatchIdxMask = (1ull << kBatchIdxBits) - 1;
 constexpr int kBatchMaxViews = 64;
+__device__ __forceinline__ uint32_t batch_elem(uint64_t key, int P)
+{
+    return (uint32_t)(key >> kBatchViewShift) * (uint32_t)P + (uint32_t)(key & kBatchIdxMask);
+}
 hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n, int P,
                                        uint32_t* tiles_s, uint2* rects_s, hipStream_t s);
 hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s);
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                   size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
-                                  const unsigned long long* K, hipStream_t s);
+                                  const unsigned long long* K, uint4* recs, hipStream_t s);
 hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, int P, const BatchView* views,
                                uint2* ranges, int T, hipStream_t s);
 
 // blend
-hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib,
+// vis_last (per pixel): 1 + the index in the tile's visible list of the last
+// Gaussian blended into the pixel (written by the forward, read by the backward)
+hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
                             float* out_color, int* out_point_id, float* out_weight_pixel,
                             float* point_weight, float* kv_sum, uint32_t* kv_cnt, hipStream_t s);
-hipError_t launch_blend_bwd(const BlendArgs& a, const float* final_T, const uint32_t* n_contrib,
-                            const float* dL_dpix, float* dL_dmean2D, float* dL_dconic,
-                            float* dL_dopacity, float* dL_dcolors, int exact, hipStream_t s);
+// The per-Gaussian sums of the backward's per-pair partials, gathered in
+// depth order from the Gaussians' pre-sort pair segments (slot s: key
+// vkeys[s], tiles_s[s] pairs from offs[s]; batch_P > 0: keys of a batch).
+struct GradGather {
+    const uint64_t* vkeys;
+    const uint32_t* tiles_s;
+    const uint32_t* offs;
+    size_t n;
+    int batch_P;
+};
+hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const float* final_T,
+                            const uint32_t* vis_last, const float* dL_dpix, float* dL_dmean2D,
+                            float* dL_dconic, float* dL_dopacity, float* dL_dcolors, int exact,
+                            hipStream_t s);
 // batched: common.ranges / pairs / kv_strips / means2D / colors / conic_opacity
 // cover the whole batch; per-pixel arrays are the views' blocks concatenated
 hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
-                                  float* final_T, uint32_t* n_contrib, float* out_color,
+                                  float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
+                                  float* out_color,
                                   int* out_point_id, float* out_weight_pixel, float* point_weight,
                                   hipStream_t s);
-hipError_t launch_blend_bwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
-                                  const float* final_T, const uint32_t* n_contrib, const float* dL_dpix,
+hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg, const BatchView* views,
+                                  int V, int T_total, const float* final_T, const uint32_t* vis_last,
+                                  const float* dL_dpix,
                                   float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
                                   float* dL_dcolors, int exact, hipStream_t s);
 

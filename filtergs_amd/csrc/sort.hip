@@ -324,22 +324,23 @@ __device__ __forceinline__ void dup_rect(bool valid, uint2 rc, size_t off, uint3
 __global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ vkeys,
                                                    const uint32_t* __restrict__ offs,
                                                    const uint2* __restrict__ rects_s, int P, int gx,
-                                                   uint64_t* __restrict__ pairs)
+                                                   uint64_t* __restrict__ pairs, uint4* __restrict__ recs)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = s < P;
     const uint2 rc = valid ? rects_s[s] : make_uint2(0u, 0u);
     const uint32_t g = valid ? (uint32_t)vkeys[s] : 0u;
     const size_t off = valid ? offs[s] : 0u;
+    if (valid && rc.x != rc.y) recs[g].w = (uint32_t)off;
     dup_rect(valid, rc, off, g, (uint32_t)gx, 0u, pairs, ~(size_t)0);
 }
 
 hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                            int P, int gx, uint64_t* pairs, hipStream_t s)
+                            int P, int gx, uint64_t* pairs, uint4* recs, hipStream_t s)
 {
     if (P == 0) return hipSuccess;
     hipLaunchKernelGGL(k_duplicate, dim3((P + 255) / 256), dim3(256), 0, s, vkeys, offs, rects_s, P,
-                       gx, pairs);
+                       gx, pairs, recs);
     return hipGetLastError();
 }
 
@@ -371,11 +372,6 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
 // so (view, t) ranges stay contiguous and the sort needs only the bits of the
 // largest view's tile count.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t batch_elem(uint64_t key, int P)
-{
-    return (uint32_t)(key >> kBatchViewShift) * (uint32_t)P + (uint32_t)(key & kBatchIdxMask);
-}
-
 __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __restrict__ vkeys,
                                                               const uint4* __restrict__ recs, size_t n,
                                                               int P, uint32_t* __restrict__ tiles_s,
@@ -411,7 +407,8 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
                                                          const uint32_t* __restrict__ offs,
                                                          const uint2* __restrict__ rects_s, size_t n,
                                                          int P, const BatchView* __restrict__ views,
-                                                         uint64_t* __restrict__ pairs, size_t cap)
+                                                         uint64_t* __restrict__ pairs, size_t cap,
+                                                         uint4* __restrict__ recs)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = s < n;
@@ -420,7 +417,9 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
     const uint32_t v = (uint32_t)(key >> kBatchViewShift);
     const uint32_t gx = valid ? (uint32_t)views[v].gx : 0u;
     const size_t off = valid ? offs[s] : 0u;
-    dup_rect(valid, rc, off, batch_elem(key, P), gx, 0u, pairs, cap);
+    const uint32_t e = batch_elem(key, P);
+    if (valid && rc.x != rc.y) recs[e].w = (uint32_t)off;
+    dup_rect(valid, rc, off, e, gx, 0u, pairs, cap);
 }
 
 // Slots [K, cap) get all-ones keys, which sort after every real tile.
@@ -435,7 +434,7 @@ __global__ __launch_bounds__(256) void k_fill_tail(uint64_t* __restrict__ pairs,
 
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                   size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
-                                  const unsigned long long* K, hipStream_t s)
+                                  const unsigned long long* K, uint4* recs, hipStream_t s)
 {
     if (cap == 0) return hipSuccess;
     size_t blocks = div_up(cap, 256);
@@ -443,7 +442,7 @@ hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, c
     hipLaunchKernelGGL(k_fill_tail, dim3((unsigned)blocks), dim3(256), 0, s, pairs, K, cap);
     if (n == 0) return hipGetLastError();
     hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys, offs,
-                       rects_s, n, P, views, pairs, cap);
+                       rects_s, n, P, views, pairs, cap, recs);
     return hipGetLastError();
 }
 
