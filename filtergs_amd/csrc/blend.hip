@@ -679,21 +679,38 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
             const float4 con = s_slot[q].con;
             const float sx = o * t[3], sy = o * t[4];
             const float ho = -0.5f * o;
+            float d[kGradTerms];
+            d[0] = t[0];
+            d[1] = t[1];
+            d[2] = t[2];
+            d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
+            d[4] = -ddely_dy * (con.y * sx + con.z * sy);
+            d[5] = ho * t[5];
+            d[6] = ho * t[6];
+            d[7] = ho * t[7];
+            d[8] = t[8];
             const uint4 rc = a.recs[g];
-            const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
-            const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
-            if (p < a.part_cap) {
-                float* d = a.part + (size_t)kGradTerms * p;
-                d[0] = t[0];
-                d[1] = t[1];
-                d[2] = t[2];
-                d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
-                d[4] = -ddely_dy * (con.y * sx + con.z * sy);
-                d[5] = ho * t[5];
-                d[6] = ho * t[6];
-                d[7] = ho * t[7];
-                d[8] = t[8];
-                a.mark[p] = 1;
+            if (rc.z <= kGatherSolo) {
+                // small rectangle: the gather sums the pairs of the Gaussian
+                const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
+                const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
+                if (p < a.part_cap) {
+                    float* dst = a.part + (size_t)kGradTerms * p;
+#pragma unroll
+                    for (int e = 0; e < kGradTerms; ++e) dst[e] = d[e];
+                    a.mark[p] = 1;
+                }
+            } else {
+                // large rectangle: many tiles, add directly
+                if (d[0] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 0], d[0]);
+                if (d[1] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 1], d[1]);
+                if (d[2] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 2], d[2]);
+                if (d[3] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 0], d[3]);
+                if (d[4] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 1], d[4]);
+                if (d[5] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 0], d[5]);
+                if (d[6] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 1], d[6]);
+                if (d[7] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 2], d[7]);
+                if (d[8] != 0.f) atomic_add_f32(&a.dL_dopacity[g], d[8]);
             }
         }
     }
@@ -702,9 +719,8 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
 // Gather of the per-pair gradients: slot s of the depth order sums the
 // marked entries of its Gaussian's pairs [offs[s], offs[s] + tiles_s[s]) and
 // stores the nine totals (plain stores: every Gaussian has one slot).
-// Segments of up to kGatherSolo pairs are summed by their own lane, longer
-// ones by the whole wave.
-constexpr uint32_t kGatherSolo = 16;
+// Only Gaussians of up to kGatherSolo tiles take this path; each is summed by
+// its own lane.
 
 __device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* __restrict__ dL_dmean2D,
                                             float* __restrict__ dL_dconic, float* __restrict__ dL_dopacity,
@@ -750,29 +766,20 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
         }
         if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
     }
-    uint64_t big = __ballot(n > kGatherSolo);
-    const uint32_t lane = (uint32_t)lane_id();
-    while (big) {
-        const int src = __ffsll((long long)big) - 1;
-        big &= big - 1;
-        const uint32_t bg = __builtin_amdgcn_readlane(g, src);
-        const uint32_t bn = __builtin_amdgcn_readlane(n, src);
-        const uint32_t boff = __builtin_amdgcn_readlane(off, src);
-        float t[kGradTerms] = {};
-        for (uint32_t k = lane; k < bn; k += FGS_WAVE) {
-            const size_t p = (size_t)boff + k;
-            if (p < cap && mark[p]) {
-#pragma unroll
-                for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < kGradTerms; ++e) t[e] = wave_total(t[e]);
-        if (lane == 0) store_grads(bg, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
-    }
 }
 
-// Clear the marks, blend, gather.
+__host__ inline BlendArgs a_out(const BlendArgs& a, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
+                                float* dL_dcolors)
+{
+    BlendArgs b = a;
+    b.dL_dmean2D = dL_dmean2D;
+    b.dL_dconic = dL_dconic;
+    b.dL_dopacity = dL_dopacity;
+    b.dL_dcolors = dL_dcolors;
+    return b;
+}
+
+// Clear the marks, blend (with the outputs bound for large Gaussians), gather.
 template <typename Launch>
 hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float* dL_dmean2D,
                                 float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s,
@@ -780,7 +787,7 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
 {
     hipError_t e = launch_fill_u32(a.mark, 0u, (a.part_cap + 3) / 4, s);
     if (e != hipSuccess) return e;
-    launch();
+    launch(a_out(a, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors));
     if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
     hipLaunchKernelGGL(k_grad_gather, dim3((unsigned)((gg.n + 255) / 256)), dim3(256), 0, s, gg, a.part,
                        a.mark, a.part_cap, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
@@ -815,12 +822,13 @@ hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg,
                                   float* dL_dopacity, float* dL_dcolors, int exact, hipStream_t s)
 {
     if (T_total == 0) return hipSuccess;
-    return blend_bwd_and_gather(common, gg, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, s, [&] {
+    return blend_bwd_and_gather(common, gg, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, s,
+                                [&](const BlendArgs& b) {
 #define FGS_BWD(BATCH)                                                                               \
-    hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V, \
+    hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, b, views, V, \
                        final_T, vis_last, dL_dpix, exact)
-        if (common.bwd_batch == 256) FGS_BWD(256);
-        else if (common.bwd_batch == 64) FGS_BWD(64);
+        if (b.bwd_batch == 256) FGS_BWD(256);
+        else if (b.bwd_batch == 64) FGS_BWD(64);
         else FGS_BWD(128);
 #undef FGS_BWD
     });
@@ -833,12 +841,13 @@ hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const floa
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-    return blend_bwd_and_gather(a, gg, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, s, [&] {
+    return blend_bwd_and_gather(a, gg, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, s,
+                                [&](const BlendArgs& b) {
 #define FGS_BWD(BATCH)                                                                              \
-    hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, vis_last, dL_dpix, \
+    hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, b, final_T, vis_last, dL_dpix, \
                        exact)
-        if (a.bwd_batch == 256) FGS_BWD(256);
-        else if (a.bwd_batch == 64) FGS_BWD(64);
+        if (b.bwd_batch == 256) FGS_BWD(256);
+        else if (b.bwd_batch == 64) FGS_BWD(64);
         else FGS_BWD(128);
 #undef FGS_BWD
     });

@@ -105,6 +105,12 @@ struct BlendArgs {
     float* part;
     uint8_t* mark;
     size_t part_cap;
+    // Gaussians of more than kGatherSolo tiles skip the partials and add
+    // their per-tile sums into the outputs with float atomics
+    float* dL_dmean2D;
+    float* dL_dconic;
+    float* dL_dopacity;
+    float* dL_dcolors;
     int W, H, gx, gy;
     const float2* means2D;
     const float* colors;       // [P,3]
@@ -171,6 +177,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 // The per-Gaussian sums of the backward's per-pair partials, gathered in
 // depth order from the Gaussians' pre-sort pair segments (slot s: key
 // vkeys[s], tiles_s[s] pairs from offs[s]; batch_P > 0: keys of a batch).
+constexpr uint32_t kGatherSolo = 16;
 struct GradGather {
     const uint64_t* vkeys;
     const uint32_t* tiles_s;
