@@ -13,15 +13,16 @@
 // where alpha >= 1/255 (the ellipse q <= 2 ln(255 o) of the conic, slightly
 // inflated) touches the strip.  Per slot and strip the largest T*alpha is
 // kept in LDS; after the batch it gives point_weight (one float-as-uint
-// atomicMax per (tile, Gaussian): exact and deterministic) and the pair's
-// strip mask -- the strips where the pair actually blended -- stored next to
-// the sorted pairs.
-// Backward: the lists come from those masks, so every listed slot blends into
-// at least one pixel of the strip and pairs that blended nowhere are never
-// loaded.  The nine per-pixel gradient terms of each Gaussian are reduced
-// across the wave (a DPP reduce-scatter inside 16-lane rows, then two lane
-// swaps), added over the waves with LDS float atomics, and published with one
-// global atomic add per term per (tile, Gaussian).
+// atomicMax per (tile, Gaussian): exact and deterministic) and the strip mask
+// of the pair -- the strips where it actually blended.  The pairs that
+// blended anywhere are appended to the tile's visible list (id + mask).
+// Backward: walks the visible list back to front; the wave lists come from
+// the masks, so every listed slot blends into at least one pixel of the
+// strip.  The nine per-pixel gradient terms of each Gaussian are reduced
+// across the wave (lane swaps across halves and rows, then DPP inside rows),
+// added over the waves with LDS float atomics, and published per (tile,
+// Gaussian): as partial sums gathered per Gaussian afterwards for Gaussians
+// of at most kGatherSolo tiles, with global float atomics for larger ones.
 #include "fgs_device.h"
 #include "fgs_internal.h"
 
@@ -745,15 +746,12 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                                                      float* __restrict__ dL_dcolors)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = s < gg.n;
-    uint32_t g = 0u, n = 0u, off = 0u;
-    if (valid) {
-        const uint64_t key = gg.vkeys[s];
-        g = gg.batch_P ? batch_elem(key, gg.batch_P) : (uint32_t)key;
-        n = gg.tiles_s[s];
-        off = gg.offs[s];
-    }
+    if (s >= gg.n) return;
+    const uint32_t n = gg.tiles_s[s];
     if (n > 0u && n <= kGatherSolo) {
+        const uint64_t key = gg.vkeys[s];
+        const uint32_t g = gg.batch_P ? batch_elem(key, gg.batch_P) : (uint32_t)key;
+        const uint32_t off = gg.offs[s];
         float t[kGradTerms] = {};
         bool any = false;
         for (uint32_t k = 0; k < n; ++k) {
