@@ -106,6 +106,10 @@ _SIGS = {
     "fgs_forward_render": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),
+    "fgs_forward_render_stats": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_void_p, ctypes.c_float, c_void_p]),
+    "fgs_stats_words": (c_size_t, [c_int32, c_int32]),
     "fgs_read_sorted": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
                                   c_void_p]),
     "fgs_image_ranges": (c_void_p, [c_void_p, c_int32, c_int32]),

@@ -419,10 +419,27 @@ int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, v
     return sync_check(s, st, "geometry");
 }
 
+size_t fgs_stats_words(int32_t W, int32_t H)
+{
+    const size_t T = (size_t)((W + 15) / 16) * (size_t)((H + 15) / 16);
+    return fgs::stats_words(T);
+}
+
 int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
                        void* geom, int64_t K, void* binning, void* image, float* out_color,
                        int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
                        float* kv_alpha_sum, uint32_t* kv_alpha_count, void* stream)
+{
+    return fgs_forward_render_stats(s, g, radii, geom, K, binning, image, out_color, out_point_id,
+                                    out_weight_pixel, out_point_weight, kv_alpha_sum, kv_alpha_count,
+                                    nullptr, 0.f, stream);
+}
+
+int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                             void* geom, int64_t K, void* binning, void* image, float* out_color,
+                             int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                             float* kv_alpha_sum, uint32_t* kv_alpha_count, double* stats,
+                             float kpc_used_thresh, void* stream)
 {
     int rc = validate(s, g);
     if (rc) return rc;
@@ -478,6 +495,15 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
         ProfScope ps(ST_RANGES, st);
         if ((rc = check(fgs::launch_ranges(b.pairs, (size_t)K, im.ranges, T, st), "ranges"))) return rc;
     }
+    if (stats) {
+        if (!(kpc_used_thresh > 0.f)) return fail(FGS_E_INVALID, "KPC used threshold must be > 0%s");
+        // the u64 counters after the per-tile arrays start at zero
+        double* cnt = stats + fgs::kStatsHead + 3 * (size_t)T;
+        if ((rc = check(fgs::launch_fill_u32(cnt, 0u, 2 * (2 + fgs::kKpcBins), st), "stats clear")))
+            return rc;
+        b.stats = stats;
+        b.stats_thresh = kpc_used_thresh;
+    }
     {
         ProfScope ps(ST_BLEND_FWD, st);
         if ((rc = check(fgs::launch_blend_fwd(b, im.final_T, im.n_contrib, im.vis_last, out_color, out_point_id,
@@ -485,6 +511,8 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
                                               kv_alpha_count, st), "blend")))
             return rc;
     }
+    if (stats && (rc = check(fgs::launch_stats_finalize(stats, T, g->P, K, st), "stats")))
+        return rc;
     return sync_check(s, st, "render");
 }
 

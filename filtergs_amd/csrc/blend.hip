@@ -178,8 +178,16 @@ struct __align__(16) FwdSlot {
 static_assert(sizeof(FwdSlot) == 64, "forward slot is 64 bytes");
 constexpr int kFwdSlotShift = 6;
 
-// Forward of one tile (one workgroup of FGS_BLOCK lanes).
-template <int BATCH>
+// KPC histogram bin edges (rasterizer_impl.cu:60-72): bin b holds sums in
+// (edge[b-1], edge[b]]; sums above the last edge are not counted.
+__constant__ float kKpcEdges[kKpcBins] = { 0.01f, 0.05f, 0.10f, 0.15f, 0.20f, 0.25f,
+                                           0.30f, 0.35f, 0.40f, 0.45f, 0.50f };
+
+// Forward of one tile (one workgroup of FGS_BLOCK lanes).  STATS: also the
+// tile's share of RenderStats (the host loops of rasterizer_impl.cu:479-556):
+// per kv pair of the tile its T*alpha sum and contribution count, binned and
+// summed without leaving the chip.
+template <int BATCH, bool STATS>
 __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, float* __restrict__ final_T,
                                                uint32_t* __restrict__ n_contrib,
                                                uint32_t* __restrict__ vis_last,
@@ -228,6 +236,14 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     uint32_t vis_n = 0;        // entries appended so far (block-uniform)
     uint32_t vlast = 0;        // 1 + list index of the pixel's last blend
     int last_slot = -1;
+    // statistics: per-lane sum of the pair sums (reduced in a fixed order at
+    // the end), used / unused counts and the histogram in LDS
+    double st_sum = 0.0;
+    int r_end = rounds;        // rounds staged (the tile stops early when saturated)
+    __shared__ uint32_t s_st[2 + kKpcBins];   // used in tile, unused, bins
+    if (STATS) {
+        if (tid < 2 + kKpcBins) s_st[tid] = 0u;
+    }
 
     for (int r = 0; r < rounds; ++r) {
         const int num_done = __syncthreads_count(done);
@@ -235,7 +251,10 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
             vlast = s_vidx[last_slot] + 1u;
             last_slot = -1;
         }
-        if (num_done == FGS_BLOCK) break;
+        if (num_done == FGS_BLOCK) {
+            r_end = r;
+            break;
+        }
         const int q = tid;
         const int k = r * BATCH + q;
         uint32_t bits = 0u;
@@ -259,36 +278,80 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
         char* const slots = reinterpret_cast<char*>(s_slot);
         // the next entry of the list is read one iteration ahead
         uint32_t next = s_list[wave][0];
-        for (int i = 0; !done && i < cnt; ++i) {
-            FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
-            next = s_list[wave][min(i + 1, BATCH - 1)];
-            const float4 g0 = sp->g0;
-            const float2 g1 = sp->g1;
-            const float4 col = sp->col;
-            const float dx = g0.x - pfx, dy = g0.y - pfy;
-            const float p2 = exponent2(g0, g1, dx, dy);
-            const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
-            const float test_T = T * (1.0f - alpha);
-            const bool pass = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
-            const bool stop = pass && test_T < 0.0001f;
-            done = done || stop;
-            if (pass && !stop) {
-                const float w = alpha * T;
-                C0 = fmaf(col.x, w, C0);
-                C1 = fmaf(col.y, w, C1);
-                C2 = fmaf(col.z, w, C2);
-                if (w > wmax) {
-                    wmax = w;
-                    max_id = (int)__float_as_uint(col.w);
+        if constexpr (STATS) {
+            // The whole wave walks the list (until all of its pixels are
+            // done), so the per-slot statistics are wave sums: one LDS atomic
+            // per wave instead of one per blending lane.
+            for (int i = 0; i < cnt; ++i) {
+                if (__all(done)) break;
+                FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
+                next = s_list[wave][min(i + 1, BATCH - 1)];
+                const float4 g0 = sp->g0;
+                const float2 g1 = sp->g1;
+                const float4 col = sp->col;
+                const float dx = g0.x - pfx, dy = g0.y - pfy;
+                const float p2 = exponent2(g0, g1, dx, dy);
+                const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+                const float test_T = T * (1.0f - alpha);
+                const bool pass = !done && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                const bool stop = pass && test_T < 0.0001f;
+                const bool blend = pass && !stop;
+                done = done || stop;
+                const float w = blend ? alpha * T : 0.f;
+                const uint64_t bl = __ballot(blend);
+                if (bl) {   // wave-uniform
+                    const float ws = wave_total(w);
+                    if (lane_id() == 0) {
+                        atomicAdd(&sp->kv, ws);
+                        atomicAdd(&sp->kc, (uint32_t)__popcll(bl));
+                    }
                 }
-                atomicMax(&sp->w[wave], __float_as_uint(w));
-                if (kv_sum) {
-                    atomicAdd(&sp->kv, w);
-                    atomicAdd(&sp->kc, 1u);
+                if (blend) {
+                    C0 = fmaf(col.x, w, C0);
+                    C1 = fmaf(col.y, w, C1);
+                    C2 = fmaf(col.z, w, C2);
+                    if (w > wmax) {
+                        wmax = w;
+                        max_id = (int)__float_as_uint(col.w);
+                    }
+                    atomicMax(&sp->w[wave], __float_as_uint(w));
+                    T = test_T;
+                    last_slot = (int)(sp - s_slot);
+                    last = (uint32_t)(r * BATCH + last_slot + 1);
                 }
-                T = test_T;
-                last_slot = (int)(sp - s_slot);
-                last = (uint32_t)(r * BATCH + last_slot + 1);
+            }
+        } else {
+            for (int i = 0; !done && i < cnt; ++i) {
+                FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
+                next = s_list[wave][min(i + 1, BATCH - 1)];
+                const float4 g0 = sp->g0;
+                const float2 g1 = sp->g1;
+                const float4 col = sp->col;
+                const float dx = g0.x - pfx, dy = g0.y - pfy;
+                const float p2 = exponent2(g0, g1, dx, dy);
+                const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+                const float test_T = T * (1.0f - alpha);
+                const bool pass = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                const bool stop = pass && test_T < 0.0001f;
+                done = done || stop;
+                if (pass && !stop) {
+                    const float w = alpha * T;
+                    C0 = fmaf(col.x, w, C0);
+                    C1 = fmaf(col.y, w, C1);
+                    C2 = fmaf(col.z, w, C2);
+                    if (w > wmax) {
+                        wmax = w;
+                        max_id = (int)__float_as_uint(col.w);
+                    }
+                    atomicMax(&sp->w[wave], __float_as_uint(w));
+                    if (kv_sum) {
+                        atomicAdd(&sp->kv, w);
+                        atomicAdd(&sp->kc, 1u);
+                    }
+                    T = test_T;
+                    last_slot = (int)(sp - s_slot);
+                    last = (uint32_t)(r * BATCH + last_slot + 1);
+                }
             }
         }
         __syncthreads();
@@ -307,6 +370,20 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                 // each kv pair belongs to exactly one tile: plain stores
                 kv_sum[range.x + k] = s_slot[q].kv;
                 kv_cnt[range.x + k] = s_slot[q].kc;
+            }
+            if (STATS) {
+                const float kv = s_slot[q].kv;
+                const uint32_t kc = s_slot[q].kc;
+                st_sum += (double)kv;
+                if (kc > 0u && kv >= a.stats_thresh) atomicAdd(&s_st[0], 1u);
+                if (kc == 0u) {
+                    atomicAdd(&s_st[1], 1u);
+                } else {
+                    int b = 0;
+#pragma unroll
+                    for (int e = 0; e < kKpcBins; ++e) b += kv > kKpcEdges[e] ? 1 : 0;
+                    if (b < kKpcBins) atomicAdd(&s_st[2 + b], 1u);
+                }
             }
         }
         // append the round's visible slots in slot order
@@ -331,6 +408,36 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     __syncthreads();
     if (last_slot >= 0) vlast = s_vidx[last_slot] + 1u;
     if (tid == 0) a.tile_vis[tile] = vis_n;
+    if (STATS) {
+        // fixed-order reduction of the tile sum (the same bits on every run)
+        __shared__ double s_red[FGS_BLOCK];
+        s_red[tid] = st_sum;
+        __syncthreads();
+#pragma unroll
+        for (int h = FGS_BLOCK / 2; h > 0; h >>= 1) {
+            if (tid < h) s_red[tid] += s_red[tid + h];
+            __syncthreads();
+        }
+        if (tid == 0) {
+            const int T = a.gx * a.gy;
+            double* st = a.stats;
+            const double nk = (double)n;
+            // pairs of rounds never staged (all pixels saturated) blended nowhere
+            const uint32_t unstaged = (uint32_t)max(0, n - r_end * BATCH);
+            st[kStatsHead + tile] = n > 0 ? (double)s_st[0] / nk : 0.0;           // tile_ratio
+            st[kStatsHead + T + tile] = (double)n;                                 // tile_total
+            st[kStatsHead + 2 * T + tile] = n > 0 ? s_red[0] / nk : 0.0;          // tile_avg
+            unsigned long long* cnt = reinterpret_cast<unsigned long long*>(st + kStatsHead + 3 * T);
+            uint32_t used = 0;
+#pragma unroll
+            for (int b = 0; b < kKpcBins; ++b) {
+                used += s_st[2 + b];
+                if (s_st[2 + b]) atomicAdd(&cnt[2 + b], (unsigned long long)s_st[2 + b]);
+            }
+            if (used) atomicAdd(&cnt[0], (unsigned long long)used);
+            if (s_st[1] + unstaged) atomicAdd(&cnt[1], (unsigned long long)(s_st[1] + unstaged));
+        }
+    }
 
     if (inside) {
         const int pix = a.W * py + px;
@@ -346,15 +453,15 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     }
 }
 
-template <int BATCH>
+template <int BATCH, bool STATS>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* final_T, uint32_t* n_contrib,
                                                          uint32_t* vis_last, float* out_color,
                                                          int* out_point_id, float* out_weight_pixel,
                                                          float* point_weight, float* kv_sum,
                                                          uint32_t* kv_cnt)
 {
-    blend_fwd_tile<BATCH>(a, blockIdx.x, final_T, n_contrib, vis_last, out_color, out_point_id,
-                          out_weight_pixel, point_weight, kv_sum, kv_cnt);
+    blend_fwd_tile<BATCH, STATS>(a, blockIdx.x, final_T, n_contrib, vis_last, out_color, out_point_id,
+                                 out_weight_pixel, point_weight, kv_sum, kv_cnt);
 }
 
 // View of a batch that owns workgroup b (views sorted by tile_base).
@@ -396,7 +503,7 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd_batch(BlendArgs common,
     const BatchView& vw = views[v];
     const BlendArgs a = bind_blend(common, vw, v);
     const size_t pb = vw.pix_base;
-    blend_fwd_tile<BATCH>(a, (int)(blockIdx.x - vw.tile_base), final_T + pb, n_contrib + pb,
+    blend_fwd_tile<BATCH, false>(a, (int)(blockIdx.x - vw.tile_base), final_T + pb, n_contrib + pb,
                           vis_last + pb, out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb,
                           point_weight, nullptr, nullptr);
 }
@@ -424,12 +531,57 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 {
     const int T = a.gx * a.gy;
     if (T == 0) return hipSuccess;
-    if (a.batch == 128)
-        hipLaunchKernelGGL(k_blend_fwd<128>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
-                           vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
-    else
-        hipLaunchKernelGGL(k_blend_fwd<256>, dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib,
-                           vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
+#define FGS_FWD(BATCH, STATS)                                                                            \
+    hipLaunchKernelGGL((k_blend_fwd<BATCH, STATS>), dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib, \
+                       vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt)
+    if (a.stats) {
+        if (a.batch == 128) FGS_FWD(128, true);
+        else FGS_FWD(256, true);
+    } else {
+        if (a.batch == 128) FGS_FWD(128, false);
+        else FGS_FWD(256, false);
+    }
+#undef FGS_FWD
+    return hipGetLastError();
+}
+
+// The packed head of RenderStats from the per-tile values and counters the
+// blend left in the buffer; bar_g is summed over the tiles in a fixed order.
+__global__ __launch_bounds__(256) void k_stats_finalize(double* st, int T, int P, int64_t K)
+{
+    __shared__ double s_red[256];
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    for (int t = tid; t < T; t += 256) acc += st[kStatsHead + 2 * (size_t)T + t];
+    s_red[tid] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) s_red[tid] += s_red[tid + h];
+        __syncthreads();
+    }
+    const unsigned long long* cnt =
+        reinterpret_cast<const unsigned long long*>(st + kStatsHead + 3 * (size_t)T);
+    if (tid < 9 + 2 * kStatsMaxBins) {
+        double v = 0.0;
+        if (tid == 0) v = T > 0 ? s_red[0] / (double)T : 0.0;
+        else if (tid == 1) v = P > 0 ? (double)K / (double)P : 0.0;
+        else if (tid == 2) v = T > 0 ? (double)K / (double)T : 0.0;
+        else if (tid == 3) v = (double)K;
+        else if (tid == 4) v = (double)T;
+        else if (tid == 5) v = (double)P;
+        else if (tid == 6) v = (double)cnt[0];
+        else if (tid == 7) v = (double)cnt[1];
+        else if (tid == 8) v = (double)kKpcBins;
+        else if (tid < 9 + kKpcBins) v = (double)kKpcEdges[tid - 9];
+        else if (tid >= 9 + kStatsMaxBins && tid < 9 + kStatsMaxBins + kKpcBins)
+            v = (double)cnt[2 + (tid - 9 - kStatsMaxBins)];
+        st[tid] = v;
+    }
+}
+
+hipError_t launch_stats_finalize(double* stats, int T, int P, int64_t K, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stats_finalize, dim3(1), dim3(256), 0, s, stats, T, P, K);
     return hipGetLastError();
 }
 

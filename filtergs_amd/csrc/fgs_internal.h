@@ -105,6 +105,9 @@ struct BlendArgs {
     float* part;
     uint8_t* mark;
     size_t part_cap;
+    // Forward statistics (RenderStats, device layout of kStats*): null = off
+    double* stats;
+    float stats_thresh;        // KPC used threshold
     // Gaussians of more than kGatherSolo tiles skip the partials and add
     // their per-tile sums into the outputs with float atomics
     float* dL_dmean2D;
@@ -178,6 +181,18 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 // depth order from the Gaussians' pre-sort pair segments (slot s: key
 // vkeys[s], tiles_s[s] pairs from offs[s]; batch_P > 0: keys of a batch).
 constexpr uint32_t kGatherSolo = 16;
+
+// RenderStats on the device (rasterizer.h:22-36, packed as
+// rasterize_points.cu:138-177): a head of 9 + 2 * kStatsMaxBins doubles
+// (bar_g, K/P, K/T, K, T, P, used kv, unused kv, bin count, bin edges, bin
+// counts), then tile_ratio[T] and tile_total[T]; after the packed part, the
+// scratch the blend fills: tile_avg[T] and 2 + kKpcBins u64 counters (used,
+// unused, bins).
+constexpr int kStatsMaxBins = 64;
+constexpr int kKpcBins = 11;
+constexpr size_t kStatsHead = 9 + 2 * kStatsMaxBins;
+inline size_t stats_words(size_t T) { return kStatsHead + 3 * T + 2 + kKpcBins; }
+hipError_t launch_stats_finalize(double* stats, int T, int P, int64_t K, hipStream_t s);
 struct GradGather {
     const uint64_t* vkeys;
     const uint32_t* tiles_s;

@@ -16,7 +16,6 @@ import torch
 from . import _lib
 from ._lib import Gaussians, RasterSettings, call, ptr
 
-KPC_BIN_EDGES = (0.01, 0.05, 0.10, 0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45, 0.50)
 MAX_KPC_BINS = 64
 
 
@@ -78,18 +77,21 @@ def _gaussians(P, means3D, sh, colors, opacity, scales, rotations, cov3D_precomp
     return g, t
 
 
-def _stats_tensor(P, T, K, kv_sum, kv_cnt, ranges, collect):
-    """RenderStats packed as rasterize_points.cu:138-177 packs them, with the
-    KPC / GTC reductions of rasterizer_impl.cu:479-556 done on the GPU."""
+def _kpc_used_thresh():
+    """KPC_USED_THRESH as rasterizer_impl.cu:44-55 reads it (default 1e-4)."""
     import os
-    thresh = 1e-4
     env = os.environ.get("KPC_USED_THRESH", "")
     if env:
         try:
             v = float(env)
-            thresh = v if v > 0 else 1e-4
+            return v if v > 0 else 1e-4
         except ValueError:
-            thresh = 1e-4
+            return 1e-4
+    return 1e-4
+
+
+def _stats_head(P, T, K):
+    """RenderStats of a forward without kv pairs (rasterize_points.cu:138-177)."""
     head = torch.zeros(9 + 2 * MAX_KPC_BINS, dtype=torch.float64)
     head[4] = float(T)
     head[5] = float(P)
@@ -98,42 +100,7 @@ def _stats_tensor(P, T, K, kv_sum, kv_cnt, ranges, collect):
         head[1] = float(K) / float(P)
     if T > 0:
         head[2] = float(K) / float(T)
-    if not (collect and K > 0):
-        return head
-    dev = kv_sum.device
-    edges = torch.tensor(KPC_BIN_EDGES, dtype=torch.float32, device=dev)
-    used_mask = kv_cnt > 0
-    b = torch.searchsorted(edges, kv_sum.contiguous(), right=False)
-    in_bins = used_mask & (b < len(KPC_BIN_EDGES))
-    hist = torch.bincount(b[in_bins], minlength=len(KPC_BIN_EDGES)).to(torch.float64)
-    used_kv = in_bins.sum()
-    unused_kv = (~used_mask).sum()
-    # per tile sums over [start, end) via an f64 prefix sum
-    rg = ranges.view(T, 2).to(torch.int64)
-    start, end = rg[:, 0], rg[:, 1]
-    n = (end - start).clamp(min=0)
-    cs = torch.zeros(K + 1, dtype=torch.float64, device=dev)
-    cs[1:] = torch.cumsum(kv_sum.to(torch.float64), 0)
-    usedt = (used_mask & (kv_sum >= thresh)).to(torch.int64)
-    cu = torch.zeros(K + 1, dtype=torch.int64, device=dev)
-    cu[1:] = torch.cumsum(usedt, 0)
-    nonempty = n > 0
-    tile_sum = torch.where(nonempty, cs[end] - cs[start], torch.zeros_like(cs[end]))
-    tile_used = torch.where(nonempty, cu[end] - cu[start], torch.zeros_like(cu[end]))
-    nf = n.to(torch.float64)
-    tile_avg = torch.where(nonempty, tile_sum / nf.clamp(min=1), torch.zeros_like(tile_sum))
-    tile_ratio = torch.where(nonempty, tile_used.to(torch.float64) / nf.clamp(min=1),
-                             torch.zeros_like(tile_sum))
-    tile_total = torch.where(nonempty, nf, torch.zeros_like(nf))
-    packed = torch.stack([tile_avg.sum() / T, used_kv.to(torch.float64),
-                          unused_kv.to(torch.float64)]).cpu()
-    head[0] = packed[0]
-    head[6] = packed[1]
-    head[7] = packed[2]
-    head[8] = float(len(KPC_BIN_EDGES))
-    head[9:9 + len(KPC_BIN_EDGES)] = torch.tensor(KPC_BIN_EDGES, dtype=torch.float32).to(torch.float64)
-    head[9 + MAX_KPC_BINS:9 + MAX_KPC_BINS + len(KPC_BIN_EDGES)] = hist.cpu()
-    return torch.cat([head, tile_ratio.cpu(), tile_total.cpu()])
+    return head
 
 
 def rasterize_gaussians(background, means3D, colors, opacity, scales, rotations, scale_modifier,
@@ -162,7 +129,6 @@ def rasterize_gaussians(background, means3D, colors, opacity, scales, rotations,
     binning = torch.empty((0,), **u8)
     img = torch.empty((0,), **u8)
     K = 0
-    kv_sum = kv_cnt = ranges = None
     if P != 0:
         s, keep_s = _settings(background, viewmatrix, projmatrix, campos, H, W, tan_fovx, tan_fovy,
                               scale_modifier, degree, prefiltered, debug, variant, use_filter,
@@ -185,18 +151,16 @@ def rasterize_gaussians(background, means3D, colors, opacity, scales, rotations,
         binning = torch.empty((lib.fgs_binning_bytes(K, W, H),), **u8)
         collect = bool(ogfs_enable_stats) and K > 0
         if collect:
-            kv_sum = torch.zeros((K,), **fopt)
-            kv_cnt = torch.zeros((K,), dtype=torch.int32, device=dev)
-        call("fgs_forward_render", ctypes.byref(s), ctypes.byref(g), radii.data_ptr(),
+            stats_dev = torch.empty((lib.fgs_stats_words(W, H),), dtype=torch.float64, device=dev)
+        call("fgs_forward_render_stats", ctypes.byref(s), ctypes.byref(g), radii.data_ptr(),
              geom.data_ptr(), K, binning.data_ptr(), img.data_ptr(), out_color.data_ptr(),
-             out_point_id.data_ptr(), out_wpix.data_ptr(), out_pw.data_ptr(), ptr(kv_sum),
-             ptr(kv_cnt), stream)
-        if collect:
-            rp = lib.fgs_image_ranges(img.data_ptr(), W, H)
-            off = rp - img.data_ptr()
-            ranges = img[off:off + 8 * T].view(torch.int32)
+             out_point_id.data_ptr(), out_wpix.data_ptr(), out_pw.data_ptr(), None, None,
+             stats_dev.data_ptr() if collect else None, _kpc_used_thresh(), stream)
         del keep_s, keep_g
-    stats = _stats_tensor(P, T, K, kv_sum, kv_cnt, ranges, bool(ogfs_enable_stats) and P != 0)
+        if collect:
+            return (K, out_color, out_point_id, out_wpix, out_pw, radii, geom, binning, img,
+                    stats_dev[:9 + 2 * MAX_KPC_BINS + 2 * T].cpu())
+    stats = _stats_head(P, T, K)
     return (K, out_color, out_point_id, out_wpix, out_pw, radii, geom, binning, img, stats)
 
 

@@ -109,6 +109,22 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
                        int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
                        float* kv_alpha_sum, uint32_t* kv_alpha_count, void* stream);
 
+/* fgs_forward_render plus RenderStats computed on the device
+ * (rasterizer.h:22-36 packed as rasterize_points.cu:138-177; the host loops
+ * of rasterizer_impl.cu:479-556 fused into the blend).  stats: device
+ * doubles, fgs_stats_words(W, H) of them, any content on entry; on return the
+ * first 9 + 2*64 + 2*T hold the packed head (bar_g_tile, K/P, K/T, K, T, P,
+ * used kv, unused kv, bin count, 64 bin edges, 64 bin counts), tile_ratio[T]
+ * and tile_total_kv[T]; the rest is scratch.  kpc_used_thresh is the
+ * KPC_USED_THRESH of rasterizer_impl.cu:44-55 (> 0).  stats may be null. */
+int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                             void* geom, int64_t K, void* binning, void* image, float* out_color,
+                             int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                             float* kv_alpha_sum, uint32_t* kv_alpha_count, double* stats,
+                             float kpc_used_thresh, void* stream);
+/* Doubles of the stats buffer of fgs_forward_render_stats at W x H. */
+size_t fgs_stats_words(int32_t image_width, int32_t image_height);
+
 /* Sorted kv list of phase 2, for inspection: keys[k] = tile << 32 |
  * float_bits(depth of vals[k]) exactly as the reference's point_list_keys,
  * vals[k] = point_list.  Either output may be null. */

@@ -251,8 +251,20 @@ def test_autograd_through_plugin_surface(orc):
     assert opac.grad.shape == opac.shape
 
 
-def test_stats_vs_oracle(orc):
-    g, cam = small_scene(n=3000, W=256, H=192, seed=161, dist=6.0)
+@pytest.mark.parametrize("case", ["sparse", "saturated"])
+def test_stats_vs_oracle(orc, case, monkeypatch):
+    """RenderStats computed inside the blend against the oracle's per-kv-pair
+    sums and counts, binned and summed as rasterizer_impl.cu:479-556 does.
+    "saturated": opaque Gaussians, so tiles stop before their last batch (the
+    pairs never staged count as unused), and a KPC_USED_THRESH override."""
+    thr = 1e-4
+    if case == "sparse":
+        g, cam = small_scene(n=3000, W=256, H=192, seed=161, dist=6.0)
+    else:
+        g, cam = small_scene(n=8000, W=256, H=192, seed=7, dist=4.0)
+        g["opacities"] = np.full_like(g["opacities"], 0.97)
+        thr = 0.05
+        monkeypatch.setenv("KPC_USED_THRESH", "0.05")
     out = gpu_forward(g, cam, variant=0, stats=True)
     from filtergs_amd.rasterizer import parse_stats
     st = parse_stats(out[9])
@@ -262,6 +274,9 @@ def test_stats_vs_oracle(orc):
     kv, cnt, rg = f["kv_alpha_sum"].astype(np.float64), f["kv_alpha_count"], f["ranges"]
     T = rg.shape[0]
     used = cnt > 0
+    if case == "saturated":
+        # some tile ends with pairs that no pixel reached
+        assert any(e > a and cnt[e - 1] == 0 for a, e in rg)
     edges = np.array([0.01, 0.05, 0.10, 0.15, 0.20, 0.25, 0.30, 0.35, 0.40, 0.45, 0.50], np.float32)
     b = np.searchsorted(edges, f["kv_alpha_sum"], side="left")
     inb = used & (b < 11)
@@ -273,6 +288,11 @@ def test_stats_vs_oracle(orc):
     tile_avg = np.array([kv[a:e].mean() if e > a else 0.0 for a, e in rg])
     assert abs(st["bar_g"] - tile_avg.sum() / T) <= 1e-5
     np.testing.assert_allclose(st["tile_total_kv"], [e - a for a, e in rg])
+    ratio = np.array([((cnt[a:e] > 0) & (f["kv_alpha_sum"][a:e] >= np.float32(thr))).mean()
+                      if e > a else 0.0 for a, e in rg])
+    # a pair sum within an ulp of the threshold may land on either side
+    dr = np.abs(np.array(st["tile_ratio"]) - ratio)
+    assert dr.max() <= 0.1 and dr.sum() <= 0.05, (dr.max(), dr.sum())
 
 
 def test_lod_cut_bitexact(orc):
