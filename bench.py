@@ -71,6 +71,69 @@ def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
     }
 
 
+def filtergs_views(tree, cams, warmup):
+    """FilterGS inference on the LoD tree (FilterGS/filtergs.py:20-223): per
+    view prepare() (the cut, radius_max 5.0 as render.yml) and render(), with
+    render() once folded into the rasterizer (rows activated in the
+    preprocess) and once on the gathered route of the reference (gather +
+    activation, finite filter, rasterizer), forward only, statistics on."""
+    import torch
+
+    from filtergs_amd.filtergs import FilterGS
+    from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+    dev = "cuda"
+    fg = FilterGS(sh_degree=1)
+    fg.register_buffer("xyz", tree["means3D"])
+    fg.register_buffer("scaling", tree["log_scales"])
+    fg.register_buffer("rotation", tree["rotations"])
+    op = tree["opacities"].clamp(1e-4, 1 - 1e-4)
+    fg.register_buffer("opacity", torch.log(op / (1 - op)))
+    fg.register_buffer("colors", tree["shs"][:, 0, :].contiguous())
+    fg.register_buffer("shs", tree["shs"][:, 1:, :].contiguous())
+    fg.active_sh_degree = 1
+
+    class Gathered:
+        def __init__(self, r):
+            self.r, self.raster_settings = r, r.raster_settings
+
+        def __call__(self, **kw):
+            return self.r(**kw)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    out = {"prepare_ms": [], "render_rows_ms": [], "render_gathered_ms": [], "rendered": []}
+    t = lambda a: torch.as_tensor(a, dtype=torch.float32, device=dev)
+    for cam in cams:
+        rs = GaussianRasterizationSettings(
+            image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
+            bg=t([0.0, 0.0, 0.0]), scale_modifier=1.0, viewmatrix=t(cam.world_view_transform),
+            projmatrix=t(cam.full_proj_transform), sh_degree=1, campos=t(cam.center), prefiltered=False,
+            debug=False)
+        r = GaussianRasterizer(rs)
+        camera = {"camera_center": t(cam.center), "full_proj_transform": t(cam.full_proj_transform)}
+
+        def run(fn):
+            for _ in range(max(1, warmup)):
+                fn()
+            torch.cuda.synchronize()
+            ev[0].record()
+            fn()
+            ev[1].record()
+            ev[1].synchronize()
+            return round(ev[0].elapsed_time(ev[1]), 3)
+
+        prep = lambda: fg.prepare(r, camera, tree_node_index_gpu=tree["tree_node_index"],
+                                  ancestor_path_gpu=tree["ancestor_path"], radius_max=5.0,
+                                  padding=0.1, radius_filter_threshold=700.0)
+        out["prepare_ms"].append(run(prep))
+        out["rendered"].append(int(fg.visibility_flag["index"].numel()))
+        out["render_rows_ms"].append(run(lambda: fg.render(camera, r)))
+        g = Gathered(r)
+        out["render_gathered_ms"].append(run(lambda: fg.render(camera, g)))
+    n = len(cams)
+    out["mean"] = {k: round(sum(out[k]) / n, 3) for k in ("prepare_ms", "render_rows_ms", "render_gathered_ms")}
+    return out
+
+
 def lod_leg(steps, warmup):
     """Urban-scale LoD: a 22.4M-node quadtree (12 levels, 2x2 roots over 2 km x
     2 km, leaves on a seeded height field, SH degree 1), the LoD cut of each of
@@ -112,6 +175,7 @@ def lod_leg(steps, warmup):
         res[rmax] = {"ms_per_view": round(ms, 3), "GB/s": round(nbytes / ms / 1e6, 1),
                      "kept_per_view": [int(m.sum().item()) for m in masks],
                      "union": int(union.sum().item())}
+    fg_res = filtergs_views(tree, cams, warmup)
     union = torch.stack([cut(c, 1.5)[0] for c in cams]).any(0)
     idx = torch.nonzero(union).reshape(-1)
     torch.cuda.synchronize()
@@ -148,6 +212,7 @@ def lod_leg(steps, warmup):
         raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
     return {"nodes": N, "leaves": n_leaf, "levels": L,
             "cut_radius_max_1.5": res[1.5], "cut_radius_max_5.0": res[5.0],
+            "filtergs_render": fg_res,
             "gather_ms": round(ms_gather, 3), "gaussians_rendered": P, "kv_pairs": K,
             "batched_fwd_bwd_ms": round(ms, 3), "Mpix/s": round(npix / ms / 1000.0, 3),
             "views": len(cams), "pixels": npix}

@@ -69,6 +69,24 @@ class Gaussians(ctypes.Structure):
     ]
 
 
+class FilterGSRows(ctypes.Structure):
+    """fgs_filtergs_rows."""
+    _fields_ = [
+        ("N", c_int32),
+        ("index", c_void_p),
+        ("n_src", c_int64),
+        ("xyz", c_void_p),
+        ("scaling", c_void_p),
+        ("rotation", c_void_p),
+        ("opacity", c_void_p),
+        ("colors", c_void_p),
+        ("shs", c_void_p),
+        ("rest_coeffs", c_int32),
+        ("sh_degree", c_int32),
+        ("campos", c_void_p),
+    ]
+
+
 class BatchView(ctypes.Structure):
     """fgs_batch_view."""
     _fields_ = [
@@ -110,6 +128,10 @@ _SIGS = {
                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, ctypes.c_float, c_void_p]),
     "fgs_stats_words": (c_size_t, [c_int32, c_int32]),
+    "fgs_filtergs_geometry": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fgs_filtergs_render": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      ctypes.c_float, c_void_p]),
     "fgs_read_sorted": (c_int32, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p,
                                   c_void_p]),
     "fgs_image_ranges": (c_void_p, [c_void_p, c_int32, c_int32]),

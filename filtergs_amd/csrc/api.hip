@@ -226,16 +226,24 @@ float ogfs_log_lambda(float eps_scale, float eps_max)
     return (float)log((double)lambda_g);
 }
 
-int validate(const fgs_raster_settings* s, const fgs_gaussians* g)
+int validate_settings(const fgs_raster_settings* s)
 {
-    if (!s || !g) return fail(FGS_E_INVALID, "null settings or gaussians%s");
-    if (g->P < 0) return fail(FGS_E_INVALID, "negative P%s");
+    if (!s) return fail(FGS_E_INVALID, "null settings%s");
     if (s->image_width <= 0 || s->image_height <= 0)
         return fail(FGS_E_INVALID, "image size must be positive%s");
     if ((s->image_width + 15) / 16 > 65535 || (s->image_height + 15) / 16 > 65535)
         return fail(FGS_E_INVALID, "image larger than 65535 tiles per side%s");
     if (!s->viewmatrix || !s->projmatrix || !s->bg || !s->campos)
         return fail(FGS_E_INVALID, "null camera pointer%s");
+    return FGS_OK;
+}
+
+int validate(const fgs_raster_settings* s, const fgs_gaussians* g)
+{
+    if (!s || !g) return fail(FGS_E_INVALID, "null settings or gaussians%s");
+    if (g->P < 0) return fail(FGS_E_INVALID, "negative P%s");
+    int rc = validate_settings(s);
+    if (rc) return rc;
     if (g->P > 0 && (!g->means3D || !g->opacities))
         return fail(FGS_E_INVALID, "null means3D or opacities%s");
     if ((g->shs == nullptr) == (g->colors_precomp == nullptr) && g->P > 0)
@@ -248,12 +256,10 @@ int validate(const fgs_raster_settings* s, const fgs_gaussians* g)
     return FGS_OK;
 }
 
-fgs::PreArgs make_pre_args(const fgs_raster_settings* s, const fgs_gaussians* g)
+fgs::PreArgs make_pre_args_settings(const fgs_raster_settings* s)
 {
     fgs::PreArgs a{};
-    a.P = g->P;
     a.D = s->sh_degree;
-    a.M = g->M;
     a.W = s->image_width;
     a.H = s->image_height;
     a.gx = (a.W + 15) / 16;
@@ -273,6 +279,14 @@ fgs::PreArgs make_pre_args(const fgs_raster_settings* s, const fgs_gaussians* g)
     a.view = s->viewmatrix;
     a.proj = s->projmatrix;
     a.campos = s->campos;
+    return a;
+}
+
+fgs::PreArgs make_pre_args(const fgs_raster_settings* s, const fgs_gaussians* g)
+{
+    fgs::PreArgs a = make_pre_args_settings(s);
+    a.P = g->P;
+    a.M = g->M;
     a.means3D = g->means3D;
     a.scales = g->cov3D_precomp ? nullptr : g->scales;
     a.rotations = g->cov3D_precomp ? nullptr : g->rotations;
@@ -373,17 +387,16 @@ size_t fgs_binning_bytes(int64_t K, int32_t W, int32_t H)
     return t;
 }
 
-int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, void* geom,
-                         int32_t* radii, uint64_t* counts, void* stream)
+// Phase 1 for P Gaussians described by a: preprocess (plain inputs, or
+// FilterGS rows), depth sort, tile offsets; counts as fgs_forward_geometry.
+static int geometry_impl(const fgs_raster_settings* s, const fgs::PreArgs& a, bool rows, void* geom,
+                  int32_t* radii, uint64_t* counts, hipStream_t st)
 {
-    int rc = validate(s, g);
-    if (rc) return rc;
-    if (!counts || (g->P > 0 && (!geom || !radii))) return fail(FGS_E_INVALID, "null output%s");
-    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (!counts || (a.P > 0 && (!geom || !radii))) return fail(FGS_E_INVALID, "null output%s");
     if ((rc = check(fgs::launch_fill_u32(counts, 0u, 8, st), "counts clear"))) return rc;
-    if (g->P == 0) return sync_check(s, st, "geometry");
-    const fgs::PreArgs a = make_pre_args(s, g);
-    Geom gm = carve_geom(geom, g->P, nullptr);
+    if (a.P == 0) return sync_check(s, st, "geometry");
+    Geom gm = carve_geom(geom, a.P, nullptr);
     fgs::PreOut o{};
     o.radii = radii;
     o.means2D = gm.means2D;
@@ -396,27 +409,73 @@ int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, v
     o.error_flag = (int*)(counts + 2);
     {
         ProfScope ps(ST_PREPROCESS, st);
-        if ((rc = check(fgs::launch_preprocess_fwd(a, o, st), "preprocess"))) return rc;
+        if ((rc = check(rows ? fgs::launch_preprocess_fwd_rows(a, o, st) : fgs::launch_preprocess_fwd(a, o, st),
+                        "preprocess")))
+            return rc;
     }
     if ((rc = sync_check(s, st, "preprocess"))) return rc;
     // depth sort of the per-Gaussian keys on bits 32..63 (4 passes -> back in A)
     {
         ProfScope ps(ST_DEPTH_SORT, st);
-        if ((rc = check(fgs::radix_sort_u64(gm.vkeysA, gm.vkeysB, (size_t)g->P, 32, 4, gm.hist,
+        if ((rc = check(fgs::radix_sort_u64(gm.vkeysA, gm.vkeysB, (size_t)a.P, 32, 4, gm.hist,
                                             gm.partials, st), "depth sort")))
             return rc;
     }
     {
         ProfScope ps(ST_OFFSETS, st);
-        if ((rc = check(fgs::launch_sorted_records(gm.vkeysA, gm.recs, g->P, gm.tiles_s, gm.rects_s, st),
+        if ((rc = check(fgs::launch_sorted_records(gm.vkeysA, gm.recs, a.P, gm.tiles_s, gm.rects_s, st),
                         "sorted records")))
             return rc;
-        if ((rc = check(fgs::launch_scan_tiles(gm.tiles_s, gm.offs, g->P, gm.partials,
+        if ((rc = check(fgs::launch_scan_tiles(gm.tiles_s, gm.offs, a.P, gm.partials,
                                                (unsigned long long*)counts, st),
                         "tile offsets")))
             return rc;
     }
     return sync_check(s, st, "geometry");
+}
+
+int fgs_forward_geometry(const fgs_raster_settings* s, const fgs_gaussians* g, void* geom,
+                         int32_t* radii, uint64_t* counts, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    return geometry_impl(s, make_pre_args(s, g), false, geom, radii, counts, (hipStream_t)stream);
+}
+
+static int validate_rows(const fgs_raster_settings* s, const fgs_filtergs_rows* f)
+{
+    int rc = validate_settings(s);
+    if (rc) return rc;
+    if (!f) return fail(FGS_E_INVALID, "null rows%s");
+    if (f->N < 0 || f->n_src < 0) return fail(FGS_E_INVALID, "negative row count%s");
+    if (f->N > 0 && (!f->index || !f->xyz || !f->scaling || !f->rotation || !f->opacity || !f->colors))
+        return fail(FGS_E_INVALID, "null parameter array%s");
+    if (f->sh_degree < 0 || f->sh_degree > 4) return fail(FGS_E_INVALID, "sh_degree must be in [0,4]%s");
+    if (f->sh_degree > 0 &&
+        (!f->shs || !f->campos || f->rest_coeffs < (f->sh_degree + 1) * (f->sh_degree + 1) - 1))
+        return fail(FGS_E_INVALID, "sh_degree > 0 needs shs with (deg+1)^2-1 coefficients and campos%s");
+    return FGS_OK;
+}
+
+int fgs_filtergs_geometry(const fgs_raster_settings* s, const fgs_filtergs_rows* f, void* geom,
+                          int32_t* radii, uint64_t* counts, void* stream)
+{
+    int rc = validate_rows(s, f);
+    if (rc) return rc;
+    fgs::PreArgs a = make_pre_args_settings(s);
+    a.P = f->N;
+    a.means3D = f->xyz;
+    a.scales = f->scaling;
+    a.rotations = f->rotation;
+    a.opacities = f->opacity;
+    a.rows_index = f->index;
+    a.rows_n_src = f->n_src;
+    a.rows_dc = f->colors;
+    a.rows_rest = f->shs;
+    a.rows_rest_floats = 3 * f->rest_coeffs;
+    a.rows_deg = f->sh_degree;
+    a.rows_campos = f->campos;
+    return geometry_impl(s, a, true, geom, radii, counts, (hipStream_t)stream);
 }
 
 size_t fgs_stats_words(int32_t W, int32_t H)
@@ -435,24 +494,24 @@ int fgs_forward_render(const fgs_raster_settings* s, const fgs_gaussians* g, con
                                     nullptr, 0.f, stream);
 }
 
-int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
-                             void* geom, int64_t K, void* binning, void* image, float* out_color,
-                             int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
-                             float* kv_alpha_sum, uint32_t* kv_alpha_count, double* stats,
-                             float kpc_used_thresh, void* stream)
+// Phase 2 for P Gaussians whose colours are colors_precomp, or the
+// preprocess's when null.
+static int render_impl(const fgs_raster_settings* s, int P, const float* colors_precomp, const int32_t* radii,
+                void* geom, int64_t K, void* binning, void* image, float* out_color,
+                int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                float* kv_alpha_sum, uint32_t* kv_alpha_count, double* stats, float kpc_used_thresh,
+                hipStream_t st)
 {
-    int rc = validate(s, g);
-    if (rc) return rc;
+    int rc;
     if (K < 0 || K > 0x7FFFFFFFll) return fail(FGS_E_CAPACITY, "kv pair count out of range%s");
     if (!image || !out_color || !out_point_id || !out_weight_pixel)
         return fail(FGS_E_INVALID, "null output%s");
     if ((kv_alpha_sum == nullptr) != (kv_alpha_count == nullptr))
         return fail(FGS_E_INVALID, "kv_alpha_sum and kv_alpha_count go together%s");
-    hipStream_t st = (hipStream_t)stream;
     const int W = s->image_width, H = s->image_height;
     const int gx = (W + 15) / 16, gy = (H + 15) / 16, T = gx * gy;
     Image im = carve_image(image, W, H, nullptr);
-    Geom gm = carve_geom(geom, g->P, nullptr);
+    Geom gm = carve_geom(geom, P, nullptr);
     fgs::BlendArgs b{};
     b.ranges = im.ranges;
     b.tile_vis = im.tile_vis;
@@ -461,7 +520,7 @@ int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* 
     b.gx = gx;
     b.gy = gy;
     b.means2D = gm.means2D;
-    b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+    b.colors = colors_precomp ? colors_precomp : gm.rgb;
     b.conic_opacity = gm.conic_opacity;
     b.bg = s->bg;
     b.batch = g_shape[0];
@@ -473,7 +532,7 @@ int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* 
         uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
         {
             ProfScope ps(ST_DUPLICATE, st);
-            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.rects_s, g->P, gx, first, gm.recs,
+            if ((rc = check(fgs::launch_duplicate(gm.vkeysA, gm.offs, gm.rects_s, P, gx, first, gm.recs,
                                                   st),
                             "duplicate")))
                 return rc;
@@ -511,9 +570,34 @@ int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* 
                                               kv_alpha_count, st), "blend")))
             return rc;
     }
-    if (stats && (rc = check(fgs::launch_stats_finalize(stats, T, g->P, K, st), "stats")))
+    if (stats && (rc = check(fgs::launch_stats_finalize(stats, T, P, K, st), "stats")))
         return rc;
     return sync_check(s, st, "render");
+}
+
+int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                             void* geom, int64_t K, void* binning, void* image, float* out_color,
+                             int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                             float* kv_alpha_sum, uint32_t* kv_alpha_count, double* stats,
+                             float kpc_used_thresh, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    return render_impl(s, g->P, g->colors_precomp, radii, geom, K, binning, image, out_color, out_point_id,
+                       out_weight_pixel, out_point_weight, kv_alpha_sum, kv_alpha_count, stats,
+                       kpc_used_thresh, (hipStream_t)stream);
+}
+
+int fgs_filtergs_render(const fgs_raster_settings* s, const fgs_filtergs_rows* f, const int32_t* radii,
+                        void* geom, int64_t K, void* binning, void* image, float* out_color,
+                        int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                        double* stats, float kpc_used_thresh, void* stream)
+{
+    int rc = validate_rows(s, f);
+    if (rc) return rc;
+    return render_impl(s, f->N, nullptr, radii, geom, K, binning, image, out_color, out_point_id,
+                       out_weight_pixel, out_point_weight, nullptr, nullptr, stats, kpc_used_thresh,
+                       (hipStream_t)stream);
 }
 
 int fgs_read_sorted(const void* geom, int32_t P, int64_t K, const void* binning, uint64_t* keys,

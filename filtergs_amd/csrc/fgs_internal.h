@@ -27,6 +27,16 @@ struct PreArgs {
     const float* shs;
     const float* colors_precomp;
     const float* cov3D_precomp;
+    // FilterGS rows (launch_preprocess_fwd_rows): Gaussian i is row
+    // rows_index[i] of the raw parameters -- means3D, scales = log scales,
+    // rotations = raw quaternions, opacities = logits, rows_dc / rows_rest
+    // the colour coefficients -- activated on load.
+    const int64_t* rows_index;
+    int64_t rows_n_src;
+    const float* rows_dc;
+    const float* rows_rest;
+    int rows_rest_floats, rows_deg;
+    const float* rows_campos;
 };
 
 // Outputs of the forward preprocess (SoA).
@@ -131,6 +141,8 @@ struct BlendArgs {
 hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t s);
 hipError_t launch_preprocess_bwd(const PreArgs& a, const PreBwdIn& in, const PreBwdOut& o,
                                  hipStream_t s);
+// error_flag bit 1: an index outside [0, rows_n_src)
+hipError_t launch_preprocess_fwd_rows(const PreArgs& a, const PreOut& o, hipStream_t s);
 hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s);
 hipError_t launch_preprocess_fwd_batch(const PreArgs& a, const BatchView* views, int V,
                                        const PreOut& o, hipStream_t s);

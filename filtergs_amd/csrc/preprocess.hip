@@ -12,6 +12,7 @@
 // FMA contraction.
 #include "fgs_device.h"
 #include "fgs_internal.h"
+#include "fgs_activation.h"
 
 namespace fgs {
 
@@ -324,6 +325,54 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOu
     write_pre(o, (size_t)i, r, !a.colors_precomp);
     const uint32_t hi = r.tiles ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
     o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
+}
+
+// FilterGS render (FilterGS/filtergs.py:98-223 in one pass): Gaussian i is
+// row rows_index[i] of the raw parameters, activated as FilterGS.get_all does
+// (scale exp, opacity sigmoid, quaternion normalised, colour SH2RGB(dc) +
+// eval_sh_wobase, no clamp), and used as the rasterizer's colours_precomp
+// input.  A row with a non-finite value, a zero scale or a zero opacity is
+// culled, as FilterGS.render drops it before rasterising (:150-167).
+__global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_rows(PreArgs a, PreOut o)
+{
+    const int i = blockIdx.x * kPreThreads + threadIdx.x;
+    if (i >= a.P) return;
+    const int64_t src = a.rows_index[i];
+    PreRes r{};
+    bool keep = src >= 0 && src < a.rows_n_src;
+    if (!keep) atomicOr(o.error_flag, 2);
+    if (keep) {
+        ActRow ar;
+        activate_row(src, a.means3D, a.scales, a.rotations, a.opacities, a.rows_dc, a.rows_rest,
+                     a.rows_rest_floats, a.rows_deg, a.rows_campos, ar);
+        bool fin = isfinite(ar.o) && ar.o > 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            fin = fin && isfinite(ar.p[k]) && isfinite(ar.s[k]) && ar.s[k] > 0.f && isfinite(ar.c[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fin = fin && isfinite(ar.q[k]);
+        if (fin) {
+            float c3[6];
+            cov3d(a.mod * ar.s[0], a.mod * ar.s[1], a.mod * ar.s[2], ar.q[0], ar.q[1], ar.q[2], ar.q[3], c3);
+            PreArgs ac = a;
+            ac.colors_precomp = a.rows_dc;   // colours come from the activation: no SH pass
+            preprocess_view(ac, ar.p[0], ar.p[1], ar.p[2], c3, ar.o, ShLds{ nullptr, 0 }, o.error_flag, r);
+            r.col[0] = ar.c[0];
+            r.col[1] = ar.c[1];
+            r.col[2] = ar.c[2];
+        }
+    }
+    write_pre(o, (size_t)i, r, true);
+    const uint32_t hi = r.tiles ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
+    o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
+}
+
+hipError_t launch_preprocess_fwd_rows(const PreArgs& a, const PreOut& o, hipStream_t s)
+{
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_preprocess_fwd_rows, dim3((a.P + kPreThreads - 1) / kPreThreads), dim3(kPreThreads),
+                       0, s, a, o);
+    return hipGetLastError();
 }
 
 // All views of a batch in one pass: the Gaussian's inputs (mean, covariance,

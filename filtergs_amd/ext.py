@@ -164,6 +164,71 @@ def rasterize_gaussians(background, means3D, colors, opacity, scales, rotations,
     return (K, out_color, out_point_id, out_wpix, out_pw, radii, geom, binning, img, stats)
 
 
+def filtergs_render(raster_settings, index, xyz, scaling, rotation, opacity, colors, shs,
+                    sh_degree, campos, use_filter=True, ogfs_epsilon_scale=0.2, ogfs_epsilon_max=0.2,
+                    ogfs_energy_floor=0.05, ogfs_radius_ratio_min=0.7, ogfs_enable_stats=True):
+    """FilterGS.get_all + FilterGS.render in one rasterizer pass
+    (FilterGS/filtergs.py:98-223): the preprocess reads row index[i] of the
+    raw parameters and activates it (no gathered copy, no finite filter pass).
+    Returns (color [3,H,W], stats) like the rasterizer call of render()."""
+    rs = raster_settings
+    dev = xyz.device
+    _lib.require_gpu(xyz)
+    H, W = int(rs.image_height), int(rs.image_width)
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    idx = index.to(device=dev, dtype=torch.int64).contiguous()
+    N = int(idx.numel())
+    f32 = dict(dtype=torch.float32, device=dev)
+    src = [t.to(**f32).contiguous() for t in (xyz, scaling, rotation, opacity, colors)]
+    deg = int(sh_degree)
+    rest = shs.to(**f32).contiguous() if (deg > 0 and _nonempty(shs)) else None
+    cp = campos.to(**f32).reshape(3).contiguous() if (deg > 0 and campos is not None) else None
+    if rest is None or cp is None:
+        deg = 0
+    out_color = torch.zeros((3, H, W), **f32)
+    s, keep_s = _settings(rs.bg, rs.viewmatrix, rs.projmatrix, rs.campos, H, W, rs.tanfovx, rs.tanfovy,
+                          rs.scale_modifier, 0, rs.prefiltered, rs.debug, 0, use_filter,
+                          ogfs_epsilon_scale, ogfs_epsilon_max, ogfs_energy_floor,
+                          ogfs_radius_ratio_min, ogfs_enable_stats, 0, dev)
+    f = _lib.FilterGSRows()
+    f.N = N
+    f.index = idx.data_ptr()
+    f.n_src = int(src[0].shape[0])
+    f.xyz, f.scaling, f.rotation, f.opacity, f.colors = [t.data_ptr() for t in src]
+    f.shs = ptr(rest)
+    f.rest_coeffs = int(rest.shape[1]) if rest is not None else 0
+    f.sh_degree = deg
+    f.campos = ptr(cp)
+    lib = _lib.load()
+    stream = _lib.stream_handle(dev)
+    u8 = dict(dtype=torch.uint8, device=dev)
+    radii = torch.zeros((N,), dtype=torch.int32, device=dev)
+    geom = torch.empty((lib.fgs_geom_bytes(N),), **u8)
+    img = torch.empty((lib.fgs_image_bytes(W, H),), **u8)
+    counts = torch.empty((4,), dtype=torch.int64, device=dev)
+    call("fgs_filtergs_geometry", ctypes.byref(s), ctypes.byref(f), geom.data_ptr(), radii.data_ptr(),
+         counts.data_ptr(), stream)
+    ch = counts.cpu()
+    if int(ch[2]) & 2:
+        raise RuntimeError("gather_attributes: index out of range")
+    K = int(ch[1])
+    if K > 0x7FFFFFFF:
+        raise RuntimeError(f"too many kv pairs ({K})")
+    binning = torch.empty((lib.fgs_binning_bytes(K, W, H),), **u8)
+    collect = bool(ogfs_enable_stats) and K > 0
+    stats_dev = torch.empty((lib.fgs_stats_words(W, H),), dtype=torch.float64, device=dev) if collect else None
+    point_id = torch.empty((H, W), dtype=torch.int32, device=dev)
+    wpix = torch.empty((H, W), **f32)
+    pw = torch.zeros((N,), **f32)
+    call("fgs_filtergs_render", ctypes.byref(s), ctypes.byref(f), radii.data_ptr(), geom.data_ptr(), K,
+         binning.data_ptr(), img.data_ptr(), out_color.data_ptr(), point_id.data_ptr(), wpix.data_ptr(),
+         pw.data_ptr(), ptr(stats_dev), _kpc_used_thresh(), stream)
+    del keep_s
+    if collect:
+        return out_color, stats_dev[:9 + 2 * MAX_KPC_BINS + 2 * T].cpu()
+    return out_color, _stats_head(N, T, K)
+
+
 def rasterize_gaussians_backward(background, means3D, radii, colors, scales, rotations,
                                  scale_modifier, cov3D_precomp, viewmatrix, projmatrix, tan_fovx,
                                  tan_fovy, dL_dout_color, sh, degree, campos, geomBuffer, R,

@@ -12,7 +12,11 @@ stages:
   normalise, SH2RGB(dc) + SH without DC toward the camera) in one fused kernel,
   ``fgs_activate_gather``, instead of a gather plus five element-wise passes;
 * ``render`` -- the finite / positive filter and the wodilate rasterizer with
-  precomputed colours.
+  precomputed colours.  With this package's rasterizer, ``get_all`` and the
+  filter are folded into the rasterizer's preprocess (``fgs_filtergs_geometry``
+  / ``fgs_filtergs_render``): it reads row ``index[i]`` of the raw buffers and
+  activates it while loading, so no gathered copy is made; other rasterizers
+  get the gathered tensors as in the reference.
 """
 from __future__ import annotations
 
@@ -21,7 +25,8 @@ import time
 import torch
 import torch.nn as nn
 
-from . import _lib, lod
+from . import _lib, ext, lod
+from . import rasterizer as _rasterizer
 from ._lib import call, ptr
 
 SH_C0 = 0.28209479177387814
@@ -182,6 +187,8 @@ class FilterGS(nn.Module):
     def render(self, camera, rasterizer, background=None, enable_stats=True):
         if self.visibility_flag is None:
             raise RuntimeError("Must call prepare() before render()")
+        if isinstance(rasterizer, _rasterizer.GaussianRasterizer) and self.visibility_flag['index'].numel() > 0:
+            return self._render_rows(camera, rasterizer, enable_stats)
         activated = self.get_all(camera)
         xyz, opacity, colors = activated['xyz'], activated['opacity'], activated['colors']
         scales, rotations = activated['scaling'], activated['rotation']
@@ -218,6 +225,31 @@ class FilterGS(nn.Module):
         ev1.record()
         ev1.synchronize()
         return {"render": out[0][:3], "timing": {"T_raster_ms": float(ev0.elapsed_time(ev1))}}
+
+    def _render_rows(self, camera, rasterizer, enable_stats):
+        """get_all + render as one rasterizer pass over the visible rows: the
+        preprocess reads and activates row index[i] itself, and rows the
+        reference would drop as non-finite are culled there."""
+        rs = rasterizer.raster_settings
+        deg = int(self.active_sh_degree)
+        campos = None
+        if deg > 0 and camera is not None:
+            campos = torch.as_tensor(camera['camera_center'])
+        shs = getattr(self, 'shs', None)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        color, stats = ext.filtergs_render(
+            rs, self.visibility_flag['index'], self.xyz, self.scaling, self.rotation, self.opacity,
+            self.colors, shs if shs is not None else torch.Tensor([]), deg, campos,
+            ogfs_epsilon_scale=rs.ogfs_epsilon_scale, ogfs_epsilon_max=rs.ogfs_epsilon_max,
+            ogfs_energy_floor=rs.ogfs_energy_floor, ogfs_radius_ratio_min=rs.ogfs_radius_ratio_min,
+            ogfs_enable_stats=bool(enable_stats))
+        ev1.record()
+        ev1.synchronize()
+        _rasterizer._RasterizeGaussians._last_stats = stats
+        rasterizer.latest_stats = _rasterizer.parse_stats(stats)
+        return {"render": color[:3], "timing": {"T_raster_ms": float(ev0.elapsed_time(ev1))}}
 
     # -- visibility (FilterGS/filtergs.py:225-302) ---------------------------
     def _visible_flag_by_camera_cuda(self, xyz, camera, padding=0.05):

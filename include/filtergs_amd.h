@@ -125,6 +125,43 @@ int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* 
 /* Doubles of the stats buffer of fgs_forward_render_stats at W x H. */
 size_t fgs_stats_words(int32_t image_width, int32_t image_height);
 
+/* FilterGS render rows (FilterGS/filtergs.py:98-223: get_all, then render):
+ * the rasterizer draws row index[i] of the raw parameter arrays for i < N,
+ * activated while it is loaded -- scale exp(scaling), opacity
+ * sigmoid(opacity), rotation normalised, colour SH2RGB(colors) +
+ * eval_sh_wobase(dir, shs) with no clamp (LoG/model/activation.py:36-43,
+ * LoG/model/sh_utils.py:31-68) used as colors_precomp.  A row with a
+ * non-finite value, a zero scale or a zero opacity is culled, as render()
+ * drops it (:150-167).  No gathered copy of the rows is made. */
+typedef struct {
+    int32_t N;                 /* rows drawn */
+    const int64_t* index;      /* [N] rows of the arrays below */
+    int64_t n_src;             /* rows of the arrays */
+    const float* xyz;          /* [n_src,3] */
+    const float* scaling;      /* [n_src,3] log scales */
+    const float* rotation;     /* [n_src,4] quaternions, not normalised */
+    const float* opacity;      /* [n_src] logits */
+    const float* colors;       /* [n_src,3] DC coefficients */
+    const float* shs;          /* [n_src,rest_coeffs,3] higher coefficients, or null */
+    int32_t rest_coeffs;
+    int32_t sh_degree;         /* active degree, 0..4 */
+    const float* campos;       /* device [3], camera centre (sh_degree > 0) */
+} fgs_filtergs_rows;
+
+/* Phase 1 for FilterGS rows; replaces gather_attributes
+ * (FilterGS/cuda/gather_attributes_kernel.cu:6-145), the activation passes and
+ * the preprocess of the rasterizer call.  Buffers and counts as
+ * fgs_forward_geometry with P = N; counts[2] bit 1 flags an index outside
+ * [0, n_src) (that row is culled). */
+int fgs_filtergs_geometry(const fgs_raster_settings* s, const fgs_filtergs_rows* f, void* geom,
+                          int32_t* radii, uint64_t* counts, void* stream);
+/* Phase 2 for FilterGS rows: fgs_forward_render_stats with the colours of the
+ * activation (stats may be null). */
+int fgs_filtergs_render(const fgs_raster_settings* s, const fgs_filtergs_rows* f, const int32_t* radii,
+                        void* geom, int64_t K, void* binning, void* image, float* out_color,
+                        int32_t* out_point_id, float* out_weight_pixel, float* out_point_weight,
+                        double* stats, float kpc_used_thresh, void* stream);
+
 /* Sorted kv list of phase 2, for inspection: keys[k] = tile << 32 |
  * float_bits(depth of vals[k]) exactly as the reference's point_list_keys,
  * vals[k] = point_list.  Either output may be null. */

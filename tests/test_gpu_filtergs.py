@@ -109,3 +109,53 @@ def test_filtergs_no_tree_fallback(orc):
     expect = vis & (rad <= 40.0)
     assert n == int(expect.sum()) > 100
     assert np.array_equal(fg.visibility_flag["flag"].cpu().numpy(), expect)
+
+
+class _Forward:
+    """A rasterizer that is not this package's class: render() then takes the
+    gather + filter + rasterizer route of the reference."""
+
+    def __init__(self, r):
+        self.r = r
+        self.raster_settings = r.raster_settings
+
+    def __call__(self, **kw):
+        return self.r(**kw)
+
+
+def test_filtergs_rows_render_matches_gathered_route():
+    tree, fg = _scene(sh_degree=2)
+    cam = _cam()
+    r = _raster(cam)
+    camera = {"camera_center": _t(cam.center), "full_proj_transform": _t(cam.full_proj_transform)}
+    n = fg.prepare(r, camera, tree_node_index_gpu=tree["tree_node_index"].to(DEV),
+                   ancestor_path_gpu=tree["ancestor_path"].to(DEV), radius_max=5.0, padding=0.1,
+                   radius_filter_threshold=700.0)
+    assert n > 100
+    fg.active_sh_degree = 2
+    # rows the reference drops before rasterising: NaN mean, zero scale, zero opacity
+    idx = fg.visibility_flag["index"]
+    fg.xyz[idx[0]] = float("nan")
+    fg.scaling[idx[1]] = float("-inf")
+    fg.opacity[idx[2]] = float("-inf")
+    fused = fg.render(camera, r)
+    st_fused = r.latest_stats
+    gathered = fg.render(camera, _Forward(r))
+    a, b = fused["render"], gathered["render"]
+    assert a.shape == b.shape == (3, cam.height, cam.width)
+    d = (a - b).abs()
+    assert float(d.max()) < 0.02
+    assert float((d > 1e-5).float().mean()) < 1e-3
+    assert st_fused is not None and st_fused["num_tiles"] == ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
+    assert fused["timing"]["T_raster_ms"] > 0.0
+
+
+def test_filtergs_rows_bad_index():
+    from filtergs_amd import ext
+    tree, fg = _scene(sh_degree=1)
+    cam = _cam()
+    r = _raster(cam)
+    bad = torch.tensor([0, 1, fg.xyz.shape[0] + 5], dtype=torch.int64, device=DEV)
+    with pytest.raises(RuntimeError, match="index out of range"):
+        ext.filtergs_render(r.raster_settings, bad, fg.xyz, fg.scaling, fg.rotation, fg.opacity,
+                            fg.colors, fg.shs, 1, _t(cam.center))
