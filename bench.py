@@ -161,6 +161,15 @@ def lod_leg(steps, warmup):
             tree["ancestor_path"], f, cam.height / (2 * cam.tanfovy), cam.tanfovx, cam.tanfovy,
             rmax, padding=0.1, radius_filter_threshold=700.0)
 
+    def cut_parent(cam, rmax):
+        f = cam.width / (2 * cam.tanfovx)
+        return lod.compute_visibility_radius_parent(
+            tree["means3D"], tree["log_scales"], tree["rotations"],
+            torch.as_tensor(cam.full_proj_transform, device=dev),
+            torch.as_tensor(cam.world_view_transform, device=dev), tree["tree_node_index"],
+            tree["index_parent"], f, cam.height / (2 * cam.tanfovy), cam.tanfovx, cam.tanfovy,
+            rmax, padding=0.1, radius_filter_threshold=700.0)
+
     res = {}
     for rmax in (5.0, 1.5):
         cut(cams[0], rmax)
@@ -170,11 +179,36 @@ def lod_leg(steps, warmup):
         ev[1].record()
         ev[1].synchronize()
         ms = ev[0].elapsed_time(ev[1]) / len(cams)
+        cut_parent(cams[0], rmax)
+        torch.cuda.synchronize()
+        ev[0].record()
+        masks_p = [cut_parent(c, rmax)[0] for c in cams]
+        ev[1].record()
+        ev[1].synchronize()
+        ms_p = ev[0].elapsed_time(ev[1]) / len(cams)
+        if not all(torch.equal(a, b) for a, b in zip(masks, masks_p)):
+            raise RuntimeError("parent-link cut differs from the ancestor-row cut")
         union = torch.stack(masks).any(0)
         nbytes = N * (44 + 4 + 9) + n_leaf * 4 * L
+        nbytes_p = N * (44 + 4 + 9) + n_leaf * 4
         res[rmax] = {"ms_per_view": round(ms, 3), "GB/s": round(nbytes / ms / 1e6, 1),
+                     "parent_links_ms_per_view": round(ms_p, 3),
+                     "parent_links_GB/s": round(nbytes_p / ms_p / 1e6, 1),
                      "kept_per_view": [int(m.sum().item()) for m in masks],
                      "union": int(union.sum().item())}
+    # the ancestor rows rebuilt on the GPU from the parent links (apps/ancestor.py)
+    sd = {"tree.index_parent": tree["index_parent"], "tree.depth": tree["level"],
+          "tree.ancestor_path": tree["ancestor_path"]}
+    lod.rebuild_ancestor_path(dict(sd))
+    torch.cuda.synchronize()
+    ev[0].record()
+    rows, _ = lod.rebuild_ancestor_path(dict(sd))
+    ev[1].record()
+    ev[1].synchronize()
+    ms_rebuild = ev[0].elapsed_time(ev[1])
+    if not torch.equal(rows, tree["ancestor_path"]):
+        raise RuntimeError("rebuilt ancestor rows differ")
+    del rows
     fg_res = filtergs_views(tree, cams, warmup)
     union = torch.stack([cut(c, 1.5)[0] for c in cams]).any(0)
     idx = torch.nonzero(union).reshape(-1)
@@ -212,7 +246,7 @@ def lod_leg(steps, warmup):
         raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
     return {"nodes": N, "leaves": n_leaf, "levels": L,
             "cut_radius_max_1.5": res[1.5], "cut_radius_max_5.0": res[5.0],
-            "filtergs_render": fg_res,
+            "filtergs_render": fg_res, "rebuild_ancestor_ms": round(ms_rebuild, 3),
             "gather_ms": round(ms_gather, 3), "gaussians_rendered": P, "kv_pairs": K,
             "batched_fwd_bwd_ms": round(ms, 3), "Mpix/s": round(npix / ms / 1000.0, 3),
             "views": len(cams), "pixels": npix}

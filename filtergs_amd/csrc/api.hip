@@ -784,6 +784,53 @@ int fgs_compute_visibility_radius(int32_t N, const float* means3D, const float* 
     return check(fgs::launch_lod_cut(a, (hipStream_t)stream), "compute_visibility_radius");
 }
 
+int fgs_compute_visibility_radius_parent(int32_t N, const float* means3D, const float* log_scales,
+                                         const float* rotations, const float* projmatrix,
+                                         const float* viewmatrix, const int32_t* tree_node_index,
+                                         const int32_t* index_parent, float focal_x, float focal_y,
+                                         float tan_fovx, float tan_fovy, float padding, float radius_max,
+                                         float radius_filter_threshold, uint8_t* mask, float* radius2d,
+                                         float* depth, void* ws, void* stream)
+{
+    if (N < 0) return fail(FGS_E_INVALID, "compute_visibility_radius: negative size%s");
+    if (N == 0) return FGS_OK;
+    if (!means3D || !log_scales || !rotations || !projmatrix || !viewmatrix || !tree_node_index ||
+        !index_parent || !mask || !radius2d || !depth || !ws)
+        return fail(FGS_E_INVALID, "compute_visibility_radius: null pointer%s");
+    fgs::LodArgs a{};
+    a.N = N;
+    a.means3d = means3D;
+    a.log_scales = log_scales;
+    a.rotations = rotations;
+    a.proj = projmatrix;
+    a.view = viewmatrix;
+    a.node_index = tree_node_index;
+    a.index_parent = index_parent;
+    a.fx = focal_x;
+    a.fy = focal_y;
+    a.tanx = tan_fovx;
+    a.tany = tan_fovy;
+    a.padding = padding;
+    a.radius_max = radius_max;
+    a.radius_filter_threshold = radius_filter_threshold;
+    a.mask = mask;
+    a.radius2d = radius2d;
+    a.depth = depth;
+    a.cull_flag = static_cast<uint8_t*>(ws);
+    return check(fgs::launch_lod_cut(a, (hipStream_t)stream), "compute_visibility_radius");
+}
+
+int fgs_rebuild_ancestor_path(int32_t N, int32_t L, const int32_t* index_parent, const int32_t* depth,
+                              int32_t* ancestor_path, int32_t* bad, void* stream)
+{
+    if (N < 0 || L < 0) return fail(FGS_E_INVALID, "rebuild_ancestor_path: negative size%s");
+    if (N == 0 || L == 0) return FGS_OK;
+    if (!index_parent || !depth || !ancestor_path || !bad)
+        return fail(FGS_E_INVALID, "rebuild_ancestor_path: null pointer%s");
+    return check(fgs::launch_rebuild_ancestor(N, L, index_parent, depth, ancestor_path, bad, (hipStream_t)stream),
+                 "rebuild_ancestor_path");
+}
+
 int fgs_visible_flag(int32_t B, int32_t N, const float* xyz, const float* proj_matrix,
                      float padding, uint8_t* valid, float* depth, float* p_proj, void* stream)
 {

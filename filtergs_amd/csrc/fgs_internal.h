@@ -239,6 +239,9 @@ struct LodArgs {
     const float* view;
     const int32_t* node_index;
     const int32_t* ancestor_path;
+    // instead of ancestor rows: walk the parent links (index_parent, -1 at a
+    // root) of a leaf, at most kLodMaxWalk steps
+    const int32_t* index_parent;
     float fx, fy, tanx, tany, padding, radius_max, radius_filter_threshold;
     uint8_t* mask;
     float* radius2d;
@@ -246,6 +249,11 @@ struct LodArgs {
     uint8_t* cull_flag;        // scratch [N]: internal & visible & 0 < r <= radius_max
 };
 hipError_t launch_lod_cut(const LodArgs& a, hipStream_t s);
+constexpr int kLodMaxWalk = 255;
+// ancestor rows [N, L] from parent links and depths (apps/ancestor.py:21-81);
+// *bad |= 1 when a chain is longer than L columns
+hipError_t launch_rebuild_ancestor(int N, int L, const int32_t* index_parent, const int32_t* depth,
+                                   int32_t* out, int* bad, hipStream_t s);
 
 // gather + activation of FilterGS.get_all
 struct ActivateArgs {

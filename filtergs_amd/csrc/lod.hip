@@ -90,6 +90,16 @@ __global__ __launch_bounds__(256) void k_lod_phase2(LodArgs a)
     bool keep = true;
     if (a.node_index[i] != -1) {
         keep = rv <= a.radius_max;
+    } else if (a.index_parent) {
+        // the same set of ancestors as the rows rebuilt from these links
+        int32_t p = a.index_parent[i];
+        for (int step = 0; p >= 0 && step < kLodMaxWalk; ++step) {
+            if (p != i && (a.cull_flag[p] & 2)) {
+                keep = false;
+                break;
+            }
+            p = a.index_parent[p];
+        }
     } else {
         const int32_t* row = a.ancestor_path + (size_t)i * a.L;
         for (int lv = 0; lv < a.L; ++lv) {
@@ -104,6 +114,44 @@ __global__ __launch_bounds__(256) void k_lod_phase2(LodArgs a)
     }
     if (keep && a.radius_filter_threshold > 0.0f && rv > a.radius_filter_threshold) keep = false;
     a.mask[i] = keep ? 1 : 0;
+}
+
+// Ancestor row of node i: column d holds its ancestor at depth d (d <
+// depth[i]), -1 elsewhere.  The reference fills rows level by level, copying
+// the parent's row (apps/ancestor.py:55-73); walking the parent links gives
+// the same rows when every parent is one level up.  A node whose parent is
+// -1 below the root level is an orphan: its row stays -1, and the rows of
+// its descendants stop at it.
+__global__ __launch_bounds__(256) void k_rebuild_ancestor(int N, int L, const int32_t* __restrict__ index_parent,
+                                                          const int32_t* __restrict__ depth,
+                                                          int32_t* __restrict__ out, int* bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int32_t* row = out + (size_t)i * L;
+    for (int c = 0; c < L; ++c) row[c] = -1;
+    int lvl = depth[i];
+    int cur = i;
+    while (lvl >= 1) {
+        const int32_t p = index_parent[cur];
+        if (p < 0 || p >= N) break;
+        if (lvl > L) {
+            atomicOr(bad, 1);
+            break;
+        }
+        row[lvl - 1] = p;
+        cur = p;
+        --lvl;
+    }
+}
+
+hipError_t launch_rebuild_ancestor(int N, int L, const int32_t* index_parent, const int32_t* depth,
+                                   int32_t* out, int* bad, hipStream_t s)
+{
+    if (N == 0 || L == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rebuild_ancestor, dim3((N + 255) / 256), dim3(256), 0, s, N, L, index_parent, depth,
+                       out, bad);
+    return hipGetLastError();
 }
 
 hipError_t launch_lod_cut(const LodArgs& a, hipStream_t s)

@@ -77,6 +77,80 @@ def compute_visibility_radius(means3d, scales, rotations, projmatrix, viewmatrix
         float(radius_max), float(radius_filter_threshold))
 
 
+def compute_visibility_radius_parent(means3d, scales, rotations, projmatrix, viewmatrix,
+                                     tree_node_index, index_parent, focal_x, focal_y, tan_fovx,
+                                     tan_fovy, radius_max, padding=0.05, radius_filter_threshold=100.0):
+    """compute_visibility_radius with the tree's parent links (tree.index_parent,
+    -1 at a root) instead of the [N, L] ancestor rows: the same mask as the
+    rows rebuild_ancestor_path makes from those links, reading 4 bytes per
+    node instead of 4 L."""
+    tensors = (means3d, scales, rotations, projmatrix, viewmatrix, tree_node_index, index_parent)
+    if not all(t.is_cuda for t in tensors):
+        raise ValueError("All input tensors must be CUDA tensors")
+    P = int(means3d.size(0))
+    dev = means3d.device
+    mask = torch.zeros((P,), dtype=torch.bool, device=dev)
+    radius2d = torch.full((P,), -1.0, dtype=torch.float32, device=dev)
+    depth = torch.zeros((P,), dtype=torch.float32, device=dev)
+    if P == 0:
+        return mask, radius2d, depth
+    m, s, r = _f32c(means3d, dev), _f32c(scales, dev), _f32c(rotations, dev)
+    pm, vm = _f32c(projmatrix, dev), _f32c(viewmatrix, dev)
+    ni = tree_node_index.to(device=dev, dtype=torch.int32).contiguous()
+    ip = index_parent.to(device=dev, dtype=torch.int32).contiguous()
+    ws = torch.empty((_lib.load().fgs_lod_ws_bytes(P),), dtype=torch.uint8, device=dev)
+    call("fgs_compute_visibility_radius_parent", P, ptr(m), ptr(s), ptr(r), ptr(pm), ptr(vm), ptr(ni),
+         ptr(ip), float(focal_x), float(focal_y), float(tan_fovx), float(tan_fovy), float(padding),
+         float(radius_max), float(radius_filter_threshold), mask.data_ptr(), radius2d.data_ptr(),
+         depth.data_ptr(), ws.data_ptr(), _lib.stream_handle(dev))
+    return mask, radius2d, depth
+
+
+def rebuild_ancestor_path(state_dict, device=None):
+    """rebuild_ancestor_path of apps/ancestor.py:21-81 with the rows built on
+    the GPU: reads tree.index_parent and tree.depth, writes
+    state_dict["tree.ancestor_path"] (and "tree.ancestor" when present) with
+    the width of an existing ancestor buffer, else depth.max() + 1; returns
+    (ancestor_path, max_depth).  The rows land on the device of
+    tree.index_parent unless ``device`` names a GPU to build them on, in
+    which case they stay there."""
+    for key in ("tree.index_parent", "tree.depth"):
+        if key not in state_dict:
+            raise KeyError(f"Missing '{key}' in checkpoint state_dict")
+    index_parent = state_dict["tree.index_parent"]
+    depth = state_dict["tree.depth"]
+    home = index_parent.device
+    dev = torch.device(device) if device is not None else (home if home.type == "cuda" else torch.device("cuda"))
+    _lib.require_gpu(torch.empty(0, device=dev))
+    ancestor_key = next((k for k in ("tree.ancestor_path", "tree.ancestor") if k in state_dict), None)
+    N = int(depth.numel())
+    if ancestor_key is not None and state_dict[ancestor_key].dim() == 2:
+        L = int(state_dict[ancestor_key].shape[1])
+        out_dtype = state_dict[ancestor_key].dtype
+    else:
+        L = int(depth.max().item()) + 1 if N > 0 else 1
+        out_dtype = torch.int32
+    ip = index_parent.to(device=dev, dtype=torch.int32).contiguous()
+    dp = depth.to(device=dev, dtype=torch.int32).contiguous()
+    out = torch.empty((N, L), dtype=torch.int32, device=dev)
+    bad = torch.zeros((1,), dtype=torch.int32, device=dev)
+    max_depth = 0
+    if N > 0 and L > 0:
+        call("fgs_rebuild_ancestor_path", N, L, ptr(ip), ptr(dp), out.data_ptr(), bad.data_ptr(),
+             _lib.stream_handle(dev))
+        if int(bad.item()):
+            raise IndexError(f"a node's depth exceeds the {L} ancestor columns")
+        max_depth = int(dp.max().item())
+    else:
+        out.fill_(-1)
+    target = dev if device is not None else home
+    out = out.to(device=target, dtype=out_dtype)
+    state_dict["tree.ancestor_path"] = out
+    if "tree.ancestor" in state_dict:
+        state_dict["tree.ancestor"] = out
+    return out, max_depth
+
+
 def _visible_flag(xyz, proj_matrix, padding, batch):
     dev = xyz.device
     B = int(xyz.size(0)) if batch else 1

@@ -99,7 +99,7 @@ def lod_tree(levels: int = 12, roots_side: int = 2, extent: float = 2000.0, sh_d
     rotations [N,4] (raw), opacities [N,1], shs [N,M,3], tree_node_index [N]
     (own index for internal nodes, -1 for leaves), ancestor_path [N,levels]
     (ancestor at depth d in column d, -1 beyond the node's depth: the
-    apps/ancestor.py format), level [N].
+    apps/ancestor.py format), index_parent [N] (-1 at a root), level [N].
     """
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -145,8 +145,12 @@ def lod_tree(levels: int = 12, roots_side: int = 2, extent: float = 2000.0, sh_d
             sh = k - d
             ad = (ly >> sh) * sides[d] + (lx >> sh) + int(offs[d])
             anc[int(offs[k]):int(offs[k + 1]), d] = ad.to(torch.int32)
+    parent = torch.full((N,), -1, dtype=torch.int32)
+    lv = level.to(torch.int64)
+    nz = lv > 0
+    parent[nz] = anc[nz, lv[nz] - 1]
     out = dict(means3D=means, log_scales=log_scales, rotations=q, opacities=opac, shs=shs,
-               tree_node_index=node_index, ancestor_path=anc, level=level)
+               tree_node_index=node_index, ancestor_path=anc, index_parent=parent, level=level)
     return {k: v.contiguous().to(dev) for k, v in out.items()}
 
 

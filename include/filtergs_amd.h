@@ -125,6 +125,28 @@ int fgs_forward_render_stats(const fgs_raster_settings* s, const fgs_gaussians* 
 /* Doubles of the stats buffer of fgs_forward_render_stats at W x H. */
 size_t fgs_stats_words(int32_t image_width, int32_t image_height);
 
+/* The LoD cut with the tree's parent links (index_parent int32 [N], -1 at a
+ * root) in place of the ancestor rows: a leaf is culled when an ancestor on
+ * its parent chain culls its subtree.  Same mask as fgs_compute_visibility_
+ * radius with the rows fgs_rebuild_ancestor_path builds from those links, and
+ * no [N, L] array (FilterGS/cuda/compute_radius_kernel_v2.cu:231-353 reads
+ * the rows; this reads 4 bytes per node). */
+int fgs_compute_visibility_radius_parent(int32_t N, const float* means3D, const float* log_scales,
+                                         const float* rotations, const float* projmatrix,
+                                         const float* viewmatrix, const int32_t* tree_node_index,
+                                         const int32_t* index_parent, float focal_x, float focal_y,
+                                         float tan_fovx, float tan_fovy, float padding, float radius_max,
+                                         float radius_filter_threshold, uint8_t* mask, float* radius2d,
+                                         float* depth, void* ws, void* stream);
+
+/* rebuild_ancestor_path (apps/ancestor.py:21-81) on the device: row i of
+ * ancestor_path [N, L] int32 gets the ancestor of node i at depth d in column
+ * d (d < depth[i]) and -1 elsewhere; index_parent and depth are int32 [N].
+ * An orphan (parent -1 below the root level) keeps a row of -1.  *bad
+ * (device int32, zero on entry) gets bit 0 when a depth exceeds L. */
+int fgs_rebuild_ancestor_path(int32_t N, int32_t L, const int32_t* index_parent, const int32_t* depth,
+                              int32_t* ancestor_path, int32_t* bad, void* stream);
+
 /* FilterGS render rows (FilterGS/filtergs.py:98-223: get_all, then render):
  * the rasterizer draws row index[i] of the raw parameter arrays for i < N,
  * activated while it is loaded -- scale exp(scaling), opacity

@@ -398,3 +398,34 @@ def activate_gather(index, xyz, log_scaling, rotation, opacity_logit, features_d
     return dict(xyz=xyz_g, scaling=np.exp(f(log_scaling)),
                 opacity=1.0 / (1.0 + np.exp(-f(opacity_logit).reshape(-1))),
                 rotation=q / qn, colors=colors)
+
+
+# ---------------------------------------------------------------------------
+# Tree formats (apps/ancestor.py), numpy.
+# ---------------------------------------------------------------------------
+def rebuild_ancestor_path(index_parent, depth, L):
+    """apps/ancestor.py:21-81 restated: rows start at -1; level by level
+    (1 .. max depth), a node with a parent copies the parent's row and puts
+    the parent in column level-1; an orphan (parent -1) gets a row of -1.
+    Returns the int32 [N, L] rows; raises IndexError when a level exceeds L
+    (the reference's column index would)."""
+    index_parent = np.asarray(index_parent, dtype=np.int64)
+    depth = np.asarray(depth, dtype=np.int64)
+    N = depth.size
+    out = np.full((N, L), -1, dtype=np.int32)
+    if N == 0 or L == 0:
+        return out
+    for level in range(1, int(depth.max()) + 1):
+        nodes = np.nonzero(depth == level)[0]
+        if nodes.size == 0:
+            continue
+        parents = index_parent[nodes]
+        valid = parents >= 0
+        if valid.any():
+            if level - 1 >= L:
+                raise IndexError(f"level {level} exceeds {L} ancestor columns")
+            nv, pv = nodes[valid], parents[valid]
+            out[nv] = out[pv]
+            out[nv, level - 1] = pv.astype(np.int32)
+        out[nodes[~valid]] = -1
+    return out
