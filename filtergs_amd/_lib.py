@@ -128,6 +128,11 @@ _SIGS = {
                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, ctypes.c_float, c_void_p]),
     "fgs_stats_words": (c_size_t, [c_int32, c_int32]),
+    "fgs_sparse_adam": (c_int32, [c_int64, c_int32, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p]),
+    "fgs_counter_update": (c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p,
+                                     c_void_p, c_int64] + [c_void_p] * 10),
     "fgs_compute_visibility_radius_parent": (c_int32, [c_int32] + [c_void_p] * 7 + [c_float] * 7 +
                                              [c_void_p] * 5),
     "fgs_rebuild_ancestor_path": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -820,6 +820,78 @@ int fgs_compute_visibility_radius_parent(int32_t N, const float* means3D, const 
     return check(fgs::launch_lod_cut(a, (hipStream_t)stream), "compute_visibility_radius");
 }
 
+int fgs_sparse_adam(int64_t n, int32_t D, const int64_t* index, const uint8_t* flag, int64_t N, float* param,
+                    const float* param_sel, const float* grad, float* exp_avg, float* exp_avg_sq,
+                    float* max_exp_avg_sq, double lr, double beta1, double beta2, double eps, double step,
+                    int32_t* bad, void* stream)
+{
+    if (n < 0 || N < 0 || D <= 0) return fail(FGS_E_INVALID, "sparse_adam: bad size%s");
+    if (n == 0) return FGS_OK;
+    if (!index || !param || !param_sel || !grad || !exp_avg || !exp_avg_sq || !bad)
+        return fail(FGS_E_INVALID, "sparse_adam: null pointer%s");
+    if (!(step >= 1.0)) return fail(FGS_E_INVALID, "sparse_adam: step must be >= 1%s");
+    fgs::SparseAdamArgs a{};
+    a.n = n;
+    a.N = N;
+    a.D = D;
+    a.index = index;
+    a.flag = flag;
+    a.param = param;
+    a.param_sel = param_sel;
+    a.grad = grad;
+    a.exp_avg = exp_avg;
+    a.exp_avg_sq = exp_avg_sq;
+    a.max_exp_avg_sq = max_exp_avg_sq;
+    a.beta1 = (float)beta1;
+    a.beta2 = (float)beta2;
+    // Python scalars of the reference (sparse_optimizer.py:51-78): 1 - beta,
+    // the bias corrections and the step size in double, then used against
+    // float32 tensors
+    a.one_m_beta1 = (float)(1.0 - beta1);
+    a.one_m_beta2 = (float)(1.0 - beta2);
+    const double bc1 = 1.0 - std::pow(beta1, step);
+    const double bc2 = 1.0 - std::pow(beta2, step);
+    a.bc2_sqrt = (float)std::sqrt(bc2);
+    a.neg_step_size = (float)(-(lr / bc1));
+    a.eps = (float)eps;
+    a.bad = bad;
+    return check(fgs::launch_sparse_adam(a, (hipStream_t)stream), "sparse_adam");
+}
+
+int fgs_counter_update(int64_t n_vis, const int64_t* visible_index, const int32_t* radii,
+                       const float* point_weight, int64_t n_pid, const int64_t* point_id,
+                       const int64_t* point_count, const float* grad, int64_t N, int32_t* create_steps,
+                       int16_t* visible_count, float* weights_max, float* weights_sum, int16_t* radii_max,
+                       int32_t* radii_max_max, int32_t* area_sum, float* grad_sum, int32_t* bad, void* stream)
+{
+    if (n_vis < 0 || n_pid < 0 || N < 0) return fail(FGS_E_INVALID, "counter_update: negative size%s");
+    if (n_vis == 0 && n_pid == 0) return FGS_OK;
+    if (!visible_index || !radii || !point_weight || (n_pid > 0 && (!point_id || !point_count)) ||
+        !create_steps || !visible_count || !weights_max || !weights_sum || !radii_max || !radii_max_max ||
+        !area_sum || !grad_sum || !bad)
+        return fail(FGS_E_INVALID, "counter_update: null pointer%s");
+    fgs::CounterArgs a{};
+    a.n_vis = n_vis;
+    a.n_pid = n_pid;
+    a.N = N;
+    a.visible_index = visible_index;
+    a.radii = radii;
+    a.point_weight = point_weight;
+    a.point_id = point_id;
+    a.point_count = point_count;
+    a.grad = grad;
+    a.create_steps = create_steps;
+    a.visible_count = visible_count;
+    a.weights_max = weights_max;
+    a.weights_sum = weights_sum;
+    a.radii_max = radii_max;
+    a.radii_max_max = radii_max_max;
+    a.area_sum = area_sum;
+    a.grad_sum = grad_sum;
+    a.bad = bad;
+    return check(fgs::launch_counter_update(a, (hipStream_t)stream), "counter_update");
+}
+
 int fgs_rebuild_ancestor_path(int32_t N, int32_t L, const int32_t* index_parent, const int32_t* depth,
                               int32_t* ancestor_path, int32_t* bad, void* stream)
 {

@@ -229,6 +229,43 @@ hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg,
                                   float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
                                   float* dL_dcolors, int exact, hipStream_t s);
 
+// training-side neighbours (train.hip)
+struct SparseAdamArgs {
+    int64_t n, N;              // selected rows, rows of the parameter
+    int D;                     // floats per row
+    const int64_t* index;      // [n] parameter row of selected row j
+    const uint8_t* flag;       // [n] row j takes the step (null: all)
+    float* param;              // [N, D]
+    const float* param_sel;    // [n, D] the gathered rows the gradient belongs to
+    const float* grad;         // [n, D]
+    float* exp_avg;            // [N, D]
+    float* exp_avg_sq;         // [N, D]
+    float* max_exp_avg_sq;     // [N, D] or null (no amsgrad)
+    float beta1, beta2, one_m_beta1, one_m_beta2, eps, bc2_sqrt, neg_step_size;
+    int* bad;                  // bit 0: a row index outside [0, N)
+};
+hipError_t launch_sparse_adam(const SparseAdamArgs& a, hipStream_t s);
+
+struct CounterArgs {
+    int64_t n_vis, n_pid, N;
+    const int64_t* visible_index;   // [n_vis] point of visible row j
+    const int32_t* radii;           // [n_vis]
+    const float* point_weight;      // [n_vis]
+    const int64_t* point_id;        // [n_pid] visible rows that are the max at some pixel
+    const int64_t* point_count;     // [n_pid] their pixel counts
+    const float* grad;              // [n_vis, 3] view-space gradient, or null
+    int32_t* create_steps;
+    int16_t* visible_count;
+    float* weights_max;
+    float* weights_sum;
+    int16_t* radii_max;
+    int32_t* radii_max_max;
+    int32_t* area_sum;
+    float* grad_sum;
+    int* bad;                       // bit 0: point index outside [0, N); bit 1: point id outside the rows
+};
+hipError_t launch_counter_update(const CounterArgs& a, hipStream_t s);
+
 // LoD cut and per-point utilities
 struct LodArgs {
     int N, L;

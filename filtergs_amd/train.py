@@ -1,0 +1,228 @@
+"""Training-side neighbours of the LoG render path on the gfx950 kernels.
+
+* ``SparseOptimizer`` -- LoG/model/sparse_optimizer.py:116-195 with the same
+  constructor, buffers and ``step(model, index, params, flag_vis)``: per
+  parameter one kernel (``fgs_sparse_adam``) reads the visible rows' Adam state
+  in place, updates it and writes the parameter rows, instead of gathering the
+  state into copies, ten element-wise passes and scattering it back.  The
+  state lives on the device (the reference can keep it on the host; with
+  288 GB of HBM it stays next to the parameters).
+* ``Counter`` -- LoG/model/counter.py:4-83 with ``update_by_output`` as two
+  kernels per render (``fgs_counter_update``).
+* ``get_expon_lr_func`` -- the learning-rate schedule (sparse_optimizer.py:6-39).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from . import _lib
+from ._lib import call, ptr
+
+
+def get_expon_lr_func(lr_init, lr_final, lr_delay_steps=0, lr_delay_mult=1.0, max_steps=1000000):
+    """Log-linear decay from lr_init at step 0 to lr_final at max_steps, with an
+    optional sine warm-up over lr_delay_steps (sparse_optimizer.py:6-39)."""
+
+    def helper(step):
+        if step < 0 or (lr_init == 0.0 and lr_final == 0.0):
+            return 0.0
+        if lr_delay_steps > 0:
+            delay_rate = lr_delay_mult + (1 - lr_delay_mult) * np.sin(
+                0.5 * np.pi * np.clip(step / lr_delay_steps, 0, 1))
+        else:
+            delay_rate = 1.0
+        t = np.clip(step / max_steps, 0, 1)
+        log_lerp = np.exp(np.log(lr_init) * (1 - t) + np.log(lr_final) * t)
+        return delay_rate * log_lerp
+
+    return helper
+
+
+class BufferDict(nn.Module):
+    """sparse_optimizer.py:95-117."""
+
+    def __init__(self, dicts):
+        super().__init__()
+        for key, value in dicts.items():
+            self.register_buffer(key, value)
+        self.keys = list(dicts.keys())
+
+    @property
+    def device(self):
+        return self[self.keys[0]].device
+
+    def __getitem__(self, key):
+        return getattr(self, key)
+
+    def __repr__(self):
+        return f'BufferDict: {self.keys}'
+
+    def items(self):
+        for key in self.keys:
+            yield key, getattr(self, key)
+
+    def get(self, key, default_value=None):
+        if key not in self.keys:
+            return default_value
+        return getattr(self, key)
+
+
+class SparseOptimizer(nn.Module):
+    """Adam over the visible rows only (sparse_optimizer.py:119-195)."""
+
+    def __init__(self, optimize_keys, lr_dict, model, device=None, xyz_scale=None, use_amsgrad=False) -> None:
+        super().__init__()
+        if device is None:
+            device = model.xyz.device
+        self.register_buffer('global_steps', torch.tensor(0, dtype=torch.float32, device=device))
+        self.device = device
+        self.optimize_keys = optimize_keys
+        self.state_keys = ['exp_avg', 'exp_avg_sq']
+        self.lr_dict = lr_dict
+        exp_avg, exp_avg_sq, max_exp_avg_sq, steps = {}, {}, {}, {}
+        for key in optimize_keys:
+            param = getattr(model, key)
+            z = torch.zeros_like(param, device=device)
+            exp_avg[key] = z
+            exp_avg_sq[key] = z.clone()
+            if use_amsgrad:
+                max_exp_avg_sq[key] = z.clone()
+            steps[key] = torch.zeros((param.shape[0],), dtype=torch.int32, device=device)
+        self.exp_avg = BufferDict(exp_avg)
+        self.exp_avg_sq = BufferDict(exp_avg_sq)
+        if use_amsgrad:
+            self.max_exp_avg_sq = BufferDict(max_exp_avg_sq)
+            self.state_keys.append('max_exp_avg_sq')
+        self.use_amsgrad = use_amsgrad
+        self.steps = BufferDict(steps)
+        if xyz_scale is not None:
+            self.xyz_scheduler_args = get_expon_lr_func(
+                lr_init=lr_dict['xyz'] * xyz_scale,
+                lr_final=lr_dict.get('xyz_final', lr_dict['xyz'] * 0.01) * xyz_scale,
+                max_steps=int(lr_dict['max_steps']))
+            if 'scaling' in lr_dict:
+                self.scaling_scheduler_args = get_expon_lr_func(
+                    lr_init=lr_dict['scaling'],
+                    lr_final=lr_dict.get('scaling_final', lr_dict['scaling'] * 1),
+                    max_steps=int(lr_dict['max_steps']))
+            self.xyz_lr = lr_dict['xyz'] * xyz_scale
+
+    def _lr(self, key, step):
+        if key == 'xyz':
+            lr = self.xyz_scheduler_args(step)
+            self.xyz_lr = lr
+            return lr
+        if key == 'scaling':
+            return self.scaling_scheduler_args(step)
+        return self.lr_dict[key]
+
+    def step(self, model, index, params, flag_vis):
+        """Adam step of row index[j] for every j with flag_vis[j], from the
+        gathered parameters ``params`` and their gradients (eps 1e-15, betas
+        0.9 / 0.999, the global step for the bias correction)."""
+        self.global_steps += 1
+        step = float(self.global_steps.item())
+        dev = params['xyz'].device if 'xyz' in params else next(iter(params.values())).device
+        _lib.require_gpu(torch.empty(0, device=dev))
+        idx = index.to(device=dev, dtype=torch.int64).contiguous()
+        flag = flag_vis.to(device=dev, dtype=torch.uint8).contiguous()
+        n = int(idx.numel())
+        if int(flag.numel()) != n:
+            raise ValueError("index and flag_vis differ in length")
+        bad = torch.zeros((1,), dtype=torch.int32, device=dev)
+        stream = _lib.stream_handle(dev)
+        for key, param in params.items():
+            if param.grad is None:
+                continue
+            lr = self._lr(key, step)
+            target = getattr(model, key)
+            m, v = self.exp_avg[key], self.exp_avg_sq[key]
+            vmax = self.max_exp_avg_sq[key] if self.use_amsgrad else None
+            for t in (target, m, v) + ((vmax,) if vmax is not None else ()):
+                if t.device != dev or t.dtype != torch.float32 or not t.is_contiguous():
+                    raise ValueError(f"{key}: parameter and optimiser state must be contiguous float32 on {dev}")
+            D = int(target[0].numel()) if target.shape[0] > 0 else 1
+            sel = param.data.to(torch.float32).contiguous()
+            grad = param.grad.to(torch.float32).contiguous()
+            call("fgs_sparse_adam", n, D, idx.data_ptr(), flag.data_ptr(), int(target.shape[0]), target.data.data_ptr(),
+                 sel.data_ptr(), grad.data_ptr(), m.data_ptr(), v.data_ptr(), ptr(vmax), float(lr), 0.9, 0.999,
+                 1e-15, step, bad.data_ptr(), stream)
+        if int(bad.item()):
+            raise IndexError("SparseOptimizer.step: index outside the parameter rows")
+
+
+class Counter(nn.Module):
+    """Per-point training statistics (counter.py:4-83)."""
+
+    def __init__(self, num_points, add_depth=False) -> None:
+        super().__init__()
+        zero_float32 = torch.zeros((num_points,), dtype=torch.float32)
+        zero_int16 = torch.zeros((num_points,), dtype=torch.int16)
+        zero_int32 = torch.zeros((num_points,), dtype=torch.int32)
+        self.register_buffer('weights_max', zero_float32.clone())
+        self.register_buffer('weights_sum', zero_float32.clone())
+        self.register_buffer('grad_sum', zero_float32.clone())
+        for name in ['radii_max', 'visible_count']:
+            self.register_buffer(name, zero_int16.clone())
+        self.register_buffer('radii_max_max', zero_int32.clone())
+        self.register_buffer('area_sum', zero_int32.clone())
+        self.register_buffer('radius3d_min', zero_float32.clone() + 1)
+        self.register_buffer('radius3d_max', zero_float32.clone() + 1)
+        self.register_buffer('create_steps', zero_int32.clone())
+
+    def get_gradmean(self):
+        return self.grad_sum / torch.clamp(self.area_sum, min=1)
+
+    def reset(self, num_points):
+        for key in ['weights_max', 'weights_sum', 'radii_max', 'radii_max_max', 'area_sum', 'grad_sum',
+                    'visible_count']:
+            data = getattr(self, key)
+            data.set_(torch.zeros((num_points,), device=data.device, dtype=data.dtype))
+
+    def reset_create_steps(self):
+        self.create_steps.fill_(0)
+
+    def update_by_output(self, output, fix_parent=False):
+        """counter.py:36-83 for every render of ``output``; a render with no
+        visible rows, no point ids or a point id past its rows is skipped, as
+        there."""
+        N = int(self.area_sum.shape[0])
+        for i in range(len(output['render'])):
+            vf = output['visibility_flag'][i]
+            visible_index = vf['index']
+            if 'index_node' in vf:
+                visible_index = torch.cat([visible_index, vf['index_node']])
+            if visible_index.numel() == 0:
+                continue
+            radii = output['radii'][i]
+            flag_vis = radii > 0
+            vf['flag_vis'] = flag_vis
+            vf['index_vis'] = torch.where(flag_vis)[0]
+            point_id = output['point_id'][i]
+            if point_id.numel() == 0:
+                continue
+            if int(point_id.max()) >= visible_index.numel():
+                continue
+            dev = self.area_sum.device
+            _lib.require_gpu(self.area_sum)
+            vi = visible_index.to(device=dev, dtype=torch.int64).contiguous()
+            rd = radii.to(device=dev, dtype=torch.int32).contiguous()
+            pw = output['point_weight'][i].data.to(device=dev, dtype=torch.float32).contiguous()
+            pid = point_id.to(device=dev, dtype=torch.int64).contiguous()
+            pc = output['point_count'][i].to(device=dev, dtype=torch.int64).contiguous()
+            g = output['viewspace_points'][i].grad
+            gr = g.to(device=dev, dtype=torch.float32).contiguous() if g is not None else None
+            if gr is not None and (gr.dim() != 2 or gr.shape[1] != 3):
+                raise ValueError("viewspace gradient must be [n, 3]")
+            bad = torch.zeros((1,), dtype=torch.int32, device=dev)
+            call("fgs_counter_update", int(vi.numel()), vi.data_ptr(), rd.data_ptr(), pw.data_ptr(),
+                 int(pid.numel()), pid.data_ptr(), pc.data_ptr(), ptr(gr), N, self.create_steps.data_ptr(),
+                 self.visible_count.data_ptr(), self.weights_max.data_ptr(), self.weights_sum.data_ptr(),
+                 self.radii_max.data_ptr(), self.radii_max_max.data_ptr(), self.area_sum.data_ptr(),
+                 self.grad_sum.data_ptr(), bad.data_ptr(), _lib.stream_handle(dev))
+            if int(bad.item()):
+                raise IndexError("Counter.update_by_output: index outside the counters")

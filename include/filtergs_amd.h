@@ -147,6 +147,36 @@ int fgs_compute_visibility_radius_parent(int32_t N, const float* means3D, const 
 int fgs_rebuild_ancestor_path(int32_t N, int32_t L, const int32_t* index_parent, const int32_t* depth,
                               int32_t* ancestor_path, int32_t* bad, void* stream);
 
+/* One step of LoG's SparseOptimizer for one parameter
+ * (LoG/model/sparse_optimizer.py:41-78, 163-195): for each selected row j < n
+ * with flag[j] (flag may be null: all rows), parameter row i = index[j] gets
+ * the Adam update of the gradient row grad[j] from the gathered row
+ * param_sel[j], and exp_avg / exp_avg_sq (/ max_exp_avg_sq with amsgrad,
+ * else null) rows i are updated in place -- the reference's gather of the
+ * state, update and scatter back, without the copies.  Arrays are float32:
+ * param, exp_avg, exp_avg_sq [N, D]; param_sel, grad [n, D].  step is the
+ * global step (>= 1); *bad (device, zero on entry) gets bit 0 for a row
+ * index outside [0, N).  Rows must be distinct. */
+int fgs_sparse_adam(int64_t n, int32_t D, const int64_t* index, const uint8_t* flag, int64_t N, float* param,
+                    const float* param_sel, const float* grad, float* exp_avg, float* exp_avg_sq,
+                    float* max_exp_avg_sq, double lr, double beta1, double beta2, double eps, double step,
+                    int32_t* bad, void* stream);
+
+/* Counter.update_by_output for one render (LoG/model/counter.py:36-83):
+ * visible row j (point visible_index[j]) with radii[j] > 0 adds to
+ * create_steps, visible_count, weights_sum and maxes weights_max, radii_max;
+ * max-weight point k (row point_id[k], point_count[k] pixels) adds its area
+ * to area_sum, |grad[j, :2]| * area to grad_sum (grad [n_vis, 3], may be
+ * null) and maxes radii_max_max.  Counters are the Counter buffers over N
+ * points; visible rows and point ids must be distinct.  *bad (device, zero
+ * on entry): bit 0 point index outside [0, N), bit 1 point id outside the
+ * rows. */
+int fgs_counter_update(int64_t n_vis, const int64_t* visible_index, const int32_t* radii,
+                       const float* point_weight, int64_t n_pid, const int64_t* point_id,
+                       const int64_t* point_count, const float* grad, int64_t N, int32_t* create_steps,
+                       int16_t* visible_count, float* weights_max, float* weights_sum, int16_t* radii_max,
+                       int32_t* radii_max_max, int32_t* area_sum, float* grad_sum, int32_t* bad, void* stream);
+
 /* FilterGS render rows (FilterGS/filtergs.py:98-223: get_all, then render):
  * the rasterizer draws row index[i] of the raw parameter arrays for i < N,
  * activated while it is loaded -- scale exp(scaling), opacity

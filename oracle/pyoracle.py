@@ -7,6 +7,7 @@ numpy; the f32 functions take float32 arrays, the f64 ones float64 arrays.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -429,3 +430,56 @@ def rebuild_ancestor_path(index_parent, depth, L):
             out[nv, level - 1] = pv.astype(np.int32)
         out[nodes[~valid]] = -1
     return out
+
+
+# ---------------------------------------------------------------------------
+# Training-side neighbours, float32 numpy in the reference's operation order.
+# ---------------------------------------------------------------------------
+def sparse_adam_step(param, param_sel, grad, exp_avg, exp_avg_sq, index, flag, lr, step,
+                     beta1=0.9, beta2=0.999, eps=1e-15, max_exp_avg_sq=None):
+    """LoG/model/sparse_optimizer.py:41-78 + 163-195 restated: rows index[j]
+    with flag[j] of param / exp_avg / exp_avg_sq (float32 [N, D]) take one
+    Adam step from param_sel[j], grad[j].  Returns updated copies
+    (param, exp_avg, exp_avg_sq, max_exp_avg_sq)."""
+    f32 = np.float32
+    p, m_all, v_all = param.copy(), exp_avg.copy(), exp_avg_sq.copy()
+    vmax_all = None if max_exp_avg_sq is None else max_exp_avg_sq.copy()
+    sel = np.nonzero(np.asarray(flag, bool))[0]
+    rows = np.asarray(index)[sel]
+    g = grad[sel].astype(f32)
+    m = m_all[rows] * f32(beta1) + f32(1.0 - beta1) * g
+    v = v_all[rows] * f32(beta2) + f32(1.0 - beta2) * (g * g)
+    bc1 = 1.0 - beta1 ** step
+    bc2 = 1.0 - beta2 ** step
+    vd = v
+    if vmax_all is not None:
+        vd = np.maximum(vmax_all[rows], v)
+        vmax_all[rows] = vd
+    denom = np.sqrt(vd) / f32(math.sqrt(bc2)) + f32(eps)
+    p[rows] = param_sel[sel] + f32(-(lr / bc1)) * (m / denom)
+    m_all[rows] = m
+    v_all[rows] = v
+    return p, m_all, v_all, vmax_all
+
+
+def counter_update(counters, visible_index, radii, point_weight, point_id, point_count, grad):
+    """LoG/model/counter.py:36-83 restated for one render (distinct rows):
+    returns updated copies of the counter dict (create_steps, visible_count,
+    weights_max, weights_sum, radii_max, radii_max_max, area_sum, grad_sum)."""
+    c = {k: v.copy() for k, v in counters.items()}
+    vi = np.asarray(visible_index)
+    vis = np.nonzero(np.asarray(radii) > 0)[0]
+    rows = vi[vis]
+    c["create_steps"][rows] += 1
+    c["visible_count"][rows] = (c["visible_count"][rows] + 1).astype(np.int16)
+    c["weights_max"][rows] = np.maximum(c["weights_max"][rows], point_weight[vis])
+    c["weights_sum"][rows] = c["weights_sum"][rows] + point_weight[vis]
+    c["radii_max"][rows] = np.maximum(c["radii_max"][rows], np.asarray(radii)[vis].astype(np.int16))
+    prow = vi[np.asarray(point_id)]
+    pc = np.asarray(point_count)
+    c["area_sum"][prow] = (c["area_sum"][prow] + pc).astype(np.int32)
+    if grad is not None:
+        gn = np.sqrt(grad[:, 0] * grad[:, 0] + grad[:, 1] * grad[:, 1]).astype(np.float32)
+        c["grad_sum"][prow] = c["grad_sum"][prow] + gn[np.asarray(point_id)] * pc.astype(np.float32)
+    c["radii_max_max"][prow] = np.maximum(pc.astype(np.int32), c["radii_max_max"][prow])
+    return c
