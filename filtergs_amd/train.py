@@ -10,6 +10,8 @@
 * ``Counter`` -- LoG/model/counter.py:4-83 with ``update_by_output`` as two
   kernels per render (``fgs_counter_update``).
 * ``get_expon_lr_func`` -- the learning-rate schedule (sparse_optimizer.py:6-39).
+* ``render_to_check`` -- the point-weight probe of LoG.prepare
+  (level_of_gaussian.py:205-219) as a forward without statistics or autograd.
 """
 from __future__ import annotations
 
@@ -226,3 +228,25 @@ class Counter(nn.Module):
                  self.grad_sum.data_ptr(), bad.data_ptr(), _lib.stream_handle(dev))
             if int(bad.item()):
                 raise IndexError("Counter.update_by_output: index outside the counters")
+
+
+@torch.no_grad()
+def render_to_check(rasterizer, xyz, scaling, rotation, opacity):
+    """point_weight of a white render of the given Gaussians
+    (LoG/model/level_of_gaussian.py:205-219).  The reference goes through the
+    rasterizer module, which also collects RenderStats (on by default); only
+    the per-Gaussian maximum weight is used, so the forward here runs without
+    statistics and without an autograd record."""
+    from . import ext
+    rs = rasterizer.raster_settings
+    P = int(xyz.shape[0])
+    if P == 0:
+        return torch.zeros((0,), dtype=torch.float32, device=xyz.device)
+    out = ext.rasterize_gaussians(
+        rs.bg, xyz, torch.ones_like(xyz), opacity, scaling, rotation, rs.scale_modifier, torch.Tensor([]),
+        rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy, rs.image_height, rs.image_width,
+        torch.Tensor([]), rs.sh_degree, rs.campos, rs.prefiltered, True, rs.debug,
+        ogfs_epsilon_scale=rs.ogfs_epsilon_scale, ogfs_epsilon_max=rs.ogfs_epsilon_max,
+        ogfs_energy_floor=rs.ogfs_energy_floor, ogfs_radius_ratio_min=rs.ogfs_radius_ratio_min,
+        ogfs_enable_stats=False)
+    return out[4]

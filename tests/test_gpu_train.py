@@ -134,3 +134,23 @@ def test_counter_update_matches_oracle():
         else:
             np.testing.assert_allclose(got, v, rtol=1e-6, atol=1e-6, err_msg=k)
     assert "flag_vis" in outputs["visibility_flag"][0]
+
+
+def test_render_to_check_equals_rasterizer_point_weight():
+    from filtergs_amd.camera import make_camera
+    from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+    from filtergs_amd.train import render_to_check
+    from tests._scenes import small_scene
+    g, cam = small_scene(n=4000, W=200, H=150, seed=12)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float32, device=DEV)
+    rs = GaussianRasterizationSettings(
+        image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
+        bg=t([0.0, 0.0, 0.0]), scale_modifier=1.0, viewmatrix=t(cam.world_view_transform),
+        projmatrix=t(cam.full_proj_transform), sh_degree=0, campos=t(cam.center), prefiltered=False,
+        debug=False)
+    r = GaussianRasterizer(rs)
+    xyz, sc, rot, op = t(g["means3D"]), t(g["scales"]), t(g["rotations"]), t(g["opacities"].reshape(-1, 1))
+    pw = render_to_check(r, xyz, sc, rot, op)
+    ref = r(means3D=xyz, means2D=torch.zeros_like(xyz), shs=None, colors_precomp=torch.ones_like(xyz),
+            opacities=op, scales=sc, rotations=rot, cov3D_precomp=None)[4]
+    assert torch.equal(pw, ref) and float(pw.max()) > 0
