@@ -137,6 +137,12 @@ _SIGS = {
                                              [c_void_p] * 5),
     "fgs_rebuild_ancestor_path": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                             c_void_p]),
+    "fgs_tree_traverse_ws_bytes": (c_size_t, [c_int32, c_int32, c_int32]),
+    "fgs_tree_traverse": (c_int32, [c_int32] + [c_void_p] * 5 + [c_int32, c_int32, c_void_p, c_int32,
+                                    c_void_p, c_void_p] + [c_float] * 5 + [c_int32, c_int32,
+                                    c_void_p, c_void_p, c_void_p]),
+    "fgs_tree_traverse_result": (c_int32, [c_int32, c_int32, c_int32, c_void_p, c_int64, c_void_p,
+                                           c_void_p]),
     "fgs_filtergs_geometry": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fgs_filtergs_render": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -3,6 +3,8 @@
 // Host orchestration of rasterizer_impl.cu:254-676 (forward / backward) and
 // of the FilterGS/cuda host functions, without allocation: every buffer is
 // supplied by the caller and carved here into 256-byte aligned sub-arrays.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -901,6 +903,121 @@ int fgs_rebuild_ancestor_path(int32_t N, int32_t L, const int32_t* index_parent,
         return fail(FGS_E_INVALID, "rebuild_ancestor_path: null pointer%s");
     return check(fgs::launch_rebuild_ancestor(N, L, index_parent, depth, ancestor_path, bad, (hipStream_t)stream),
                  "rebuild_ancestor_path");
+}
+
+namespace {
+// Workspace of fgs_tree_traverse: control words, per-block counts and
+// offsets, two frontiers, the id list and the per-candidate flags.
+struct TravWs {
+    int32_t* ctl;
+    uint32_t* counts;
+    uint32_t* offs;
+    int32_t* fa;
+    int32_t* fb;
+    int32_t* out;
+    uint8_t* flags;
+    size_t bytes;
+};
+static TravWs trav_carve(void* base, int32_t N, int32_t C, int32_t n_roots)
+{
+    const size_t cap = (size_t)N + (size_t)n_roots;
+    const size_t nflags = std::max((size_t)n_roots, cap * (size_t)C);
+    Carver c{ (char*)base };
+    TravWs w{};
+    w.ctl = c.take<int32_t>(fgs::kTravCtl);
+    w.counts = c.take<uint32_t>(2 * fgs::kTravGrid);
+    w.offs = c.take<uint32_t>(2 * fgs::kTravGrid);
+    w.fa = c.take<int32_t>(cap);
+    w.fb = c.take<int32_t>(cap);
+    w.out = c.take<int32_t>(cap);
+    w.flags = c.take<uint8_t>(nflags);
+    w.bytes = c.off;
+    return w;
+}
+}  // namespace
+
+size_t fgs_tree_traverse_ws_bytes(int32_t N, int32_t C, int32_t n_roots)
+{
+    if (N < 0 || C < 1 || n_roots < 0) return 0;
+    return trav_carve(nullptr, N, C, n_roots).bytes;
+}
+
+int fgs_tree_traverse(int32_t N, const float* xyz, const float* scaling, const float* rotation,
+                      const int32_t* node_index, const int32_t* tree, int32_t n_nodes, int32_t C,
+                      const int64_t* root_index, int32_t n_roots, const float* projmatrix,
+                      const float* viewmatrix, float focal_x, float focal_y, float tan_fovx,
+                      float tan_fovy, float min_resolution_pixel, int32_t max_depth,
+                      int32_t level_hint, void* ws, int64_t* count, void* stream)
+{
+    if (N < 0 || n_nodes < 0 || C < 1 || n_roots < 0)
+        return fail(FGS_E_INVALID, "tree_traverse: bad size%s");
+    if ((size_t)N + (size_t)n_roots > (size_t)INT32_MAX / (size_t)C)
+        return fail(FGS_E_INVALID, "tree_traverse: tree too large%s");
+    if (!count) return fail(FGS_E_INVALID, "tree_traverse: null count%s");
+    *count = 0;
+    if (n_roots == 0) return FGS_OK;
+    if (!xyz || !scaling || !rotation || !node_index || (n_nodes > 0 && !tree) || !root_index ||
+        !projmatrix || !viewmatrix || !ws)
+        return fail(FGS_E_INVALID, "tree_traverse: null pointer%s");
+    // the reference stops expanding after level min(30, max_depth)
+    const int limit = std::max(0, std::min(30, (int)max_depth));
+    const int levels = level_hint >= 0 ? std::min(limit, (int)level_hint) : limit;
+    const bool append = levels == limit;
+    TravWs w = trav_carve(ws, N, C, n_roots);
+    fgs::TravArgs a{};
+    a.N = N;
+    a.C = C;
+    a.n_nodes = n_nodes;
+    a.xyz = xyz;
+    a.scaling = scaling;
+    a.rotation = rotation;
+    a.node_index = node_index;
+    a.tree = tree;
+    a.proj = projmatrix;
+    a.view = viewmatrix;
+    a.fx = focal_x;
+    a.fy = focal_y;
+    a.tanx = tan_fovx;
+    a.tany = tan_fovy;
+    a.min_px = min_resolution_pixel;
+    a.cap_frontier = N + n_roots;
+    a.cap_out = N + n_roots;
+    a.ctl = w.ctl;
+    a.counts = w.counts;
+    a.offs = w.offs;
+    a.flags = w.flags;
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if ((rc = check(fgs::launch_traverse(a, root_index, n_roots, w.fa, w.fb, w.out, levels, append, st),
+                    "tree_traverse")))
+        return rc;
+    int32_t ctl[fgs::kTravCtl];
+    if ((rc = check(hipMemcpyAsync(ctl, w.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st), "tree_traverse")))
+        return rc;
+    if ((rc = check(hipStreamSynchronize(st), "tree_traverse"))) return rc;
+    if (ctl[fgs::kTravErr] & 1)
+        return fail(FGS_E_INVALID, "tree_traverse: a root, node row or child id is out of range%s");
+    const int64_t last = ctl[levels & 1 ? fgs::kTravFrontierB : fgs::kTravFrontierA];
+    int64_t n = ctl[fgs::kTravOut] + (append ? last : 0);
+    if ((ctl[fgs::kTravErr] & 2) || n > (int64_t)a.cap_out)
+        return fail(FGS_E_CAPACITY,
+                    "tree_traverse: more than N + n_roots ids (duplicate roots or a cyclic tree)%s");
+    if (!append && last != 0)
+        return fail(FGS_E_INVALID, "tree_traverse: level_hint is below the tree's depth%s");
+    *count = n;
+    return FGS_OK;
+}
+
+int fgs_tree_traverse_result(int32_t N, int32_t C, int32_t n_roots, const void* ws, int64_t count,
+                             int64_t* out, void* stream)
+{
+    if (N < 0 || C < 1 || n_roots < 0 || count < 0 || count > (int64_t)N + n_roots)
+        return fail(FGS_E_INVALID, "tree_traverse_result: bad size%s");
+    if (count == 0) return FGS_OK;
+    if (!ws || !out) return fail(FGS_E_INVALID, "tree_traverse_result: null pointer%s");
+    TravWs w = trav_carve(const_cast<void*>(ws), N, C, n_roots);
+    return check(fgs::launch_traverse_emit(w.out, count, out, (hipStream_t)stream),
+                 "tree_traverse_result");
 }
 
 int fgs_visible_flag(int32_t B, int32_t N, const float* xyz, const float* proj_matrix,

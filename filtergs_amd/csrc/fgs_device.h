@@ -135,6 +135,35 @@ __device__ __forceinline__ void get_rect(float px, float py, int r, int gx, int 
 __device__ __forceinline__ float logf_cr(float x) { return (float)log((double)x); }
 __device__ __forceinline__ float expf_cr(float x) { return (float)exp((double)x); }
 
+// 3-sigma screen radius of compute_radius_cuda
+// (FilterGS/cuda/compute_radius_kernel_v2.cu:180-229, the same kernel as
+// LoG/cuda/compute_radius_kernel.cu:107-156): scale and rotation used as given,
+// 0 outside |NDC x|, |NDC y| <= 1.3 (compared in double, as the reference's
+// double literal promotes them), 0.3 px^2 floor on the diagonal of the 2D
+// covariance, 0 for a zero determinant.
+__device__ __forceinline__ float radius_v2_point(const float* proj, const float* view, float fx,
+                                                 float fy, float tanx, float tany, float px,
+                                                 float py, float pz, float sx, float sy, float sz,
+                                                 float qr, float qx, float qy, float qz)
+{
+    float hom[4];
+    xf4(proj, px, py, pz, hom);
+    const float p_w = 1.0f / (hom[3] + 0.0000001f);
+    const float nx = hom[0] * p_w, ny = hom[1] * p_w;
+    if ((double)nx < -1.3 || (double)nx > 1.3 || (double)ny < -1.3 || (double)ny > 1.3) return 0.f;
+    float c3[6];
+    cov3d(sx, sy, sz, qr, qx, qy, qz, c3);
+    Cov2D cv;
+    cov2d(view, px, py, pz, fx, fy, tanx, tany, c3, cv);
+    const float ca = fmaxf(cv.a, 0.3f), cb = cv.b, cc = fmaxf(cv.c, 0.3f);
+    const float det = ca * cc - cb * cb;
+    if (det == 0.0f) return 0.f;
+    const float mid = 0.5f * (ca + cc);
+    const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
+    const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
+    return 3.f * sqrtf(fmaxf(l1, l2));
+}
+
 // Wave64 helpers.
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ float wave_sum(float v)

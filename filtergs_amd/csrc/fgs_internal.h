@@ -266,6 +266,32 @@ struct CounterArgs {
 };
 hipError_t launch_counter_update(const CounterArgs& a, hipStream_t s);
 
+// TensorTree.traverse (traverse.hip)
+constexpr int kTravGrid = 1024;    // blocks of the per-level count / scatter launches
+constexpr int kTravBlock = 256;
+constexpr int kTravCtl = 8;        // control words at the head of the workspace
+constexpr int kTravFrontierA = 0, kTravFrontierB = 1, kTravOut = 2, kTravErr = 3;
+struct TravArgs {
+    int32_t N, C, n_nodes;
+    const float* xyz;              // [N, 3]
+    const float* scaling;          // [N, 3] raw (log) scales
+    const float* rotation;         // [N, 4] raw quaternions
+    const int32_t* node_index;     // [N] row of the node in `tree`, -1 for a leaf
+    const int32_t* tree;           // [n_nodes, C] child ids, -1 for an empty slot
+    const float* proj;
+    const float* view;
+    float fx, fy, tanx, tany, min_px;
+    int32_t cap_frontier, cap_out; // capacities of the frontier and output arrays
+    int32_t* ctl;                  // [kTravCtl] sizes and error bits (bit 0 bad index, bit 1 overflow)
+    uint32_t* counts;              // [2 * kTravGrid] per-block keep / next counts
+    uint32_t* offs;                // [2 * kTravGrid] their exclusive scans
+    uint8_t* flags;                // per-candidate decision of the level
+};
+hipError_t launch_traverse(const TravArgs& a, const int64_t* roots, int32_t n_roots,
+                           int32_t* frontier_a, int32_t* frontier_b, int32_t* out, int levels,
+                           bool append, hipStream_t s);
+hipError_t launch_traverse_emit(const int32_t* ids, int64_t n, int64_t* out, hipStream_t s);
+
 // LoD cut and per-point utilities
 struct LodArgs {
     int N, L;

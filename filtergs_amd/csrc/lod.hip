@@ -170,27 +170,11 @@ __global__ __launch_bounds__(256) void k_radius_v2(int P, const float* means3d, 
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P) return;
-    float out = 0.f;
-    const float px = means3d[3 * i], py = means3d[3 * i + 1], pz = means3d[3 * i + 2];
-    float hom[4];
-    xf4(proj, px, py, pz, hom);
-    const float p_w = 1.0f / (hom[3] + 0.0000001f);
-    const float qx = hom[0] * p_w, qy = hom[1] * p_w;
-    if (!((double)qx < -1.3 || (double)qx > 1.3 || (double)qy < -1.3 || (double)qy > 1.3)) {
-        float c3[6];
-        cov3d(scales[3 * i], scales[3 * i + 1], scales[3 * i + 2], rotations[4 * i],
-              rotations[4 * i + 1], rotations[4 * i + 2], rotations[4 * i + 3], c3);
-        Cov2D cv;
-        cov2d(view, px, py, pz, fx, fy, tanx, tany, c3, cv);
-        const float ca = fmaxf(cv.a, 0.3f), cb = cv.b, cc = fmaxf(cv.c, 0.3f);
-        const float det = ca * cc - cb * cb;
-        if (det != 0.0f) {
-            const float mid = 0.5f * (ca + cc);
-            const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
-            const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
-            out = 3.f * sqrtf(fmaxf(l1, l2));
-        }
-    }
+    const float out = radius_v2_point(proj, view, fx, fy, tanx, tany, means3d[3 * i],
+                                      means3d[3 * i + 1], means3d[3 * i + 2], scales[3 * i],
+                                      scales[3 * i + 1], scales[3 * i + 2], rotations[4 * i],
+                                      rotations[4 * i + 1], rotations[4 * i + 2],
+                                      rotations[4 * i + 3]);
     radii[i] = out;
 }
 

@@ -147,6 +147,38 @@ int fgs_compute_visibility_radius_parent(int32_t N, const float* means3D, const 
 int fgs_rebuild_ancestor_path(int32_t N, int32_t L, const int32_t* index_parent, const int32_t* depth,
                               int32_t* ancestor_path, int32_t* bad, void* stream);
 
+/* TensorTree.traverse (LoG/model/tensor_tree.py:133-188, with the radius of
+ * Gaussian.compute_radius, LoG/model/level_of_gaussian.py:66-88, and
+ * LoG/cuda/compute_radius_kernel.cu:107-156) on the device, without a host
+ * wait per level.
+ *   xyz [N,3], scaling [N,3] (raw, exp-activated here), rotation [N,4] (raw,
+ *   normalised here): the Gaussian parameters, float;
+ *   node_index [N] int32: row of node i in tree, -1 for a leaf;
+ *   tree [n_nodes, C] int32: child ids, -1 for an empty slot;
+ *   root_index [n_roots] int64: the roots to start from;
+ *   min_resolution_pixel: a node whose radius is below it is kept whole;
+ *   max_depth: the reference's argument (levels past min(30, max_depth) are
+ *   not expanded; the frontier there is returned as is);
+ *   level_hint: levels to run at most (>= the tree's depth below the roots),
+ *   or -1 for min(30, max_depth); a hint below the depth returns an error.
+ * ws: fgs_tree_traverse_ws_bytes(N, C, n_roots) device bytes.  Waits for the
+ * stream once and stores the number of ids in *count (host); the ids stay in
+ * ws until fgs_tree_traverse_result copies them out, in the reference's order
+ * (kept roots, then the kept nodes of level 1, 2, ..., then the frontier at
+ * the level limit).  Errors: ids out of range (FGS_E_INVALID), more than
+ * N + n_roots ids (FGS_E_CAPACITY: duplicate roots or a cyclic tree). */
+size_t fgs_tree_traverse_ws_bytes(int32_t N, int32_t C, int32_t n_roots);
+int fgs_tree_traverse(int32_t N, const float* xyz, const float* scaling, const float* rotation,
+                      const int32_t* node_index, const int32_t* tree, int32_t n_nodes, int32_t C,
+                      const int64_t* root_index, int32_t n_roots, const float* projmatrix,
+                      const float* viewmatrix, float focal_x, float focal_y, float tan_fovx,
+                      float tan_fovy, float min_resolution_pixel, int32_t max_depth,
+                      int32_t level_hint, void* ws, int64_t* count, void* stream);
+/* The count ids of the last fgs_tree_traverse on ws as int64 into out [count]
+ * (the reference returns an int64 index tensor). */
+int fgs_tree_traverse_result(int32_t N, int32_t C, int32_t n_roots, const void* ws, int64_t count,
+                             int64_t* out, void* stream);
+
 /* One step of LoG's SparseOptimizer for one parameter
  * (LoG/model/sparse_optimizer.py:41-78, 163-195): for each selected row j < n
  * with flag[j] (flag may be null: all rows), parameter row i = index[j] gets

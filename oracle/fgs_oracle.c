@@ -245,31 +245,60 @@ void orc_lod_cut(int32_t N, const float* means3d, const float* log_scales, const
     free(vis);
 }
 
-/* compute_radius_cuda of FilterGS (:180-229): raw scales and quaternion. */
+/* compute_radius_cuda of FilterGS (compute_radius_kernel_v2.cu:180-229; the
+ * same kernel as LoG/cuda/compute_radius_kernel.cu:107-156) for one point:
+ * scale and quaternion used as given. */
+static float orc_radius_v2_point(const float* projmatrix, const float* viewmatrix, float focal_x,
+                                 float focal_y, float tanx, float tany, float px, float py,
+                                 float pz, float sx, float sy, float sz, float qr, float qx,
+                                 float qy, float qz)
+{
+    float hom[4];
+    xf4_f32(projmatrix, px, py, pz, hom);
+    const float p_w = 1.0f / (hom[3] + 0.0000001f);
+    const float nx = hom[0] * p_w, ny = hom[1] * p_w;
+    if ((double)nx < -1.3 || (double)nx > 1.3 || (double)ny < -1.3 || (double)ny > 1.3) return 0.f;
+    float c3[6];
+    cov3d_f32(sx, sy, sz, qr, qx, qy, qz, c3);
+    cov2d_t_f32 cv;
+    cov2d_f32(viewmatrix, px, py, pz, focal_x, focal_y, tanx, tany, c3, &cv);
+    const float a = fmaxf(cv.a, 0.3f), b = cv.b, c = fmaxf(cv.c, 0.3f);
+    const float det = a * c - b * b;
+    if (det == 0.0f) return 0.f;
+    const float mid = 0.5f * (a + c);
+    const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
+    const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
+    return 3.f * sqrtf(fmaxf(l1, l2));
+}
+
 void orc_radius_v2(int32_t P, const float* means3d, const float* scales, const float* rotations,
                    const float* projmatrix, const float* viewmatrix, float focal_x, float focal_y,
                    float tanx, float tany, float* radii)
 {
-    for (int i = 0; i < P; ++i) {
-        radii[i] = 0;
-        const float px = means3d[3 * i], py = means3d[3 * i + 1], pz = means3d[3 * i + 2];
-        float hom[4];
-        xf4_f32(projmatrix, px, py, pz, hom);
-        const float p_w = 1.0f / (hom[3] + 0.0000001f);
-        const float qx = hom[0] * p_w, qy = hom[1] * p_w;
-        if ((double)qx < -1.3 || (double)qx > 1.3 || (double)qy < -1.3 || (double)qy > 1.3) continue;
-        float c3[6];
-        cov3d_f32(scales[3 * i], scales[3 * i + 1], scales[3 * i + 2], rotations[4 * i],
-                  rotations[4 * i + 1], rotations[4 * i + 2], rotations[4 * i + 3], c3);
-        cov2d_t_f32 cv;
-        cov2d_f32(viewmatrix, px, py, pz, focal_x, focal_y, tanx, tany, c3, &cv);
-        const float a = fmaxf(cv.a, 0.3f), b = cv.b, c = fmaxf(cv.c, 0.3f);
-        const float det = a * c - b * b;
-        if (det == 0.0f) continue;
-        const float mid = 0.5f * (a + c);
-        const float l1 = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
-        const float l2 = mid - sqrtf(fmaxf(0.1f, mid * mid - det));
-        radii[i] = 3.f * sqrtf(fmaxf(l1, l2));
+    for (int i = 0; i < P; ++i)
+        radii[i] = orc_radius_v2_point(projmatrix, viewmatrix, focal_x, focal_y, tanx, tany,
+                                       means3d[3 * i], means3d[3 * i + 1], means3d[3 * i + 2],
+                                       scales[3 * i], scales[3 * i + 1], scales[3 * i + 2],
+                                       rotations[4 * i], rotations[4 * i + 1], rotations[4 * i + 2],
+                                       rotations[4 * i + 3]);
+}
+
+/* LoG's Gaussian.compute_radius (level_of_gaussian.py:66-88) of the rows ids:
+ * scaling = exp(raw) (correctly rounded), rotation = raw / max(|raw|, 1e-12)
+ * (F.normalize, squares summed in index order), then compute_radius_cuda. */
+void orc_log_radius(int64_t n, const int64_t* ids, const float* xyz, const float* scaling,
+                    const float* rotation, const float* projmatrix, const float* viewmatrix,
+                    float focal_x, float focal_y, float tanx, float tany, float* radii)
+{
+    for (int64_t k = 0; k < n; ++k) {
+        const int64_t i = ids[k];
+        const float* q = rotation + 4 * i;
+        const float qn = fmaxf(sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]), 1e-12f);
+        radii[k] = orc_radius_v2_point(projmatrix, viewmatrix, focal_x, focal_y, tanx, tany,
+                                       xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2],
+                                       orc_expf_cr(scaling[3 * i]), orc_expf_cr(scaling[3 * i + 1]),
+                                       orc_expf_cr(scaling[3 * i + 2]), q[0] / qn, q[1] / qn,
+                                       q[2] / qn, q[3] / qn);
     }
 }
 

@@ -268,6 +268,60 @@ def radius_v2(means3d, scales, rotations, projmatrix, viewmatrix, fx, fy, tanx, 
     return out
 
 
+def log_radius(ids, xyz, scaling, rotation, projmatrix, viewmatrix, fx, fy, tanx, tany):
+    """LoG's Gaussian.compute_radius (LoG/model/level_of_gaussian.py:66-88) of
+    rows `ids`: raw scaling through a correctly rounded exp, raw rotation
+    normalised (eps 1e-12), then compute_radius_cuda
+    (LoG/cuda/compute_radius_kernel.cu:107-156) in FP32."""
+    ids = _c(np.asarray(ids).reshape(-1), np.int64)
+    out = np.zeros(ids.size, np.float32)
+    if ids.size:
+        lib().orc_log_radius(ctypes.c_int64(ids.size), _p(ids), _p(_c(xyz, np.float32)),
+                             _p(_c(scaling, np.float32)), _p(_c(rotation, np.float32)),
+                             _p(_c(np.asarray(projmatrix).reshape(-1), np.float32)),
+                             _p(_c(np.asarray(viewmatrix).reshape(-1), np.float32)),
+                             ctypes.c_float(fx), ctypes.c_float(fy), ctypes.c_float(tanx),
+                             ctypes.c_float(tany), _p(out))
+    return out
+
+
+def tree_traverse(node_index, tree, root_index, radius_fn, min_resolution_pixel=3.0,
+                  max_depth=1000):
+    """TensorTree.traverse (LoG/model/tensor_tree.py:133-188) restated with
+    numpy: `radius_fn(ids)` gives the screen radius of rows `ids`.
+
+    Level 0 keeps the roots whose radius is below min_resolution_pixel or that
+    have no children; the others form the frontier.  Each further level (1 ..
+    min(30, max_depth)) expands the frontier into its children in (frontier
+    position, child slot) order, empty slots dropped, keeps the small ones
+    and the leaves and continues with the rest; at the limit the frontier
+    itself is appended.  Returns int64 ids: kept roots, then each level's kept
+    children, then the frontier at the limit."""
+    node_index = np.asarray(node_index, np.int64)
+    tree = np.asarray(tree, np.int64)
+    assert tree.ndim == 2, "tree is [n_nodes, max_child]"
+    roots = np.asarray(root_index, np.int64).reshape(-1)
+    r = radius_fn(roots)
+    keep = (r < min_resolution_pixel) | (node_index[roots] == -1)
+    parts = [roots[keep]]
+    index = roots[~keep]
+    level = 1
+    while True:
+        if level > 30 or level > max_depth:
+            parts.append(index)
+            break
+        child = tree[node_index[index]].reshape(-1)
+        child = child[child != -1]
+        r = radius_fn(child)
+        keep = (r < min_resolution_pixel) | (node_index[child] == -1)
+        parts.append(child[keep])
+        if (~keep).sum() == 0:
+            break
+        index = child[~keep]
+        level += 1
+    return np.concatenate(parts).astype(np.int64)
+
+
 def radius_aa(means3d, scales, rotations, viewmatrix, projmatrix, tanx, tany, H, W):
     means3d = _c(means3d, np.float32)
     P = means3d.reshape(-1, 3).shape[0]

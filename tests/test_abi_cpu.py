@@ -49,3 +49,12 @@ def test_null_arguments_fail_cleanly():
     rc = lib.fgs_mark_visible(-1, None, None, None, None, None)
     assert rc == 1
     del ctypes
+
+
+def test_traverse_workspace_query():
+    lib = _lib.load()
+    N, C, R = 100000, 4, 300
+    assert lib.fgs_tree_traverse_ws_bytes(N, C, R) >= 3 * 4 * (N + R) + C * (N + R)
+    assert lib.fgs_tree_traverse_ws_bytes(N, 8, R) > lib.fgs_tree_traverse_ws_bytes(N, 4, R)
+    assert lib.fgs_tree_traverse_ws_bytes(-1, C, R) == 0
+    assert lib.fgs_tree_traverse_ws_bytes(N, 0, R) == 0
