@@ -134,6 +134,79 @@ def filtergs_views(tree, cams, warmup):
     return out
 
 
+def traverse_views(tree, cams, min_px=3.0):
+    """LoG's TensorTree.traverse (LoG/model/tensor_tree.py:133-188) of the
+    LoD tree in TensorTree layout from each aerial view, on the device, and
+    the reference's level loop in torch for comparison (per level: children
+    gathered, radius of exp(scaling) / normalize(rotation) by the
+    compute_radius kernel, keep/next masks, a host wait on the next-level
+    count).  Wall times: both wait for their result."""
+    import time
+
+    import torch
+
+    from filtergs_amd import lod, workloads
+    from filtergs_amd.tensor_tree import tree_traverse
+    dev = "cuda"
+    node_index, rows = workloads.children_table(tree["index_parent"], 4)
+    roots = torch.nonzero(tree["index_parent"] < 0).reshape(-1)
+    xyz, lsc, rot = tree["means3D"], tree["log_scales"], tree["rotations"]
+    hint = int(tree["level"].max().item())
+    ni64 = node_index.long()
+
+    def torch_loop(cam, fx, fy, pm, vm):
+        def radius(ids):
+            return lod.compute_radius_module.compute_radius(
+                xyz[ids], torch.exp(lsc[ids]), torch.nn.functional.normalize(rot[ids]), pm, vm, fx, fy,
+                cam.tanfovx, cam.tanfovy)
+        r = radius(roots)
+        keep = (r < min_px) | (ni64[roots] == -1)
+        parts = [roots[keep]]
+        index = roots[~keep]
+        level = 1
+        while True:
+            if level > 30:
+                parts.append(index)
+                break
+            child = rows[ni64[index]].reshape(-1).long()
+            child = child[child != -1]
+            r = radius(child)
+            keep = (r < min_px) | (ni64[child] == -1)
+            parts.append(child[keep])
+            go = ~keep
+            if go.sum() == 0:
+                break
+            index = child[go]
+            level += 1
+        return torch.cat(parts)
+
+    out = {"device_ms": [], "torch_loop_ms": [], "ids": [], "equal": [], "differing_ids": []}
+    for cam in cams:
+        fx = cam.width / (2 * cam.tanfovx)
+        fy = cam.height / (2 * cam.tanfovy)
+        pm = torch.as_tensor(cam.full_proj_transform, dtype=torch.float32, device=dev)
+        vm = torch.as_tensor(cam.world_view_transform, dtype=torch.float32, device=dev)
+        dev_fn = lambda: tree_traverse(xyz, lsc, rot, node_index, rows, roots, vm, pm, fx, fy, cam.tanfovx,
+                                       cam.tanfovy, min_px, 1000, level_hint=hint)
+        ref_fn = lambda: torch_loop(cam, fx, fy, pm, vm)
+        res = {}
+        for name, fn in (("device_ms", dev_fn), ("torch_loop_ms", ref_fn)):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res[name] = fn()
+            torch.cuda.synchronize()
+            out[name].append(round((time.perf_counter() - t0) * 1e3, 3))
+        a, b = res["device_ms"], res["torch_loop_ms"]
+        out["ids"].append(int(a.numel()))
+        out["equal"].append(bool(a.numel() == b.numel() and torch.equal(a, b)))
+        sa, sb = set(a.tolist()), set(b.tolist())
+        out["differing_ids"].append(len(sa ^ sb))
+    n = len(cams)
+    out["mean"] = {k: round(sum(out[k]) / n, 3) for k in ("device_ms", "torch_loop_ms")}
+    return out
+
+
 def lod_leg(steps, warmup):
     """Urban-scale LoD: a 22.4M-node quadtree (12 levels, 2x2 roots over 2 km x
     2 km, leaves on a seeded height field, SH degree 1), the LoD cut of each of
@@ -210,6 +283,7 @@ def lod_leg(steps, warmup):
         raise RuntimeError("rebuilt ancestor rows differ")
     del rows
     fg_res = filtergs_views(tree, cams, warmup)
+    trav_res = traverse_views(tree, cams)
     union = torch.stack([cut(c, 1.5)[0] for c in cams]).any(0)
     idx = torch.nonzero(union).reshape(-1)
     torch.cuda.synchronize()
@@ -246,7 +320,8 @@ def lod_leg(steps, warmup):
         raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
     return {"nodes": N, "leaves": n_leaf, "levels": L,
             "cut_radius_max_1.5": res[1.5], "cut_radius_max_5.0": res[5.0],
-            "filtergs_render": fg_res, "rebuild_ancestor_ms": round(ms_rebuild, 3),
+            "filtergs_render": fg_res, "tensor_tree_traverse": trav_res,
+            "rebuild_ancestor_ms": round(ms_rebuild, 3),
             "gather_ms": round(ms_gather, 3), "gaussians_rendered": P, "kv_pairs": K,
             "batched_fwd_bwd_ms": round(ms, 3), "Mpix/s": round(npix / ms / 1000.0, 3),
             "views": len(cams), "pixels": npix}

@@ -20,7 +20,9 @@ def _f32c(t, dev):
 
 def _compute_radius(means3D, scales, rotations, projmatrix, viewmatrix, focal_x, focal_y,
                     tan_fovx, tan_fovy):
-    """compute_radius (compute_radius_kernel_v2.cu:355-380): raw scales and quaternions."""
+    """compute_radius (compute_radius_kernel_v2.cu:355-380): scales and quaternions used as
+    given (LoG passes exp / normalize, level_of_gaussian.py:66-84; the FilterGS fallback,
+    filtergs.py:261-268, passes its raw buffers)."""
     P = int(means3D.size(0))
     radii = torch.zeros((P,), dtype=means3D.dtype, device=means3D.device)
     if P == 0:

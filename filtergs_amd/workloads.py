@@ -154,6 +154,35 @@ def lod_tree(levels: int = 12, roots_side: int = 2, extent: float = 2000.0, sh_d
     return {k: v.contiguous().to(dev) for k, v in out.items()}
 
 
+def children_table(index_parent, max_child):
+    """TensorTree layout (LoG/model/tensor_tree.py:4-26) of a tree given by
+    parent links: node_index [N] int32 (the node's row in `tree`, -1 for a
+    node without children) and tree [rows, max_child] int32 (child ids in
+    ascending id order, -1 in unused slots).  Rows follow the parents' ids."""
+    import torch
+    parent = index_parent.to(torch.int64)
+    N = parent.numel()
+    dev = parent.device
+    has_child = torch.zeros(N, dtype=torch.bool, device=dev)
+    nz = parent >= 0
+    has_child[parent[nz]] = True
+    rank = torch.cumsum(has_child.to(torch.int64), 0) - 1
+    node_index = torch.where(has_child, rank, torch.full_like(rank, -1)).to(torch.int32)
+    child = torch.nonzero(nz).reshape(-1)
+    order = torch.argsort(parent[child] * N + child)
+    child = child[order]
+    par = parent[child]
+    first = torch.ones_like(par, dtype=torch.bool)
+    first[1:] = par[1:] != par[:-1]
+    start = torch.cummax(torch.where(first, torch.arange(par.numel(), device=dev), torch.zeros_like(par)), 0).values
+    slot = torch.arange(par.numel(), device=dev) - start
+    if slot.numel() and int(slot.max()) >= max_child:
+        raise ValueError("a node has more than max_child children")
+    tree = torch.full((int(has_child.sum()), max_child), -1, dtype=torch.int32, device=dev)
+    tree[rank[par], slot] = child.to(torch.int32)
+    return node_index, tree
+
+
 def aerial_cameras(n_views: int = 8, extent: float = 2000.0, seed: int = 5,
                    altitude_range=(150.0, 400.0), width: int = 1920, height: int = 1080):
     """Oblique aerial views over the LoD terrain: positions uniform over the
