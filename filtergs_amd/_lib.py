@@ -137,6 +137,7 @@ _SIGS = {
                                              [c_void_p] * 5),
     "fgs_rebuild_ancestor_path": (c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
                                             c_void_p]),
+    "fgs_activate_backward": (c_int32, [c_int64] + [c_void_p] * 4 + [c_int32, c_int32] + [c_void_p] * 11),
     "fgs_tree_traverse_ws_bytes": (c_size_t, [c_int32, c_int32, c_int32]),
     "fgs_tree_traverse": (c_int32, [c_int32] + [c_void_p] * 5 + [c_int32, c_int32, c_void_p, c_int32,
                                     c_void_p, c_void_p] + [c_float] * 5 + [c_int32, c_int32,

@@ -1061,9 +1061,10 @@ int fgs_activate_gather(int64_t N, const int64_t* index, int64_t n_src, const fl
     const int need = (active_sh_degree + 1) * (active_sh_degree + 1) - 1;
     if (active_sh_degree > 0 && (rest_coeffs < need || !features_rest || !campos))
         return fail(FGS_E_INVALID, "activate: SH rest coefficients / camera centre missing for the degree%s");
-    if (N > 0 && (!index || !xyz || !log_scaling || !rotation || !opacity_logit || !features_dc ||
-                  !xyz_out || !scaling_out || !rotation_out || !opacity_out || !colors_out || !bad_index))
+    if (N > 0 && (!xyz || !log_scaling || !rotation || !opacity_logit || !features_dc ||
+                  !scaling_out || !rotation_out || !opacity_out || !colors_out || !bad_index))
         return fail(FGS_E_INVALID, "activate: null pointer%s");
+    if (!index && N > n_src) return fail(FGS_E_INVALID, "activate: more rows than the source holds%s");
     fgs::ActivateArgs a{};
     a.N = N;
     a.n_src = n_src;
@@ -1084,6 +1085,44 @@ int fgs_activate_gather(int64_t N, const int64_t* index, int64_t n_src, const fl
     a.colors_o = colors_out;
     a.bad_index = bad_index;
     return check(fgs::launch_activate_gather(a, (hipStream_t)stream), "activate_gather");
+}
+
+int fgs_activate_backward(int64_t N, const float* xyz, const float* log_scaling, const float* rotation,
+                          const float* opacity_logit, int32_t rest_coeffs, int32_t active_sh_degree,
+                          const float* campos, const float* dL_dscaling, const float* dL_drotation,
+                          const float* dL_dopacity, const float* dL_dcolors, float* dL_dlog_scaling,
+                          float* dL_drotation_raw, float* dL_dopacity_logit, float* dL_dfeatures_dc,
+                          float* dL_dfeatures_rest, void* stream)
+{
+    if (N < 0 || rest_coeffs < 0) return fail(FGS_E_INVALID, "activate_backward: negative size%s");
+    if (active_sh_degree < 0 || active_sh_degree > 4)
+        return fail(FGS_E_INVALID, "activate_backward: SH degree must be in [0, 4]%s");
+    if (N == 0) return FGS_OK;
+    const int need = (active_sh_degree + 1) * (active_sh_degree + 1) - 1;
+    if (dL_dfeatures_rest && active_sh_degree > 0 && (rest_coeffs < need || !xyz || !campos))
+        return fail(FGS_E_INVALID, "activate_backward: SH rest coefficients / camera centre missing for the degree%s");
+    if ((dL_dlog_scaling && !log_scaling) || (dL_drotation_raw && !rotation) ||
+        (dL_dopacity_logit && !opacity_logit))
+        return fail(FGS_E_INVALID, "activate_backward: null input for a requested gradient%s");
+    fgs::ActivateBwdArgs a{};
+    a.N = N;
+    a.xyz = xyz;
+    a.log_scale = log_scaling;
+    a.rot = rotation;
+    a.opacity_logit = opacity_logit;
+    a.rest_floats = rest_coeffs * 3;
+    a.deg = active_sh_degree;
+    a.campos = campos;
+    a.g_scale = dL_dscaling;
+    a.g_rot = dL_drotation;
+    a.g_opacity = dL_dopacity;
+    a.g_colors = dL_dcolors;
+    a.d_log_scale = dL_dlog_scaling;
+    a.d_rot = dL_drotation_raw;
+    a.d_opacity_logit = dL_dopacity_logit;
+    a.d_dc = dL_dfeatures_dc;
+    a.d_rest = dL_dfeatures_rest;
+    return check(fgs::launch_activate_bwd(a, (hipStream_t)stream), "activate_backward");
 }
 
 /* ---- batched multi-view path ------------------------------------------- */

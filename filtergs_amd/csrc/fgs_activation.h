@@ -60,6 +60,45 @@ __device__ __forceinline__ float sh_wobase(int deg, const float* __restrict__ r,
     return res;
 }
 
+// The basis functions of eval_sh_wobase themselves: b[k - 1] multiplies
+// coefficient k (k = 1 .. (deg+1)^2 - 1), each formed as the factor the
+// reference multiplies the coefficient by.
+__device__ __forceinline__ void sh_basis(int deg, float x, float y, float z, float* __restrict__ b)
+{
+    b[0] = -kA_C1 * y;
+    b[1] = kA_C1 * z;
+    b[2] = -kA_C1 * x;
+    if (deg > 1) {
+        const float xx = x * x, yy = y * y, zz = z * z;
+        const float xy = x * y, yz = y * z, xz = x * z;
+        b[3] = kA_C2[0] * xy;
+        b[4] = kA_C2[1] * yz;
+        b[5] = kA_C2[2] * (2.0f * zz - xx - yy);
+        b[6] = kA_C2[3] * xz;
+        b[7] = kA_C2[4] * (xx - yy);
+        if (deg > 2) {
+            b[8] = kA_C3[0] * y * (3.f * xx - yy);
+            b[9] = kA_C3[1] * xy * z;
+            b[10] = kA_C3[2] * y * (4.f * zz - xx - yy);
+            b[11] = kA_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
+            b[12] = kA_C3[4] * x * (4.f * zz - xx - yy);
+            b[13] = kA_C3[5] * z * (xx - yy);
+            b[14] = kA_C3[6] * x * (xx - 3.f * yy);
+            if (deg > 3) {
+                b[15] = kA_C4[0] * xy * (xx - yy);
+                b[16] = kA_C4[1] * yz * (3.f * xx - yy);
+                b[17] = kA_C4[2] * xy * (7.f * zz - 1.f);
+                b[18] = kA_C4[3] * yz * (7.f * zz - 3.f);
+                b[19] = kA_C4[4] * (zz * (35.f * zz - 30.f) + 3.f);
+                b[20] = kA_C4[5] * xz * (7.f * zz - 3.f);
+                b[21] = kA_C4[6] * (xx - yy) * (7.f * zz - 1.f);
+                b[22] = kA_C4[7] * xz * (xx - 3.f * yy);
+                b[23] = kA_C4[8] * (xx * (xx - 3.f * yy) - yy * (3.f * xx - yy));
+            }
+        }
+    }
+}
+
 // One activated FilterGS row: mean, scale, unit quaternion, opacity and
 // colour of row src (rest_floats = 3 * rest coefficients per row).
 struct ActRow {

@@ -339,6 +339,26 @@ struct ActivateArgs {
     int* bad_index;
 };
 hipError_t launch_activate_gather(const ActivateArgs& a, hipStream_t s);
+struct ActivateBwdArgs {
+    int64_t N;
+    const float* xyz;            // [N,3] (SH direction, deg > 0)
+    const float* log_scale;      // [N,3]
+    const float* rot;            // [N,4] raw
+    const float* opacity_logit;  // [N]
+    int rest_floats;             // floats per row of d_rest
+    int deg;
+    const float* campos;
+    const float* g_scale;        // [N,3] gradients of the activated rows (null: zero)
+    const float* g_rot;          // [N,4]
+    const float* g_opacity;      // [N]
+    const float* g_colors;       // [N,3]
+    float* d_log_scale;          // outputs (null: not wanted)
+    float* d_rot;
+    float* d_opacity_logit;
+    float* d_dc;
+    float* d_rest;               // [N, rest_floats]
+};
+hipError_t launch_activate_bwd(const ActivateBwdArgs& a, hipStream_t s);
 hipError_t launch_radius_v2(int P, const float* means3d, const float* scales,
                             const float* rotations, const float* proj, const float* view,
                             float fx, float fy, float tanx, float tany, float* radii,

@@ -12,6 +12,12 @@
 * ``get_expon_lr_func`` -- the learning-rate schedule (sparse_optimizer.py:6-39).
 * ``render_to_check`` -- the point-weight probe of LoG.prepare
   (level_of_gaussian.py:205-219) as a forward without statistics or autograd.
+* ``activate_root_return`` / ``get_all`` -- LoG's training gather
+  (level_of_gaussian.py:260-294) with Activation.activate_root_return
+  (LoG/model/activation.py:27-43) as one forward and one backward kernel
+  (``fgs_activate_gather`` on the rows in place, ``fgs_activate_backward``)
+  instead of the exp / sigmoid / normalize / SH chains of element-wise torch
+  kernels and their autograd graph.
 """
 from __future__ import annotations
 
@@ -250,3 +256,100 @@ def render_to_check(rasterizer, xyz, scaling, rotation, opacity):
         ogfs_energy_floor=rs.ogfs_energy_floor, ogfs_radius_ratio_min=rs.ogfs_radius_ratio_min,
         ogfs_enable_stats=False)
     return out[4]
+
+
+class _ActivateRows(torch.autograd.Function):
+    """Activation.activate_root_return (LoG/model/activation.py:36-43) of
+    rows in place: scaling = exp, rotation = F.normalize, opacity = sigmoid,
+    colors = SH2RGB(dc) + eval_sh_wobase(dir, rest) for active degree > 0
+    with dir from the detached means (activation.py:27-35)."""
+
+    @staticmethod
+    def forward(ctx, scaling, rotation, opacity, colors, shs, xyz, campos, deg):
+        N = int(scaling.shape[0])
+        dev = scaling.device
+        rest_coeffs = int(shs.shape[1]) if deg > 0 else 0
+        f32 = dict(dtype=torch.float32, device=dev)
+        out_s = torch.empty((N, 3), **f32)
+        out_r = torch.empty((N, 4), **f32)
+        out_o = torch.empty(opacity.shape, **f32)
+        out_c = torch.empty((N, 3), **f32)
+        if N:
+            bad = torch.zeros((1,), dtype=torch.int32, device=dev)
+            src = [xyz, scaling, rotation, opacity, colors]
+            call("fgs_activate_gather", N, None, N, *[ptr(t) for t in src],
+                 ptr(shs) if deg > 0 else None, rest_coeffs, int(deg), ptr(campos) if deg > 0 else None,
+                 None, out_s.data_ptr(), out_r.data_ptr(), out_o.data_ptr(), out_c.data_ptr(),
+                 bad.data_ptr(), _lib.stream_handle(dev))
+        ctx.deg = int(deg)
+        ctx.rest_shape = tuple(shs.shape) if shs is not None else None
+        ctx.save_for_backward(scaling, rotation, opacity, xyz, campos)
+        return out_s, out_r, out_o, out_c
+
+    @staticmethod
+    def backward(ctx, g_s, g_r, g_o, g_c):
+        scaling, rotation, opacity, xyz, campos = ctx.saved_tensors
+        N = int(scaling.shape[0])
+        deg = ctx.deg
+        dev = scaling.device
+        f32 = dict(dtype=torch.float32, device=dev)
+        d_s = torch.empty((N, 3), **f32)
+        d_r = torch.empty((N, 4), **f32)
+        d_o = torch.empty(opacity.shape, **f32)
+        d_dc = torch.empty((N, 3), **f32)
+        want_rest = deg > 0 and ctx.rest_shape is not None
+        d_rest = torch.empty(ctx.rest_shape, **f32) if want_rest else None
+        c = lambda t: t.contiguous() if t is not None else None
+        g_s, g_r, g_o, g_c = c(g_s), c(g_r), c(g_o), c(g_c)
+        if N:
+            call("fgs_activate_backward", N, ptr(xyz) if want_rest else None, ptr(scaling), ptr(rotation),
+                 ptr(opacity), int(ctx.rest_shape[1]) if want_rest else 0, deg if want_rest else 0,
+                 ptr(campos) if want_rest else None, ptr(g_s), ptr(g_r), ptr(g_o), ptr(g_c),
+                 d_s.data_ptr(), d_r.data_ptr(), d_o.data_ptr(), d_dc.data_ptr(),
+                 d_rest.data_ptr() if want_rest and d_rest.numel() else None, _lib.stream_handle(dev))
+        return d_s, d_r, d_o, d_dc, d_rest, None, None, None
+
+
+def activate_root_return(ret, camera, active_sh_degree):
+    """Activation.activate_root_return(ret, camera, active_sh_degree)
+    (LoG/model/activation.py:36-43) on the device: same dict in (xyz,
+    scaling, opacity, rotation, colors, and shs when the degree uses them),
+    same dict out, gradients to the raw rows through one backward kernel."""
+    xyz = ret['xyz']
+    _lib.require_gpu(xyz)
+    deg = int(active_sh_degree) if camera is not None else 0
+    shs = ret.get('shs') if deg > 0 else None
+    if deg > 0 and shs is None:
+        raise ValueError("activate_root_return: SH degree > 0 needs ret['shs']")
+    dev = xyz.device
+    c = lambda t: t.contiguous() if t is not None else None
+    campos = (camera['camera_center'].to(device=dev, dtype=torch.float32).reshape(3).contiguous()
+              if deg > 0 else None)
+    scaling, rotation, opacity, colors = _ActivateRows.apply(
+        c(ret['scaling']), c(ret['rotation']), c(ret['opacity']), c(ret['colors']), c(shs),
+        xyz.detach().contiguous(), campos, deg)
+    return {'xyz': xyz, 'scaling': scaling, 'opacity': opacity, 'rotation': rotation, 'colors': colors}
+
+
+def get_all(log_model, camera, rasterizer=None):
+    """LoG.get_all(camera, rasterizer) (LoG/model/level_of_gaussian.py:260-294)
+    with the activation fused: the visible rows (and with fix_parent the
+    visible parents, untrained) are gathered as the reference does, leaves
+    become nn.Parameter in training and land in visibility_flag['params'],
+    then `activate_root_return` above."""
+    g = log_model.gaussian
+    vf = g.visibility_flag
+    visible_index = vf['index']
+    pick = lambda val, idx: nn.Parameter(val[idx]) if log_model.training else val[idx]
+    if log_model.fix_parent:
+        ret = {key: pick(val, visible_index) for key, val in g.items()}
+        vf['params'] = ret
+        if 'index_node' in vf:
+            index_node = vf['index_node']
+            ret = {key: torch.cat([ret[key], val[index_node]]) for key, val in g.items()}
+    else:
+        if 'index_node' in vf:
+            visible_index = torch.cat([visible_index, vf['index_node']])
+        ret = {key: pick(val, visible_index) for key, val in g.items()}
+        vf['params'] = ret
+    return activate_root_return(ret, camera, g.active_sh_degree)

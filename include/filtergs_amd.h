@@ -415,6 +415,26 @@ int fgs_activate_gather(int64_t N, const int64_t* index, int64_t n_src, const fl
                         int32_t active_sh_degree, const float* campos, float* xyz_out,
                         float* scaling_out, float* rotation_out, float* opacity_out, float* colors_out,
                         int32_t* bad_index, void* stream);
+/* index may be null: row j is source row j (N <= n_src), the in-place rows of
+ * LoG.get_all's gathered leaves (LoG/model/level_of_gaussian.py:260-294);
+ * xyz_out may be null (no copy of the means). */
+
+/* Backward of Activation.activate_root_return (LoG/model/activation.py:27-43)
+ * for rows given in place, as LoG.get_all's autograd graph computes it
+ * (level_of_gaussian.py:260-294): from the gradients of the activated
+ * scaling [N,3], rotation [N,4], opacity [N] and colors [N,3] (any may be
+ * null: zero), the gradients of the raw log-scales, quaternions, opacity
+ * logits, SH DC [N,3] and SH rest [N, rest_coeffs, 3] (any output may be
+ * null: not written).  Rotation: (g - y (y.g)) / |q| with y = q / |q|, or
+ * g / 1e-12 below F.normalize's eps; rest coefficients past the active degree
+ * get 0; the SH direction is the detached mean minus campos, so no mean
+ * gradient arises. */
+int fgs_activate_backward(int64_t N, const float* xyz, const float* log_scaling, const float* rotation,
+                          const float* opacity_logit, int32_t rest_coeffs, int32_t active_sh_degree,
+                          const float* campos, const float* dL_dscaling, const float* dL_drotation,
+                          const float* dL_dopacity, const float* dL_dcolors, float* dL_dlog_scaling,
+                          float* dL_drotation_raw, float* dL_dopacity_logit, float* dL_dfeatures_dc,
+                          float* dL_dfeatures_rest, void* stream);
 
 /* ---- misc ---------------------------------------------------------------- */
 /* Per-stage timing with HIP events recorded on the launch stream (off by

@@ -455,6 +455,37 @@ def activate_gather(index, xyz, log_scaling, rotation, opacity_logit, features_d
                 rotation=q / qn, colors=colors)
 
 
+def activate_backward(xyz, log_scaling, rotation, opacity_logit, rest_coeffs, active_sh_degree,
+                      campos, g_scaling, g_rotation, g_opacity, g_colors):
+    """Backward of Activation.activate_root_return (LoG/model/activation.py:
+    27-43) for rows in place, in float64: gradients of the raw log-scales,
+    quaternions, opacity logits, SH DC and SH rest [n, rest_coeffs, 3] from
+    those of the activated scaling, rotation, opacity and colors.  The SH
+    direction uses the detached mean (activation.py:30-33), so the mean gets
+    no gradient; F.normalize's clamp at 1e-12 stops the norm term below it."""
+    f = lambda a: np.asarray(a, np.float64)
+    ls, q, z = f(log_scaling), f(rotation), f(opacity_logit).reshape(-1)
+    n = len(z)
+    d_ls = f(g_scaling) * np.exp(ls)
+    g = f(g_rotation)
+    nrm = np.linalg.norm(q, axis=1, keepdims=True)
+    safe = np.where(nrm >= 1e-12, nrm, 1.0)
+    y = q / safe
+    d_rot = np.where(nrm >= 1e-12, (g - y * (y * g).sum(1, keepdims=True)) / safe, g / 1e-12)
+    o = 1.0 / (1.0 + np.exp(-z))
+    d_z = f(g_opacity).reshape(-1) * (1.0 - o) * o
+    gc = f(g_colors)
+    d_dc = gc * _SH_C0
+    d_rest = np.zeros((n, rest_coeffs, 3))
+    if active_sh_degree > 0:
+        d = f(xyz) - f(campos)[None]
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        B = sh_wobase_basis(d, active_sh_degree)
+        d_rest[:, :B.shape[1], :] = B[:, :, None] * gc[:, None, :]
+    return dict(log_scaling=d_ls, rotation=d_rot, opacity_logit=d_z, features_dc=d_dc,
+                features_rest=d_rest)
+
+
 # ---------------------------------------------------------------------------
 # Tree formats (apps/ancestor.py), numpy.
 # ---------------------------------------------------------------------------

@@ -154,3 +154,126 @@ def test_render_to_check_equals_rasterizer_point_weight():
     ref = r(means3D=xyz, means2D=torch.zeros_like(xyz), shs=None, colors_precomp=torch.ones_like(xyz),
             opacities=op, scales=sc, rotations=rot, cov3D_precomp=None)[4]
     assert torch.equal(pw, ref) and float(pw.max()) > 0
+
+
+def _act_data(n, deg, seed):
+    rng = np.random.default_rng(seed)
+    K = max(3, (deg + 1) ** 2 - 1) + 1
+    d = dict(xyz=rng.normal(0, 2, (n, 3)), scaling=rng.normal(-1, 0.5, (n, 3)), rotation=rng.normal(0, 1, (n, 4)),
+             opacity=rng.normal(0, 2, (n, 1)), colors=rng.normal(0, 0.5, (n, 3)),
+             shs=rng.normal(0, 0.1, (n, K, 3)))
+    d["rotation"][:3] = 0.0
+    return {k: v.astype(np.float32) for k, v in d.items()}
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4])
+def test_activate_rows_forward_backward_vs_oracle(deg):
+    """The fused activation of LoG's training gather (activation.py:27-43):
+    outputs against the oracle forward, raw-row gradients against the oracle
+    backward (float64), with zero quaternions among the rows."""
+    from filtergs_amd.train import activate_root_return
+    n = 20000
+    d = _act_data(n, deg, 40 + deg)
+    campos = np.array([0.5, -8.0, 1.5], np.float32)
+    leaves = {k: nn.Parameter(torch.as_tensor(v, device=DEV)) for k, v in d.items()}
+    camera = {"camera_center": torch.as_tensor(campos, device=DEV)}
+    out = activate_root_return(leaves, camera, deg)
+    assert out["xyz"] is leaves["xyz"]
+    rng = np.random.default_rng(7)
+    g = {k: rng.normal(0, 1, tuple(out[k].shape)).astype(np.float32)
+         for k in ("scaling", "rotation", "opacity", "colors")}
+    torch.autograd.backward([out[k] for k in g], [torch.as_tensor(v, device=DEV) for v in g.values()])
+    K = d["shs"].shape[1]
+    ref = orc.activate_gather(np.arange(n), d["xyz"], d["scaling"], d["rotation"], d["opacity"].reshape(-1),
+                              d["colors"], d["shs"] if deg else None, deg, campos)
+    for k in ("scaling", "rotation", "colors"):
+        np.testing.assert_allclose(out[k].detach().cpu().numpy(), ref[k], rtol=2e-5, atol=2e-6)
+    np.testing.assert_allclose(out["opacity"].detach().cpu().numpy().reshape(-1), ref["opacity"],
+                               rtol=2e-5, atol=2e-6)
+    bw = orc.activate_backward(d["xyz"], d["scaling"], d["rotation"], d["opacity"], K, deg, campos,
+                               g["scaling"], g["rotation"], g["opacity"], g["colors"])
+    pairs = (("scaling", "log_scaling"), ("rotation", "rotation"), ("opacity", "opacity_logit"),
+             ("colors", "features_dc"))
+    for leaf, key in pairs:
+        got = leaves[leaf].grad.detach().cpu().numpy().reshape(bw[key].shape)
+        want = bw[key]
+        # rows 0-2 hold zero quaternions (rotation gradient g / 1e-12): own scale
+        for rows in (slice(0, 3), slice(3, None)):
+            scale = np.abs(want[rows]).max()
+            err = np.abs(got[rows] - want[rows]).max()
+            assert err <= 2e-5 * scale, (leaf, rows, err, scale)
+    if deg == 0:
+        assert leaves["shs"].grad is None
+    else:
+        got = leaves["shs"].grad.detach().cpu().numpy()
+        assert np.abs(got - bw["features_rest"]).max() <= 2e-5 * np.abs(bw["features_rest"]).max()
+        assert not got[:, (deg + 1) ** 2 - 1:].any()
+    assert leaves["xyz"].grad is None
+
+
+def _torch_log_activation(ret, campos, deg):
+    """The reference's activation sequence in torch (activation.py:27-43,
+    sh_utils.py:31-72), for the drop-in comparison."""
+    C0, C1 = 0.28209479177387814, 0.4886025119029199
+    C2 = [1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792,
+          0.5462742152960396]
+    colors = ret["colors"] * C0 + 0.5
+    if deg > 0:
+        d = ret["xyz"].detach() - campos[None]
+        d = d / torch.norm(d, dim=-1, keepdim=True)
+        x, y, z = d[..., 0:1], d[..., 1:2], d[..., 2:3]
+        sh = ret["shs"]
+        r = -C1 * y * sh[..., 0, :] + C1 * z * sh[..., 1, :] - C1 * x * sh[..., 2, :]
+        if deg > 1:
+            xx, yy, zz, xy, yz, xz = x * x, y * y, z * z, x * y, y * z, x * z
+            r = (r + C2[0] * xy * sh[..., 3, :] + C2[1] * yz * sh[..., 4, :] +
+                 C2[2] * (2.0 * zz - xx - yy) * sh[..., 5, :] + C2[3] * xz * sh[..., 6, :] +
+                 C2[4] * (xx - yy) * sh[..., 7, :])
+        colors = colors + r
+    return {"xyz": ret["xyz"], "scaling": torch.exp(ret["scaling"]), "opacity": torch.sigmoid(ret["opacity"]),
+            "rotation": torch.nn.functional.normalize(ret["rotation"]), "colors": colors}
+
+
+@pytest.mark.parametrize("fix_parent", [True, False])
+def test_log_get_all_drop_in(fix_parent):
+    """get_all(log_model, camera) against LoG.get_all's sequence
+    (level_of_gaussian.py:260-294) in torch: same outputs, same gradients on
+    the gathered leaves in visibility_flag['params']."""
+    from types import SimpleNamespace
+    from filtergs_amd.train import get_all
+    N, deg = 50000, 2
+    d = _act_data(N, deg, 3)
+    bufs = {k: torch.as_tensor(v, device=DEV) for k, v in d.items()}
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    perm = torch.randperm(N, generator=gen)
+    index, index_node = perm[:20000].to(DEV), perm[20000:23000].to(DEV)
+    campos = torch.tensor([0.5, -8.0, 1.5], device=DEV)
+    camera = {"camera_center": campos}
+
+    def model():
+        g = SimpleNamespace(items=lambda: iter(bufs.items()), active_sh_degree=deg,
+                            visibility_flag={"index": index, "index_node": index_node})
+        return SimpleNamespace(gaussian=g, fix_parent=fix_parent, training=True)
+
+    ours_m, ref_m = model(), model()
+    ours = get_all(ours_m, camera)
+    # the reference's gather, then its activation chain
+    vf = ref_m.gaussian.visibility_flag
+    if fix_parent:
+        leaves = {k: nn.Parameter(v[index]) for k, v in bufs.items()}
+        ret = {k: torch.cat([leaves[k], bufs[k][index_node]]) for k in bufs}
+    else:
+        leaves = {k: nn.Parameter(v[torch.cat([index, index_node])]) for k, v in bufs.items()}
+        ret = leaves
+    vf["params"] = leaves
+    ref = _torch_log_activation(ret, campos, deg)
+    gen = torch.Generator(device=DEV).manual_seed(9)
+    loss_w = {k: torch.randn(ref[k].shape, device=DEV, generator=gen) for k in ("scaling", "rotation", "opacity", "colors")}
+    for out in (ours, ref):
+        sum((out[k] * loss_w[k]).sum() for k in loss_w).backward()
+    for k in ("xyz", "scaling", "rotation", "opacity", "colors"):
+        torch.testing.assert_close(ours[k], ref[k], rtol=1e-5, atol=1e-6)
+    ours_leaves = ours_m.gaussian.visibility_flag["params"]
+    for k in ("scaling", "rotation", "opacity", "colors", "shs"):
+        torch.testing.assert_close(ours_leaves[k].grad, leaves[k].grad, rtol=1e-4, atol=1e-5)
+    assert ours_leaves["xyz"].grad is None and leaves["xyz"].grad is None

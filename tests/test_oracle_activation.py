@@ -62,3 +62,59 @@ def test_elementwise_activations(orc):
     np.testing.assert_array_equal(out["rotation"][2], [0.0, 0, 0, 0])   # F.normalize of zero
     np.testing.assert_allclose(out["rotation"][0], [0.5] * 4)
     np.testing.assert_allclose(out["colors"], 0.5)                     # SH2RGB(0)
+
+
+def _act_inputs(n=60, deg=3, seed=0):
+    rng = np.random.default_rng(seed)
+    K = (deg + 1) ** 2 - 1 if deg > 0 else 3
+    d = dict(xyz=rng.normal(0, 2, (n, 3)), ls=rng.normal(-1, 0.5, (n, 3)), q=rng.normal(0, 1, (n, 4)),
+             z=rng.normal(0, 2, n), dc=rng.normal(0, 0.5, (n, 3)), rest=rng.normal(0, 0.1, (n, K + 2, 3)),
+             campos=np.array([0.3, -7.0, 1.0]),
+             gs=rng.normal(0, 1, (n, 3)), gr=rng.normal(0, 1, (n, 4)), go=rng.normal(0, 1, n),
+             gc=rng.normal(0, 1, (n, 3)))
+    d["q"][0] = 0.0            # zero quaternion: below F.normalize's eps
+    d["q"][1] = 1e-13          # |q| = 2e-13, below eps too
+    return d
+
+
+def _loss_of_forward(orc, d, deg, ls, q, z, dc, rest):
+    act = orc.activate_gather(np.arange(len(z)), d["xyz"], ls, q, z, dc, rest, deg, d["campos"])
+    return ((act["scaling"] * d["gs"]).sum() + (act["rotation"] * d["gr"]).sum() +
+            (act["opacity"] * d["go"]).sum() + (act["colors"] * d["gc"]).sum())
+
+
+def test_activation_backward_by_finite_differences(orc):
+    """The oracle backward against central differences of the oracle forward
+    (float64), row by row on a sample of entries, degrees 0-4.  Rows 0-1 have
+    quaternions below F.normalize's eps, where the forward is q / 1e-12
+    (linear with slope 1e12), checked with a step scaled to it."""
+    for deg in (0, 1, 2, 3, 4):
+        d = _act_inputs(deg=deg, seed=deg)
+        K = d["rest"].shape[1]
+        bw = orc.activate_backward(d["xyz"], d["ls"], d["q"], d["z"], K, deg, d["campos"],
+                                   d["gs"], d["gr"], d["go"], d["gc"])
+        args = dict(ls=d["ls"], q=d["q"], z=d["z"], dc=d["dc"], rest=d["rest"])
+        rng = np.random.default_rng(100 + deg)
+        for name, key, shape in (("ls", "log_scaling", None), ("q", "rotation", None),
+                                 ("z", "opacity_logit", None), ("dc", "features_dc", None),
+                                 ("rest", "features_rest", None)):
+            a = args[name]
+            for _ in range(12):
+                pos = tuple(int(rng.integers(0, s)) for s in a.shape)
+                if name == "q" and pos[0] < 2:
+                    h = 1e-16
+                else:
+                    h = 1e-6 * max(1.0, abs(a[pos]))
+                plus = {k: v.copy() for k, v in args.items()}
+                minus = {k: v.copy() for k, v in args.items()}
+                plus[name][pos] += h
+                minus[name][pos] -= h
+                fd = (_loss_of_forward(orc, d, deg, **plus) - _loss_of_forward(orc, d, deg, **minus)) / (2 * h)
+                an = bw[key][pos]
+                assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (deg, name, pos, fd, an)
+        if deg == 0:
+            assert not bw["features_rest"].any()
+        else:
+            used = (deg + 1) ** 2 - 1
+            assert not bw["features_rest"][:, used:].any()
+        np.testing.assert_allclose(bw["rotation"][0], d["gr"][0] / 1e-12)
