@@ -54,14 +54,17 @@ def parse():
 def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
     """Algorithmic HBM bytes per launch of each stage.  P Gaussians, V views,
     Pv visible (Gaussian, view) pairs, K kv pairs, T tiles and npix pixels over
-    all views.  The batched preprocess reads the Gaussians once for all views
-    and the batched depth sort makes 5 passes over V*P keys."""
+    all views.  The batched preprocess reads the Gaussians once for all views;
+    the plain depth sort makes 4 passes over all P keys (culled ones sort
+    last), the batched one 5 passes over the Pv visible keys only (the
+    preprocess appends just those)."""
     params = P * (44 + 12 * M)
     grads_out = P * (92 + 12 * M)
+    sorted_keys = P if V == 1 else Pv
     return {
         "preprocess": params + V * P * 8 + Pv * 40,
-        "depth_sort": (4 if V == 1 else 5) * 24 * V * P,
-        "tile_offsets": 8 * V * P,
+        "depth_sort": (4 if V == 1 else 5) * 24 * sorted_keys,
+        "tile_offsets": 8 * sorted_keys,
         "duplicate": 20 * Pv + 8 * K,
         "tile_sort": passes_tile * 24 * K,
         "ranges": 8 * K + 8 * T,

@@ -120,6 +120,8 @@ struct Geom {
     uint32_t* offs;
     uint32_t* hist;
     uint32_t* partials;
+    uint32_t* seg_cnt;              // batch: keys per preprocess wave
+    unsigned long long* nvis;       // batch: visible (view, Gaussian) keys after the depth sort
 };
 
 size_t scan_words_for_hist(size_t n)
@@ -148,6 +150,8 @@ Geom carve_geom(void* buf, int P, size_t* total)
     size_t pw = fgs::scan_partials_words(n);
     const size_t ph = scan_words_for_hist(n);
     g.partials = c.take<uint32_t>(pw > ph ? pw : ph);
+    g.seg_cnt = c.take<uint32_t>(n / FGS_WAVE + 1);
+    g.nvis = c.take<unsigned long long>(1);
     if (total) *total = c.off;
     return g;
 }
@@ -690,7 +694,7 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         b.bg = s->bg;
         b.bwd_batch = g_shape[1];
         ProfScope ps(ST_BLEND_BWD, st);
-        const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, 0 };
+        const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, 0, nullptr };
         if ((rc = check(fgs::launch_blend_bwd(b, gg, im.final_T, im.vis_last, dL_dpix, dL_dmeans2D,
                                               dconic, dL_dopacity, dL_dcolors, s->grad_mode == 1, st),
                         "blend backward")))
@@ -1277,25 +1281,28 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         o.recs = gm.recs;
         o.vkeys = gm.vkeysA;
         o.error_flag = (int*)(counts + 2);
+        o.seg_cnt = gm.seg_cnt;
         {
             ProfScope ps(ST_PREPROCESS, st);
             if ((rc = check(fgs::launch_preprocess_fwd_batch(a, views, V, o, st), "batch preprocess")))
                 return rc;
         }
-        // (view, depth, index) order: bits 26..63, 5 passes -> result in B
+        // (view, depth, index) order of the visible keys: bits 26..63, 5
+        // passes -> result in B, their number in gm.nvis
         {
             ProfScope ps(ST_DEPTH_SORT, st);
-            if ((rc = check(fgs::radix_sort_u64(gm.vkeysA, gm.vkeysB, n, fgs::kBatchIdxBits, 5, gm.hist,
-                                                gm.partials, st), "batch depth sort")))
+            if ((rc = check(fgs::radix_sort_u64_segments(gm.vkeysA, gm.vkeysB, gm.seg_cnt, P, V,
+                                                         fgs::kBatchIdxBits, 5, gm.hist, gm.partials,
+                                                         gm.nvis, st), "batch depth sort")))
                 return rc;
         }
         unsigned long long* cnt = (unsigned long long*)counts;
         {
             ProfScope ps(ST_OFFSETS, st);
-            if ((rc = check(fgs::launch_sorted_records_batch(gm.vkeysB, gm.recs, n, P, gm.tiles_s,
+            if ((rc = check(fgs::launch_sorted_records_batch(gm.vkeysB, gm.recs, n, gm.nvis, P, gm.tiles_s,
                                                              gm.rects_s, st), "batch records")))
                 return rc;
-            if ((rc = check(fgs::launch_scan_tiles_batch(gm.tiles_s, gm.offs, n, gm.partials, cnt, st),
+            if ((rc = check(fgs::launch_scan_tiles_batch(gm.tiles_s, gm.offs, n, gm.nvis, gm.partials, cnt, st),
                             "batch offsets")))
                 return rc;
         }
@@ -1307,8 +1314,8 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
             uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
             {
                 ProfScope ps(ST_DUPLICATE, st);
-                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, P, views,
-                                                            first, (size_t)K_cap, cnt + 1, gm.recs, st),
+                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, gm.nvis, P,
+                                                            views, first, (size_t)K_cap, cnt + 1, gm.recs, st),
                                 "batch duplicate")))
                     return rc;
             }
@@ -1385,7 +1392,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         b.conic_opacity = gm.conic_opacity;
         b.bwd_batch = g_shape[1];
         ProfScope ps(ST_BLEND_BWD, st);
-        const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P };
+        const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P, gm.nvis };
         if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,
                                                     dL_dmeans2D, dconic, dop_view, dL_dcolors,
                                                     s->grad_mode == 1, st), "batch blend backward")))

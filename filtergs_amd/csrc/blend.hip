@@ -898,7 +898,8 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                                                      float* __restrict__ dL_dcolors)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= gg.n) return;
+    const size_t live = gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
+    if (s >= live) return;
     const uint32_t n = gg.tiles_s[s];
     if (n > 0u && n <= kGatherSolo) {
         const uint64_t key = gg.vkeys[s];

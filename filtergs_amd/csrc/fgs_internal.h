@@ -50,6 +50,7 @@ struct PreOut {
     uint4* recs;                        // packed tile rectangle (x0|y0<<16, x1|y1<<16), tiles, 0
     uint64_t* vkeys;                    // (depth_bits << 32 | i), culled: (~0 << 32 | i)
     int* error_flag;                    // bit 0: prefiltered point culled
+    uint32_t* seg_cnt;                  // batch: keys each wave appended to its segment
 };
 
 struct PreBwdIn {
@@ -159,6 +160,13 @@ hipError_t launch_scan_tiles(const uint32_t* tiles_s, uint32_t* offs, int P, uin
 size_t radix_hist_words(size_t n);
 hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
                           uint32_t* hist, uint32_t* partials, hipStream_t s);
+// Batched depth sort over the per-wave segments the batched preprocess wrote
+// (segment w: seg_cnt[w] keys from w * 64 * V): the first pass reads the
+// segments, the later ones the dense result, whose size the first pass's
+// scan stores in *n_dev.  The sorted keys end in b (5 passes).
+hipError_t radix_sort_u64_segments(uint64_t* a, uint64_t* b, const uint32_t* seg_cnt, int P, int V,
+                                   int shift_lo, int n_passes, uint32_t* hist, uint32_t* partials,
+                                   unsigned long long* n_dev, hipStream_t s);
 // duplicate also records in recs[g].w the position of the Gaussian's first pair
 hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                             int P, int gx, uint64_t* pairs, uint4* recs, hipStream_t s);
@@ -173,12 +181,17 @@ __device__ __forceinline__ uint32_t batch_elem(uint64_t key, int P)
 {
     return (uint32_t)(key >> kBatchViewShift) * (uint32_t)P + (uint32_t)(key & kBatchIdxMask);
 }
-hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n, int P,
-                                       uint32_t* tiles_s, uint2* rects_s, hipStream_t s);
-hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n, uint32_t* partials,
+// The batched binning stages run over the first *n_dev of n slots (the
+// visible (view, Gaussian) keys the depth sort left).
+hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n,
+                                       const unsigned long long* n_dev, int P, uint32_t* tiles_s,
+                                       uint2* rects_s, hipStream_t s);
+hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n,
+                                   const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s);
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                                  size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
+                                  size_t n, const unsigned long long* n_dev, int P,
+                                  const BatchView* views, uint64_t* pairs, size_t cap,
                                   const unsigned long long* K, uint4* recs, hipStream_t s);
 hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, int P, const BatchView* views,
                                uint2* ranges, int T, hipStream_t s);
@@ -211,6 +224,7 @@ struct GradGather {
     const uint32_t* offs;
     size_t n;
     int batch_P;
+    const unsigned long long* n_dev;   // batch: the first *n_dev of the n slots are in use
 };
 hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const float* final_T,
                             const uint32_t* vis_last, const float* dL_dpix, float* dL_dmean2D,

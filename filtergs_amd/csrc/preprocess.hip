@@ -397,6 +397,14 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
     gaussian_inputs(a, i, p, c3);
     const float opacity_in = a.opacities[i];
     const ShLds sh{ s_sh, (int)threadIdx.x };
+    // Only visible (view, Gaussian) keys go to the sort: each wave appends
+    // its own to a segment of 64 * V slots (ballot ranks, no atomics) and
+    // records how many; the key order inside a segment does not matter, the
+    // keys are unique.
+    uint64_t* seg = o.vkeys + (size_t)(i / FGS_WAVE) * FGS_WAVE * V;
+    const int lane = lane_id();
+    const uint64_t below = lane ? (~0ull >> (FGS_WAVE - lane)) : 0ull;
+    uint32_t appended = 0;
     for (int v = 0; v < V; ++v) {
         PreArgs av = a;
         bind_view(av, views[v]);
@@ -404,9 +412,14 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
         preprocess_view(av, p[0], p[1], p[2], c3, opacity_in, sh, o.error_flag, r);
         const size_t e = (size_t)v * a.P + i;
         write_pre(o, e, r, !a.colors_precomp);
-        const uint64_t hi = r.tiles ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
-        o.vkeys[e] = ((uint64_t)v << 58) | (hi << 26) | (uint32_t)i;
+        const bool vis = r.tiles != 0;
+        const uint64_t m = __ballot(vis);
+        if (vis)
+            seg[appended + __popcll(m & below)] =
+                ((uint64_t)v << 58) | ((uint64_t)__float_as_uint(r.depth) << 26) | (uint32_t)i;
+        appended += (uint32_t)__popcll(m);
     }
+    if (lane == 0) o.seg_cnt[i / FGS_WAVE] = appended;
 }
 
 // ---------------------------------------------------------------------------

@@ -62,11 +62,22 @@ struct PlainLoader {
     __device__ uint32_t operator()(size_t s) const { return in[s]; }
 };
 
+// n_dev (may be null): only the first min(n, *n_dev) elements are scanned;
+// the grid still covers n.
+__device__ __forceinline__ size_t live_count(size_t n, const unsigned long long* n_dev)
+{
+    if (!n_dev) return n;
+    const size_t d = (size_t)*n_dev;
+    return d < n ? d : n;
+}
+
 template <typename Loader>
 __global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, uint32_t* partials,
-                                                          unsigned long long* nz_count)
+                                                          unsigned long long* nz_count,
+                                                          const unsigned long long* n_dev)
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    n = live_count(n, n_dev);
     const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
     uint32_t acc = 0, nz = 0;
 #pragma unroll
@@ -108,9 +119,10 @@ __global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, 
 
 template <typename Loader>
 __global__ __launch_bounds__(kThreads) void k_scan_down(Loader ld, size_t n, const uint32_t* partials,
-                                                        uint32_t* out)
+                                                        uint32_t* out, const unsigned long long* n_dev)
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    n = live_count(n, n_dev);
     const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
     uint32_t v[kIpt];
     uint32_t acc = 0;
@@ -133,14 +145,16 @@ size_t scan_partials_words(size_t n) { return div_up(n > 0 ? n : 1, kChunk); }
 template <typename Loader>
 static hipError_t exclusive_scan(Loader ld, size_t n, uint32_t* out, uint32_t* partials,
                                  hipStream_t s, unsigned long long* total = nullptr,
-                                 unsigned long long* nz_count = nullptr)
+                                 unsigned long long* nz_count = nullptr,
+                                 const unsigned long long* n_dev = nullptr)
 {
     if (n == 0) return hipSuccess;
     const size_t nb = div_up(n, kChunk);
     hipLaunchKernelGGL(k_scan_reduce<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials,
-                       nz_count);
+                       nz_count, n_dev);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kThreads), 0, s, partials, nb, total);
-    hipLaunchKernelGGL(k_scan_down<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials, out);
+    hipLaunchKernelGGL(k_scan_down<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials, out,
+                       n_dev);
     return hipGetLastError();
 }
 
@@ -178,75 +192,144 @@ hipError_t launch_scan_tiles(const uint32_t* tiles_s, uint32_t* offs, int P, uin
 // ---------------------------------------------------------------------------
 // LSD radix sort of u64 keys on 8-bit digits.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_rs_count(const uint64_t* __restrict__ in, size_t n,
-                                                       int shift, uint32_t* __restrict__ hist,
-                                                       size_t nb)
+// Adds the 8-bit digits of keys src[0 .. c) to the per-wave LDS histograms.
+__device__ __forceinline__ void rs_count_run(const uint64_t* __restrict__ src, size_t c, int shift,
+                                             uint32_t (*s_h)[kRadix])
 {
-    __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
     const int wave = threadIdx.x / FGS_WAVE;
-    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][threadIdx.x] = 0;
-    __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kChunk;
-#pragma unroll 4
-    for (int j = 0; j < kIpt; ++j) {
-        const size_t e = base + (size_t)j * kThreads + threadIdx.x;
-        if (e < n) {
-            const uint32_t d = (uint32_t)(in[e] >> shift) & (kRadix - 1);
-            atomicAdd(&s_h[wave][d], 1u);
-        }
+    for (size_t e = threadIdx.x; e < c; e += kThreads) {
+        const uint32_t d = (uint32_t)(src[e] >> shift) & (kRadix - 1);
+        atomicAdd(&s_h[wave][d], 1u);
     }
+}
+
+__device__ __forceinline__ void rs_count_store(uint32_t (*s_h)[kRadix], uint32_t* __restrict__ hist, size_t nb)
+{
     __syncthreads();
     const uint32_t c = s_h[0][threadIdx.x] + s_h[1][threadIdx.x] + s_h[2][threadIdx.x] + s_h[3][threadIdx.x];
     hist[(size_t)threadIdx.x * nb + blockIdx.x] = c;
 }
 
-__global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restrict__ in,
-                                                         uint64_t* __restrict__ out, size_t n,
-                                                         int shift, const uint32_t* __restrict__ offs,
-                                                         size_t nb)
+__global__ __launch_bounds__(kThreads) void k_rs_count(const uint64_t* __restrict__ in, size_t n,
+                                                       int shift, uint32_t* __restrict__ hist,
+                                                       size_t nb, const unsigned long long* n_dev)
 {
-    __shared__ uint32_t s_cnt[kThreads / FGS_WAVE][kRadix];
-    __shared__ uint32_t s_off[kRadix];
+    __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][threadIdx.x] = 0;
+    __syncthreads();
+    n = live_count(n, n_dev);
+    const size_t base = (size_t)blockIdx.x * kChunk;
+    if (base < n) rs_count_run(in + base, n - base < (size_t)kChunk ? n - base : (size_t)kChunk, shift, s_h);
+    rs_count_store(s_h, hist, nb);
+}
+
+// One step of the stable scatter: the block's 256 keys (valid where given)
+// are ranked among equal digits with wave64 ballots and an LDS prefix over
+// the four waves, then written at the digit's running offset in s_off.
+__device__ __forceinline__ void rs_scatter_step(bool valid, uint64_t key, int shift,
+                                                uint32_t (*s_cnt)[kRadix], uint32_t* s_off,
+                                                uint64_t* __restrict__ out)
+{
     const int wave = threadIdx.x / FGS_WAVE;
     const int lane = lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
+    const uint32_t d = (uint32_t)(key >> shift) & (kRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = __popcll(peers & lt_mask);
+    if (valid && rank == 0) s_cnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    {
+        const int t = threadIdx.x;
+        uint32_t run = s_off[t];
+#pragma unroll
+        for (int w = 0; w < kThreads / FGS_WAVE; ++w) {
+            const uint32_t c = s_cnt[w][t];
+            s_cnt[w][t] = run;
+            run += c;
+        }
+        s_off[t] = run;
+    }
+    __syncthreads();
+    if (valid) out[s_cnt[wave][d] + rank] = key;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void rs_scatter_init(uint32_t (*s_cnt)[kRadix], uint32_t* s_off,
+                                                const uint32_t* __restrict__ offs, size_t nb)
+{
 #pragma unroll
     for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
     s_off[threadIdx.x] = offs[(size_t)threadIdx.x * nb + blockIdx.x];
     __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restrict__ in,
+                                                         uint64_t* __restrict__ out, size_t n,
+                                                         int shift, const uint32_t* __restrict__ offs,
+                                                         size_t nb, const unsigned long long* n_dev)
+{
+    __shared__ uint32_t s_cnt[kThreads / FGS_WAVE][kRadix];
+    __shared__ uint32_t s_off[kRadix];
+    n = live_count(n, n_dev);
     const size_t base = (size_t)blockIdx.x * kChunk;
+    if (base >= n) return;
+    rs_scatter_init(s_cnt, s_off, offs, nb);
     for (int j = 0; j < kIpt; ++j) {
         const size_t e = base + (size_t)j * kThreads + threadIdx.x;
         const bool valid = e < n;
-        const uint64_t key = valid ? in[e] : 0ull;
-        const uint32_t d = (uint32_t)(key >> shift) & (kRadix - 1);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
+        rs_scatter_step(valid, valid ? in[e] : 0ull, shift, s_cnt, s_off, out);
+    }
+}
+
+// Segmented input of the batched preprocess: segment w holds seg_cnt[w]
+// keys from w * seg_len; block b owns segments [b * spb, (b + 1) * spb).
+__global__ __launch_bounds__(kThreads) void k_rs_count_seg(const uint64_t* __restrict__ in,
+                                                           const uint32_t* __restrict__ seg_cnt,
+                                                           size_t n_seg, size_t seg_len, int spb,
+                                                           int shift, uint32_t* __restrict__ hist,
+                                                           size_t nb)
+{
+    __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][threadIdx.x] = 0;
+    __syncthreads();
+    for (int q = 0; q < spb; ++q) {
+        const size_t w = (size_t)blockIdx.x * spb + q;
+        if (w >= n_seg) break;
+        const size_t c = seg_cnt[w] < seg_len ? seg_cnt[w] : seg_len;
+        rs_count_run(in + w * seg_len, c, shift, s_h);
+    }
+    rs_count_store(s_h, hist, nb);
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_scatter_seg(const uint64_t* __restrict__ in,
+                                                             uint64_t* __restrict__ out,
+                                                             const uint32_t* __restrict__ seg_cnt,
+                                                             size_t n_seg, size_t seg_len, int spb,
+                                                             int shift, const uint32_t* __restrict__ offs,
+                                                             size_t nb)
+{
+    __shared__ uint32_t s_cnt[kThreads / FGS_WAVE][kRadix];
+    __shared__ uint32_t s_off[kRadix];
+    rs_scatter_init(s_cnt, s_off, offs, nb);
+    for (int q = 0; q < spb; ++q) {
+        const size_t w = (size_t)blockIdx.x * spb + q;
+        if (w >= n_seg) break;
+        const size_t c = seg_cnt[w] < seg_len ? seg_cnt[w] : seg_len;
+        const uint64_t* src = in + w * seg_len;
+        for (size_t j0 = 0; j0 < c; j0 += kThreads) {
+            const size_t e = j0 + threadIdx.x;
+            const bool valid = e < c;
+            rs_scatter_step(valid, valid ? src[e] : 0ull, shift, s_cnt, s_off, out);
         }
-        const uint32_t rank = __popcll(peers & lt_mask);
-        if (valid && rank == 0) s_cnt[wave][d] = __popcll(peers);
-        __syncthreads();
-        {
-            const int t = threadIdx.x;
-            uint32_t run = s_off[t];
-#pragma unroll
-            for (int w = 0; w < kThreads / FGS_WAVE; ++w) {
-                const uint32_t c = s_cnt[w][t];
-                s_cnt[w][t] = run;
-                run += c;
-            }
-            s_off[t] = run;
-        }
-        __syncthreads();
-        if (valid) out[s_cnt[wave][d] + rank] = key;
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
-        __syncthreads();
     }
 }
 
@@ -264,14 +347,55 @@ hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int 
     uint64_t* dst = b;
     for (int p = 0; p < n_passes; ++p) {
         const int shift = shift_lo + 8 * p;
-        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nb), dim3(kThreads), 0, s, src, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nb), dim3(kThreads), 0, s, src, n, shift, hist, nb,
+                           (const unsigned long long*)nullptr);
         hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(kThreads), 0, s, src, dst, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(kThreads), 0, s, src, dst, n, shift, hist, nb,
+                           (const unsigned long long*)nullptr);
         uint64_t* t = src;
         src = dst;
         dst = t;
     }
+    return hipGetLastError();
+}
+
+// Segment length 64 * V (one preprocess wave's keys); spb segments per block
+// make a block cover at least kChunk slots, so the first pass needs no more
+// histogram columns than a dense pass over V * P keys.
+hipError_t radix_sort_u64_segments(uint64_t* a, uint64_t* b, const uint32_t* seg_cnt, int P, int V,
+                                   int shift_lo, int n_passes, uint32_t* hist, uint32_t* partials,
+                                   unsigned long long* n_dev, hipStream_t s)
+{
+    const size_t n = (size_t)P * (size_t)V;
+    if (n == 0 || n_passes < 1) return hipSuccess;
+    const size_t seg_len = (size_t)FGS_WAVE * (size_t)V;
+    const size_t n_seg = div_up((size_t)P, FGS_WAVE);
+    const int spb = (int)div_up((size_t)kChunk, seg_len);
+    const size_t nb_seg = div_up(n_seg, (size_t)spb);
+    hipLaunchKernelGGL(k_rs_count_seg, dim3((unsigned)nb_seg), dim3(kThreads), 0, s, a, seg_cnt, n_seg,
+                       seg_len, spb, shift_lo, hist, nb_seg);
+    hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb_seg, hist, partials, s, n_dev);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rs_scatter_seg, dim3((unsigned)nb_seg), dim3(kThreads), 0, s, a, b, seg_cnt, n_seg,
+                       seg_len, spb, shift_lo, hist, nb_seg);
+    // the rest: dense over the first *n_dev slots; ping-pong so the result ends in b
+    const size_t nb = div_up(n, kChunk);
+    uint64_t* src = b;
+    uint64_t* dst = a;
+    for (int p = 1; p < n_passes; ++p) {
+        const int shift = shift_lo + 8 * p;
+        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nb), dim3(kThreads), 0, s, src, n, shift, hist, nb,
+                           (const unsigned long long*)n_dev);
+        e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(kThreads), 0, s, src, dst, n, shift, hist, nb,
+                           (const unsigned long long*)n_dev);
+        uint64_t* t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != b) return hipErrorInvalidValue;   // needs an odd number of passes
     return hipGetLastError();
 }
 
@@ -382,11 +506,12 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __restrict__ vkeys,
                                                               const uint4* __restrict__ recs, size_t n,
+                                                              const unsigned long long* __restrict__ n_dev,
                                                               int P, uint32_t* __restrict__ tiles_s,
                                                               uint2* __restrict__ rects_s)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    if (s >= live_count(n, n_dev)) return;
     const uint64_t key = vkeys[s];
     // culled elements (all-ones depth field) have no rectangle stored
     const bool culled = ((key >> kBatchIdxBits) & 0xFFFFFFFFull) == 0xFFFFFFFFull;
@@ -395,30 +520,35 @@ __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __
     rects_s[s] = make_uint2(r.x, r.y);
 }
 
-hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n, int P,
-                                       uint32_t* tiles_s, uint2* rects_s, hipStream_t s)
+hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n,
+                                       const unsigned long long* n_dev, int P, uint32_t* tiles_s,
+                                       uint2* rects_s, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_sorted_records_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys,
-                       recs, n, P, tiles_s, rects_s);
+                       recs, n, n_dev, P, tiles_s, rects_s);
     return hipGetLastError();
 }
 
-hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n, uint32_t* partials,
+hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n,
+                                   const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s)
 {
-    return exclusive_scan(PlainLoader{ tiles_s }, n, offs, partials, s, counters + 1, counters);
+    return exclusive_scan(PlainLoader{ tiles_s }, n, offs, partials, s, counters + 1, counters, n_dev);
 }
 
 // Pairs past the capacity are dropped (the caller compares K with it).
 __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restrict__ vkeys,
                                                          const uint32_t* __restrict__ offs,
                                                          const uint2* __restrict__ rects_s, size_t n,
+                                                         const unsigned long long* __restrict__ n_dev,
                                                          int P, const BatchView* __restrict__ views,
                                                          uint64_t* __restrict__ pairs, size_t cap,
                                                          uint4* __restrict__ recs)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    n = live_count(n, n_dev);
+    if ((size_t)blockIdx.x * blockDim.x >= n) return;   // whole block past the live keys
     const bool valid = s < n;
     const uint2 rc = valid ? rects_s[s] : make_uint2(0u, 0u);
     const uint64_t key = valid ? vkeys[s] : 0ull;
@@ -441,7 +571,8 @@ __global__ __launch_bounds__(256) void k_fill_tail(uint64_t* __restrict__ pairs,
 }
 
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                                  size_t n, int P, const BatchView* views, uint64_t* pairs, size_t cap,
+                                  size_t n, const unsigned long long* n_dev, int P,
+                                  const BatchView* views, uint64_t* pairs, size_t cap,
                                   const unsigned long long* K, uint4* recs, hipStream_t s)
 {
     if (cap == 0) return hipSuccess;
@@ -450,7 +581,7 @@ hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, c
     hipLaunchKernelGGL(k_fill_tail, dim3((unsigned)blocks), dim3(256), 0, s, pairs, K, cap);
     if (n == 0) return hipGetLastError();
     hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys, offs,
-                       rects_s, n, P, views, pairs, cap, recs);
+                       rects_s, n, n_dev, P, views, pairs, cap, recs);
     return hipGetLastError();
 }
 

@@ -126,3 +126,32 @@ def test_batch_graph_replay_and_capacity():
     graph.replay()
     torch.cuda.synchronize()
     assert not torch.equal(br.color, ref_color)
+
+
+@pytest.mark.parametrize("V", [1, 20])
+def test_batch_view_counts(V):
+    """One view and more views than a 4096-key sort chunk holds per wave
+    (the depth sort reads the preprocess's per-wave segments of 64 * V
+    keys): the batch equals the per-view renders, with a Gaussian count
+    that leaves the last wave partial."""
+    g, _ = small_scene(n=5037, seed=191, dist=5.0)
+    base = _cams()
+    cams = [base[v % 4] if v < 4 else make_camera(64 + 8 * v, 48 + 4 * v,
+                                                    center=(4.0 * np.cos(v), 4.0 * np.sin(v), 0.5 * (v % 3)),
+                                                    fovx_deg=50.0 + v)
+            for v in range(V)]
+    P = g["means3D"].shape[0]
+    rng = np.random.default_rng(V)
+    dLs = [rng.normal(size=(3, c.height, c.width)).astype(np.float32) for c in cams]
+    singles = [_single(g, c, (0.1, 0.2, 0.3), dL) for c, dL in zip(cams, dLs)]
+    br = BatchRasterizer(cams, P, bg=(0.1, 0.2, 0.3))
+    br.bind(_t(g["means3D"]), _t(g["opacities"]), _t(g["scales"]), _t(g["rotations"]), shs=_t(g["shs"]))
+    K = br.forward()
+    assert K == sum(int(s[0][0]) for s in singles)
+    for v, (out, _) in enumerate(singles):
+        assert torch.equal(br.radii[v], out[5]), v
+        assert torch.equal(br.view_color(v), out[1]), v
+        assert torch.equal(br.view_pixels(br.point_id, v), out[2]), v
+    grads = br.backward([_t(d) for d in dLs])
+    summed = sum(s[1][3] for s in singles)
+    assert _rel(grads["means3D"].reshape(summed.shape), summed) <= 1e-4
