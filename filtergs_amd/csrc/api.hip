@@ -1322,7 +1322,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
             {
                 ProfScope ps(ST_TILE_SORT, st);
                 if ((rc = check(fgs::radix_sort_u64(first, second, (size_t)K_cap, 32, passes, bn.hist,
-                                                    bn.partials, st), "batch tile sort")))
+                                                    bn.partials, st, cnt + 1), "batch tile sort")))
                     return rc;
             }
             b.pairs = bn.pairsA;
@@ -1335,7 +1335,8 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     }
     {
         ProfScope ps(ST_RANGES, st);
-        if ((rc = check(fgs::launch_ranges_batch(b.pairs, b.pairs ? (size_t)K_cap : 0, P, views, im.ranges, T, st),
+        if ((rc = check(fgs::launch_ranges_batch(b.pairs, b.pairs ? (size_t)K_cap : 0,
+                                                 (const unsigned long long*)counts + 1, P, views, im.ranges, T, st),
                         "batch ranges")))
             return rc;
     }

@@ -159,7 +159,8 @@ hipError_t launch_scan_tiles(const uint32_t* tiles_s, uint32_t* offs, int P, uin
                              unsigned long long* counters, hipStream_t s);
 size_t radix_hist_words(size_t n);
 hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
-                          uint32_t* hist, uint32_t* partials, hipStream_t s);
+                          uint32_t* hist, uint32_t* partials, hipStream_t s,
+                          const unsigned long long* n_dev = nullptr);
 // Batched depth sort over the per-wave segments the batched preprocess wrote
 // (segment w: seg_cnt[w] keys from w * 64 * V): the first pass reads the
 // segments, the later ones the dense result, whose size the first pass's
@@ -193,8 +194,8 @@ hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, c
                                   size_t n, const unsigned long long* n_dev, int P,
                                   const BatchView* views, uint64_t* pairs, size_t cap,
                                   const unsigned long long* K, uint4* recs, hipStream_t s);
-hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, int P, const BatchView* views,
-                               uint2* ranges, int T, hipStream_t s);
+hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, const unsigned long long* K, int P,
+                               const BatchView* views, uint2* ranges, int T, hipStream_t s);
 
 // blend
 // vis_last (per pixel): 1 + the index in the tile's visible list of the last

@@ -335,11 +335,13 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_seg(const uint64_t* __r
 
 size_t radix_hist_words(size_t n) { return (size_t)kRadix * div_up(n > 0 ? n : 1, kChunk); }
 
-// Sorts n keys on n_passes 8-bit digits starting at bit shift_lo.  The input
-// is in a; after the call the sorted keys are in a if n_passes is even and
-// in b otherwise.
+// Sorts n keys on n_passes 8-bit digits starting at bit shift_lo; with
+// n_dev only the first min(n, *n_dev) (read on the device).  The input is in
+// a; after the call the sorted keys are in a if n_passes is even and in b
+// otherwise.
 hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int n_passes,
-                          uint32_t* hist, uint32_t* partials, hipStream_t s)
+                          uint32_t* hist, uint32_t* partials, hipStream_t s,
+                          const unsigned long long* n_dev)
 {
     if (n == 0) return hipSuccess;
     const size_t nb = div_up(n, kChunk);
@@ -347,12 +349,11 @@ hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int 
     uint64_t* dst = b;
     for (int p = 0; p < n_passes; ++p) {
         const int shift = shift_lo + 8 * p;
-        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nb), dim3(kThreads), 0, s, src, n, shift, hist, nb,
-                           (const unsigned long long*)nullptr);
+        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nb), dim3(kThreads), 0, s, src, n, shift, hist, nb, n_dev);
         hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(kThreads), 0, s, src, dst, n, shift, hist, nb,
-                           (const unsigned long long*)nullptr);
+                           n_dev);
         uint64_t* t = src;
         src = dst;
         dst = t;
@@ -512,10 +513,8 @@ __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= live_count(n, n_dev)) return;
-    const uint64_t key = vkeys[s];
-    // culled elements (all-ones depth field) have no rectangle stored
-    const bool culled = ((key >> kBatchIdxBits) & 0xFFFFFFFFull) == 0xFFFFFFFFull;
-    const uint4 r = culled ? make_uint4(0u, 0u, 0u, 0u) : recs[batch_elem(key, P)];
+    // only visible (view, Gaussian) keys reach the sort
+    const uint4 r = recs[batch_elem(vkeys[s], P)];
     tiles_s[s] = r.z;
     rects_s[s] = make_uint2(r.x, r.y);
 }
@@ -560,60 +559,47 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
     dup_rect(valid, rc, off, e, gx, 0u, pairs, cap);
 }
 
-// Slots [K, cap) get all-ones keys, which sort after every real tile.
-__global__ __launch_bounds__(256) void k_fill_tail(uint64_t* __restrict__ pairs,
-                                                   const unsigned long long* __restrict__ K, size_t cap)
-{
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t k0 = (size_t)*K;
-    for (size_t k = k0 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < cap; k += stride)
-        pairs[k] = ~0ull;
-}
-
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                   size_t n, const unsigned long long* n_dev, int P,
                                   const BatchView* views, uint64_t* pairs, size_t cap,
                                   const unsigned long long* K, uint4* recs, hipStream_t s)
 {
-    if (cap == 0) return hipSuccess;
-    size_t blocks = div_up(cap, 256);
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_fill_tail, dim3((unsigned)blocks), dim3(256), 0, s, pairs, K, cap);
-    if (n == 0) return hipGetLastError();
+    (void)K;
+    if (cap == 0 || n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys, offs,
                        rects_s, n, n_dev, P, views, pairs, cap, recs);
     return hipGetLastError();
 }
 
 // identifyTileRanges over the batch: a range runs while (tile, view) stays
-// the same; it is stored at the view's tile_base + tile.  The all-ones tail
-// keys are skipped.
+// the same; it is stored at the view's tile_base + tile.  Only the first
+// min(cap, *K) sorted pairs exist.
 __device__ __forceinline__ uint64_t tile_view(uint64_t key, uint32_t P)
 {
     return ((key >> 32) << 32) | ((uint32_t)key / P);
 }
 
-__global__ __launch_bounds__(256) void k_ranges_batch(const uint64_t* __restrict__ sorted, size_t n,
+__global__ __launch_bounds__(256) void k_ranges_batch(const uint64_t* __restrict__ sorted, size_t cap,
+                                                      const unsigned long long* __restrict__ K,
                                                       uint32_t P, const BatchView* __restrict__ views,
                                                       uint2* __restrict__ ranges)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = live_count(cap, K);
     if (k >= n) return;
     const uint64_t key = sorted[k];
-    if (key == ~0ull) return;
     const uint64_t tv = tile_view(key, P);
     const uint32_t gt = views[(uint32_t)tv].tile_base + (uint32_t)(key >> 32);
     if (k == 0 || tile_view(sorted[k - 1], P) != tv) ranges[gt].x = (uint32_t)k;
-    if (k == n - 1 || sorted[k + 1] == ~0ull || tile_view(sorted[k + 1], P) != tv)
-        ranges[gt].y = (uint32_t)(k + 1);
+    if (k == n - 1 || tile_view(sorted[k + 1], P) != tv) ranges[gt].y = (uint32_t)(k + 1);
 }
 
-hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, int P, const BatchView* views,
-                               uint2* ranges, int T, hipStream_t s)
+hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, const unsigned long long* K, int P,
+                               const BatchView* views, uint2* ranges, int T, hipStream_t s)
 {
     hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
     if (e != hipSuccess || cap == 0) return e;
-    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted, cap,
+    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted, cap, K,
                        (uint32_t)P, views, ranges);
     return hipGetLastError();
 }
