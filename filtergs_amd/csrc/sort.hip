@@ -272,21 +272,91 @@ __device__ __forceinline__ void rs_scatter_init(uint32_t (*s_cnt)[kRadix], uint3
     __syncthreads();
 }
 
+// Dense pass.  The block's 4096 keys stay in registers while they are
+// ranked: wave w owns keys [w * 1024, (w + 1) * 1024) of the chunk, lane l
+// item j is key w * 1024 + j * 64 + l (coalesced loads), and items are ranked
+// in that order against per-wave digit counters in LDS, which only the owning
+// wave touches, so no barrier is needed until all 16 items are done.  The
+// keys are then placed in digit order in LDS and written out run by run, so
+// consecutive lanes store consecutive addresses.
 __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restrict__ in,
                                                          uint64_t* __restrict__ out, size_t n,
                                                          int shift, const uint32_t* __restrict__ offs,
                                                          size_t nb, const unsigned long long* n_dev)
 {
-    __shared__ uint32_t s_cnt[kThreads / FGS_WAVE][kRadix];
-    __shared__ uint32_t s_off[kRadix];
+    constexpr int kWaves = kThreads / FGS_WAVE;
+    constexpr int kPerWave = kChunk / kWaves;
+    __shared__ uint32_t s_cnt[kWaves][kRadix];   // per-wave digit counts, then local positions
+    __shared__ uint32_t s_start[kRadix];         // local start of each digit's run
+    __shared__ uint32_t s_glob[kRadix];          // global start of the block's run of each digit
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint64_t s_keys[kChunk];
     n = live_count(n, n_dev);
     const size_t base = (size_t)blockIdx.x * kChunk;
     if (base >= n) return;
-    rs_scatter_init(s_cnt, s_off, offs, nb);
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+    const int lane = lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
+    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
+    __syncthreads();
+    const size_t wbase = base + (size_t)wave * kPerWave;
+    uint64_t key[kIpt];
+    uint32_t rank[kIpt];
+#pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-        const size_t e = base + (size_t)j * kThreads + threadIdx.x;
-        const bool valid = e < n;
-        rs_scatter_step(valid, valid ? in[e] : 0ull, shift, s_cnt, s_off, out);
+        const size_t e = wbase + (size_t)j * FGS_WAVE + lane;
+        key[j] = e < n ? in[e] : 0ull;
+    }
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const bool valid = wbase + (size_t)j * FGS_WAVE + lane < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t below = __popcll(peers & lt_mask);
+        const uint32_t before = valid ? s_cnt[wave][d] : 0u;
+        rank[j] = before + below;
+        if (valid && below == 0) s_cnt[wave][d] = before + __popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t c[kWaves], tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            c[w] = s_cnt[w][t];
+            tot += c[w];
+        }
+        uint32_t all;
+        uint32_t run = block_exclusive_scan(tot, s_wave, &all);
+        s_start[t] = run;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            s_cnt[w][t] = run;
+            run += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        if (wbase + (size_t)j * FGS_WAVE + lane < n) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
+            s_keys[s_cnt[wave][d] + rank[j]] = key[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
+    for (uint32_t p = t; p < live; p += kThreads) {
+        const uint64_t k = s_keys[p];
+        const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
+        out[s_glob[d] + (p - s_start[d])] = k;
     }
 }
 
