@@ -897,25 +897,27 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                                                      float* __restrict__ dL_dopacity,
                                                      float* __restrict__ dL_dcolors)
 {
-    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // grid-stride: with a device count the grid is sized for the capacity
     const size_t live = gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
-    if (s >= live) return;
-    const uint32_t n = gg.tiles_s[s];
-    if (n > 0u && n <= kGatherSolo) {
-        const uint64_t key = gg.vkeys[s];
-        const uint32_t g = gg.batch_P ? batch_elem(key, gg.batch_P) : (uint32_t)key;
-        const uint32_t off = gg.offs[s];
-        float t[kGradTerms] = {};
-        bool any = false;
-        for (uint32_t k = 0; k < n; ++k) {
-            const size_t p = (size_t)off + k;
-            if (p < cap && mark[p]) {
-                any = true;
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < live;
+         s += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t n = gg.tiles_s[s];
+        if (n > 0u && n <= kGatherSolo) {
+            const uint64_t key = gg.vkeys[s];
+            const uint32_t g = gg.batch_P ? batch_elem(key, gg.batch_P) : (uint32_t)key;
+            const uint32_t off = gg.offs[s];
+            float t[kGradTerms] = {};
+            bool any = false;
+            for (uint32_t k = 0; k < n; ++k) {
+                const size_t p = (size_t)off + k;
+                if (p < cap && mark[p]) {
+                    any = true;
 #pragma unroll
-                for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
+                    for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
+                }
             }
+            if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
         }
-        if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
     }
 }
 
@@ -940,7 +942,9 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
     if (e != hipSuccess) return e;
     launch(a_out(a, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors));
     if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
-    hipLaunchKernelGGL(k_grad_gather, dim3((unsigned)((gg.n + 255) / 256)), dim3(256), 0, s, gg, a.part,
+    size_t blocks = (gg.n + 255) / 256;
+    if (gg.n_dev && blocks > kStrideBlocks) blocks = kStrideBlocks;
+    hipLaunchKernelGGL(k_grad_gather, dim3((unsigned)blocks), dim3(256), 0, s, gg, a.part,
                        a.mark, a.part_cap, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
     return hipGetLastError();
 }

@@ -174,6 +174,9 @@ hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const u
 hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T, hipStream_t s);
 
 // batched binning (depth key (v << 58) | (depth_bits << 26) | i)
+// Grid of the grid-stride kernels that run over a device-side count: enough
+// 256-lane blocks to fill the chip several times over.
+constexpr size_t kStrideBlocks = 4096;
 constexpr int kBatchViewShift = 58;
 constexpr int kBatchIdxBits = 26;
 constexpr uint64_t kBatchIdxMask = (1ull << kBatchIdxBits) - 1;

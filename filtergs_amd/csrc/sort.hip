@@ -644,11 +644,6 @@ hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, c
 // identifyTileRanges over the batch: a range runs while (tile, view) stays
 // the same; it is stored at the view's tile_base + tile.  Only the first
 // min(cap, *K) sorted pairs exist.
-__device__ __forceinline__ uint64_t tile_view(uint64_t key, uint32_t P)
-{
-    return ((key >> 32) << 32) | ((uint32_t)key / P);
-}
-
 __global__ __launch_bounds__(256) void k_ranges_batch(const uint64_t* __restrict__ sorted, size_t cap,
                                                       const unsigned long long* __restrict__ K,
                                                       uint32_t P, const BatchView* __restrict__ views,
@@ -658,10 +653,17 @@ __global__ __launch_bounds__(256) void k_ranges_batch(const uint64_t* __restrict
     const size_t n = live_count(cap, K);
     if (k >= n) return;
     const uint64_t key = sorted[k];
-    const uint64_t tv = tile_view(key, P);
-    const uint32_t gt = views[(uint32_t)tv].tile_base + (uint32_t)(key >> 32);
-    if (k == 0 || tile_view(sorted[k - 1], P) != tv) ranges[gt].x = (uint32_t)k;
-    if (k == n - 1 || tile_view(sorted[k + 1], P) != tv) ranges[gt].y = (uint32_t)(k + 1);
+    // one division: a neighbour is in the same (tile, view) run when it has
+    // the same tile and its element lies in this view's [v P, (v + 1) P)
+    const uint32_t v = (uint32_t)key / P;
+    const uint32_t lo = v * P;
+    const uint32_t tile = (uint32_t)(key >> 32);
+    const auto same = [&](uint64_t o) {
+        return (uint32_t)(o >> 32) == tile && (uint32_t)o - lo < P;
+    };
+    const uint32_t gt = views[v].tile_base + tile;
+    if (k == 0 || !same(sorted[k - 1])) ranges[gt].x = (uint32_t)k;
+    if (k == n - 1 || !same(sorted[k + 1])) ranges[gt].y = (uint32_t)(k + 1);
 }
 
 hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, const unsigned long long* K, int P,
