@@ -735,11 +735,23 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd_batch(PreArgs a,
         float m[3], c3[6];
         gaussian_inputs(a, i, m, c3);
         float dop = 0.f;
-        bool any = false;
-        for (int v = 0; v < V; ++v) {
+        // the views this Gaussian is visible in, gathered first (independent
+        // loads), then only those are walked: each lane its own views
+        uint64_t vis = 0ull;
+        int v0 = 0;
+        for (; v0 + 8 <= V; v0 += 8) {
+            int rr[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rr[u] = in.radii[(size_t)(v0 + u) * a.P + i];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vis |= (uint64_t)(rr[u] > 0) << (v0 + u);
+        }
+        for (; v0 < V; ++v0) vis |= (uint64_t)(in.radii[(size_t)v0 * a.P + i] > 0) << v0;
+        const bool any = vis != 0ull;
+        while (vis) {
+            const int v = __builtin_ctzll(vis);
+            vis &= vis - 1ull;
             const size_t e = (size_t)v * a.P + i;
-            if (in.radii[e] <= 0) continue;
-            any = true;
             PreArgs av = a;
             bind_view(av, views[v]);
             preprocess_bwd_view(av, in, e, m[0], m[1], m[2], c3, shr, dm, dcov, s_sh, t);
