@@ -140,6 +140,9 @@ def test_batch_view_counts(V):
                                                     center=(4.0 * np.cos(v), 4.0 * np.sin(v), 0.5 * (v % 3)),
                                                     fovx_deg=50.0 + v)
             for v in range(V)]
+    if V > 1:
+        # a view that sees nothing: every (view, Gaussian) key of it is culled
+        cams[V // 2] = make_camera(80, 60, center=(0.0, -5.0, 0.5), target=(0.0, -10.0, 0.5), fovx_deg=50.0)
     P = g["means3D"].shape[0]
     rng = np.random.default_rng(V)
     dLs = [rng.normal(size=(3, c.height, c.width)).astype(np.float32) for c in cams]
@@ -148,6 +151,8 @@ def test_batch_view_counts(V):
     br.bind(_t(g["means3D"]), _t(g["opacities"]), _t(g["scales"]), _t(g["rotations"]), shs=_t(g["shs"]))
     K = br.forward()
     assert K == sum(int(s[0][0]) for s in singles)
+    if V > 1:
+        assert int(singles[V // 2][0][0]) == 0 and not br.radii[V // 2].any()
     for v, (out, _) in enumerate(singles):
         assert torch.equal(br.radii[v], out[5]), v
         assert torch.equal(br.view_color(v), out[1]), v
