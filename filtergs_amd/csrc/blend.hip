@@ -153,6 +153,24 @@ __device__ __forceinline__ float wave_total(float x)
     return sum_xor32(sum_xor16(x));
 }
 
+// Wave maximum of a non-negative value (every lane receives it).
+__device__ __forceinline__ float wave_maxf(float x)
+{
+    x = fmaxf(x, dpp<0x128>(x));
+    x = fmaxf(x, dpp<0x124>(x));
+    x = fmaxf(x, dpp<0x122>(x));
+    x = fmaxf(x, dpp<0x121>(x));
+    {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+        x = fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+    }
+    {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+        x = fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+    }
+    return x;
+}
+
 // Exponent of the Gaussian in log2 units at offset (dx, dy) from its mean,
 // from the staged terms (a', b') = g0.zw and c' = g1.x.  Shared by the
 // forward and the backward so that both take the same alpha decisions.
@@ -240,6 +258,10 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     // the end), used / unused counts and the histogram in LDS
     double st_sum = 0.0;
     int r_end = rounds;        // rounds staged (the tile stops early when saturated)
+    // walk choice of the wave: dense lists (many lanes blend per entry) use
+    // the wave-uniform walk with a DPP maximum, sparse ones the divergent
+    // walk with per-lane LDS maxima; decided from the previous round
+    bool dense_walk = true;
     __shared__ uint32_t s_st[2 + kKpcBins];   // used in tile, unused, bins
     if (STATS) {
         if (tid < 2 + kKpcBins) s_st[tid] = 0u;
@@ -299,11 +321,14 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                 done = done || stop;
                 const float w = blend ? alpha * T : 0.f;
                 const uint64_t bl = __ballot(blend);
-                if (bl) {   // wave-uniform
+                const bool many = __popcll(bl) > 12;   // wave-uniform
+                if (bl) {
                     const float ws = wave_total(w);
+                    const float wm = many ? wave_maxf(w) : 0.f;
                     if (lane_id() == 0) {
                         atomicAdd(&sp->kv, ws);
                         atomicAdd(&sp->kc, (uint32_t)__popcll(bl));
+                        if (many) sp->w[wave] = __float_as_uint(wm);
                     }
                 }
                 if (blend) {
@@ -314,45 +339,97 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                         wmax = w;
                         max_id = (int)__float_as_uint(col.w);
                     }
-                    atomicMax(&sp->w[wave], __float_as_uint(w));
+                    if (!many) atomicMax(&sp->w[wave], __float_as_uint(w));
                     T = test_T;
                     last_slot = (int)(sp - s_slot);
                     last = (uint32_t)(r * BATCH + last_slot + 1);
                 }
             }
         } else {
-            for (int i = 0; !done && i < cnt; ++i) {
-                FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
-                next = s_list[wave][min(i + 1, BATCH - 1)];
-                const float4 g0 = sp->g0;
-                const float2 g1 = sp->g1;
-                const float4 col = sp->col;
-                const float dx = g0.x - pfx, dy = g0.y - pfy;
-                const float p2 = exponent2(g0, g1, dx, dy);
-                const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
-                const float test_T = T * (1.0f - alpha);
-                const bool pass = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
-                const bool stop = pass && test_T < 0.0001f;
-                done = done || stop;
-                if (pass && !stop) {
-                    const float w = alpha * T;
-                    C0 = fmaf(col.x, w, C0);
-                    C1 = fmaf(col.y, w, C1);
-                    C2 = fmaf(col.z, w, C2);
-                    if (w > wmax) {
-                        wmax = w;
-                        max_id = (int)__float_as_uint(col.w);
+            // Dense walk: wave-uniform, the per-strip maximum weight of the
+            // slot is a DPP wave maximum stored by one lane (each slot is
+            // visited once by each wave) instead of an LDS atomic from every
+            // blending lane, which serialises when most lanes blend.  Sparse
+            // walk: divergent, per-lane LDS atomics.
+            uint32_t act_sum = 0, act_iters = 0;
+            if (dense_walk) {
+                for (int i = 0; i < cnt; ++i) {
+                    if (__all(done)) break;
+                    FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
+                    next = s_list[wave][min(i + 1, BATCH - 1)];
+                    const float4 g0 = sp->g0;
+                    const float2 g1 = sp->g1;
+                    const float4 col = sp->col;
+                    const float dx = g0.x - pfx, dy = g0.y - pfy;
+                    const float p2 = exponent2(g0, g1, dx, dy);
+                    const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+                    const float test_T = T * (1.0f - alpha);
+                    const bool pass = !done && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                    const bool stop = pass && test_T < 0.0001f;
+                    const bool blend = pass && !stop;
+                    done = done || stop;
+                    const float w = blend ? alpha * T : 0.f;
+                    const uint64_t blm = __ballot(blend);
+                    act_sum += (uint32_t)__popcll(blm);
+                    act_iters += 1u;
+                    if (blm) {   // wave-uniform
+                        const float wm = wave_maxf(w);
+                        if (lane_id() == 0) sp->w[wave] = __float_as_uint(wm);
+                        if (kv_sum) {
+                            if (blend) {
+                                atomicAdd(&sp->kv, w);
+                                atomicAdd(&sp->kc, 1u);
+                            }
+                        }
                     }
-                    atomicMax(&sp->w[wave], __float_as_uint(w));
-                    if (kv_sum) {
-                        atomicAdd(&sp->kv, w);
-                        atomicAdd(&sp->kc, 1u);
+                    if (blend) {
+                        C0 = fmaf(col.x, w, C0);
+                        C1 = fmaf(col.y, w, C1);
+                        C2 = fmaf(col.z, w, C2);
+                        if (w > wmax) {
+                            wmax = w;
+                            max_id = (int)__float_as_uint(col.w);
+                        }
+                        T = test_T;
+                        last_slot = (int)(sp - s_slot);
+                        last = (uint32_t)(r * BATCH + last_slot + 1);
                     }
-                    T = test_T;
-                    last_slot = (int)(sp - s_slot);
-                    last = (uint32_t)(r * BATCH + last_slot + 1);
+                }
+            } else {
+                for (int i = 0; !done && i < cnt; ++i) {
+                    FwdSlot* const sp = reinterpret_cast<FwdSlot*>(slots + next);
+                    next = s_list[wave][min(i + 1, BATCH - 1)];
+                    const float4 g0 = sp->g0;
+                    const float2 g1 = sp->g1;
+                    const float4 col = sp->col;
+                    const float dx = g0.x - pfx, dy = g0.y - pfy;
+                    const float p2 = exponent2(g0, g1, dx, dy);
+                    const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+                    const float test_T = T * (1.0f - alpha);
+                    const bool pass = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                    const bool stop = pass && test_T < 0.0001f;
+                    done = done || stop;
+                    if (pass && !stop) {
+                        const float w = alpha * T;
+                        C0 = fmaf(col.x, w, C0);
+                        C1 = fmaf(col.y, w, C1);
+                        C2 = fmaf(col.z, w, C2);
+                        if (w > wmax) {
+                            wmax = w;
+                            max_id = (int)__float_as_uint(col.w);
+                        }
+                        atomicMax(&sp->w[wave], __float_as_uint(w));
+                        if (kv_sum) {
+                            atomicAdd(&sp->kv, w);
+                            atomicAdd(&sp->kc, 1u);
+                        }
+                        T = test_T;
+                        last_slot = (int)(sp - s_slot);
+                        last = (uint32_t)(r * BATCH + last_slot + 1);
+                    }
                 }
             }
+            if (dense_walk && act_iters > 0u) dense_walk = act_sum > 24u * act_iters;
         }
         __syncthreads();
         uint32_t used = 0u, g = 0u;
