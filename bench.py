@@ -61,7 +61,7 @@ def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
     params = P * (44 + 12 * M)
     grads_out = P * (92 + 12 * M)
     sorted_keys = P if V == 1 else Pv
-    return {
+    out = {
         "preprocess": params + V * P * 8 + Pv * 40,
         "depth_sort": (4 if V == 1 else 5) * 24 * sorted_keys,
         "tile_offsets": 8 * sorted_keys,
@@ -72,6 +72,11 @@ def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
         "blend_bwd": 40 * K + 20 * npix + 36 * Pv,
         "preprocess_bwd": (Pv * (44 + 12 * M + 36) if V == 1 else params + 36 * Pv) + grads_out,
     }
+    # the blend kernels alone (their stages also hold the pair-mark clear and
+    # the gradient gather): the same algorithmic bytes
+    out["blend_fwd_kernel"] = out["blend_fwd"]
+    out["blend_bwd_kernel"] = out["blend_bwd"]
+    return out
 
 
 def filtergs_views(tree, cams, warmup):
@@ -469,8 +474,10 @@ def main():
     T = int(br.sizes.tiles)
     nbb = stage_bytes(P, Pv, K, T, npix, M, (T.bit_length() + 7) // 8, V=args.views)
     stb = stages_of(profb, nbb)
-    dominant = max(stb, key=lambda k: stb[k]["ms"])
-    achieved = nbb[dominant] / stb[dominant]["ms"] / 1e6
+    dominant = max((k for k in stb if not k.endswith("_kernel")), key=lambda k: stb[k]["ms"])
+    # the kernel's own launch time where the stage holds more than the kernel
+    kern = stb.get(dominant + "_kernel", stb[dominant])
+    achieved = nbb[dominant] / kern["ms"] / 1e6
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
     if os.path.exists(tpath):
@@ -515,7 +522,8 @@ def main():
                                  "Mpix/s": round(1920 * 1080 / ms1 / 1000.0, 3),
                                  "visible": Pv1, "kv_pairs": K1, "stages": st1,
                                  "hbm_stages": hbm_frac(st1, nb1)},
-        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1),
+        "roofline": {"kernel": dominant, "launch_ms": kern["ms"], "bound": "hbm",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }

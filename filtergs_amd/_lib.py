@@ -252,7 +252,7 @@ def require_gpu(*tensors):
 
 
 STAGES = ("preprocess", "depth_sort", "tile_offsets", "duplicate", "tile_sort", "ranges",
-          "blend_fwd", "blend_bwd", "preprocess_bwd")
+          "blend_fwd", "blend_bwd", "preprocess_bwd", "blend_fwd_kernel", "blend_bwd_kernel")
 
 
 def profile_enable(on=True):

@@ -24,8 +24,11 @@ thread_local std::string g_last_error;
 int g_shape[2] = { 256, 128 };
 
 // Optional per-stage timing with HIP events recorded on the launch stream.
+// The last two time the blend kernels alone inside their stages (the
+// backward stage also clears the pair marks and runs the gradient gather).
 enum Stage { ST_PREPROCESS, ST_DEPTH_SORT, ST_OFFSETS, ST_DUPLICATE, ST_TILE_SORT, ST_RANGES,
-             ST_BLEND_FWD, ST_BLEND_BWD, ST_PREPROCESS_BWD, ST_COUNT };
+             ST_BLEND_FWD, ST_BLEND_BWD, ST_PREPROCESS_BWD, ST_BLEND_FWD_KERNEL, ST_BLEND_BWD_KERNEL,
+             ST_COUNT };
 bool g_prof = false;
 hipEvent_t g_ev[ST_COUNT][2];
 bool g_ev_ready = false;
@@ -47,21 +50,25 @@ void prof_collect()
     }
 }
 
+void prof_begin(int k, hipStream_t st)
+{
+    if (!g_prof) return;
+    if (g_pending[k]) prof_collect();
+    (void)hipEventRecord(g_ev[k][0], st);
+}
+
+void prof_end(int k, hipStream_t st)
+{
+    if (!g_prof) return;
+    (void)hipEventRecord(g_ev[k][1], st);
+    g_pending[k] = true;
+}
+
 struct ProfScope {
     int k;
     hipStream_t st;
-    ProfScope(int stage, hipStream_t s) : k(stage), st(s)
-    {
-        if (!g_prof) return;
-        if (g_pending[k]) prof_collect();
-        (void)hipEventRecord(g_ev[k][0], st);
-    }
-    ~ProfScope()
-    {
-        if (!g_prof) return;
-        (void)hipEventRecord(g_ev[k][1], st);
-        g_pending[k] = true;
-    }
+    ProfScope(int stage, hipStream_t s) : k(stage), st(s) { prof_begin(k, st); }
+    ~ProfScope() { prof_end(k, st); }
 };
 
 int fail(int code, const char* fmt, const char* detail = "")
@@ -367,6 +374,7 @@ int fgs_profile_read(double* ms, uint64_t* calls, int32_t n, int32_t reset)
 }
 
 const char* fgs_last_error(void) { return g_last_error.c_str(); }
+
 int fgs_abi_version(void) { return FGS_ABI_VERSION; }
 float fgs_ogfs_log_lambda(float eps_scale, float eps_max) { return ogfs_log_lambda(eps_scale, eps_max); }
 
@@ -1418,3 +1426,12 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
 }
 
 }  // extern "C"
+
+namespace fgs {
+void prof_kernel(bool backward, bool begin, hipStream_t s)
+{
+    const int k = backward ? ST_BLEND_BWD_KERNEL : ST_BLEND_FWD_KERNEL;
+    if (begin) prof_begin(k, s);
+    else prof_end(k, s);
+}
+}  // namespace fgs

@@ -591,6 +591,7 @@ hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* view
                                   float* point_weight, hipStream_t s)
 {
     if (T_total == 0) return hipSuccess;
+    prof_kernel(false, true, s);
     if (common.batch == 128)
         hipLaunchKernelGGL(k_blend_fwd_batch<128>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
                            final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
@@ -599,6 +600,7 @@ hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* view
         hipLaunchKernelGGL(k_blend_fwd_batch<256>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
                            final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
                            point_weight);
+    prof_kernel(false, false, s);
     return hipGetLastError();
 }
 
@@ -611,6 +613,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 #define FGS_FWD(BATCH, STATS)                                                                            \
     hipLaunchKernelGGL((k_blend_fwd<BATCH, STATS>), dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib, \
                        vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt)
+    prof_kernel(false, true, s);
     if (a.stats) {
         if (a.batch == 128) FGS_FWD(128, true);
         else FGS_FWD(256, true);
@@ -619,6 +622,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
         else FGS_FWD(256, false);
     }
 #undef FGS_FWD
+    prof_kernel(false, false, s);
     return hipGetLastError();
 }
 
@@ -1017,7 +1021,9 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
 {
     hipError_t e = launch_fill_u32(a.mark, 0u, (a.part_cap + 3) / 4, s);
     if (e != hipSuccess) return e;
+    prof_kernel(true, true, s);
     launch(a_out(a, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors));
+    prof_kernel(true, false, s);
     if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
     size_t blocks = (gg.n + 255) / 256;
     if (gg.n_dev && blocks > kStrideBlocks) blocks = kStrideBlocks;

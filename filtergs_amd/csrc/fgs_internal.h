@@ -284,6 +284,9 @@ struct CounterArgs {
 };
 hipError_t launch_counter_update(const CounterArgs& a, hipStream_t s);
 
+// Optional timing of the blend kernels alone (a no-op unless profiling is on).
+void prof_kernel(bool backward, bool begin, hipStream_t s);
+
 // TensorTree.traverse (traverse.hip)
 constexpr int kTravGrid = 1024;    // blocks of the per-level count / scatter launches
 constexpr int kTravBlock = 256;
