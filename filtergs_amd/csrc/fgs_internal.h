@@ -209,7 +209,10 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
 // The per-Gaussian sums of the backward's per-pair partials, gathered in
 // depth order from the Gaussians' pre-sort pair segments (slot s: key
 // vkeys[s], tiles_s[s] pairs from offs[s]; batch_P > 0: keys of a batch).
-constexpr uint32_t kGatherSolo = 16;
+// Gaussians of up to kGatherSolo tiles take this path, larger ones add their
+// per-tile sums with global float atomics (at 16 the atomics of the close
+// orbit views cost more than the longer per-lane gather at 128 does).
+constexpr uint32_t kGatherSolo = 128;
 
 // RenderStats on the device (rasterizer.h:22-36, packed as
 // rasterize_points.cu:138-177): a head of 9 + 2 * kStatsMaxBins doubles
