@@ -953,8 +953,7 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
 // Gather of the per-pair gradients: slot s of the depth order sums the
 // marked entries of its Gaussian's pairs [offs[s], offs[s] + tiles_s[s]) and
 // stores the nine totals (plain stores: every Gaussian has one slot).
-// Only Gaussians of up to kGatherSolo tiles take this path; each is summed by
-// its own lane.
+// Only Gaussians of up to kGatherSolo tiles take this path.
 
 __device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* __restrict__ dL_dmean2D,
                                             float* __restrict__ dL_dconic, float* __restrict__ dL_dopacity,
@@ -971,6 +970,12 @@ __device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* _
     dL_dopacity[g] = t[8];
 }
 
+// Segments of up to kGatherLane pairs are summed by their own lane; longer
+// ones (up to kGatherSolo) by the whole wave, one at a time, lanes striding
+// over the pairs and the nine sums reduced across the wave, so a long
+// segment does not hold up its wave for a hundred dependent loads.
+constexpr uint32_t kGatherLane = 16;
+
 __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float* __restrict__ part,
                                                      const uint8_t* __restrict__ mark, size_t cap,
                                                      float* __restrict__ dL_dmean2D,
@@ -978,15 +983,18 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                                                      float* __restrict__ dL_dopacity,
                                                      float* __restrict__ dL_dcolors)
 {
-    // grid-stride: with a device count the grid is sized for the capacity
+    // grid-stride in whole blocks (the wave steps below need every lane):
+    // with a device count the grid is sized for the capacity
     const size_t live = gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
-    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < live;
-         s += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t n = gg.tiles_s[s];
-        if (n > 0u && n <= kGatherSolo) {
-            const uint64_t key = gg.vkeys[s];
-            const uint32_t g = gg.batch_P ? batch_elem(key, gg.batch_P) : (uint32_t)key;
-            const uint32_t off = gg.offs[s];
+    const int lane = lane_id();
+    for (size_t b0 = (size_t)blockIdx.x * blockDim.x; b0 < live; b0 += (size_t)gridDim.x * blockDim.x) {
+        const size_t s = b0 + threadIdx.x;
+        const bool valid = s < live;
+        const uint32_t n = valid ? gg.tiles_s[s] : 0u;
+        const bool gathered = n > 0u && n <= kGatherSolo;
+        const uint32_t g = gathered ? (gg.batch_P ? batch_elem(gg.vkeys[s], gg.batch_P) : (uint32_t)gg.vkeys[s]) : 0u;
+        const uint32_t off = gathered ? gg.offs[s] : 0u;
+        if (gathered && n <= kGatherLane) {
             float t[kGradTerms] = {};
             bool any = false;
             for (uint32_t k = 0; k < n; ++k) {
@@ -998,6 +1006,29 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                 }
             }
             if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+        }
+        uint64_t big = __ballot(gathered && n > kGatherLane);
+        while (big) {
+            const int src = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const uint32_t bn = __builtin_amdgcn_readlane(n, src);
+            const uint32_t boff = __builtin_amdgcn_readlane(off, src);
+            const uint32_t bg = __builtin_amdgcn_readlane(g, src);
+            float t[kGradTerms] = {};
+            bool mine = false;
+            for (uint32_t k = lane; k < bn; k += FGS_WAVE) {
+                const size_t p = (size_t)boff + k;
+                if (p < cap && mark[p]) {
+                    mine = true;
+#pragma unroll
+                    for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
+                }
+            }
+            if (__any(mine)) {   // wave-uniform
+#pragma unroll
+                for (int e = 0; e < kGradTerms; ++e) t[e] = wave_total(t[e]);
+                if (lane == src) store_grads(bg, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+            }
         }
     }
 }
