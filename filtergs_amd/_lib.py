@@ -193,6 +193,7 @@ _SIGS = {
     "fgs_profile_read": (c_int32, [c_void_p, c_void_p, c_int32, c_int32]),
     "fgs_set_work_shape": (c_int32, [c_int32, c_int32]),
     "fgs_get_work_shape": (c_int32, [c_void_p]),
+    "fgs_set_blend_walk": (c_int32, [c_int32]),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_abi_version": (c_int32, []),
     "fgs_ogfs_log_lambda": (c_float, [c_float, c_float]),
@@ -271,6 +272,12 @@ def profile_read(reset=True):
 def set_work_shape(fwd_batch=256, bwd_batch=128):
     """Blend work shape: Gaussians per LDS batch (forward 128/256, backward 64/128/256)."""
     call("fgs_set_work_shape", int(fwd_batch), int(bwd_batch))
+
+
+def set_blend_walk(mode=0):
+    """Forward blend walk: 0 adaptive (default), 1 wave-uniform, 2 divergent
+    (same images, different speed)."""
+    call("fgs_set_blend_walk", int(mode))
 
 
 def get_work_shape():

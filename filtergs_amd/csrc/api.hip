@@ -22,6 +22,8 @@ thread_local std::string g_last_error;
 // Blend work shape (Gaussians per LDS batch of the forward and of the
 // backward), process-wide; set by fgs_set_work_shape.
 int g_shape[2] = { 256, 128 };
+// Forward blend walk policy (fgs_set_blend_walk).
+int g_walk = 0;
 
 // Optional per-stage timing with HIP events recorded on the launch stream.
 // The last two time the blend kernels alone inside their stages (the
@@ -336,6 +338,13 @@ int fgs_set_work_shape(int32_t fwd_batch, int32_t bwd_batch)
     return FGS_OK;
 }
 
+int fgs_set_blend_walk(int32_t mode)
+{
+    if (mode < 0 || mode > 2) return fail(FGS_E_INVALID, "walk mode must be 0, 1 or 2%s");
+    g_walk = mode;
+    return FGS_OK;
+}
+
 int fgs_get_work_shape(int32_t* out2)
 {
     if (!out2) return fail(FGS_E_INVALID, "null output%s");
@@ -538,6 +547,7 @@ static int render_impl(const fgs_raster_settings* s, int P, const float* colors_
     b.conic_opacity = gm.conic_opacity;
     b.bg = s->bg;
     b.batch = g_shape[0];
+    b.walk = g_walk;
     if (K > 0) {
         if (!binning || !geom || !radii) return fail(FGS_E_INVALID, "null binning/geometry buffer%s");
         Binning bn = carve_binning(binning, K, nullptr);
@@ -1274,6 +1284,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     b.ranges = im.ranges;
     b.tile_vis = im.tile_vis;
     b.batch = g_shape[0];
+    b.walk = g_walk;
     b.id_stride = (uint32_t)P;
     if (n > 0) {
         if (!geom || !radii || (K_cap > 0 && !binning) || !out_point_weight)

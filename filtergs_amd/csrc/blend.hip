@@ -261,7 +261,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     // walk choice of the wave: dense lists (many lanes blend per entry) use
     // the wave-uniform walk with a DPP maximum, sparse ones the divergent
     // walk with per-lane LDS maxima; decided from the previous round
-    bool dense_walk = true;
+    bool dense_walk = a.walk != 2;
     __shared__ uint32_t s_st[2 + kKpcBins];   // used in tile, unused, bins
     if (STATS) {
         if (tid < 2 + kKpcBins) s_st[tid] = 0u;
@@ -429,7 +429,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                     }
                 }
             }
-            if (dense_walk && act_iters > 0u) dense_walk = act_sum > 24u * act_iters;
+            if (a.walk == 0 && dense_walk && act_iters > 0u) dense_walk = act_sum > 24u * act_iters;
         }
         __syncthreads();
         uint32_t used = 0u, g = 0u;

@@ -453,6 +453,12 @@ int fgs_profile_read(double* ms, uint64_t* calls, int32_t n, int32_t reset);
  * autotuner picks it.  fgs_get_work_shape fills {fwd_batch, bwd_batch}. */
 int fgs_set_work_shape(int32_t fwd_batch, int32_t bwd_batch);
 int fgs_get_work_shape(int32_t* out2);
+/* Forward blend walk, process-wide: 0 adaptive (default: each wave starts
+ * with the wave-uniform walk and its DPP maximum, and switches to the
+ * divergent walk with per-lane LDS maxima when few of its lanes blend per
+ * entry), 1 wave-uniform only, 2 divergent only.  All three render the same
+ * bits; only the speed differs.  No counterpart in the reference. */
+int fgs_set_blend_walk(int32_t mode);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */

@@ -180,6 +180,26 @@ def _grads_gpu(out, g, cam, dL, variant, grad_mode, path="shs", bg=(0.1, 0.2, 0.
 @pytest.mark.parametrize("variant,grad_mode", [(0, 0), (0, 1), (1, 0)])
 def test_backward_vs_oracle_f64(orc, variant, grad_mode):
     g, cam = small_scene(n=3000, W=256, H=192, seed=141 + variant, dist=6.0)
+    _check_backward(orc, g, cam, variant, grad_mode)
+
+
+def test_backward_large_gaussians_vs_oracle(orc):
+    """Gaussians of more than 16 tiles (their per-tile sums gathered by the
+    whole wave) and of more than 128 tiles (added with float atomics) next to
+    small ones."""
+    g, cam = small_scene(n=3000, W=256, H=192, seed=151, dist=6.0)
+    rng = np.random.default_rng(3)
+    mid = rng.choice(3000, 60, replace=False)
+    g["scales"][mid] = rng.uniform(0.15, 0.35, (60, 3)).astype(np.float32)
+    g["scales"][mid[:3]] = 1.2
+    g["opacities"][mid] = np.float32(0.3)
+    out = _check_backward(orc, g, cam, 0, 0)
+    tiles = _np(ext.geom_views(out[6], 3000)["tiles"]).astype(np.int64)
+    assert ((tiles > 16) & (tiles <= 128)).sum() >= 10, np.sort(tiles)[-70:]
+    assert (tiles > 128).sum() >= 1
+
+
+def _check_backward(orc, g, cam, variant, grad_mode):
     bg = (0.1, 0.2, 0.3)
     out = gpu_forward(g, cam, variant=variant, bg=bg, grad_mode=grad_mode)
     rng = np.random.default_rng(7)
@@ -222,6 +242,35 @@ def test_backward_vs_oracle_f64(orc, variant, grad_mode):
         err = np.abs(a - r).max()
         assert e_gpu <= max(1e-4, 3 * e_f32 + 1e-5), (name, e_gpu, e_f32)
         assert err <= 2e-3 * scale + 1e-9, (name, err, scale)
+    return out
+
+
+@pytest.mark.parametrize("stats", [False, True])
+def test_blend_walks_agree(stats):
+    """The forward's wave-uniform walk (DPP maximum), its divergent walk
+    (per-lane LDS maxima) and the adaptive choice between them render the
+    same bits: image, point ids, pixel weights, point weights (with the
+    statistics on too; their per-pair float sums meet in LDS atomics, whose
+    order is not fixed, so the statistics are checked against the oracle in
+    test_stats_vs_oracle instead).  Two scenes: large overlapping Gaussians
+    (most lanes blend at each entry) and a far view of many small ones (few
+    do), both with tiles of several staging rounds."""
+    from filtergs_amd import _lib
+    scenes = [small_scene(n=20000, W=320, H=240, seed=5, dist=4.0),
+              small_scene(n=60000, W=320, H=240, seed=6, dist=14.0)]
+    try:
+        for g, cam in scenes:
+            outs = []
+            for mode in (0, 1, 2):
+                _lib.set_blend_walk(mode)
+                outs.append(gpu_forward(g, cam, stats=stats))
+            ranges = _np(ext.image_ranges(outs[0][8], cam.width, cam.height)).astype(np.int64)
+            assert (ranges[:, 1] - ranges[:, 0]).max() > 256   # several rounds in some tile
+            for o in outs[1:]:
+                for k in (1, 2, 3, 4, 5):
+                    assert torch.equal(o[k], outs[0][k]), k
+    finally:
+        _lib.set_blend_walk(0)
 
 
 def test_autograd_through_plugin_surface(orc):
