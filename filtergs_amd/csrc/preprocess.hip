@@ -435,10 +435,9 @@ struct ShReg {
 // dcov (dL/dcov3D) and the Gaussian's dL/dsh column s_out[f * kPitch + t].
 // (mx, my, mz) is the mean, c3 the 3D covariance, shr the SH row (first nu
 // floats); the per-view gradients of the blend are read at element e.
-__device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreBwdIn& in, size_t e,
-                                                    float mx, float my, float mz, const float* c3,
-                                                    const float* shr, float* dm, float* dcov,
-                                                    float* s_out, int t)
+__device__ __forceinline__ void bwd_view_geom(const PreArgs& a, const float* dconic, const float* dmean2D,
+                                              float mx, float my, float mz, const float* c3,
+                                              float* dmv, float* dcov)
 {
     const bool exact = a.grad_mode == 1;
     Cov2D cv;
@@ -456,7 +455,7 @@ __device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreB
         ca = fmaxf(ca, 0.3f);
         cc = fmaxf(cc, 0.3f);
     }
-    const float dcx = in.dL_dconic[3 * e], dcy = in.dL_dconic[3 * e + 1], dcz = in.dL_dconic[3 * e + 2];
+    const float dcx = dconic[0], dcy = dconic[1], dcz = dconic[2];
     const float denom = ca * cc - cb * cb;
     const float denom2inv = exact ? 1.0f / (denom * denom) : 1.0f / ((denom * denom) + 0.0000001f);
     float dL_da = 0.f, dL_db = 0.f, dL_dc = 0.f;
@@ -495,7 +494,6 @@ __device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreB
     const float dL_dtz = -a.fx * tz2 * dL_dJ00 - a.fy * tz2 * dL_dJ11 + (2.f * a.fx * cv.t[0]) * tz3 * dL_dJ02 +
                          (2.f * a.fy * cv.t[1]) * tz3 * dL_dJ12;
     const float* V = a.view;
-    float dmv[3];
     dmv[0] = V[0] * dL_dtx + V[1] * dL_dty + V[2] * dL_dtz;
     dmv[1] = V[4] * dL_dtx + V[5] * dL_dty + V[6] * dL_dtz;
     dmv[2] = V[8] * dL_dtx + V[9] * dL_dty + V[10] * dL_dtz;
@@ -507,23 +505,29 @@ __device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreB
     const float m_w = 1.0f / (hom[3] + 0.0000001f);
     const float mul1 = hom[0] * m_w * m_w;
     const float mul2 = hom[1] * m_w * m_w;
-    const float g2x = in.dL_dmean2D[3 * e], g2y = in.dL_dmean2D[3 * e + 1];
+    const float g2x = dmean2D[0], g2y = dmean2D[1];
     dmv[0] += (Pm[0] * m_w - Pm[3] * mul1) * g2x + (Pm[1] * m_w - Pm[3] * mul2) * g2y;
     dmv[1] += (Pm[4] * m_w - Pm[7] * mul1) * g2x + (Pm[5] * m_w - Pm[7] * mul2) * g2y;
     dmv[2] += (Pm[8] * m_w - Pm[11] * mul1) * g2x + (Pm[9] * m_w - Pm[11] * mul2) * g2y;
+}
 
-    // SH backward (backward.cu:20-139)
-    if (a.shs) {
+// SH backward (backward.cu:20-139) of one Gaussian in one view: dL/dsh into
+// the LDS column s_out[f * kPitch + t], the view direction's term into dmv.
+__device__ __forceinline__ void bwd_view_sh(const PreArgs& a, const float* campos, const float* dcolors,
+                                            float mx, float my, float mz, const float* shr,
+                                            float* dmv, float* s_out, int t)
+{
+    {
         const ShReg sh{ shr };
         const int deg = a.D;
         float d[3], dor[3];
-        view_dir(a.campos, mx, my, mz, d, dor);
+        view_dir(campos, mx, my, mz, d, dor);
         const float x = d[0], y = d[1], z = d[2];
         float dRGB[3];
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
             const float v = sh_eval(deg, sh, ch, x, y, z) + 0.5f;
-            dRGB[ch] = v < 0.f ? 0.f : in.dL_dcolors[3 * e + ch];
+            dRGB[ch] = v < 0.f ? 0.f : dcolors[ch];
         }
         float ddx[3] = { 0.f, 0.f, 0.f }, ddy[3] = { 0.f, 0.f, 0.f }, ddz[3] = { 0.f, 0.f, 0.f };
 #pragma unroll
@@ -585,6 +589,16 @@ __device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreB
         dmv[1] += (-dor[0] * dor[1] * ddir0 + (sum2 - dor[1] * dor[1]) * ddir1 - dor[2] * dor[1] * ddir2) * invsum32;
         dmv[2] += (-dor[0] * dor[2] * ddir0 - dor[1] * dor[2] * ddir1 + (sum2 - dor[2] * dor[2]) * ddir2) * invsum32;
     }
+}
+
+__device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreBwdIn& in, size_t e,
+                                                    float mx, float my, float mz, const float* c3,
+                                                    const float* shr, float* dm, float* dcov,
+                                                    float* s_out, int t)
+{
+    float dmv[3];
+    bwd_view_geom(a, in.dL_dconic + 3 * e, in.dL_dmean2D + 3 * e, mx, my, mz, c3, dmv, dcov);
+    if (a.shs) bwd_view_sh(a, a.campos, in.dL_dcolors + 3 * e, mx, my, mz, shr, dmv, s_out, t);
 #pragma unroll
     for (int k = 0; k < 3; ++k) dm[k] += dmv[k];
 }
@@ -712,56 +726,130 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd(PreArgs a, PreBw
     if (has_sh) bwd_store_sh(a, o, base, nrows, nu, s_sh, t);
 }
 
-// All views of a batch: per-view blend gradients at e = v * P + i are folded
-// into one set of parameter gradients (dL/dcov3D is summed over the views
-// before the scale / rotation backward, which is linear in it).  The
-// per-view dL/dopacity of the blend is summed into o.dL_dopacity.
-__global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd_batch(PreArgs a, const BatchView* views,
-                                                                      int V, PreBwdIn in,
-                                                                      const float* dL_dopacity_view,
-                                                                      PreBwdOut o)
+// Bit v of the result: Gaussian i is visible in view v (radius > 0).
+__device__ __forceinline__ uint64_t visible_views(const int* radii, int P, int V, int i)
+{
+    uint64_t vis = 0ull;
+    int v0 = 0;
+    for (; v0 + 8 <= V; v0 += 8) {
+        int rr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rr[u] = radii[(size_t)(v0 + u) * P + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vis |= (uint64_t)(rr[u] > 0) << (v0 + u);
+    }
+    for (; v0 < V; ++v0) vis |= (uint64_t)(radii[(size_t)v0 * P + i] > 0) << v0;
+    return vis;
+}
+
+// All views of a batch, in two passes over the Gaussians so that neither
+// holds both the SH row and the covariance chain in registers.  The SH pass
+// sums dL/dsh over the visible views and leaves the view-direction term of
+// dL/dmean3D in o.dL_dmeans3D; the geometry pass adds the cov2D and
+// projection terms to it, sums dL/dcov3D over the views before the scale /
+// rotation backward (linear in it) and sums the per-view dL/dopacity of the
+// blend into o.dL_dopacity.  Per-view gradients are read at e = v * P + i.
+// Both passes keep the views' constants in LDS and load the next visible
+// view's blend gradients before working the current one.  The SH pass is held
+// to 3 waves per SIMD (its LDS column of dL/dsh allows no more), the geometry
+// pass to 5 (89 VGPRs, no spills).
+__global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(3)))
+void k_preprocess_bwd_batch_sh(PreArgs a, const BatchView* views, int V, PreBwdIn in, PreBwdOut o)
 {
     __shared__ float s_sh[kMaxShFloats * kPitch];
+    __shared__ float s_cam[3 * kBatchMaxViews];
     const int t = threadIdx.x;
     const int base = blockIdx.x * kPreThreads;
     const int i = base + t;
     const int nrows = min(kPreThreads, a.P - base);
-    const bool has_sh = a.shs != nullptr;
     const int nu = (a.D + 1) * (a.D + 1) * 3;
+    for (int k = t; k < 3 * V; k += kPreThreads) s_cam[k] = views[k / 3].campos[k % 3];
     float shr[kMaxShFloats];
-    if (has_sh) bwd_stage_sh(a, base, nrows, nu, shr, s_sh, t);
+    bwd_stage_sh(a, base, nrows, nu, shr, s_sh, t);   // its barrier also publishes s_cam
     if (i < a.P) {
-        float dm[3] = { 0.f, 0.f, 0.f }, dcov[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
-        float m[3], c3[6];
-        gaussian_inputs(a, i, m, c3);
-        float dop = 0.f;
-        // the views this Gaussian is visible in, gathered first (independent
-        // loads), then only those are walked: each lane its own views
-        uint64_t vis = 0ull;
-        int v0 = 0;
-        for (; v0 + 8 <= V; v0 += 8) {
-            int rr[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) rr[u] = in.radii[(size_t)(v0 + u) * a.P + i];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) vis |= (uint64_t)(rr[u] > 0) << (v0 + u);
+        const float mx = a.means3D[3 * i], my = a.means3D[3 * i + 1], mz = a.means3D[3 * i + 2];
+        float dm[3] = { 0.f, 0.f, 0.f };
+        uint64_t vis = visible_views(in.radii, a.P, V, i);
+        int v = vis ? __builtin_ctzll(vis) : 0;
+        float gc[3] = { 0.f, 0.f, 0.f };
+        if (vis) {
+            const float* src = in.dL_dcolors + 3 * ((size_t)v * a.P + i);
+            gc[0] = src[0]; gc[1] = src[1]; gc[2] = src[2];
         }
-        for (; v0 < V; ++v0) vis |= (uint64_t)(in.radii[(size_t)v0 * a.P + i] > 0) << v0;
-        const bool any = vis != 0ull;
         while (vis) {
-            const int v = __builtin_ctzll(vis);
+            // Opaque to the optimiser: products of the SH row with constants
+            // are not hoisted out of the loop as a second set of 48 registers
+            // (168 VGPRs with spills in the loop, 136 without).
+#pragma unroll
+            for (int f = 0; f < kMaxShFloats; ++f) asm volatile("" : "+v"(shr[f]));
             vis &= vis - 1ull;
-            const size_t e = (size_t)v * a.P + i;
-            PreArgs av = a;
-            bind_view(av, views[v]);
-            preprocess_bwd_view(av, in, e, m[0], m[1], m[2], c3, shr, dm, dcov, s_sh, t);
-            dop += dL_dopacity_view[e];
+            const int vn = vis ? __builtin_ctzll(vis) : v;
+            const float* nsrc = in.dL_dcolors + 3 * ((size_t)vn * a.P + i);
+            const float gn[3] = { nsrc[0], nsrc[1], nsrc[2] };
+            bwd_view_sh(a, s_cam + 3 * v, gc, mx, my, mz, shr, dm, s_sh, t);
+            v = vn;
+            gc[0] = gn[0]; gc[1] = gn[1]; gc[2] = gn[2];
         }
-        preprocess_bwd_finish(a, o, i, any, dm, dcov);
-        if (any && a.grad_mode == 1 && a.variant == 0 && a.opacities[i] < 1e-6f) dop = 0.f;
-        o.dL_dopacity[i] = dop;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = dm[k];
     }
-    if (has_sh) bwd_store_sh(a, o, base, nrows, nu, s_sh, t);
+    bwd_store_sh(a, o, base, nrows, nu, s_sh, t);
+}
+
+__global__ __launch_bounds__(kPreThreads) __attribute__((amdgpu_waves_per_eu(5)))
+void k_preprocess_bwd_batch_geom(PreArgs a, const BatchView* views, int V, PreBwdIn in,
+                                 const float* dL_dopacity_view, PreBwdOut o)
+{
+    __shared__ float s_mat[kBatchMaxViews][32];   // view, then projection matrix
+    __shared__ float4 s_cam[kBatchMaxViews];      // tanx, tany, fx, fy
+    for (int k = threadIdx.x; k < 32 * V; k += kPreThreads)
+        s_mat[k >> 5][k & 31] = (k & 31) < 16 ? views[k >> 5].view[k & 15] : views[k >> 5].proj[k & 15];
+    for (int k = threadIdx.x; k < V; k += kPreThreads)
+        s_cam[k] = make_float4(views[k].tanx, views[k].tany, views[k].fx, views[k].fy);
+    __syncthreads();
+    const int i = blockIdx.x * kPreThreads + threadIdx.x;
+    if (i >= a.P) return;
+    float dm[3] = { 0.f, 0.f, 0.f }, dcov[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+    if (a.shs) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dm[k] = o.dL_dmeans3D[3 * i + k];
+    }
+    float m[3], c3[6];
+    gaussian_inputs(a, i, m, c3);
+    float dop = 0.f;
+    uint64_t vis = visible_views(in.radii, a.P, V, i);
+    const bool any = vis != 0ull;
+    int v = vis ? __builtin_ctzll(vis) : 0;
+    float gcur[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };   // dconic[3], dmean2D[2], dopacity
+    auto load6 = [&](int w, float* g) {
+        const size_t e = (size_t)w * a.P + i;
+        g[0] = in.dL_dconic[3 * e]; g[1] = in.dL_dconic[3 * e + 1]; g[2] = in.dL_dconic[3 * e + 2];
+        g[3] = in.dL_dmean2D[3 * e]; g[4] = in.dL_dmean2D[3 * e + 1];
+        g[5] = dL_dopacity_view[e];
+    };
+    if (vis) load6(v, gcur);
+    while (vis) {
+        vis &= vis - 1ull;
+        const int vn = vis ? __builtin_ctzll(vis) : v;
+        float gn[6];
+        load6(vn, gn);
+        PreArgs av = a;
+        av.view = s_mat[v];
+        av.proj = s_mat[v] + 16;
+        const float4 cam = s_cam[v];
+        av.tanx = cam.x; av.tany = cam.y; av.fx = cam.z; av.fy = cam.w;
+        float dmv[3];
+        bwd_view_geom(av, gcur, gcur + 3, m[0], m[1], m[2], c3, dmv, dcov);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) dm[k] += dmv[k];
+        dop += gcur[5];
+        v = vn;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gcur[k] = gn[k];
+    }
+    preprocess_bwd_finish(a, o, i, any, dm, dcov);
+    if (any && a.grad_mode == 1 && a.variant == 0 && a.opacities[i] < 1e-6f) dop = 0.f;
+    o.dL_dopacity[i] = dop;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
@@ -800,8 +888,10 @@ hipError_t launch_preprocess_bwd_batch(const PreArgs& a, const BatchView* views,
                                        const PreBwdOut& o, hipStream_t s)
 {
     if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_preprocess_bwd_batch, dim3((a.P + kPreThreads - 1) / kPreThreads),
-                       dim3(kPreThreads), 0, s, a, views, V, in, dL_dopacity_view, o);
+    const dim3 grid((a.P + kPreThreads - 1) / kPreThreads);
+    if (a.shs) hipLaunchKernelGGL(k_preprocess_bwd_batch_sh, grid, dim3(kPreThreads), 0, s, a, views, V, in, o);
+    hipLaunchKernelGGL(k_preprocess_bwd_batch_geom, grid, dim3(kPreThreads), 0, s, a, views, V, in,
+                       dL_dopacity_view, o);
     return hipGetLastError();
 }
 

@@ -128,12 +128,13 @@ def test_batch_graph_replay_and_capacity():
     assert not torch.equal(br.color, ref_color)
 
 
-@pytest.mark.parametrize("V", [1, 20])
+@pytest.mark.parametrize("V", [1, 20, 64])
 def test_batch_view_counts(V):
-    """One view and more views than a 4096-key sort chunk holds per wave
-    (the depth sort reads the preprocess's per-wave segments of 64 * V
-    keys): the batch equals the per-view renders, with a Gaussian count
-    that leaves the last wave partial."""
+    """One view, more views than a 4096-key sort chunk holds per wave (the
+    depth sort reads the preprocess's per-wave segments of 64 * V keys) and
+    the most views a batch takes (the backward's per-Gaussian masks of
+    visible views are 64 bits): the batch equals the per-view renders, with
+    a Gaussian count that leaves the last wave partial."""
     g, _ = small_scene(n=5037, seed=191, dist=5.0)
     base = _cams()
     cams = [base[v % 4] if v < 4 else make_camera(64 + 8 * v, 48 + 4 * v,
@@ -158,5 +159,7 @@ def test_batch_view_counts(V):
         assert torch.equal(br.view_color(v), out[1]), v
         assert torch.equal(br.view_pixels(br.point_id, v), out[2]), v
     grads = br.backward([_t(d) for d in dLs])
-    summed = sum(s[1][3] for s in singles)
-    assert _rel(grads["means3D"].reshape(summed.shape), summed) <= 1e-4
+    for k, idx in (("opacities", 2), ("means3D", 3), ("cov3D", 4), ("shs", 5), ("scales", 6),
+                   ("rotations", 7)):
+        summed = sum(s[1][idx] for s in singles)
+        assert _rel(grads[k].reshape(summed.shape), summed) <= 1e-4, k
