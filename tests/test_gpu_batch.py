@@ -27,19 +27,19 @@ def _cams():
             make_camera(130, 70, center=(0.5, 5.5, 0.2), fovx_deg=65.0)]
 
 
-def _single(g, cam, bg, dL, colors=None, grad_mode=0):
+def _single(g, cam, bg, dL, colors=None, grad_mode=0, deg=3):
     P = g["means3D"].shape[0]
     sh = _t(g["shs"]) if colors is None else torch.Tensor([])
     col = _t(colors) if colors is not None else torch.Tensor([])
     vm, pm, cp = _t(cam.world_view_transform), _t(cam.full_proj_transform), _t(cam.center)
     out = ext.rasterize_gaussians(_t(bg), _t(g["means3D"]), col, _t(g["opacities"].reshape(P, 1)),
                                   _t(g["scales"]), _t(g["rotations"]), 1.0, torch.Tensor([]), vm, pm,
-                                  cam.tanfovx, cam.tanfovy, cam.height, cam.width, sh, 3, cp, False,
+                                  cam.tanfovx, cam.tanfovy, cam.height, cam.width, sh, deg, cp, False,
                                   True, False, ogfs_enable_stats=False, grad_mode=grad_mode)
     K, color, pid, wpix, pw, radii, geom, binning, img, _ = out
     gr = ext.rasterize_gaussians_backward(_t(bg), _t(g["means3D"]), radii, col, _t(g["scales"]),
                                           _t(g["rotations"]), 1.0, torch.Tensor([]), vm, pm,
-                                          cam.tanfovx, cam.tanfovy, _t(dL), sh, 3, cp, geom, K,
+                                          cam.tanfovx, cam.tanfovy, _t(dL), sh, deg, cp, geom, K,
                                           binning, img, False, grad_mode=grad_mode,
                                           opacities=_t(g["opacities"].reshape(P)))
     return out, gr
@@ -161,5 +161,30 @@ def test_batch_view_counts(V):
     grads = br.backward([_t(d) for d in dLs])
     for k, idx in (("opacities", 2), ("means3D", 3), ("cov3D", 4), ("shs", 5), ("scales", 6),
                    ("rotations", 7)):
+        summed = sum(s[1][idx] for s in singles)
+        assert _rel(grads[k].reshape(summed.shape), summed) <= 1e-4, k
+
+
+@pytest.mark.parametrize("deg", [0, 1, 2])
+def test_batch_sh_degree_below_storage(deg):
+    """Rows of 16 SH coefficients rendered at a lower active degree: the
+    batched backward's SH pass reads the first (deg + 1)^2 coefficients and
+    gives the rest zero gradient, like the per-view path."""
+    g, _ = small_scene(n=9000, seed=201 + deg, dist=5.0)
+    cams = _cams()[:3]
+    P = g["means3D"].shape[0]
+    rng = np.random.default_rng(30 + deg)
+    dLs = [rng.normal(size=(3, c.height, c.width)).astype(np.float32) for c in cams]
+    singles = [_single(g, c, (0.1, 0.2, 0.3), dL, deg=deg) for c, dL in zip(cams, dLs)]
+    br = BatchRasterizer(cams, P, bg=(0.1, 0.2, 0.3), sh_degree=deg)
+    br.bind(_t(g["means3D"]), _t(g["opacities"]), _t(g["scales"]), _t(g["rotations"]), shs=_t(g["shs"]))
+    K = br.forward()
+    assert K == sum(int(s[0][0]) for s in singles)
+    for v, (out, _) in enumerate(singles):
+        assert torch.equal(br.view_color(v), out[1]), v
+    grads = br.backward([_t(d) for d in dLs])
+    nact = (deg + 1) ** 2
+    assert not grads["shs"].reshape(P, -1, 3)[:, nact:].any()
+    for k, idx in (("means3D", 3), ("cov3D", 4), ("shs", 5), ("opacities", 2)):
         summed = sum(s[1][idx] for s in singles)
         assert _rel(grads[k].reshape(summed.shape), summed) <= 1e-4, k
