@@ -1293,11 +1293,9 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         fgs::PreOut o{};
         o.radii = radii;
         o.means2D = gm.means2D;
-        o.depths = gm.depths;
         o.conic_opacity = gm.conic_opacity;
         o.rgb = gm.rgb;
-        o.tiles = gm.tiles;
-        o.recs = gm.recs;
+        o.recs = gm.recs;             // depths and tile counts are not stored per element
         o.vkeys = gm.vkeysA;
         o.error_flag = (int*)(counts + 2);
         o.seg_cnt = gm.seg_cnt;

@@ -411,7 +411,17 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
         PreRes r;
         preprocess_view(av, p[0], p[1], p[2], c3, opacity_in, sh, o.error_flag, r);
         const size_t e = (size_t)v * a.P + i;
-        write_pre(o, e, r, !a.colors_precomp);
+        // no depth or tile count: the depth travels in the key, the tile
+        // count in the record, and nothing later reads either array
+        o.radii[e] = r.radius;
+        o.recs[e] = r.rec;
+        o.means2D[e] = r.mean2d;
+        o.conic_opacity[e] = r.con;
+        if (!a.colors_precomp) {
+            o.rgb[3 * e + 0] = r.col[0];
+            o.rgb[3 * e + 1] = r.col[1];
+            o.rgb[3 * e + 2] = r.col[2];
+        }
         const bool vis = r.tiles != 0;
         const uint64_t m = __ballot(vis);
         if (vis)
