@@ -21,7 +21,11 @@
  *   blend bwd +
  *   preprocess bwd: FP64 central finite differences of the f64 forward
  *                  (grad_mode exact), and reference == exact where the two
- *                  coincide (dilate variant, no alpha clamp, mod = 1).
+ *                  coincide (dilate variant, no alpha clamp, mod = 1); the
+ *                  reference mode's cov2D epsilon by the ratio
+ *                  reference / exact = denom^2 / (denom^2 + 1e-7).  The
+ *                  reference mode's pass-through of the alpha clamp has no
+ *                  such check: parity unpinned.
  */
 
 #define CF(x) ((R)(x##f))
