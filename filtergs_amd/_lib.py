@@ -181,6 +181,7 @@ _SIGS = {
                                       c_void_p]),
     "fgs_batch_sizes_query": (c_int32, [c_void_p, c_int32, c_int32, c_void_p]),
     "fgs_batch_binning_bytes": (c_size_t, [c_int64]),
+    "fgs_batch_offsets": (c_int32, [c_void_p, c_int64, c_void_p]),
     "fgs_batch_plan": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "fgs_batch_forward": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -284,3 +285,26 @@ def get_work_shape():
     out = (ctypes.c_int32 * 2)()
     call("fgs_get_work_shape", out)
     return tuple(int(v) for v in out)
+
+
+def host_copy(args):
+    """Every tensor of ``args`` copied to host memory (other items as they
+    are): the arguments of a call, taken before the call can change them."""
+    return tuple(a.detach().cpu().clone() if isinstance(a, torch.Tensor) else a for a in args)
+
+
+def guarded_call(fn, args, kwargs, debug, dump_path, stage):
+    """``fn(*args, **kwargs)``.  With ``debug`` set, a host copy of ``args`` is
+    taken first; if the call raises, the copy is written to ``dump_path`` with
+    torch.save and the exception propagates.  This is the plugin surfaces'
+    debug convention: the snapshot files are named snapshot_fw.dump and
+    snapshot_bw.dump (antialias/diff_gaussian_rasterization*/__init__.py)."""
+    if not debug:
+        return fn(*args, **kwargs)
+    saved = host_copy(args)
+    try:
+        return fn(*args, **kwargs)
+    except Exception:
+        torch.save(saved, dump_path)
+        print(f"\n[filtergs_amd] the rasterizer {stage} raised; its arguments are saved in {dump_path}")
+        raise

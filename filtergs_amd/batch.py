@@ -224,6 +224,42 @@ class BatchRasterizer:
         b = int(self.pix_base[v])
         return t[b:b + h * w].view(h, w)
 
+    def view_sorted(self, v, K):
+        """View v's sorted kv list of the last forward, in the plain path's
+        form: (tile [n] int64, gaussian [n] int64, ranges [T_v, 2] int64 into
+        those arrays, (0, 0) for empty tiles).  The batch stores the pairs of
+        all views sorted by (view-local tile, view, depth, index); this
+        gathers view v's runs in tile order.  K: the pair count the forward
+        returned (at most the capacity)."""
+        offs = (ctypes.c_size_t * 4)()
+        call("fgs_batch_offsets", ctypes.byref(self.sizes), self.K_cap, offs)
+        K = min(int(K), self.K_cap)
+        T = int(self.sizes.tiles)
+        rg = self.image[int(offs[1]):int(offs[1]) + 8 * T].view(torch.int32).view(T, 2)
+        base = int(sum(((h + 15) // 16) * ((w + 15) // 16) for h, w in self.shapes[:v]))
+        h, w = self.shapes[v]
+        Tv = ((h + 15) // 16) * ((w + 15) // 16)
+        r = rg[base:base + Tv].to(torch.int64)
+        lens = (r[:, 1] - r[:, 0]).clamp_min(0)
+        starts = torch.cumsum(lens, 0) - lens
+        n = int(lens.sum())
+        pos = torch.arange(n, device=r.device) - torch.repeat_interleave(starts, lens) + \
+            torch.repeat_interleave(r[:, 0], lens)
+        pairs = self.binning[int(offs[0]):int(offs[0]) + 8 * K].view(torch.int64)[pos]
+        tile = (pairs >> 32) & 0xFFFFFFFF
+        elem = pairs & 0xFFFFFFFF
+        ranges = torch.stack([starts, starts + lens], 1) * (lens > 0)[:, None]
+        return tile, elem - v * self.P, ranges
+
+    def view_state(self, v):
+        """final_T and n_contrib of view v ([H, W] each) after a forward."""
+        offs = (ctypes.c_size_t * 4)()
+        call("fgs_batch_offsets", ctypes.byref(self.sizes), self.K_cap, offs)
+        npix = int(self.sizes.pixels)
+        fT = self.image[int(offs[2]):int(offs[2]) + 4 * npix].view(torch.float32)
+        nc = self.image[int(offs[3]):int(offs[3]) + 4 * npix].view(torch.int32)
+        return self.view_pixels(fT, v), self.view_pixels(nc, v)
+
     # -- graph capture and autotuning ---------------------------------------
     def capture(self, dL_dpix, out=None, warmup: int = 2):
         """Record forward + backward in a HIP graph.  The capacity must

@@ -6,7 +6,7 @@ import (LoG/render/renderer.py:1, FilterGS/filtergs.py:3-5):
 * ``diff_gaussian_rasterization_wodilate`` (and its ``._C``) ->
   ``filtergs_amd.rasterizer`` / ``filtergs_amd.ext``;
 * ``diff_gaussian_rasterization`` (and its ``._C``) -> ``filtergs_amd.dilate`` /
-  ``filtergs_amd.ext``;
+  ``filtergs_amd.dilate_ext`` (the 19-argument, 8-result graphdeco signatures);
 * ``FilterGS.cuda.compute_radius``, ``FilterGS.cuda.visible_flag``,
   ``FilterGS.cuda.gather_attributes`` -> ``filtergs_amd.lod``;
 * ``FilterGS`` / ``FilterGS.filtergs.FilterGS`` -> ``filtergs_amd.filtergs``
@@ -23,7 +23,7 @@ import importlib.util
 import sys
 import types
 
-from . import dilate, ext, filtergs, lod, rasterizer
+from . import dilate, dilate_ext, ext, filtergs, lod, rasterizer
 
 _FILTERGS_CUDA = ("compute_radius", "visible_flag", "gather_attributes")
 
@@ -42,7 +42,7 @@ def install(replace: bool = True) -> None:
     put("diff_gaussian_rasterization_wodilate", rasterizer)
     put("diff_gaussian_rasterization_wodilate._C", ext)
     put("diff_gaussian_rasterization", dilate)
-    put("diff_gaussian_rasterization._C", ext)
+    put("diff_gaussian_rasterization._C", dilate_ext)
 
     if "FilterGS" not in sys.modules and importlib.util.find_spec("FilterGS") is None:
         # no FilterGS package here: provide this package's FilterGS class
@@ -62,7 +62,7 @@ def install(replace: bool = True) -> None:
 
 def uninstall() -> None:
     """Drop the aliases that point at this package's modules."""
-    ours = {rasterizer, dilate, ext, lod, filtergs}
+    ours = {rasterizer, dilate, dilate_ext, ext, lod, filtergs}
     for name in list(sys.modules):
         mod = sys.modules[name]
         if mod in ours and not name.startswith("filtergs_amd"):

@@ -1236,6 +1236,20 @@ int fgs_batch_plan(const fgs_batch_view* views, int32_t V, void* plan, void* str
     return check(hipStreamSynchronize(st), "plan upload");
 }
 
+int fgs_batch_offsets(const fgs_batch_sizes* sz, int64_t K_cap, size_t* out4)
+{
+    if (!sz || !out4) return fail(FGS_E_INVALID, "null sizes or output%s");
+    if (K_cap < 0) return fail(FGS_E_INVALID, "negative capacity%s");
+    static char probe[256];
+    Binning bn = carve_binning(probe, K_cap, nullptr);
+    Image im = carve_image_n(probe, (size_t)sz->pixels, (size_t)sz->tiles, nullptr);
+    out4[0] = (size_t)((char*)bn.pairsA - probe);
+    out4[1] = (size_t)((char*)im.ranges - probe);
+    out4[2] = (size_t)((char*)im.final_T - probe);
+    out4[3] = (size_t)((char*)im.n_contrib - probe);
+    return FGS_OK;
+}
+
 static int batch_common(const fgs_raster_settings* s, const fgs_gaussians* g, const fgs_batch_sizes* sz)
 {
     if (!s || !g || !sz) return fail(FGS_E_INVALID, "null settings, gaussians or sizes%s");

@@ -13,9 +13,9 @@ from typing import NamedTuple
 import torch
 import torch.nn as nn
 
-from . import ext
-from . import ext as _C  # the reference module exposes its extension as _C
-from .rasterizer import GRAD_MODES, cpu_deep_copy_tuple
+from . import dilate_ext as _C
+from ._lib import guarded_call, host_copy as cpu_deep_copy_tuple  # noqa: F401 (module surface)
+from .rasterizer import GRAD_MODES
 
 
 def rasterize_gaussians(means3D, means2D, sh, colors_precomp, opacities, scales, rotations,
@@ -29,28 +29,21 @@ def _gm(rs):
 
 
 class _RasterizeGaussians(torch.autograd.Function):
+    """Forward and backward through ``_C`` with the dilate module's own
+    19- and 21-argument calls (``__init__.py:60-143``)."""
+
     @staticmethod
     def forward(ctx, means3D, means2D, sh, colors_precomp, opacities, scales, rotations,
                 cov3Ds_precomp, raster_settings):
-        args = (raster_settings.bg, means3D, colors_precomp, opacities, scales, rotations,
-                raster_settings.scale_modifier, cov3Ds_precomp, raster_settings.viewmatrix,
-                raster_settings.projmatrix, raster_settings.tanfovx, raster_settings.tanfovy,
-                raster_settings.image_height, raster_settings.image_width, sh,
-                raster_settings.sh_degree, raster_settings.campos, raster_settings.prefiltered,
-                True, raster_settings.debug)
-        kw = dict(ogfs_enable_stats=False, variant=1, grad_mode=_gm(raster_settings))
-        if raster_settings.debug:
-            cpu_args = cpu_deep_copy_tuple(args)
-            try:
-                out = ext.rasterize_gaussians(*args, **kw)
-            except Exception as ex:
-                torch.save(cpu_args, "snapshot_fw.dump")
-                print("\nAn error occured in forward. Please forward snapshot_fw.dump for debugging.")
-                raise ex
-        else:
-            out = ext.rasterize_gaussians(*args, **kw)
-        num_rendered, color, point_id, _wpix, point_weight, radii, geomBuffer, binningBuffer, imgBuffer, _ = out
-        ctx.raster_settings = raster_settings
+        rs = raster_settings
+        args = (rs.bg, means3D, colors_precomp, opacities, scales, rotations, rs.scale_modifier,
+                cov3Ds_precomp, rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy,
+                rs.image_height, rs.image_width, sh, rs.sh_degree, rs.campos, rs.prefiltered,
+                rs.debug)
+        (num_rendered, color, point_id, point_weight, radii, geomBuffer, binningBuffer,
+         imgBuffer) = guarded_call(_C.rasterize_gaussians, args, dict(grad_mode=_gm(rs)), rs.debug,
+                                   "snapshot_fw.dump", "forward")
+        ctx.raster_settings = rs
         ctx.num_rendered = num_rendered
         ctx.save_for_backward(colors_precomp, means3D, scales, rotations, cov3Ds_precomp, radii, sh,
                               geomBuffer, binningBuffer, imgBuffer, opacities)
@@ -65,17 +58,9 @@ class _RasterizeGaussians(torch.autograd.Function):
                 cov3Ds_precomp, rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy,
                 grad_out_color, sh, rs.sh_degree, rs.campos, geomBuffer, ctx.num_rendered,
                 binningBuffer, imgBuffer, rs.debug)
-        kw = dict(variant=1, use_filter=True, grad_mode=_gm(rs), opacities=opacities)
-        if rs.debug:
-            cpu_args = cpu_deep_copy_tuple(args)
-            try:
-                grads = ext.rasterize_gaussians_backward(*args, **kw)
-            except Exception as ex:
-                torch.save(cpu_args, "snapshot_bw.dump")
-                print("\nAn error occured in backward. Writing snapshot_bw.dump for debugging.\n")
-                raise ex
-        else:
-            grads = ext.rasterize_gaussians_backward(*args, **kw)
+        grads = guarded_call(_C.rasterize_gaussians_backward, args,
+                             dict(grad_mode=_gm(rs), opacities=opacities), rs.debug,
+                             "snapshot_bw.dump", "backward")
         (grad_means2D, grad_colors_precomp, grad_opacities, grad_means3D, grad_cov3Ds_precomp,
          grad_sh, grad_scales, grad_rotations) = grads
         if grad_colors_precomp.shape[-1] > 3:
@@ -110,7 +95,7 @@ class GaussianRasterizer(nn.Module):
     def markVisible(self, positions):
         with torch.no_grad():
             rs = self.raster_settings
-            return ext.mark_visible(positions, rs.viewmatrix, rs.projmatrix)
+            return _C.mark_visible(positions, rs.viewmatrix, rs.projmatrix)
 
     def forward(self, means3D, means2D, opacities, shs=None, colors_precomp=None, scales=None,
                 rotations=None, cov3D_precomp=None):

@@ -17,14 +17,10 @@ import torch
 import torch.nn as nn
 
 from . import ext
-from . import ext as _C  # the reference module exposes its extension as _C
+from . import ext as _C  # noqa: F401 (the module's extension is reachable as _C)
+from ._lib import guarded_call, host_copy as cpu_deep_copy_tuple  # noqa: F401 (module surface)
 
 GRAD_MODES = {"reference": 0, "exact": 1}
-
-
-def cpu_deep_copy_tuple(input_tuple):
-    return tuple(item.cpu().clone() if isinstance(item, torch.Tensor) else item
-                 for item in input_tuple)
 
 
 def _grad_mode(settings) -> int:
@@ -63,16 +59,8 @@ class _RasterizeGaussians(torch.autograd.Function):
                 use_filter, raster_settings.debug, ogfs_epsilon_scale, ogfs_epsilon_max,
                 ogfs_energy_floor, ogfs_radius_ratio_min, ogfs_enable_stats)
         gm = _grad_mode(raster_settings)
-        if raster_settings.debug:
-            cpu_args = cpu_deep_copy_tuple(args)
-            try:
-                out = ext.rasterize_gaussians(*args, grad_mode=gm)
-            except Exception as ex:
-                torch.save(cpu_args, "snapshot_fw.dump")
-                print("\nAn error occured in forward. Please forward snapshot_fw.dump for debugging.")
-                raise ex
-        else:
-            out = ext.rasterize_gaussians(*args, grad_mode=gm)
+        out = guarded_call(ext.rasterize_gaussians, args, dict(grad_mode=gm), raster_settings.debug,
+                           "snapshot_fw.dump", "forward")
         (num_rendered, color, point_id, point_weight_pixel, point_weight, radii, geomBuffer,
          binningBuffer, imgBuffer, stats) = out
         _RasterizeGaussians._last_stats = stats
@@ -97,16 +85,8 @@ class _RasterizeGaussians(torch.autograd.Function):
                 num_rendered, binningBuffer, imgBuffer, raster_settings.debug)
         kw = dict(variant=_RasterizeGaussians.variant, use_filter=ctx.use_filter,
                   grad_mode=ctx.grad_mode, opacities=opacities)
-        if raster_settings.debug:
-            cpu_args = cpu_deep_copy_tuple(args)
-            try:
-                grads = ext.rasterize_gaussians_backward(*args, **kw)
-            except Exception as ex:
-                torch.save(cpu_args, "snapshot_bw.dump")
-                print("\nAn error occured in backward. Writing snapshot_bw.dump for debugging.\n")
-                raise ex
-        else:
-            grads = ext.rasterize_gaussians_backward(*args, **kw)
+        grads = guarded_call(ext.rasterize_gaussians_backward, args, kw, raster_settings.debug,
+                             "snapshot_bw.dump", "backward")
         (grad_means2D, grad_colors_precomp, grad_opacities, grad_means3D, grad_cov3Ds_precomp,
          grad_sh, grad_scales, grad_rotations) = grads
         if grad_colors_precomp.shape[-1] > 3:

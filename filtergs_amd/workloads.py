@@ -65,6 +65,11 @@ def config(name: str, seed: int = 0):
         return gaussians(100_000, 5.0, 3, seed), camera_c0()
     if name == "c1":
         return gaussians(1_000_000, 20.0, 3, seed), camera_c1()
+    if name == "c2a":
+        # 4096 Gaussians in [-1, 1]^3 seen from distance 4: most of them on
+        # screen, 128x128 (the differentiation check)
+        return gaussians(4096, 1.0, 3, seed), make_camera(128, 128, center=(0.0, -4.0, 0.0),
+                                                          fovx_deg=60.0, fovy_deg=60.0)
     if name == "c2b":
         return gaussians(3_000_000, 20.0, 3, seed), camera_c1()
     raise KeyError(name)
@@ -201,3 +206,46 @@ def aerial_cameras(n_views: int = 8, extent: float = 2000.0, seed: int = 5,
                                 target=(tx, ty, float(terrain_height(np.array(tx), np.array(ty)))),
                                 fovx_deg=60.0, up=(0.0, 0.0, 1.0), znear=0.5, zfar=5000.0))
     return cams
+
+
+def ssim_window(size: int = 11, sigma: float = 1.5):
+    """Normalised 2D Gaussian window (outer product of the 1D one)."""
+    import torch
+    x = torch.arange(size, dtype=torch.float64) - size // 2
+    w = torch.exp(-x * x / (2.0 * sigma * sigma))
+    w = w / w.sum()
+    return (w[:, None] * w[None, :]).to(torch.float32)
+
+
+def l1_ssim_loss(render, target, window=None):
+    """The training objective of configuration 2b, LoG's photometric loss
+    (LoG/render/renderer.py:260-273 with LoG/render/loss.py:6-44):
+    0.8 * mean |render - target| + 0.2 * (1 - mean SSIM), SSIM over 11x11
+    Gaussian windows (sigma 1.5) on the valid region (no padding), per
+    channel, C1 = 0.01^2, C2 = 0.03^2.  render, target: [3, H, W] or
+    [B, 3, H, W] float tensors on one device."""
+    import torch
+    import torch.nn.functional as F
+    x = render if render.dim() == 4 else render[None]
+    y = target if target.dim() == 4 else target[None]
+    C = x.shape[1]
+    w = (window if window is not None else ssim_window()).to(device=x.device, dtype=x.dtype)
+    k = w.expand(C, 1, *w.shape)
+
+    def blur(t):
+        return F.conv2d(t, k, groups=C)
+
+    mx, my = blur(x), blur(y)
+    sxx = blur(x * x) - mx * mx
+    syy = blur(y * y) - my * my
+    sxy = blur(x * y) - mx * my
+    c1, c2 = 0.01 ** 2, 0.03 ** 2
+    ssim = ((2 * mx * my + c1) * (2 * sxy + c2)) / ((mx * mx + my * my + c1) * (sxx + syy + c2))
+    return 0.8 * (x - y).abs().mean() + 0.2 * (1.0 - ssim.mean())
+
+
+def band_camera(cam, y0: int, rows: int):
+    """The camera of rows [y0, y0 + rows) of `cam`'s image (same pose and
+    focal lengths, principal point moved up by y0)."""
+    return Camera(cam.width, rows, cam.fx, cam.fy, cam.cx, cam.cy - y0, cam.R, cam.T, cam.znear,
+                  cam.zfar)

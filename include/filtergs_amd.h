@@ -352,6 +352,16 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
                        float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
                        float* dL_dsh, float* dL_dscales, float* dL_drot, void* ws, void* stream);
 
+/* Byte offsets inside the batch buffers, for inspection (tests, tools):
+ * out4[0] the sorted kv pairs in the binning buffer of capacity K_cap (u64,
+ * (view-local tile << 32) | (v * P + i), min(K, K_cap) of them);
+ * out4[1] the tile ranges in the image buffer (uint2 per batch tile, views
+ * in plan order); out4[2] final_T and out4[3] n_contrib there (per pixel,
+ * views concatenated).  No counterpart in the reference: its per-view
+ * equivalents are the buffers RasterizeGaussiansCUDA returns
+ * (rasterize_points.cu:36-179). */
+int fgs_batch_offsets(const fgs_batch_sizes* sz, int64_t K_cap, size_t* out4);
+
 /* ---- per-point utilities ------------------------------------------------ */
 /* _C.mark_visible / markVisible (rasterize_points.cu:299-318,
  * rasterizer_impl.cu:97-109): present[i] = z_view > 0.2. */

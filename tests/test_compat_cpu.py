@@ -55,6 +55,49 @@ def test_dilate_names(shims):
         "cov3D_precomp"]
 
 
+def test_dilate_C_signatures(shims):
+    """The dilate wrapper's _C calls (antialias/diff_gaussian_rasterization/
+    __init__.py:60-92, :103-143): rasterize_gaussians takes 19 positional
+    arguments and returns 8 values, the backward 21; neither carries the
+    wodilate's use_filter / OGFS arguments."""
+    from diff_gaussian_rasterization import _C
+    from filtergs_amd import dilate_ext, ext
+    assert _C is dilate_ext
+    assert _C.rasterize_gaussians is not ext.rasterize_gaussians
+
+    def positional(fn):
+        return [p.name for p in inspect.signature(fn).parameters.values()
+                if p.kind == inspect.Parameter.POSITIONAL_OR_KEYWORD]
+    fwd = positional(_C.rasterize_gaussians)
+    assert len(fwd) == 19 and fwd[-2:] == ["prefiltered", "debug"]
+    assert "use_filter" not in fwd and not any(n.startswith("ogfs") for n in fwd)
+    bwd = positional(_C.rasterize_gaussians_backward)
+    assert len(bwd) == 21 and bwd[-1] == "debug"
+    assert positional(_C.mark_visible) == ["means3D", "viewmatrix", "projmatrix"]
+    # the wrapper passes exactly the reference's argument tuples: a fake _C
+    # records them and returns 8 forward values
+    from filtergs_amd import dilate
+    import torch
+    seen = {}
+
+    def fake_fwd(*args, **kw):
+        seen["fwd"] = (len(args), kw)
+        z = torch.zeros(1)
+        return 0, z, z, z, torch.zeros(1, dtype=torch.int32), z, z, z
+
+    orig = dilate._C.rasterize_gaussians
+    dilate._C.rasterize_gaussians = fake_fwd
+    try:
+        rs = dilate.GaussianRasterizationSettings(4, 4, 1.0, 1.0, torch.zeros(3), 1.0, torch.eye(4),
+                                                  torch.eye(4), 0, torch.zeros(3), False, False)
+        out = dilate.GaussianRasterizer(rs)(torch.zeros(1, 3), torch.zeros(1, 3), torch.ones(1, 1),
+                                            colors_precomp=torch.ones(1, 3), scales=torch.ones(1, 3),
+                                            rotations=torch.ones(1, 4))
+    finally:
+        dilate._C.rasterize_gaussians = orig
+    assert seen["fwd"][0] == 19 and len(out) == 4
+
+
 def test_filtergs_cuda_names(shims):
     from FilterGS.cuda.compute_radius import compute_radius_module, compute_visibility_radius
     from FilterGS.cuda.gather_attributes import gather_attributes

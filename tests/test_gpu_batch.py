@@ -188,3 +188,96 @@ def test_batch_sh_degree_below_storage(deg):
     for k, idx in (("means3D", 3), ("cov3D", 4), ("shs", 5), ("opacities", 2)):
         summed = sum(s[1][idx] for s in singles)
         assert _rel(grads[k].reshape(summed.shape), summed) <= 1e-4, k
+
+
+def _rel_np(a, r):
+    return float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-30))
+
+
+@pytest.mark.parametrize("grad_mode", [0, 1])
+@pytest.mark.parametrize("scale_modifier", [0.5, 1.7])
+def test_batch_vs_oracle(orc, grad_mode, scale_modifier):
+    """The batched kernels against the oracle directly, per view: radii, the
+    preprocess outputs, the sorted kv list and the tile ranges bit-exact
+    against the FP32 emulation; images against FP32 and FP64; n_contrib on
+    the pixels whose colour agrees; per-view dL/dmean2D, dL/dcolors and the
+    parameter gradients summed over the views against FP64.  Both gradient
+    modes and scale modifiers off 1, with a tenth of the Gaussians at
+    opacity in [0.99, 1) so the alpha clamp is met."""
+    from tests._scenes import orc_settings
+    g, _ = small_scene(n=12000, seed=231, dist=5.0)
+    P = g["means3D"].shape[0]
+    rng = np.random.default_rng(232)
+    hi = rng.choice(P, P // 10, replace=False)
+    g["opacities"][hi] = rng.uniform(0.99, 0.9999, hi.size).astype(np.float32)
+    cams = _cams()
+    bgs = [(0.1, 0.2, 0.3), (0.0, 0.0, 0.0), (1.0, 0.5, 0.25), (0.3, 0.3, 0.3)]
+    dLs = [rng.normal(size=(3, c.height, c.width)).astype(np.float32) for c in cams]
+    br = BatchRasterizer(cams, P, bg=bgs, grad_mode=grad_mode, scale_modifier=scale_modifier)
+    br.bind(_t(g["means3D"]), _t(g["opacities"]), _t(g["scales"]), _t(g["rotations"]), shs=_t(g["shs"]))
+    K = br.forward()
+    assert 0 < K <= br.K_cap
+    grads = br.backward([_t(d) for d in dLs])
+    gv = ext.geom_views(br.geom, len(cams) * P)
+    names = ("dL_dopacity", "dL_dmeans3D", "dL_dsh", "dL_dscales", "dL_drot")
+    keys = ("opacities", "means3D", "shs", "scales", "rotations")
+    sums = {m: {k: 0.0 for k in names} for m in ("f64", "f32")}
+    Ksum = 0
+    for v, (cam, bg, dL) in enumerate(zip(cams, bgs, dLs)):
+        s = orc_settings(orc, cam, sh_degree=3, bg=bg, grad_mode=grad_mode,
+                         scale_modifier=scale_modifier)
+        o = {}
+        for mode, dt in (("f32", np.float32), ("f64", np.float64)):
+            d = {k: g[k].astype(dt) for k in g}
+            f = orc.render(s, d["means3D"], d["opacities"], d["scales"], d["rotations"], d["shs"],
+                           mode=mode, nthreads=8, decision_hash=True)
+            b = orc.backward(s, f, dL.astype(dt), d["means3D"], d["opacities"], d["scales"],
+                             d["rotations"], d["shs"], mode=mode)
+            o[mode] = (f, b)
+            for k in names:
+                sums[mode][k] = sums[mode][k] + b[k].reshape(P, -1)
+        f32, f64 = o["f32"][0], o["f64"][0]
+        pre = f32["pre32"]
+        # integer stage and its FP32 inputs
+        assert np.array_equal(_np(br.radii[v]), pre["radii"]), v
+        vis = pre["radii"] > 0
+        sl = slice(v * P, (v + 1) * P)
+        for k in ("means2D", "conic_opacity"):
+            assert np.array_equal(_np(gv[k][sl])[vis].view(np.uint32), pre[k][vis].view(np.uint32)), (v, k)
+        tile, gid, ranges = br.view_sorted(v, K)
+        okeys = f32["keys"]
+        assert np.array_equal(_np(tile).astype(np.uint64), okeys >> np.uint64(32)), v
+        assert np.array_equal(_np(gid).astype(np.uint32), f32["vals"]), v
+        assert np.array_equal(_np(ranges).astype(np.uint32), f32["ranges"]), v
+        Ksum += len(f32["vals"])
+        # image
+        img = _np(br.view_color(v))
+        d32 = np.abs(img - f32["color"]).max(0)
+        agree = d32 <= 2e-5
+        assert (~agree).sum() <= 10 + 1e-5 * agree.size and d32.max() <= 0.01, v
+        same = agree & (f64["decision_hash"] == f32["decision_hash"])
+        assert np.abs(img - f64["color"]).max(0)[same].max() <= 1e-4, v
+        assert (~same).sum() <= 1e-3 * same.size, v
+        fT, nc = br.view_state(v)
+        assert (_np(nc).astype(np.uint32)[agree] != f32["n_contrib"][agree]).sum() <= 2, v
+        oT = f32["final_T"]
+        assert (np.abs(_np(fT) - oT)[agree] <= 1e-4 * oT[agree] + 1e-7).mean() > 0.999, v
+        # per-view screen-space gradients
+        b64 = o["f64"][1]
+        b32 = o["f32"][1]
+        for k, t in (("dL_dmean2D", br.dL_dmeans2D[v]), ("dL_dcolors", br.dL_dcolors[v])):
+            a = _np(t).astype(np.float64)
+            e, e32 = _rel_np(a, b64[k]), _rel_np(b32[k].astype(np.float64), b64[k])
+            assert e <= max(1e-4, 3 * e32 + 1e-5), (v, k, e, e32)
+    assert K == Ksum
+    for k, name in zip(keys, names):
+        a = _np(grads[k]).astype(np.float64).reshape(P, -1)
+        r = sums["f64"][name]
+        r32 = sums["f32"][name].astype(np.float64)
+        e, e32 = _rel_np(a, r), _rel_np(r32, r)
+        assert e <= max(1e-4, 3 * e32 + 1e-5), (k, e, e32)
+        assert np.abs(a - r).max() <= 2e-3 * np.abs(r).max() + 1e-9, k
+
+
+def _np(t):
+    return t.detach().cpu().numpy()

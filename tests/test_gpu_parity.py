@@ -35,7 +35,7 @@ def _cam_tensors(cam):
 
 
 def gpu_forward(g, cam, variant=0, path="shs", sh_degree=3, bg=(0.0, 0.0, 0.0), stats=False,
-                use_filter=True, grad_mode=0, **ogfs):
+                use_filter=True, grad_mode=0, scale_modifier=1.0, **ogfs):
     P = g["means3D"].shape[0]
     ct = _cam_tensors(cam)
     sh = _t(g["shs"]) if path == "shs" else torch.Tensor([])
@@ -46,8 +46,8 @@ def gpu_forward(g, cam, variant=0, path="shs", sh_degree=3, bg=(0.0, 0.0, 0.0), 
     scales = torch.Tensor([]) if path == "cov3D" else _t(g["scales"])
     rots = torch.Tensor([]) if path == "cov3D" else _t(g["rotations"])
     out = ext.rasterize_gaussians(
-        _t(bg), _t(g["means3D"]), colors, _t(g["opacities"].reshape(P, 1)), scales, rots, 1.0, cov,
-        ct["viewmatrix"], ct["projmatrix"], cam.tanfovx, cam.tanfovy, cam.height, cam.width, sh,
+        _t(bg), _t(g["means3D"]), colors, _t(g["opacities"].reshape(P, 1)), scales, rots,
+        scale_modifier, cov, ct["viewmatrix"], ct["projmatrix"], cam.tanfovx, cam.tanfovy, cam.height, cam.width, sh,
         sh_degree, ct["campos"], False, use_filter, False, ogfs_enable_stats=stats,
         variant=variant, grad_mode=grad_mode, **ogfs)
     return out
@@ -97,7 +97,8 @@ def _check_forward(orc, g, cam, variant, path="shs", bg=(0.1, 0.2, 0.3), f64=Tru
     agree = d32 <= 2e-5
     assert (~agree).sum() <= 10 + 1e-5 * agree.size, int((~agree).sum())
     assert d32.max() <= 0.01, float(d32.max())
-    res = dict(out=out, o32=o32, gpu_f32_flips=int((~agree).sum()))
+    side = _check_side_outputs(o32, out, img, cam, agree)
+    res = dict(out=out, o32=o32, gpu_f32_flips=int((~agree).sum()), side=side)
     if f64:
         o64 = orc.render(s, g["means3D"].astype(np.float64), g["opacities"].astype(np.float64),
                          mode="f64", nthreads=8, decision_hash=True,
@@ -111,6 +112,83 @@ def _check_forward(orc, g, cam, variant, path="shs", bg=(0.1, 0.2, 0.3), f64=Tru
         res["maxabs_f64"] = float(d[same].max())
         res["flips"] = flips
     return res
+
+
+def _pixel_weights(o32, W, px, py):
+    """T*alpha of every Gaussian that blends into pixel (px, py), walking its
+    tile's sorted list in FP64 with the blend rules of forward.cu:406-567
+    (skip power > 0 and alpha < 1/255, stop before T drops below 1e-4)."""
+    gx = (W + 15) // 16
+    t = (py // 16) * gx + px // 16
+    a, e = (int(x) for x in o32["ranges"][t])
+    m2 = o32["pre32"]["means2D"].astype(np.float64)
+    co = o32["pre32"]["conic_opacity"].astype(np.float64)
+    T, w = 1.0, {}
+    for k in range(a, e):
+        g = int(o32["vals"][k])
+        dx, dy = m2[g, 0] - px, m2[g, 1] - py
+        power = -0.5 * (co[g, 0] * dx * dx + co[g, 2] * dy * dy) - co[g, 1] * dx * dy
+        if power > 0:
+            continue
+        alpha = min(0.99, co[g, 3] * np.exp(power))
+        if alpha < 1 / 255:
+            continue
+        if T * (1 - alpha) < 1e-4:
+            break
+        w[g] = T * alpha
+        T *= 1 - alpha
+    return w
+
+
+def _check_side_outputs(o32, out, img, cam, agree):
+    """The blend's other outputs against the oracle's FP32 emulation, on the
+    pixels whose colour agrees (so their alpha / T decisions agree):
+    n_contrib bit-exact; final_T and weight_pixel (max T*alpha, strict '>'
+    keeps the first maximum in depth order, forward.cu:517-525) within FP32
+    rounding of the hardware exp; point_id equal, or -- where two Gaussians
+    tie within rounding -- the GPU's choice carries the same weight;
+    point_weight (the per-Gaussian maximum of T*alpha over all pixels,
+    forward.cu:565) within rounding except for Gaussians seen by a pixel whose
+    decisions flipped."""
+    K, color, point_id, wpix, pw, radii, geom, binning, imgb, _ = out
+    W, H = cam.width, cam.height
+    iv = ext.image_views(imgb, W, H)
+    nc, fT = _np(iv["n_contrib"]).astype(np.uint32), _np(iv["final_T"])
+    pid, wp, pwg = _np(point_id), _np(wpix), _np(pw)
+    # n_contrib: exact wherever the decisions agree
+    bad_nc = agree & (nc != o32["n_contrib"])
+    assert bad_nc.sum() <= 2 + 1e-5 * agree.size, int(bad_nc.sum())
+    ok = agree & ~bad_nc
+    # final transmittance: a product of up to n_contrib factors (1 - alpha);
+    # the hardware exp2 of a power with ~1e-6 absolute rounding moves
+    # (1 - alpha) by up to alpha / (1 - alpha) * 1e-6 relative
+    oT = o32["final_T"]
+    dT = np.abs(fT - oT)
+    assert (dT[ok] <= 1e-4 * oT[ok] + 1e-7).all(), float((dT[ok] / np.maximum(oT[ok], 1e-30)).max())
+    # pixels nothing blends into: id -1, weight 0, T = 1
+    none = ok & (o32["point_id"] == -1)
+    assert (pid[none] == -1).all() and (wp[none] == 0).all() and (fT[none] == 1.0).all()
+    # maximum weight per pixel
+    owp = o32["weight_pixel"]
+    assert (np.abs(wp[ok] - owp[ok]) <= 1e-4 * owp[ok] + 1e-7).all()
+    # point id: equal, or a tie within rounding (checked by walking the list)
+    diff = np.argwhere(ok & (pid != o32["point_id"]))
+    assert len(diff) <= 20 + 1e-4 * agree.size, len(diff)
+    for py, px in diff[:40]:
+        w = _pixel_weights(o32, W, int(px), int(py))
+        g_gpu, g_orc = int(pid[py, px]), int(o32["point_id"][py, px])
+        assert g_gpu in w, (px, py, g_gpu)
+        assert abs(w[g_gpu] - w[g_orc]) <= 1e-4 * w[g_orc], (px, py, w[g_gpu], w[g_orc])
+    # point_weight: rounding everywhere except Gaussians on flipped pixels
+    opw = o32["point_weight"]
+    close = np.abs(pwg - opw) <= 1e-4 * opw + 1e-7
+    nflip = int((~agree).sum()) + int(bad_nc.sum())
+    assert (~close).sum() <= 4 * nflip + 2, (int((~close).sum()), nflip)
+    assert np.abs(pwg - opw).max() <= 0.01
+    assert (pwg >= 0).all() and ((pwg > 0) == (opw > 0)).sum() >= len(opw) - 4 * nflip - 2
+    return dict(n_contrib_mismatch=int(bad_nc.sum()), point_id_ties=len(diff),
+                point_weight_off=int((~close).sum()), final_T_maxrel=float(
+                    (dT[ok] / np.maximum(oT[ok], 1e-30)).max()) if ok.any() else 0.0)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -164,15 +242,16 @@ def test_empty_and_all_culled(orc):
     assert np.allclose(img, np.array([0.25, 0.5, 0.75])[:, None, None])
 
 
-def _grads_gpu(out, g, cam, dL, variant, grad_mode, path="shs", bg=(0.1, 0.2, 0.3)):
+def _grads_gpu(out, g, cam, dL, variant, grad_mode, path="shs", bg=(0.1, 0.2, 0.3),
+               scale_modifier=1.0):
     K, color, point_id, wpix, pw, radii, geom, binning, img, _ = out
     ct = _cam_tensors(cam)
     P = g["means3D"].shape[0]
     sh = _t(g["shs"]) if path == "shs" else torch.Tensor([])
     colors = _t(g["colors"]) if path == "colors" else torch.Tensor([])
     return ext.rasterize_gaussians_backward(
-        _t(bg), _t(g["means3D"]), radii, colors, _t(g["scales"]), _t(g["rotations"]), 1.0,
-        torch.Tensor([]), ct["viewmatrix"], ct["projmatrix"], cam.tanfovx, cam.tanfovy, _t(dL), sh,
+        _t(bg), _t(g["means3D"]), radii, colors, _t(g["scales"]), _t(g["rotations"]),
+        scale_modifier, torch.Tensor([]), ct["viewmatrix"], ct["projmatrix"], cam.tanfovx, cam.tanfovy, _t(dL), sh,
         3, ct["campos"], geom, K, binning, img, False, variant=variant, use_filter=True,
         grad_mode=grad_mode, opacities=_t(g["opacities"].reshape(P)))
 
@@ -199,23 +278,36 @@ def test_backward_large_gaussians_vs_oracle(orc):
     assert (tiles > 128).sum() >= 1
 
 
-def _check_backward(orc, g, cam, variant, grad_mode):
-    bg = (0.1, 0.2, 0.3)
-    out = gpu_forward(g, cam, variant=variant, bg=bg, grad_mode=grad_mode)
-    rng = np.random.default_rng(7)
-    dL = rng.normal(size=(3, cam.height, cam.width)).astype(np.float32)
-    gg = _grads_gpu(out, g, cam, dL, variant, grad_mode, bg=bg)
-    names = ["dL_dmean2D", "dL_dcolors", "dL_dopacity", "dL_dmeans3D", "dL_dcov3D", "dL_dsh",
-             "dL_dscales", "dL_drot"]
-    s = orc_settings(orc, cam, sh_degree=3, variant=variant, bg=bg, grad_mode=grad_mode)
+GRAD_NAMES = ["dL_dmean2D", "dL_dcolors", "dL_dopacity", "dL_dmeans3D", "dL_dcov3D", "dL_dsh",
+              "dL_dscales", "dL_drot"]
+
+
+def _check_backward(orc, g, cam, variant, grad_mode, scale_modifier=1.0, dL=None, bg=(0.1, 0.2, 0.3)):
+    out = gpu_forward(g, cam, variant=variant, bg=bg, grad_mode=grad_mode,
+                      scale_modifier=scale_modifier)
+    if dL is None:
+        rng = np.random.default_rng(7)
+        dL = rng.normal(size=(3, cam.height, cam.width)).astype(np.float32)
+    gg = _grads_gpu(out, g, cam, dL, variant, grad_mode, bg=bg, scale_modifier=scale_modifier)
+    _compare_grads(orc, g, cam, dL, dict(zip(GRAD_NAMES, gg)), variant, grad_mode, bg,
+                   scale_modifier)
+    return out
+
+
+def _compare_grads(orc, g, cam, dL, got, variant, grad_mode, bg, scale_modifier=1.0, path="shs"):
+    """Each GPU gradient in ``got`` (oracle names) against the FP64 oracle of
+    the same scene and dL/dpix."""
+    s = orc_settings(orc, cam, sh_degree=3, variant=variant, bg=bg, grad_mode=grad_mode,
+                     scale_modifier=scale_modifier)
 
     def oracle_grads(mode):
         dt = np.float64 if mode == "f64" else np.float32
         d = {k: g[k].astype(dt) for k in g}
-        f = orc.render(s, d["means3D"], d["opacities"], d["scales"], d["rotations"], d["shs"],
-                       mode=mode, nthreads=8)
+        col = dict(colors=d["colors"]) if path == "colors" else dict(shs=d["shs"])
+        f = orc.render(s, d["means3D"], d["opacities"], d["scales"], d["rotations"],
+                       mode=mode, nthreads=8, **col)
         ob = orc.backward(s, f, dL.astype(dt), d["means3D"], d["opacities"], d["scales"],
-                          d["rotations"], d["shs"], mode=mode)
+                          d["rotations"], mode=mode, **col)
         return dict(dL_dmean2D=ob["dL_dmean2D"], dL_dcolors=ob["dL_dcolors"],
                     dL_dopacity=ob["dL_dopacity"].reshape(-1, 1), dL_dmeans3D=ob["dL_dmeans3D"],
                     dL_dcov3D=ob["dL_dcov3D"], dL_dsh=ob["dL_dsh"], dL_dscales=ob["dL_dscales"],
@@ -233,7 +325,7 @@ def _check_backward(orc, g, cam, variant, grad_mode):
     # FP32 oracle itself.  The max-abs bound leaves room for the rare pixel
     # whose blend decision differs between the hardware exp and FP64 (one
     # step: alpha*T*dL <= ~1/255 * |dL|).
-    for name, t in zip(names, gg):
+    for name, t in got.items():
         a = _np(t).astype(np.float64)
         r = ref[name].reshape(a.shape)
         r32 = ref32[name].reshape(a.shape).astype(np.float64)
@@ -242,7 +334,44 @@ def _check_backward(orc, g, cam, variant, grad_mode):
         err = np.abs(a - r).max()
         assert e_gpu <= max(1e-4, 3 * e_f32 + 1e-5), (name, e_gpu, e_f32)
         assert err <= 2e-3 * scale + 1e-9, (name, err, scale)
-    return out
+
+
+@pytest.mark.parametrize("variant,grad_mode", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_c2a_backward_vs_oracle_and_fd(orc, variant, grad_mode):
+    """Configuration 2a (4096 Gaussians in [-1,1]^3 at 128x128): every GPU
+    gradient against the FP64 oracle; in exact mode also the GPU gradient of
+    a sample of means, scales, rotations, opacities and SH coefficients
+    against FP64 central differences (eps 1e-6) of the oracle forward, the
+    tile structure held fixed and samples whose perturbation flips a blend
+    decision skipped."""
+    g, cam = workloads.config("c2a", seed=71 + variant)
+    _check_backward(orc, g, cam, variant, grad_mode)
+    if grad_mode != 1:
+        return
+    from tests.test_oracle_grad import _Problem
+    prob = _Problem(orc, variant, "shs", seed=71 + variant, scene=(g, cam))
+    bg = (0.2, 0.5, 0.1)
+    out = gpu_forward(g, cam, variant=variant, bg=bg, grad_mode=1)
+    gg = _grads_gpu(out, g, cam, prob.dL.astype(np.float32), variant, 1, bg=bg)
+    gpu = dict(means3D=_np(gg[3]), opacities=_np(gg[2]).reshape(-1), scales=_np(gg[6]),
+               rotations=_np(gg[7]), shs=_np(gg[5]))
+    rng = np.random.default_rng(73)
+    vis = np.flatnonzero(prob.radii > 0)
+    assert vis.size > 3000
+    checked = 0
+    for name, a in gpu.items():
+        per = a.reshape(a.shape[0], -1).shape[1]
+        flat = a.reshape(-1).astype(np.float64)
+        scale = np.abs(flat).max()
+        for gi in rng.choice(vis, 8, replace=False):
+            for comp in rng.choice(per, min(per, 4), replace=False):
+                idx = int(gi) * per + int(comp)
+                fd = prob.fd(name, idx)
+                if fd is None:
+                    continue
+                assert abs(flat[idx] - fd) <= 1e-3 * abs(fd) + 1e-4 * scale, (name, gi, comp, flat[idx], fd)
+                checked += 1
+    assert checked >= 100
 
 
 @pytest.mark.parametrize("stats", [False, True])
@@ -273,31 +402,139 @@ def test_blend_walks_agree(stats):
         _lib.set_blend_walk(0)
 
 
-def test_autograd_through_plugin_surface(orc):
-    """GaussianRasterizer forward/backward through torch.autograd."""
+@pytest.mark.parametrize("grad_mode,path", [("reference", "shs"), ("exact", "shs"),
+                                            ("reference", "colors")])
+def test_autograd_through_plugin_surface(orc, grad_mode, path):
+    """GaussianRasterizer forward / backward through torch.autograd: every
+    leaf's .grad against the FP64 oracle for loss = sum(image * dL), so a
+    gradient routed to the wrong input (means3D vs means2D, SH vs colours)
+    fails."""
     from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
     g, cam = small_scene(n=2000, W=160, H=128, seed=151, dist=6.0)
+    bg = (0.1, 0.2, 0.3)
+    rng = np.random.default_rng(152)
+    g["colors"] = rng.uniform(0, 1, size=(2000, 3)).astype(np.float32)
     ct = _cam_tensors(cam)
     rs = GaussianRasterizationSettings(
         image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
-        bg=_t([0.0, 0.0, 0.0]), scale_modifier=1.0, viewmatrix=ct["viewmatrix"],
+        bg=_t(bg), scale_modifier=1.0, viewmatrix=ct["viewmatrix"],
         projmatrix=ct["projmatrix"], sh_degree=3, campos=ct["campos"], prefiltered=False,
-        debug=False)
+        debug=False, grad_mode=grad_mode)
     r = GaussianRasterizer(rs)
-    params = {k: _t(g[k]).requires_grad_(True) for k in ("means3D", "scales", "rotations", "shs")}
+    keys = ("means3D", "scales", "rotations") + (("shs",) if path == "shs" else ("colors",))
+    params = {k: _t(g[k]).requires_grad_(True) for k in keys}
     opac = _t(g["opacities"].reshape(-1, 1)).requires_grad_(True)
     means2D = torch.zeros_like(params["means3D"], requires_grad=True)
     color, radii, pid, wpix, pw = r(means3D=params["means3D"], means2D=means2D, opacities=opac,
-                                    shs=params["shs"], scales=params["scales"],
-                                    rotations=params["rotations"])
+                                    shs=params.get("shs"), colors_precomp=params.get("colors"),
+                                    scales=params["scales"], rotations=params["rotations"])
     assert r.latest_stats is not None and r.latest_stats["num_gaussians"] == 2000
     assert "tile_ratio" in r.latest_stats and len(r.latest_stats["tile_ratio"]) == r.latest_stats["num_tiles"]
-    loss = (color * color).sum()
-    loss.backward()
-    for k, p in params.items():
-        assert p.grad is not None and torch.isfinite(p.grad).all(), k
-    assert means2D.grad is not None and means2D.grad.abs().sum() > 0
-    assert opac.grad.shape == opac.shape
+    dL = rng.normal(size=(3, cam.height, cam.width)).astype(np.float32)
+    (color * _t(dL)).sum().backward()
+    got = dict(dL_dmean2D=means2D.grad, dL_dopacity=opac.grad, dL_dmeans3D=params["means3D"].grad,
+               dL_dscales=params["scales"].grad, dL_drot=params["rotations"].grad)
+    if path == "shs":
+        got["dL_dsh"] = params["shs"].grad
+    else:
+        got["dL_dcolors"] = params["colors"].grad
+    shapes = dict(dL_dmean2D=(2000, 3), dL_dopacity=(2000, 1), dL_dmeans3D=(2000, 3),
+                  dL_dscales=(2000, 3), dL_drot=(2000, 4), dL_dsh=(2000, 16, 3), dL_dcolors=(2000, 3))
+    for k, v in got.items():
+        assert v is not None and tuple(v.shape) == shapes[k], (k, None if v is None else v.shape)
+    _compare_grads(orc, g, cam, dL, got, 0, 1 if grad_mode == "exact" else 0, bg, path=path)
+
+
+@pytest.mark.parametrize("grad_mode", ["reference", "exact"])
+def test_dilate_plugin_surface_vs_oracle(orc, grad_mode):
+    """The dilate plugin (antialias/diff_gaussian_rasterization/__init__.py)
+    forward and backward through torch.autograd against oracle variant 1
+    (+0.3 dilation, 3-sigma radius): radii bit-exact, image and point weights
+    against the FP32 emulation, every gradient against FP64; and its _C
+    called directly with the 19-argument signature returns the same 8
+    values."""
+    from filtergs_amd import dilate, dilate_ext
+    g, cam = small_scene(n=3000, W=192, H=144, seed=221, dist=6.0)
+    bg = (0.1, 0.2, 0.3)
+    ct = _cam_tensors(cam)
+    rs = dilate.GaussianRasterizationSettings(
+        image_height=cam.height, image_width=cam.width, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
+        bg=_t(bg), scale_modifier=1.0, viewmatrix=ct["viewmatrix"], projmatrix=ct["projmatrix"],
+        sh_degree=3, campos=ct["campos"], prefiltered=False, debug=False, grad_mode=grad_mode)
+    params = {k: _t(g[k]).requires_grad_(True) for k in ("means3D", "scales", "rotations", "shs")}
+    opac = _t(g["opacities"].reshape(-1, 1)).requires_grad_(True)
+    means2D = torch.zeros_like(params["means3D"], requires_grad=True)
+    color, radii, pid, pw = dilate.GaussianRasterizer(rs)(
+        means3D=params["means3D"], means2D=means2D, opacities=opac, shs=params["shs"],
+        scales=params["scales"], rotations=params["rotations"])
+    s = orc_settings(orc, cam, sh_degree=3, variant=1, bg=bg)
+    o32 = orc.render(s, g["means3D"], g["opacities"], g["scales"], g["rotations"], g["shs"],
+                     mode="f32", nthreads=8)
+    assert np.array_equal(_np(radii), o32["pre32"]["radii"])
+    d32 = np.abs(_np(color) - o32["color"]).max(0)
+    assert (d32 > 2e-5).sum() <= 10 + 1e-5 * d32.size and d32.max() <= 0.01
+    opw = o32["point_weight"]
+    off = np.abs(_np(pw) - opw) > 1e-4 * opw + 1e-7
+    assert off.sum() <= 4 * int((d32 > 2e-5).sum()) + 2
+    agree = d32 <= 2e-5
+    assert (_np(pid)[agree] == o32["point_id"][agree]).mean() > 0.999
+    rng = np.random.default_rng(222)
+    dL = rng.normal(size=(3, cam.height, cam.width)).astype(np.float32)
+    (color * _t(dL)).sum().backward()
+    got = dict(dL_dmean2D=means2D.grad, dL_dopacity=opac.grad, dL_dmeans3D=params["means3D"].grad,
+               dL_dscales=params["scales"].grad, dL_drot=params["rotations"].grad,
+               dL_dsh=params["shs"].grad)
+    _compare_grads(orc, g, cam, dL, got, 1, 1 if grad_mode == "exact" else 0, bg)
+    # the _C entry with graphdeco's 19 positional arguments, 8 results
+    with torch.no_grad():
+        out = dilate_ext.rasterize_gaussians(
+            rs.bg, params["means3D"], torch.Tensor([]), opac, params["scales"], params["rotations"],
+            1.0, torch.Tensor([]), rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy,
+            rs.image_height, rs.image_width, params["shs"], 3, rs.campos, False, False)
+    assert len(out) == 8
+    num_rendered, c2, pid2, pw2, radii2 = out[:5]
+    assert num_rendered == len(o32["vals"])
+    assert torch.equal(c2, color) and torch.equal(pid2, pid) and torch.equal(pw2, pw)
+    assert torch.equal(radii2, radii)
+    gr = dilate_ext.rasterize_gaussians_backward(
+        rs.bg, params["means3D"], radii2, torch.Tensor([]), params["scales"], params["rotations"],
+        1.0, torch.Tensor([]), rs.viewmatrix, rs.projmatrix, rs.tanfovx, rs.tanfovy, _t(dL),
+        params["shs"], 3, rs.campos, out[5], num_rendered, out[6], out[7], False,
+        grad_mode=1 if grad_mode == "exact" else 0, opacities=opac.detach())
+    assert len(gr) == 8
+    d = (gr[3].double() - params["means3D"].grad.double()).norm() / params["means3D"].grad.double().norm()
+    assert float(d) <= 1e-5
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("grad_mode", [0, 1])
+@pytest.mark.parametrize("scale_modifier", [0.5, 1.7])
+def test_backward_clamp_and_scale_modifier(orc, variant, grad_mode, scale_modifier):
+    """Half of the Gaussians with opacity in [0.99, 1) and three times the
+    size, so o*G passes 0.99 at the pixels next to their centres: the 0.99
+    alpha clamp is hit (the reference backward passes the gradient through
+    it, the exact mode stops it, backward.cu:499,538).  scale_modifier != 1:
+    the reference's dL/dscale is taken w.r.t. mod*scale and not multiplied by
+    the modifier (backward.cu:295,323-325); the exact mode is.  Both grad
+    modes, both variants, against the FP64 oracle."""
+    n = 2500
+    g, cam = small_scene(n=n, W=224, H=160, seed=211 + variant, dist=6.0)
+    rng = np.random.default_rng(212)
+    hi = rng.choice(n, n // 2, replace=False)
+    g["opacities"][hi] = rng.uniform(0.99, 0.9999, hi.size).astype(np.float32)
+    g["scales"][hi] *= np.float32(3.0)
+    _check_backward(orc, g, cam, variant, grad_mode, scale_modifier)
+    # the clamp is reached: o*G >= 0.99 at the pixel nearest to a centre
+    s = orc_settings(orc, cam, sh_degree=3, variant=variant, scale_modifier=scale_modifier)
+    pre = orc.preprocess(s, g["means3D"], g["opacities"], g["scales"], g["rotations"], g["shs"])
+    vis = np.flatnonzero(pre["radii"] > 0)
+    m = pre["means2D"][vis].astype(np.float64)
+    pix = np.rint(m)
+    d = m - pix
+    co = pre["conic_opacity"][vis].astype(np.float64)
+    G = np.exp(-0.5 * (co[:, 0] * d[:, 0] ** 2 + co[:, 2] * d[:, 1] ** 2) - co[:, 1] * d[:, 0] * d[:, 1])
+    inside = (pix[:, 0] >= 0) & (pix[:, 0] < cam.width) & (pix[:, 1] >= 0) & (pix[:, 1] < cam.height)
+    assert ((co[:, 3] * G >= 0.99) & inside).sum() >= 20
 
 
 @pytest.mark.parametrize("case", ["sparse", "saturated"])

@@ -12,9 +12,11 @@ EPS = 1e-6
 
 
 class _Problem:
-    def __init__(self, orc, variant, path, seed, n=300, W=48, H=40, opac_cap=None):
+    def __init__(self, orc, variant, path, seed, n=300, W=48, H=40, opac_cap=None, scene=None):
         self.orc = orc
-        g, cam = small_scene(n=n, W=W, H=H, seed=seed, dist=5.0, half=1.2)
+        g, cam = scene if scene is not None else small_scene(n=n, W=W, H=H, seed=seed, dist=5.0,
+                                                             half=1.2)
+        n, W, H = g["means3D"].shape[0], cam.width, cam.height
         self.s = orc_settings(orc, cam, sh_degree=3, variant=variant, grad_mode=1,
                               bg=(0.2, 0.5, 0.1))
         p = {k: g[k].astype(np.float64) for k in ("means3D", "scales", "rotations", "opacities", "shs")}
@@ -49,10 +51,11 @@ class _Problem:
         return a
 
     def forward(self, p):
-        pre = self.orc.preprocess(self.s, **self._args(p, np.float64))
+        nt = 1 if p["means3D"].shape[0] < 1000 else 8
+        pre = self.orc.preprocess(self.s, **self._args(p, np.float64), nthreads=nt)
         P = pre["radii"].shape[0]
         f = self.orc.blend_fwd(self.s, P, self.ranges, self.vals, pre["means2D"], pre["rgb"],
-                               pre["conic_opacity"], mode="f64", decision_hash=True)
+                               pre["conic_opacity"], mode="f64", decision_hash=True, nthreads=nt)
         f["pre"] = pre
         f["loss"] = float((f["color"] * self.dL).sum())
         return f
@@ -142,3 +145,15 @@ def test_reference_equals_exact_where_they_coincide(orc):
     for k in ana_exact:
         np.testing.assert_allclose(ana_ref[k], ana_exact[k], rtol=1e-5,
                                    atol=1e-6 * np.abs(ana_exact[k]).max())
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_fd_exact_c2a_size(orc, variant):
+    """The differentiation check at its configured size: 4096 Gaussians at
+    128x128 (FP64 central differences, eps 1e-6, on a sample of the visible
+    Gaussians' means, scales, rotations, opacities and SH)."""
+    from filtergs_amd import workloads
+    prob = _Problem(orc, variant, "shs", seed=61 + variant,
+                    scene=workloads.config("c2a", seed=61 + variant))
+    assert (prob.radii > 0).sum() > 3000
+    assert _check(prob, n_per_param=6, seed=variant) > 150
