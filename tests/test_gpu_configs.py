@@ -44,7 +44,7 @@ def test_c2b_training_step(orc):
     color.retain_grad()
     loss = workloads.l1_ssim_loss(color, target)
     loss.backward()
-    L0 = float(loss)
+    L0 = float(loss.detach())
     assert np.isfinite(L0) and 0 < L0 < 1
     for k, p in leaves.items():
         assert p.grad is not None and p.grad.shape == p.shape and torch.isfinite(p.grad).all(), k

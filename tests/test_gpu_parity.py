@@ -158,13 +158,19 @@ def _check_side_outputs(o32, out, img, cam, agree):
     # n_contrib: exact wherever the decisions agree
     bad_nc = agree & (nc != o32["n_contrib"])
     assert bad_nc.sum() <= 2 + 1e-5 * agree.size, int(bad_nc.sum())
-    ok = agree & ~bad_nc
     # final transmittance: a product of up to n_contrib factors (1 - alpha);
     # the hardware exp2 of a power with ~1e-6 absolute rounding moves
-    # (1 - alpha) by up to alpha / (1 - alpha) * 1e-6 relative
+    # (1 - alpha) by up to alpha / (1 - alpha) * 1e-6 relative.  A pixel whose
+    # colour agrees can still take the other side of an alpha >= 1/255 test
+    # where its T is already small (that step moves the colour by less than
+    # 2e-5): T then differs by one factor (1 - alpha), alpha within rounding
+    # of 1/255 (two such steps at most).
     oT = o32["final_T"]
-    dT = np.abs(fT - oT)
-    assert (dT[ok] <= 1e-4 * oT[ok] + 1e-7).all(), float((dT[ok] / np.maximum(oT[ok], 1e-30)).max())
+    relT = np.abs(fT - oT) / np.maximum(oT, 1e-30)
+    offT = agree & ~bad_nc & (relT > 1e-4)
+    assert offT.sum() <= 10 + 2e-4 * agree.size, int(offT.sum())
+    assert (relT[offT] <= 2.02 / 255).all(), float(relT[offT].max())
+    ok = agree & ~bad_nc & ~offT
     # pixels nothing blends into: id -1, weight 0, T = 1
     none = ok & (o32["point_id"] == -1)
     assert (pid[none] == -1).all() and (wp[none] == 0).all() and (fT[none] == 1.0).all()
@@ -177,18 +183,19 @@ def _check_side_outputs(o32, out, img, cam, agree):
     for py, px in diff[:40]:
         w = _pixel_weights(o32, W, int(px), int(py))
         g_gpu, g_orc = int(pid[py, px]), int(o32["point_id"][py, px])
-        assert g_gpu in w, (px, py, g_gpu)
+        assert g_gpu in w and g_orc in w, (px, py, g_gpu, g_orc)
         assert abs(w[g_gpu] - w[g_orc]) <= 1e-4 * w[g_orc], (px, py, w[g_gpu], w[g_orc])
-    # point_weight: rounding everywhere except Gaussians on flipped pixels
+    # point_weight: within rounding except for Gaussians blended into a pixel
+    # whose decisions differ (at most n_contrib of them per such pixel)
     opw = o32["point_weight"]
     close = np.abs(pwg - opw) <= 1e-4 * opw + 1e-7
-    nflip = int((~agree).sum()) + int(bad_nc.sum())
-    assert (~close).sum() <= 4 * nflip + 2, (int((~close).sum()), nflip)
+    touched = int(o32["n_contrib"][~ok].astype(np.int64).sum() + nc[~ok].astype(np.int64).sum())
+    assert (~close).sum() <= touched + 2, (int((~close).sum()), touched)
     assert np.abs(pwg - opw).max() <= 0.01
-    assert (pwg >= 0).all() and ((pwg > 0) == (opw > 0)).sum() >= len(opw) - 4 * nflip - 2
-    return dict(n_contrib_mismatch=int(bad_nc.sum()), point_id_ties=len(diff),
-                point_weight_off=int((~close).sum()), final_T_maxrel=float(
-                    (dT[ok] / np.maximum(oT[ok], 1e-30)).max()) if ok.any() else 0.0)
+    assert (pwg >= 0).all()
+    return dict(n_contrib_mismatch=int(bad_nc.sum()), final_T_steps=int(offT.sum()),
+                point_id_ties=len(diff), point_weight_off=int((~close).sum()),
+                final_T_maxrel=float(relT[ok].max()) if ok.any() else 0.0)
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -521,7 +528,7 @@ def test_backward_clamp_and_scale_modifier(orc, variant, grad_mode, scale_modifi
     g, cam = small_scene(n=n, W=224, H=160, seed=211 + variant, dist=6.0)
     rng = np.random.default_rng(212)
     hi = rng.choice(n, n // 2, replace=False)
-    g["opacities"][hi] = rng.uniform(0.99, 0.9999, hi.size).astype(np.float32)
+    g["opacities"][hi, 0] = rng.uniform(0.99, 0.9999, hi.size).astype(np.float32)
     g["scales"][hi] *= np.float32(3.0)
     _check_backward(orc, g, cam, variant, grad_mode, scale_modifier)
     # the clamp is reached: o*G >= 0.99 at the pixel nearest to a centre
