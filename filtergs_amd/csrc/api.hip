@@ -5,6 +5,7 @@
 // supplied by the caller and carved here into 256-byte aligned sub-arrays.
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -667,15 +668,16 @@ size_t fgs_backward_ws_bytes(int32_t P)
     return align256((size_t)(P > 0 ? P : 1) * 3 * sizeof(float));
 }
 
-int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
-                 const void* geom, int64_t K, const void* binning, const void* image,
-                 const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity,
-                 float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh, float* dL_dscales,
-                 float* dL_drot, void* ws, void* stream)
+}  // extern "C"
+
+// The backward of one view; ws holds P * 3 floats (dL/dconic), cleared here.
+static int backward_impl(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                         const void* geom, int64_t K, const void* binning, const void* image,
+                         const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors,
+                         float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh,
+                         float* dL_dscales, float* dL_drot, void* ws, void* stream)
 {
-    int rc = validate(s, g);
-    if (rc) return rc;
-    if (g->P == 0) return FGS_OK;
+    int rc = 0;
     if (!geom || !image || !radii || !dL_dpix || !dL_dmeans2D || !dL_dcolors || !dL_dopacity ||
         !dL_dmeans3D || !dL_dcov3D || !ws)
         return fail(FGS_E_INVALID, "null buffer%s");
@@ -733,6 +735,68 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
         if ((rc = check(fgs::launch_preprocess_bwd(a, in, o, st), "preprocess backward"))) return rc;
     }
     return sync_check(s, st, "backward");
+}
+
+// Workspace of fgs_backward_noalloc: dL/dconic of up to
+// FGS_NOALLOC_MAX_GAUSSIANS Gaussians (48 MiB), one copy on every device the
+// library runs on, held until the process ends.  Host code never names it
+// directly: its device address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) float scratch[(size_t)FGS_NOALLOC_MAX_GAUSSIANS * 3];
+
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call made with that device current.
+static int scratch_address(void** out)
+{
+    int dev = -1;
+    int rc = check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(FGS_E_INVALID, "device ordinal out of range%s");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if ((rc = check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address"))) return rc;
+        if (!p) return fail(FGS_E_LAUNCH, "scratch address is null%s");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FGS_OK;
+}
+
+extern "C" {
+
+int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                 const void* geom, int64_t K, const void* binning, const void* image,
+                 const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity,
+                 float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh, float* dL_dscales,
+                 float* dL_drot, void* ws, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    if (g->P == 0) return FGS_OK;
+    return backward_impl(s, g, radii, geom, K, binning, image, dL_dpix, dL_dmeans2D, dL_dcolors,
+                         dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales, dL_drot, ws, stream);
+}
+
+int fgs_backward_noalloc(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                         const void* geom, int64_t K, const void* binning, const void* image,
+                         const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors,
+                         float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh,
+                         float* dL_dscales, float* dL_drot, void* stream)
+{
+    int rc = validate(s, g);
+    if (rc) return rc;
+    if (g->P > FGS_NOALLOC_MAX_GAUSSIANS)
+        return fail(FGS_E_CAPACITY, "fgs_backward_noalloc: more Gaussians than its workspace holds "
+                                   "(FGS_NOALLOC_MAX_GAUSSIANS); use fgs_backward with a workspace%s");
+    if (g->P == 0) return FGS_OK;
+    void* ws = nullptr;
+    if ((rc = scratch_address(&ws))) return rc;
+    return backward_impl(s, g, radii, geom, K, binning, image, dL_dpix, dL_dmeans2D, dL_dcolors,
+                         dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales, dL_drot, ws, stream);
 }
 
 int fgs_mark_visible(int32_t P, const float* means3D, const float* viewmatrix,

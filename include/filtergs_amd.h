@@ -277,6 +277,24 @@ int fgs_backward(const fgs_raster_settings* s, const fgs_gaussians* g, const int
                  float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh, float* dL_dscales,
                  float* dL_drot, void* ws, void* stream);
 
+/* fgs_backward without the workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace (dL/dconic, 12 bytes per Gaussian) is a static
+ * device array of the library, 48 MiB on every device the library runs on,
+ * held until the process ends; calls with more than
+ * FGS_NOALLOC_MAX_GAUSSIANS Gaussians fail with FGS_E_CAPACITY.  The array
+ * is cleared by every call before use.  Every call on one device shares
+ * that device's array: calls that overlap on one device (concurrent
+ * streams, threads) race and must be serialised by the caller.  The stream
+ * must belong to the current device.  Allocates nothing. */
+#define FGS_NOALLOC_MAX_GAUSSIANS 4194304
+int fgs_backward_noalloc(const fgs_raster_settings* s, const fgs_gaussians* g, const int32_t* radii,
+                         const void* geom, int64_t K, const void* binning, const void* image,
+                         const float* dL_dpix, float* dL_dmeans2D, float* dL_dcolors,
+                         float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh,
+                         float* dL_dscales, float* dL_drot, void* stream);
+
 /* ---- batched multi-view path --------------------------------------------
  * No single counterpart in the reference: it replaces V sequential
  * _C.rasterize_gaussians / _C.rasterize_gaussians_backward calls
