@@ -47,63 +47,64 @@ class FilterGS(nn.Module):
 
     # -- construction (FilterGS/filtergs.py:304-359) -----------------------
     def register_by_pointcloud(self, xyz, colors, scales, init_opacity, **init_ply):
-        print(f'[{self.__class__.__name__}] {self.log_radius(scales)}')
-        scales = torch.clamp(scales, min=scales.mean() / 4, max=scales.mean() * 4)
-        print(f'[{self.__class__.__name__}] -> {self.log_radius(scales)}')
-        scaling = torch.log(scales)[:, None].repeat(1, 3)
-        colors = (colors - 0.5) / SH_C0
-        shs = None
-        if self.max_sh_degree > 0:
-            shs = torch.zeros((colors.shape[0], (self.max_sh_degree + 1) ** 2 - 1, 3),
-                              dtype=torch.float32)
-        opacity = torch.ones_like(xyz[:, :1]) * init_opacity
-        opacity = torch.log(opacity / (1 - opacity))
-        rotation = self.init_rotation(xyz.shape[0], xyz.device)
+        """Initial buffers from a point cloud (the reference's
+        register_by_pointcloud).  Each point gets an isotropic scale, clamped
+        to [mean / 4, 4 mean] and stored as its log; its colour as SH DC,
+        (c - 0.5) / C0; zero SH rest coefficients; the logit of init_opacity;
+        the identity rotation.  With init_ply['height'] a ground grid is
+        appended (create_from_ground); as in the reference only its scales go
+        through the inverse activation (log), its colours (0.5) and opacities
+        (ground_opacity) are stored as they are."""
+        m = scales.mean()
+        clamped = scales.clamp(min=m / 4, max=m * 4)
+        print(f'[FilterGS] point-cloud scales {self.log_radius(scales)}, clamped to {self.log_radius(clamped)}')
+        n = xyz.shape[0]
+        o = torch.full((n, 1), init_opacity, dtype=xyz.dtype, device=xyz.device)
+        cols = {'xyz': [xyz], 'colors': [(colors - 0.5) / SH_C0],
+                'scaling': [clamped.log().unsqueeze(1).expand(n, 3).clone()],
+                'opacity': [torch.log(o / (1 - o))], 'rotation': [self.init_rotation(n, xyz.device)]}
+        rest = (self.max_sh_degree + 1) ** 2 - 1 if self.max_sh_degree > 0 else 0
+        if rest:
+            cols['shs'] = [torch.zeros((n, rest, 3), dtype=torch.float32)]
         if 'height' in init_ply:
-            local_min, local_max = xyz.min(dim=0), xyz.max(dim=0)
-            xyz_g, colors_g, scaling_g, opacity_g = self.create_from_ground(
-                local_min, local_max, init_ply['init_step'], init_ply['height'],
+            g_xyz, g_col, g_scale, g_op = self.create_from_ground(
+                xyz.min(dim=0), xyz.max(dim=0), init_ply['init_step'], init_ply['height'],
                 init_ply['ground_opacity'])
-            rotation_g = self.init_rotation(xyz_g.shape[0], xyz.device)
-            print(f'[{self.__class__.__name__}] add {xyz_g.shape[0]} ground points')
-            xyz = torch.cat([xyz, xyz_g], dim=0)
-            opacity = torch.cat([opacity, opacity_g], dim=0)
-            colors = torch.cat([colors, colors_g], dim=0)
-            scaling = torch.cat([scaling, torch.log(scaling_g)], dim=0)
-            rotation = torch.cat([rotation, rotation_g], dim=0)
-            if shs is not None:
-                shs = torch.cat([shs, torch.zeros((xyz_g.shape[0], *shs.shape[1:]),
-                                                  dtype=torch.float32)], dim=0)
-        self.register_buffer('scaling', scaling)
-        self.register_buffer('colors', colors)
-        self.register_buffer('xyz', xyz)
-        self.register_buffer('opacity', opacity)
-        self.register_buffer('rotation', rotation)
-        self.keys.extend(['scaling', 'colors', 'xyz', 'opacity', 'rotation'])
-        if shs is not None:
-            self.register_buffer('shs', shs)
-            self.keys.append('shs')
+            k = g_xyz.shape[0]
+            print(f'[FilterGS] {k} ground points appended')
+            for name, t in (('xyz', g_xyz), ('colors', g_col), ('scaling', g_scale.log()), ('opacity', g_op),
+                            ('rotation', self.init_rotation(k, xyz.device))):
+                cols[name].append(t)
+            if rest:
+                cols['shs'].append(torch.zeros((k, rest, 3), dtype=torch.float32))
+        for name in ('scaling', 'colors', 'xyz', 'opacity', 'rotation', 'shs'):
+            if name in cols:
+                self.register_buffer(name, torch.cat(cols[name], dim=0))
+                self.keys.append(name)
 
     @staticmethod
     def create_from_ground(local_min, local_max, init_step, height, init_opacity=0.9, padding=0.05):
-        x = torch.arange(local_min[0][0] - padding, local_max[0][0] + padding, init_step)
-        y = torch.arange(local_min[0][1] - padding, local_max[0][1] + padding, init_step)
-        x, y = torch.meshgrid(x, y, indexing="ij")
-        xyz = torch.stack((x, y), axis=-1).reshape(-1, 2)
-        xyz = torch.cat([xyz, torch.zeros((xyz.shape[0], 1)) + height], dim=1)
-        colors = torch.zeros_like(xyz) + 0.5
-        scaling = torch.zeros_like(xyz) + init_step
-        scaling[:, 2] = init_step * 0.1
-        opacity = torch.zeros((xyz.shape[0], 1)) + init_opacity
-        return xyz, colors, scaling, opacity
+        """A flat ground grid at z = height: spacing init_step over the xy box
+        of local_min / local_max (the (values, indices) results of
+        xyz.min(dim=0) / xyz.max(dim=0)) widened by padding.  Returns the
+        points, colours 0.5, scales (step, step, step / 10) and opacities
+        init_opacity, none of them activated."""
+        lo, hi = local_min[0], local_max[0]
+        gx, gy = torch.meshgrid(torch.arange(lo[0] - padding, hi[0] + padding, init_step),
+                                torch.arange(lo[1] - padding, hi[1] + padding, init_step), indexing="ij")
+        n = gx.numel()
+        xyz = torch.stack([gx.reshape(-1), gy.reshape(-1), torch.full((n,), float(height))], dim=1)
+        scaling = torch.tensor([init_step, init_step, init_step * 0.1], dtype=torch.float32).repeat(n, 1)
+        return xyz, torch.full((n, 3), 0.5), scaling, torch.full((n, 1), float(init_opacity))
 
     def init_rotation(self, num_points, device):
-        rot = torch.zeros((num_points, 4), dtype=torch.float32, device=device)
-        rot[:, 0] = 1.
-        return rot
+        """num_points identity quaternions (w, x, y, z) = (1, 0, 0, 0)."""
+        return torch.tensor([1.0, 0.0, 0.0, 0.0], dtype=torch.float32, device=device).repeat(num_points, 1)
 
     def log_radius(self, scales):
-        return f'scales: [{scales.min().item():.4f}~{scales.mean().item():.4f}~{scales.max().item():.4f}]'
+        """min / mean / max of a scale tensor as text."""
+        lo, mid, hi = (float(v) for v in (scales.min(), scales.mean(), scales.max()))
+        return f'{lo:.4f} / {mid:.4f} / {hi:.4f}'
 
     # -- stage 1 -------------------------------------------------------------
     def prepare(self, rasterizer, camera, tree=None, tree_node_index_gpu=None, ancestor_path_gpu=None,

@@ -23,7 +23,6 @@ from __future__ import annotations
 
 import math
 
-import numpy as np
 import torch
 import torch.nn as nn
 
@@ -32,92 +31,96 @@ from ._lib import call, ptr
 
 
 def get_expon_lr_func(lr_init, lr_final, lr_delay_steps=0, lr_delay_mult=1.0, max_steps=1000000):
-    """Log-linear decay from lr_init at step 0 to lr_final at max_steps, with an
-    optional sine warm-up over lr_delay_steps (sparse_optimizer.py:6-39)."""
+    """Learning-rate schedule with the signature of sparse_optimizer.py:6-39.
 
-    def helper(step):
+    The rate moves geometrically from lr_init at step 0 to lr_final at
+    max_steps (log lr linear in step / max_steps, clipped to [0, 1]).  With
+    lr_delay_steps > 0 it is scaled by a warm-up factor that rises from
+    lr_delay_mult to 1 along a quarter sine period over the first
+    lr_delay_steps steps.  A negative step, or lr_init = lr_final = 0, gives 0.
+    """
+    log_a = math.log(lr_init) if lr_init > 0 else -math.inf
+    log_b = math.log(lr_final) if lr_final > 0 else -math.inf
+
+    def rate(step):
         if step < 0 or (lr_init == 0.0 and lr_final == 0.0):
             return 0.0
-        if lr_delay_steps > 0:
-            delay_rate = lr_delay_mult + (1 - lr_delay_mult) * np.sin(
-                0.5 * np.pi * np.clip(step / lr_delay_steps, 0, 1))
-        else:
-            delay_rate = 1.0
-        t = np.clip(step / max_steps, 0, 1)
-        log_lerp = np.exp(np.log(lr_init) * (1 - t) + np.log(lr_final) * t)
-        return delay_rate * log_lerp
+        u = min(max(step / max_steps, 0.0), 1.0)
+        lr = math.exp(log_a * (1 - u) + log_b * u)
+        if lr_delay_steps <= 0:
+            return lr
+        ramp = math.sin(0.5 * math.pi * min(max(step / lr_delay_steps, 0.0), 1.0))
+        return (lr_delay_mult + (1 - lr_delay_mult) * ramp) * lr
 
-    return helper
+    return rate
 
 
 class BufferDict(nn.Module):
-    """sparse_optimizer.py:95-117."""
+    """A name -> tensor mapping held as module buffers, so a state_dict
+    stores each tensor under ``<attribute>.<name>`` (the optimiser-state
+    layout of sparse_optimizer.py:96-120).  ``keys`` lists the names in
+    insertion order; ``device`` is the device of the first tensor."""
 
-    def __init__(self, dicts):
+    def __init__(self, tensors):
         super().__init__()
-        for key, value in dicts.items():
-            self.register_buffer(key, value)
-        self.keys = list(dicts.keys())
+        self.keys = []
+        for name, t in tensors.items():
+            self.register_buffer(name, t)
+            self.keys.append(name)
 
     @property
     def device(self):
-        return self[self.keys[0]].device
+        return self._buffers[self.keys[0]].device
 
-    def __getitem__(self, key):
-        return getattr(self, key)
+    def __getitem__(self, name):
+        return self._buffers[name]
+
+    def __contains__(self, name):
+        return name in self.keys
+
+    def __len__(self):
+        return len(self.keys)
 
     def __repr__(self):
         return f'BufferDict: {self.keys}'
 
     def items(self):
-        for key in self.keys:
-            yield key, getattr(self, key)
+        return ((name, self._buffers[name]) for name in self.keys)
 
-    def get(self, key, default_value=None):
-        if key not in self.keys:
-            return default_value
-        return getattr(self, key)
+    def get(self, name, default_value=None):
+        return self._buffers[name] if name in self.keys else default_value
 
 
 class SparseOptimizer(nn.Module):
-    """Adam over the visible rows only (sparse_optimizer.py:119-195)."""
+    """Adam over the visible rows only, with the constructor, buffers and
+    state_dict layout of sparse_optimizer.py:122-155 (global_steps, then the
+    moment dicts exp_avg / exp_avg_sq / max_exp_avg_sq and the per-row step
+    counts ``steps``) and its learning-rate schedules for xyz and scaling."""
 
     def __init__(self, optimize_keys, lr_dict, model, device=None, xyz_scale=None, use_amsgrad=False) -> None:
         super().__init__()
-        if device is None:
-            device = model.xyz.device
-        self.register_buffer('global_steps', torch.tensor(0, dtype=torch.float32, device=device))
+        device = model.xyz.device if device is None else device
         self.device = device
         self.optimize_keys = optimize_keys
-        self.state_keys = ['exp_avg', 'exp_avg_sq']
         self.lr_dict = lr_dict
-        exp_avg, exp_avg_sq, max_exp_avg_sq, steps = {}, {}, {}, {}
-        for key in optimize_keys:
-            param = getattr(model, key)
-            z = torch.zeros_like(param, device=device)
-            exp_avg[key] = z
-            exp_avg_sq[key] = z.clone()
-            if use_amsgrad:
-                max_exp_avg_sq[key] = z.clone()
-            steps[key] = torch.zeros((param.shape[0],), dtype=torch.int32, device=device)
-        self.exp_avg = BufferDict(exp_avg)
-        self.exp_avg_sq = BufferDict(exp_avg_sq)
-        if use_amsgrad:
-            self.max_exp_avg_sq = BufferDict(max_exp_avg_sq)
-            self.state_keys.append('max_exp_avg_sq')
         self.use_amsgrad = use_amsgrad
-        self.steps = BufferDict(steps)
+        self.register_buffer('global_steps', torch.zeros((), dtype=torch.float32, device=device))
+        shapes = {k: getattr(model, k) for k in optimize_keys}
+        self.state_keys = ['exp_avg', 'exp_avg_sq'] + (['max_exp_avg_sq'] if use_amsgrad else [])
+        for moment in self.state_keys:
+            setattr(self, moment, BufferDict({k: torch.zeros_like(p, device=device) for k, p in shapes.items()}))
+        self.steps = BufferDict({k: torch.zeros((p.shape[0],), dtype=torch.int32, device=device)
+                                 for k, p in shapes.items()})
         if xyz_scale is not None:
-            self.xyz_scheduler_args = get_expon_lr_func(
-                lr_init=lr_dict['xyz'] * xyz_scale,
-                lr_final=lr_dict.get('xyz_final', lr_dict['xyz'] * 0.01) * xyz_scale,
-                max_steps=int(lr_dict['max_steps']))
+            horizon = int(lr_dict['max_steps'])
+            self.xyz_lr = lr_dict['xyz'] * xyz_scale
+            xyz_end = lr_dict.get('xyz_final', lr_dict['xyz'] * 0.01) * xyz_scale
+            self.xyz_scheduler_args = get_expon_lr_func(lr_init=self.xyz_lr, lr_final=xyz_end,
+                                                        max_steps=horizon)
             if 'scaling' in lr_dict:
                 self.scaling_scheduler_args = get_expon_lr_func(
-                    lr_init=lr_dict['scaling'],
-                    lr_final=lr_dict.get('scaling_final', lr_dict['scaling'] * 1),
-                    max_steps=int(lr_dict['max_steps']))
-            self.xyz_lr = lr_dict['xyz'] * xyz_scale
+                    lr_init=lr_dict['scaling'], lr_final=lr_dict.get('scaling_final', lr_dict['scaling']),
+                    max_steps=horizon)
 
     def _lr(self, key, step):
         if key == 'xyz':
@@ -163,33 +166,34 @@ class SparseOptimizer(nn.Module):
             raise IndexError("SparseOptimizer.step: index outside the parameter rows")
 
 
+# Counter buffers in state_dict order with their dtype and initial value
+# (LoG/model/counter.py:5-20).
+_COUNTER_BUFFERS = (
+    ('weights_max', torch.float32, 0), ('weights_sum', torch.float32, 0), ('grad_sum', torch.float32, 0),
+    ('radii_max', torch.int16, 0), ('visible_count', torch.int16, 0),
+    ('radii_max_max', torch.int32, 0), ('area_sum', torch.int32, 0),
+    ('radius3d_min', torch.float32, 1), ('radius3d_max', torch.float32, 1),
+    ('create_steps', torch.int32, 0))
+# the buffers reset() clears (counter.py:26-30)
+_COUNTER_RESET = ('weights_max', 'weights_sum', 'radii_max', 'radii_max_max', 'area_sum', 'grad_sum',
+                  'visible_count')
+
+
 class Counter(nn.Module):
-    """Per-point training statistics (counter.py:4-83)."""
+    """Per-point training statistics with LoG's buffers (counter.py:4-83)."""
 
     def __init__(self, num_points, add_depth=False) -> None:
         super().__init__()
-        zero_float32 = torch.zeros((num_points,), dtype=torch.float32)
-        zero_int16 = torch.zeros((num_points,), dtype=torch.int16)
-        zero_int32 = torch.zeros((num_points,), dtype=torch.int32)
-        self.register_buffer('weights_max', zero_float32.clone())
-        self.register_buffer('weights_sum', zero_float32.clone())
-        self.register_buffer('grad_sum', zero_float32.clone())
-        for name in ['radii_max', 'visible_count']:
-            self.register_buffer(name, zero_int16.clone())
-        self.register_buffer('radii_max_max', zero_int32.clone())
-        self.register_buffer('area_sum', zero_int32.clone())
-        self.register_buffer('radius3d_min', zero_float32.clone() + 1)
-        self.register_buffer('radius3d_max', zero_float32.clone() + 1)
-        self.register_buffer('create_steps', zero_int32.clone())
+        for name, dtype, init in _COUNTER_BUFFERS:
+            self.register_buffer(name, torch.full((num_points,), init, dtype=dtype))
 
     def get_gradmean(self):
         return self.grad_sum / torch.clamp(self.area_sum, min=1)
 
     def reset(self, num_points):
-        for key in ['weights_max', 'weights_sum', 'radii_max', 'radii_max_max', 'area_sum', 'grad_sum',
-                    'visible_count']:
-            data = getattr(self, key)
-            data.set_(torch.zeros((num_points,), device=data.device, dtype=data.dtype))
+        for name in _COUNTER_RESET:
+            t = getattr(self, name)
+            t.set_(torch.zeros((num_points,), device=t.device, dtype=t.dtype))
 
     def reset_create_steps(self):
         self.create_steps.fill_(0)

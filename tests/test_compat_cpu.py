@@ -161,3 +161,36 @@ def test_filtergs_register_by_pointcloud_buffers():
     assert torch.allclose(fg.colors, torch.full((4, 3), 0.25 / 0.28209479177387814))
     assert torch.allclose(torch.sigmoid(fg.opacity), torch.full((4, 1), 0.1))
     assert fg.shs.shape == (4, 8, 3) and fg.rotation[:, 0].eq(1).all()
+
+
+def test_filtergs_register_with_ground_grid():
+    """The ground grid of register_by_pointcloud (filtergs.py:322-339,
+    :350-362): a grid of spacing init_step over the cloud's xy box widened by
+    0.05, at z = height, scales (step, step, step / 10) stored as logs, and
+    -- unlike the cloud's points -- colours 0.5 and opacities ground_opacity
+    stored without the inverse activations; the per-point scales are clamped
+    to [mean / 4, 4 mean] first."""
+    import torch
+    from filtergs_amd.filtergs import FilterGS
+    fg = FilterGS(sh_degree=1)
+    xyz = torch.tensor([[0.0, 0.0, 1.0], [1.0, 0.5, 2.0], [0.5, 1.0, 0.0]])
+    scales = torch.tensor([0.01, 1.0, 10.0])       # mean 3.67: clamped to [0.9175, 14.67]
+    fg.register_by_pointcloud(xyz, torch.full((3, 3), 0.5), scales, 0.5, height=-1.0, init_step=0.5,
+                              ground_opacity=0.8)
+    m = float(scales.mean())
+    s = torch.tensor([m / 4, 1.0, 10.0])
+    ax = torch.arange(-0.05, 1.05, 0.5)             # 3 steps per axis over [-0.05, 1.05)
+    n = ax.numel() ** 2
+    assert fg.xyz.shape == (3 + n, 3)
+    assert torch.allclose(fg.scaling[:3], s.log()[:, None].expand(3, 3))
+    g = fg.xyz[3:]
+    assert torch.allclose(g[:, 2], torch.full((n,), -1.0))
+    assert torch.allclose(g[:, 0].reshape(ax.numel(), -1)[:, 0], ax)      # x outer, y inner ("ij")
+    assert torch.allclose(g[:, 1].reshape(ax.numel(), -1)[0], ax)
+    assert torch.allclose(fg.scaling[3:], torch.log(torch.tensor([0.5, 0.5, 0.05])).expand(n, 3))
+    assert torch.allclose(fg.colors[3:], torch.full((n, 3), 0.5))
+    assert torch.allclose(fg.opacity[3:], torch.full((n, 1), 0.8))
+    assert torch.allclose(fg.opacity[:3], torch.zeros((3, 1)))           # logit(0.5)
+    assert fg.shs.shape == (3 + n, 3, 3) and not fg.shs.any()
+    assert (fg.rotation == torch.tensor([1.0, 0.0, 0.0, 0.0])).all()
+    assert fg.keys[-6:] == ['scaling', 'colors', 'xyz', 'opacity', 'rotation', 'shs']
