@@ -198,6 +198,8 @@ _SIGS = {
     "fgs_set_work_shape": (c_int32, [c_int32, c_int32]),
     "fgs_get_work_shape": (c_int32, [c_void_p]),
     "fgs_set_blend_walk": (c_int32, [c_int32]),
+    "fgs_set_backward_layout": (c_int32, [c_int32]),
+    "fgs_get_backward_layout": (c_int32, []),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_abi_version": (c_int32, []),
     "fgs_ogfs_log_lambda": (c_float, [c_float, c_float]),
@@ -276,6 +278,16 @@ def profile_read(reset=True):
 def set_work_shape(fwd_batch=256, bwd_batch=128):
     """Blend work shape: Gaussians per LDS batch (forward 128/256, backward 64/128/256)."""
     call("fgs_set_work_shape", int(fwd_batch), int(bwd_batch))
+
+
+def set_backward_layout(layout=1):
+    """Backward tile layout: 0 four waves per tile (one strip each), 1 one
+    wave per tile with four pixels per lane (fgs_set_backward_layout)."""
+    call("fgs_set_backward_layout", int(layout))
+
+
+def get_backward_layout():
+    return int(load().fgs_get_backward_layout())
 
 
 def set_blend_walk(mode=0):

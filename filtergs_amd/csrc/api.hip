@@ -25,6 +25,8 @@ thread_local std::string g_last_error;
 int g_shape[2] = { 256, 128 };
 // Forward blend walk policy (fgs_set_blend_walk).
 int g_walk = 0;
+// Backward tile layout (fgs_set_backward_layout): one wave per tile by default.
+int g_bwd_layout = 1;
 
 // Optional per-stage timing with HIP events recorded on the launch stream.
 // The last two time the blend kernels alone inside their stages (the
@@ -345,6 +347,15 @@ int fgs_set_blend_walk(int32_t mode)
     g_walk = mode;
     return FGS_OK;
 }
+
+int fgs_set_backward_layout(int32_t layout)
+{
+    if (layout != 0 && layout != 1) return fail(FGS_E_INVALID, "backward layout must be 0 or 1%s");
+    g_bwd_layout = layout;
+    return FGS_OK;
+}
+
+int fgs_get_backward_layout(void) { return g_bwd_layout; }
 
 int fgs_get_work_shape(int32_t* out2)
 {
@@ -713,6 +724,7 @@ static int backward_impl(const fgs_raster_settings* s, const fgs_gaussians* g, c
         b.conic_opacity = gm.conic_opacity;
         b.bg = s->bg;
         b.bwd_batch = g_shape[1];
+        b.bwd_layout = g_bwd_layout;
         ProfScope ps(ST_BLEND_BWD, st);
         const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, 0, nullptr };
         if ((rc = check(fgs::launch_blend_bwd(b, gg, im.final_T, im.vis_last, dL_dpix, dL_dmeans2D,
@@ -1487,6 +1499,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
         b.conic_opacity = gm.conic_opacity;
         b.bwd_batch = g_shape[1];
+        b.bwd_layout = g_bwd_layout;
         ProfScope ps(ST_BLEND_BWD, st);
         const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P, gm.nvis };
         if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,

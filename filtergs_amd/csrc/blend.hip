@@ -950,6 +950,193 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     }
 }
 
+// Publishes the nine sums of a (tile, Gaussian) pair: the opacity, the conic
+// and the screen-space constants applied, then stored at the pair's pre-sort
+// position for the gather (rectangles of at most kGatherSolo tiles) or added
+// with global float atomics.
+__device__ __forceinline__ void publish_pair(const BlendArgs& a, int tx, int ty, uint32_t g, float o,
+                                             float4 con, const float* t)
+{
+    const float ddelx_dx = 0.5f * (float)a.W;
+    const float ddely_dy = 0.5f * (float)a.H;
+    const float sx = o * t[3], sy = o * t[4];
+    const float ho = -0.5f * o;
+    float d[kGradTerms];
+    d[0] = t[0];
+    d[1] = t[1];
+    d[2] = t[2];
+    d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
+    d[4] = -ddely_dy * (con.y * sx + con.z * sy);
+    d[5] = ho * t[5];
+    d[6] = ho * t[6];
+    d[7] = ho * t[7];
+    d[8] = t[8];
+    const uint4 rc = a.recs[g];
+    if (rc.z <= kGatherSolo) {
+        const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
+        const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
+        if (p < a.part_cap) {
+            float* dst = a.part + (size_t)kGradTerms * p;
+#pragma unroll
+            for (int e = 0; e < kGradTerms; ++e) dst[e] = d[e];
+            a.mark[p] = 1;
+        }
+    } else {
+        if (d[0] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 0], d[0]);
+        if (d[1] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 1], d[1]);
+        if (d[2] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 2], d[2]);
+        if (d[3] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 0], d[3]);
+        if (d[4] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 1], d[4]);
+        if (d[5] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 0], d[5]);
+        if (d[6] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 1], d[6]);
+        if (d[7] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 2], d[7]);
+        if (d[8] != 0.f) atomic_add_f32(&a.dL_dopacity[g], d[8]);
+    }
+}
+
+// Backward of one tile by a single wave: lane l owns column l % 16 and, in
+// each of the four strips, row l / 16 of the strip (4 pixels per lane, one
+// per strip).  Per staged Gaussian the wave works the strips its mask names
+// (a wave-uniform branch each), adds the strips' terms in registers and
+// reduces the nine sums across the wave once; the sums go straight to the
+// slot's LDS row (one wave: no atomics, and a fixed summation order).  The
+// per-pixel arithmetic is that of blend_bwd_tile, term for term.
+constexpr int kWaveBatch = FGS_WAVE;
+
+__device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile,
+                                                    const float* __restrict__ final_Ts,
+                                                    const uint32_t* __restrict__ vis_last,
+                                                    const float* __restrict__ dL_dpix, int exact)
+{
+    const int tx = tile % a.gx, ty = tile / a.gx;
+    const int lane = lane_id();
+    const int px = tx * FGS_TILE + (lane & (FGS_TILE - 1));
+    const int py0 = ty * FGS_TILE + (lane / FGS_TILE);
+    float pfx = (float)px;
+    asm volatile("" : "+v"(pfx));
+    const int HW = a.W * a.H;
+
+    __shared__ BwdSlot s_slot[kWaveBatch];
+    __shared__ float s_g[kWaveBatch][kGradTerms];
+
+    float pfy[kStrips], T[kStrips], dp0[kStrips], dp1[kStrips], dp2[kStrips], tfb[kStrips];
+    float acc_d[kStrips], last_d[kStrips], last_alpha[kStrips];
+    uint32_t last[kStrips];
+    uint32_t lmax = 0u;
+#pragma unroll
+    for (int k = 0; k < kStrips; ++k) {
+        const int py = py0 + 4 * k;
+        pfy[k] = (float)py;
+        asm volatile("" : "+v"(pfy[k]));
+        const bool inside = px < a.W && py < a.H;
+        const int pix = a.W * py + px;
+        const float T_final = inside ? final_Ts[pix] : 0.f;
+        T[k] = T_final;
+        last[k] = inside ? vis_last[pix] : 0u;
+        dp0[k] = inside ? dL_dpix[pix] : 0.f;
+        dp1[k] = inside ? dL_dpix[HW + pix] : 0.f;
+        dp2[k] = inside ? dL_dpix[2 * HW + pix] : 0.f;
+        const float bg_dot = a.bg[0] * dp0[k] + a.bg[1] * dp1[k] + a.bg[2] * dp2[k];
+        tfb[k] = -T_final * bg_dot;
+        acc_d[k] = 0.f;
+        last_d[k] = 0.f;
+        last_alpha[k] = 0.f;
+        lmax = last[k] > lmax ? last[k] : lmax;
+    }
+    const uint32_t wmax = __builtin_amdgcn_readfirstlane(wave_max_u32(lmax));
+    const int n = (int)min(wmax, a.tile_vis[tile]);
+    const uint2 range = a.ranges[tile];
+    const bool h3 = (lane >> 3) & 1;
+    const bool pub_op = lane == 1;
+    const bool pub = (lane & 7) == 0 || pub_op;
+    const uint32_t term = pub_op ? 8u : scatter_term(h3);
+
+    for (int r = 0; r * kWaveBatch < n; ++r) {
+        const int k0 = r * kWaveBatch;
+        const int cnt = min(kWaveBatch, n - k0);
+        uint32_t bits = 0u;
+        __syncthreads();
+        if (lane < cnt) {
+            const int pos = n - 1 - (k0 + lane);   // back to front
+            bits = a.kv_strips[range.x + pos];
+            const uint32_t g = a.vis_ids[range.x + pos];
+            const float2 m = a.means2D[g];
+            const float4 co = a.conic_opacity[g];
+            s_slot[lane].g0 = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+            s_slot[lane].g1 = make_float2(-0.5f * kLog2e * co.z, co.w);
+            s_slot[lane].con = co;
+            s_slot[lane].col = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
+                                           __uint_as_float(g));
+        }
+#pragma unroll
+        for (int t = 0; t < kGradTerms; ++t) s_g[lane][t] = 0.f;
+        __syncthreads();
+        // slot j holds position n-1-(k0+j), which blended into a pixel iff it
+        // lies below that pixel's vis_last
+        int thr[kStrips];
+#pragma unroll
+        for (int k = 0; k < kStrips; ++k) thr[k] = n - 1 - k0 - (int)last[k];
+        for (int j = 0; j < cnt; ++j) {
+            const uint32_t sb = __builtin_amdgcn_readlane(bits, j);
+            const BwdSlot* const sp = &s_slot[j];
+            const float4 g0 = sp->g0;
+            const float2 g1 = sp->g1;
+            const float dx = g0.x - pfx;
+            float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+            float vsum = 0.f;
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < kStrips; ++k) {
+                if (!((sb >> k) & 1u)) continue;   // wave-uniform
+                const float dy = g0.y - pfy[k];
+                const float p2 = exponent2(g0, g1, dx, dy);
+                const float G = __builtin_amdgcn_exp2f(fminf(p2, 0.0f));
+                const float alpha_raw = g1.y * G;
+                const float alpha = fminf(0.99f, alpha_raw);
+                const bool act = j > thr[k] && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                if (!__any(act)) continue;   // wave-uniform
+                any = true;
+                const float4 col = sp->col;
+                const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
+                const float Tn = T[k] * inv;
+                const float c_d = col.x * dp0[k] + col.y * dp1[k] + col.z * dp2[k];
+                const float acc_n = acc_d[k] + last_alpha[k] * (last_d[k] - acc_d[k]);
+                const float dLa = (c_d - acc_n) * Tn + inv * tfb[k];
+                const bool flows = act && !(exact && alpha_raw > 0.99f);
+                const float dch = act ? alpha * Tn : 0.f;
+                const float vop = flows ? G * dLa : 0.f;
+                v[0] += dch * dp0[k];
+                v[1] += dch * dp1[k];
+                v[2] += dch * dp2[k];
+                const float vdx = vop * dx, vdy = vop * dy;
+                v[3] += vdx;
+                v[4] += vdy;
+                v[5] += vdx * dx;
+                v[6] += vdx * dy;
+                v[7] += vdy * dy;
+                vsum += vop;
+                if (act) {
+                    T[k] = Tn;
+                    acc_d[k] = acc_n;
+                    last_d[k] = c_d;
+                    last_alpha[k] = alpha;
+                }
+            }
+            if (!any) continue;   // wave-uniform
+            const float z = wave_reduce_scatter8(v, h3);
+            const float zop = wave_total(vsum);
+            if (pub) s_g[j][term] = pub_op ? zop : z;
+        }
+        __syncthreads();
+        if (bits != 0u) {
+            float t[kGradTerms];
+#pragma unroll
+            for (int e = 0; e < kGradTerms; ++e) t[e] = s_g[lane][e];
+            publish_pair(a, tx, ty, __float_as_uint(s_slot[lane].col.w), s_slot[lane].g1.y, s_slot[lane].con, t);
+        }
+    }
+}
+
 // Gather of the per-pair gradients: slot s of the depth order sums the
 // marked entries of its Gaussian's pairs [offs[s], offs[s] + tiles_s[s]) and
 // stores the nine totals (plain stores: every Gaussian has one slot).
@@ -1063,6 +1250,26 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_wave(BlendArgs a, const float* final_Ts,
+                                                             const uint32_t* vis_last, const float* dL_dpix,
+                                                             int exact)
+{
+    blend_bwd_tile_wave(a, blockIdx.x, final_Ts, vis_last, dL_dpix, exact);
+}
+
+__global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_batch_wave(BlendArgs common, const BatchView* views,
+                                                                   int V, const float* final_Ts,
+                                                                   const uint32_t* vis_last,
+                                                                   const float* dL_dpix, int exact)
+{
+    const int v = find_view(views, V, blockIdx.x);
+    const BatchView& vw = views[v];
+    const BlendArgs a = bind_blend(common, vw, v);
+    const size_t pb = vw.pix_base;
+    blend_bwd_tile_wave(a, (int)(blockIdx.x - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb,
+                        exact);
+}
+
 template <int BATCH>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* final_Ts,
                                                          const uint32_t* vis_last, const float* dL_dpix,
@@ -1096,7 +1303,10 @@ hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg,
 #define FGS_BWD(BATCH)                                                                               \
     hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, b, views, V, \
                        final_T, vis_last, dL_dpix, exact)
-        if (b.bwd_batch == 256) FGS_BWD(256);
+        if (b.bwd_layout == 1)
+            hipLaunchKernelGGL(k_blend_bwd_batch_wave, dim3(T_total), dim3(FGS_WAVE), 0, s, b, views, V,
+                               final_T, vis_last, dL_dpix, exact);
+        else if (b.bwd_batch == 256) FGS_BWD(256);
         else if (b.bwd_batch == 64) FGS_BWD(64);
         else FGS_BWD(128);
 #undef FGS_BWD
@@ -1115,7 +1325,10 @@ hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const floa
 #define FGS_BWD(BATCH)                                                                              \
     hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, b, final_T, vis_last, dL_dpix, \
                        exact)
-        if (b.bwd_batch == 256) FGS_BWD(256);
+        if (b.bwd_layout == 1)
+            hipLaunchKernelGGL(k_blend_bwd_wave, dim3(T), dim3(FGS_WAVE), 0, s, b, final_T, vis_last, dL_dpix,
+                               exact);
+        else if (b.bwd_batch == 256) FGS_BWD(256);
         else if (b.bwd_batch == 64) FGS_BWD(64);
         else FGS_BWD(128);
 #undef FGS_BWD

@@ -487,6 +487,16 @@ int fgs_get_work_shape(int32_t* out2);
  * entry), 1 wave-uniform only, 2 divergent only.  All three render the same
  * bits; only the speed differs.  No counterpart in the reference. */
 int fgs_set_blend_walk(int32_t mode);
+/* Backward tile layout (process-wide; no counterpart in the reference):
+ * 0 -- four waves per 16x16 tile, each wave one 16x4 strip, the waves'
+ *      per-Gaussian sums met with LDS float atomics;
+ * 1 -- one wave per tile, four pixels per lane (one per strip): the strips'
+ *      terms are added in registers and reduced across the wave once per
+ *      (tile, Gaussian), in a fixed order (the default).
+ * Both compute the same per-pixel terms; gradients differ by float
+ * summation order only. */
+int fgs_set_backward_layout(int32_t layout);
+int fgs_get_backward_layout(void);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */

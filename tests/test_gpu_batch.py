@@ -51,8 +51,18 @@ def _rel(a, b):
     return float((a - b).norm() / b.norm().clamp_min(1e-30))
 
 
+@pytest.fixture(params=[0, 1], ids=["layout0", "layout1"])
+def bwd_layout(request):
+    """Both backward tile layouts (four waves / one wave per tile)."""
+    from filtergs_amd import _lib
+    before = _lib.get_backward_layout()
+    _lib.set_backward_layout(request.param)
+    yield request.param
+    _lib.set_backward_layout(before)
+
+
 @pytest.mark.parametrize("path", ["shs", "colors"])
-def test_batch_matches_per_view(path):
+def test_batch_matches_per_view(path, bwd_layout):
     g, _ = small_scene(n=20000, seed=171, dist=5.0)
     cams = _cams()
     P = g["means3D"].shape[0]

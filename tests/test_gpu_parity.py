@@ -263,23 +263,38 @@ def _grads_gpu(out, g, cam, dL, variant, grad_mode, path="shs", bg=(0.1, 0.2, 0.
         grad_mode=grad_mode, opacities=_t(g["opacities"].reshape(P)))
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("variant,grad_mode", [(0, 0), (0, 1), (1, 0)])
-def test_backward_vs_oracle_f64(orc, variant, grad_mode):
+def test_backward_vs_oracle_f64(orc, variant, grad_mode, layout):
+    """Both backward tile layouts (four waves per tile / one wave per tile)."""
+    from filtergs_amd import _lib
     g, cam = small_scene(n=3000, W=256, H=192, seed=141 + variant, dist=6.0)
-    _check_backward(orc, g, cam, variant, grad_mode)
+    before = _lib.get_backward_layout()
+    try:
+        _lib.set_backward_layout(layout)
+        _check_backward(orc, g, cam, variant, grad_mode)
+    finally:
+        _lib.set_backward_layout(before)
 
 
-def test_backward_large_gaussians_vs_oracle(orc):
+@pytest.mark.parametrize("layout", [0, 1])
+def test_backward_large_gaussians_vs_oracle(orc, layout):
     """Gaussians of more than 16 tiles (their per-tile sums gathered by the
     whole wave) and of more than 128 tiles (added with float atomics) next to
-    small ones."""
+    small ones, in both backward tile layouts."""
+    from filtergs_amd import _lib
     g, cam = small_scene(n=3000, W=256, H=192, seed=151, dist=6.0)
     rng = np.random.default_rng(3)
     mid = rng.choice(3000, 60, replace=False)
     g["scales"][mid] = rng.uniform(0.15, 0.35, (60, 3)).astype(np.float32)
     g["scales"][mid[:3]] = 1.2
     g["opacities"][mid] = np.float32(0.3)
-    out = _check_backward(orc, g, cam, 0, 0)
+    before = _lib.get_backward_layout()
+    try:
+        _lib.set_backward_layout(layout)
+        out = _check_backward(orc, g, cam, 0, 0)
+    finally:
+        _lib.set_backward_layout(before)
     tiles = _np(ext.geom_views(out[6], 3000)["tiles"]).astype(np.int64)
     assert ((tiles > 16) & (tiles <= 128)).sum() >= 10, np.sort(tiles)[-70:]
     assert (tiles > 128).sum() >= 1
@@ -657,8 +672,8 @@ def test_c1_scale_keys_bitexact_and_image(orc):
 
 
 def test_work_shapes_agree():
-    """Every blend work shape (Gaussians per LDS batch, forward and backward)
-    renders the same image, n_contrib, max-weight ids and point weights bit
+    """Every blend work shape (Gaussians per LDS batch, forward and backward;
+    both backward tile layouts) renders the same image, n_contrib, max-weight ids and point weights bit
     for bit, and the same gradients up to float summation order.  The scene
     puts several hundred Gaussians on most tiles so every shape runs several
     batches."""
@@ -669,8 +684,9 @@ def test_work_shapes_agree():
     bg = (0.1, 0.2, 0.3)
     base = None
     try:
-        for fb, bb in ((256, 128), (128, 64), (256, 256), (128, 128)):
+        for fb, bb, lay in ((256, 128, 1), (128, 64, 0), (256, 256, 0), (128, 128, 0), (256, 128, 0)):
             _lib.set_work_shape(fb, bb)
+            _lib.set_backward_layout(lay)
             assert _lib.get_work_shape() == (fb, bb)
             out = gpu_forward(g, cam, bg=bg)
             gr = _grads_gpu(out, g, cam, dL, 0, 0, bg=bg)
@@ -692,5 +708,8 @@ def test_work_shapes_agree():
                 assert d <= (1e-5 if k < 3 else 1e-4), (fb, bb, "gradient", k, d)
     finally:
         _lib.set_work_shape(256, 128)
+        _lib.set_backward_layout(1)
     with pytest.raises(Exception):
         _lib.set_work_shape(512, 128)
+    with pytest.raises(Exception):
+        _lib.set_backward_layout(2)
