@@ -200,6 +200,8 @@ _SIGS = {
     "fgs_set_blend_walk": (c_int32, [c_int32]),
     "fgs_set_backward_layout": (c_int32, [c_int32]),
     "fgs_get_backward_layout": (c_int32, []),
+    "fgs_set_forward_layout": (c_int32, [c_int32]),
+    "fgs_get_forward_layout": (c_int32, []),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_abi_version": (c_int32, []),
     "fgs_ogfs_log_lambda": (c_float, [c_float, c_float]),
@@ -288,6 +290,17 @@ def set_backward_layout(layout=1):
 
 def get_backward_layout():
     return int(load().fgs_get_backward_layout())
+
+
+def set_forward_layout(layout=2):
+    """Forward tile layout (without statistics): 0 four waves per tile, 1 one
+    wave per tile with four pixels per lane, 2 (default) layout 1 for
+    launches of at least 12288 tiles (fgs_set_forward_layout)."""
+    call("fgs_set_forward_layout", int(layout))
+
+
+def get_forward_layout():
+    return int(load().fgs_get_forward_layout())
 
 
 def set_blend_walk(mode=0):

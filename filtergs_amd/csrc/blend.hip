@@ -530,6 +530,235 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
     }
 }
 
+// The record of one Gaussian a staging lane loads: mean, conic and opacity,
+// colour, id.
+struct StageRec {
+    float2 m;
+    float4 co;
+    float c0, c1, c2;
+    uint32_t g;
+};
+
+__device__ __forceinline__ StageRec load_stage(const BlendArgs& a, uint32_t g)
+{
+    StageRec r;
+    r.g = g;
+    r.m = a.means2D[g];
+    r.co = a.conic_opacity[g];
+    r.c0 = a.colors[3 * g + 0];
+    r.c1 = a.colors[3 * g + 1];
+    r.c2 = a.colors[3 * g + 2];
+    return r;
+}
+
+// Forward of one tile by a single wave (no statistics): lane l owns column
+// l % 16 and, in each of the four strips, row l / 16 of the strip (4 pixels
+// per lane).  Batches of 64 Gaussians of the tile's range are staged in LDS,
+// one per lane, with the strip mask of the conservative alpha >= 1/255 box;
+// each staged Gaussian is worked strip by strip (wave-uniform branches on the
+// mask and on the strips' saturation).  A pixel sees the same Gaussians in
+// the same order with the same operations as in blend_fwd_tile, so the
+// images, n_contrib, point ids and weights are the same bits.  point_weight
+// takes one DPP wave maximum per (tile, Gaussian) over the four strips.
+constexpr int kFwdWaveBatch = FGS_WAVE;
+
+__device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile, float* __restrict__ final_T,
+                                                    uint32_t* __restrict__ n_contrib,
+                                                    uint32_t* __restrict__ vis_last,
+                                                    float* __restrict__ out_color,
+                                                    int* __restrict__ out_point_id,
+                                                    float* __restrict__ out_weight_pixel,
+                                                    float* __restrict__ point_weight,
+                                                    float* __restrict__ kv_sum, uint32_t* __restrict__ kv_cnt)
+{
+#pragma clang fp contract(off)
+    const int tx = tile % a.gx, ty = tile / a.gx;
+    const int lane = lane_id();
+    const int px = tx * FGS_TILE + (lane & (FGS_TILE - 1));
+    const int py0 = ty * FGS_TILE + (lane / FGS_TILE);
+    float pfx = (float)px;
+    asm volatile("" : "+v"(pfx));
+    const float tx0 = (float)(tx * FGS_TILE), ty0 = (float)(ty * FGS_TILE);
+    const uint2 range = a.ranges[tile];
+    const int n = (int)(range.y - range.x);
+
+    __shared__ FwdSlot s_slot[kFwdWaveBatch];
+    __shared__ uint32_t s_vidx[kFwdWaveBatch];
+
+    float pfy[kStrips], T[kStrips], C0[kStrips], C1[kStrips], C2[kStrips], wmax[kStrips];
+    int max_id[kStrips], last_slot[kStrips];
+    uint32_t last[kStrips], vlast[kStrips];
+    bool done[kStrips];
+    uint32_t live = 0u;   // strips with a pixel not yet saturated (wave-uniform)
+#pragma unroll
+    for (int k = 0; k < kStrips; ++k) {
+        const int py = py0 + 4 * k;
+        pfy[k] = (float)py;
+        asm volatile("" : "+v"(pfy[k]));
+        T[k] = 1.0f;
+        C0[k] = C1[k] = C2[k] = 0.f;
+        wmax[k] = 0.f;
+        max_id[k] = -1;
+        last_slot[k] = -1;
+        last[k] = 0u;
+        vlast[k] = 0u;
+        done[k] = !(px < a.W && py < a.H);
+        live |= (__any(!done[k]) ? 1u : 0u) << k;
+    }
+    uint32_t vis_n = 0;   // visible-list entries so far (wave-uniform)
+
+    // Staging is software-pipelined: while round r is worked, the records of
+    // round r + 1 are in flight into registers and the ids of round r + 2
+    // into another (the id loads and the record loads they feed are one
+    // round apart, so neither waits on the other).
+    StageRec cur, nxt;
+    uint32_t id_next = 0u;
+    if (lane < n) cur = load_stage(a, (uint32_t)a.pairs[range.x + lane]);
+    if (kFwdWaveBatch + lane < n) id_next = (uint32_t)a.pairs[range.x + kFwdWaveBatch + lane];
+
+    for (int r = 0; r * kFwdWaveBatch < n && live != 0u; ++r) {
+        const int k0 = r * kFwdWaveBatch;
+        const int cnt = min(kFwdWaveBatch, n - k0);
+        uint32_t bits = 0u;
+        __syncthreads();
+        if (lane < cnt) {
+            s_slot[lane].g0 = make_float4(cur.m.x, cur.m.y, -0.5f * kLog2e * cur.co.x, -kLog2e * cur.co.y);
+            s_slot[lane].g1 = make_float2(-0.5f * kLog2e * cur.co.z, cur.co.w);
+            s_slot[lane].col = make_float4(cur.c0, cur.c1, cur.c2, __uint_as_float(cur.g));
+            bits = strip_bits_ellipse(cur.m, cur.co, tx0, ty0);
+        }
+        if (k0 + kFwdWaveBatch + lane < n) nxt = load_stage(a, id_next);
+        if (k0 + 2 * kFwdWaveBatch + lane < n) id_next = (uint32_t)a.pairs[range.x + k0 + 2 * kFwdWaveBatch + lane];
+        s_slot[lane].w[0] = 0u;   // point weight of the slot (max T*alpha, float bits)
+        s_slot[lane].w[1] = 0u;   // strips it blended into
+        s_slot[lane].kv = 0.f;
+        s_slot[lane].kc = 0u;
+        __syncthreads();
+        for (int j = 0; j < cnt; ++j) {
+            const uint32_t sb = __builtin_amdgcn_readlane(bits, j) & live;
+            if (sb == 0u) continue;   // wave-uniform
+            FwdSlot* const sp = &s_slot[j];
+            const float4 g0 = sp->g0;
+            const float2 g1 = sp->g1;
+            const float dx = g0.x - pfx;
+            float wl = 0.f, wsum = 0.f;
+            uint32_t used = 0u, nbl = 0u;
+            bool stopped = false;
+#pragma unroll
+            for (int k = 0; k < kStrips; ++k) {
+                if (!((sb >> k) & 1u)) continue;   // wave-uniform
+                const float dy = g0.y - pfy[k];
+                const float p2 = exponent2(g0, g1, dx, dy);
+                const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
+                const float test_T = T[k] * (1.0f - alpha);
+                const bool pass = !done[k] && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                const bool stop = pass && test_T < 0.0001f;
+                const bool blend = pass && !stop;
+                done[k] = done[k] || stop;
+                stopped = stopped || stop;
+                const uint64_t bl = __ballot(blend);
+                if (bl == 0u) continue;   // wave-uniform
+                used |= 1u << k;
+                nbl += (uint32_t)__popcll(bl);
+                if (blend) {
+                    const float4 col = sp->col;
+                    const float w = alpha * T[k];
+                    C0[k] = fmaf(col.x, w, C0[k]);
+                    C1[k] = fmaf(col.y, w, C1[k]);
+                    C2[k] = fmaf(col.z, w, C2[k]);
+                    if (w > wmax[k]) {
+                        wmax[k] = w;
+                        max_id[k] = (int)__float_as_uint(col.w);
+                    }
+                    wl = fmaxf(wl, w);
+                    wsum += w;
+                    T[k] = test_T;
+                    last_slot[k] = j;
+                    last[k] = (uint32_t)(k0 + j + 1);
+                }
+            }
+            if (__any(stopped)) {   // wave-uniform
+                uint32_t lv = 0u;
+#pragma unroll
+                for (int k = 0; k < kStrips; ++k) lv |= (__any(!done[k]) ? 1u : 0u) << k;
+                live = lv;
+            }
+            if (used != 0u) {   // wave-uniform
+                const float pw = wave_maxf(wl);
+                if (kv_sum) {
+                    const float ws = wave_total(wsum);
+                    if (lane == 0) {
+                        sp->kv = ws;
+                        sp->kc = nbl;
+                    }
+                }
+                if (lane == 0) {
+                    sp->w[0] = __float_as_uint(pw);
+                    sp->w[1] = used;
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t used = 0u, g = 0u;
+        if (lane < cnt) {
+            g = __float_as_uint(s_slot[lane].col.w);
+            used = s_slot[lane].w[1];
+            const uint32_t m = s_slot[lane].w[0];
+            if (m != 0u) atomicMax((uint32_t*)&point_weight[g], m);
+            if (kv_sum) {
+                kv_sum[range.x + k0 + lane] = s_slot[lane].kv;
+                kv_cnt[range.x + k0 + lane] = s_slot[lane].kc;
+            }
+        }
+        // append the round's visible slots in slot order
+        const uint64_t vb = __ballot(used != 0u);
+        if (used != 0u) {
+            const uint32_t idx = vis_n + __builtin_amdgcn_mbcnt_hi((uint32_t)(vb >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)vb, 0u));
+            a.vis_ids[range.x + idx] = g;
+            a.kv_strips[range.x + idx] = (uint8_t)used;
+            s_vidx[lane] = idx;
+        }
+        vis_n += (uint32_t)__popcll(vb);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kStrips; ++k) {
+            if (last_slot[k] >= 0) {
+                vlast[k] = s_vidx[last_slot[k]] + 1u;
+                last_slot[k] = -1;
+            }
+        }
+        cur = nxt;
+    }
+    if (lane == 0) a.tile_vis[tile] = vis_n;
+    const int HW = a.W * a.H;
+#pragma unroll
+    for (int k = 0; k < kStrips; ++k) {
+        const int py = py0 + 4 * k;
+        if (px < a.W && py < a.H) {
+            const int pix = a.W * py + px;
+            final_T[pix] = T[k];
+            n_contrib[pix] = last[k];
+            vis_last[pix] = vlast[k];
+            out_color[pix] = fmaf(T[k], a.bg[0], C0[k]);
+            out_color[HW + pix] = fmaf(T[k], a.bg[1], C1[k]);
+            out_color[2 * HW + pix] = fmaf(T[k], a.bg[2], C2[k]);
+            out_point_id[pix] = max_id[k] >= 0 ? (int)((uint32_t)max_id[k] - a.id_base) : -1;
+            out_weight_pixel[pix] = wmax[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(FGS_WAVE) void k_blend_fwd_wave(BlendArgs a, float* final_T, uint32_t* n_contrib,
+                                                             uint32_t* vis_last, float* out_color,
+                                                             int* out_point_id, float* out_weight_pixel,
+                                                             float* point_weight, float* kv_sum,
+                                                             uint32_t* kv_cnt)
+{
+    blend_fwd_tile_wave(a, blockIdx.x, final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
+                        point_weight, kv_sum, kv_cnt);
+}
+
 template <int BATCH, bool STATS>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* final_T, uint32_t* n_contrib,
                                                          uint32_t* vis_last, float* out_color,
@@ -585,6 +814,21 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd_batch(BlendArgs common,
                           point_weight, nullptr, nullptr);
 }
 
+__global__ __launch_bounds__(FGS_WAVE) void k_blend_fwd_batch_wave(BlendArgs common, const BatchView* views,
+                                                                   int V, float* final_T, uint32_t* n_contrib,
+                                                                   uint32_t* vis_last, float* out_color,
+                                                                   int* out_point_id, float* out_weight_pixel,
+                                                                   float* point_weight)
+{
+    const int v = find_view(views, V, blockIdx.x);
+    const BatchView& vw = views[v];
+    const BlendArgs a = bind_blend(common, vw, v);
+    const size_t pb = vw.pix_base;
+    blend_fwd_tile_wave(a, (int)(blockIdx.x - vw.tile_base), final_T + pb, n_contrib + pb, vis_last + pb,
+                        out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb, point_weight, nullptr,
+                        nullptr);
+}
+
 hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
                                   float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
                                   float* out_color, int* out_point_id, float* out_weight_pixel,
@@ -592,7 +836,11 @@ hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* view
 {
     if (T_total == 0) return hipSuccess;
     prof_kernel(false, true, s);
-    if (common.batch == 128)
+    if (common.fwd_layout == 1)
+        hipLaunchKernelGGL(k_blend_fwd_batch_wave, dim3(T_total), dim3(FGS_WAVE), 0, s, common, views, V,
+                           final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
+                           point_weight);
+    else if (common.batch == 128)
         hipLaunchKernelGGL(k_blend_fwd_batch<128>, dim3(T_total), dim3(FGS_BLOCK), 0, s, common, views, V,
                            final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
                            point_weight);
@@ -614,7 +862,10 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
     hipLaunchKernelGGL((k_blend_fwd<BATCH, STATS>), dim3(T), dim3(FGS_BLOCK), 0, s, a, final_T, n_contrib, \
                        vis_last, out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt)
     prof_kernel(false, true, s);
-    if (a.stats) {
+    if (a.fwd_layout == 1 && !a.stats) {
+        hipLaunchKernelGGL(k_blend_fwd_wave, dim3(T), dim3(FGS_WAVE), 0, s, a, final_T, n_contrib, vis_last,
+                           out_color, out_point_id, out_weight_pixel, point_weight, kv_sum, kv_cnt);
+    } else if (a.stats) {
         if (a.batch == 128) FGS_FWD(128, true);
         else FGS_FWD(256, true);
     } else {

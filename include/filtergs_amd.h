@@ -497,6 +497,14 @@ int fgs_set_blend_walk(int32_t mode);
  * summation order only. */
 int fgs_set_backward_layout(int32_t layout);
 int fgs_get_backward_layout(void);
+/* Forward tile layout without statistics (process-wide; no counterpart in
+ * the reference): 0 -- four waves per tile with per-wave compacted lists;
+ * 1 -- one wave per tile, four pixels per lane; 2 -- layout 1 for launches
+ * of at least 12288 tiles, else 0 (the default: single waves per tile need
+ * that many tiles to fill the GPU).  Identical outputs, bit for bit; the
+ * forward with statistics keeps layout 0. */
+int fgs_set_forward_layout(int32_t layout);
+int fgs_get_forward_layout(void);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */

@@ -61,8 +61,18 @@ def bwd_layout(request):
     _lib.set_backward_layout(before)
 
 
+@pytest.fixture(params=[0, 1], ids=["fwd0", "fwd1"])
+def fwd_layout(request):
+    """Both forward tile layouts."""
+    from filtergs_amd import _lib
+    before = _lib.get_forward_layout()
+    _lib.set_forward_layout(request.param)
+    yield request.param
+    _lib.set_forward_layout(before)
+
+
 @pytest.mark.parametrize("path", ["shs", "colors"])
-def test_batch_matches_per_view(path, bwd_layout):
+def test_batch_matches_per_view(path, bwd_layout, fwd_layout):
     g, _ = small_scene(n=20000, seed=171, dist=5.0)
     cams = _cams()
     P = g["means3D"].shape[0]

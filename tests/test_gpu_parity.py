@@ -410,11 +410,16 @@ def test_blend_walks_agree(stats):
     scenes = [small_scene(n=20000, W=320, H=240, seed=5, dist=4.0),
               small_scene(n=60000, W=320, H=240, seed=6, dist=14.0)]
     try:
+        _lib.set_forward_layout(0)   # the walk policy belongs to the four-wave layout
         for g, cam in scenes:
             outs = []
             for mode in (0, 1, 2):
                 _lib.set_blend_walk(mode)
                 outs.append(gpu_forward(g, cam, stats=stats))
+            _lib.set_forward_layout(1)
+            if not stats:   # the one-wave layout: the same bits again
+                outs.append(gpu_forward(g, cam, stats=stats))
+            _lib.set_forward_layout(0)
             ranges = _np(ext.image_ranges(outs[0][8], cam.width, cam.height)).astype(np.int64)
             assert (ranges[:, 1] - ranges[:, 0]).max() > 256   # several rounds in some tile
             for o in outs[1:]:
@@ -422,6 +427,7 @@ def test_blend_walks_agree(stats):
                     assert torch.equal(o[k], outs[0][k]), k
     finally:
         _lib.set_blend_walk(0)
+        _lib.set_forward_layout(2)
 
 
 @pytest.mark.parametrize("grad_mode,path", [("reference", "shs"), ("exact", "shs"),
@@ -673,7 +679,7 @@ def test_c1_scale_keys_bitexact_and_image(orc):
 
 def test_work_shapes_agree():
     """Every blend work shape (Gaussians per LDS batch, forward and backward;
-    both backward tile layouts) renders the same image, n_contrib, max-weight ids and point weights bit
+    both tile layouts of the forward and of the backward) renders the same image, n_contrib, max-weight ids and point weights bit
     for bit, and the same gradients up to float summation order.  The scene
     puts several hundred Gaussians on most tiles so every shape runs several
     batches."""
@@ -684,9 +690,11 @@ def test_work_shapes_agree():
     bg = (0.1, 0.2, 0.3)
     base = None
     try:
-        for fb, bb, lay in ((256, 128, 1), (128, 64, 0), (256, 256, 0), (128, 128, 0), (256, 128, 0)):
+        for fb, bb, lay, flay in ((256, 128, 1, 2), (128, 64, 0, 1), (256, 256, 0, 0), (128, 128, 0, 1),
+                                  (256, 128, 0, 0), (256, 128, 1, 1)):
             _lib.set_work_shape(fb, bb)
             _lib.set_backward_layout(lay)
+            _lib.set_forward_layout(flay)
             assert _lib.get_work_shape() == (fb, bb)
             out = gpu_forward(g, cam, bg=bg)
             gr = _grads_gpu(out, g, cam, dL, 0, 0, bg=bg)
@@ -709,7 +717,10 @@ def test_work_shapes_agree():
     finally:
         _lib.set_work_shape(256, 128)
         _lib.set_backward_layout(1)
+        _lib.set_forward_layout(2)
     with pytest.raises(Exception):
         _lib.set_work_shape(512, 128)
     with pytest.raises(Exception):
         _lib.set_backward_layout(2)
+    with pytest.raises(Exception):
+        _lib.set_forward_layout(3)
