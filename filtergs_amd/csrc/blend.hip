@@ -1333,6 +1333,8 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
             const float4 g0 = sp->g0;
             const float2 g1 = sp->g1;
             const float dx = g0.x - pfx;
+            // per lane the strips share the column, so dx: the strips add
+            // v, v dy and v dy^2, and v dx, v dx^2, v dx dy follow per entry
             float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
             float vsum = 0.f;
             bool any = false;
@@ -1359,11 +1361,8 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
                 v[0] += dch * dp0[k];
                 v[1] += dch * dp1[k];
                 v[2] += dch * dp2[k];
-                const float vdx = vop * dx, vdy = vop * dy;
-                v[3] += vdx;
+                const float vdy = vop * dy;
                 v[4] += vdy;
-                v[5] += vdx * dx;
-                v[6] += vdx * dy;
                 v[7] += vdy * dy;
                 vsum += vop;
                 if (act) {
@@ -1374,6 +1373,9 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
                 }
             }
             if (!any) continue;   // wave-uniform
+            v[3] = vsum * dx;
+            v[5] = v[3] * dx;
+            v[6] = v[4] * dx;
             const float z = wave_reduce_scatter8(v, h3);
             const float zop = wave_total(vsum);
             if (pub) s_g[j][term] = pub_op ? zop : z;
