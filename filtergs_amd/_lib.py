@@ -201,6 +201,7 @@ _SIGS = {
     "fgs_set_backward_layout": (c_int32, [c_int32]),
     "fgs_get_backward_layout": (c_int32, []),
     "fgs_set_forward_layout": (c_int32, [c_int32]),
+    "fgs_set_xcd_chunk": (c_int32, [c_int32]),
     "fgs_get_forward_layout": (c_int32, []),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_abi_version": (c_int32, []),
@@ -301,6 +302,12 @@ def set_forward_layout(layout=2):
 
 def get_forward_layout():
     return int(load().fgs_get_forward_layout())
+
+
+def set_xcd_chunk(chunk=32):
+    """Tile order of the batched one-wave blend kernels: runs of `chunk`
+    consecutive tiles per XCD (0: launch order) (fgs_set_xcd_chunk)."""
+    call("fgs_set_xcd_chunk", int(chunk))
 
 
 def set_blend_walk(mode=0):

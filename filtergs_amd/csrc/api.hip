@@ -33,6 +33,9 @@ int g_bwd_layout = 1;
 // mean occupancy fell from 19.6 to 11.9 waves per CU and the tiles with the
 // most pairs, worked by one SIMD each, set the kernel's length.
 int g_fwd_layout = 2;
+// Tiles per XCD run of the batched one-wave blend kernels (fgs_set_xcd_chunk).
+// 16-32 measured best on the 16 orbit views (blend backward 6.92 -> 6.80 ms).
+int g_xcd_chunk = 32;
 constexpr int64_t kFwdWaveMinTiles = 12288;
 int fwd_layout_for(int64_t tiles)
 {
@@ -378,6 +381,13 @@ int fgs_set_forward_layout(int32_t layout)
 
 int fgs_get_forward_layout(void) { return g_fwd_layout; }
 
+int fgs_set_xcd_chunk(int32_t chunk)
+{
+    if (chunk < 0 || chunk > 4096) return fail(FGS_E_INVALID, "xcd chunk must be in [0, 4096]%s");
+    g_xcd_chunk = chunk;
+    return FGS_OK;
+}
+
 int fgs_get_work_shape(int32_t* out2)
 {
     if (!out2) return fail(FGS_E_INVALID, "null output%s");
@@ -582,6 +592,7 @@ static int render_impl(const fgs_raster_settings* s, int P, const float* colors_
     b.batch = g_shape[0];
     b.walk = g_walk;
     b.fwd_layout = fwd_layout_for((int64_t)T);
+    b.xcd_chunk = g_xcd_chunk;
     if (K > 0) {
         if (!binning || !geom || !radii) return fail(FGS_E_INVALID, "null binning/geometry buffer%s");
         Binning bn = carve_binning(binning, K, nullptr);
@@ -747,6 +758,7 @@ static int backward_impl(const fgs_raster_settings* s, const fgs_gaussians* g, c
         b.bg = s->bg;
         b.bwd_batch = g_shape[1];
         b.bwd_layout = g_bwd_layout;
+        b.xcd_chunk = g_xcd_chunk;
         ProfScope ps(ST_BLEND_BWD, st);
         const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, 0, nullptr };
         if ((rc = check(fgs::launch_blend_bwd(b, gg, im.final_T, im.vis_last, dL_dpix, dL_dmeans2D,
@@ -1398,6 +1410,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     b.batch = g_shape[0];
     b.walk = g_walk;
     b.fwd_layout = fwd_layout_for((int64_t)T);
+    b.xcd_chunk = g_xcd_chunk;
     b.id_stride = (uint32_t)P;
     if (n > 0) {
         if (!geom || !radii || (K_cap > 0 && !binning) || !out_point_weight)
@@ -1523,6 +1536,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         b.conic_opacity = gm.conic_opacity;
         b.bwd_batch = g_shape[1];
         b.bwd_layout = g_bwd_layout;
+        b.xcd_chunk = g_xcd_chunk;
         ProfScope ps(ST_BLEND_BWD, st);
         const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P, gm.nvis };
         if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,

@@ -770,6 +770,22 @@ __global__ __launch_bounds__(FGS_BLOCK) void k_blend_fwd(BlendArgs a, float* fin
                                  out_weight_pixel, point_weight, kv_sum, kv_cnt);
 }
 
+// Workgroup -> batch tile.  Workgroups go to the eight XCDs round robin
+// (workgroup b to XCD b % 8), each with its own L2.  With chunk C > 0, XCD x
+// takes runs of C consecutive tiles (its k-th workgroup works tile
+// ((k / C) * 8 + x) * C + k % C), so neighbouring tiles, which stage many of
+// the same Gaussians, share an L2; the runs interleave over the XCDs, which
+// keeps their loads even.  Tiles past the last whole 8 C group map to
+// themselves.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n, uint32_t C)
+{
+    if (C == 0u) return b;
+    const uint32_t n8 = n / (8u * C) * (8u * C);
+    if (b >= n8) return b;
+    const uint32_t x = b % 8u, k = b / 8u;
+    return ((k / C) * 8u + x) * C + k % C;
+}
+
 // View of a batch that owns workgroup b (views sorted by tile_base).
 __device__ __forceinline__ int find_view(const BatchView* views, int V, uint32_t b)
 {
@@ -820,11 +836,12 @@ __global__ __launch_bounds__(FGS_WAVE) void k_blend_fwd_batch_wave(BlendArgs com
                                                                    int* out_point_id, float* out_weight_pixel,
                                                                    float* point_weight)
 {
-    const int v = find_view(views, V, blockIdx.x);
+    const uint32_t b = xcd_tile(blockIdx.x, gridDim.x, (uint32_t)common.xcd_chunk);
+    const int v = find_view(views, V, b);
     const BatchView& vw = views[v];
     const BlendArgs a = bind_blend(common, vw, v);
     const size_t pb = vw.pix_base;
-    blend_fwd_tile_wave(a, (int)(blockIdx.x - vw.tile_base), final_T + pb, n_contrib + pb, vis_last + pb,
+    blend_fwd_tile_wave(a, (int)(b - vw.tile_base), final_T + pb, n_contrib + pb, vis_last + pb,
                         out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb, point_weight, nullptr,
                         nullptr);
 }
@@ -1515,11 +1532,12 @@ __global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_batch_wave(BlendArgs com
                                                                    const uint32_t* vis_last,
                                                                    const float* dL_dpix, int exact)
 {
-    const int v = find_view(views, V, blockIdx.x);
+    const uint32_t b = xcd_tile(blockIdx.x, gridDim.x, (uint32_t)common.xcd_chunk);
+    const int v = find_view(views, V, b);
     const BatchView& vw = views[v];
     const BlendArgs a = bind_blend(common, vw, v);
     const size_t pb = vw.pix_base;
-    blend_bwd_tile_wave(a, (int)(blockIdx.x - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb,
+    blend_bwd_tile_wave(a, (int)(b - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb,
                         exact);
 }
 

@@ -135,6 +135,7 @@ struct BlendArgs {
     int batch, bwd_batch;
     int walk;                  // forward walk: 0 adaptive, 1 wave-uniform, 2 divergent
     int bwd_layout;            // backward: 0 four waves per tile (one strip each), 1 one wave per tile
+    int xcd_chunk;             // batched one-wave blend: runs of tiles per XCD (0: launch order)
     int fwd_layout;            // forward without statistics: 0 four waves per tile, 1 one wave per tile
     // batched path: kv values are elements v * P + i; point ids report i
     uint32_t id_stride;        // P of a batch, 0 for one view

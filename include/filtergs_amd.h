@@ -505,6 +505,11 @@ int fgs_get_backward_layout(void);
  * forward with statistics keeps layout 0. */
 int fgs_set_forward_layout(int32_t layout);
 int fgs_get_forward_layout(void);
+/* Tile order of the batched one-wave blend kernels (process-wide; no
+ * counterpart in the reference): 0 launch order; C > 0 gives each XCD runs
+ * of C consecutive tiles, so that neighbouring tiles share an L2 (default
+ * 32).  Outputs do not depend on it. */
+int fgs_set_xcd_chunk(int32_t chunk);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */

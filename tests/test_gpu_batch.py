@@ -309,3 +309,35 @@ def test_batch_vs_oracle(orc, grad_mode, scale_modifier):
 
 def _np(t):
     return t.detach().cpu().numpy()
+
+
+def test_batch_tile_order_does_not_change_outputs():
+    """The XCD run length of the batched one-wave blend kernels (0 = launch
+    order) moves work between L2s only: images and per-view outputs are the
+    same bits, gradients equal up to float summation order."""
+    from filtergs_amd import _lib
+    g, _ = small_scene(n=20000, seed=241, dist=5.0)
+    base = _cams()
+    cams = [make_camera(256 + 32 * v, 144 + 16 * v, center=(4.0 * np.cos(v), 4.0 * np.sin(v), 0.5),
+                        fovx_deg=55.0) for v in range(12)] + base
+    P = g["means3D"].shape[0]
+    br = BatchRasterizer(cams, P, bg=(0.1, 0.2, 0.3))
+    br.bind(_t(g["means3D"]), _t(g["opacities"]), _t(g["scales"]), _t(g["rotations"]), shs=_t(g["shs"]))
+    dL = _t(np.random.default_rng(242).normal(size=(3 * int(br.sizes.pixels),)).astype(np.float32))
+    res = []
+    _lib.set_forward_layout(1)      # the one-wave forward at any tile count
+    _lib.set_backward_layout(1)
+    try:
+        for chunk in (0, 1, 32, 5):
+            _lib.set_xcd_chunk(chunk)
+            br.forward()
+            gr = br.backward(dL)
+            res.append((br.color.clone(), br.point_id.clone(), br.point_weight.clone(),
+                        {k: v.clone() for k, v in gr.items() if v is not None}))
+    finally:
+        _lib.set_xcd_chunk(32)
+        _lib.set_forward_layout(2)
+    for r in res[1:]:
+        assert torch.equal(r[0], res[0][0]) and torch.equal(r[1], res[0][1]) and torch.equal(r[2], res[0][2])
+        for k in r[3]:
+            assert _rel(r[3][k], res[0][3][k]) <= 1e-4, k
