@@ -12,10 +12,17 @@ random dL/dimage.  Inputs are resident in HBM before timing starts.
 replayed from a HIP graph, work shape picked by the autotuner) in Mpix/s.
 Beside it the line reports the same 16 views rendered by 16 sequential plain
 per-view calls, and the plain per-view path on one 1920x1080 view of the same
-scene with its per-stage times.  The roofline entry covers the dominant
-kernel of the batched step (HIP events on the launch stream around every
-stage of an eager batched step); the CPU baseline is the oracle on a bounded
-sample of the 1080p view, timed on this host.
+scene with its per-stage times, 16 orbit views all at 1920x1080 batched
+against sequential plain calls, and a training step of 3M Gaussians at
+1920x1080 (L1 + SSIM through the plugin's autograd, Adam).  The roofline
+entry covers the dominant kernel of the batched step (HIP events on the
+launch stream around every stage of an eager batched step).  The CPU
+baseline is the oracle on a bounded sample of the 1080p view (a 256-row band,
+FP32 and FP64, per stage, on the host's threads), whose FP64 image is the
+reference of the GPU image's max-abs error on the same band; beside it the
+oracle's per-stage times on the CPU configurations (100k at 800x800, 1M at
+1080p, 4096 at 128x128 with the finite-difference cost) on one thread and on
+all threads.
 """
 import argparse
 import json
@@ -42,6 +49,10 @@ def parse():
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=20, help="steps of the plain 1080p leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-configs", action="store_true",
+                    help="skip the oracle's per-stage timings of the CPU configurations")
+    ap.add_argument("--no-train", action="store_true", help="skip the L1+SSIM training-step leg")
+    ap.add_argument("--no-1080p-batch", action="store_true", help="skip the 16x1080p batched leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-lod", action="store_true", help="skip the urban LoD leg")
     ap.add_argument("--cpu-band", type=int, default=256, help="rows of the CPU baseline sample")
@@ -335,29 +346,281 @@ def lod_leg(steps, warmup):
             "views": len(cams), "pixels": npix}
 
 
-def cpu_baseline(g, cam, dL, band):
-    """Oracle (FP32 emulation) fwd+bwd on a horizontal band of the 1080p view:
-    all Gaussians are preprocessed, pixels of the central `band` rows are
-    blended forward and backward.  Mpix/s = band pixels / wall time."""
-    from filtergs_amd.camera import Camera
+def train_leg(g_t, cam, steps, warmup):
+    """Training step of 3M Gaussians at 1920x1080 through the plugin's
+    autograd (LoG/render/renderer.py:260-273): render, 0.8 L1 + 0.2 (1 -
+    SSIM) against a seeded target, backward to the five parameter tensors and
+    means2D, and a dense Adam step over them; timed with and without the
+    optimizer step."""
+    import torch
+
+    from filtergs_amd import workloads
+    from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+    dev = "cuda"
+    t = lambda a: torch.as_tensor(a, dtype=torch.float32, device=dev)
+    H, W = cam.height, cam.width
+    rs = GaussianRasterizationSettings(
+        image_height=H, image_width=W, tanfovx=cam.tanfovx, tanfovy=cam.tanfovy,
+        bg=t([0.0, 0.0, 0.0]), scale_modifier=1.0, viewmatrix=t(cam.world_view_transform),
+        projmatrix=t(cam.full_proj_transform), sh_degree=3, campos=t(cam.center), prefiltered=False,
+        debug=False)
+    r = GaussianRasterizer(rs)
+    means, scales, rots, opac, shs = (x.detach().clone().requires_grad_(True) for x in g_t)
+    means2D = torch.zeros_like(means, requires_grad=True)
+    params = [means, scales, rots, opac, shs]
+    opt = torch.optim.Adam(params, lr=1e-6, foreach=True)
+    target = torch.rand((3, H, W), generator=torch.Generator(device=dev).manual_seed(5), device=dev)
+    win = workloads.ssim_window().to(dev)
+
+    def step(with_opt):
+        for p in params + [means2D]:
+            p.grad = None
+        color = r(means3D=means, means2D=means2D, opacities=opac, shs=shs, scales=scales,
+                  rotations=rots)[0]
+        loss = workloads.l1_ssim_loss(color, target, win)
+        loss.backward()
+        if with_opt:
+            opt.step()
+        return loss
+
+    res = {}
+    for name, wo in (("render_loss_backward_ms", False), ("step_with_adam_ms", True)):
+        for _ in range(max(1, warmup)):
+            step(wo)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            loss = step(wo)
+        torch.cuda.synchronize()
+        res[name] = round((time.perf_counter() - t0) / steps * 1e3, 3)
+    res["loss"] = round(float(loss.detach()), 6)
+    res["Mpix/s"] = round(H * W / res["render_loss_backward_ms"] / 1e3, 3)
+    res["workload"] = f"{means.shape[0]} Gaussians, {W}x{H}, SH3, 0.8 L1 + 0.2 (1 - SSIM 11x11)"
+    return res
+
+
+def batch_1080p_leg(g_t, views, steps, warmup, plain_step):
+    """`views` orbit cameras at 1920x1080 over the same scene: the batched
+    fwd+bwd (graph replay) against the views rendered by sequential plain
+    per-view calls."""
+    import torch
+
+    from filtergs_amd import workloads
+    from filtergs_amd.batch import BatchRasterizer
+    dev = "cuda"
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev)
+    means, scales, rots, opac, shs = g_t
+    cams = workloads.orbit_cameras(views, seed=5, radius_range=(8.0, 30.0), resolutions=((1920, 1080),))
+    rng = np.random.default_rng(6)
+    dLs = [t(rng.normal(size=(3, c.height, c.width)).astype(np.float32)) for c in cams]
+    cts = [(t(c.world_view_transform), t(c.full_proj_transform), t(c.center)) for c in cams]
+    bg0 = t([0.0, 0.0, 0.0])
+
+    def seq():
+        for c, ct, d in zip(cams, cts, dLs):
+            plain_step(c, ct, d, bg0)
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    seq()
+    ms_seq = timed(seq, max(2, steps // 2))
+    br = BatchRasterizer(cams, means.shape[0], bg=(0.0, 0.0, 0.0))
+    br.bind(means, opac.reshape(-1), scales, rots, shs=shs)
+    K = br.forward()
+    dL = torch.cat([d.reshape(-1) for d in dLs])
+    grads = br.backward(dL)
+    graph, _ = br.capture(dL, out=grads)
+    for _ in range(warmup):
+        graph.replay()
+    ms = timed(graph.replay, steps)
+    if int(br.counts[1].item()) > br.K_cap:
+        raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
+    npix = views * 1920 * 1080
+    out = {"views": views, "kv_pairs": K, "batched_ms": round(ms, 3),
+           "batched_ms_per_view": round(ms / views, 3), "batched_Mpix/s": round(npix / ms / 1e3, 3),
+           "sequential_ms": round(ms_seq, 3), "sequential_ms_per_view": round(ms_seq / views, 3),
+           "batched_speedup": round(ms_seq / ms, 3)}
+    del br, graph, grads
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_stages(orc, s, a, mode, nthreads, dL=None):
+    """Wall time of each oracle stage for one view, in ms: preprocess (the
+    FP32 tile structure, plus the FP64 attributes in mode f64), binning
+    (duplicate, stable sort, ranges), blend forward and, when dL is given,
+    blend backward and preprocess backward.  Inputs `a` are already in the
+    mode's type.  Returns (stage dict, forward result, backward result)."""
+    t = {}
+    t0 = time.perf_counter()
+    p32 = orc.preprocess(s, *a["f32"], mode="f32", nthreads=nthreads)
+    pre = p32 if mode == "f32" else orc.preprocess(s, *a["f64"], mode="f64", nthreads=nthreads)
+    t["preprocess"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    keys, vals, ranges = orc.bin_gaussians(p32["radii"], p32["rect"], p32["depths"], s.image_width,
+                                           s.image_height)
+    t["binning"] = time.perf_counter() - t0
+    P = p32["radii"].shape[0]
+    t0 = time.perf_counter()
+    f = orc.blend_fwd(s, P, ranges, vals, pre["means2D"], pre["rgb"], pre["conic_opacity"], mode,
+                      decision_hash=True, nthreads=nthreads)
+    t["blend_fwd"] = time.perf_counter() - t0
+    gp = None
+    if dL is not None:
+        m = a[mode]
+        t0 = time.perf_counter()
+        gb = orc.blend_bwd(s, P, ranges, vals, pre["means2D"], pre["rgb"], pre["conic_opacity"],
+                           f["final_T"], f["n_contrib"], dL, mode, nthreads=nthreads)
+        t["blend_bwd"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gp = orc.preprocess_bwd(s, m[0], p32["radii"], pre["cov3D"], gb["dL_dmean2D"], gb["dL_dconic"],
+                                gb["dL_dopacity"], gb["dL_dcolors"], m[2], m[3], m[1], m[4], mode,
+                                nthreads=nthreads)
+        t["preprocess_bwd"] = time.perf_counter() - t0
+    out = {k: round(v * 1e3, 2) for k, v in t.items()}
+    out["fwd"] = round(sum(t[k] for k in ("preprocess", "binning", "blend_fwd")) * 1e3, 2)
+    if dL is not None:
+        out["bwd"] = round((t["blend_bwd"] + t["preprocess_bwd"]) * 1e3, 2)
+    out["kv_pairs"] = int(len(vals))
+    return out, f, gp
+
+
+def _typed(g):
+    keys = ("means3D", "opacities", "scales", "rotations", "shs")
+    return {m: tuple(np.ascontiguousarray(g[k], dt) for k in keys)
+            for m, dt in (("f32", np.float32), ("f64", np.float64))}
+
+
+def cpu_configs(cores):
+    """The oracle per stage on the CPU configurations, FP64 and FP32, on one
+    thread and on `cores` threads: 0 -- 100k Gaussians at 800x800, dilate
+    variant, forward; 1 -- 1M Gaussians at 1920x1080, forward and backward
+    (full frame on all threads, a 64-row band on one thread: per-Mpix rates);
+    2a -- 4096 Gaussians at 128x128, forward and backward, and the FP64
+    central finite difference of one parameter (two forwards on one thread),
+    extrapolated to all 59 parameters of every Gaussian."""
+    from filtergs_amd import workloads
+    from oracle import pyoracle as orc
+    out = {}
+
+    def settings(cam, variant=0):
+        return orc.settings(cam.height, cam.width, cam.tanfovx, cam.tanfovy, cam.world_view_transform,
+                            cam.full_proj_transform, cam.center, sh_degree=3, variant=variant)
+
+    def entry(st, cam, threads, rows=None):
+        tot = st["fwd"] + st.get("bwd", 0.0)
+        return {"threads": threads, "rows": rows or cam.height, "stages_ms": st,
+                "Mpix/s": round(cam.width * cam.height / tot / 1e3, 4)}
+
+    # 0: forward only, dilate
+    g, cam = workloads.config("c0")
+    a = _typed(g)
+    s = settings(cam, variant=1)
+    out["c0_100k_800x800_dilate_fwd"] = {
+        m: [entry(oracle_stages(orc, s, a, m, n)[0], cam, n) for n in (1, cores)] for m in ("f64", "f32")}
+    # 1: 1M at 1080p
+    g, cam = workloads.config("c1")
+    a = _typed(g)
+    s = settings(cam)
+    rng = np.random.default_rng(11)
+    dL = rng.normal(size=(3, cam.height, cam.width))
+    rows = 64
+    y0 = (cam.height - rows) // 2
+    bcam = workloads.band_camera(cam, y0, rows)
+    bs = settings(bcam)
+    res = {}
+    for m in ("f64", "f32"):
+        d = np.ascontiguousarray(dL, np.float64 if m == "f64" else np.float32)
+        full = entry(oracle_stages(orc, s, a, m, cores, d)[0], cam, cores)
+        band = entry(oracle_stages(orc, bs, a, m, 1, np.ascontiguousarray(d[:, y0:y0 + rows]))[0], bcam,
+                     1, rows)
+        band["ms_per_view_extrapolated"] = round(
+            band["stages_ms"]["preprocess"] + band["stages_ms"]["preprocess_bwd"] +
+            (band["stages_ms"]["binning"] + band["stages_ms"]["blend_fwd"] +
+             band["stages_ms"]["blend_bwd"]) * cam.height / rows, 1)
+        band["Mpix/s"] = round(cam.width * cam.height / band["ms_per_view_extrapolated"] / 1e3, 4)
+        res[m] = [band, full]
+    out["c1_1M_1080p_fwd_bwd"] = res
+    del g, a, dL
+    # 2a: 4096 at 128x128, and the finite-difference cost
+    g, cam = workloads.config("c2a")
+    a = _typed(g)
+    s = settings(cam)
+    dL = np.random.default_rng(12).normal(size=(3, cam.height, cam.width))
+    oracle_stages(orc, s, a, "f32", cores)      # thread pool start-up, untimed
+    res = {m: [entry(oracle_stages(orc, s, a, m, n,
+                                   np.ascontiguousarray(dL, np.float64 if m == "f64" else np.float32))[0],
+                     cam, n) for n in (1, cores)] for m in ("f64", "f32")}
+    m64 = list(a["f64"])
+    n_fd = 16
+    t0 = time.perf_counter()
+    for j in range(n_fd):
+        i = j * 37 % m64[0].shape[0]
+        for sgn in (1.0, -1.0):
+            mm = m64[0].copy()
+            mm[i, 0] += sgn * 1e-6
+            orc.render(s, mm, *m64[1:], mode="f64", nthreads=1)
+    per = (time.perf_counter() - t0) / n_fd
+    n_par = g["means3D"].shape[0] * (3 + 1 + 3 + 4 + 3 * g["shs"].shape[1])
+    res["fd_f64"] = {"ms_per_parameter": round(per * 1e3, 2), "parameters": n_par,
+                     "all_parameters_s_extrapolated": round(per * n_par, 1), "threads": 1}
+    out["c2a_4096_128x128"] = res
+    return out
+
+
+def cpu_baseline(g, cam, dL, band, gpu_band):
+    """The oracle on a horizontal band of the 3M-Gaussian 1080p view (all
+    Gaussians preprocessed, the band's pixels blended forward and backward)
+    on all host threads: FP32 emulation (the reported value) and FP64, per
+    stage.  The FP64 band image is also the reference for the GPU image of
+    the same band (max-abs on pixels where the FP32 and FP64 oracles and the
+    GPU take the same alpha/transmittance decisions; the rest counted)."""
+    from filtergs_amd import workloads
     from oracle import pyoracle as orc
     y0 = (cam.height - band) // 2
-    sub = Camera(cam.width, band, cam.fx, cam.fy, cam.cx, cam.cy - y0, cam.R, cam.T, cam.znear,
-                 cam.zfar)
+    sub = workloads.band_camera(cam, y0, band)
     s = orc.settings(band, cam.width, sub.tanfovx, sub.tanfovy, sub.world_view_transform,
                      sub.full_proj_transform, sub.center, sh_degree=3)
     cores = min(os.cpu_count() or 1, 16)
-    t0 = time.perf_counter()
-    f = orc.render(s, g["means3D"], g["opacities"], g["scales"], g["rotations"], g["shs"],
-                   mode="f32", nthreads=cores)
-    orc.backward(s, f, np.ascontiguousarray(dL[:, y0:y0 + band, :]), g["means3D"],
-                 g["opacities"], g["scales"], g["rotations"], g["shs"], mode="f32")
-    dt = time.perf_counter() - t0
-    return {"value": round(cam.width * band / dt / 1e6, 4), "unit": "Mpix/s", "cores": cores,
-            "kind": "port",
-            "sample": f"oracle FP32 fwd+bwd of rows {y0}..{y0 + band} of one {cam.width}x{cam.height} "
-                      f"view, all {g['means3D'].shape[0]} Gaussians preprocessed "
-                      f"(preprocess/blend-forward on {cores} threads, backward 1 thread), {dt:.1f} s"}
+    a = _typed(g)
+    d32 = np.ascontiguousarray(dL[:, y0:y0 + band, :], np.float32)
+    st32, f32, _ = oracle_stages(orc, s, a, "f32", cores, d32)
+    st64, f64, _ = oracle_stages(orc, s, a, "f64", cores, d32.astype(np.float64))
+    tot = (st32["fwd"] + st32["bwd"]) / 1e3
+    same = (f64["decision_hash"] == f32["decision_hash"])
+    gd32 = np.abs(gpu_band - f32["color"]).max(0)
+    agree = same & (gd32 <= 2e-5)
+    d64 = np.abs(gpu_band - f64["color"]).max(0)
+    accuracy = {"pixels": int(agree.size), "maxabs_vs_f64": float(d64[agree].max()),
+                "maxabs_vs_f64_all_pixels": float(d64.max()),
+                "decision_flips": int((~agree).sum()),
+                "maxabs_vs_f32_emulation": float(gd32.max()),
+                "sample": f"rows {y0}..{y0 + band} of the {cam.width}x{cam.height} view, "
+                          f"{g['means3D'].shape[0]} Gaussians"}
+    out = {"value": round(cam.width * band / tot / 1e6, 4), "unit": "Mpix/s", "cores": cores,
+           "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "sample": f"oracle FP32 emulation fwd+bwd of rows {y0}..{y0 + band} of one "
+                     f"{cam.width}x{cam.height} view, all {g['means3D'].shape[0]} Gaussians "
+                     f"preprocessed, every stage on {cores} threads, {tot:.1f} s",
+           "stages_ms_f32": st32, "stages_ms_f64": st64,
+           "f64_Mpix/s": round(cam.width * band / (st64["fwd"] + st64["bwd"]) / 1e3, 4)}
+    return out, accuracy
 
 
 def main():
@@ -432,6 +695,14 @@ def main():
     T1 = 120 * 68
     nb1 = stage_bytes(P, Pv1, K1, T1, 1920 * 1080, M, (T1.bit_length() + 7) // 8)
     st1 = stages_of(prof1, nb1)
+    # the GPU image of the band the CPU baseline renders (its FP64 reference)
+    yb = (cam1.height - args.cpu_band) // 2
+    bcam = workloads.band_camera(cam1, yb, args.cpu_band)
+    bct = (t(bcam.world_view_transform), t(bcam.full_proj_transform), t(bcam.center))
+    gpu_band = ext.rasterize_gaussians(bg0, means, empty, opac, scales, rots, 1.0, empty, bct[0], bct[1],
+                                       bcam.tanfovx, bcam.tanfovy, bcam.height, bcam.width, shs, 3,
+                                       bct[2], False, True, False, ogfs_enable_stats=False)[1]
+    gpu_band = gpu_band.cpu().numpy()
 
     # ---- the views: 16 sequential plain calls ------------------------------
     cams = workloads.orbit_cameras(args.views, seed=3, radius_range=(8.0, 30.0))
@@ -527,12 +798,24 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    g_t = (means, scales, rots, opac, shs)
+    br = graph = grads = None
+    torch.cuda.empty_cache()
+    if not args.no_1080p_batch:
+        line["batched_16x1080p"] = batch_1080p_leg(g_t, args.views, args.steps, args.warmup, plain_step)
+        torch.cuda.empty_cache()
+    if not args.no_train:
+        line["train_step_1080p"] = train_leg(g_t, cam1, max(3, args.steps // 2), args.warmup)
     if not args.no_lod:
-        del br, graph, grads, means, scales, rots, opac, shs
+        del means, scales, rots, opac, shs, g_t
         torch.cuda.empty_cache()
         line["lod_c4"] = lod_leg(max(2, args.steps // 2), args.warmup)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(g, cam1, dL1, args.cpu_band)
+        cb, acc = cpu_baseline(g, cam1, dL1, args.cpu_band, gpu_band)
+        line["accuracy"] = acc
+        if not args.no_cpu_configs:
+            cb["configs"] = cpu_configs(cb["cores"])
+        line["cpu_baseline"] = cb
     print(json.dumps(line))
 
 

@@ -118,17 +118,19 @@ void orc_blend_fwd_f64(const orc_settings* s, int32_t P, const uint32_t* ranges,
 /* ---- blend backward (backward.cu:399-557) -------------------------------- */
 /* dL_dpix[3,H,W].  Accumulates (caller zeroes): dL_dmean2D[3P] (x,y used),
  * dL_dconic[3P] (a, b/2-convention, c as in backward.cu:549-551),
- * dL_dopacity[P], dL_dcolors[3P]. */
+ * dL_dopacity[P], dL_dcolors[3P].  nthreads > 1 splits tiles over threads
+ * with per-thread accumulators (10 values for each Gaussian of the list)
+ * summed in thread order. */
 void orc_blend_bwd_f32(const orc_settings* s, int32_t P, const uint32_t* ranges,
                        const uint32_t* point_list, const float* means2D, const float* colors,
                        const float* conic_opacity, const float* final_T, const uint32_t* n_contrib,
                        const float* dL_dpix, float* dL_dmean2D, float* dL_dconic,
-                       float* dL_dopacity, float* dL_dcolors);
+                       float* dL_dopacity, float* dL_dcolors, int32_t nthreads);
 void orc_blend_bwd_f64(const orc_settings* s, int32_t P, const uint32_t* ranges,
                        const uint32_t* point_list, const double* means2D, const double* colors,
                        const double* conic_opacity, const double* final_T, const uint32_t* n_contrib,
                        const double* dL_dpix, double* dL_dmean2D, double* dL_dconic,
-                       double* dL_dopacity, double* dL_dcolors);
+                       double* dL_dopacity, double* dL_dcolors, int32_t nthreads);
 
 /* ---- preprocess backward (backward.cu:144-396) --------------------------- */
 /* Inputs as preprocess plus radii (the mask), cov3D (computed or precomputed),
@@ -141,13 +143,13 @@ void orc_preprocess_bwd_f32(const orc_settings* s, int32_t P, int32_t M, const f
                             const float* shs, const float* cov3D, const int32_t* radii,
                             const float* dL_dmean2D, const float* dL_dconic, float* dL_dopacity,
                             const float* dL_dcolors, float* dL_dmeans3D, float* dL_dcov3D,
-                            float* dL_dsh, float* dL_dscales, float* dL_drot);
+                            float* dL_dsh, float* dL_dscales, float* dL_drot, int32_t nthreads);
 void orc_preprocess_bwd_f64(const orc_settings* s, int32_t P, int32_t M, const double* means3D,
                             const double* scales, const double* rotations, const double* opacities,
                             const double* shs, const double* cov3D, const int32_t* radii,
                             const double* dL_dmean2D, const double* dL_dconic, double* dL_dopacity,
                             const double* dL_dcolors, double* dL_dmeans3D, double* dL_dcov3D,
-                            double* dL_dsh, double* dL_dscales, double* dL_drot);
+                            double* dL_dsh, double* dL_dscales, double* dL_drot, int32_t nthreads);
 
 /* ---- LoD cut and neighbours (FilterGS/cuda, antialias compute_radius) ---- */
 /* compute_visibility_radius (compute_radius_kernel_v2.cu:109-169, 231-353). */

@@ -23,9 +23,13 @@
  *                  (grad_mode exact), and reference == exact where the two
  *                  coincide (dilate variant, no alpha clamp, mod = 1); the
  *                  reference mode's cov2D epsilon by the ratio
- *                  reference / exact = denom^2 / (denom^2 + 1e-7).  The
- *                  reference mode's pass-through of the alpha clamp has no
- *                  such check: parity unpinned.
+ *                  reference / exact = denom^2 / (denom^2 + 1e-7); the
+ *                  alpha clamp by the closed form of one Gaussian with
+ *                  o*G > 0.99 on a pixel centre (reference mode passes
+ *                  dL/do = G (c - bg) dL/dpix, exact mode 0).
+ *   nthreads > 1 : blend fwd/bwd split tiles over threads (per-thread
+ *                  accumulators reduced in thread order), preprocess fwd/bwd
+ *                  split Gaussians; nthreads = 1 keeps the sequential order.
  */
 
 #define CF(x) ((R)(x##f))
@@ -411,18 +415,19 @@ void SFX(orc_blend_fwd)(const orc_settings* s, int32_t P, const uint32_t* ranges
 }
 
 /* ---- blend backward (backward.cu:399-557) ------------------------------- */
-void SFX(orc_blend_bwd)(const orc_settings* s, int32_t P, const uint32_t* ranges,
-                        const uint32_t* point_list, const R* means2D, const R* colors,
-                        const R* conic_opacity, const R* final_T, const uint32_t* n_contrib,
-                        const R* dL_dpix, R* dL_dmean2D, R* dL_dconic,
-                        R* dL_dopacity, R* dL_dcolors)
+/* One tile's pixels, gradients accumulated into the given arrays at row
+ * slot[g] of Gaussian g (at row g when slot is NULL). */
+static void SFX(blend_bwd_tile)(const orc_settings* s, int tile, const uint32_t* ranges,
+                                const uint32_t* point_list, const R* means2D, const R* colors,
+                                const R* conic_opacity, const R* final_T, const uint32_t* n_contrib,
+                                const R* dL_dpix, const int32_t* slot, R* dL_dmean2D, R* dL_dconic,
+                                R* dL_dopacity, R* dL_dcolors)
 {
-    (void)P;
     const int W = s->image_width, H = s->image_height;
-    const int gx = (W + 15) / 16, gy = (H + 15) / 16;
+    const int gx = (W + 15) / 16;
     const R ddelx_dx = (R)(0.5 * (double)W), ddely_dy = (R)(0.5 * (double)H);
     const int exact = s->grad_mode == 1;
-    for (int tile = 0; tile < gx * gy; ++tile) {
+    {
         const int tx = tile % gx, ty = tile / gx;
         const uint32_t r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
         for (int ly = 0; ly < 16; ++ly) {
@@ -447,6 +452,7 @@ void SFX(orc_blend_bwd)(const orc_settings* s, int32_t P, const uint32_t* ranges
                     if (contributor >= last_contributor)
                         continue;
                     const uint32_t g = point_list[k];
+                    const size_t o = slot ? (size_t)slot[g] : (size_t)g;
                     const R dx = means2D[2 * g] - pfx, dy = means2D[2 * g + 1] - pfy;
                     const R* co = conic_opacity + 4 * (size_t)g;
                     const R power = CF(-0.5) * (co[0] * dx * dx + co[2] * dy * dy) - co[1] * dx * dy;
@@ -465,7 +471,7 @@ void SFX(orc_blend_bwd)(const orc_settings* s, int32_t P, const uint32_t* ranges
                         accum_rec[ch] = last_alpha * last_color[ch] + (CF(1.0) - last_alpha) * accum_rec[ch];
                         last_color[ch] = cc;
                         dL_dalpha += (cc - accum_rec[ch]) * dpix[ch];
-                        dL_dcolors[3 * (size_t)g + ch] += dchannel_dcolor * dpix[ch];
+                        dL_dcolors[3 * o + ch] += dchannel_dcolor * dpix[ch];
                     }
                     dL_dalpha *= T;
                     last_alpha = alpha;
@@ -476,16 +482,79 @@ void SFX(orc_blend_bwd)(const orc_settings* s, int32_t P, const uint32_t* ranges
                     const R gdx = G * dx, gdy = G * dy;
                     const R dG_ddelx = -gdx * co[0] - gdy * co[1];
                     const R dG_ddely = -gdy * co[2] - gdx * co[1];
-                    dL_dmean2D[3 * (size_t)g + 0] += dL_dG * dG_ddelx * ddelx_dx;
-                    dL_dmean2D[3 * (size_t)g + 1] += dL_dG * dG_ddely * ddely_dy;
-                    dL_dconic[3 * (size_t)g + 0] += CF(-0.5) * gdx * dx * dL_dG;
-                    dL_dconic[3 * (size_t)g + 1] += CF(-0.5) * gdx * dy * dL_dG;
-                    dL_dconic[3 * (size_t)g + 2] += CF(-0.5) * gdy * dy * dL_dG;
-                    dL_dopacity[g] += G * dL_dalpha * dalpha_dog;
+                    dL_dmean2D[3 * o + 0] += dL_dG * dG_ddelx * ddelx_dx;
+                    dL_dmean2D[3 * o + 1] += dL_dG * dG_ddely * ddely_dy;
+                    dL_dconic[3 * o + 0] += CF(-0.5) * gdx * dx * dL_dG;
+                    dL_dconic[3 * o + 1] += CF(-0.5) * gdx * dy * dL_dG;
+                    dL_dconic[3 * o + 2] += CF(-0.5) * gdy * dy * dL_dG;
+                    dL_dopacity[o] += G * dL_dalpha * dalpha_dog;
                 }
             }
         }
     }
+}
+
+void SFX(orc_blend_bwd)(const orc_settings* s, int32_t P, const uint32_t* ranges,
+                        const uint32_t* point_list, const R* means2D, const R* colors,
+                        const R* conic_opacity, const R* final_T, const uint32_t* n_contrib,
+                        const R* dL_dpix, R* dL_dmean2D, R* dL_dconic,
+                        R* dL_dopacity, R* dL_dcolors, int32_t nthreads)
+{
+    const int gx = (s->image_width + 15) / 16, gy = (s->image_height + 15) / 16;
+    const int T_ = gx * gy;
+    const int nt = nthreads > 0 ? nthreads : 1;
+    if (nt == 1 || P <= 0) {
+        for (int tile = 0; tile < T_; ++tile)
+            SFX(blend_bwd_tile)(s, tile, ranges, point_list, means2D, colors, conic_opacity, final_T,
+                                n_contrib, dL_dpix, NULL, dL_dmean2D, dL_dconic, dL_dopacity,
+                                dL_dcolors);
+        return;
+    }
+    /* accumulator rows only for the U Gaussians in the list; per thread:
+     * mean2D[3U] conic[3U] opacity[U] colour[3U]; tiles dealt out
+     * statically so that each run sums in the same order */
+    uint32_t K = 0;
+    for (int t = 0; t < T_; ++t)
+        if (ranges[2 * t + 1] > K) K = ranges[2 * t + 1];
+    int32_t* slot = (int32_t*)malloc((size_t)P * sizeof(int32_t));
+    for (int32_t g = 0; g < P; ++g) slot[g] = -1;
+    int32_t U = 0;
+    for (uint32_t k = 0; k < K; ++k)
+        if (slot[point_list[k]] < 0) slot[point_list[k]] = U++;
+    const size_t per = (size_t)10 * (size_t)(U > 0 ? U : 1);
+    R* priv = (R*)calloc((size_t)nt * per, sizeof(R));
+    const size_t u3 = 3 * (size_t)U;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 2) num_threads(nt)
+#endif
+    for (int tile = 0; tile < T_; ++tile) {
+#ifdef _OPENMP
+        R* b = priv + (size_t)omp_get_thread_num() * per;
+#else
+        R* b = priv;
+#endif
+        SFX(blend_bwd_tile)(s, tile, ranges, point_list, means2D, colors, conic_opacity, final_T,
+                            n_contrib, dL_dpix, slot, b, b + u3, b + 2 * u3, b + 2 * u3 + U);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int g = 0; g < P; ++g) {
+        const int32_t u = slot[g];
+        if (u < 0)
+            continue;
+        for (int t = 0; t < nt; ++t) {
+            const R* b = priv + (size_t)t * per;
+            for (int k = 0; k < 3; ++k) {
+                dL_dmean2D[3 * (size_t)g + k] += b[3 * (size_t)u + k];
+                dL_dconic[3 * (size_t)g + k] += b[u3 + 3 * (size_t)u + k];
+                dL_dcolors[3 * (size_t)g + k] += b[2 * u3 + U + 3 * (size_t)u + k];
+            }
+            dL_dopacity[g] += b[2 * u3 + u];
+        }
+    }
+    free(priv);
+    free(slot);
 }
 
 /* ---- preprocess backward (backward.cu:144-396) -------------------------- */
@@ -494,7 +563,7 @@ void SFX(orc_preprocess_bwd)(const orc_settings* s, int32_t P, int32_t M, const 
                              const R* shs, const R* cov3D, const int32_t* radii,
                              const R* dL_dmean2D, const R* dL_dconic, R* dL_dopacity,
                              const R* dL_dcolors, R* dL_dmeans3D, R* dL_dcov3D,
-                             R* dL_dsh, R* dL_dscales, R* dL_drot)
+                             R* dL_dsh, R* dL_dscales, R* dL_drot, int32_t nthreads)
 {
     const int W = s->image_width, H = s->image_height;
     const float focal_y_f = (float)H / (2.0f * s->tanfovy);
@@ -506,6 +575,9 @@ void SFX(orc_preprocess_bwd)(const orc_settings* s, int32_t P, int32_t M, const 
     const int deg = s->sh_degree;
     const float* V = s->viewmatrix;
     const float* Pm = s->projmatrix;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static, 256) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
     for (int i = 0; i < P; ++i) {
         if (!(radii[i] > 0))
             continue;

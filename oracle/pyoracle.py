@@ -196,7 +196,7 @@ def blend_fwd(s: OrcSettings, P, ranges, point_list, means2D, colors, conic_opac
 
 
 def blend_bwd(s: OrcSettings, P, ranges, point_list, means2D, colors, conic_opacity, final_T,
-              n_contrib, dL_dpix, mode="f32"):
+              n_contrib, dL_dpix, mode="f32", nthreads=1):
     dt = _dt(mode)
     ranges = _c(ranges, np.uint32)
     point_list = _c(point_list, np.uint32)
@@ -208,12 +208,13 @@ def blend_bwd(s: OrcSettings, P, ranges, point_list, means2D, colors, conic_opac
     fn(ctypes.byref(s), ctypes.c_int32(P), _p(ranges), _p(point_list), _p(_c(means2D, dt)),
        _p(_c(colors, dt)), _p(_c(conic_opacity, dt)), _p(_c(final_T, dt)),
        _p(_c(n_contrib, np.uint32)), _p(_c(dL_dpix, dt)), _p(g["dL_dmean2D"]),
-       _p(g["dL_dconic"]), _p(g["dL_dopacity"]), _p(g["dL_dcolors"]))
+       _p(g["dL_dconic"]), _p(g["dL_dopacity"]), _p(g["dL_dcolors"]), ctypes.c_int32(nthreads))
     return g
 
 
 def preprocess_bwd(s: OrcSettings, means3D, radii, cov3D, dL_dmean2D, dL_dconic, dL_dopacity,
-                   dL_dcolors, scales=None, rotations=None, opacities=None, shs=None, mode="f32"):
+                   dL_dcolors, scales=None, rotations=None, opacities=None, shs=None, mode="f32",
+                   nthreads=1):
     dt = _dt(mode)
     means3D = _c(means3D, dt).reshape(-1, 3)
     P = means3D.shape[0]
@@ -228,7 +229,8 @@ def preprocess_bwd(s: OrcSettings, means3D, radii, cov3D, dL_dmean2D, dL_dconic,
        _p(_c(rotations, dt)), _p(_c(opacities, dt)), _p(sh), _p(_c(cov3D, dt)),
        _p(_c(radii, np.int32)), _p(_c(dL_dmean2D, dt)), _p(_c(dL_dconic, dt)), _p(dop),
        _p(_c(dL_dcolors, dt)), _p(g["dL_dmeans3D"]), _p(g["dL_dcov3D"]),
-       _p(g["dL_dsh"] if M > 0 else None), _p(g["dL_dscales"]), _p(g["dL_drot"]))
+       _p(g["dL_dsh"] if M > 0 else None), _p(g["dL_dscales"]), _p(g["dL_drot"]),
+       ctypes.c_int32(nthreads))
     g["dL_dopacity"] = dop
     return g
 
@@ -380,14 +382,14 @@ def render(s: OrcSettings, means3D, opacities, scales=None, rotations=None, shs=
 
 
 def backward(s: OrcSettings, fwd, dL_dpix, means3D, opacities, scales=None, rotations=None,
-             shs=None, colors=None, cov3D=None, mode="f32"):
+             shs=None, colors=None, cov3D=None, mode="f32", nthreads=1):
     pre = fwd["pre"]
     P = pre["radii"].shape[0]
     gb = blend_bwd(s, P, fwd["ranges"], fwd["vals"], pre["means2D"], pre["rgb"],
-                   pre["conic_opacity"], fwd["final_T"], fwd["n_contrib"], dL_dpix, mode)
+                   pre["conic_opacity"], fwd["final_T"], fwd["n_contrib"], dL_dpix, mode, nthreads)
     gp = preprocess_bwd(s, means3D, fwd["pre32"]["radii"], pre["cov3D"], gb["dL_dmean2D"],
                         gb["dL_dconic"], gb["dL_dopacity"], gb["dL_dcolors"], scales, rotations,
-                        opacities, shs, mode)
+                        opacities, shs, mode, nthreads)
     out = dict(gp)
     out["dL_dmean2D"] = gb["dL_dmean2D"]
     out["dL_dcolors"] = gb["dL_dcolors"]

@@ -157,3 +157,25 @@ def test_fd_exact_c2a_size(orc, variant):
                     scene=workloads.config("c2a", seed=61 + variant))
     assert (prob.radii > 0).sum() > 3000
     assert _check(prob, n_per_param=6, seed=variant) > 150
+
+
+@pytest.mark.parametrize("mode", ["f64", "f32"])
+def test_threaded_backward_equals_sequential(orc, mode):
+    """The threaded backward (tiles and Gaussians split over threads, per-thread
+    accumulators) gives the sequential sums up to summation order, and the
+    same bits on every run with the same thread count."""
+    g, cam = small_scene(n=3000, W=200, H=136, seed=71, dist=5.0)
+    s = orc_settings(orc, cam, sh_degree=3, grad_mode=1)
+    args = (g["means3D"], g["opacities"], g["scales"], g["rotations"], g["shs"])
+    f = orc.render(s, *args, mode=mode)
+    dL = np.random.default_rng(72).normal(size=(3, cam.height, cam.width))
+    seq = orc.backward(s, f, dL, *args, mode=mode)
+    thr = orc.backward(s, f, dL, *args, mode=mode, nthreads=5)
+    thr2 = orc.backward(s, f, dL, *args, mode=mode, nthreads=5)
+    rtol = 1e-11 if mode == "f64" else 2e-4
+    for k in seq:
+        a, b = np.asarray(seq[k], np.float64), np.asarray(thr[k], np.float64)
+        scale = np.abs(a).max() + 1e-30
+        assert np.abs(a - b).max() <= rtol * scale, k
+        assert np.array_equal(thr[k], thr2[k]), k
+    assert np.abs(seq["dL_dmeans3D"]).max() > 0
