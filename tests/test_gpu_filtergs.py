@@ -1,7 +1,8 @@
 """The FilterGS caller API on the GPU: LoD-cut index bit-exact, fused
 gather + activation against the oracle's FP64 activation, the no-tree
 fallback's mask bit-exact against the oracle's frustum and radius, and the
-render against the rasterizer fed the oracle's activated attributes."""
+fused render against the oracle's FP32 emulation and FP64 renders of the
+activated rows."""
 import numpy as np
 import pytest
 import torch
@@ -10,6 +11,7 @@ from filtergs_amd import workloads
 from filtergs_amd.camera import make_camera
 from filtergs_amd.filtergs import FilterGS
 from filtergs_amd.rasterizer import GaussianRasterizationSettings, GaussianRasterizer
+from tests._scenes import orc_settings
 
 pytestmark = pytest.mark.gpu
 
@@ -80,15 +82,31 @@ def test_filtergs_lod_pipeline(orc):
     np.testing.assert_allclose(ref["opacity"], op[idx, 0], rtol=1e-5)
 
     out = fg.render(camera, r)
-    img = out["render"]
-    # the same render from the oracle's activation (rounded to float32)
-    ref_img = r(means3D=_t(ref["xyz"]), means2D=torch.zeros((idx.size, 3), device=DEV), shs=None,
-                colors_precomp=_t(ref["colors"]), opacities=_t(ref["opacity"].reshape(-1, 1)),
-                scales=_t(ref["scaling"]), rotations=_t(ref["rotation"]), ogfs_enable_stats=False)[0]
-    d = (img - ref_img).abs()
-    assert float(d.max()) < 0.02
-    assert float((d > 1e-4).float().mean()) < 1e-3
+    img = out["render"].cpu().numpy()
     assert out["timing"]["T_raster_ms"] >= 0.0
+    # the fused route (activation inside the preprocess) against the oracle's
+    # FP32 emulation fed the activated rows: equal up to the hardware exp,
+    # which can flip an alpha/transmittance decision on a rare pixel
+    s = orc_settings(orc, cam, sh_degree=0)
+    a32 = {k: v.cpu().numpy() for k, v in act.items()}
+    o32 = orc.render(s, a32["xyz"], a32["opacity"].reshape(-1), scales=a32["scaling"],
+                     rotations=a32["rotation"], colors=a32["colors"], mode="f32", nthreads=8,
+                     decision_hash=True)
+    d32 = np.abs(img - o32["color"]).max(0)
+    agree = d32 <= 2e-5
+    assert (~agree).sum() <= 10 + 1e-5 * agree.size, int((~agree).sum())
+    assert d32.max() <= 0.01, float(d32.max())
+    # and against the FP64 oracle fed the FP64 activation: 1e-4 on pixels
+    # whose decisions FP32 and FP64 share
+    o64 = orc.render(s, ref["xyz"].astype(np.float64), ref["opacity"].reshape(-1).astype(np.float64),
+                     scales=ref["scaling"].astype(np.float64),
+                     rotations=ref["rotation"].astype(np.float64),
+                     colors=ref["colors"].astype(np.float64), mode="f64", nthreads=8,
+                     decision_hash=True)
+    same = (o64["decision_hash"] == o32["decision_hash"]) & agree
+    d64 = np.abs(img - o64["color"]).max(0)
+    assert d64[same].max() <= 1e-4, float(d64[same].max())
+    assert (~same).sum() <= 1e-3 * same.size, int((~same).sum())
 
 
 def test_filtergs_no_tree_fallback(orc):
@@ -143,9 +161,12 @@ def test_filtergs_rows_render_matches_gathered_route():
     gathered = fg.render(camera, _Forward(r))
     a, b = fused["render"], gathered["render"]
     assert a.shape == b.shape == (3, cam.height, cam.width)
-    d = (a - b).abs()
-    assert float(d.max()) < 0.02
-    assert float((d > 1e-5).float().mean()) < 1e-3
+    # the same rows activated in the preprocess or by the gather kernel: equal
+    # up to rounding, but for a rare pixel whose alpha/transmittance decision
+    # flips (bounded by one blend step)
+    d = (a - b).abs().amax(0)
+    assert int((d > 1e-5).sum()) <= 10 + 1e-5 * d.numel(), int((d > 1e-5).sum())
+    assert float(d.max()) <= 0.01, float(d.max())
     assert st_fused is not None and st_fused["num_tiles"] == ((cam.width + 15) // 16) * ((cam.height + 15) // 16)
     assert fused["timing"]["T_raster_ms"] > 0.0
 
