@@ -9,7 +9,9 @@ blend) and backward (blend backward, preprocess backward) of every view, with
 random dL/dimage.  Inputs are resident in HBM before timing starts.
 
 ``value`` is the batched path (one launch per stage for all views, the step
-replayed from a HIP graph, work shape picked by the autotuner) in Mpix/s.
+replayed from a HIP graph, blend configuration -- tile layouts, launch order
+or persistent launches over a cost-ordered (view, tile) queue -- picked by the
+autotuner) in Mpix/s.
 Beside it the line reports the same 16 views rendered by 16 sequential plain
 per-view calls, and the plain per-view path on one 1920x1080 view of the same
 scene with its per-stage times, 16 orbit views all at 1920x1080 batched
@@ -785,7 +787,8 @@ def main():
                    "visible_pairs": Pv, "work_shape": list(_lib.get_work_shape())},
         "batched": {"graph_ms": round(ms_graph, 3), "eager_ms": round(ms_eager, 3),
                     "ms_per_view": round(ms_graph / args.views, 3),
-                    "autotune_ms": {f"{a}/{b}": round(v, 3) for (a, b), v in (tuned or {}).items()},
+                    "autotune_ms": {k: round(v, 3) for k, v in (tuned or {}).items()},
+                    "blend_config": getattr(br, "tuned", "launch_xcd32"),
                     "stages": stb, "hbm_stages": hbm_frac(stb, nbb)},
         "sequential_plain": {"ms": round(ms_seq, 3), "Mpix/s": round(npix / ms_seq / 1000.0, 3),
                              "batched_speedup": round(ms_seq / ms_graph, 3)},

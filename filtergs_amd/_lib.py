@@ -202,6 +202,8 @@ _SIGS = {
     "fgs_get_backward_layout": (c_int32, []),
     "fgs_set_forward_layout": (c_int32, [c_int32]),
     "fgs_set_xcd_chunk": (c_int32, [c_int32]),
+    "fgs_set_tile_queue": (c_int32, [c_int32, c_int32]),
+    "fgs_get_tile_queue": (c_int32, []),
     "fgs_get_forward_layout": (c_int32, []),
     "fgs_last_error": (ctypes.c_char_p, []),
     "fgs_abi_version": (c_int32, []),
@@ -308,6 +310,18 @@ def set_xcd_chunk(chunk=32):
     """Tile order of the batched one-wave blend kernels: runs of `chunk`
     consecutive tiles per XCD (0: launch order) (fgs_set_xcd_chunk)."""
     call("fgs_set_xcd_chunk", int(chunk))
+
+
+def set_tile_queue(chunk=0, by_cost=True):
+    """Batched one-wave blends as persistent launches over a (view, tile)
+    queue of runs of `chunk` consecutive tiles, dealt to one queue per XCD;
+    `by_cost` orders the runs heaviest first.  0 launches one workgroup per
+    tile (fgs_set_tile_queue)."""
+    call("fgs_set_tile_queue", int(chunk), int(bool(by_cost)))
+
+
+def get_tile_queue():
+    return int(load().fgs_get_tile_queue())
 
 
 def set_blend_walk(mode=0):

@@ -9,7 +9,8 @@ tile sort cover every view, one blend launch renders every tile of every view,
 and the backward folds the per-view gradients into one set of parameter
 gradients.  The kv-pair buffer has a capacity, so a step needs no host read
 and ``capture()`` can record it in a HIP graph; ``autotune()`` picks the blend
-work shape by timing the step.
+configuration (tile layouts, launch order or a persistent launch over a
+cost-ordered (view, tile) queue, LDS batch depth) by timing the step.
 
 Per view the forward outputs equal those of ``ext.rasterize_gaussians`` bit
 for bit (images, radii, point ids, point weights); gradients equal the sum of
@@ -282,14 +283,47 @@ class BatchRasterizer:
             self._backward_launch(dL_dpix, out)
         return graph, out
 
-    def autotune(self, dL_dpix, shapes=((256, 128), (256, 64), (256, 256), (128, 128), (128, 64)),
-                 iters: int = 3):
-        """Time forward + backward for each blend work shape and keep the
-        fastest (process-wide).  Returns {shape: ms}."""
+    # Blend configurations the autotuner tries: each entry overrides the
+    # defaults in _TUNE_BASE.  "tile_queue" > 0 runs the one-wave blends as
+    # persistent launches over the cost-ordered (view, tile) queue with that
+    # many tiles per XCD run; "xcd_chunk" is the run length of the launch
+    # order without the queue; the layouts pick one wave or four waves per
+    # tile; "work_shape" is the four-wave layout's LDS batch depth.
+    _TUNE_BASE = {"forward_layout": 2, "backward_layout": 1, "xcd_chunk": 32, "tile_queue": 0,
+                  "by_cost": True, "work_shape": (256, 128)}
+    TUNE_CONFIGS = {
+        "launch_xcd32": {},
+        "launch_xcd0": {"xcd_chunk": 0},
+        "queue8": {"tile_queue": 8},
+        "queue32": {"tile_queue": 32},
+        "queue128": {"tile_queue": 128},
+        "queue32_launch_order": {"tile_queue": 32, "by_cost": False},
+        "queue1_launch_order": {"tile_queue": 1, "by_cost": False},
+        "fwd4wave_256": {"forward_layout": 0},
+        "bwd4wave_128": {"backward_layout": 0},
+    }
+
+    @staticmethod
+    def apply_config(cfg):
+        """Set the process-wide blend knobs of one autotuner configuration."""
+        c = dict(BatchRasterizer._TUNE_BASE)
+        c.update(cfg)
+        _lib.set_forward_layout(c["forward_layout"])
+        _lib.set_backward_layout(c["backward_layout"])
+        _lib.set_xcd_chunk(c["xcd_chunk"])
+        _lib.set_tile_queue(c["tile_queue"], c["by_cost"])
+        _lib.set_work_shape(*c["work_shape"])
+
+    def autotune(self, dL_dpix, configs=None, iters: int = 3):
+        """Time forward + backward under each blend configuration (layouts,
+        tile order: launch order or persistent queue and its run length, LDS
+        batch depth) and keep the fastest, process-wide.  The outputs do not
+        depend on the choice.  Returns {label: ms}."""
+        configs = self.TUNE_CONFIGS if configs is None else configs
         out = self.grad_buffers()
         res = {}
-        for fb, bb in shapes:
-            _lib.set_work_shape(fb, bb)
+        for label, cfg in configs.items():
+            self.apply_config(cfg)
             self._forward_launch()
             self._backward_launch(dL_dpix, out)
             t0 = torch.cuda.Event(enable_timing=True)
@@ -300,7 +334,8 @@ class BatchRasterizer:
                 self._backward_launch(dL_dpix, out)
             t1.record()
             t1.synchronize()
-            res[(fb, bb)] = t0.elapsed_time(t1) / iters
+            res[label] = t0.elapsed_time(t1) / iters
         best = min(res, key=res.get)
-        _lib.set_work_shape(*best)
+        self.apply_config(configs[best])
+        self.tuned = best
         return res

@@ -36,6 +36,11 @@ int g_fwd_layout = 2;
 // Tiles per XCD run of the batched one-wave blend kernels (fgs_set_xcd_chunk).
 // 16-32 measured best on the 16 orbit views (blend backward 6.92 -> 6.80 ms).
 int g_xcd_chunk = 32;
+// Batched one-wave blends as persistent launches over a cost-ordered tile
+// queue, runs of g_queue_chunk tiles per XCD (fgs_set_tile_queue; 0: one
+// workgroup per tile in launch order).
+int g_queue_chunk = 0;
+int g_queue_by_cost = 1;
 constexpr int64_t kFwdWaveMinTiles = 12288;
 int fwd_layout_for(int64_t tiles)
 {
@@ -215,6 +220,14 @@ struct Image {
     uint2* ranges;
     uint32_t* vis_last;   // per pixel: end of its blends in the tile's visible list
     uint32_t* tile_vis;   // per tile: length of the visible list
+    // the tile queue of a persistent batched blend (fgs::build_tile_queue):
+    // cost keys, the cost order, the radix pass's histogram and scan
+    // partials, and the eight claim counters
+    uint64_t* qkeysA;
+    uint64_t* qkeysB;
+    uint32_t* qhist;
+    uint32_t* qpart;
+    uint32_t* qhead;
 };
 
 // Per-pixel state of npix pixels and T tiles (a batch concatenates its views).
@@ -227,6 +240,11 @@ Image carve_image_n(void* buf, size_t npix, size_t T, size_t* total)
     im.ranges = c.take<uint2>(T > 0 ? T : 1);
     im.vis_last = c.take<uint32_t>(npix > 0 ? npix : 1);
     im.tile_vis = c.take<uint32_t>(T > 0 ? T : 1);
+    im.qkeysA = c.take<uint64_t>(T > 0 ? T : 1);
+    im.qkeysB = c.take<uint64_t>(T > 0 ? T : 1);
+    im.qhist = c.take<uint32_t>(fgs::radix_hist_words(T));
+    im.qpart = c.take<uint32_t>(scan_words_for_hist(T));
+    im.qhead = c.take<uint32_t>(8);
     if (total) *total = c.off;
     return im;
 }
@@ -387,6 +405,17 @@ int fgs_set_xcd_chunk(int32_t chunk)
     g_xcd_chunk = chunk;
     return FGS_OK;
 }
+
+int fgs_set_tile_queue(int32_t chunk, int32_t by_cost)
+{
+    if (chunk < 0 || chunk > 4096) return fail(FGS_E_INVALID, "queue chunk must be in [0, 4096]%s");
+    if (by_cost != 0 && by_cost != 1) return fail(FGS_E_INVALID, "by_cost must be 0 or 1%s");
+    g_queue_chunk = chunk;
+    g_queue_by_cost = by_cost;
+    return FGS_OK;
+}
+
+int fgs_get_tile_queue(void) { return g_queue_chunk; }
 
 int fgs_get_work_shape(int32_t* out2)
 {
@@ -1485,6 +1514,15 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     }
     {
         ProfScope ps(ST_BLEND_FWD, st);
+        if (g_queue_chunk > 0 && b.fwd_layout == 1 && T > 0) {
+            if ((rc = check(fgs::build_tile_queue(im.ranges, nullptr, (uint32_t)T, (uint32_t)g_queue_chunk,
+                                                  g_queue_by_cost, im.qkeysA, im.qkeysB, im.qhist, im.qpart,
+                                                  im.qhead, st), "batch tile queue")))
+                return rc;
+            b.queue_order = im.qkeysB;
+            b.queue_head = im.qhead;
+            b.queue_chunk = (uint32_t)g_queue_chunk;
+        }
         if ((rc = check(fgs::launch_blend_fwd_batch(b, views, V, T, im.final_T, im.n_contrib, im.vis_last,
                                                     out_color,
                                                     out_point_id, out_weight_pixel, out_point_weight,
@@ -1538,6 +1576,15 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         b.bwd_layout = g_bwd_layout;
         b.xcd_chunk = g_xcd_chunk;
         ProfScope ps(ST_BLEND_BWD, st);
+        if (g_queue_chunk > 0 && b.bwd_layout == 1 && T > 0) {
+            if ((rc = check(fgs::build_tile_queue(im.ranges, im.tile_vis, (uint32_t)T, (uint32_t)g_queue_chunk,
+                                                      g_queue_by_cost, im.qkeysA, im.qkeysB, im.qhist,
+                                                      im.qpart, im.qhead, st), "batch tile queue")))
+                return rc;
+            b.queue_order = im.qkeysB;
+            b.queue_head = im.qhead;
+            b.queue_chunk = (uint32_t)g_queue_chunk;
+        }
         const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P, gm.nvis };
         if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,
                                                     dL_dmeans2D, dconic, dop_view, dL_dcolors,

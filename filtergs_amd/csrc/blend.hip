@@ -794,6 +794,138 @@ __device__ __forceinline__ int find_view(const BatchView* views, int V, uint32_t
     return v;
 }
 
+// ---- persistent launches over a cost-ordered tile queue -------------------
+//
+// Queue items are runs of C consecutive tiles of the batch numbering (the
+// tiles of one run are neighbours in one view and stage many of the same
+// Gaussians).  The runs, heaviest first, are dealt round robin to eight
+// queues: queue q holds the runs at positions q, q + 8, q + 16, ... of the
+// order, and its k-th tile is tile k % C of the run at position
+// 8 (k / C) + q.  Workgroup b starts on queue b % 8, the queue of its XCD
+// (workgroups go to the XCDs round robin), so each XCD works its runs
+// heaviest first with their tiles sharing its L2; when its queue is drained
+// it claims from the other seven in turn.
+__device__ __forceinline__ uint32_t queue_len(uint32_t runs, uint32_t C, uint32_t q)
+{
+    return (runs / 8u + (q < runs % 8u ? 1u : 0u)) * C;
+}
+
+// Lane 0 claims the next item of the queue whose counter is h; the value is
+// lane 0's (the caller reads it with readfirstlane).  (Claiming four items
+// per atomic measured slower: 20.9 against 20.6 ms for the 16-view step.)
+__device__ __forceinline__ uint32_t queue_claim(uint32_t* h)
+{
+    uint32_t k = 0u;
+    if (lane_id() == 0) k = atomicAdd(h, 1u);
+    return k;
+}
+
+// Runs work(tile) for every tile the calling wave claims, until every queue
+// is drained.  The claim of the next item is issued before the current tile
+// is worked, so its latency overlaps the tile's first loads.  All control
+// flow here is wave-uniform.
+template <typename Work>
+__device__ __forceinline__ void drain_queue(const BlendArgs& c, uint32_t n, Work&& work)
+{
+    const uint32_t C = c.queue_chunk;
+    const uint32_t runs = (n + C - 1u) / C;
+    const uint32_t q0 = blockIdx.x & 7u;
+    uint32_t qs = 0u, q = q0;
+    uint32_t len = queue_len(runs, C, q);
+    uint32_t k = queue_claim(c.queue_head + q);
+    for (;;) {
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= len) {
+            if (++qs == 8u) return;
+            q = (q0 + qs) & 7u;
+            len = queue_len(runs, C, q);
+            k = queue_claim(c.queue_head + q);
+            continue;
+        }
+        const uint32_t t = (uint32_t)c.queue_order[(k / C) * 8u + q] * C + k % C;
+        const uint32_t k_next = queue_claim(c.queue_head + q);
+        if (t < n) work(t);   // the last run may be short
+        k = k_next;
+    }
+}
+
+// One lane per run: its cost (the sum of its tiles' visible-list lengths
+// when tile_vis is given, else of their kv counts) in quarter-octave classes,
+// 4 * bit_length(c) + the two bits below the leading one (0 for c = 0, at
+// most 131).  by_cost: key (255 - class) << 32 | run into keys, so a stable
+// sort on the class puts the heaviest runs first; else the run itself into
+// order (launch order).  The first eight lanes clear the claim counters.
+__global__ __launch_bounds__(256) void k_tile_cost_keys(const uint2* __restrict__ ranges,
+                                                        const uint32_t* __restrict__ tile_vis, uint32_t T,
+                                                        uint32_t C, int by_cost, uint64_t* __restrict__ keys,
+                                                        uint64_t* __restrict__ order, uint32_t* __restrict__ head)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t runs = (T + C - 1u) / C;
+    if (r < 8u) head[r] = 0u;
+    if (r >= runs) return;
+    if (!by_cost) {
+        order[r] = r;
+        return;
+    }
+    uint64_t c64 = 0u;
+    const uint32_t t1 = min(T, (r + 1u) * C);
+    for (uint32_t t = r * C; t < t1; ++t) {
+        if (tile_vis) {
+            c64 += tile_vis[t];
+        } else {
+            const uint2 rg = ranges[t];
+            c64 += rg.y - rg.x;
+        }
+    }
+    const uint32_t c = c64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)c64;
+    uint32_t cls = 0u;
+    if (c != 0u) {
+        const uint32_t bl = 32u - (uint32_t)__clz(c);
+        const uint32_t mant = bl >= 3u ? (c >> (bl - 3u)) & 3u : (c << (3u - bl)) & 3u;
+        cls = 4u * bl + mant;
+    }
+    keys[r] = ((uint64_t)(255u - cls) << 32) | r;
+}
+
+hipError_t build_tile_queue(const uint2* ranges, const uint32_t* tile_vis, uint32_t T, uint32_t C, int by_cost,
+                            uint64_t* keysA, uint64_t* keysB, uint32_t* hist, uint32_t* partials,
+                            uint32_t* head, hipStream_t s)
+{
+    if (C == 0u) return hipErrorInvalidValue;
+    const uint32_t runs = (T + C - 1u) / C;
+    const uint32_t blocks = (runs + 8u + 255u) / 256u;
+    hipLaunchKernelGGL(k_tile_cost_keys, dim3(blocks), dim3(256), 0, s, ranges, tile_vis, T, C, by_cost, keysA,
+                       keysB, head);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || runs == 0 || !by_cost) return e;
+    return radix_sort_u64(keysA, keysB, runs, 32, 1, hist, partials, s);
+}
+
+// Workgroups of a persistent launch: as many one-wave workgroups as the
+// device holds at once (occupancy x CUs, a multiple of 8), at most one per
+// tile rounded up to 8.  Cached per device and kernel.
+static uint32_t persistent_grid(const void* fn, int slot, uint32_t T)
+{
+    constexpr int kDevs = 64;
+    static uint32_t cache[2][kDevs];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevs) dev = 0;
+    uint32_t g = cache[slot][dev];
+    if (g == 0u) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, FGS_WAVE, 0) != hipSuccess || per_cu < 1)
+            per_cu = 8;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        g = (uint32_t)per_cu * (uint32_t)cus / 8u * 8u;
+        if (g < 8u) g = 8u;
+        cache[slot][dev] = g;
+    }
+    const uint32_t cap = (T + 7u) / 8u * 8u;
+    return g < cap ? g : cap;
+}
+
 // The view-dependent fields of a BlendArgs; per-pixel pointers are offset by
 // the caller with pix_base.
 __device__ __forceinline__ BlendArgs bind_blend(const BlendArgs& common, const BatchView& vw, int v)
@@ -846,6 +978,27 @@ __global__ __launch_bounds__(FGS_WAVE) void k_blend_fwd_batch_wave(BlendArgs com
                         nullptr);
 }
 
+// The same tiles as k_blend_fwd_batch_wave, claimed from the cost-ordered
+// queue by a persistent grid.  Each tile is worked exactly as there, so the
+// outputs are the same bits.
+__global__ __launch_bounds__(FGS_WAVE) __attribute__((amdgpu_waves_per_eu(5))) void k_blend_fwd_batch_queue(BlendArgs common, const BatchView* views,
+                                                                    int V, uint32_t T_total, float* final_T,
+                                                                    uint32_t* n_contrib, uint32_t* vis_last,
+                                                                    float* out_color, int* out_point_id,
+                                                                    float* out_weight_pixel,
+                                                                    float* point_weight)
+{
+    drain_queue(common, T_total, [&](uint32_t b) {
+        const int v = find_view(views, V, b);
+        const BatchView& vw = views[v];
+        const BlendArgs a = bind_blend(common, vw, v);
+        const size_t pb = vw.pix_base;
+        blend_fwd_tile_wave(a, (int)(b - vw.tile_base), final_T + pb, n_contrib + pb, vis_last + pb,
+                            out_color + 3 * pb, out_point_id + pb, out_weight_pixel + pb, point_weight,
+                            nullptr, nullptr);
+    });
+}
+
 hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* views, int V, int T_total,
                                   float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
                                   float* out_color, int* out_point_id, float* out_weight_pixel,
@@ -853,7 +1006,13 @@ hipError_t launch_blend_fwd_batch(const BlendArgs& common, const BatchView* view
 {
     if (T_total == 0) return hipSuccess;
     prof_kernel(false, true, s);
-    if (common.fwd_layout == 1)
+    if (common.fwd_layout == 1 && common.queue_order) {
+        const uint32_t grid =
+            persistent_grid(reinterpret_cast<const void*>(&k_blend_fwd_batch_queue), 0, (uint32_t)T_total);
+        hipLaunchKernelGGL(k_blend_fwd_batch_queue, dim3(grid), dim3(FGS_WAVE), 0, s, common, views, V,
+                           (uint32_t)T_total, final_T, n_contrib, vis_last, out_color, out_point_id,
+                           out_weight_pixel, point_weight);
+    } else if (common.fwd_layout == 1)
         hipLaunchKernelGGL(k_blend_fwd_batch_wave, dim3(T_total), dim3(FGS_WAVE), 0, s, common, views, V,
                            final_T, n_contrib, vis_last, out_color, out_point_id, out_weight_pixel,
                            point_weight);
@@ -1541,6 +1700,24 @@ __global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_batch_wave(BlendArgs com
                         exact);
 }
 
+// The same tiles as k_blend_bwd_batch_wave, claimed from the cost-ordered
+// queue by a persistent grid.
+__global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_batch_queue(BlendArgs common, const BatchView* views,
+                                                                    int V, uint32_t T_total,
+                                                                    const float* final_Ts,
+                                                                    const uint32_t* vis_last,
+                                                                    const float* dL_dpix, int exact)
+{
+    drain_queue(common, T_total, [&](uint32_t b) {
+        const int v = find_view(views, V, b);
+        const BatchView& vw = views[v];
+        const BlendArgs a = bind_blend(common, vw, v);
+        const size_t pb = vw.pix_base;
+        blend_bwd_tile_wave(a, (int)(b - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb,
+                            exact);
+    });
+}
+
 template <int BATCH>
 __global__ __launch_bounds__(FGS_BLOCK) void k_blend_bwd(BlendArgs a, const float* final_Ts,
                                                          const uint32_t* vis_last, const float* dL_dpix,
@@ -1574,7 +1751,12 @@ hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg,
 #define FGS_BWD(BATCH)                                                                               \
     hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, b, views, V, \
                        final_T, vis_last, dL_dpix, exact)
-        if (b.bwd_layout == 1)
+        if (b.bwd_layout == 1 && b.queue_order) {
+            const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(&k_blend_bwd_batch_queue), 1,
+                                                  (uint32_t)T_total);
+            hipLaunchKernelGGL(k_blend_bwd_batch_queue, dim3(grid), dim3(FGS_WAVE), 0, s, b, views, V,
+                               (uint32_t)T_total, final_T, vis_last, dL_dpix, exact);
+        } else if (b.bwd_layout == 1)
             hipLaunchKernelGGL(k_blend_bwd_batch_wave, dim3(T_total), dim3(FGS_WAVE), 0, s, b, views, V,
                                final_T, vis_last, dL_dpix, exact);
         else if (b.bwd_batch == 256) FGS_BWD(256);

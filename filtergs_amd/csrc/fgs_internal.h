@@ -140,7 +140,27 @@ struct BlendArgs {
     // batched path: kv values are elements v * P + i; point ids report i
     uint32_t id_stride;        // P of a batch, 0 for one view
     uint32_t id_base;          // v * P (set per view)
+    // Batched one-wave blend as a persistent launch: runs of queue_chunk
+    // consecutive tiles, in the order of queue_order (run = low 32 bits),
+    // dealt round robin to eight queues, one per XCD, each claimed tile by
+    // tile through its counter in queue_head[8]; a workgroup drains the
+    // queue of its XCD, then the others.  Null queue_order: one workgroup per
+    // tile.
+    const uint64_t* queue_order;
+    uint32_t* queue_head;
+    uint32_t queue_chunk;
 };
+
+// The work queue of a persistent batched blend launch over T tiles in runs
+// of C consecutive tiles: with by_cost, keys (255 - cost class) << 32 | run
+// in keysA and one stable radix pass on the class into keysB, the heaviest
+// runs first; without, the runs in launch order in keysB.  The eight claim
+// counters in head are cleared.  The cost of a tile is its visible-list
+// length when tile_vis is given (backward), else its kv count (forward);
+// classes are quarter octaves of a run's summed cost.
+hipError_t build_tile_queue(const uint2* ranges, const uint32_t* tile_vis, uint32_t T, uint32_t C, int by_cost,
+                            uint64_t* keysA, uint64_t* keysB, uint32_t* hist, uint32_t* partials,
+                            uint32_t* head, hipStream_t s);
 
 // launchers (return hipError_t of the launch)
 hipError_t launch_preprocess_fwd(const PreArgs& a, const PreOut& o, hipStream_t s);

@@ -510,6 +510,18 @@ int fgs_get_forward_layout(void);
  * of C consecutive tiles, so that neighbouring tiles share an L2 (default
  * 32).  Outputs do not depend on it. */
 int fgs_set_xcd_chunk(int32_t chunk);
+/* Batched one-wave blends as persistent launches (process-wide; no
+ * counterpart in the reference, which renders one view per launch,
+ * LoG/render/renderer.py:234): with chunk C > 0 the forward and the
+ * backward each run one grid of as many one-wave workgroups as the GPU
+ * holds at once, which claim the batch's (view, tile) items tile by tile
+ * from a work queue of runs of C consecutive tiles, dealt round robin to
+ * one queue per XCD.  by_cost = 1 orders the runs by estimated cost,
+ * heaviest first (kv count for the forward, visible-list length for the
+ * backward); 0 keeps the launch order.  chunk 0 (the default) launches one
+ * workgroup per tile.  Outputs do not depend on it. */
+int fgs_set_tile_queue(int32_t chunk, int32_t by_cost);
+int fgs_get_tile_queue(void);
 const char* fgs_last_error(void);
 int fgs_abi_version(void);
 /* OGFS log(lambda_G) for the given epsilons (forward.cu:346-351). */

@@ -312,32 +312,45 @@ def _np(t):
 
 
 def test_batch_tile_order_does_not_change_outputs():
-    """The XCD run length of the batched one-wave blend kernels (0 = launch
-    order) moves work between L2s only: images and per-view outputs are the
-    same bits, gradients equal up to float summation order."""
+    """The tile order of the batched one-wave blend kernels moves work
+    between L2s and waves only: the XCD run length (0 = launch order) and the
+    persistent launches over the cost-ordered queue (several run lengths, on
+    more tiles than the persistent grid holds waves, so waves claim many
+    tiles) give the same image, point-id and weight bits, and gradients equal
+    up to float summation order.  The outputs are poisoned before each run,
+    so a tile the queue skipped or worked twice would show."""
     from filtergs_amd import _lib
     g, _ = small_scene(n=20000, seed=241, dist=5.0)
     base = _cams()
-    cams = [make_camera(256 + 32 * v, 144 + 16 * v, center=(4.0 * np.cos(v), 4.0 * np.sin(v), 0.5),
+    cams = [make_camera(512 + 32 * v, 288 + 16 * v, center=(4.0 * np.cos(v), 4.0 * np.sin(v), 0.5),
                         fovx_deg=55.0) for v in range(12)] + base
     P = g["means3D"].shape[0]
     br = BatchRasterizer(cams, P, bg=(0.1, 0.2, 0.3))
+    assert br.sizes.tiles > 10000
     br.bind(_t(g["means3D"]), _t(g["opacities"]), _t(g["scales"]), _t(g["rotations"]), shs=_t(g["shs"]))
     dL = _t(np.random.default_rng(242).normal(size=(3 * int(br.sizes.pixels),)).astype(np.float32))
     res = []
     _lib.set_forward_layout(1)      # the one-wave forward at any tile count
     _lib.set_backward_layout(1)
     try:
-        for chunk in (0, 1, 32, 5):
+        for chunk, queue, by_cost in ((0, 0, 1), (1, 0, 1), (32, 0, 1), (5, 0, 1), (32, 1, 1), (32, 32, 1),
+                                      (0, 7, 1), (0, 4096, 1), (0, 1, 0), (0, 13, 0)):
             _lib.set_xcd_chunk(chunk)
+            _lib.set_tile_queue(queue, by_cost)
+            br.color.fill_(float("nan"))
+            br.point_id.fill_(-7)
+            br.weight_pixel.fill_(-1.0)
             br.forward()
             gr = br.backward(dL)
             res.append((br.color.clone(), br.point_id.clone(), br.point_weight.clone(),
-                        {k: v.clone() for k, v in gr.items() if v is not None}))
+                        br.weight_pixel.clone(), {k: v.clone() for k, v in gr.items() if v is not None}))
     finally:
         _lib.set_xcd_chunk(32)
+        _lib.set_tile_queue(0)
         _lib.set_forward_layout(2)
+    assert not torch.isnan(res[0][0]).any() and int((res[0][1] == -7).sum()) == 0
     for r in res[1:]:
-        assert torch.equal(r[0], res[0][0]) and torch.equal(r[1], res[0][1]) and torch.equal(r[2], res[0][2])
-        for k in r[3]:
-            assert _rel(r[3][k], res[0][3][k]) <= 1e-4, k
+        for a, b in zip(r[:4], res[0][:4]):
+            assert torch.equal(a, b)
+        for k in r[4]:
+            assert _rel(r[4][k], res[0][4][k]) <= 1e-4, k
