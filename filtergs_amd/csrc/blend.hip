@@ -153,31 +153,62 @@ __device__ __forceinline__ float wave_total(float x)
     return sum_xor32(sum_xor16(x));
 }
 
-// Wave maximum of a non-negative value (every lane receives it).
-__device__ __forceinline__ float wave_maxf(float x)
+// Maximum of two non-negative floats on their bit patterns (the order of
+// non-negative IEEE floats is that of their bits): one integer max, no NaN
+// quieting.
+__device__ __forceinline__ float max_nonneg(float a, float b)
 {
-    x = fmaxf(x, dpp<0x128>(x));
-    x = fmaxf(x, dpp<0x124>(x));
-    x = fmaxf(x, dpp<0x122>(x));
-    x = fmaxf(x, dpp<0x121>(x));
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+
+// Wave maximum of a non-negative value (every lane receives it).
+__device__ __forceinline__ float wave_maxf(float xf)
+{
+    uint32_t x = __float_as_uint(xf);
+    x = max(x, dpp_u<0x128>(x));
+    x = max(x, dpp_u<0x124>(x));
+    x = max(x, dpp_u<0x122>(x));
+    x = max(x, dpp_u<0x121>(x));
     {
-        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-        x = fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        x = max((uint32_t)r[0], (uint32_t)r[1]);
     }
     {
-        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-        x = fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        x = max((uint32_t)r[0], (uint32_t)r[1]);
     }
-    return x;
+    return __uint_as_float(x);
 }
 
 // Exponent of the Gaussian in log2 units at offset (dx, dy) from its mean,
-// from the staged terms (a', b') = g0.zw and c' = g1.x.  Shared by the
-// forward and the backward so that both take the same alpha decisions.
-__device__ __forceinline__ float exponent2(float4 g0, float2 g1, float dx, float dy)
+// from the staged terms (a', b') = g0.zw and c' = g1.x:
+// dy (c' dy + b' dx) + a' dx^2, two fused steps per pixel row once the
+// column terms b' dx and a' dx^2 are formed (the one-wave kernels form them
+// once per Gaussian and lane: exponent2_col / exponent2_row give the same
+// bits).  Shared by the forward and the backward so that both take the same
+// alpha decisions.
+struct ExpCol {
+    float bdx;    // b' dx
+    float adx2;   // a' dx^2
+};
+__device__ __forceinline__ ExpCol exponent2_col(float4 g0, float dx)
 {
 #pragma clang fp contract(off)
-    return fmaf(g0.z * dx, dx, fmaf(g1.x * dy, dy, (g0.w * dx) * dy));
+    return ExpCol{ g0.w * dx, (g0.z * dx) * dx };
+}
+__device__ __forceinline__ float exponent2_row(ExpCol c, float2 g1, float dy)
+{
+    return fmaf(dy, fmaf(g1.x, dy, c.bdx), c.adx2);
+}
+__device__ __forceinline__ float exponent2(float4 g0, float2 g1, float dx, float dy)
+{
+    return exponent2_row(exponent2_col(g0, dx), g1, dy);
 }
 
 // ---------------------------------------------------------------------------
@@ -561,6 +592,18 @@ __device__ __forceinline__ StageRec load_stage(const BlendArgs& a, uint32_t g)
 // images, n_contrib, point ids and weights are the same bits.  point_weight
 // takes one DPP wave maximum per (tile, Gaussian) over the four strips.
 constexpr int kFwdWaveBatch = FGS_WAVE;
+constexpr float kFarRow = 1e30f;
+constexpr int kLiveEvery = 8;
+
+// Strips of the wave with a pixel that has not finished (a bit per strip).
+__device__ __forceinline__ uint32_t live_strips(const float* pfy)
+{
+    uint32_t lv = 0u;
+#pragma unroll
+    for (int k = 0; k < kStrips; ++k)
+        lv |= (__builtin_amdgcn_ballot_w64(pfy[k] != kFarRow) != 0u ? 1u : 0u) << k;
+    return lv;
+}
 
 __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile, float* __restrict__ final_T,
                                                     uint32_t* __restrict__ n_contrib,
@@ -585,26 +628,27 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
     __shared__ FwdSlot s_slot[kFwdWaveBatch];
     __shared__ uint32_t s_vidx[kFwdWaveBatch];
 
+    // A finished pixel (saturated, or outside the image) moves its row to
+    // kFarRow: dy is then of order -1e30 and the exponent -inf (or +inf for
+    // a conic that is not positive definite), so alpha < 1/255 or power > 0
+    // and the pixel neither blends nor stops again.
     float pfy[kStrips], T[kStrips], C0[kStrips], C1[kStrips], C2[kStrips], wmax[kStrips];
-    int max_id[kStrips], last_slot[kStrips];
+    int max_id[kStrips];
     uint32_t last[kStrips], vlast[kStrips];
-    bool done[kStrips];
     uint32_t live = 0u;   // strips with a pixel not yet saturated (wave-uniform)
 #pragma unroll
     for (int k = 0; k < kStrips; ++k) {
         const int py = py0 + 4 * k;
-        pfy[k] = (float)py;
+        pfy[k] = (px < a.W && py < a.H) ? (float)py : kFarRow;
         asm volatile("" : "+v"(pfy[k]));
         T[k] = 1.0f;
         C0[k] = C1[k] = C2[k] = 0.f;
         wmax[k] = 0.f;
         max_id[k] = -1;
-        last_slot[k] = -1;
         last[k] = 0u;
         vlast[k] = 0u;
-        done[k] = !(px < a.W && py < a.H);
-        live |= (__any(!done[k]) ? 1u : 0u) << k;
     }
+    live = live_strips(pfy);
     uint32_t vis_n = 0;   // visible-list entries so far (wave-uniform)
 
     // Staging is software-pipelined: while round r is worked, the records of
@@ -635,32 +679,35 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
         s_slot[lane].kc = 0u;
         __syncthreads();
         for (int j = 0; j < cnt; ++j) {
+            // strips whose pixels have all finished drop out of the walk,
+            // checked every kLiveEvery Gaussians and after each round
+            if (j != 0 && (j & (kLiveEvery - 1)) == 0) live = live_strips(pfy);   // wave-uniform
             const uint32_t sb = __builtin_amdgcn_readlane(bits, j) & live;
             if (sb == 0u) continue;   // wave-uniform
             FwdSlot* const sp = &s_slot[j];
             const float4 g0 = sp->g0;
             const float2 g1 = sp->g1;
-            const float dx = g0.x - pfx;
+            const ExpCol ec = exponent2_col(g0, g0.x - pfx);
             float wl = 0.f, wsum = 0.f;
             uint32_t used = 0u, nbl = 0u;
-            bool stopped = false;
 #pragma unroll
             for (int k = 0; k < kStrips; ++k) {
                 if (!((sb >> k) & 1u)) continue;   // wave-uniform
                 const float dy = g0.y - pfy[k];
-                const float p2 = exponent2(g0, g1, dx, dy);
+                const float p2 = exponent2_row(ec, g1, dy);
                 const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
                 const float test_T = T[k] * (1.0f - alpha);
-                const bool pass = !done[k] && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
-                const bool stop = pass && test_T < 0.0001f;
-                const bool blend = pass && !stop;
-                done[k] = done[k] || stop;
-                stopped = stopped || stop;
-                const uint64_t bl = __ballot(blend);
-                if (bl == 0u) continue;   // wave-uniform
-                used |= 1u << k;
+                const bool hit = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
+                const bool blend = hit && test_T >= 0.0001f;
+                // the ballot of blend from those of its compares (the ballot of
+                // a compare is its lane mask; of an "and", a materialised 0/1)
+                const uint64_t bl = __builtin_amdgcn_ballot_w64(!(p2 > 0.0f)) &
+                                    __builtin_amdgcn_ballot_w64(!(alpha < 1.0f / 255.0f)) &
+                                    __builtin_amdgcn_ballot_w64(test_T >= 0.0001f);
+                pfy[k] = (hit && test_T < 0.0001f) ? kFarRow : pfy[k];   // saturated: finished
+                used |= (bl != 0u ? 1u : 0u) << k;
                 nbl += (uint32_t)__popcll(bl);
-                if (blend) {
+                if (blend) {   // skipped by the whole wave when no lane blends
                     const float4 col = sp->col;
                     const float w = alpha * T[k];
                     C0[k] = fmaf(col.x, w, C0[k]);
@@ -670,18 +717,11 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
                         wmax[k] = w;
                         max_id[k] = (int)__float_as_uint(col.w);
                     }
-                    wl = fmaxf(wl, w);
+                    wl = max_nonneg(wl, w);
                     wsum += w;
                     T[k] = test_T;
-                    last_slot[k] = j;
                     last[k] = (uint32_t)(k0 + j + 1);
                 }
-            }
-            if (__any(stopped)) {   // wave-uniform
-                uint32_t lv = 0u;
-#pragma unroll
-                for (int k = 0; k < kStrips; ++k) lv |= (__any(!done[k]) ? 1u : 0u) << k;
-                live = lv;
             }
             if (used != 0u) {   // wave-uniform
                 const float pw = wave_maxf(wl);
@@ -698,6 +738,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
                 }
             }
         }
+        live = live_strips(pfy);
         __syncthreads();
         uint32_t used = 0u, g = 0u;
         if (lane < cnt) {
@@ -721,12 +762,10 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
         }
         vis_n += (uint32_t)__popcll(vb);
         __syncthreads();
+        // a pixel that blended in this round last did so at slot last - 1 - k0
 #pragma unroll
         for (int k = 0; k < kStrips; ++k) {
-            if (last_slot[k] >= 0) {
-                vlast[k] = s_vidx[last_slot[k]] + 1u;
-                last_slot[k] = -1;
-            }
+            if (last[k] > (uint32_t)k0) vlast[k] = s_vidx[last[k] - 1u - (uint32_t)k0] + 1u;
         }
         cur = nxt;
     }
@@ -908,7 +947,7 @@ hipError_t build_tile_queue(const uint2* ranges, const uint32_t* tile_vis, uint3
 static uint32_t persistent_grid(const void* fn, int slot, uint32_t T)
 {
     constexpr int kDevs = 64;
-    static uint32_t cache[2][kDevs];
+    static uint32_t cache[3][kDevs];   // forward, backward, exact-mode backward
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevs) dev = 0;
     uint32_t g = cache[slot][dev];
@@ -1430,10 +1469,11 @@ __device__ __forceinline__ void publish_pair(const BlendArgs& a, int tx, int ty,
 // per-pixel arithmetic is that of blend_bwd_tile, term for term.
 constexpr int kWaveBatch = FGS_WAVE;
 
+template <bool EXACT>
 __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile,
-                                                    const float* __restrict__ final_Ts,
-                                                    const uint32_t* __restrict__ vis_last,
-                                                    const float* __restrict__ dL_dpix, int exact)
+                                                      const float* __restrict__ final_Ts,
+                                                      const uint32_t* __restrict__ vis_last,
+                                                      const float* __restrict__ dL_dpix)
 {
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int lane = lane_id();
@@ -1509,6 +1549,7 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
             const float4 g0 = sp->g0;
             const float2 g1 = sp->g1;
             const float dx = g0.x - pfx;
+            const ExpCol ec = exponent2_col(g0, dx);
             // per lane the strips share the column, so dx: the strips add
             // v, v dy and v dy^2, and v dx, v dx^2, v dx dy follow per entry
             float v[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
@@ -1518,8 +1559,10 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
             for (int k = 0; k < kStrips; ++k) {
                 if (!((sb >> k) & 1u)) continue;   // wave-uniform
                 const float dy = g0.y - pfy[k];
-                const float p2 = exponent2(g0, g1, dx, dy);
-                const float G = __builtin_amdgcn_exp2f(fminf(p2, 0.0f));
+                const float p2 = exponent2_row(ec, g1, dy);
+                // G is only used where the pair acts (power <= 0); elsewhere
+                // it may be inf, and every use of it there is a select
+                const float G = __builtin_amdgcn_exp2f(p2);
                 const float alpha_raw = g1.y * G;
                 const float alpha = fminf(0.99f, alpha_raw);
                 const bool act = j > thr[k] && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
@@ -1531,7 +1574,7 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
                 const float c_d = col.x * dp0[k] + col.y * dp1[k] + col.z * dp2[k];
                 const float acc_n = acc_d[k] + last_alpha[k] * (last_d[k] - acc_d[k]);
                 const float dLa = (c_d - acc_n) * Tn + inv * tfb[k];
-                const bool flows = act && !(exact && alpha_raw > 0.99f);
+                const bool flows = act && !(EXACT && alpha_raw > 0.99f);
                 const float dch = act ? alpha * Tn : 0.f;
                 const float vop = flows ? G * dLa : 0.f;
                 v[0] += dch * dp0[k];
@@ -1679,42 +1722,45 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
     return hipGetLastError();
 }
 
+// The one-wave backward kernels come in one instance per gradient mode
+// (EXACT): one inlined copy each keeps their register count, and so their
+// occupancy, that of a single path.
+template <bool EXACT>
 __global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_wave(BlendArgs a, const float* final_Ts,
-                                                             const uint32_t* vis_last, const float* dL_dpix,
-                                                             int exact)
+                                                             const uint32_t* vis_last, const float* dL_dpix)
 {
-    blend_bwd_tile_wave(a, blockIdx.x, final_Ts, vis_last, dL_dpix, exact);
+    blend_bwd_tile_wave<EXACT>(a, blockIdx.x, final_Ts, vis_last, dL_dpix);
 }
 
+template <bool EXACT>
 __global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_batch_wave(BlendArgs common, const BatchView* views,
                                                                    int V, const float* final_Ts,
                                                                    const uint32_t* vis_last,
-                                                                   const float* dL_dpix, int exact)
+                                                                   const float* dL_dpix)
 {
     const uint32_t b = xcd_tile(blockIdx.x, gridDim.x, (uint32_t)common.xcd_chunk);
     const int v = find_view(views, V, b);
     const BatchView& vw = views[v];
     const BlendArgs a = bind_blend(common, vw, v);
     const size_t pb = vw.pix_base;
-    blend_bwd_tile_wave(a, (int)(b - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb,
-                        exact);
+    blend_bwd_tile_wave<EXACT>(a, (int)(b - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb);
 }
 
 // The same tiles as k_blend_bwd_batch_wave, claimed from the cost-ordered
 // queue by a persistent grid.
+template <bool EXACT>
 __global__ __launch_bounds__(FGS_WAVE) void k_blend_bwd_batch_queue(BlendArgs common, const BatchView* views,
                                                                     int V, uint32_t T_total,
                                                                     const float* final_Ts,
                                                                     const uint32_t* vis_last,
-                                                                    const float* dL_dpix, int exact)
+                                                                    const float* dL_dpix)
 {
     drain_queue(common, T_total, [&](uint32_t b) {
         const int v = find_view(views, V, b);
         const BatchView& vw = views[v];
         const BlendArgs a = bind_blend(common, vw, v);
         const size_t pb = vw.pix_base;
-        blend_bwd_tile_wave(a, (int)(b - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb,
-                            exact);
+        blend_bwd_tile_wave<EXACT>(a, (int)(b - vw.tile_base), final_Ts + pb, vis_last + pb, dL_dpix + 3 * pb);
     });
 }
 
@@ -1752,13 +1798,21 @@ hipError_t launch_blend_bwd_batch(const BlendArgs& common, const GradGather& gg,
     hipLaunchKernelGGL(k_blend_bwd_batch<BATCH>, dim3(T_total), dim3(FGS_BLOCK), 0, s, b, views, V, \
                        final_T, vis_last, dL_dpix, exact)
         if (b.bwd_layout == 1 && b.queue_order) {
-            const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(&k_blend_bwd_batch_queue), 1,
-                                                  (uint32_t)T_total);
-            hipLaunchKernelGGL(k_blend_bwd_batch_queue, dim3(grid), dim3(FGS_WAVE), 0, s, b, views, V,
-                               (uint32_t)T_total, final_T, vis_last, dL_dpix, exact);
-        } else if (b.bwd_layout == 1)
-            hipLaunchKernelGGL(k_blend_bwd_batch_wave, dim3(T_total), dim3(FGS_WAVE), 0, s, b, views, V,
-                               final_T, vis_last, dL_dpix, exact);
+            const void* k = exact ? reinterpret_cast<const void*>(&k_blend_bwd_batch_queue<true>)
+                                  : reinterpret_cast<const void*>(&k_blend_bwd_batch_queue<false>);
+            const uint32_t grid = persistent_grid(k, exact ? 2 : 1, (uint32_t)T_total);
+            if (exact)
+                hipLaunchKernelGGL(k_blend_bwd_batch_queue<true>, dim3(grid), dim3(FGS_WAVE), 0, s, b, views, V,
+                                   (uint32_t)T_total, final_T, vis_last, dL_dpix);
+            else
+                hipLaunchKernelGGL(k_blend_bwd_batch_queue<false>, dim3(grid), dim3(FGS_WAVE), 0, s, b, views, V,
+                                   (uint32_t)T_total, final_T, vis_last, dL_dpix);
+        } else if (b.bwd_layout == 1 && exact)
+            hipLaunchKernelGGL(k_blend_bwd_batch_wave<true>, dim3(T_total), dim3(FGS_WAVE), 0, s, b, views, V,
+                               final_T, vis_last, dL_dpix);
+        else if (b.bwd_layout == 1)
+            hipLaunchKernelGGL(k_blend_bwd_batch_wave<false>, dim3(T_total), dim3(FGS_WAVE), 0, s, b, views, V,
+                               final_T, vis_last, dL_dpix);
         else if (b.bwd_batch == 256) FGS_BWD(256);
         else if (b.bwd_batch == 64) FGS_BWD(64);
         else FGS_BWD(128);
@@ -1778,9 +1832,12 @@ hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const floa
 #define FGS_BWD(BATCH)                                                                              \
     hipLaunchKernelGGL(k_blend_bwd<BATCH>, dim3(T), dim3(FGS_BLOCK), 0, s, b, final_T, vis_last, dL_dpix, \
                        exact)
-        if (b.bwd_layout == 1)
-            hipLaunchKernelGGL(k_blend_bwd_wave, dim3(T), dim3(FGS_WAVE), 0, s, b, final_T, vis_last, dL_dpix,
-                               exact);
+        if (b.bwd_layout == 1 && exact)
+            hipLaunchKernelGGL(k_blend_bwd_wave<true>, dim3(T), dim3(FGS_WAVE), 0, s, b, final_T, vis_last,
+                               dL_dpix);
+        else if (b.bwd_layout == 1)
+            hipLaunchKernelGGL(k_blend_bwd_wave<false>, dim3(T), dim3(FGS_WAVE), 0, s, b, final_T, vis_last,
+                               dL_dpix);
         else if (b.bwd_batch == 256) FGS_BWD(256);
         else if (b.bwd_batch == 64) FGS_BWD(64);
         else FGS_BWD(128);
