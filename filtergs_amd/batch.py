@@ -246,9 +246,13 @@ class BatchRasterizer:
         n = int(lens.sum())
         pos = torch.arange(n, device=r.device) - torch.repeat_interleave(starts, lens) + \
             torch.repeat_interleave(r[:, 0], lens)
-        pairs = self.binning[int(offs[0]):int(offs[0]) + 8 * K].view(torch.int64)[pos]
-        tile = (pairs >> 32) & 0xFFFFFFFF
-        elem = pairs & 0xFFFFFFFF
+        # tile keys (view-local tile << 6 | view) and elements, u32 arrays of
+        # the capacity's length one after the other
+        k0 = int(offs[0])
+        keys = self.binning[k0:k0 + 4 * K].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        vals = self.binning[k0 + 4 * self.K_cap:k0 + 4 * self.K_cap + 4 * K].view(torch.int32)
+        tile = keys[pos] >> 6
+        elem = vals.to(torch.int64)[pos] & 0xFFFFFFFF
         ranges = torch.stack([starts, starts + lens], 1) * (lens > 0)[:, None]
         return tile, elem - v * self.P, ranges
 

@@ -214,6 +214,26 @@ Binning carve_binning(void* buf, int64_t K, size_t* total)
     return b;
 }
 
+// A batch's kv pairs in the binning buffer: tile keys and element values as
+// u32 arrays, A in the first pair array's bytes, B in the second's.
+struct BatchPairs {
+    uint32_t* keysA;
+    uint32_t* valsA;
+    uint32_t* keysB;
+    uint32_t* valsB;
+};
+
+BatchPairs batch_pairs(const Binning& bn, int64_t K)
+{
+    const size_t n = (size_t)(K > 0 ? K : 1);
+    BatchPairs p;
+    p.keysA = reinterpret_cast<uint32_t*>(bn.pairsA);
+    p.valsA = p.keysA + n;
+    p.keysB = reinterpret_cast<uint32_t*>(bn.pairsB);
+    p.valsB = p.keysB + n;
+    return p;
+}
+
 struct Image {
     float* final_T;
     uint32_t* n_contrib;
@@ -622,6 +642,7 @@ static int render_impl(const fgs_raster_settings* s, int P, const float* colors_
     b.walk = g_walk;
     b.fwd_layout = fwd_layout_for((int64_t)T);
     b.xcd_chunk = g_xcd_chunk;
+    const uint64_t* sorted_pairs = nullptr;
     if (K > 0) {
         if (!binning || !geom || !radii) return fail(FGS_E_INVALID, "null binning/geometry buffer%s");
         Binning bn = carve_binning(binning, K, nullptr);
@@ -642,15 +663,15 @@ static int render_impl(const fgs_raster_settings* s, int P, const float* colors_
                 return rc;
         }
         if ((rc = sync_check(s, st, "binning"))) return rc;
-        b.pairs = bn.pairsA;
+        sorted_pairs = bn.pairsA;
+        b.pair_ids = reinterpret_cast<const uint32_t*>(bn.pairsA);
+        b.pair_shift = 1;
         b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);   // free once sorted
         b.kv_strips = bn.strips;
-    } else {
-        b.pairs = nullptr;
     }
     {
         ProfScope ps(ST_RANGES, st);
-        if ((rc = check(fgs::launch_ranges(b.pairs, (size_t)K, im.ranges, T, st), "ranges"))) return rc;
+        if ((rc = check(fgs::launch_ranges(sorted_pairs, (size_t)K, im.ranges, T, st), "ranges"))) return rc;
     }
     if (stats) {
         if (!(kpc_used_thresh > 0.f)) return fail(FGS_E_INVALID, "KPC used threshold must be > 0%s");
@@ -770,7 +791,8 @@ static int backward_impl(const fgs_raster_settings* s, const fgs_gaussians* g, c
         fgs::BlendArgs b{};
         b.ranges = im.ranges;
         b.tile_vis = im.tile_vis;
-        b.pairs = bn.pairsA;
+        b.pair_ids = reinterpret_cast<const uint32_t*>(bn.pairsA);
+        b.pair_shift = 1;
         b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);
         b.kv_strips = bn.strips;
         b.recs = gm.recs;
@@ -1302,6 +1324,9 @@ static int batch_validate(const fgs_batch_view* views, int32_t V, int32_t P)
             return fail(FGS_E_INVALID, "image size must be positive%s");
         if ((w.image_width + 15) / 16 > 65535 || (w.image_height + 15) / 16 > 65535)
             return fail(FGS_E_INVALID, "image larger than 65535 tiles per side%s");
+        if ((uint64_t)((w.image_width + 15) / 16) * (uint64_t)((w.image_height + 15) / 16) >=
+            (uint64_t)fgs::kBatchMaxViewTiles)
+            return fail(FGS_E_INVALID, "a batch view holds at most 2^26 tiles%s");
     }
     return FGS_OK;
 }
@@ -1382,7 +1407,7 @@ int fgs_batch_offsets(const fgs_batch_sizes* sz, int64_t K_cap, size_t* out4)
     static char probe[256];
     Binning bn = carve_binning(probe, K_cap, nullptr);
     Image im = carve_image_n(probe, (size_t)sz->pixels, (size_t)sz->tiles, nullptr);
-    out4[0] = (size_t)((char*)bn.pairsA - probe);
+    out4[0] = (size_t)((char*)batch_pairs(bn, K_cap).keysA - probe);
     out4[1] = (size_t)((char*)im.ranges - probe);
     out4[2] = (size_t)((char*)im.final_T - probe);
     out4[3] = (size_t)((char*)im.n_contrib - probe);
@@ -1441,6 +1466,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     b.fwd_layout = fwd_layout_for((int64_t)T);
     b.xcd_chunk = g_xcd_chunk;
     b.id_stride = (uint32_t)P;
+    const uint32_t* sorted_keys = nullptr;
     if (n > 0) {
         if (!geom || !radii || (K_cap > 0 && !binning) || !out_point_weight)
             return fail(FGS_E_INVALID, "null geometry / binning buffer or output%s");
@@ -1480,25 +1506,30 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         }
         if (K_cap > 0) {
             Binning bn = carve_binning(binning, K_cap, nullptr);
-            // pairs carry the view's own tile number: the largest view sets the passes
+            const BatchPairs bp = batch_pairs(bn, K_cap);
+            // keys carry the view's own tile number: the largest view sets the passes
             const int passes = (bit_length((uint32_t)sz->max_view_tiles) + 7) / 8;
-            uint64_t* first = (passes % 2 == 0) ? bn.pairsA : bn.pairsB;
-            uint64_t* second = (passes % 2 == 0) ? bn.pairsB : bn.pairsA;
+            const bool even = passes % 2 == 0;
             {
                 ProfScope ps(ST_DUPLICATE, st);
-                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, gm.nvis, P,
-                                                            views, first, (size_t)K_cap, cnt + 1, gm.recs, st),
+                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, gm.nvis, P, views,
+                                                            even ? bp.keysA : bp.keysB, even ? bp.valsA : bp.valsB,
+                                                            (size_t)K_cap, gm.recs, st),
                                 "batch duplicate")))
                     return rc;
             }
             {
                 ProfScope ps(ST_TILE_SORT, st);
-                if ((rc = check(fgs::radix_sort_u64(first, second, (size_t)K_cap, 32, passes, bn.hist,
-                                                    bn.partials, st, cnt + 1), "batch tile sort")))
+                if ((rc = check(fgs::radix_sort_kv32(even ? bp.keysA : bp.keysB, even ? bp.valsA : bp.valsB,
+                                                     even ? bp.keysB : bp.keysA, even ? bp.valsB : bp.valsA,
+                                                     (size_t)K_cap, fgs::kBatchKeyViewBits, passes, bn.hist,
+                                                     bn.partials, st, cnt + 1), "batch tile sort")))
                     return rc;
             }
-            b.pairs = bn.pairsA;
-            b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);   // free once sorted
+            sorted_keys = bp.keysA;
+            b.pair_ids = bp.valsA;
+            b.pair_shift = 0;
+            b.vis_ids = bp.keysB;   // free once sorted
             b.kv_strips = bn.strips;
         }
         b.means2D = gm.means2D;
@@ -1507,8 +1538,8 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     }
     {
         ProfScope ps(ST_RANGES, st);
-        if ((rc = check(fgs::launch_ranges_batch(b.pairs, b.pairs ? (size_t)K_cap : 0,
-                                                 (const unsigned long long*)counts + 1, P, views, im.ranges, T, st),
+        if ((rc = check(fgs::launch_ranges_batch(sorted_keys, sorted_keys ? (size_t)K_cap : 0,
+                                                 (const unsigned long long*)counts + 1, views, im.ranges, T, st),
                         "batch ranges")))
             return rc;
     }
@@ -1559,11 +1590,13 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
     if ((rc = check(fgs::launch_fill_u32(dop_view, 0u, n, st), "ws clear"))) return rc;
     if (K_cap > 0) {
         Binning bn = carve_binning(const_cast<void*>(binning), K_cap, nullptr);
+        const BatchPairs bp = batch_pairs(bn, K_cap);
         fgs::BlendArgs b{};
         b.ranges = im.ranges;
         b.tile_vis = im.tile_vis;
-        b.pairs = bn.pairsA;
-        b.vis_ids = reinterpret_cast<uint32_t*>(bn.pairsB);
+        b.pair_ids = bp.valsA;
+        b.pair_shift = 0;
+        b.vis_ids = bp.keysB;
         b.kv_strips = bn.strips;
         b.recs = gm.recs;
         b.part = bn.part;

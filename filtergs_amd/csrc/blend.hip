@@ -29,6 +29,12 @@
 namespace fgs {
 
 constexpr float kLog2e = 1.4426950408889634f;
+
+// The Gaussian (element) of sorted kv pair k.
+__device__ __forceinline__ uint32_t pair_id(const BlendArgs& a, uint32_t k)
+{
+    return a.pair_ids[(size_t)k << a.pair_shift];
+}
 constexpr int kStrips = FGS_BLOCK / FGS_WAVE;   // 4 waves, one 16x4 strip each
 
 // Minimum over the rectangle [dx0, dx1] x [dy0, dy1] (offsets from the mean)
@@ -313,7 +319,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
         uint32_t bits = 0u;
         if (q < BATCH) {
             if (k < n) {
-                const uint32_t g = (uint32_t)a.pairs[range.x + k];
+                const uint32_t g = pair_id(a, range.x + k);
                 const float2 m = a.means2D[g];
                 const float4 co = a.conic_opacity[g];
                 s_slot[q].g0 = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
@@ -657,8 +663,8 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
     // round apart, so neither waits on the other).
     StageRec cur, nxt;
     uint32_t id_next = 0u;
-    if (lane < n) cur = load_stage(a, (uint32_t)a.pairs[range.x + lane]);
-    if (kFwdWaveBatch + lane < n) id_next = (uint32_t)a.pairs[range.x + kFwdWaveBatch + lane];
+    if (lane < n) cur = load_stage(a, pair_id(a, range.x + lane));
+    if (kFwdWaveBatch + lane < n) id_next = pair_id(a, range.x + kFwdWaveBatch + lane);
 
     for (int r = 0; r * kFwdWaveBatch < n && live != 0u; ++r) {
         const int k0 = r * kFwdWaveBatch;
@@ -672,7 +678,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
             bits = strip_bits_ellipse(cur.m, cur.co, tx0, ty0);
         }
         if (k0 + kFwdWaveBatch + lane < n) nxt = load_stage(a, id_next);
-        if (k0 + 2 * kFwdWaveBatch + lane < n) id_next = (uint32_t)a.pairs[range.x + k0 + 2 * kFwdWaveBatch + lane];
+        if (k0 + 2 * kFwdWaveBatch + lane < n) id_next = pair_id(a, range.x + k0 + 2 * kFwdWaveBatch + lane);
         s_slot[lane].w[0] = 0u;   // point weight of the slot (max T*alpha, float bits)
         s_slot[lane].w[1] = 0u;   // strips it blended into
         s_slot[lane].kv = 0.f;

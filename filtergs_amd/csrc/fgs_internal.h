@@ -99,7 +99,11 @@ __device__ inline void bind_view(PreArgs& a, const BatchView& v)
 
 struct BlendArgs {
     const uint2* ranges;
-    const uint64_t* pairs;     // sorted (tile << 32 | gaussian)
+    // the Gaussian (element) of sorted kv pair k: pair_ids[k << pair_shift]
+    // (the low words of the plain path's u64 pairs, shift 1; the batch's
+    // value array, shift 0)
+    const uint32_t* pair_ids;
+    uint32_t pair_shift;
     // Written by the forward, read by the backward: per tile, the kv values
     // that blended into at least one pixel, in blend order, from the tile's
     // first kv position (vis_ids[ranges[t].x + i]); the strips of the tile
@@ -217,11 +221,24 @@ hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs,
 hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n,
                                    const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s);
+// kv pairs of a batch are two u32 arrays: the tile key (view-local tile <<
+// kBatchKeyViewBits) | view, and the element v * P + i.  A stable sort on the
+// tile bits (from bit kBatchKeyViewBits) leaves each tile's pairs in view
+// order, each view's in depth order, and a (tile, view) run is a run of equal
+// keys.
+constexpr int kBatchKeyViewBits = 6;
+constexpr uint32_t kBatchMaxViewTiles = 1u << (32 - kBatchKeyViewBits);
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                   size_t n, const unsigned long long* n_dev, int P,
-                                  const BatchView* views, uint64_t* pairs, size_t cap,
-                                  const unsigned long long* K, uint4* recs, hipStream_t s);
-hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, const unsigned long long* K, int P,
+                                  const BatchView* views, uint32_t* keys, uint32_t* vals, size_t cap,
+                                  uint4* recs, hipStream_t s);
+// Stable LSD sort of n (key, value) u32 pairs on n_passes 8-bit digits of the
+// key from bit shift_lo (only the first min(n, *n_dev) with n_dev); the
+// result is in (ka, va) if n_passes is even, else in (kb, vb).
+hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, size_t n, int shift_lo,
+                           int n_passes, uint32_t* hist, uint32_t* partials, hipStream_t s,
+                           const unsigned long long* n_dev);
+hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const unsigned long long* K,
                                const BatchView* views, uint2* ranges, int T, hipStream_t s);
 
 // blend

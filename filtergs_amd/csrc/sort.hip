@@ -478,12 +478,34 @@ hipError_t radix_sort_u64_segments(uint64_t* a, uint64_t* b, const uint32_t* seg
 // stores coalesce and one huge splat does not hold up its wave.
 constexpr uint32_t kDupSolo = 16;
 
-// Writes the kv pairs (tile_base + y * gx + x) << 32 | val of rectangle rc
-// (packed x0 | y0 << 16, x1 | y1 << 16) at off; every argument is the
-// calling lane's own.  Every lane of the wave calls it (valid = false for
-// lanes without an element); pairs at or past cap are dropped.
+// kv pair stores: the plain path's u64 (tile << 32) | val, and a batch's
+// two u32 arrays, key (view-local tile << kBatchKeyViewBits) | view and val.
+struct PairStore64 {
+    uint64_t* pairs;
+    __device__ void operator()(size_t p, uint32_t tile, uint32_t tag, uint32_t val) const
+    {
+        (void)tag;
+        pairs[p] = ((uint64_t)tile << 32) | val;
+    }
+};
+struct PairStoreBatch {
+    uint32_t* keys;
+    uint32_t* vals;
+    __device__ void operator()(size_t p, uint32_t tile, uint32_t view, uint32_t val) const
+    {
+        keys[p] = (tile << kBatchKeyViewBits) | view;
+        vals[p] = val;
+    }
+};
+
+// Writes the kv pairs of tiles tile_base + y * gx + x of rectangle rc
+// (packed x0 | y0 << 16, x1 | y1 << 16) at off through st (with tag, the
+// batch's view); every argument is the calling lane's own.  Every lane of
+// the wave calls it (valid = false for lanes without an element); pairs at
+// or past cap are dropped.
+template <typename Store>
 __device__ __forceinline__ void dup_rect(bool valid, uint2 rc, size_t off, uint32_t val, uint32_t gx,
-                                         uint32_t tile_base, uint64_t* __restrict__ pairs, size_t cap)
+                                         uint32_t tile_base, uint32_t tag, Store st, size_t cap)
 {
     const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
     const uint32_t w = (valid && x1 > x0 && y1 > y0) ? x1 - x0 : 0u;
@@ -491,7 +513,7 @@ __device__ __forceinline__ void dup_rect(bool valid, uint2 rc, size_t off, uint3
     if (n > 0 && n <= kDupSolo) {
         for (uint32_t y = y0; y < y1; ++y)
             for (uint32_t x = x0; x < x1; ++x, ++off)
-                if (off < cap) pairs[off] = ((uint64_t)(tile_base + y * gx + x) << 32) | val;
+                if (off < cap) st(off, tile_base + y * gx + x, tag, val);
     }
     uint64_t big = __ballot(n > kDupSolo);
     const uint32_t lane = (uint32_t)lane_id();
@@ -505,13 +527,14 @@ __device__ __forceinline__ void dup_rect(bool valid, uint2 rc, size_t off, uint3
         const uint32_t bval = __builtin_amdgcn_readlane(val, src);
         const uint32_t bgx = __builtin_amdgcn_readlane(gx, src);
         const uint32_t btb = __builtin_amdgcn_readlane(tile_base, src);
+        const uint32_t btag = __builtin_amdgcn_readlane(tag, src);
         const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)off, src);
         const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(off >> 32), src);
         const size_t boff = ((size_t)hi << 32) | lo;
         for (uint32_t k = lane; k < bn; k += FGS_WAVE) {
             const uint32_t dy = k / bw;
             const uint32_t x = bx0 + (k - dy * bw), y = by0 + dy;
-            if (boff + k < cap) pairs[boff + k] = ((uint64_t)(btb + y * bgx + x) << 32) | bval;
+            if (boff + k < cap) st(boff + k, btb + y * bgx + x, btag, bval);
         }
     }
 }
@@ -535,7 +558,7 @@ __global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ 
     const uint32_t g = valid ? (uint32_t)vkeys[s] : 0u;
     const size_t off = valid ? offs[s] : 0u;
     if (valid && gathered(rc)) recs[g].w = (uint32_t)off;
-    dup_rect(valid, rc, off, g, (uint32_t)gx, 0u, pairs, ~(size_t)0);
+    dup_rect(valid, rc, off, g, (uint32_t)gx, 0u, 0u, PairStore64{ pairs }, ~(size_t)0);
 }
 
 hipError_t launch_duplicate(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
@@ -612,8 +635,8 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
                                                          const uint2* __restrict__ rects_s, size_t n,
                                                          const unsigned long long* __restrict__ n_dev,
                                                          int P, const BatchView* __restrict__ views,
-                                                         uint64_t* __restrict__ pairs, size_t cap,
-                                                         uint4* __restrict__ recs)
+                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                                         size_t cap, uint4* __restrict__ recs)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     n = live_count(n, n_dev);
@@ -626,53 +649,194 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
     const size_t off = valid ? offs[s] : 0u;
     const uint32_t e = batch_elem(key, P);
     if (valid && gathered(rc)) recs[e].w = (uint32_t)off;
-    dup_rect(valid, rc, off, e, gx, 0u, pairs, cap);
+    dup_rect(valid, rc, off, e, gx, 0u, v, PairStoreBatch{ keys, vals }, cap);
 }
 
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                   size_t n, const unsigned long long* n_dev, int P,
-                                  const BatchView* views, uint64_t* pairs, size_t cap,
-                                  const unsigned long long* K, uint4* recs, hipStream_t s)
+                                  const BatchView* views, uint32_t* keys, uint32_t* vals, size_t cap,
+                                  uint4* recs, hipStream_t s)
 {
-    (void)K;
     if (cap == 0 || n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys, offs,
-                       rects_s, n, n_dev, P, views, pairs, cap, recs);
+                       rects_s, n, n_dev, P, views, keys, vals, cap, recs);
     return hipGetLastError();
 }
 
-// identifyTileRanges over the batch: a range runs while (tile, view) stays
-// the same; it is stored at the view's tile_base + tile.  Only the first
+// ---------------------------------------------------------------------------
+// LSD radix sort of (u32 key, u32 value) pairs on 8-bit digits (the batched
+// tile sort): the passes of radix_sort_u64 with the keys and values in two
+// arrays, so the counting pass reads 4 bytes per pair.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_rs_count32(const uint32_t* __restrict__ in, size_t n, int shift,
+                                                         uint32_t* __restrict__ hist, size_t nb,
+                                                         const unsigned long long* n_dev)
+{
+    __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][threadIdx.x] = 0;
+    __syncthreads();
+    n = live_count(n, n_dev);
+    const size_t base = (size_t)blockIdx.x * kChunk;
+    if (base < n) {
+        const size_t c = n - base < (size_t)kChunk ? n - base : (size_t)kChunk;
+        const int wave = threadIdx.x / FGS_WAVE;
+        const uint32_t* src = in + base;
+        // four keys per 16-byte load (base is a multiple of kChunk)
+        const size_t c4 = c / 4;
+        for (size_t q = threadIdx.x; q < c4; q += kThreads) {
+            const uint4 k4 = reinterpret_cast<const uint4*>(src)[q];
+            atomicAdd(&s_h[wave][(k4.x >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&s_h[wave][(k4.y >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&s_h[wave][(k4.z >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&s_h[wave][(k4.w >> shift) & (kRadix - 1)], 1u);
+        }
+        for (size_t e = 4 * c4 + threadIdx.x; e < c; e += kThreads)
+            atomicAdd(&s_h[wave][(src[e] >> shift) & (kRadix - 1)], 1u);
+    }
+    rs_count_store(s_h, hist, nb);
+}
+
+// The dense scatter of k_rs_scatter for (key, value) pairs.
+__global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin,
+                                                              uint32_t* __restrict__ kout,
+                                                              uint32_t* __restrict__ vout, size_t n, int shift,
+                                                              const uint32_t* __restrict__ offs, size_t nb,
+                                                              const unsigned long long* n_dev)
+{
+    constexpr int kWaves = kThreads / FGS_WAVE;
+    constexpr int kPerWave = kChunk / kWaves;
+    __shared__ uint32_t s_cnt[kWaves][kRadix];
+    __shared__ uint32_t s_start[kRadix];
+    __shared__ uint32_t s_glob[kRadix];
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t s_keys[kChunk];
+    __shared__ uint32_t s_vals[kChunk];
+    n = live_count(n, n_dev);
+    const size_t base = (size_t)blockIdx.x * kChunk;
+    if (base >= n) return;
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+    const int lane = lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
+    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
+    __syncthreads();
+    const size_t wbase = base + (size_t)wave * kPerWave;
+    uint32_t key[kIpt], val[kIpt], rank[kIpt];
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const size_t e = wbase + (size_t)j * FGS_WAVE + lane;
+        key[j] = e < n ? kin[e] : 0u;
+        val[j] = e < n ? vin[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const bool valid = wbase + (size_t)j * FGS_WAVE + lane < n;
+        const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t below = __popcll(peers & lt_mask);
+        const uint32_t before = valid ? s_cnt[wave][d] : 0u;
+        rank[j] = before + below;
+        if (valid && below == 0) s_cnt[wave][d] = before + __popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t c[kWaves], tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            c[w] = s_cnt[w][t];
+            tot += c[w];
+        }
+        uint32_t all;
+        uint32_t run = block_exclusive_scan(tot, s_wave, &all);
+        s_start[t] = run;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            s_cnt[w][t] = run;
+            run += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        if (wbase + (size_t)j * FGS_WAVE + lane < n) {
+            const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+            const uint32_t q = s_cnt[wave][d] + rank[j];
+            s_keys[q] = key[j];
+            s_vals[q] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
+    for (uint32_t p = t; p < live; p += kThreads) {
+        const uint32_t k = s_keys[p];
+        const uint32_t d = (k >> shift) & (kRadix - 1);
+        const size_t o = s_glob[d] + (p - s_start[d]);
+        kout[o] = k;
+        vout[o] = s_vals[p];
+    }
+}
+
+hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, size_t n, int shift_lo,
+                           int n_passes, uint32_t* hist, uint32_t* partials, hipStream_t s,
+                           const unsigned long long* n_dev)
+{
+    if (n == 0) return hipSuccess;
+    const size_t nb = div_up(n, kChunk);
+    uint32_t *ks = ka, *vs = va, *kd = kb, *vd = vb;
+    for (int p = 0; p < n_passes; ++p) {
+        const int shift = shift_lo + 8 * p;
+        hipLaunchKernelGGL(k_rs_count32, dim3((unsigned)nb), dim3(kThreads), 0, s, ks, n, shift, hist, nb, n_dev);
+        hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_scatter_kv32, dim3((unsigned)nb), dim3(kThreads), 0, s, ks, vs, kd, vd, n, shift,
+                           hist, nb, n_dev);
+        uint32_t* t = ks;
+        ks = kd;
+        kd = t;
+        t = vs;
+        vs = vd;
+        vd = t;
+    }
+    return hipGetLastError();
+}
+
+// identifyTileRanges over the batch: a (tile, view) run is a run of equal
+// tile keys; it is stored at the view's tile_base + tile.  Only the first
 // min(cap, *K) sorted pairs exist.
-__global__ __launch_bounds__(256) void k_ranges_batch(const uint64_t* __restrict__ sorted, size_t cap,
+__global__ __launch_bounds__(256) void k_ranges_batch(const uint32_t* __restrict__ sorted, size_t cap,
                                                       const unsigned long long* __restrict__ K,
-                                                      uint32_t P, const BatchView* __restrict__ views,
+                                                      const BatchView* __restrict__ views,
                                                       uint2* __restrict__ ranges)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = live_count(cap, K);
     if (k >= n) return;
-    const uint64_t key = sorted[k];
-    // one division: a neighbour is in the same (tile, view) run when it has
-    // the same tile and its element lies in this view's [v P, (v + 1) P)
-    const uint32_t v = (uint32_t)key / P;
-    const uint32_t lo = v * P;
-    const uint32_t tile = (uint32_t)(key >> 32);
-    const auto same = [&](uint64_t o) {
-        return (uint32_t)(o >> 32) == tile && (uint32_t)o - lo < P;
-    };
-    const uint32_t gt = views[v].tile_base + tile;
-    if (k == 0 || !same(sorted[k - 1])) ranges[gt].x = (uint32_t)k;
-    if (k == n - 1 || !same(sorted[k + 1])) ranges[gt].y = (uint32_t)(k + 1);
+    const uint32_t key = sorted[k];
+    const bool first = k == 0 || sorted[k - 1] != key;
+    const bool last = k == n - 1 || sorted[k + 1] != key;
+    if (!first && !last) return;
+    const uint32_t v = key & ((1u << kBatchKeyViewBits) - 1u);
+    const uint32_t gt = views[v].tile_base + (key >> kBatchKeyViewBits);
+    if (first) ranges[gt].x = (uint32_t)k;
+    if (last) ranges[gt].y = (uint32_t)(k + 1);
 }
 
-hipError_t launch_ranges_batch(const uint64_t* sorted, size_t cap, const unsigned long long* K, int P,
+hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const unsigned long long* K,
                                const BatchView* views, uint2* ranges, int T, hipStream_t s)
 {
     hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
     if (e != hipSuccess || cap == 0) return e;
-    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted, cap, K,
-                       (uint32_t)P, views, ranges);
+    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted_keys, cap, K,
+                       views, ranges);
     return hipGetLastError();
 }
 

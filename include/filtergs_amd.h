@@ -371,8 +371,9 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
                        float* dL_dsh, float* dL_dscales, float* dL_drot, void* ws, void* stream);
 
 /* Byte offsets inside the batch buffers, for inspection (tests, tools):
- * out4[0] the sorted kv pairs in the binning buffer of capacity K_cap (u64,
- * (view-local tile << 32) | (v * P + i), min(K, K_cap) of them);
+ * out4[0] the sorted kv pairs in the binning buffer of capacity K_cap: K_cap
+ * u32 tile keys ((view-local tile << 6) | v), then K_cap u32 elements
+ * (v * P + i), min(K, K_cap) of each valid;
  * out4[1] the tile ranges in the image buffer (uint2 per batch tile, views
  * in plan order); out4[2] final_T and out4[3] n_contrib there (per pixel,
  * views concatenated).  No counterpart in the reference: its per-view
