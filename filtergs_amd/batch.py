@@ -15,6 +15,12 @@ cost-ordered (view, tile) queue, LDS batch depth) by timing the step.
 Per view the forward outputs equal those of ``ext.rasterize_gaussians`` bit
 for bit (images, radii, point ids, point weights); gradients equal the sum of
 the per-view backward calls up to float summation order.
+
+Per-view records and screen-space gradients live in the batch's element
+layout: Gaussian i seen by view v is element v * P64 + 64 * (i // 64) + r,
+with r the rank of i among the Gaussians of its 64-group that view v sees and
+P64 = P rounded up to 64 (culled pairs have no element).  ``view_elements``,
+``view_grads`` and ``view_records`` map them back to [P, ...] per view.
 """
 from __future__ import annotations
 
@@ -92,8 +98,10 @@ class BatchRasterizer:
         self.weight_pixel = torch.zeros((npix,), **f32)
         self.point_weight = torch.zeros((self.V, self.P), **f32)
         self.counts = torch.zeros((4,), dtype=torch.int64, device=self.device)
-        self.dL_dmeans2D = torch.zeros((self.V, self.P, 3), **f32)
-        self.dL_dcolors = torch.zeros((self.V, self.P, 3), **f32)
+        self.P64 = (self.P + 63) // 64 * 64
+        # per-view screen-space gradients in the element layout
+        self.dL_dmeans2D = torch.zeros((self.V, self.P64, 3), **f32)
+        self.dL_dcolors = torch.zeros((self.V, self.P64, 3), **f32)
         self.K_cap = 0
         self.binning = torch.empty((0,), **u8)
         self.pix_base = np.concatenate([[0], np.cumsum([h * w for h, w in self.shapes])]).astype(np.int64)
@@ -204,7 +212,8 @@ class BatchRasterizer:
         """dL_dpix: flat [3 * pixels] (the views' [3,H,W] blocks
         concatenated) or a list of [3,H,W] tensors.  Returns the parameter
         gradients summed over views; per-view dL/dmeans2D and dL/dcolors stay
-        in ``dL_dmeans2D`` / ``dL_dcolors`` ([V, P, 3])."""
+        in ``dL_dmeans2D`` / ``dL_dcolors`` ([V, P64, 3], element layout; see
+        ``view_grads``)."""
         if isinstance(dL_dpix, (list, tuple)):
             dL_dpix = torch.cat([t.reshape(-1) for t in dL_dpix])
         dL_dpix = dL_dpix.to(dtype=torch.float32).contiguous()
@@ -229,8 +238,9 @@ class BatchRasterizer:
         """View v's sorted kv list of the last forward, in the plain path's
         form: (tile [n] int64, gaussian [n] int64, ranges [T_v, 2] int64 into
         those arrays, (0, 0) for empty tiles).  The batch stores the pairs of
-        all views sorted by (view-local tile, view, depth, index); this
-        gathers view v's runs in tile order.  K: the pair count the forward
+        all views sorted by (view-local tile, view, depth, index) with
+        elements for values; this gathers view v's runs in tile order and
+        maps the elements back to Gaussian indices.  K: the pair count the forward
         returned (at most the capacity)."""
         offs = (ctypes.c_size_t * 4)()
         call("fgs_batch_offsets", ctypes.byref(self.sizes), self.K_cap, offs)
@@ -254,7 +264,45 @@ class BatchRasterizer:
         tile = keys[pos] >> 6
         elem = vals.to(torch.int64)[pos] & 0xFFFFFFFF
         ranges = torch.stack([starts, starts + lens], 1) * (lens > 0)[:, None]
-        return tile, elem - v * self.P, ranges
+        # elements back to Gaussian indices
+        e = self.view_elements(v)
+        inv = torch.zeros((self.P64,), dtype=torch.int64, device=self.device)
+        m = e >= 0
+        inv[e[m] - v * self.P64] = torch.arange(self.P, device=self.device, dtype=torch.int64)[m]
+        return tile, inv[elem - v * self.P64], ranges
+
+    def view_elements(self, v):
+        """Element index (into [V * P64]) of each of view v's Gaussians after a
+        forward, -1 where the view does not see it ([P] int64)."""
+        vis = torch.zeros((self.P64,), dtype=torch.int64, device=self.device)
+        vis[:self.P] = (self.radii[v] > 0).to(torch.int64)
+        grp = vis.view(-1, 64)
+        rank = torch.cumsum(grp, 1) - grp
+        base = torch.arange(0, self.P64, 64, device=self.device, dtype=torch.int64)[:, None]
+        e = (v * self.P64 + base + rank).view(-1)[:self.P]
+        return torch.where(vis[:self.P] > 0, e, torch.full_like(e, -1))
+
+    def _gather_rows(self, t, v, width):
+        e = self.view_elements(v)
+        flat = t.reshape(-1, width)
+        out = torch.zeros((self.P, width), dtype=t.dtype, device=self.device)
+        m = e >= 0
+        out[m] = flat[e[m]]
+        return out
+
+    def view_grads(self, v):
+        """dL/dmeans2D and dL/dcolors of view v after a backward, [P, 3]
+        each (zero where the view does not see the Gaussian)."""
+        return (self._gather_rows(self.dL_dmeans2D, v, 3), self._gather_rows(self.dL_dcolors, v, 3))
+
+    def view_records(self, v):
+        """The preprocess records of view v after a forward: means2D [P, 2],
+        conic_opacity [P, 4], rgb [P, 3] (zero where the view does not see the
+        Gaussian)."""
+        from .ext import geom_views
+        gv = geom_views(self.geom, self.V * self.P64)
+        return {k: self._gather_rows(gv[k], v, w) for k, w in
+                (("means2D", 2), ("conic_opacity", 4), ("rgb", 3))}
 
     def view_state(self, v):
         """final_T and n_contrib of view v ([H, W] each) after a forward."""

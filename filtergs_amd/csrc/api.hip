@@ -154,6 +154,8 @@ struct Geom {
     uint32_t* partials;
     uint32_t* seg_cnt;              // batch: keys per preprocess wave
     unsigned long long* nvis;       // batch: visible (view, Gaussian) keys after the depth sort
+    uint64_t* vis_masks;            // batch: visibility ballot per (view, 64-group)
+    uint32_t* gid;                  // batch: Gaussian index of each element
 };
 
 size_t scan_words_for_hist(size_t n)
@@ -162,7 +164,9 @@ size_t scan_words_for_hist(size_t n)
     return fgs::scan_partials_words(fgs::radix_hist_words(n));
 }
 
-Geom carve_geom(void* buf, int P, size_t* total)
+// Per-element geometry of P Gaussians, or of a batch's P = V * P64 elements
+// (batch: with the visibility ballots and element -> Gaussian indices).
+Geom carve_geom(void* buf, int P, size_t* total, bool batch = false)
 {
     Carver c{ static_cast<char*>(buf) };
     Geom g;
@@ -184,6 +188,13 @@ Geom carve_geom(void* buf, int P, size_t* total)
     g.partials = c.take<uint32_t>(pw > ph ? pw : ph);
     g.seg_cnt = c.take<uint32_t>(n / FGS_WAVE + 1);
     g.nvis = c.take<unsigned long long>(1);
+    if (batch) {
+        g.vis_masks = c.take<uint64_t>(n / FGS_WAVE + 1);
+        g.gid = c.take<uint32_t>(n);
+    } else {
+        g.vis_masks = nullptr;
+        g.gid = nullptr;
+    }
     if (total) *total = c.off;
     return g;
 }
@@ -1316,8 +1327,8 @@ static int batch_validate(const fgs_batch_view* views, int32_t V, int32_t P)
     if (V <= 0 || V > fgs::kBatchMaxViews) return fail(FGS_E_INVALID, "views must be in [1, 64]%s");
     if (P < 0 || (uint64_t)P > fgs::kBatchIdxMask + 1)
         return fail(FGS_E_INVALID, "a batch holds at most 2^26 Gaussians%s");
-    if ((uint64_t)V * (uint64_t)P > 0x7FFFFFFFull)
-        return fail(FGS_E_INVALID, "views * Gaussians must stay below 2^31%s");
+    if ((uint64_t)V * (uint64_t)fgs::batch_p64(P) > 0x7FFFFFFFull)
+        return fail(FGS_E_INVALID, "views * Gaussians (rounded up to 64) must stay below 2^31%s");
     for (int v = 0; v < V; ++v) {
         const fgs_batch_view& w = views[v];
         if (w.image_width <= 0 || w.image_height <= 0)
@@ -1350,10 +1361,11 @@ int fgs_batch_sizes_query(const fgs_batch_view* views, int32_t V, int32_t P, fgs
     out->max_view_tiles = Tmax;
     out->pixels = npix;
     out->plan_bytes = align256(sizeof(fgs::BatchView) * (size_t)V);
-    carve_geom(nullptr, V * P, &out->geom_bytes);
+    const size_t ne = (size_t)V * fgs::batch_p64(P);   // elements (batch_elem layout)
+    carve_geom(nullptr, (int)ne, &out->geom_bytes, true);
     carve_image_n(nullptr, (size_t)npix, (size_t)T, &out->image_bytes);
-    out->backward_ws_bytes = align256((size_t)(V * P > 0 ? V * P : 1) * 3 * sizeof(float)) +
-                             align256((size_t)(V * P > 0 ? V * P : 1) * sizeof(float));
+    out->backward_ws_bytes = align256((ne > 0 ? ne : 1) * 3 * sizeof(float)) +
+                             align256((ne > 0 ? ne : 1) * sizeof(float));
     return FGS_OK;
 }
 
@@ -1453,11 +1465,12 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
     hipStream_t st = (hipStream_t)stream;
     const int V = sz->views, P = g->P;
     const size_t n = (size_t)V * (size_t)P;
+    const size_t ne = (size_t)V * fgs::batch_p64(P);   // elements (batch_elem layout)
     const int T = (int)sz->tiles;
     const fgs::BatchView* views = static_cast<const fgs::BatchView*>(plan);
     if ((rc = check(fgs::launch_fill_u32(counts, 0u, 8, st), "counts clear"))) return rc;
     Image im = carve_image_n(image, (size_t)sz->pixels, (size_t)T, nullptr);
-    Geom gm = carve_geom(geom, (int)(n > 0 ? n : 1), nullptr);
+    Geom gm = carve_geom(geom, (int)(ne > 0 ? ne : 1), nullptr, true);
     fgs::BlendArgs b{};
     b.ranges = im.ranges;
     b.tile_vis = im.tile_vis;
@@ -1480,6 +1493,8 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         o.vkeys = gm.vkeysA;
         o.error_flag = (int*)(counts + 2);
         o.seg_cnt = gm.seg_cnt;
+        o.vis_masks = gm.vis_masks;
+        o.gid = gm.gid;
         {
             ProfScope ps(ST_PREPROCESS, st);
             if ((rc = check(fgs::launch_preprocess_fwd_batch(a, views, V, o, st), "batch preprocess")))
@@ -1533,8 +1548,9 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
             b.kv_strips = bn.strips;
         }
         b.means2D = gm.means2D;
-        b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+        b.colors = gm.rgb;   // the preprocess copies colors_precomp to the elements
         b.conic_opacity = gm.conic_opacity;
+        b.gid = gm.gid;
     }
     {
         ProfScope ps(ST_RANGES, st);
@@ -1572,8 +1588,8 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
     int rc = batch_common(s, g, sz);
     if (rc) return rc;
     const int V = sz->views, P = g->P;
-    const size_t n = (size_t)V * (size_t)P;
-    if (n == 0) return FGS_OK;
+    const size_t n = (size_t)V * fgs::batch_p64(P);   // elements (batch_elem layout)
+    if (P == 0) return FGS_OK;
     if (!plan || !radii || !geom || !image || (K_cap > 0 && !binning) || !dL_dpix || !dL_dmeans2D || !dL_dcolors ||
         !dL_dopacity || !dL_dmeans3D || !dL_dcov3D || !ws)
         return fail(FGS_E_INVALID, "null input, output or workspace%s");
@@ -1582,7 +1598,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
     hipStream_t st = (hipStream_t)stream;
     const int T = (int)sz->tiles;
     const fgs::BatchView* views = static_cast<const fgs::BatchView*>(plan);
-    Geom gm = carve_geom(const_cast<void*>(geom), (int)n, nullptr);
+    Geom gm = carve_geom(const_cast<void*>(geom), (int)n, nullptr, true);
     Image im = carve_image_n(const_cast<void*>(image), (size_t)sz->pixels, (size_t)T, nullptr);
     float* dconic = static_cast<float*>(ws);
     float* dop_view = reinterpret_cast<float*>(static_cast<char*>(ws) + align256(n * 3 * sizeof(float)));
@@ -1603,7 +1619,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         b.mark = bn.mark;
         b.part_cap = (size_t)K_cap;
         b.means2D = gm.means2D;
-        b.colors = g->colors_precomp ? g->colors_precomp : gm.rgb;
+        b.colors = gm.rgb;
         b.conic_opacity = gm.conic_opacity;
         b.bwd_batch = g_shape[1];
         b.bwd_layout = g_bwd_layout;
@@ -1618,14 +1634,15 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
             b.queue_head = im.qhead;
             b.queue_chunk = (uint32_t)g_queue_chunk;
         }
-        const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, n, P, gm.nvis };
+        const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, (size_t)V * (size_t)P, fgs::batch_p64(P),
+                                  gm.nvis };
         if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,
                                                     dL_dmeans2D, dconic, dop_view, dL_dcolors,
                                                     s->grad_mode == 1, st), "batch blend backward")))
             return rc;
     }
     const fgs::PreArgs a = batch_pre_args(s, g);
-    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors };
+    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors, gm.vis_masks };
     fgs::PreBwdOut o{};
     o.dL_dopacity = dL_dopacity;
     o.dL_dmeans3D = dL_dmeans3D;

@@ -35,6 +35,12 @@ __device__ __forceinline__ uint32_t pair_id(const BlendArgs& a, uint32_t k)
 {
     return a.pair_ids[(size_t)k << a.pair_shift];
 }
+
+// The Gaussian index of element g (a batch's elements carry it in gid).
+__device__ __forceinline__ uint32_t gauss_id(const BlendArgs& a, uint32_t g)
+{
+    return a.gid ? a.gid[g] : g;
+}
 constexpr int kStrips = FGS_BLOCK / FGS_WAVE;   // 4 waves, one 16x4 strip each
 
 // Minimum over the rectangle [dx0, dx1] x [dy0, dy1] (offsets from the mean)
@@ -317,15 +323,17 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
         const int q = tid;
         const int k = r * BATCH + q;
         uint32_t bits = 0u;
+        uint32_t stage_g = 0u;   // the element this lane staged
         if (q < BATCH) {
             if (k < n) {
                 const uint32_t g = pair_id(a, range.x + k);
+                stage_g = g;
                 const float2 m = a.means2D[g];
                 const float4 co = a.conic_opacity[g];
                 s_slot[q].g0 = make_float4(m.x, m.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
                 s_slot[q].g1 = make_float2(-0.5f * kLog2e * co.z, co.w);
                 s_slot[q].col = make_float4(a.colors[3 * g + 0], a.colors[3 * g + 1], a.colors[3 * g + 2],
-                                            __uint_as_float(g));
+                                            __uint_as_float(gauss_id(a, g)));
                 bits = strip_bits_ellipse(m, co, tx0, ty0);
             }
 #pragma unroll
@@ -471,7 +479,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
         __syncthreads();
         uint32_t used = 0u, g = 0u;
         if (q < BATCH && k < n) {
-            g = __float_as_uint(s_slot[q].col.w);
+            g = stage_g;
             uint32_t m = 0u;
 #pragma unroll
             for (int w = 0; w < kStrips; ++w) {
@@ -479,7 +487,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
                 m = sw > m ? sw : m;
                 used |= (sw != 0u ? 1u : 0u) << w;
             }
-            if (m != 0u) atomicMax((uint32_t*)&point_weight[g], m);
+            if (m != 0u) atomicMax((uint32_t*)&point_weight[a.id_base + __float_as_uint(s_slot[q].col.w)], m);
             if (kv_sum) {
                 // each kv pair belongs to exactly one tile: plain stores
                 kv_sum[range.x + k] = s_slot[q].kv;
@@ -562,7 +570,7 @@ __device__ __forceinline__ void blend_fwd_tile(const BlendArgs& a, int tile, flo
         out_color[pix] = fmaf(T, a.bg[0], C0);
         out_color[HW + pix] = fmaf(T, a.bg[1], C1);
         out_color[2 * HW + pix] = fmaf(T, a.bg[2], C2);
-        out_point_id[pix] = max_id >= 0 ? (int)((uint32_t)max_id - a.id_base) : -1;
+        out_point_id[pix] = max_id;
         out_weight_pixel[pix] = wmax;
     }
 }
@@ -573,13 +581,15 @@ struct StageRec {
     float2 m;
     float4 co;
     float c0, c1, c2;
-    uint32_t g;
+    uint32_t g;    // element
+    uint32_t id;   // Gaussian index (point ids and weights)
 };
 
 __device__ __forceinline__ StageRec load_stage(const BlendArgs& a, uint32_t g)
 {
     StageRec r;
     r.g = g;
+    r.id = gauss_id(a, g);
     r.m = a.means2D[g];
     r.co = a.conic_opacity[g];
     r.c0 = a.colors[3 * g + 0];
@@ -674,7 +684,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
         if (lane < cnt) {
             s_slot[lane].g0 = make_float4(cur.m.x, cur.m.y, -0.5f * kLog2e * cur.co.x, -kLog2e * cur.co.y);
             s_slot[lane].g1 = make_float2(-0.5f * kLog2e * cur.co.z, cur.co.w);
-            s_slot[lane].col = make_float4(cur.c0, cur.c1, cur.c2, __uint_as_float(cur.g));
+            s_slot[lane].col = make_float4(cur.c0, cur.c1, cur.c2, __uint_as_float(cur.id));
             bits = strip_bits_ellipse(cur.m, cur.co, tx0, ty0);
         }
         if (k0 + kFwdWaveBatch + lane < n) nxt = load_stage(a, id_next);
@@ -748,10 +758,10 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
         __syncthreads();
         uint32_t used = 0u, g = 0u;
         if (lane < cnt) {
-            g = __float_as_uint(s_slot[lane].col.w);
+            g = cur.g;
             used = s_slot[lane].w[1];
             const uint32_t m = s_slot[lane].w[0];
-            if (m != 0u) atomicMax((uint32_t*)&point_weight[g], m);
+            if (m != 0u) atomicMax((uint32_t*)&point_weight[a.id_base + cur.id], m);
             if (kv_sum) {
                 kv_sum[range.x + k0 + lane] = s_slot[lane].kv;
                 kv_cnt[range.x + k0 + lane] = s_slot[lane].kc;
@@ -788,7 +798,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
             out_color[pix] = fmaf(T[k], a.bg[0], C0[k]);
             out_color[HW + pix] = fmaf(T[k], a.bg[1], C1[k]);
             out_color[2 * HW + pix] = fmaf(T[k], a.bg[2], C2[k]);
-            out_point_id[pix] = max_id[k] >= 0 ? (int)((uint32_t)max_id[k] - a.id_base) : -1;
+            out_point_id[pix] = max_id[k];
             out_weight_pixel[pix] = wmax[k];
         }
     }
@@ -1657,7 +1667,7 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
         const bool valid = s < live;
         const uint32_t n = valid ? gg.tiles_s[s] : 0u;
         const bool gathered = n > 0u && n <= kGatherSolo;
-        const uint32_t g = gathered ? (gg.batch_P ? batch_elem(gg.vkeys[s], gg.batch_P) : (uint32_t)gg.vkeys[s]) : 0u;
+        const uint32_t g = gathered ? (gg.batch_P64 ? batch_elem(gg.vkeys[s], gg.batch_P64) : (uint32_t)gg.vkeys[s]) : 0u;
         const uint32_t off = gathered ? gg.offs[s] : 0u;
         if (gathered && n <= kGatherLane) {
             float t[kGradTerms] = {};

@@ -51,13 +51,16 @@ struct PreOut {
     uint64_t* vkeys;                    // (depth_bits << 32 | i), culled: (~0 << 32 | i)
     int* error_flag;                    // bit 0: prefiltered point culled
     uint32_t* seg_cnt;                  // batch: keys each wave appended to its segment
+    uint64_t* vis_masks;                // batch: per (view, 64-group) visibility ballot
+    uint32_t* gid;                      // batch: Gaussian index of each element
 };
 
 struct PreBwdIn {
     const int* radii;
-    const float* dL_dmean2D;   // [P,3] (x, y used)
-    const float* dL_dconic;    // [P,3]
-    const float* dL_dcolors;   // [P,3]
+    const float* dL_dmean2D;   // [P,3] (x, y used); a batch: per element
+    const float* dL_dconic;    // [P,3]; a batch: per element
+    const float* dL_dcolors;   // [P,3]; a batch: per element
+    const uint64_t* vis_masks; // batch: the element layout
 };
 struct PreBwdOut {
     float* dL_dopacity;        // [P], modified in exact mode
@@ -112,6 +115,9 @@ struct BlendArgs {
     uint32_t* vis_ids;
     uint8_t* kv_strips;
     uint32_t* tile_vis;
+    // batch: the Gaussian index i of element g (gid[g]; null: g itself),
+    // reported in point ids and used for point_weight[id_base + i]
+    const uint32_t* gid;
     // Backward: the nine gradient sums of a (tile, Gaussian) pair are stored
     // at part[9 p ..] with mark[p] = 1, p = the pair's position before the
     // tile sort (recs[g].w, the Gaussian's first pair, plus the row-major
@@ -141,7 +147,8 @@ struct BlendArgs {
     int bwd_layout;            // backward: 0 four waves per tile (one strip each), 1 one wave per tile
     int xcd_chunk;             // batched one-wave blend: runs of tiles per XCD (0: launch order)
     int fwd_layout;            // forward without statistics: 0 four waves per tile, 1 one wave per tile
-    // batched path: kv values are elements v * P + i; point ids report i
+    // batched path: kv values are elements (see batch_elem); point ids
+    // report i, point weights go to point_weight[v * P + i]
     uint32_t id_stride;        // P of a batch, 0 for one view
     uint32_t id_base;          // v * P (set per view)
     // Batched one-wave blend as a persistent launch: runs of queue_chunk
@@ -209,9 +216,23 @@ constexpr int kBatchViewShift = 58;
 constexpr int kBatchIdxBits = 26;
 constexpr uint64_t kBatchIdxMask = (1ull << kBatchIdxBits) - 1;
 constexpr int kBatchMaxViews = 64;
-__device__ __forceinline__ uint32_t batch_elem(uint64_t key, int P)
+// Element layout of a batch's per-view records and blend gradients: Gaussian
+// i visible in view v is element v * P64 + 64 * (i / 64) + r, with r the rank
+// of i among the Gaussians of its 64-group that view v sees (bit i % 64 of
+// vis_masks[v * P64 / 64 + i / 64]) and P64 = P rounded up to 64.  The
+// visible elements of a 64-group are packed at the front of its 64 slots, so
+// the preprocess writes whole runs and culled elements take no stores.  The
+// low 26 bits of a depth key hold 64 * (i / 64) + r.
+__host__ __device__ inline uint32_t batch_p64(int P) { return ((uint32_t)P + 63u) & ~63u; }
+__device__ __forceinline__ uint32_t batch_elem(uint64_t key, uint32_t P64)
 {
-    return (uint32_t)(key >> kBatchViewShift) * (uint32_t)P + (uint32_t)(key & kBatchIdxMask);
+    return (uint32_t)(key >> kBatchViewShift) * P64 + (uint32_t)(key & kBatchIdxMask);
+}
+__device__ __forceinline__ uint32_t batch_slot(const uint64_t* __restrict__ vis_masks, uint32_t P64, int v, int i)
+{
+    const uint32_t l = (uint32_t)i & 63u;
+    const uint64_t m = vis_masks[(size_t)v * (P64 >> 6) + ((uint32_t)i >> 6)];
+    return (uint32_t)v * P64 + ((uint32_t)i & ~63u) + (uint32_t)__popcll(m & ((1ull << l) - 1ull));
 }
 // The batched binning stages run over the first *n_dev of n slots (the
 // visible (view, Gaussian) keys the depth sort left).
@@ -249,7 +270,7 @@ hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_cont
                             float* point_weight, float* kv_sum, uint32_t* kv_cnt, hipStream_t s);
 // The per-Gaussian sums of the backward's per-pair partials, gathered in
 // depth order from the Gaussians' pre-sort pair segments (slot s: key
-// vkeys[s], tiles_s[s] pairs from offs[s]; batch_P > 0: keys of a batch).
+// vkeys[s], tiles_s[s] pairs from offs[s]; batch_P64 > 0: keys of a batch).
 // Gaussians of up to kGatherSolo tiles take this path, larger ones add their
 // per-tile sums with global float atomics (at 16 the atomics of the close
 // orbit views cost more than the longer per-lane gather at 128 does).
@@ -271,7 +292,7 @@ struct GradGather {
     const uint32_t* tiles_s;
     const uint32_t* offs;
     size_t n;
-    int batch_P;
+    uint32_t batch_P64;                // batch: P rounded up to 64 (element layout); 0: one view
     const unsigned long long* n_dev;   // batch: the first *n_dev of the n slots are in use
 };
 hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const float* final_T,

@@ -376,10 +376,15 @@ hipError_t launch_preprocess_fwd_rows(const PreArgs& a, const PreOut& o, hipStre
 }
 
 // All views of a batch in one pass: the Gaussian's inputs (mean, covariance,
-// opacity, SH row) are read once and reused for every view.  Outputs of view
-// v go to element e = v * P + i of the [V * P] arrays; the depth key is
-// (v << 58) | (depth_bits << 26) | i, so one sort on bits 26..63 orders every
-// view by depth and keeps the views apart.
+// opacity, SH row) are read once and reused for every view.  radii[v * P + i]
+// is written for every (view, Gaussian); the records of a visible one (mean,
+// conic and opacity, colour, tile rectangle, Gaussian index) go to its
+// element (batch_elem / batch_slot: packed per 64-group and view, so each
+// wave stores one run per view), with the group's visibility ballot in
+// vis_masks.  The depth key is (v << 58) | (depth_bits << 26) | j with j the
+// element's offset in its view, so one sort on bits 26..63 orders every view
+// by depth (ties by Gaussian index) and keeps the views apart.  colors_precomp
+// of a batch is per view ([V, P, 3]) and is copied to the elements.
 __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a, const BatchView* views,
                                                                       int V, PreOut o)
 {
@@ -397,10 +402,12 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
     gaussian_inputs(a, i, p, c3);
     const float opacity_in = a.opacities[i];
     const ShLds sh{ s_sh, (int)threadIdx.x };
+    const uint32_t P64 = batch_p64(a.P);
     // Only visible (view, Gaussian) keys go to the sort: each wave appends
     // its own to a segment of 64 * V slots (ballot ranks, no atomics) and
     // records how many; the key order inside a segment does not matter, the
     // keys are unique.
+    const uint32_t w64 = (uint32_t)i & ~63u;   // the wave's 64-group (i / 64) * 64
     uint64_t* seg = o.vkeys + (size_t)(i / FGS_WAVE) * FGS_WAVE * V;
     const int lane = lane_id();
     const uint64_t below = lane ? (~0ull >> (FGS_WAVE - lane)) : 0ull;
@@ -410,23 +417,32 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
         bind_view(av, views[v]);
         PreRes r;
         preprocess_view(av, p[0], p[1], p[2], c3, opacity_in, sh, o.error_flag, r);
-        const size_t e = (size_t)v * a.P + i;
-        // no depth or tile count: the depth travels in the key, the tile
-        // count in the record, and nothing later reads either array
-        o.radii[e] = r.radius;
-        o.recs[e] = r.rec;
-        o.means2D[e] = r.mean2d;
-        o.conic_opacity[e] = r.con;
-        if (!a.colors_precomp) {
-            o.rgb[3 * e + 0] = r.col[0];
-            o.rgb[3 * e + 1] = r.col[1];
-            o.rgb[3 * e + 2] = r.col[2];
-        }
+        o.radii[(size_t)v * a.P + i] = r.radius;
         const bool vis = r.tiles != 0;
         const uint64_t m = __ballot(vis);
-        if (vis)
-            seg[appended + __popcll(m & below)] =
-                ((uint64_t)v << 58) | ((uint64_t)__float_as_uint(r.depth) << 26) | (uint32_t)i;
+        const uint32_t rank = (uint32_t)__popcll(m & below);
+        if (vis) {
+            // no depth or tile count: the depth travels in the key, the tile
+            // count in the record
+            const size_t e = (size_t)v * P64 + w64 + rank;
+            o.recs[e] = r.rec;
+            o.means2D[e] = r.mean2d;
+            o.conic_opacity[e] = r.con;
+            o.gid[e] = (uint32_t)i;
+            float c0 = r.col[0], c1 = r.col[1], c2 = r.col[2];
+            if (a.colors_precomp) {
+                const size_t q = 3 * ((size_t)v * a.P + i);
+                c0 = a.colors_precomp[q];
+                c1 = a.colors_precomp[q + 1];
+                c2 = a.colors_precomp[q + 2];
+            }
+            o.rgb[3 * e + 0] = c0;
+            o.rgb[3 * e + 1] = c1;
+            o.rgb[3 * e + 2] = c2;
+            seg[appended + rank] =
+                ((uint64_t)v << 58) | ((uint64_t)__float_as_uint(r.depth) << 26) | (w64 + rank);
+        }
+        if (lane == 0) o.vis_masks[(size_t)v * (P64 >> 6) + (i >> 6)] = m;
         appended += (uint32_t)__popcll(m);
     }
     if (lane == 0) o.seg_cnt[i / FGS_WAVE] = appended;
@@ -737,18 +753,16 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_bwd(PreArgs a, PreBw
 }
 
 // Bit v of the result: Gaussian i is visible in view v (radius > 0).
-__device__ __forceinline__ uint64_t visible_views(const int* radii, int P, int V, int i)
+// The views that see Gaussian i, from the batch's visibility ballots (one
+// wave-uniform load per view: a wave's Gaussians share their 64-group).
+__device__ __forceinline__ uint64_t batch_visible_views(const uint64_t* __restrict__ vis_masks, uint32_t P64,
+                                                        int V, int i)
 {
+    const uint32_t nw = P64 >> 6;
+    const uint32_t grp = __builtin_amdgcn_readfirstlane((uint32_t)i >> 6);
+    const uint32_t l = (uint32_t)i & 63u;
     uint64_t vis = 0ull;
-    int v0 = 0;
-    for (; v0 + 8 <= V; v0 += 8) {
-        int rr[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) rr[u] = radii[(size_t)(v0 + u) * P + i];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) vis |= (uint64_t)(rr[u] > 0) << (v0 + u);
-    }
-    for (; v0 < V; ++v0) vis |= (uint64_t)(radii[(size_t)v0 * P + i] > 0) << v0;
+    for (int v = 0; v < V; ++v) vis |= ((vis_masks[(size_t)v * nw + grp] >> l) & 1ull) << v;
     return vis;
 }
 
@@ -758,7 +772,8 @@ __device__ __forceinline__ uint64_t visible_views(const int* radii, int P, int V
 // dL/dmean3D in o.dL_dmeans3D; the geometry pass adds the cov2D and
 // projection terms to it, sums dL/dcov3D over the views before the scale /
 // rotation backward (linear in it) and sums the per-view dL/dopacity of the
-// blend into o.dL_dopacity.  Per-view gradients are read at e = v * P + i.
+// blend into o.dL_dopacity.  Per-view gradients are read at the Gaussian's
+// element of the view (batch_slot).
 // Both passes keep the views' constants in LDS and load the next visible
 // view's blend gradients before working the current one.  The SH pass is held
 // to 3 waves per SIMD (its LDS column of dL/dsh allows no more), the geometry
@@ -779,11 +794,12 @@ void k_preprocess_bwd_batch_sh(PreArgs a, const BatchView* views, int V, PreBwdI
     if (i < a.P) {
         const float mx = a.means3D[3 * i], my = a.means3D[3 * i + 1], mz = a.means3D[3 * i + 2];
         float dm[3] = { 0.f, 0.f, 0.f };
-        uint64_t vis = visible_views(in.radii, a.P, V, i);
+        const uint32_t P64 = batch_p64(a.P);
+        uint64_t vis = batch_visible_views(in.vis_masks, P64, V, i);
         int v = vis ? __builtin_ctzll(vis) : 0;
         float gc[3] = { 0.f, 0.f, 0.f };
         if (vis) {
-            const float* src = in.dL_dcolors + 3 * ((size_t)v * a.P + i);
+            const float* src = in.dL_dcolors + 3 * (size_t)batch_slot(in.vis_masks, P64, v, i);
             gc[0] = src[0]; gc[1] = src[1]; gc[2] = src[2];
         }
         while (vis) {
@@ -794,7 +810,7 @@ void k_preprocess_bwd_batch_sh(PreArgs a, const BatchView* views, int V, PreBwdI
             for (int f = 0; f < kMaxShFloats; ++f) asm volatile("" : "+v"(shr[f]));
             vis &= vis - 1ull;
             const int vn = vis ? __builtin_ctzll(vis) : v;
-            const float* nsrc = in.dL_dcolors + 3 * ((size_t)vn * a.P + i);
+            const float* nsrc = in.dL_dcolors + 3 * (size_t)batch_slot(in.vis_masks, P64, vn, i);
             const float gn[3] = { nsrc[0], nsrc[1], nsrc[2] };
             bwd_view_sh(a, s_cam + 3 * v, gc, mx, my, mz, shr, dm, s_sh, t);
             v = vn;
@@ -827,12 +843,13 @@ void k_preprocess_bwd_batch_geom(PreArgs a, const BatchView* views, int V, PreBw
     float m[3], c3[6];
     gaussian_inputs(a, i, m, c3);
     float dop = 0.f;
-    uint64_t vis = visible_views(in.radii, a.P, V, i);
+    const uint32_t P64 = batch_p64(a.P);
+    uint64_t vis = batch_visible_views(in.vis_masks, P64, V, i);
     const bool any = vis != 0ull;
     int v = vis ? __builtin_ctzll(vis) : 0;
     float gcur[6] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };   // dconic[3], dmean2D[2], dopacity
     auto load6 = [&](int w, float* g) {
-        const size_t e = (size_t)w * a.P + i;
+        const size_t e = batch_slot(in.vis_masks, P64, w, i);
         g[0] = in.dL_dconic[3 * e]; g[1] = in.dL_dconic[3 * e + 1]; g[2] = in.dL_dconic[3 * e + 2];
         g[3] = in.dL_dmean2D[3 * e]; g[4] = in.dL_dmean2D[3 * e + 1];
         g[5] = dL_dopacity_view[e];

@@ -590,13 +590,14 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
 }
 
 // ---------------------------------------------------------------------------
-// Batched binning.  Depth keys of a batch are (v << 58) | (depth_bits << 26) | i
-// (element e = v * P + i), so one sort orders every view by depth with the
-// views in sequence.  kv pairs carry the view's own tile number and e: a
-// stable sort on the tile bits alone then leaves, for every tile number t,
-// the pairs of view 0, view 1, ... one after the other, each in depth order,
-// so (view, t) ranges stay contiguous and the sort needs only the bits of the
-// largest view's tile count.
+// Batched binning.  Depth keys of a batch are (v << 58) | (depth_bits << 26) | j
+// (element batch_elem; j orders a view's Gaussians like their index), so one
+// sort orders every view by depth with the views in sequence.  kv pairs carry
+// the view's own tile number and the element: a stable sort on the tile bits
+// alone then leaves, for every tile number t, the pairs of view 0, view 1,
+// ... one after the other, each in depth order, so (view, t) ranges stay
+// contiguous and the sort needs only the bits of the largest view's tile
+// count.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __restrict__ vkeys,
                                                               const uint4* __restrict__ recs, size_t n,
@@ -607,7 +608,7 @@ __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= live_count(n, n_dev)) return;
     // only visible (view, Gaussian) keys reach the sort
-    const uint4 r = recs[batch_elem(vkeys[s], P)];
+    const uint4 r = recs[batch_elem(vkeys[s], batch_p64(P))];
     tiles_s[s] = r.z;
     rects_s[s] = make_uint2(r.x, r.y);
 }
@@ -647,7 +648,7 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
     const uint32_t v = (uint32_t)(key >> kBatchViewShift);
     const uint32_t gx = valid ? (uint32_t)views[v].gx : 0u;
     const size_t off = valid ? offs[s] : 0u;
-    const uint32_t e = batch_elem(key, P);
+    const uint32_t e = batch_elem(key, batch_p64(P));
     if (valid && gathered(rc)) recs[e].w = (uint32_t)off;
     dup_rect(valid, rc, off, e, gx, 0u, v, PairStoreBatch{ keys, vals }, cap);
 }

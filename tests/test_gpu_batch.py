@@ -103,8 +103,9 @@ def test_batch_matches_per_view(path, bwd_layout, fwd_layout):
         assert torch.equal(br.point_weight[v], out[4]), v
     grads = br.backward([_t(d) for d in dLs])
     for v, (_, gr) in enumerate(singles):
-        assert _rel(br.dL_dmeans2D[v], gr[0]) <= 1e-5, v
-        assert _rel(br.dL_dcolors[v], gr[1]) <= 1e-5, v
+        gm2, gcol = br.view_grads(v)
+        assert _rel(gm2, gr[0]) <= 1e-5, v
+        assert _rel(gcol, gr[1]) <= 1e-5, v
     summed = {k: sum(s[1][i] for s in singles) for k, i in
               (("opacities", 2), ("means3D", 3), ("cov3D", 4), ("shs", 5), ("scales", 6),
                ("rotations", 7))}
@@ -238,7 +239,6 @@ def test_batch_vs_oracle(orc, grad_mode, scale_modifier):
     K = br.forward()
     assert 0 < K <= br.K_cap
     grads = br.backward([_t(d) for d in dLs])
-    gv = ext.geom_views(br.geom, len(cams) * P)
     names = ("dL_dopacity", "dL_dmeans3D", "dL_dsh", "dL_dscales", "dL_drot")
     keys = ("opacities", "means3D", "shs", "scales", "rotations")
     sums = {m: {k: 0.0 for k in names} for m in ("f64", "f32")}
@@ -261,9 +261,9 @@ def test_batch_vs_oracle(orc, grad_mode, scale_modifier):
         # integer stage and its FP32 inputs
         assert np.array_equal(_np(br.radii[v]), pre["radii"]), v
         vis = pre["radii"] > 0
-        sl = slice(v * P, (v + 1) * P)
+        rec = br.view_records(v)
         for k in ("means2D", "conic_opacity"):
-            assert np.array_equal(_np(gv[k][sl])[vis].view(np.uint32), pre[k][vis].view(np.uint32)), (v, k)
+            assert np.array_equal(_np(rec[k])[vis].view(np.uint32), pre[k][vis].view(np.uint32)), (v, k)
         tile, gid, ranges = br.view_sorted(v, K)
         okeys = f32["keys"]
         assert np.array_equal(_np(tile).astype(np.uint64), okeys >> np.uint64(32)), v
@@ -285,7 +285,7 @@ def test_batch_vs_oracle(orc, grad_mode, scale_modifier):
         # per-view screen-space gradients
         b64 = o["f64"][1]
         b32 = o["f32"][1]
-        for k, t in (("dL_dmean2D", br.dL_dmeans2D[v]), ("dL_dcolors", br.dL_dcolors[v])):
+        for k, t in zip(("dL_dmean2D", "dL_dcolors"), br.view_grads(v)):
             a = _np(t).astype(np.float64)
             e, e32 = _rel_np(a, b64[k]), _rel_np(b32[k].astype(np.float64), b64[k])
             assert e <= max(1e-4, 3 * e32 + 1e-5), (v, k, e, e32)

@@ -112,7 +112,7 @@ def test_c3_batch_equals_per_view_full_size(orc):
         assert torch.equal(br.view_pixels(br.point_id, v), fwd[2]), v
         assert torch.equal(br.view_pixels(br.weight_pixel, v), fwd[3]), v
         assert torch.equal(br.point_weight[v], fwd[4]), v
-        for a, b in ((br.dL_dmeans2D[v], gr[0]), (br.dL_dcolors[v], gr[1])):
+        for a, b in zip(br.view_grads(v), (gr[0], gr[1])):
             d = float((a.double() - b.double()).norm() / b.double().norm().clamp_min(1e-30))
             assert d <= 1e-5, (v, d)
         part = [gr[i].double() for i in (2, 3, 5, 6, 7)]
