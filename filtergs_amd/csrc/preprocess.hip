@@ -66,6 +66,12 @@ struct ShLds {
     __device__ float operator()(int f) const { return s[f * kPitch + t]; }
 };
 
+// An SH row held in the thread's registers (float f at r[f]).
+struct ShReg {
+    const float* r;
+    __device__ float operator()(int f) const { return r[f]; }
+};
+
 template <int NU>
 __device__ __forceinline__ void stage_rows_fixed(const float* __restrict__ src, int rs, int nrows, float* s)
 {
@@ -173,8 +179,9 @@ struct PreRes {
     float col[3];
 };
 
+template <typename Sh>
 __device__ __forceinline__ void preprocess_view(const PreArgs& a, float px, float py, float pz,
-                                                const float* c3, float opacity_in, const ShLds& sh,
+                                                const float* c3, float opacity_in, const Sh& sh,
                                                 int* error_flag, PreRes& r)
 {
     int radius_px = 0;
@@ -388,20 +395,33 @@ hipError_t launch_preprocess_fwd_rows(const PreArgs& a, const PreOut& o, hipStre
 __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a, const BatchView* views,
                                                                       int V, PreOut o)
 {
-    __shared__ float s_sh[kMaxShFloats * kPitch];
-    const int base = blockIdx.x * kPreThreads;
-    const int i = base + threadIdx.x;
-    const int nrows = min(kPreThreads, a.P - base);
-    const int nu = (a.D + 1) * (a.D + 1) * 3;
-    if (!a.colors_precomp) {
-        stage_rows(a.shs + (size_t)base * a.M * 3, a.M * 3, nu, nrows, s_sh);
-        __syncthreads();
-    }
+    const int i = blockIdx.x * kPreThreads + threadIdx.x;
     if (i >= a.P) return;
+    const int nu = (a.D + 1) * (a.D + 1) * 3;
+    // The Gaussian's SH row in registers, read once for all views (no LDS:
+    // the block's rows would cap the kernel at three waves per SIMD).
+    float shr[kMaxShFloats];
+    if (!a.colors_precomp) {
+        const int rs = a.M * 3;
+        const float* row = a.shs + (size_t)i * rs;
+        if (nu == 48 && rs == 48 && ((uintptr_t)a.shs & 15) == 0) {
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                const float4 v4 = reinterpret_cast<const float4*>(row)[q];
+                shr[4 * q] = v4.x;
+                shr[4 * q + 1] = v4.y;
+                shr[4 * q + 2] = v4.z;
+                shr[4 * q + 3] = v4.w;
+            }
+        } else {
+#pragma unroll
+            for (int f = 0; f < kMaxShFloats; ++f) shr[f] = f < nu ? row[f] : 0.f;
+        }
+    }
     float p[3], c3[6];
     gaussian_inputs(a, i, p, c3);
     const float opacity_in = a.opacities[i];
-    const ShLds sh{ s_sh, (int)threadIdx.x };
+    const ShReg sh{ shr };
     const uint32_t P64 = batch_p64(a.P);
     // Only visible (view, Gaussian) keys go to the sort: each wave appends
     // its own to a segment of 64 * V slots (ballot ranks, no atomics) and
@@ -452,10 +472,6 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
 // Backward: backward.cu:144-396 fused (cov2D chain, projection chain, SH, cov3D).
 // Every output entry of every Gaussian is written (zeros where radius == 0).
 // ---------------------------------------------------------------------------
-struct ShReg {
-    const float* r;
-    __device__ float operator()(int f) const { return r[f]; }
-};
 
 // Backward of one Gaussian in one view, accumulated into dm (dL/dmean3D),
 // dcov (dL/dcov3D) and the Gaussian's dL/dsh column s_out[f * kPitch + t].
