@@ -172,8 +172,8 @@ class BatchRasterizer:
              _lib.stream_handle(self.device))
 
     def _backward_launch(self, dL_dpix, out):
-        self.dL_dmeans2D.zero_()
-        self.dL_dcolors.zero_()
+        # dL_dmeans2D / dL_dcolors need no clearing: the backward clears the
+        # visible elements and no other element is read
         call("fgs_batch_backward", ctypes.byref(self.settings), ctypes.byref(self._g),
              ctypes.byref(self.sizes), self.plan.data_ptr(), self.radii.data_ptr(),
              self.geom.data_ptr(), self.image.data_ptr(), ptr(self.binning), self.K_cap,

@@ -1600,10 +1600,13 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
     const fgs::BatchView* views = static_cast<const fgs::BatchView*>(plan);
     Geom gm = carve_geom(const_cast<void*>(geom), (int)n, nullptr, true);
     Image im = carve_image_n(const_cast<void*>(image), (size_t)sz->pixels, (size_t)T, nullptr);
+    // per-element blend gradients: only the visible elements are cleared
+    // (nothing reads the others)
     float* dconic = static_cast<float*>(ws);
     float* dop_view = reinterpret_cast<float*>(static_cast<char*>(ws) + align256(n * 3 * sizeof(float)));
-    if ((rc = check(fgs::launch_fill_u32(dconic, 0u, n * 3, st), "ws clear"))) return rc;
-    if ((rc = check(fgs::launch_fill_u32(dop_view, 0u, n, st), "ws clear"))) return rc;
+    if ((rc = check(fgs::launch_clear_elements(gm.vis_masks, n / FGS_WAVE, dL_dmeans2D, dconic, dL_dcolors,
+                                               dop_view, st), "element clear")))
+        return rc;
     if (K_cap > 0) {
         Binning bn = carve_binning(const_cast<void*>(binning), K_cap, nullptr);
         const BatchPairs bp = batch_pairs(bn, K_cap);

@@ -1239,6 +1239,59 @@ struct __align__(16) BwdSlot {
 };
 static_assert(sizeof(BwdSlot) == 64, "backward slot is 64 bytes");
 
+// A Gaussian of more than kGatherSolo tiles owns at least 9 * (kGatherSolo + 1)
+// floats of partial slots and uses none of them for partials: its sums accumulate
+// there in kBigCopies copies (chosen by tile) of nine terms, each term on
+// its own 128-byte line, so the float atomics of its tiles spread over 36
+// lines instead of meeting on one.
+constexpr int kBigLine = 32;                        // floats per 128-byte line
+constexpr int kBigCopies = 4;
+constexpr int kBigCopyStride = 9 * kBigLine;        // floats per copy
+static_assert(kBigCopies * kBigCopyStride <= 9 * (kGatherSolo + 1), "copies fit the slots of a large Gaussian");
+__device__ __forceinline__ int big_copy(int tx, int ty) { return (tx + 2 * ty) & (kBigCopies - 1); }
+
+// Publishes the nine sums of a (tile, Gaussian) pair: the opacity, the conic
+// and the screen-space constants applied, then stored at the pair's pre-sort
+// position for the gather (rectangles of at most kGatherSolo tiles), or, for
+// a larger rectangle, added with global float atomics into the partial slot
+// of the Gaussian's first pair (its other slots stay unused); either way the
+// gather writes the Gaussian's totals.  recs[g].w is the first pair's
+// position.
+__device__ __forceinline__ void publish_pair(const BlendArgs& a, int tx, int ty, uint32_t g, float o,
+                                             float4 con, const float* t)
+{
+    const float ddelx_dx = 0.5f * (float)a.W;
+    const float ddely_dy = 0.5f * (float)a.H;
+    const float sx = o * t[3], sy = o * t[4];
+    const float ho = -0.5f * o;
+    float d[kGradTerms];
+    d[0] = t[0];
+    d[1] = t[1];
+    d[2] = t[2];
+    d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
+    d[4] = -ddely_dy * (con.y * sx + con.z * sy);
+    d[5] = ho * t[5];
+    d[6] = ho * t[6];
+    d[7] = ho * t[7];
+    d[8] = t[8];
+    const uint4 rc = a.recs[g];
+    if (rc.z <= kGatherSolo) {
+        const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
+        const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
+        if (p < a.part_cap) {
+            float* dst = a.part + (size_t)kGradTerms * p;
+#pragma unroll
+            for (int e = 0; e < kGradTerms; ++e) dst[e] = d[e];
+            a.mark[p] = 1;
+        }
+    } else if ((size_t)rc.w + kGatherSolo < a.part_cap) {   // its slots lie inside the buffer
+        float* acc = a.part + (size_t)kGradTerms * rc.w + big_copy(tx, ty) * kBigCopyStride;
+#pragma unroll
+        for (int e = 0; e < kGradTerms; ++e)
+            if (d[e] != 0.f) atomic_add_f32(&acc[e * kBigLine], d[e]);
+    }
+}
+
 // Backward of one tile (one workgroup of FGS_BLOCK lanes).
 //
 // Per pixel the walk goes back to front from T_final like the reference
@@ -1300,8 +1353,6 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
     for (int w = 0; w < kStrips; ++w) bmax = s_wmax[w] > bmax ? s_wmax[w] : bmax;
     const int n = (int)min(bmax, a.tile_vis[tile]);
     const int rounds = (n + BATCH - 1) / BATCH;
-    const float ddelx_dx = 0.5f * (float)a.W;
-    const float ddely_dy = 0.5f * (float)a.H;
 
     // lane roles in the reduction: lanes 0, 8, ..., 56 publish the eight
     // scattered sums, lane 1 the dopacity total
@@ -1385,94 +1436,11 @@ __device__ __forceinline__ void blend_bwd_tile(const BlendArgs& a, int tile, con
         }
         __syncthreads();
         if (bits != 0u) {
-            // the pair's gradient in the layout of the gather: dcolor x3,
-            // dmean2D x2, dconic x3, dopacity, stored at its pre-sort position
-            const uint32_t g = __float_as_uint(s_slot[q].col.w);
-            const float o = s_slot[q].g1.y;
             float t[kGradTerms];
 #pragma unroll
             for (int e = 0; e < kGradTerms; ++e) t[e] = s_g[q][e];
-            const float4 con = s_slot[q].con;
-            const float sx = o * t[3], sy = o * t[4];
-            const float ho = -0.5f * o;
-            float d[kGradTerms];
-            d[0] = t[0];
-            d[1] = t[1];
-            d[2] = t[2];
-            d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
-            d[4] = -ddely_dy * (con.y * sx + con.z * sy);
-            d[5] = ho * t[5];
-            d[6] = ho * t[6];
-            d[7] = ho * t[7];
-            d[8] = t[8];
-            const uint4 rc = a.recs[g];
-            if (rc.z <= kGatherSolo) {
-                // small rectangle: the gather sums the pairs of the Gaussian
-                const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
-                const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
-                if (p < a.part_cap) {
-                    float* dst = a.part + (size_t)kGradTerms * p;
-#pragma unroll
-                    for (int e = 0; e < kGradTerms; ++e) dst[e] = d[e];
-                    a.mark[p] = 1;
-                }
-            } else {
-                // large rectangle: many tiles, add directly
-                if (d[0] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 0], d[0]);
-                if (d[1] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 1], d[1]);
-                if (d[2] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 2], d[2]);
-                if (d[3] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 0], d[3]);
-                if (d[4] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 1], d[4]);
-                if (d[5] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 0], d[5]);
-                if (d[6] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 1], d[6]);
-                if (d[7] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 2], d[7]);
-                if (d[8] != 0.f) atomic_add_f32(&a.dL_dopacity[g], d[8]);
-            }
+            publish_pair(a, tx, ty, __float_as_uint(s_slot[q].col.w), s_slot[q].g1.y, s_slot[q].con, t);
         }
-    }
-}
-
-// Publishes the nine sums of a (tile, Gaussian) pair: the opacity, the conic
-// and the screen-space constants applied, then stored at the pair's pre-sort
-// position for the gather (rectangles of at most kGatherSolo tiles) or added
-// with global float atomics.
-__device__ __forceinline__ void publish_pair(const BlendArgs& a, int tx, int ty, uint32_t g, float o,
-                                             float4 con, const float* t)
-{
-    const float ddelx_dx = 0.5f * (float)a.W;
-    const float ddely_dy = 0.5f * (float)a.H;
-    const float sx = o * t[3], sy = o * t[4];
-    const float ho = -0.5f * o;
-    float d[kGradTerms];
-    d[0] = t[0];
-    d[1] = t[1];
-    d[2] = t[2];
-    d[3] = -ddelx_dx * (con.x * sx + con.y * sy);
-    d[4] = -ddely_dy * (con.y * sx + con.z * sy);
-    d[5] = ho * t[5];
-    d[6] = ho * t[6];
-    d[7] = ho * t[7];
-    d[8] = t[8];
-    const uint4 rc = a.recs[g];
-    if (rc.z <= kGatherSolo) {
-        const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
-        const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
-        if (p < a.part_cap) {
-            float* dst = a.part + (size_t)kGradTerms * p;
-#pragma unroll
-            for (int e = 0; e < kGradTerms; ++e) dst[e] = d[e];
-            a.mark[p] = 1;
-        }
-    } else {
-        if (d[0] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 0], d[0]);
-        if (d[1] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 1], d[1]);
-        if (d[2] != 0.f) atomic_add_f32(&a.dL_dcolors[3 * (size_t)g + 2], d[2]);
-        if (d[3] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 0], d[3]);
-        if (d[4] != 0.f) atomic_add_f32(&a.dL_dmean2D[3 * (size_t)g + 1], d[4]);
-        if (d[5] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 0], d[5]);
-        if (d[6] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 1], d[6]);
-        if (d[7] != 0.f) atomic_add_f32(&a.dL_dconic[3 * (size_t)g + 2], d[7]);
-        if (d[8] != 0.f) atomic_add_f32(&a.dL_dopacity[g], d[8]);
     }
 }
 
@@ -1648,7 +1616,10 @@ __device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* _
 // Segments of up to kGatherLane pairs are summed by their own lane; longer
 // ones (up to kGatherSolo) by the whole wave, one at a time, lanes striding
 // over the pairs and the nine sums reduced across the wave, so a long
-// segment does not hold up its wave for a hundred dependent loads.
+// segment does not hold up its wave for a hundred dependent loads.  Larger
+// rectangles read the sums the blend added into their first pair's slot.
+// Gaussians of at most kGatherSolo tiles none of whose pairs blended store
+// nothing: the outputs start at zero.
 constexpr uint32_t kGatherLane = 16;
 
 __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float* __restrict__ part,
@@ -1667,8 +1638,9 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
         const bool valid = s < live;
         const uint32_t n = valid ? gg.tiles_s[s] : 0u;
         const bool gathered = n > 0u && n <= kGatherSolo;
-        const uint32_t g = gathered ? (gg.batch_P64 ? batch_elem(gg.vkeys[s], gg.batch_P64) : (uint32_t)gg.vkeys[s]) : 0u;
-        const uint32_t off = gathered ? gg.offs[s] : 0u;
+        const uint32_t g = n > 0u ? (gg.batch_P64 ? batch_elem(gg.vkeys[s], gg.batch_P64) : (uint32_t)gg.vkeys[s])
+                                  : 0u;
+        const uint32_t off = n > 0u ? gg.offs[s] : 0u;
         if (gathered && n <= kGatherLane) {
             float t[kGradTerms] = {};
             bool any = false;
@@ -1681,6 +1653,16 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                 }
             }
             if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+        } else if (n > kGatherSolo) {
+            float t[kGradTerms] = {};
+            if ((size_t)off + kGatherSolo < cap) {
+                const float* acc = part + (size_t)kGradTerms * off;
+#pragma unroll
+                for (int c = 0; c < kBigCopies; ++c)
+#pragma unroll
+                    for (int e = 0; e < kGradTerms; ++e) t[e] += acc[c * kBigCopyStride + e * kBigLine];
+            }
+            store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
         }
         uint64_t big = __ballot(gathered && n > kGatherLane);
         while (big) {
@@ -1708,18 +1690,27 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
     }
 }
 
-__host__ inline BlendArgs a_out(const BlendArgs& a, float* dL_dmean2D, float* dL_dconic, float* dL_dopacity,
-                                float* dL_dcolors)
+// Clears the accumulation slot (first pair's partials) of every Gaussian of
+// more than kGatherSolo tiles before the blend adds into it.
+__global__ __launch_bounds__(256) void k_clear_big(GradGather gg, float* __restrict__ part, size_t cap)
 {
-    BlendArgs b = a;
-    b.dL_dmean2D = dL_dmean2D;
-    b.dL_dconic = dL_dconic;
-    b.dL_dopacity = dL_dopacity;
-    b.dL_dcolors = dL_dcolors;
-    return b;
+    const size_t live = gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < live; s += (size_t)gridDim.x * blockDim.x) {
+        if (gg.tiles_s[s] > kGatherSolo) {
+            const size_t off = gg.offs[s];
+            if (off + kGatherSolo < cap) {
+                float* acc = part + (size_t)kGradTerms * off;
+#pragma unroll
+                for (int c = 0; c < kBigCopies; ++c)
+#pragma unroll
+                    for (int e = 0; e < kGradTerms; ++e) acc[c * kBigCopyStride + e * kBigLine] = 0.f;
+            }
+        }
+    }
 }
 
-// Clear the marks, blend (with the outputs bound for large Gaussians), gather.
+
+// Clear the marks and the large Gaussians' slots, blend, gather into the outputs.
 template <typename Launch>
 hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float* dL_dmean2D,
                                 float* dL_dconic, float* dL_dopacity, float* dL_dcolors, hipStream_t s,
@@ -1727,8 +1718,13 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
 {
     hipError_t e = launch_fill_u32(a.mark, 0u, (a.part_cap + 3) / 4, s);
     if (e != hipSuccess) return e;
+    if (gg.n > 0) {
+        size_t blocks = (gg.n + 255) / 256;
+        if (blocks > kStrideBlocks) blocks = kStrideBlocks;
+        hipLaunchKernelGGL(k_clear_big, dim3((unsigned)blocks), dim3(256), 0, s, gg, a.part, a.part_cap);
+    }
     prof_kernel(true, true, s);
-    launch(a_out(a, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors));
+    launch(a);
     prof_kernel(true, false, s);
     if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
     size_t blocks = (gg.n + 255) / 256;

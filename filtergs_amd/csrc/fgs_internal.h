@@ -121,7 +121,9 @@ struct BlendArgs {
     // Backward: the nine gradient sums of a (tile, Gaussian) pair are stored
     // at part[9 p ..] with mark[p] = 1, p = the pair's position before the
     // tile sort (recs[g].w, the Gaussian's first pair, plus the row-major
-    // offset of the tile in its rectangle); p < part_cap.
+    // offset of the tile in its rectangle); p < part_cap.  A Gaussian of more
+    // than kGatherSolo tiles adds its sums into part[9 recs[g].w ..] with
+    // float atomics instead.
     const uint4* recs;
     float* part;
     uint8_t* mark;
@@ -129,12 +131,6 @@ struct BlendArgs {
     // Forward statistics (RenderStats, device layout of kStats*): null = off
     double* stats;
     float stats_thresh;        // KPC used threshold
-    // Gaussians of more than kGatherSolo tiles skip the partials and add
-    // their per-tile sums into the outputs with float atomics
-    float* dL_dmean2D;
-    float* dL_dconic;
-    float* dL_dopacity;
-    float* dL_dcolors;
     int W, H, gx, gy;
     const float2* means2D;
     const float* colors;       // [P,3]
@@ -182,6 +178,10 @@ hipError_t launch_preprocess_fwd_rows(const PreArgs& a, const PreOut& o, hipStre
 hipError_t launch_fill_u32(void* p, uint32_t v, size_t n_words, hipStream_t s);
 hipError_t launch_preprocess_fwd_batch(const PreArgs& a, const BatchView* views, int V,
                                        const PreOut& o, hipStream_t s);
+// Zeroes the blend gradients of a batch's visible elements (groups = V *
+// P64 / 64 visibility ballots): a3, b3, c3 hold three floats per element.
+hipError_t launch_clear_elements(const uint64_t* vis_masks, size_t groups, float* a3, float* b3, float* c3,
+                                 float* d1, hipStream_t s);
 hipError_t launch_preprocess_bwd_batch(const PreArgs& a, const BatchView* views, int V,
                                        const PreBwdIn& in, const float* dL_dopacity_view,
                                        const PreBwdOut& o, hipStream_t s);

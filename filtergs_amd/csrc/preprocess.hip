@@ -895,6 +895,38 @@ void k_preprocess_bwd_batch_geom(PreArgs a, const BatchView* views, int V, PreBw
     o.dL_dopacity[i] = dop;
 }
 
+// Zeroes the per-element blend gradients (three [.,3] arrays and one [.])
+// of every visible element of a batch, before the blend adds into them: one
+// wave per (view, 64-group), its visible elements being the run at the front
+// of the group's 64 slots.  Culled slots are never read and stay as they are.
+__global__ __launch_bounds__(256) void k_clear_elements(const uint64_t* __restrict__ vis_masks, size_t groups,
+                                                        float* __restrict__ a3, float* __restrict__ b3,
+                                                        float* __restrict__ c3, float* __restrict__ d1)
+{
+    const size_t grp = (size_t)blockIdx.x * (256 / FGS_WAVE) + threadIdx.x / FGS_WAVE;
+    if (grp >= groups) return;
+    const uint32_t lane = (uint32_t)lane_id();
+    if (lane >= (uint32_t)__popcll(vis_masks[grp])) return;
+    const size_t e = grp * FGS_WAVE + lane;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a3[3 * e + k] = 0.f;
+        b3[3 * e + k] = 0.f;
+        c3[3 * e + k] = 0.f;
+    }
+    d1[e] = 0.f;
+}
+
+hipError_t launch_clear_elements(const uint64_t* vis_masks, size_t groups, float* a3, float* b3, float* c3,
+                                 float* d1, hipStream_t s)
+{
+    if (groups == 0) return hipSuccess;
+    const size_t per = 256 / FGS_WAVE;
+    hipLaunchKernelGGL(k_clear_elements, dim3((unsigned)((groups + per - 1) / per)), dim3(256), 0, s, vis_masks,
+                       groups, a3, b3, c3, d1);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
     const size_t stride = (size_t)gridDim.x * blockDim.x;

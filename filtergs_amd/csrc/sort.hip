@@ -539,14 +539,6 @@ __device__ __forceinline__ void dup_rect(bool valid, uint2 rc, size_t off, uint3
     }
 }
 
-// Rectangles whose backward partials are gathered per Gaussian (at most
-// kGatherSolo tiles); only they need the position of their first pair.
-__device__ __forceinline__ bool gathered(uint2 rc)
-{
-    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
-    return x1 > x0 && y1 > y0 && (x1 - x0) * (y1 - y0) <= kGatherSolo;
-}
-
 __global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ vkeys,
                                                    const uint32_t* __restrict__ offs,
                                                    const uint2* __restrict__ rects_s, int P, int gx,
@@ -557,7 +549,7 @@ __global__ __launch_bounds__(256) void k_duplicate(const uint64_t* __restrict__ 
     const uint2 rc = valid ? rects_s[s] : make_uint2(0u, 0u);
     const uint32_t g = valid ? (uint32_t)vkeys[s] : 0u;
     const size_t off = valid ? offs[s] : 0u;
-    if (valid && gathered(rc)) recs[g].w = (uint32_t)off;
+    if (valid) recs[g].w = (uint32_t)off;
     dup_rect(valid, rc, off, g, (uint32_t)gx, 0u, 0u, PairStore64{ pairs }, ~(size_t)0);
 }
 
@@ -649,7 +641,7 @@ __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restr
     const uint32_t gx = valid ? (uint32_t)views[v].gx : 0u;
     const size_t off = valid ? offs[s] : 0u;
     const uint32_t e = batch_elem(key, batch_p64(P));
-    if (valid && gathered(rc)) recs[e].w = (uint32_t)off;
+    if (valid) recs[e].w = (uint32_t)off;
     dup_rect(valid, rc, off, e, gx, 0u, v, PairStoreBatch{ keys, vals }, cap);
 }
 

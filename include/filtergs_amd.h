@@ -341,7 +341,12 @@ typedef struct {
     size_t backward_ws_bytes; /* device bytes of the backward workspace      */
 } fgs_batch_sizes;
 
-/* Sizes for V views of P Gaussians (V <= 64, P <= 2^26, V*P < 2^31). */
+/* Sizes for V views of P Gaussians (V <= 64, P <= 2^26, V*P64 < 2^31 with
+ * P64 = P rounded up to 64, at most 2^26 tiles per view).
+ * Element layout of the per-view records and screen-space gradients of a
+ * batch: Gaussian i seen by view v is element v*P64 + 64*(i/64) + r, r = the
+ * rank of i among the Gaussians of its 64-group that view v sees (radius > 0
+ * in radii[v*P + i]); culled (view, Gaussian) pairs have no element. */
 int fgs_batch_sizes_query(const fgs_batch_view* views, int32_t V, int32_t P, fgs_batch_sizes* out);
 size_t fgs_batch_binning_bytes(int64_t K_cap);
 /* Uploads the per-view records into plan (plan_bytes, device).  Synchronises
@@ -359,8 +364,9 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
                       const void* plan, void* geom, void* image, void* binning, int64_t K_cap,
                       int32_t* radii, float* out_color, int32_t* out_point_id, float* out_weight_pixel,
                       float* out_point_weight, uint64_t* counts, void* stream);
-/* Backward of every view.  dL_dpix: the views' [3,H,W] blocks.  Per view
- * (zero on entry, accumulated): dL_dmeans2D [V,P,3], dL_dcolors [V,P,3].
+/* Backward of every view.  dL_dpix: the views' [3,H,W] blocks.  Per view,
+ * in the element layout (V*P64*3 floats each, written for every element,
+ * the slots of culled pairs left as they were): dL_dmeans2D, dL_dcolors.
  * Summed over views (fully written): dL_dopacity [P], dL_dmeans3D [P,3],
  * dL_dcov3D [P,6], dL_dsh [P,M,3], dL_dscales [P,3], dL_drot [P,4].
  * ws: backward_ws_bytes. */
