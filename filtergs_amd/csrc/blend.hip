@@ -714,13 +714,14 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
                 const float alpha = fminf(0.99f, g1.y * __builtin_amdgcn_exp2f(p2));
                 const float test_T = T[k] * (1.0f - alpha);
                 const bool hit = !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
-                const bool blend = hit && test_T >= 0.0001f;
+                const bool keep = test_T >= 0.0001f;   // test_T is never NaN: !keep is test_T < 1e-4
+                const bool blend = hit && keep;
                 // the ballot of blend from those of its compares (the ballot of
                 // a compare is its lane mask; of an "and", a materialised 0/1)
                 const uint64_t bl = __builtin_amdgcn_ballot_w64(!(p2 > 0.0f)) &
                                     __builtin_amdgcn_ballot_w64(!(alpha < 1.0f / 255.0f)) &
-                                    __builtin_amdgcn_ballot_w64(test_T >= 0.0001f);
-                pfy[k] = (hit && test_T < 0.0001f) ? kFarRow : pfy[k];   // saturated: finished
+                                    __builtin_amdgcn_ballot_w64(keep);
+                pfy[k] = (hit && !keep) ? kFarRow : pfy[k];   // saturated: finished
                 used |= (bl != 0u ? 1u : 0u) << k;
                 nbl += (uint32_t)__popcll(bl);
                 if (blend) {   // skipped by the whole wave when no lane blends
@@ -1471,7 +1472,10 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
     __shared__ float s_g[kWaveBatch][kGradTerms];
 
     float pfy[kStrips], T[kStrips], dp0[kStrips], dp1[kStrips], dp2[kStrips], tfb[kStrips];
-    float acc_d[kStrips], last_d[kStrips], last_alpha[kStrips];
+    // acc: the colour of the Gaussians already walked (those in front of
+    // the next one), dotted with dL/dpix -- the reference's accum_rec,
+    // updated as soon as a Gaussian is walked rather than at the next one
+    float acc[kStrips];
     uint32_t last[kStrips];
     uint32_t lmax = 0u;
 #pragma unroll
@@ -1489,9 +1493,7 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
         dp2[k] = inside ? dL_dpix[2 * HW + pix] : 0.f;
         const float bg_dot = a.bg[0] * dp0[k] + a.bg[1] * dp1[k] + a.bg[2] * dp2[k];
         tfb[k] = -T_final * bg_dot;
-        acc_d[k] = 0.f;
-        last_d[k] = 0.f;
-        last_alpha[k] = 0.f;
+        acc[k] = 0.f;
         lmax = last[k] > lmax ? last[k] : lmax;
     }
     const uint32_t wmax = __builtin_amdgcn_readfirstlane(wave_max_u32(lmax));
@@ -1550,29 +1552,31 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
                 const float alpha_raw = g1.y * G;
                 const float alpha = fminf(0.99f, alpha_raw);
                 const bool act = j > thr[k] && !(p2 > 0.0f) && !(alpha < 1.0f / 255.0f);
-                if (!__any(act)) continue;   // wave-uniform
+                // the ballot of act from those of its compares (lane masks)
+                if ((__builtin_amdgcn_ballot_w64(j > thr[k]) & __builtin_amdgcn_ballot_w64(!(p2 > 0.0f)) &
+                     __builtin_amdgcn_ballot_w64(!(alpha < 1.0f / 255.0f))) == 0u)
+                    continue;   // wave-uniform
                 any = true;
                 const float4 col = sp->col;
                 const float inv = __builtin_amdgcn_rcpf(1.f - alpha);
                 const float Tn = T[k] * inv;
                 const float c_d = col.x * dp0[k] + col.y * dp1[k] + col.z * dp2[k];
-                const float acc_n = acc_d[k] + last_alpha[k] * (last_d[k] - acc_d[k]);
-                const float dLa = (c_d - acc_n) * Tn + inv * tfb[k];
-                const bool flows = act && !(EXACT && alpha_raw > 0.99f);
-                const float dch = act ? alpha * Tn : 0.f;
-                const float vop = flows ? G * dLa : 0.f;
-                v[0] += dch * dp0[k];
-                v[1] += dch * dp1[k];
-                v[2] += dch * dp2[k];
-                const float vdy = vop * dy;
-                v[4] += vdy;
-                v[7] += vdy * dy;
-                vsum += vop;
-                if (act) {
+                const float cd_acc = c_d - acc[k];
+                const float dLa = cd_acc * Tn + inv * tfb[k];
+                if (act) {   // the sums and the state of the acting lanes only
+                    const float dch = alpha * Tn;
+                    v[0] += dch * dp0[k];
+                    v[1] += dch * dp1[k];
+                    v[2] += dch * dp2[k];
+                    if (!(EXACT && alpha_raw > 0.99f)) {   // exact mode: no flow through the clamp
+                        const float vop = G * dLa;
+                        const float vdy = vop * dy;
+                        v[4] += vdy;
+                        v[7] += vdy * dy;
+                        vsum += vop;
+                    }
+                    acc[k] = acc[k] + alpha * cd_acc;
                     T[k] = Tn;
-                    acc_d[k] = acc_n;
-                    last_d[k] = c_d;
-                    last_alpha[k] = alpha;
                 }
             }
             if (!any) continue;   // wave-uniform
