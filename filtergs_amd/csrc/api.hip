@@ -1527,9 +1527,11 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
             const bool even = passes % 2 == 0;
             {
                 ProfScope ps(ST_DUPLICATE, st);
-                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, n, gm.nvis, P, views,
-                                                            even ? bp.keysA : bp.keysB, even ? bp.valsA : bp.valsB,
-                                                            (size_t)K_cap, gm.recs, st),
+                // the tile sort's histogram is free until the sort: chunk owners
+                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, gm.tiles_s, n, gm.nvis,
+                                                            cnt + 1, P, views, even ? bp.keysA : bp.keysB,
+                                                            even ? bp.valsA : bp.valsB, (size_t)K_cap, bn.hist,
+                                                            gm.recs, st),
                                 "batch duplicate")))
                     return rc;
             }

@@ -249,10 +249,11 @@ hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size
 // keys.
 constexpr int kBatchKeyViewBits = 6;
 constexpr uint32_t kBatchMaxViewTiles = 1u << (32 - kBatchKeyViewBits);
+// chunk_first: scratch of cap / 4096 + 1 words (free before the tile sort).
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                                  size_t n, const unsigned long long* n_dev, int P,
-                                  const BatchView* views, uint32_t* keys, uint32_t* vals, size_t cap,
-                                  uint4* recs, hipStream_t s);
+                                  const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
+                                  const unsigned long long* K_dev, int P, const BatchView* views, uint32_t* keys,
+                                  uint32_t* vals, size_t cap, uint32_t* chunk_first, uint4* recs, hipStream_t s);
 // Stable LSD sort of n (key, value) u32 pairs on n_passes 8-bit digits of the
 // key from bit shift_lo (only the first min(n, *n_dev) with n_dev); the
 // result is in (ka, va) if n_passes is even, else in (kb, vb).

@@ -622,37 +622,129 @@ hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size
     return exclusive_scan(PlainLoader{ tiles_s }, n, offs, partials, s, counters + 1, counters, n_dev);
 }
 
+// Block-wide exclusive maximum of one u32 per thread (256 threads = 4 waves).
+__device__ __forceinline__ uint32_t block_exclusive_max(uint32_t v, uint32_t* s_wave)
+{
+    const int lane = lane_id();
+    const int wave = threadIdx.x / FGS_WAVE;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < FGS_WAVE; off <<= 1) {
+        const uint32_t n = __shfl_up(incl, off, FGS_WAVE);
+        if (lane >= off) incl = max(incl, n);
+    }
+    uint32_t excl = __shfl_up(incl, 1, FGS_WAVE);
+    if (lane == 0) excl = 0u;
+    if (lane == FGS_WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w)
+        if (w < wave) excl = max(excl, s_wave[w]);
+    __syncthreads();
+    return excl;
+}
+
+// The batched duplication works in chunks of kDupChunk output positions.
+// k_dup_chunk_owners records, per chunk, the element (depth-order slot)
+// whose pairs hold the chunk's first position; the elements' pair runs
+// [offs[s], offs[s] + tiles_s[s]) tile [0, K) without gaps.
+constexpr uint32_t kDupChunk = 4096;
+constexpr int kDupPer = kDupChunk / 256;   // positions per thread
+
+__global__ __launch_bounds__(256) void k_dup_chunk_owners(const uint32_t* __restrict__ offs,
+                                                          const uint32_t* __restrict__ tiles_s, size_t n,
+                                                          const unsigned long long* __restrict__ n_dev, size_t cap,
+                                                          uint32_t* __restrict__ chunk_first)
+{
+    const size_t live = live_count(n, n_dev);
+    for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < live; s += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t off = offs[s], end = off + tiles_s[s];
+        for (uint64_t c = (off + kDupChunk - 1) / kDupChunk; c * kDupChunk < end && c * kDupChunk < cap; ++c)
+            chunk_first[c] = (uint32_t)s;
+    }
+}
+
 // Pairs past the capacity are dropped (the caller compares K with it).
+// Block b writes output positions [b * kDupChunk, (b + 1) * kDupChunk): the
+// elements starting inside the chunk mark their first position in LDS, a
+// block maximum-scan turns the marks into the owner of every position, and
+// each thread writes positions t, t + 256, ... (every store instruction
+// covers 256 consecutive words), the tile of a position being the pair's
+// index in its owner's row-major rectangle.  No lane walks a rectangle of
+// its own, so large rectangles do not hold up a wave.
 __global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restrict__ vkeys,
                                                          const uint32_t* __restrict__ offs,
                                                          const uint2* __restrict__ rects_s, size_t n,
                                                          const unsigned long long* __restrict__ n_dev,
-                                                         int P, const BatchView* __restrict__ views,
+                                                         const unsigned long long* __restrict__ K_dev,
+                                                         const uint32_t* __restrict__ chunk_first, int P,
+                                                         const BatchView* __restrict__ views,
                                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
                                                          size_t cap, uint4* __restrict__ recs)
 {
-    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    n = live_count(n, n_dev);
-    if ((size_t)blockIdx.x * blockDim.x >= n) return;   // whole block past the live keys
-    const bool valid = s < n;
-    const uint2 rc = valid ? rects_s[s] : make_uint2(0u, 0u);
-    const uint64_t key = valid ? vkeys[s] : 0ull;
-    const uint32_t v = (uint32_t)(key >> kBatchViewShift);
-    const uint32_t gx = valid ? (uint32_t)views[v].gx : 0u;
-    const size_t off = valid ? offs[s] : 0u;
-    const uint32_t e = batch_elem(key, batch_p64(P));
-    if (valid) recs[e].w = (uint32_t)off;
-    dup_rect(valid, rc, off, e, gx, 0u, v, PairStoreBatch{ keys, vals }, cap);
+    __shared__ uint32_t s_own[kDupChunk];
+    __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    const size_t K = live_count(cap, K_dev);
+    const size_t base = (size_t)blockIdx.x * kDupChunk;
+    if (base >= K) return;   // whole chunk past the pairs
+    const size_t live = live_count(n, n_dev);
+    const int t = threadIdx.x;
+    const uint32_t s_lo = chunk_first[blockIdx.x];
+    // the elements overlapping the chunk: s_lo up to the next chunk's owner
+    const size_t s_end = base + kDupChunk < K ? (size_t)chunk_first[blockIdx.x + 1] + 1 : live;
+    const uint32_t P64 = batch_p64(P);
+#pragma unroll
+    for (int q = 0; q < kDupPer; ++q) s_own[t + 256 * q] = 0u;
+    __syncthreads();
+    for (size_t s = (size_t)s_lo + (size_t)t; s < s_end; s += 256) {
+        const uint32_t off = offs[s];
+        if ((size_t)off >= base && (size_t)off < base + kDupChunk) {
+            if (s > s_lo) s_own[off - base] = (uint32_t)(s - s_lo);
+            recs[batch_elem(vkeys[s], P64)].w = off;   // the element's first pair
+        }
+    }
+    __syncthreads();
+    // inclusive maximum over the chunk (owners grow with the position)
+    uint32_t m = 0u;
+#pragma unroll
+    for (int i = 0; i < kDupPer; ++i) {
+        m = max(m, s_own[kDupPer * t + i]);
+        s_own[kDupPer * t + i] = m;
+    }
+    const uint32_t carry = block_exclusive_max(m, s_wave);
+#pragma unroll
+    for (int i = 0; i < kDupPer; ++i) s_own[kDupPer * t + i] = max(s_own[kDupPer * t + i], carry);
+    __syncthreads();
+#pragma unroll 4
+    for (int q = 0; q < kDupPer; ++q) {
+        const uint32_t p = (uint32_t)t + 256u * (uint32_t)q;
+        const size_t pos = base + p;
+        if (pos >= K) break;
+        const size_t s = (size_t)s_lo + s_own[p];
+        const uint2 rc = rects_s[s];
+        const uint64_t key = vkeys[s];
+        const uint32_t v = (uint32_t)(key >> kBatchViewShift);
+        const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
+        const uint32_t k = (uint32_t)(pos - offs[s]);
+        const uint32_t dy = k / w;
+        const uint32_t tile = (y0 + dy) * (uint32_t)views[v].gx + x0 + (k - dy * w);
+        keys[pos] = (tile << kBatchKeyViewBits) | v;
+        vals[pos] = batch_elem(key, P64);
+    }
 }
 
 hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                                  size_t n, const unsigned long long* n_dev, int P,
-                                  const BatchView* views, uint32_t* keys, uint32_t* vals, size_t cap,
-                                  uint4* recs, hipStream_t s)
+                                  const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
+                                  const unsigned long long* K_dev, int P, const BatchView* views, uint32_t* keys,
+                                  uint32_t* vals, size_t cap, uint32_t* chunk_first, uint4* recs, hipStream_t s)
 {
     if (cap == 0 || n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys, offs,
-                       rects_s, n, n_dev, P, views, keys, vals, cap, recs);
+    size_t blocks = div_up(n, 256);
+    if (blocks > kStrideBlocks) blocks = kStrideBlocks;
+    hipLaunchKernelGGL(k_dup_chunk_owners, dim3((unsigned)blocks), dim3(256), 0, s, offs, tiles_s, n, n_dev, cap,
+                       chunk_first);
+    hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(cap, kDupChunk)), dim3(256), 0, s, vkeys, offs,
+                       rects_s, n, n_dev, K_dev, chunk_first, P, views, keys, vals, cap, recs);
     return hipGetLastError();
 }
 
