@@ -1512,7 +1512,7 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         unsigned long long* cnt = (unsigned long long*)counts;
         {
             ProfScope ps(ST_OFFSETS, st);
-            if ((rc = check(fgs::launch_sorted_records_batch(gm.vkeysB, gm.recs, n, gm.nvis, P, gm.tiles_s,
+            if ((rc = check(fgs::launch_sorted_records_batch(gm.vkeysB, gm.recs, n, gm.nvis, P, views, gm.tiles_s,
                                                              gm.rects_s, st), "batch records")))
                 return rc;
             if ((rc = check(fgs::launch_scan_tiles_batch(gm.tiles_s, gm.offs, n, gm.nvis, gm.partials, cnt, st),
@@ -1522,24 +1522,27 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         if (K_cap > 0) {
             Binning bn = carve_binning(binning, K_cap, nullptr);
             const BatchPairs bp = batch_pairs(bn, K_cap);
-            // keys carry the view's own tile number: the largest view sets the passes
+            // keys carry the view's own tile number: the largest view sets the
+            // passes; the first is fused with the duplication and the result
+            // ends in A
             const int passes = (bit_length((uint32_t)sz->max_view_tiles) + 7) / 8;
             const bool even = passes % 2 == 0;
+            uint32_t* k1 = even ? bp.keysB : bp.keysA;
+            uint32_t* v1 = even ? bp.valsB : bp.valsA;
             {
                 ProfScope ps(ST_DUPLICATE, st);
-                // the tile sort's histogram is free until the sort: chunk owners
-                if ((rc = check(fgs::launch_duplicate_batch(gm.vkeysB, gm.offs, gm.rects_s, gm.tiles_s, n, gm.nvis,
-                                                            cnt + 1, P, views, even ? bp.keysA : bp.keysB,
-                                                            even ? bp.valsA : bp.valsB, (size_t)K_cap, bn.hist,
-                                                            gm.recs, st),
+                // the pair terms are free until the backward: chunk owners
+                if ((rc = check(fgs::launch_duplicate_sort_batch(
+                                    gm.vkeysB, gm.offs, gm.rects_s, gm.tiles_s, n, gm.nvis, cnt + 1, P, views,
+                                    (size_t)K_cap, reinterpret_cast<uint32_t*>(bn.part), gm.recs,
+                                    fgs::kBatchKeyViewBits, k1, v1, bn.hist, bn.partials, st),
                                 "batch duplicate")))
                     return rc;
             }
-            {
+            if (passes > 1) {
                 ProfScope ps(ST_TILE_SORT, st);
-                if ((rc = check(fgs::radix_sort_kv32(even ? bp.keysA : bp.keysB, even ? bp.valsA : bp.valsB,
-                                                     even ? bp.keysB : bp.keysA, even ? bp.valsB : bp.valsA,
-                                                     (size_t)K_cap, fgs::kBatchKeyViewBits, passes, bn.hist,
+                if ((rc = check(fgs::radix_sort_kv32(k1, v1, even ? bp.keysA : bp.keysB, even ? bp.valsA : bp.valsB,
+                                                     (size_t)K_cap, fgs::kBatchKeyViewBits + 8, passes - 1, bn.hist,
                                                      bn.partials, st, cnt + 1), "batch tile sort")))
                     return rc;
             }

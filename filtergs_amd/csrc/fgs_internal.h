@@ -235,10 +235,11 @@ __device__ __forceinline__ uint32_t batch_slot(const uint64_t* __restrict__ vis_
     return (uint32_t)v * P64 + ((uint32_t)i & ~63u) + (uint32_t)__popcll(m & ((1ull << l) - 1ull));
 }
 // The batched binning stages run over the first *n_dev of n slots (the
-// visible (view, Gaussian) keys the depth sort left).
+// visible (view, Gaussian) keys the depth sort left).  rects_s: each
+// rectangle as (first tile, width | the view's tiles per row << 16).
 hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n,
-                                       const unsigned long long* n_dev, int P, uint32_t* tiles_s,
-                                       uint2* rects_s, hipStream_t s);
+                                       const unsigned long long* n_dev, int P, const BatchView* views,
+                                       uint32_t* tiles_s, uint2* rects_s, hipStream_t s);
 hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n,
                                    const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s);
@@ -249,11 +250,16 @@ hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size
 // keys.
 constexpr int kBatchKeyViewBits = 6;
 constexpr uint32_t kBatchMaxViewTiles = 1u << (32 - kBatchKeyViewBits);
-// chunk_first: scratch of cap / 4096 + 1 words (free before the tile sort).
-hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                                  const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
-                                  const unsigned long long* K_dev, int P, const BatchView* views, uint32_t* keys,
-                                  uint32_t* vals, size_t cap, uint32_t* chunk_first, uint4* recs, hipStream_t s);
+// The duplication of the first min(n, *n_dev) depth-ordered keys into their
+// min(cap, *K_dev) kv pairs, fused with the first tile-sort pass: the pairs
+// land in (keys, vals) ordered by the digit at bit shift, without being
+// written in duplication order.  chunk_first: scratch of cap / 4096 + 1
+// words; hist / partials: the sort's histogram and scan scratch.
+hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
+                                       const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
+                                       const unsigned long long* K_dev, int P, const BatchView* views, size_t cap,
+                                       uint32_t* chunk_first, uint4* recs, int shift, uint32_t* keys,
+                                       uint32_t* vals, uint32_t* hist, uint32_t* partials, hipStream_t s);
 // Stable LSD sort of n (key, value) u32 pairs on n_passes 8-bit digits of the
 // key from bit shift_lo (only the first min(n, *n_dev) with n_dev); the
 // result is in (ka, va) if n_passes is even, else in (kb, vb).

@@ -594,24 +594,29 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
 __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __restrict__ vkeys,
                                                               const uint4* __restrict__ recs, size_t n,
                                                               const unsigned long long* __restrict__ n_dev,
-                                                              int P, uint32_t* __restrict__ tiles_s,
+                                                              int P, const BatchView* __restrict__ views,
+                                                              uint32_t* __restrict__ tiles_s,
                                                               uint2* __restrict__ rects_s)
 {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= live_count(n, n_dev)) return;
     // only visible (view, Gaussian) keys reach the sort
-    const uint4 r = recs[batch_elem(vkeys[s], batch_p64(P))];
+    const uint64_t key = vkeys[s];
+    const uint4 r = recs[batch_elem(key, batch_p64(P))];
+    const uint32_t gx = (uint32_t)views[(uint32_t)(key >> kBatchViewShift)].gx;
+    const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16;
     tiles_s[s] = r.z;
-    rects_s[s] = make_uint2(r.x, r.y);
+    // the rectangle as its first tile and row width | view row stride << 16
+    rects_s[s] = make_uint2(y0 * gx + x0, ((r.y & 0xFFFFu) - x0) | (gx << 16));
 }
 
 hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n,
-                                       const unsigned long long* n_dev, int P, uint32_t* tiles_s,
-                                       uint2* rects_s, hipStream_t s)
+                                       const unsigned long long* n_dev, int P, const BatchView* views,
+                                       uint32_t* tiles_s, uint2* rects_s, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_sorted_records_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys,
-                       recs, n, n_dev, P, tiles_s, rects_s);
+                       recs, n, n_dev, P, views, tiles_s, rects_s);
     return hipGetLastError();
 }
 
@@ -650,6 +655,13 @@ __device__ __forceinline__ uint32_t block_exclusive_max(uint32_t v, uint32_t* s_
 // [offs[s], offs[s] + tiles_s[s]) tile [0, K) without gaps.
 constexpr uint32_t kDupChunk = 4096;
 constexpr int kDupPer = kDupChunk / 256;   // positions per thread
+static_assert(kDupChunk == (uint32_t)kChunk, "a duplication chunk is one sort block");
+// The owner table in LDS holds position p at own_slot(p): one pad word per
+// 16 positions, so the scan's threads (16 consecutive positions each) read
+// distinct banks.
+constexpr uint32_t kDupOwnWords = kDupChunk + kDupChunk / kDupPer;
+__device__ __forceinline__ uint32_t own_slot(uint32_t p) { return p + p / kDupPer; }
+static_assert(kDupOwnWords <= 2u * (uint32_t)kChunk, "the owner table fits the sort block's LDS");
 
 __global__ __launch_bounds__(256) void k_dup_chunk_owners(const uint32_t* __restrict__ offs,
                                                           const uint32_t* __restrict__ tiles_s, size_t n,
@@ -662,90 +674,6 @@ __global__ __launch_bounds__(256) void k_dup_chunk_owners(const uint32_t* __rest
         for (uint64_t c = (off + kDupChunk - 1) / kDupChunk; c * kDupChunk < end && c * kDupChunk < cap; ++c)
             chunk_first[c] = (uint32_t)s;
     }
-}
-
-// Pairs past the capacity are dropped (the caller compares K with it).
-// Block b writes output positions [b * kDupChunk, (b + 1) * kDupChunk): the
-// elements starting inside the chunk mark their first position in LDS, a
-// block maximum-scan turns the marks into the owner of every position, and
-// each thread writes positions t, t + 256, ... (every store instruction
-// covers 256 consecutive words), the tile of a position being the pair's
-// index in its owner's row-major rectangle.  No lane walks a rectangle of
-// its own, so large rectangles do not hold up a wave.
-__global__ __launch_bounds__(256) void k_duplicate_batch(const uint64_t* __restrict__ vkeys,
-                                                         const uint32_t* __restrict__ offs,
-                                                         const uint2* __restrict__ rects_s, size_t n,
-                                                         const unsigned long long* __restrict__ n_dev,
-                                                         const unsigned long long* __restrict__ K_dev,
-                                                         const uint32_t* __restrict__ chunk_first, int P,
-                                                         const BatchView* __restrict__ views,
-                                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                                         size_t cap, uint4* __restrict__ recs)
-{
-    __shared__ uint32_t s_own[kDupChunk];
-    __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
-    const size_t K = live_count(cap, K_dev);
-    const size_t base = (size_t)blockIdx.x * kDupChunk;
-    if (base >= K) return;   // whole chunk past the pairs
-    const size_t live = live_count(n, n_dev);
-    const int t = threadIdx.x;
-    const uint32_t s_lo = chunk_first[blockIdx.x];
-    // the elements overlapping the chunk: s_lo up to the next chunk's owner
-    const size_t s_end = base + kDupChunk < K ? (size_t)chunk_first[blockIdx.x + 1] + 1 : live;
-    const uint32_t P64 = batch_p64(P);
-#pragma unroll
-    for (int q = 0; q < kDupPer; ++q) s_own[t + 256 * q] = 0u;
-    __syncthreads();
-    for (size_t s = (size_t)s_lo + (size_t)t; s < s_end; s += 256) {
-        const uint32_t off = offs[s];
-        if ((size_t)off >= base && (size_t)off < base + kDupChunk) {
-            if (s > s_lo) s_own[off - base] = (uint32_t)(s - s_lo);
-            recs[batch_elem(vkeys[s], P64)].w = off;   // the element's first pair
-        }
-    }
-    __syncthreads();
-    // inclusive maximum over the chunk (owners grow with the position)
-    uint32_t m = 0u;
-#pragma unroll
-    for (int i = 0; i < kDupPer; ++i) {
-        m = max(m, s_own[kDupPer * t + i]);
-        s_own[kDupPer * t + i] = m;
-    }
-    const uint32_t carry = block_exclusive_max(m, s_wave);
-#pragma unroll
-    for (int i = 0; i < kDupPer; ++i) s_own[kDupPer * t + i] = max(s_own[kDupPer * t + i], carry);
-    __syncthreads();
-#pragma unroll 4
-    for (int q = 0; q < kDupPer; ++q) {
-        const uint32_t p = (uint32_t)t + 256u * (uint32_t)q;
-        const size_t pos = base + p;
-        if (pos >= K) break;
-        const size_t s = (size_t)s_lo + s_own[p];
-        const uint2 rc = rects_s[s];
-        const uint64_t key = vkeys[s];
-        const uint32_t v = (uint32_t)(key >> kBatchViewShift);
-        const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
-        const uint32_t k = (uint32_t)(pos - offs[s]);
-        const uint32_t dy = k / w;
-        const uint32_t tile = (y0 + dy) * (uint32_t)views[v].gx + x0 + (k - dy * w);
-        keys[pos] = (tile << kBatchKeyViewBits) | v;
-        vals[pos] = batch_elem(key, P64);
-    }
-}
-
-hipError_t launch_duplicate_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
-                                  const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
-                                  const unsigned long long* K_dev, int P, const BatchView* views, uint32_t* keys,
-                                  uint32_t* vals, size_t cap, uint32_t* chunk_first, uint4* recs, hipStream_t s)
-{
-    if (cap == 0 || n == 0) return hipSuccess;
-    size_t blocks = div_up(n, 256);
-    if (blocks > kStrideBlocks) blocks = kStrideBlocks;
-    hipLaunchKernelGGL(k_dup_chunk_owners, dim3((unsigned)blocks), dim3(256), 0, s, offs, tiles_s, n, n_dev, cap,
-                       chunk_first);
-    hipLaunchKernelGGL(k_duplicate_batch, dim3((unsigned)div_up(cap, kDupChunk)), dim3(256), 0, s, vkeys, offs,
-                       rects_s, n, n_dev, K_dev, chunk_first, P, views, keys, vals, cap, recs);
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -781,44 +709,28 @@ __global__ __launch_bounds__(kThreads) void k_rs_count32(const uint32_t* __restr
     rs_count_store(s_h, hist, nb);
 }
 
-// The dense scatter of k_rs_scatter for (key, value) pairs.
-__global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __restrict__ kin,
-                                                              const uint32_t* __restrict__ vin,
-                                                              uint32_t* __restrict__ kout,
-                                                              uint32_t* __restrict__ vout, size_t n, int shift,
-                                                              const uint32_t* __restrict__ offs, size_t nb,
-                                                              const unsigned long long* n_dev)
+// The ranked write of one 4096-pair block of a (key, value) pass (the
+// dense scatter of k_rs_scatter): key[j] / val[j] are item j of the thread
+// (block position 1024 * wave + 64 j + lane; valid below live), s_glob holds
+// the global start of the block's run of each digit and s_cnt zeros.
+__device__ __forceinline__ void rs_kv32_rank_write(const uint32_t* key, const uint32_t* val, uint32_t live,
+                                                   int shift, uint32_t* __restrict__ kout,
+                                                   uint32_t* __restrict__ vout,
+                                                   uint32_t (*s_cnt)[kRadix], uint32_t* s_start,
+                                                   const uint32_t* s_glob, uint32_t* s_wave, uint32_t* s_keys,
+                                                   uint32_t* s_vals)
 {
     constexpr int kWaves = kThreads / FGS_WAVE;
     constexpr int kPerWave = kChunk / kWaves;
-    __shared__ uint32_t s_cnt[kWaves][kRadix];
-    __shared__ uint32_t s_start[kRadix];
-    __shared__ uint32_t s_glob[kRadix];
-    __shared__ uint32_t s_wave[kWaves];
-    __shared__ uint32_t s_keys[kChunk];
-    __shared__ uint32_t s_vals[kChunk];
-    n = live_count(n, n_dev);
-    const size_t base = (size_t)blockIdx.x * kChunk;
-    if (base >= n) return;
     const int t = threadIdx.x;
     const int wave = t / FGS_WAVE;
     const int lane = lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
-    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
-    __syncthreads();
-    const size_t wbase = base + (size_t)wave * kPerWave;
-    uint32_t key[kIpt], val[kIpt], rank[kIpt];
+    const uint32_t wbase = (uint32_t)wave * kPerWave;
+    uint32_t rank[kIpt];
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-        const size_t e = wbase + (size_t)j * FGS_WAVE + lane;
-        key[j] = e < n ? kin[e] : 0u;
-        val[j] = e < n ? vin[e] : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < kIpt; ++j) {
-        const bool valid = wbase + (size_t)j * FGS_WAVE + lane < n;
+        const bool valid = wbase + (uint32_t)j * FGS_WAVE + (uint32_t)lane < live;
         const uint32_t d = (key[j] >> shift) & (kRadix - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -852,7 +764,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-        if (wbase + (size_t)j * FGS_WAVE + lane < n) {
+        if (wbase + (uint32_t)j * FGS_WAVE + (uint32_t)lane < live) {
             const uint32_t d = (key[j] >> shift) & (kRadix - 1);
             const uint32_t q = s_cnt[wave][d] + rank[j];
             s_keys[q] = key[j];
@@ -860,7 +772,6 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __
         }
     }
     __syncthreads();
-    const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
     for (uint32_t p = t; p < live; p += kThreads) {
         const uint32_t k = s_keys[p];
         const uint32_t d = (k >> shift) & (kRadix - 1);
@@ -868,6 +779,210 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __
         kout[o] = k;
         vout[o] = s_vals[p];
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin,
+                                                              uint32_t* __restrict__ kout,
+                                                              uint32_t* __restrict__ vout, size_t n, int shift,
+                                                              const uint32_t* __restrict__ offs, size_t nb,
+                                                              const unsigned long long* n_dev)
+{
+    constexpr int kWaves = kThreads / FGS_WAVE;
+    constexpr int kPerWave = kChunk / kWaves;
+    __shared__ uint32_t s_cnt[kWaves][kRadix];
+    __shared__ uint32_t s_start[kRadix];
+    __shared__ uint32_t s_glob[kRadix];
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t s_keys[kChunk];
+    __shared__ uint32_t s_vals[kChunk];
+    n = live_count(n, n_dev);
+    const size_t base = (size_t)blockIdx.x * kChunk;
+    if (base >= n) return;
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+    const int lane = lane_id();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
+    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
+    __syncthreads();
+    const size_t wbase = base + (size_t)wave * kPerWave;
+    uint32_t key[kIpt], val[kIpt];
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const size_t e = wbase + (size_t)j * FGS_WAVE + lane;
+        key[j] = e < n ? kin[e] : 0u;
+        val[j] = e < n ? vin[e] : 0u;
+    }
+    const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
+    rs_kv32_rank_write(key, val, live, shift, kout, vout, s_cnt, s_start, s_glob, s_wave, s_keys, s_vals);
+}
+
+// Pairs past the capacity are dropped (the caller compares K with it).
+// Chunk b holds output positions [b * kDupChunk, (b + 1) * kDupChunk): the
+// elements starting inside the chunk mark their first position in LDS and
+// a block maximum-scan turns the marks into the owner of every position
+// (s_own[p] = owner - chunk_first[b]).  The tile of a position is then the
+// pair's index in its owner's row-major rectangle, so no lane walks a
+// rectangle of its own and large rectangles do not hold up a wave.  With
+// recs given, each element starting in the chunk records its first pair.
+struct DupChunk {
+    size_t base, K;
+    uint32_t s_lo;
+};
+
+__device__ __forceinline__ DupChunk dup_chunk_owners(const uint64_t* __restrict__ vkeys,
+                                                     const uint32_t* __restrict__ offs, size_t n,
+                                                     const unsigned long long* __restrict__ n_dev, size_t K,
+                                                     const uint32_t* __restrict__ chunk_first, uint32_t P64,
+                                                     uint4* __restrict__ recs, uint32_t* s_own, uint32_t* s_wave)
+{
+    const int t = threadIdx.x;
+    DupChunk c;
+    c.base = (size_t)blockIdx.x * kDupChunk;
+    c.K = K;
+    c.s_lo = chunk_first[blockIdx.x];
+    // the elements overlapping the chunk: s_lo up to the next chunk's owner
+    const size_t s_end = c.base + kDupChunk < K ? (size_t)chunk_first[blockIdx.x + 1] + 1 : live_count(n, n_dev);
+#pragma unroll
+    for (int q = 0; q < kDupPer; ++q) s_own[own_slot(t + 256 * q)] = 0u;
+    __syncthreads();
+    for (size_t s = (size_t)c.s_lo + (size_t)t; s < s_end; s += 256) {
+        const uint32_t off = offs[s];
+        if ((size_t)off >= c.base && (size_t)off < c.base + kDupChunk) {
+            if (s > c.s_lo) s_own[own_slot(off - (uint32_t)c.base)] = (uint32_t)(s - c.s_lo);
+            if (recs) recs[batch_elem(vkeys[s], P64)].w = off;   // the element's first pair
+        }
+    }
+    __syncthreads();
+    // inclusive maximum over the chunk (owners grow with the position)
+    uint32_t m = 0u;
+#pragma unroll
+    for (int i = 0; i < kDupPer; ++i) {
+        m = max(m, s_own[(kDupPer + 1) * t + i]);
+        s_own[(kDupPer + 1) * t + i] = m;
+    }
+    const uint32_t carry = block_exclusive_max(m, s_wave);
+#pragma unroll
+    for (int i = 0; i < kDupPer; ++i) s_own[(kDupPer + 1) * t + i] = max(s_own[(kDupPer + 1) * t + i], carry);
+    __syncthreads();
+    return c;
+}
+
+// q = k / w and r = k % w for w >= 1, k < 2^26 and q < 2^20: the float
+// estimate is within 3 of q (float(k) within 2 of k, the reciprocal within
+// an ulp) and the fix loops step it to q.
+__device__ __forceinline__ uint32_t div_rows(uint32_t k, uint32_t w, uint32_t& r)
+{
+    uint32_t q = (uint32_t)((float)k * __builtin_amdgcn_rcpf((float)w));
+    int32_t rr = (int32_t)k - (int32_t)(q * w);
+    while (rr < 0) {
+        --q;
+        rr += (int32_t)w;
+    }
+    while (rr >= (int32_t)w) {
+        ++q;
+        rr -= (int32_t)w;
+    }
+    r = (uint32_t)rr;
+    return q;
+}
+
+// The (tile << kBatchKeyViewBits | view, element) pair at chunk position p;
+// the owner's rectangle is (first tile, width | row stride << 16).
+__device__ __forceinline__ void dup_pair(const DupChunk& c, uint32_t p, const uint32_t* s_own,
+                                         const uint64_t* __restrict__ vkeys, const uint32_t* __restrict__ offs,
+                                         const uint2* __restrict__ rects_s, uint32_t P64, uint32_t& key,
+                                         uint32_t& val)
+{
+    const size_t s = (size_t)c.s_lo + s_own[own_slot(p)];
+    const uint2 rc = rects_s[s];
+    const uint64_t vk = vkeys[s];
+    const uint32_t k = (uint32_t)(c.base + p - offs[s]);
+    uint32_t dx;
+    const uint32_t dy = div_rows(k, rc.y & 0xFFFFu, dx);
+    const uint32_t tile = rc.x + dy * (rc.y >> 16) + dx;
+    key = (tile << kBatchKeyViewBits) | (uint32_t)(vk >> kBatchViewShift);
+    val = batch_elem(vk, P64);
+}
+
+// Duplication fused with the first tile-sort pass.  Counting half: the
+// chunk's pairs are generated in registers and their digits counted into
+// the pass histogram (chunks past K count nothing); the elements record
+// their first pair.
+__global__ __launch_bounds__(256) void k_dup_count_batch(const uint64_t* __restrict__ vkeys,
+                                                         const uint32_t* __restrict__ offs,
+                                                         const uint2* __restrict__ rects_s, size_t n,
+                                                         const unsigned long long* __restrict__ n_dev,
+                                                         const unsigned long long* __restrict__ K_dev,
+                                                         const uint32_t* __restrict__ chunk_first, int P,
+                                                         const BatchView* __restrict__ views, size_t cap,
+                                                         uint4* __restrict__ recs, int shift,
+                                                         uint32_t* __restrict__ hist, size_t nb)
+{
+    __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
+    __shared__ uint32_t s_own[kDupOwnWords];
+    __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+#pragma unroll
+    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][t] = 0;
+    const size_t K = live_count(cap, K_dev);
+    if ((size_t)blockIdx.x * kDupChunk < K) {
+        const uint32_t P64 = batch_p64(P);
+        const DupChunk c = dup_chunk_owners(vkeys, offs, n, n_dev, K, chunk_first, P64, recs, s_own, s_wave);
+#pragma unroll 4
+        for (int q = 0; q < kDupPer; ++q) {
+            const uint32_t p = (uint32_t)t + 256u * (uint32_t)q;
+            if (c.base + p >= K) break;
+            uint32_t key, val;
+            dup_pair(c, p, s_own, vkeys, offs, rects_s, P64, key, val);
+            atomicAdd(&s_h[wave][(key >> shift) & (kRadix - 1)], 1u);
+        }
+    }
+    rs_count_store(s_h, hist, nb);
+}
+
+// Scattering half: the chunk's pairs are generated again, in the item
+// layout of k_rs_scatter_kv32, and written ranked by digit.  The owner
+// table shares its LDS with the ranked keys.
+__global__ __launch_bounds__(256) void k_dup_scatter_batch(const uint64_t* __restrict__ vkeys,
+                                                           const uint32_t* __restrict__ offs,
+                                                           const uint2* __restrict__ rects_s, size_t n,
+                                                           const unsigned long long* __restrict__ n_dev,
+                                                           const unsigned long long* __restrict__ K_dev,
+                                                           const uint32_t* __restrict__ chunk_first, int P,
+                                                           const BatchView* __restrict__ views, size_t cap,
+                                                           int shift, const uint32_t* __restrict__ hist, size_t nb,
+                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout)
+{
+    constexpr int kWaves = kThreads / FGS_WAVE;
+    constexpr int kPerWave = kChunk / kWaves;
+    __shared__ uint32_t s_cnt[kWaves][kRadix];
+    __shared__ uint32_t s_start[kRadix];
+    __shared__ uint32_t s_glob[kRadix];
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t s_kv[2 * kChunk];   // ranked keys, then values; first the owner table
+    const size_t K = live_count(cap, K_dev);
+    if ((size_t)blockIdx.x * kDupChunk >= K) return;
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+    const int lane = lane_id();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
+    s_glob[t] = hist[(size_t)t * nb + blockIdx.x];
+    const uint32_t P64 = batch_p64(P);
+    const DupChunk c = dup_chunk_owners(vkeys, offs, n, n_dev, K, chunk_first, P64, nullptr, s_kv, s_wave);
+    const uint32_t live = (uint32_t)(K - c.base < (size_t)kChunk ? K - c.base : (size_t)kChunk);
+    uint32_t key[kIpt], val[kIpt];
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const uint32_t p = (uint32_t)wave * kPerWave + (uint32_t)j * FGS_WAVE + (uint32_t)lane;
+        key[j] = 0u;
+        val[j] = 0u;
+        if (p < live) dup_pair(c, p, s_kv, vkeys, offs, rects_s, P64, key[j], val[j]);
+    }
+    rs_kv32_rank_write(key, val, live, shift, kout, vout, s_cnt, s_start, s_glob, s_wave, s_kv, s_kv + kChunk);
 }
 
 hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, size_t n, int shift_lo,
@@ -894,25 +1009,66 @@ hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* v
     return hipGetLastError();
 }
 
+hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
+                                       const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
+                                       const unsigned long long* K_dev, int P, const BatchView* views, size_t cap,
+                                       uint32_t* chunk_first, uint4* recs, int shift, uint32_t* keys,
+                                       uint32_t* vals, uint32_t* hist, uint32_t* partials, hipStream_t s)
+{
+    if (cap == 0 || n == 0) return hipSuccess;
+    size_t blocks = div_up(n, 256);
+    if (blocks > kStrideBlocks) blocks = kStrideBlocks;
+    hipLaunchKernelGGL(k_dup_chunk_owners, dim3((unsigned)blocks), dim3(256), 0, s, offs, tiles_s, n, n_dev, cap,
+                       chunk_first);
+    const size_t nb = div_up(cap, kChunk);
+    hipLaunchKernelGGL(k_dup_count_batch, dim3((unsigned)nb), dim3(kThreads), 0, s, vkeys, offs, rects_s, n, n_dev,
+                       K_dev, chunk_first, P, views, cap, recs, shift, hist, nb);
+    hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dup_scatter_batch, dim3((unsigned)nb), dim3(kThreads), 0, s, vkeys, offs, rects_s, n,
+                       n_dev, K_dev, chunk_first, P, views, cap, shift, hist, nb, keys, vals);
+    return hipGetLastError();
+}
+
 // identifyTileRanges over the batch: a (tile, view) run is a run of equal
 // tile keys; it is stored at the view's tile_base + tile.  Only the first
-// min(cap, *K) sorted pairs exist.
+// min(cap, *K) sorted pairs exist.  Each thread takes four consecutive keys
+// (one 16-byte load) and their two neighbours.
 __global__ __launch_bounds__(256) void k_ranges_batch(const uint32_t* __restrict__ sorted, size_t cap,
                                                       const unsigned long long* __restrict__ K,
                                                       const BatchView* __restrict__ views,
                                                       uint2* __restrict__ ranges)
 {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = live_count(cap, K);
-    if (k >= n) return;
-    const uint32_t key = sorted[k];
-    const bool first = k == 0 || sorted[k - 1] != key;
-    const bool last = k == n - 1 || sorted[k + 1] != key;
-    if (!first && !last) return;
-    const uint32_t v = key & ((1u << kBatchKeyViewBits) - 1u);
-    const uint32_t gt = views[v].tile_base + (key >> kBatchKeyViewBits);
-    if (first) ranges[gt].x = (uint32_t)k;
-    if (last) ranges[gt].y = (uint32_t)(k + 1);
+    const size_t k0 = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (k0 >= n) return;
+    uint32_t c[4];
+    if (k0 + 4 <= n) {
+        const uint4 q = *reinterpret_cast<const uint4*>(sorted + k0);
+        c[0] = q.x;
+        c[1] = q.y;
+        c[2] = q.z;
+        c[3] = q.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = k0 + i < n ? sorted[k0 + i] : 0u;
+    }
+    uint32_t prev = k0 > 0 ? sorted[k0 - 1] : 0u;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const size_t k = k0 + i;
+        if (k >= n) break;
+        const uint32_t key = c[i];
+        const uint32_t next = i < 3 ? c[i + 1] : (k + 1 < n ? sorted[k + 1] : 0u);
+        const bool first = k == 0 || prev != key;
+        const bool last = k == n - 1 || next != key;
+        prev = key;
+        if (!first && !last) continue;
+        const uint32_t v = key & ((1u << kBatchKeyViewBits) - 1u);
+        const uint32_t gt = views[v].tile_base + (key >> kBatchKeyViewBits);
+        if (first) ranges[gt].x = (uint32_t)k;
+        if (last) ranges[gt].y = (uint32_t)(k + 1);
+    }
 }
 
 hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const unsigned long long* K,
@@ -920,7 +1076,7 @@ hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const un
 {
     hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
     if (e != hipSuccess || cap == 0) return e;
-    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 256)), dim3(256), 0, s, sorted_keys, cap, K,
+    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 4 * 256)), dim3(256), 0, s, sorted_keys, cap, K,
                        views, ranges);
     return hipGetLastError();
 }
