@@ -156,6 +156,10 @@ struct Geom {
     unsigned long long* nvis;       // batch: visible (view, Gaussian) keys after the depth sort
     uint64_t* vis_masks;            // batch: visibility ballot per (view, 64-group)
     uint32_t* gid;                  // batch: Gaussian index of each element
+    uint32_t* grp_tiles;            // batch: tiles per (view, 64-group)
+    uint32_t* grp_base;             // batch: first gradient slot per (view, 64-group)
+    uint32_t* big;                  // batch: elements of more than kGatherSolo tiles (in depths' words)
+    unsigned long long* nbig;       // batch: their count
 };
 
 size_t scan_words_for_hist(size_t n)
@@ -191,9 +195,17 @@ Geom carve_geom(void* buf, int P, size_t* total, bool batch = false)
     if (batch) {
         g.vis_masks = c.take<uint64_t>(n / FGS_WAVE + 1);
         g.gid = c.take<uint32_t>(n);
+        g.grp_tiles = c.take<uint32_t>(n / FGS_WAVE + 1);
+        g.grp_base = c.take<uint32_t>(n / FGS_WAVE + 1);
+        g.big = reinterpret_cast<uint32_t*>(g.depths);   // a batch keeps no per-element depth
+        g.nbig = c.take<unsigned long long>(1);
     } else {
         g.vis_masks = nullptr;
         g.gid = nullptr;
+        g.grp_tiles = nullptr;
+        g.grp_base = nullptr;
+        g.big = nullptr;
+        g.nbig = nullptr;
     }
     if (total) *total = c.off;
     return g;
@@ -822,7 +834,8 @@ static int backward_impl(const fgs_raster_settings* s, const fgs_gaussians* g, c
         b.bwd_layout = g_bwd_layout;
         b.xcd_chunk = g_xcd_chunk;
         ProfScope ps(ST_BLEND_BWD, st);
-        const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, 0, nullptr };
+        const fgs::GradGather gg{ gm.vkeysA, gm.tiles_s, gm.offs, (size_t)g->P, nullptr, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr };
         if ((rc = check(fgs::launch_blend_bwd(b, gg, im.final_T, im.vis_last, dL_dpix, dL_dmeans2D,
                                               dconic, dL_dopacity, dL_dcolors, s->grad_mode == 1, st),
                         "blend backward")))
@@ -1495,9 +1508,17 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
         o.seg_cnt = gm.seg_cnt;
         o.vis_masks = gm.vis_masks;
         o.gid = gm.gid;
+        o.grp_tiles = gm.grp_tiles;
+        o.big_list = gm.big;
+        o.big_count = gm.nbig;
         {
             ProfScope ps(ST_PREPROCESS, st);
-            if ((rc = check(fgs::launch_preprocess_fwd_batch(a, views, V, o, st), "batch preprocess")))
+            if ((rc = check(fgs::launch_fill_u32(gm.nbig, 0u, 2, st), "big count clear")) ||
+                (rc = check(fgs::launch_preprocess_fwd_batch(a, views, V, o, st), "batch preprocess")))
+                return rc;
+            // each (view, 64-group) owns a run of backward gradient slots
+            if ((rc = check(fgs::launch_scan_u32(gm.grp_tiles, ne / FGS_WAVE, gm.grp_base, gm.partials, st),
+                            "batch slot bases")))
                 return rc;
         }
         // (view, depth, index) order of the visible keys: bits 26..63, 5
@@ -1533,9 +1554,9 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
                 ProfScope ps(ST_DUPLICATE, st);
                 // the pair terms are free until the backward: chunk owners
                 if ((rc = check(fgs::launch_duplicate_sort_batch(
-                                    gm.vkeysB, gm.offs, gm.rects_s, gm.tiles_s, n, gm.nvis, cnt + 1, P, views,
-                                    (size_t)K_cap, reinterpret_cast<uint32_t*>(bn.part), gm.recs,
-                                    fgs::kBatchKeyViewBits, k1, v1, bn.hist, bn.partials, st),
+                                    gm.vkeysB, gm.offs, gm.rects_s, gm.tiles_s, n, gm.nvis, cnt + 1, P, (size_t)K_cap,
+                                    reinterpret_cast<uint32_t*>(bn.part), fgs::kBatchKeyViewBits, k1, v1, bn.hist,
+                                    bn.partials, st),
                                 "batch duplicate")))
                     return rc;
             }
@@ -1605,12 +1626,12 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
     const fgs::BatchView* views = static_cast<const fgs::BatchView*>(plan);
     Geom gm = carve_geom(const_cast<void*>(geom), (int)n, nullptr, true);
     Image im = carve_image_n(const_cast<void*>(image), (size_t)sz->pixels, (size_t)T, nullptr);
-    // per-element blend gradients: only the visible elements are cleared
-    // (nothing reads the others)
+    // per-element blend gradients: the gather stores those of every visible
+    // element (nothing reads the others); without pairs they are cleared
     float* dconic = static_cast<float*>(ws);
     float* dop_view = reinterpret_cast<float*>(static_cast<char*>(ws) + align256(n * 3 * sizeof(float)));
-    if ((rc = check(fgs::launch_clear_elements(gm.vis_masks, n / FGS_WAVE, dL_dmeans2D, dconic, dL_dcolors,
-                                               dop_view, st), "element clear")))
+    if (K_cap == 0 && (rc = check(fgs::launch_clear_elements(gm.vis_masks, n / FGS_WAVE, dL_dmeans2D, dconic,
+                                                             dL_dcolors, dop_view, st), "element clear")))
         return rc;
     if (K_cap > 0) {
         Binning bn = carve_binning(const_cast<void*>(binning), K_cap, nullptr);
@@ -1623,6 +1644,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
         b.vis_ids = bp.keysB;
         b.kv_strips = bn.strips;
         b.recs = gm.recs;
+        b.slot_base = gm.grp_base;
         b.part = bn.part;
         b.mark = bn.mark;
         b.part_cap = (size_t)K_cap;
@@ -1642,8 +1664,13 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
             b.queue_head = im.qhead;
             b.queue_chunk = (uint32_t)g_queue_chunk;
         }
-        const fgs::GradGather gg{ gm.vkeysB, gm.tiles_s, gm.offs, (size_t)V * (size_t)P, fgs::batch_p64(P),
-                                  gm.nvis };
+        // the visible elements in element order (the depth order's record
+        // arrays are free after the forward)
+        if ((rc = check(fgs::launch_visible_list(gm.vis_masks, n / FGS_WAVE, gm.offs, gm.tiles_s, gm.partials, st),
+                        "visible elements")))
+            return rc;
+        const fgs::GradGather gg{ nullptr, nullptr, nullptr, (size_t)V * (size_t)P, gm.nvis, gm.tiles_s, gm.recs,
+                                  gm.grp_base, gm.big, gm.nbig };
         if ((rc = check(fgs::launch_blend_bwd_batch(b, gg, views, V, T, im.final_T, im.vis_last, dL_dpix,
                                                     dL_dmeans2D, dconic, dop_view, dL_dcolors,
                                                     s->grad_mode == 1, st), "batch blend backward")))

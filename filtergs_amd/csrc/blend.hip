@@ -1256,8 +1256,8 @@ __device__ __forceinline__ int big_copy(int tx, int ty) { return (tx + 2 * ty) &
 // position for the gather (rectangles of at most kGatherSolo tiles), or, for
 // a larger rectangle, added with global float atomics into the partial slot
 // of the Gaussian's first pair (its other slots stay unused); either way the
-// gather writes the Gaussian's totals.  recs[g].w is the first pair's
-// position.
+// gather writes the Gaussian's totals.  The first slot is recs[g].w, plus
+// slot_base[g / 64] in a batch.
 __device__ __forceinline__ void publish_pair(const BlendArgs& a, int tx, int ty, uint32_t g, float o,
                                              float4 con, const float* t)
 {
@@ -1276,17 +1276,18 @@ __device__ __forceinline__ void publish_pair(const BlendArgs& a, int tx, int ty,
     d[7] = ho * t[7];
     d[8] = t[8];
     const uint4 rc = a.recs[g];
+    const size_t first = (size_t)rc.w + (a.slot_base ? (size_t)a.slot_base[g >> 6] : 0u);
     if (rc.z <= kGatherSolo) {
         const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, w = (rc.y & 0xFFFFu) - x0;
-        const size_t p = (size_t)rc.w + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
+        const size_t p = first + (size_t)((uint32_t)ty - y0) * w + ((uint32_t)tx - x0);
         if (p < a.part_cap) {
             float* dst = a.part + (size_t)kGradTerms * p;
 #pragma unroll
             for (int e = 0; e < kGradTerms; ++e) dst[e] = d[e];
             a.mark[p] = 1;
         }
-    } else if ((size_t)rc.w + kGatherSolo < a.part_cap) {   // its slots lie inside the buffer
-        float* acc = a.part + (size_t)kGradTerms * rc.w + big_copy(tx, ty) * kBigCopyStride;
+    } else if (first + kGatherSolo < a.part_cap) {   // its slots lie inside the buffer
+        float* acc = a.part + (size_t)kGradTerms * first + big_copy(tx, ty) * kBigCopyStride;
 #pragma unroll
         for (int e = 0; e < kGradTerms; ++e)
             if (d[e] != 0.f) atomic_add_f32(&acc[e * kBigLine], d[e]);
@@ -1597,20 +1598,22 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
     }
 }
 
-// Gather of the per-pair gradients: slot s of the depth order sums the
-// marked entries of its Gaussian's pairs [offs[s], offs[s] + tiles_s[s]) and
-// stores the nine totals (plain stores: every Gaussian has one slot).
-// Only Gaussians of up to kGatherSolo tiles take this path.
+// Gather of the per-pair gradients: each Gaussian (one view: slot s of the
+// depth order; batch: element e) sums the marked part entries of its slots
+// [off, off + n) and stores the nine totals (plain stores: every Gaussian
+// has one item).
 
+// whole: also the third (zero) component of dL_dmean2D.
 __device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* __restrict__ dL_dmean2D,
                                             float* __restrict__ dL_dconic, float* __restrict__ dL_dopacity,
-                                            float* __restrict__ dL_dcolors)
+                                            float* __restrict__ dL_dcolors, bool whole)
 {
     dL_dcolors[3 * (size_t)g + 0] = t[0];
     dL_dcolors[3 * (size_t)g + 1] = t[1];
     dL_dcolors[3 * (size_t)g + 2] = t[2];
     dL_dmean2D[3 * (size_t)g + 0] = t[3];
     dL_dmean2D[3 * (size_t)g + 1] = t[4];
+    if (whole) dL_dmean2D[3 * (size_t)g + 2] = 0.f;
     dL_dconic[3 * (size_t)g + 0] = t[5];
     dL_dconic[3 * (size_t)g + 1] = t[6];
     dL_dconic[3 * (size_t)g + 2] = t[7];
@@ -1622,9 +1625,41 @@ __device__ __forceinline__ void store_grads(uint32_t g, const float* t, float* _
 // over the pairs and the nine sums reduced across the wave, so a long
 // segment does not hold up its wave for a hundred dependent loads.  Larger
 // rectangles read the sums the blend added into their first pair's slot.
-// Gaussians of at most kGatherSolo tiles none of whose pairs blended store
-// nothing: the outputs start at zero.
+// One view: Gaussians of at most kGatherSolo tiles none of whose pairs
+// blended store nothing (the outputs start at zero); a batch stores every
+// visible element's gradients, whole, in element order.
 constexpr uint32_t kGatherLane = 16;
+
+__device__ __forceinline__ size_t gather_live(const GradGather& gg)
+{
+    return gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
+}
+
+// Item i of the gather: the Gaussian g with n tiles (0: none) whose slots
+// start at off.  A batch's items are its visible elements in element order
+// (neighbouring lanes read neighbouring records and slot runs and store to
+// neighbouring outputs).
+__device__ __forceinline__ void gather_item(const GradGather& gg, size_t i, size_t live, uint32_t& n, uint32_t& g,
+                                            uint32_t& off)
+{
+    n = 0u;
+    g = 0u;
+    off = 0u;
+    if (i >= live) return;
+    if (gg.slot_base) {
+        const uint32_t e = gg.elems[i];
+        const uint4 rc = gg.recs[e];
+        n = rc.z;
+        g = e;
+        off = gg.slot_base[e >> 6] + rc.w;
+    } else {
+        n = gg.tiles_s[i];
+        if (n > 0u) {
+            g = (uint32_t)gg.vkeys[i];
+            off = gg.offs[i];
+        }
+    }
+}
 
 __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float* __restrict__ part,
                                                      const uint8_t* __restrict__ mark, size_t cap,
@@ -1635,16 +1670,13 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
 {
     // grid-stride in whole blocks (the wave steps below need every lane):
     // with a device count the grid is sized for the capacity
-    const size_t live = gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
+    const size_t live = gather_live(gg);
     const int lane = lane_id();
+    const bool all = gg.slot_base != nullptr;
     for (size_t b0 = (size_t)blockIdx.x * blockDim.x; b0 < live; b0 += (size_t)gridDim.x * blockDim.x) {
-        const size_t s = b0 + threadIdx.x;
-        const bool valid = s < live;
-        const uint32_t n = valid ? gg.tiles_s[s] : 0u;
+        uint32_t n, g, off;
+        gather_item(gg, b0 + threadIdx.x, live, n, g, off);
         const bool gathered = n > 0u && n <= kGatherSolo;
-        const uint32_t g = n > 0u ? (gg.batch_P64 ? batch_elem(gg.vkeys[s], gg.batch_P64) : (uint32_t)gg.vkeys[s])
-                                  : 0u;
-        const uint32_t off = n > 0u ? gg.offs[s] : 0u;
         if (gathered && n <= kGatherLane) {
             float t[kGradTerms] = {};
             bool any = false;
@@ -1656,7 +1688,7 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                     for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
                 }
             }
-            if (any) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+            if (any || all) store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, all);
         } else if (n > kGatherSolo) {
             float t[kGradTerms] = {};
             if ((size_t)off + kGatherSolo < cap) {
@@ -1666,7 +1698,7 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
 #pragma unroll
                     for (int e = 0; e < kGradTerms; ++e) t[e] += acc[c * kBigCopyStride + e * kBigLine];
             }
-            store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+            store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, all);
         }
         uint64_t big = __ballot(gathered && n > kGatherLane);
         while (big) {
@@ -1685,23 +1717,34 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
                     for (int e = 0; e < kGradTerms; ++e) t[e] += part[(size_t)kGradTerms * p + e];
                 }
             }
-            if (__any(mine)) {   // wave-uniform
+            if (__any(mine) || all) {   // wave-uniform
 #pragma unroll
                 for (int e = 0; e < kGradTerms; ++e) t[e] = wave_total(t[e]);
-                if (lane == src) store_grads(bg, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+                if (lane == src) store_grads(bg, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, all);
             }
         }
     }
 }
 
-// Clears the accumulation slot (first pair's partials) of every Gaussian of
-// more than kGatherSolo tiles before the blend adds into it.
+// Clears the accumulation slot (first slot's partials) of every Gaussian of
+// more than kGatherSolo tiles before the blend adds into it (a batch: those
+// of the big list).
 __global__ __launch_bounds__(256) void k_clear_big(GradGather gg, float* __restrict__ part, size_t cap)
 {
-    const size_t live = gg.n_dev && (size_t)*gg.n_dev < gg.n ? (size_t)*gg.n_dev : gg.n;
+    const size_t live = gg.big ? (size_t)*gg.n_big : gather_live(gg);
     for (size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x; s < live; s += (size_t)gridDim.x * blockDim.x) {
-        if (gg.tiles_s[s] > kGatherSolo) {
-            const size_t off = gg.offs[s];
+        uint32_t n, o;
+        if (gg.big) {
+            const uint32_t e = gg.big[s];
+            const uint4 rc = gg.recs[e];
+            n = rc.z;
+            o = gg.slot_base[e >> 6] + rc.w;
+        } else {
+            n = gg.tiles_s[s];
+            o = n > kGatherSolo ? gg.offs[s] : 0u;
+        }
+        if (n > kGatherSolo) {
+            const size_t off = o;
             if (off + kGatherSolo < cap) {
                 float* acc = part + (size_t)kGradTerms * off;
 #pragma unroll
@@ -1724,7 +1767,7 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
     if (e != hipSuccess) return e;
     if (gg.n > 0) {
         size_t blocks = (gg.n + 255) / 256;
-        if (blocks > kStrideBlocks) blocks = kStrideBlocks;
+        if (blocks > (gg.big ? 256u : kStrideBlocks)) blocks = gg.big ? 256u : kStrideBlocks;
         hipLaunchKernelGGL(k_clear_big, dim3((unsigned)blocks), dim3(256), 0, s, gg, a.part, a.part_cap);
     }
     prof_kernel(true, true, s);

@@ -53,6 +53,13 @@ struct PreOut {
     uint32_t* seg_cnt;                  // batch: keys each wave appended to its segment
     uint64_t* vis_masks;                // batch: per (view, 64-group) visibility ballot
     uint32_t* gid;                      // batch: Gaussian index of each element
+    // batch: tiles of each (view, 64-group); recs[e].w of an element is its
+    // group's tiles before it (the offset of its slots in the group's run)
+    uint32_t* grp_tiles;
+    // batch: the elements of more than kGatherSolo tiles, appended in any
+    // order (big_count cleared by the caller)
+    uint32_t* big_list;
+    unsigned long long* big_count;
 };
 
 struct PreBwdIn {
@@ -119,12 +126,15 @@ struct BlendArgs {
     // reported in point ids and used for point_weight[id_base + i]
     const uint32_t* gid;
     // Backward: the nine gradient sums of a (tile, Gaussian) pair are stored
-    // at part[9 p ..] with mark[p] = 1, p = the pair's position before the
-    // tile sort (recs[g].w, the Gaussian's first pair, plus the row-major
-    // offset of the tile in its rectangle); p < part_cap.  A Gaussian of more
-    // than kGatherSolo tiles adds its sums into part[9 recs[g].w ..] with
-    // float atomics instead.
+    // at part[9 p ..] with mark[p] = 1, p = the Gaussian's first slot plus
+    // the row-major offset of the tile in its rectangle; p < part_cap.  The
+    // first slot is recs[g].w (one view: the Gaussian's first pair before
+    // the tile sort), plus slot_base[g / 64] when slot_base is given (batch:
+    // the slots run per (view, 64-group) of the element layout).  A Gaussian
+    // of more than kGatherSolo tiles adds its sums into the part entries of
+    // its first slot with float atomics instead.
     const uint4* recs;
+    const uint32_t* slot_base;
     float* part;
     uint8_t* mark;
     size_t part_cap;
@@ -240,6 +250,13 @@ __device__ __forceinline__ uint32_t batch_slot(const uint64_t* __restrict__ vis_
 hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n,
                                        const unsigned long long* n_dev, int P, const BatchView* views,
                                        uint32_t* tiles_s, uint2* rects_s, hipStream_t s);
+// out[i] = in[0] + ... + in[i - 1] for i < n (u32).
+hipError_t launch_scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* partials, hipStream_t s);
+// The visible elements of a batch in element order: grp_start[g] = visible
+// elements of the (view, 64-group)s before g, list[grp_start[g] + r] =
+// 64 g + r for r below group g's visible count.
+hipError_t launch_visible_list(const uint64_t* vis_masks, size_t n_groups, uint32_t* grp_start, uint32_t* list,
+                               uint32_t* partials, hipStream_t s);
 hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n,
                                    const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s);
@@ -257,9 +274,9 @@ constexpr uint32_t kBatchMaxViewTiles = 1u << (32 - kBatchKeyViewBits);
 // words; hist / partials: the sort's histogram and scan scratch.
 hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                        const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
-                                       const unsigned long long* K_dev, int P, const BatchView* views, size_t cap,
-                                       uint32_t* chunk_first, uint4* recs, int shift, uint32_t* keys,
-                                       uint32_t* vals, uint32_t* hist, uint32_t* partials, hipStream_t s);
+                                       const unsigned long long* K_dev, int P, size_t cap, uint32_t* chunk_first,
+                                       int shift, uint32_t* keys, uint32_t* vals, uint32_t* hist,
+                                       uint32_t* partials, hipStream_t s);
 // Stable LSD sort of n (key, value) u32 pairs on n_passes 8-bit digits of the
 // key from bit shift_lo (only the first min(n, *n_dev) with n_dev); the
 // result is in (ka, va) if n_passes is even, else in (kb, vb).
@@ -275,12 +292,11 @@ hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const un
 hipError_t launch_blend_fwd(const BlendArgs& a, float* final_T, uint32_t* n_contrib, uint32_t* vis_last,
                             float* out_color, int* out_point_id, float* out_weight_pixel,
                             float* point_weight, float* kv_sum, uint32_t* kv_cnt, hipStream_t s);
-// The per-Gaussian sums of the backward's per-pair partials, gathered in
-// depth order from the Gaussians' pre-sort pair segments (slot s: key
-// vkeys[s], tiles_s[s] pairs from offs[s]; batch_P64 > 0: keys of a batch).
-// Gaussians of up to kGatherSolo tiles take this path, larger ones add their
-// per-tile sums with global float atomics (at 16 the atomics of the close
-// orbit views cost more than the longer per-lane gather at 128 does).
+// The per-Gaussian sums of the backward's per-pair partials are gathered
+// from each Gaussian's run of slots (GradGather).  Gaussians of up to
+// kGatherSolo tiles store one slot per pair, larger ones add their per-tile
+// sums with global float atomics (at 16 the atomics of the close orbit views
+// cost more than the longer per-lane gather at 128 does).
 constexpr uint32_t kGatherSolo = 128;
 
 // RenderStats on the device (rasterizer.h:22-36, packed as
@@ -294,13 +310,24 @@ constexpr int kKpcBins = 11;
 constexpr size_t kStatsHead = 9 + 2 * kStatsMaxBins;
 inline size_t stats_words(size_t T) { return kStatsHead + 3 * T + 2 + kKpcBins; }
 hipError_t launch_stats_finalize(double* stats, int T, int P, int64_t K, hipStream_t s);
+// One view: slot s of the depth order is Gaussian vkeys[s] (low 32 bits)
+// with tiles_s[s] tiles and first pair offs[s].  Batch (slot_base given):
+// item j is element e = elems[j] (the visible elements in element order),
+// with recs[e].z tiles and first slot slot_base[e / 64] + recs[e].w; the
+// gather stores every item's gradients (zero where no pair blended), and the
+// elements of more than kGatherSolo tiles are big[0 .. *n_big).  The first
+// *n_dev (null: n) of the n items are in use.
 struct GradGather {
     const uint64_t* vkeys;
     const uint32_t* tiles_s;
     const uint32_t* offs;
     size_t n;
-    uint32_t batch_P64;                // batch: P rounded up to 64 (element layout); 0: one view
-    const unsigned long long* n_dev;   // batch: the first *n_dev of the n slots are in use
+    const unsigned long long* n_dev;
+    const uint32_t* elems;
+    const uint4* recs;
+    const uint32_t* slot_base;
+    const uint32_t* big;
+    const unsigned long long* n_big;
 };
 hipError_t launch_blend_bwd(const BlendArgs& a, const GradGather& gg, const float* final_T,
                             const uint32_t* vis_last, const float* dL_dpix, float* dL_dmean2D,

@@ -441,11 +441,24 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
         const bool vis = r.tiles != 0;
         const uint64_t m = __ballot(vis);
         const uint32_t rank = (uint32_t)__popcll(m & below);
+        // the group's tiles before this lane (the lanes past P have exited:
+        // they are all above the live ones, which read only lanes below)
+        uint32_t tincl = vis ? r.tiles : 0u;
+        if (m != 0u) {
+#pragma unroll
+            for (int d = 1; d < FGS_WAVE; d <<= 1) {
+                const uint32_t u = __shfl_up(tincl, d, FGS_WAVE);
+                if (lane >= d) tincl += u;
+            }
+        }
         if (vis) {
-            // no depth or tile count: the depth travels in the key, the tile
-            // count in the record
+            // no depth: the depth travels in the key; the record holds the
+            // tile count and the element's slot offset in its group's run
             const size_t e = (size_t)v * P64 + w64 + rank;
-            o.recs[e] = r.rec;
+            uint4 rec = r.rec;
+            rec.w = tincl - r.tiles;
+            o.recs[e] = rec;
+            if (r.tiles > kGatherSolo) o.big_list[atomicAdd(o.big_count, 1ull)] = (uint32_t)e;
             o.means2D[e] = r.mean2d;
             o.conic_opacity[e] = r.con;
             o.gid[e] = (uint32_t)i;
@@ -462,7 +475,11 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
             seg[appended + rank] =
                 ((uint64_t)v << 58) | ((uint64_t)__float_as_uint(r.depth) << 26) | (w64 + rank);
         }
-        if (lane == 0) o.vis_masks[(size_t)v * (P64 >> 6) + (i >> 6)] = m;
+        const uint32_t gtot = __builtin_amdgcn_readlane(tincl, 63 - __clzll(__ballot(true)));
+        if (lane == 0) {
+            o.vis_masks[(size_t)v * (P64 >> 6) + (i >> 6)] = m;
+            o.grp_tiles[(size_t)v * (P64 >> 6) + (i >> 6)] = gtot;
+        }
         appended += (uint32_t)__popcll(m);
     }
     if (lane == 0) o.seg_cnt[i / FGS_WAVE] = appended;

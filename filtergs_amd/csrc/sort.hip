@@ -620,6 +620,46 @@ hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs,
     return hipGetLastError();
 }
 
+hipError_t launch_scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* partials, hipStream_t s)
+{
+    return exclusive_scan(PlainLoader{ in }, n, out, partials, s);
+}
+
+struct PopLoader {
+    const uint64_t* masks;
+    __device__ uint32_t operator()(size_t s) const { return (uint32_t)__popcll(masks[s]); }
+};
+
+// One wave per 64 groups: lane l loads group l's mask and start, then the
+// wave writes the groups' visible elements one group after the other.
+__global__ __launch_bounds__(256) void k_visible_list(const uint64_t* __restrict__ vis_masks, size_t n_groups,
+                                                      const uint32_t* __restrict__ grp_start,
+                                                      uint32_t* __restrict__ list)
+{
+    const size_t g0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) & ~(size_t)(FGS_WAVE - 1);
+    if (g0 >= n_groups) return;
+    const uint32_t lane = (uint32_t)lane_id();
+    const size_t gl = g0 + lane;
+    const uint32_t cnt = gl < n_groups ? (uint32_t)__popcll(vis_masks[gl]) : 0u;
+    const uint32_t start = gl < n_groups ? grp_start[gl] : 0u;
+    const uint32_t ng = n_groups - g0 < (size_t)FGS_WAVE ? (uint32_t)(n_groups - g0) : (uint32_t)FGS_WAVE;
+    for (uint32_t k = 0; k < ng; ++k) {
+        const uint32_t c = __builtin_amdgcn_readlane(cnt, k);
+        if (lane < c) list[__builtin_amdgcn_readlane(start, k) + lane] = (uint32_t)((g0 + k) * FGS_WAVE) + lane;
+    }
+}
+
+hipError_t launch_visible_list(const uint64_t* vis_masks, size_t n_groups, uint32_t* grp_start, uint32_t* list,
+                               uint32_t* partials, hipStream_t s)
+{
+    if (n_groups == 0) return hipSuccess;
+    hipError_t e = exclusive_scan(PopLoader{ vis_masks }, n_groups, grp_start, partials, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_visible_list, dim3((unsigned)div_up(n_groups, 256)), dim3(256), 0, s, vis_masks, n_groups,
+                       grp_start, list);
+    return hipGetLastError();
+}
+
 hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size_t n,
                                    const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s)
@@ -824,18 +864,16 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __
 // a block maximum-scan turns the marks into the owner of every position
 // (s_own[p] = owner - chunk_first[b]).  The tile of a position is then the
 // pair's index in its owner's row-major rectangle, so no lane walks a
-// rectangle of its own and large rectangles do not hold up a wave.  With
-// recs given, each element starting in the chunk records its first pair.
+// rectangle of its own and large rectangles do not hold up a wave.
 struct DupChunk {
     size_t base, K;
     uint32_t s_lo;
 };
 
-__device__ __forceinline__ DupChunk dup_chunk_owners(const uint64_t* __restrict__ vkeys,
-                                                     const uint32_t* __restrict__ offs, size_t n,
+__device__ __forceinline__ DupChunk dup_chunk_owners(const uint32_t* __restrict__ offs, size_t n,
                                                      const unsigned long long* __restrict__ n_dev, size_t K,
-                                                     const uint32_t* __restrict__ chunk_first, uint32_t P64,
-                                                     uint4* __restrict__ recs, uint32_t* s_own, uint32_t* s_wave)
+                                                     const uint32_t* __restrict__ chunk_first, uint32_t* s_own,
+                                                     uint32_t* s_wave)
 {
     const int t = threadIdx.x;
     DupChunk c;
@@ -847,12 +885,15 @@ __device__ __forceinline__ DupChunk dup_chunk_owners(const uint64_t* __restrict_
 #pragma unroll
     for (int q = 0; q < kDupPer; ++q) s_own[own_slot(t + 256 * q)] = 0u;
     __syncthreads();
-    for (size_t s = (size_t)c.s_lo + (size_t)t; s < s_end; s += 256) {
-        const uint32_t off = offs[s];
-        if ((size_t)off >= c.base && (size_t)off < c.base + kDupChunk) {
-            if (s > c.s_lo) s_own[own_slot(off - (uint32_t)c.base)] = (uint32_t)(s - c.s_lo);
-            if (recs) recs[batch_elem(vkeys[s], P64)].w = off;   // the element's first pair
-        }
+    // two elements per thread in flight (a chunk overlaps a few hundred)
+    for (size_t s = (size_t)c.s_lo + (size_t)t; s < s_end; s += 512) {
+        const bool has2 = s + 256 < s_end;
+        const uint32_t off0 = offs[s];
+        const uint32_t off1 = has2 ? offs[s + 256] : 0u;
+        if ((size_t)off0 >= c.base && (size_t)off0 < c.base + kDupChunk && s > c.s_lo)
+            s_own[own_slot(off0 - (uint32_t)c.base)] = (uint32_t)(s - c.s_lo);
+        if (has2 && (size_t)off1 >= c.base && (size_t)off1 < c.base + kDupChunk)
+            s_own[own_slot(off1 - (uint32_t)c.base)] = (uint32_t)(s + 256 - c.s_lo);
     }
     __syncthreads();
     // inclusive maximum over the chunk (owners grow with the position)
@@ -888,37 +929,52 @@ __device__ __forceinline__ uint32_t div_rows(uint32_t k, uint32_t w, uint32_t& r
     return q;
 }
 
-// The (tile << kBatchKeyViewBits | view, element) pair at chunk position p;
-// the owner's rectangle is (first tile, width | row stride << 16).
-__device__ __forceinline__ void dup_pair(const DupChunk& c, uint32_t p, const uint32_t* s_own,
-                                         const uint64_t* __restrict__ vkeys, const uint32_t* __restrict__ offs,
-                                         const uint2* __restrict__ rects_s, uint32_t P64, uint32_t& key,
-                                         uint32_t& val)
+// The (tile << kBatchKeyViewBits | view, element) pairs at chunk positions
+// p0 + i * stride, i < N; the owner's rectangle is (first tile, width | row
+// stride << 16).  All owner reads, then all record loads, are issued before
+// any is used.  Positions past K hold the chunk's last owner (the scan
+// carries it), so their loads stay in bounds; their pairs are not used.
+template <int N>
+__device__ __forceinline__ void dup_pairs(const DupChunk& c, uint32_t p0, uint32_t stride, const uint32_t* s_own,
+                                          const uint64_t* __restrict__ vkeys, const uint32_t* __restrict__ offs,
+                                          const uint2* __restrict__ rects_s, uint32_t P64, uint32_t* key,
+                                          uint32_t* val)
 {
-    const size_t s = (size_t)c.s_lo + s_own[own_slot(p)];
-    const uint2 rc = rects_s[s];
-    const uint64_t vk = vkeys[s];
-    const uint32_t k = (uint32_t)(c.base + p - offs[s]);
-    uint32_t dx;
-    const uint32_t dy = div_rows(k, rc.y & 0xFFFFu, dx);
-    const uint32_t tile = rc.x + dy * (rc.y >> 16) + dx;
-    key = (tile << kBatchKeyViewBits) | (uint32_t)(vk >> kBatchViewShift);
-    val = batch_elem(vk, P64);
+    uint32_t own[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) own[i] = s_own[own_slot(p0 + (uint32_t)i * stride)];
+    uint2 rc[N];
+    uint64_t vk[N];
+    uint32_t of[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const size_t s = (size_t)c.s_lo + own[i];
+        rc[i] = rects_s[s];
+        vk[i] = vkeys[s];
+        of[i] = offs[s];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const uint32_t k = (uint32_t)(c.base + p0 + (uint32_t)i * stride - of[i]);
+        uint32_t dx;
+        const uint32_t dy = div_rows(k, max(rc[i].y & 0xFFFFu, 1u), dx);
+        const uint32_t tile = rc[i].x + dy * (rc[i].y >> 16) + dx;
+        key[i] = (tile << kBatchKeyViewBits) | (uint32_t)(vk[i] >> kBatchViewShift);
+        val[i] = batch_elem(vk[i], P64);
+    }
 }
 
 // Duplication fused with the first tile-sort pass.  Counting half: the
 // chunk's pairs are generated in registers and their digits counted into
-// the pass histogram (chunks past K count nothing); the elements record
-// their first pair.
+// the pass histogram (chunks past K count nothing).
 __global__ __launch_bounds__(256) void k_dup_count_batch(const uint64_t* __restrict__ vkeys,
                                                          const uint32_t* __restrict__ offs,
                                                          const uint2* __restrict__ rects_s, size_t n,
                                                          const unsigned long long* __restrict__ n_dev,
                                                          const unsigned long long* __restrict__ K_dev,
                                                          const uint32_t* __restrict__ chunk_first, int P,
-                                                         const BatchView* __restrict__ views, size_t cap,
-                                                         uint4* __restrict__ recs, int shift,
-                                                         uint32_t* __restrict__ hist, size_t nb)
+                                                         size_t cap, int shift, uint32_t* __restrict__ hist,
+                                                         size_t nb)
 {
     __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
     __shared__ uint32_t s_own[kDupOwnWords];
@@ -930,14 +986,16 @@ __global__ __launch_bounds__(256) void k_dup_count_batch(const uint64_t* __restr
     const size_t K = live_count(cap, K_dev);
     if ((size_t)blockIdx.x * kDupChunk < K) {
         const uint32_t P64 = batch_p64(P);
-        const DupChunk c = dup_chunk_owners(vkeys, offs, n, n_dev, K, chunk_first, P64, recs, s_own, s_wave);
-#pragma unroll 4
-        for (int q = 0; q < kDupPer; ++q) {
-            const uint32_t p = (uint32_t)t + 256u * (uint32_t)q;
-            if (c.base + p >= K) break;
-            uint32_t key, val;
-            dup_pair(c, p, s_own, vkeys, offs, rects_s, P64, key, val);
-            atomicAdd(&s_h[wave][(key >> shift) & (kRadix - 1)], 1u);
+        const DupChunk c = dup_chunk_owners(offs, n, n_dev, K, chunk_first, s_own, s_wave);
+        constexpr int kRound = 8;
+#pragma unroll
+        for (int r = 0; r < kDupPer / kRound; ++r) {
+            const uint32_t p0 = (uint32_t)t + 256u * kRound * (uint32_t)r;
+            uint32_t key[kRound], val[kRound];
+            dup_pairs<kRound>(c, p0, 256u, s_own, vkeys, offs, rects_s, P64, key, val);
+#pragma unroll
+            for (int i = 0; i < kRound; ++i)
+                if (c.base + p0 + 256u * (uint32_t)i < K) atomicAdd(&s_h[wave][(key[i] >> shift) & (kRadix - 1)], 1u);
         }
     }
     rs_count_store(s_h, hist, nb);
@@ -952,8 +1010,7 @@ __global__ __launch_bounds__(256) void k_dup_scatter_batch(const uint64_t* __res
                                                            const unsigned long long* __restrict__ n_dev,
                                                            const unsigned long long* __restrict__ K_dev,
                                                            const uint32_t* __restrict__ chunk_first, int P,
-                                                           const BatchView* __restrict__ views, size_t cap,
-                                                           int shift, const uint32_t* __restrict__ hist, size_t nb,
+                                                           size_t cap, int shift, const uint32_t* __restrict__ hist, size_t nb,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout)
 {
     constexpr int kWaves = kThreads / FGS_WAVE;
@@ -972,16 +1029,15 @@ __global__ __launch_bounds__(256) void k_dup_scatter_batch(const uint64_t* __res
     for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
     s_glob[t] = hist[(size_t)t * nb + blockIdx.x];
     const uint32_t P64 = batch_p64(P);
-    const DupChunk c = dup_chunk_owners(vkeys, offs, n, n_dev, K, chunk_first, P64, nullptr, s_kv, s_wave);
+    const DupChunk c = dup_chunk_owners(offs, n, n_dev, K, chunk_first, s_kv, s_wave);
     const uint32_t live = (uint32_t)(K - c.base < (size_t)kChunk ? K - c.base : (size_t)kChunk);
+    // item j of the thread is position 1024 * wave + 64 j + lane
     uint32_t key[kIpt], val[kIpt];
+    constexpr int kRound = 8;
 #pragma unroll
-    for (int j = 0; j < kIpt; ++j) {
-        const uint32_t p = (uint32_t)wave * kPerWave + (uint32_t)j * FGS_WAVE + (uint32_t)lane;
-        key[j] = 0u;
-        val[j] = 0u;
-        if (p < live) dup_pair(c, p, s_kv, vkeys, offs, rects_s, P64, key[j], val[j]);
-    }
+    for (int r = 0; r < kIpt / kRound; ++r)
+        dup_pairs<kRound>(c, (uint32_t)wave * kPerWave + (uint32_t)lane + FGS_WAVE * kRound * (uint32_t)r, FGS_WAVE,
+                          s_kv, vkeys, offs, rects_s, P64, key + kRound * r, val + kRound * r);
     rs_kv32_rank_write(key, val, live, shift, kout, vout, s_cnt, s_start, s_glob, s_wave, s_kv, s_kv + kChunk);
 }
 
@@ -1011,9 +1067,9 @@ hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* v
 
 hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                        const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
-                                       const unsigned long long* K_dev, int P, const BatchView* views, size_t cap,
-                                       uint32_t* chunk_first, uint4* recs, int shift, uint32_t* keys,
-                                       uint32_t* vals, uint32_t* hist, uint32_t* partials, hipStream_t s)
+                                       const unsigned long long* K_dev, int P, size_t cap, uint32_t* chunk_first,
+                                       int shift, uint32_t* keys, uint32_t* vals, uint32_t* hist,
+                                       uint32_t* partials, hipStream_t s)
 {
     if (cap == 0 || n == 0) return hipSuccess;
     size_t blocks = div_up(n, 256);
@@ -1022,11 +1078,11 @@ hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* of
                        chunk_first);
     const size_t nb = div_up(cap, kChunk);
     hipLaunchKernelGGL(k_dup_count_batch, dim3((unsigned)nb), dim3(kThreads), 0, s, vkeys, offs, rects_s, n, n_dev,
-                       K_dev, chunk_first, P, views, cap, recs, shift, hist, nb);
+                       K_dev, chunk_first, P, cap, shift, hist, nb);
     hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_dup_scatter_batch, dim3((unsigned)nb), dim3(kThreads), 0, s, vkeys, offs, rects_s, n,
-                       n_dev, K_dev, chunk_first, P, views, cap, shift, hist, nb, keys, vals);
+                       n_dev, K_dev, chunk_first, P, cap, shift, hist, nb, keys, vals);
     return hipGetLastError();
 }
 
