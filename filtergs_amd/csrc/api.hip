@@ -1544,9 +1544,12 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
             Binning bn = carve_binning(binning, K_cap, nullptr);
             const BatchPairs bp = batch_pairs(bn, K_cap);
             // keys carry the view's own tile number: the largest view sets the
-            // passes; the first is fused with the duplication and the result
-            // ends in A
-            const int passes = (bit_length((uint32_t)sz->max_view_tiles) + 7) / 8;
+            // passes, and the bits are split evenly over them (shorter digits
+            // rank faster and leave longer runs to write); the first pass is
+            // fused with the duplication and the result ends in A
+            const int bits = bit_length((uint32_t)sz->max_view_tiles);
+            const int passes = (bits + 7) / 8;
+            const int db = (bits + passes - 1) / passes;
             const bool even = passes % 2 == 0;
             uint32_t* k1 = even ? bp.keysB : bp.keysA;
             uint32_t* v1 = even ? bp.valsB : bp.valsA;
@@ -1555,16 +1558,16 @@ int fgs_batch_forward(const fgs_raster_settings* s, const fgs_gaussians* g, cons
                 // the pair terms are free until the backward: chunk owners
                 if ((rc = check(fgs::launch_duplicate_sort_batch(
                                     gm.vkeysB, gm.offs, gm.rects_s, gm.tiles_s, n, gm.nvis, cnt + 1, P, (size_t)K_cap,
-                                    reinterpret_cast<uint32_t*>(bn.part), fgs::kBatchKeyViewBits, k1, v1, bn.hist,
-                                    bn.partials, st),
+                                    reinterpret_cast<uint32_t*>(bn.part), fgs::kBatchKeyViewBits, db, k1, v1,
+                                    bn.hist, bn.partials, st),
                                 "batch duplicate")))
                     return rc;
             }
             if (passes > 1) {
                 ProfScope ps(ST_TILE_SORT, st);
                 if ((rc = check(fgs::radix_sort_kv32(k1, v1, even ? bp.keysA : bp.keysB, even ? bp.valsA : bp.valsB,
-                                                     (size_t)K_cap, fgs::kBatchKeyViewBits + 8, passes - 1, bn.hist,
-                                                     bn.partials, st, cnt + 1), "batch tile sort")))
+                                                     (size_t)K_cap, fgs::kBatchKeyViewBits + db, passes - 1, db,
+                                                     bn.hist, bn.partials, st, cnt + 1), "batch tile sort")))
                     return rc;
             }
             sorted_keys = bp.keysA;

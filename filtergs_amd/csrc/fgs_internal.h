@@ -269,19 +269,19 @@ constexpr int kBatchKeyViewBits = 6;
 constexpr uint32_t kBatchMaxViewTiles = 1u << (32 - kBatchKeyViewBits);
 // The duplication of the first min(n, *n_dev) depth-ordered keys into their
 // min(cap, *K_dev) kv pairs, fused with the first tile-sort pass: the pairs
-// land in (keys, vals) ordered by the digit at bit shift, without being
-// written in duplication order.  chunk_first: scratch of cap / 4096 + 1
+// land in (keys, vals) ordered by the db-bit digit at bit shift, without
+// being written in duplication order.  chunk_first: scratch of cap / 4096 + 1
 // words; hist / partials: the sort's histogram and scan scratch.
 hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                        const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
                                        const unsigned long long* K_dev, int P, size_t cap, uint32_t* chunk_first,
-                                       int shift, uint32_t* keys, uint32_t* vals, uint32_t* hist,
+                                       int shift, int db, uint32_t* keys, uint32_t* vals, uint32_t* hist,
                                        uint32_t* partials, hipStream_t s);
-// Stable LSD sort of n (key, value) u32 pairs on n_passes 8-bit digits of the
-// key from bit shift_lo (only the first min(n, *n_dev) with n_dev); the
-// result is in (ka, va) if n_passes is even, else in (kb, vb).
+// Stable LSD sort of n (key, value) u32 pairs on n_passes digits of db <= 8
+// bits of the key from bit shift_lo (only the first min(n, *n_dev) with
+// n_dev); the result is in (ka, va) if n_passes is even, else in (kb, vb).
 hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, size_t n, int shift_lo,
-                           int n_passes, uint32_t* hist, uint32_t* partials, hipStream_t s,
+                           int n_passes, int db, uint32_t* hist, uint32_t* partials, hipStream_t s,
                            const unsigned long long* n_dev);
 hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const unsigned long long* K,
                                const BatchView* views, uint2* ranges, int T, hipStream_t s);

@@ -722,9 +722,10 @@ __global__ __launch_bounds__(256) void k_dup_chunk_owners(const uint32_t* __rest
 // arrays, so the counting pass reads 4 bytes per pair.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kThreads) void k_rs_count32(const uint32_t* __restrict__ in, size_t n, int shift,
-                                                         uint32_t* __restrict__ hist, size_t nb,
+                                                         int db, uint32_t* __restrict__ hist, size_t nb,
                                                          const unsigned long long* n_dev)
 {
+    const uint32_t dm = (1u << db) - 1u;
     __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
     for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_h[w][threadIdx.x] = 0;
     __syncthreads();
@@ -738,13 +739,13 @@ __global__ __launch_bounds__(kThreads) void k_rs_count32(const uint32_t* __restr
         const size_t c4 = c / 4;
         for (size_t q = threadIdx.x; q < c4; q += kThreads) {
             const uint4 k4 = reinterpret_cast<const uint4*>(src)[q];
-            atomicAdd(&s_h[wave][(k4.x >> shift) & (kRadix - 1)], 1u);
-            atomicAdd(&s_h[wave][(k4.y >> shift) & (kRadix - 1)], 1u);
-            atomicAdd(&s_h[wave][(k4.z >> shift) & (kRadix - 1)], 1u);
-            atomicAdd(&s_h[wave][(k4.w >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&s_h[wave][(k4.x >> shift) & dm], 1u);
+            atomicAdd(&s_h[wave][(k4.y >> shift) & dm], 1u);
+            atomicAdd(&s_h[wave][(k4.z >> shift) & dm], 1u);
+            atomicAdd(&s_h[wave][(k4.w >> shift) & dm], 1u);
         }
         for (size_t e = 4 * c4 + threadIdx.x; e < c; e += kThreads)
-            atomicAdd(&s_h[wave][(src[e] >> shift) & (kRadix - 1)], 1u);
+            atomicAdd(&s_h[wave][(src[e] >> shift) & dm], 1u);
     }
     rs_count_store(s_h, hist, nb);
 }
@@ -754,7 +755,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_count32(const uint32_t* __restr
 // (block position 1024 * wave + 64 j + lane; valid below live), s_glob holds
 // the global start of the block's run of each digit and s_cnt zeros.
 __device__ __forceinline__ void rs_kv32_rank_write(const uint32_t* key, const uint32_t* val, uint32_t live,
-                                                   int shift, uint32_t* __restrict__ kout,
+                                                   int shift, int db, uint32_t* __restrict__ kout,
                                                    uint32_t* __restrict__ vout,
                                                    uint32_t (*s_cnt)[kRadix], uint32_t* s_start,
                                                    const uint32_t* s_glob, uint32_t* s_wave, uint32_t* s_keys,
@@ -767,14 +768,16 @@ __device__ __forceinline__ void rs_kv32_rank_write(const uint32_t* key, const ui
     const int lane = lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
     const uint32_t wbase = (uint32_t)wave * kPerWave;
+    const uint32_t dm = (1u << db) - 1u;
     uint32_t rank[kIpt];
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
         const bool valid = wbase + (uint32_t)j * FGS_WAVE + (uint32_t)lane < live;
-        const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+        const uint32_t d = (key[j] >> shift) & dm;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
+            if (b >= db) break;
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -805,7 +808,7 @@ __device__ __forceinline__ void rs_kv32_rank_write(const uint32_t* key, const ui
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
         if (wbase + (uint32_t)j * FGS_WAVE + (uint32_t)lane < live) {
-            const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+            const uint32_t d = (key[j] >> shift) & dm;
             const uint32_t q = s_cnt[wave][d] + rank[j];
             s_keys[q] = key[j];
             s_vals[q] = val[j];
@@ -814,7 +817,7 @@ __device__ __forceinline__ void rs_kv32_rank_write(const uint32_t* key, const ui
     __syncthreads();
     for (uint32_t p = t; p < live; p += kThreads) {
         const uint32_t k = s_keys[p];
-        const uint32_t d = (k >> shift) & (kRadix - 1);
+        const uint32_t d = (k >> shift) & dm;
         const size_t o = s_glob[d] + (p - s_start[d]);
         kout[o] = k;
         vout[o] = s_vals[p];
@@ -825,7 +828,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __
                                                               const uint32_t* __restrict__ vin,
                                                               uint32_t* __restrict__ kout,
                                                               uint32_t* __restrict__ vout, size_t n, int shift,
-                                                              const uint32_t* __restrict__ offs, size_t nb,
+                                                              int db, const uint32_t* __restrict__ offs, size_t nb,
                                                               const unsigned long long* n_dev)
 {
     constexpr int kWaves = kThreads / FGS_WAVE;
@@ -855,7 +858,7 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_kv32(const uint32_t* __
         val[j] = e < n ? vin[e] : 0u;
     }
     const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
-    rs_kv32_rank_write(key, val, live, shift, kout, vout, s_cnt, s_start, s_glob, s_wave, s_keys, s_vals);
+    rs_kv32_rank_write(key, val, live, shift, db, kout, vout, s_cnt, s_start, s_glob, s_wave, s_keys, s_vals);
 }
 
 // Pairs past the capacity are dropped (the caller compares K with it).
@@ -973,8 +976,8 @@ __global__ __launch_bounds__(256) void k_dup_count_batch(const uint64_t* __restr
                                                          const unsigned long long* __restrict__ n_dev,
                                                          const unsigned long long* __restrict__ K_dev,
                                                          const uint32_t* __restrict__ chunk_first, int P,
-                                                         size_t cap, int shift, uint32_t* __restrict__ hist,
-                                                         size_t nb)
+                                                         size_t cap, int shift, int db,
+                                                         uint32_t* __restrict__ hist, size_t nb)
 {
     __shared__ uint32_t s_h[kThreads / FGS_WAVE][kRadix];
     __shared__ uint32_t s_own[kDupOwnWords];
@@ -995,7 +998,8 @@ __global__ __launch_bounds__(256) void k_dup_count_batch(const uint64_t* __restr
             dup_pairs<kRound>(c, p0, 256u, s_own, vkeys, offs, rects_s, P64, key, val);
 #pragma unroll
             for (int i = 0; i < kRound; ++i)
-                if (c.base + p0 + 256u * (uint32_t)i < K) atomicAdd(&s_h[wave][(key[i] >> shift) & (kRadix - 1)], 1u);
+                if (c.base + p0 + 256u * (uint32_t)i < K)
+                    atomicAdd(&s_h[wave][(key[i] >> shift) & ((1u << db) - 1u)], 1u);
         }
     }
     rs_count_store(s_h, hist, nb);
@@ -1010,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_dup_scatter_batch(const uint64_t* __res
                                                            const unsigned long long* __restrict__ n_dev,
                                                            const unsigned long long* __restrict__ K_dev,
                                                            const uint32_t* __restrict__ chunk_first, int P,
-                                                           size_t cap, int shift, const uint32_t* __restrict__ hist, size_t nb,
+                                                           size_t cap, int shift, int db, const uint32_t* __restrict__ hist, size_t nb,
                                                            uint32_t* __restrict__ kout, uint32_t* __restrict__ vout)
 {
     constexpr int kWaves = kThreads / FGS_WAVE;
@@ -1038,23 +1042,23 @@ __global__ __launch_bounds__(256) void k_dup_scatter_batch(const uint64_t* __res
     for (int r = 0; r < kIpt / kRound; ++r)
         dup_pairs<kRound>(c, (uint32_t)wave * kPerWave + (uint32_t)lane + FGS_WAVE * kRound * (uint32_t)r, FGS_WAVE,
                           s_kv, vkeys, offs, rects_s, P64, key + kRound * r, val + kRound * r);
-    rs_kv32_rank_write(key, val, live, shift, kout, vout, s_cnt, s_start, s_glob, s_wave, s_kv, s_kv + kChunk);
+    rs_kv32_rank_write(key, val, live, shift, db, kout, vout, s_cnt, s_start, s_glob, s_wave, s_kv, s_kv + kChunk);
 }
 
 hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, size_t n, int shift_lo,
-                           int n_passes, uint32_t* hist, uint32_t* partials, hipStream_t s,
+                           int n_passes, int db, uint32_t* hist, uint32_t* partials, hipStream_t s,
                            const unsigned long long* n_dev)
 {
     if (n == 0) return hipSuccess;
     const size_t nb = div_up(n, kChunk);
     uint32_t *ks = ka, *vs = va, *kd = kb, *vd = vb;
     for (int p = 0; p < n_passes; ++p) {
-        const int shift = shift_lo + 8 * p;
-        hipLaunchKernelGGL(k_rs_count32, dim3((unsigned)nb), dim3(kThreads), 0, s, ks, n, shift, hist, nb, n_dev);
+        const int shift = shift_lo + db * p;
+        hipLaunchKernelGGL(k_rs_count32, dim3((unsigned)nb), dim3(kThreads), 0, s, ks, n, shift, db, hist, nb, n_dev);
         hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_rs_scatter_kv32, dim3((unsigned)nb), dim3(kThreads), 0, s, ks, vs, kd, vd, n, shift,
-                           hist, nb, n_dev);
+                           db, hist, nb, n_dev);
         uint32_t* t = ks;
         ks = kd;
         kd = t;
@@ -1068,7 +1072,7 @@ hipError_t radix_sort_kv32(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* v
 hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* offs, const uint2* rects_s,
                                        const uint32_t* tiles_s, size_t n, const unsigned long long* n_dev,
                                        const unsigned long long* K_dev, int P, size_t cap, uint32_t* chunk_first,
-                                       int shift, uint32_t* keys, uint32_t* vals, uint32_t* hist,
+                                       int shift, int db, uint32_t* keys, uint32_t* vals, uint32_t* hist,
                                        uint32_t* partials, hipStream_t s)
 {
     if (cap == 0 || n == 0) return hipSuccess;
@@ -1078,11 +1082,11 @@ hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* of
                        chunk_first);
     const size_t nb = div_up(cap, kChunk);
     hipLaunchKernelGGL(k_dup_count_batch, dim3((unsigned)nb), dim3(kThreads), 0, s, vkeys, offs, rects_s, n, n_dev,
-                       K_dev, chunk_first, P, cap, shift, hist, nb);
+                       K_dev, chunk_first, P, cap, shift, db, hist, nb);
     hipError_t e = exclusive_scan(PlainLoader{ hist }, (size_t)kRadix * nb, hist, partials, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_dup_scatter_batch, dim3((unsigned)nb), dim3(kThreads), 0, s, vkeys, offs, rects_s, n,
-                       n_dev, K_dev, chunk_first, P, cap, shift, hist, nb, keys, vals);
+                       n_dev, K_dev, chunk_first, P, cap, shift, db, hist, nb, keys, vals);
     return hipGetLastError();
 }
 
