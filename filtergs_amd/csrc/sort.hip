@@ -57,9 +57,27 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return wave_off + incl - v;
 }
 
+// Loaders of the scans: v[0 .. kIpt) = the values at base .. base + kIpt
+// (zero at or past n); the plain one reads 16 bytes at a time.
 struct PlainLoader {
     const uint32_t* in;
-    __device__ uint32_t operator()(size_t s) const { return in[s]; }
+    __device__ void load(size_t base, size_t n, uint32_t* v) const
+    {
+        if (base + kIpt <= n && ((uintptr_t)in & 15u) == 0) {
+            const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+            for (int q = 0; q < kIpt / 4; ++q) {
+                const uint4 x = p[q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kIpt; ++j) v[j] = base + j < n ? in[base + j] : 0u;
+        }
+    }
 };
 
 // n_dev (may be null): only the first min(n, *n_dev) elements are scanned;
@@ -78,15 +96,19 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, u
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
     n = live_count(n, n_dev);
+    if ((size_t)blockIdx.x * kChunk >= n) {   // past the live values: nothing to add
+        if (threadIdx.x == 0) partials[blockIdx.x] = 0u;
+        return;
+    }
     const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
+    uint32_t v[kIpt];
+    ld.load(base, n, v);
     uint32_t acc = 0, nz = 0;
 #pragma unroll
-    for (int j = 0; j < kIpt; ++j)
-        if (base + j < n) {
-            const uint32_t v = ld(base + j);
-            acc += v;
-            nz += v != 0u;
-        }
+    for (int j = 0; j < kIpt; ++j) {
+        acc += v[j];
+        nz += v[j] != 0u;
+    }
     uint32_t total;
     block_exclusive_scan(acc, s_wave, &total);
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
@@ -97,20 +119,28 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, u
     }
 }
 
-// Single workgroup: exclusive scan of partials[0..nb) in place; the grand
-// total (64-bit) goes to *total when given.
+// Single workgroup: exclusive scan of partials[0..nb) in place, kIpt per
+// thread per step; the grand total (64-bit) goes to *total when given.
 __global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, size_t nb,
                                                             unsigned long long* total_out)
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
     uint32_t carry = 0;
     unsigned long long carry64 = 0;
-    for (size_t base = 0; base < nb; base += kThreads) {
-        const size_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? partials[i] : 0u;
+    for (size_t b0 = 0; b0 < nb; b0 += kChunk) {
+        const size_t base = b0 + (size_t)threadIdx.x * kIpt;
+        uint32_t v[kIpt];
+        PlainLoader{ partials }.load(base, nb, v);
+        uint32_t acc = 0;
+#pragma unroll
+        for (int j = 0; j < kIpt; ++j) acc += v[j];
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, s_wave, &total);
-        if (i < nb) partials[i] = carry + ex;
+        uint32_t run = carry + block_exclusive_scan(acc, s_wave, &total);
+#pragma unroll
+        for (int j = 0; j < kIpt; ++j) {
+            if (base + j < nb) partials[base + j] = run;
+            run += v[j];
+        }
         carry += total;
         carry64 += total;
     }
@@ -123,20 +153,36 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(Loader ld, size_t n, con
 {
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
     n = live_count(n, n_dev);
+    if ((size_t)blockIdx.x * kChunk >= n) return;
     const size_t base = (size_t)blockIdx.x * kChunk + (size_t)threadIdx.x * kIpt;
     uint32_t v[kIpt];
+    ld.load(base, n, v);
     uint32_t acc = 0;
 #pragma unroll
-    for (int j = 0; j < kIpt; ++j) {
-        v[j] = (base + j < n) ? ld(base + j) : 0u;
-        acc += v[j];
-    }
+    for (int j = 0; j < kIpt; ++j) acc += v[j];
     uint32_t total;
     uint32_t run = block_exclusive_scan(acc, s_wave, &total) + partials[blockIdx.x];
+    if (base + kIpt <= n && ((uintptr_t)out & 15u) == 0) {
+        uint4* o = reinterpret_cast<uint4*>(out + base);
 #pragma unroll
-    for (int j = 0; j < kIpt; ++j) {
-        if (base + j < n) out[base + j] = run;
-        run += v[j];
+        for (int q = 0; q < kIpt / 4; ++q) {
+            uint4 x;
+            x.x = run;
+            run += v[4 * q];
+            x.y = run;
+            run += v[4 * q + 1];
+            x.z = run;
+            run += v[4 * q + 2];
+            x.w = run;
+            run += v[4 * q + 3];
+            o[q] = x;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kIpt; ++j) {
+            if (base + j < n) out[base + j] = run;
+            run += v[j];
+        }
     }
 }
 
@@ -627,7 +673,11 @@ hipError_t launch_scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t
 
 struct PopLoader {
     const uint64_t* masks;
-    __device__ uint32_t operator()(size_t s) const { return (uint32_t)__popcll(masks[s]); }
+    __device__ void load(size_t base, size_t n, uint32_t* v) const
+    {
+#pragma unroll
+        for (int j = 0; j < kIpt; ++j) v[j] = base + j < n ? (uint32_t)__popcll(masks[base + j]) : 0u;
+    }
 };
 
 // One wave per 64 groups: lane l loads group l's mask and start, then the
