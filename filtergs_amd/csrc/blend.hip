@@ -1726,6 +1726,160 @@ __global__ __launch_bounds__(256) void k_grad_gather(GradGather gg, const float*
     }
 }
 
+// Segmented inclusive wave sum of N values per lane: head marks the first
+// lane of a segment (lane 0 starts one in any case).  Kogge-Stone steps of
+// 1, 2, 4 and 8 lanes inside each 16-lane row (DPP row shifts), then the
+// rows take the running sum of the row before them (lanes 15, 31 and 47)
+// up to their first head.  The order of the additions depends on the lane
+// positions only.
+template <int N>
+__device__ __forceinline__ void wave_segmented_sum(float* x, bool head)
+{
+    const int lane = lane_id();
+    const int col = lane & 15;
+    uint32_t f = head || lane == 0 ? 1u : 0u;
+#define FGS_SEG_STEP(D)                                                                              \
+    {                                                                                                \
+        const uint32_t fu = dpp_u<0x110 + D>(f);                                                     \
+        float xu[N];                                                                                 \
+        _Pragma("unroll") for (int e = 0; e < N; ++e) xu[e] = dpp<0x110 + D>(x[e]);                  \
+        if (col >= D) {                                                                              \
+            if (!f) {                                                                                \
+                _Pragma("unroll") for (int e = 0; e < N; ++e) x[e] += xu[e];                         \
+            }                                                                                        \
+            f |= fu;                                                                                 \
+        }                                                                                            \
+    }
+    FGS_SEG_STEP(1)
+    FGS_SEG_STEP(2)
+    FGS_SEG_STEP(4)
+    FGS_SEG_STEP(8)
+#undef FGS_SEG_STEP
+    // rows 1 and 3 take lanes 15 and 47, then rows 2 and 3 lane 31
+    const int row = lane >> 4;
+    {
+        const uint32_t f15 = __builtin_amdgcn_readlane(f, 15), f47 = __builtin_amdgcn_readlane(f, 47);
+        float xu[N];
+#pragma unroll
+        for (int e = 0; e < N; ++e) {
+            const float a = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(x[e]), 15));
+            const float b = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(x[e]), 47));
+            xu[e] = row == 1 ? a : b;
+        }
+        if (row & 1) {
+            if (!f) {
+#pragma unroll
+                for (int e = 0; e < N; ++e) x[e] += xu[e];
+            }
+            f |= row == 1 ? f15 : f47;
+        }
+    }
+    {
+        const uint32_t f31 = __builtin_amdgcn_readlane(f, 31);
+        float xu[N];
+#pragma unroll
+        for (int e = 0; e < N; ++e) xu[e] = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(x[e]), 31));
+        if (row >= 2) {
+            if (!f) {
+#pragma unroll
+                for (int e = 0; e < N; ++e) x[e] += xu[e];
+            }
+            f |= f31;
+        }
+    }
+}
+
+// Batch gather: one wave per 64 consecutive visible elements of the list.
+// Their slot runs follow each other (the list is in element order and the
+// runs are numbered in that order), so the wave walks the slots of its
+// elements of up to kGatherSolo tiles together, 64 a step, one per lane: the
+// marks and sums load coalesced, a segmented wave sum adds up each
+// element's slots of the step, and the last lane of each segment adds it
+// into the element's totals in LDS.  Larger elements read the copies the
+// blend accumulated into.
+__global__ __launch_bounds__(FGS_WAVE) void k_grad_gather_elems(GradGather gg, const float* __restrict__ part,
+                                                                const uint8_t* __restrict__ mark, size_t cap,
+                                                                float* __restrict__ dL_dmean2D,
+                                                                float* __restrict__ dL_dconic,
+                                                                float* __restrict__ dL_dopacity,
+                                                                float* __restrict__ dL_dcolors)
+{
+    __shared__ uint32_t s_pre[FGS_WAVE];   // slots of the wave's solo elements before the lane's
+    __shared__ uint32_t s_off[FGS_WAVE];   // the lane's first slot
+    __shared__ uint32_t s_start[FGS_WAVE]; // step: 1 + the lane whose run starts at the position
+    __shared__ float s_acc[kGradTerms][FGS_WAVE];
+    const int lane = lane_id();
+    const size_t live = gather_live(gg);
+    for (size_t j0 = (size_t)blockIdx.x * FGS_WAVE; j0 < live; j0 += (size_t)gridDim.x * FGS_WAVE) {
+        uint32_t n, g, off;
+        gather_item(gg, j0 + lane, live, n, g, off);
+        const uint32_t ns = n <= kGatherSolo ? n : 0u;
+        uint32_t incl = ns;
+#pragma unroll
+        for (int d = 1; d < FGS_WAVE; d <<= 1) {
+            const uint32_t u = __shfl_up(incl, d, FGS_WAVE);
+            if (lane >= d) incl += u;
+        }
+        const uint32_t pre = incl - ns;
+        const uint32_t total = __builtin_amdgcn_readlane(incl, FGS_WAVE - 1);
+        __syncthreads();   // the previous elements' totals are read
+        s_pre[lane] = pre;
+        s_off[lane] = off;
+#pragma unroll
+        for (int e = 0; e < kGradTerms; ++e) s_acc[e][lane] = 0.f;
+        uint32_t carry = 0u;   // the lane owning the previous step's last slot
+        for (uint32_t r0 = 0; r0 < total; r0 += FGS_WAVE) {
+            s_start[lane] = 0u;
+            __syncthreads();
+            if (ns > 0u && pre >= r0 && pre < r0 + FGS_WAVE) s_start[pre - r0] = (uint32_t)lane + 1u;
+            __syncthreads();
+            const uint32_t st = s_start[lane];
+            uint32_t ow = st;
+#pragma unroll
+            for (int d = 1; d < FGS_WAVE; d <<= 1) {
+                const uint32_t u = __shfl_up(ow, d, FGS_WAVE);
+                if (lane >= d) ow = max(ow, u);
+            }
+            const uint32_t owner = ow ? ow - 1u : carry;
+            carry = __builtin_amdgcn_readlane(owner, FGS_WAVE - 1);
+            const uint32_t idx = r0 + (uint32_t)lane;
+            const bool act = idx < total;
+            const size_t p = (size_t)s_off[owner] + (idx - s_pre[owner]);
+            float x[kGradTerms];
+#pragma unroll
+            for (int e = 0; e < kGradTerms; ++e) x[e] = 0.f;
+            if (act && p < cap && mark[p]) {
+#pragma unroll
+                for (int e = 0; e < kGradTerms; ++e) x[e] = part[(size_t)kGradTerms * p + e];
+            }
+            wave_segmented_sum<kGradTerms>(x, st != 0u);
+            const bool next_head = __shfl_down(st, 1, FGS_WAVE) != 0u;
+            const bool tail = act && (lane == FGS_WAVE - 1 || idx + 1 == total || next_head);
+            if (tail) {
+#pragma unroll
+                for (int e = 0; e < kGradTerms; ++e) s_acc[e][owner] += x[e];
+            }
+            __syncthreads();
+        }
+        if (n > 0u && n <= kGatherSolo) {
+            float t[kGradTerms];
+#pragma unroll
+            for (int e = 0; e < kGradTerms; ++e) t[e] = s_acc[e][lane];
+            store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, true);
+        } else if (n > kGatherSolo) {
+            float t[kGradTerms] = {};
+            if ((size_t)off + kGatherSolo < cap) {
+                const float* acc = part + (size_t)kGradTerms * off;
+#pragma unroll
+                for (int c = 0; c < kBigCopies; ++c)
+#pragma unroll
+                    for (int e = 0; e < kGradTerms; ++e) t[e] += acc[c * kBigCopyStride + e * kBigLine];
+            }
+            store_grads(g, t, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors, true);
+        }
+    }
+}
+
 // Clears the accumulation slot (first slot's partials) of every Gaussian of
 // more than kGatherSolo tiles before the blend adds into it (a batch: those
 // of the big list).
@@ -1774,6 +1928,13 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
     launch(a);
     prof_kernel(true, false, s);
     if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
+    if (gg.slot_base) {   // batch: one wave per 64 visible elements
+        size_t blocks = (gg.n + FGS_WAVE - 1) / FGS_WAVE;
+        if (blocks > 4 * kStrideBlocks) blocks = 4 * kStrideBlocks;
+        hipLaunchKernelGGL(k_grad_gather_elems, dim3((unsigned)blocks), dim3(FGS_WAVE), 0, s, gg, a.part, a.mark,
+                           a.part_cap, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
+        return hipGetLastError();
+    }
     size_t blocks = (gg.n + 255) / 256;
     if (gg.n_dev && blocks > kStrideBlocks) blocks = kStrideBlocks;
     hipLaunchKernelGGL(k_grad_gather, dim3((unsigned)blocks), dim3(256), 0, s, gg, a.part,

@@ -179,10 +179,17 @@ struct PreRes {
     float col[3];
 };
 
+// The view-independent log of the opacity the OGFS radius uses (variant 0).
+__device__ __forceinline__ float pre_log_opacity(const PreArgs& a, float opacity_in)
+{
+    return a.variant == 1 ? 0.f : logf_cr(fmaxf(opacity_in, 1e-6f));
+}
+
+// log_opacity: pre_log_opacity(a, opacity_in).
 template <typename Sh>
 __device__ __forceinline__ void preprocess_view(const PreArgs& a, float px, float py, float pz,
-                                                const float* c3, float opacity_in, const Sh& sh,
-                                                int* error_flag, PreRes& r)
+                                                const float* c3, float opacity_in, float log_opacity,
+                                                const Sh& sh, int* error_flag, PreRes& r)
 {
     int radius_px = 0;
     uint32_t tiles = 0;
@@ -228,7 +235,6 @@ __device__ __forceinline__ void preprocess_view(const PreArgs& a, float px, floa
             if (radius_px <= 0) alive = false;
         } else {
             opacity = fmaxf(opacity_in, 1e-6f);
-            const float log_opacity = logf_cr(opacity);
             const float energy_clamped = fmaxf(log_opacity - a.log_lambda, a.energy_floor);
             const float mid = 0.5f * (ca + cc);
             const float disc = fmaxf(0.0f, mid * mid - det);
@@ -327,8 +333,8 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd(PreArgs a, PreOu
     float p[3], c3[6];
     gaussian_inputs(a, i, p, c3);
     PreRes r;
-    preprocess_view(a, p[0], p[1], p[2], c3, a.opacities[i], ShLds{ s_sh, (int)threadIdx.x },
-                    o.error_flag, r);
+    preprocess_view(a, p[0], p[1], p[2], c3, a.opacities[i], pre_log_opacity(a, a.opacities[i]),
+                    ShLds{ s_sh, (int)threadIdx.x }, o.error_flag, r);
     write_pre(o, (size_t)i, r, !a.colors_precomp);
     const uint32_t hi = r.tiles ? __float_as_uint(r.depth) : 0xFFFFFFFFu;
     o.vkeys[i] = ((uint64_t)hi << 32) | (uint32_t)i;
@@ -363,7 +369,8 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_rows(PreArgs a, 
             cov3d(a.mod * ar.s[0], a.mod * ar.s[1], a.mod * ar.s[2], ar.q[0], ar.q[1], ar.q[2], ar.q[3], c3);
             PreArgs ac = a;
             ac.colors_precomp = a.rows_dc;   // colours come from the activation: no SH pass
-            preprocess_view(ac, ar.p[0], ar.p[1], ar.p[2], c3, ar.o, ShLds{ nullptr, 0 }, o.error_flag, r);
+            preprocess_view(ac, ar.p[0], ar.p[1], ar.p[2], c3, ar.o, pre_log_opacity(ac, ar.o), ShLds{ nullptr, 0 },
+                            o.error_flag, r);
             r.col[0] = ar.c[0];
             r.col[1] = ar.c[1];
             r.col[2] = ar.c[2];
@@ -421,6 +428,7 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
     float p[3], c3[6];
     gaussian_inputs(a, i, p, c3);
     const float opacity_in = a.opacities[i];
+    const float log_opacity = pre_log_opacity(a, opacity_in);   // once for all views
     const ShReg sh{ shr };
     const uint32_t P64 = batch_p64(a.P);
     // Only visible (view, Gaussian) keys go to the sort: each wave appends
@@ -436,7 +444,7 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
         PreArgs av = a;
         bind_view(av, views[v]);
         PreRes r;
-        preprocess_view(av, p[0], p[1], p[2], c3, opacity_in, sh, o.error_flag, r);
+        preprocess_view(av, p[0], p[1], p[2], c3, opacity_in, log_opacity, sh, o.error_flag, r);
         o.radii[(size_t)v * a.P + i] = r.radius;
         const bool vis = r.tiles != 0;
         const uint64_t m = __ballot(vis);
