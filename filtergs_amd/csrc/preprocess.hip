@@ -26,9 +26,12 @@ __constant__ float kSH_C3[7] = { -0.5900435899266435f, 2.890611442640554f, -0.45
 
 // SH colour of one channel, before the +0.5 shift.  sh(f) returns float f of
 // the Gaussian's coefficient row (coefficient k, channel c at f = 3k + c).
+// The colour is no integer's input: its products and sums may fuse (as the
+// reference's compiler fuses them), unlike the rest of this file.
 template <typename Sh>
 __device__ __forceinline__ float sh_eval(int deg, const Sh& sh, int c, float x, float y, float z)
 {
+#pragma clang fp contract(fast)
 #define SH(k) sh((k) * 3 + c)
     float res = kSH_C0 * SH(0);
     if (deg > 0) {
