@@ -843,7 +843,7 @@ static int backward_impl(const fgs_raster_settings* s, const fgs_gaussians* g, c
         if ((rc = sync_check(s, st, "blend backward"))) return rc;
     }
     const fgs::PreArgs a = make_pre_args(s, g);
-    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors };
+    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors, nullptr, gm.rgb };
     fgs::PreBwdOut o{};
     o.dL_dopacity = dL_dopacity;
     o.dL_dmeans3D = dL_dmeans3D;
@@ -1680,7 +1680,7 @@ int fgs_batch_backward(const fgs_raster_settings* s, const fgs_gaussians* g, con
             return rc;
     }
     const fgs::PreArgs a = batch_pre_args(s, g);
-    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors, gm.vis_masks };
+    fgs::PreBwdIn in{ radii, dL_dmeans2D, dconic, dL_dcolors, gm.vis_masks, gm.rgb };
     fgs::PreBwdOut o{};
     o.dL_dopacity = dL_dopacity;
     o.dL_dmeans3D = dL_dmeans3D;

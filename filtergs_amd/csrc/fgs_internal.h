@@ -68,6 +68,10 @@ struct PreBwdIn {
     const float* dL_dconic;    // [P,3]; a batch: per element
     const float* dL_dcolors;   // [P,3]; a batch: per element
     const uint64_t* vis_masks; // batch: the element layout
+    // The forward's colours, [P,3] or per element: a channel the SH clamp
+    // zeroed holds -0, so the backward reads the clamp instead of
+    // evaluating the SH again.
+    const float* rgb;
 };
 struct PreBwdOut {
     float* dL_dopacity;        // [P], modified in exact mode

@@ -268,7 +268,8 @@ __device__ __forceinline__ void preprocess_view(const PreArgs& a, float px, floa
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
                 const float v = sh_eval(a.D, sh, ch, d[0], d[1], d[2]) + 0.5f;
-                col[ch] = v < 0.0f ? 0.0f : v;
+                // clamped: -0, which blends like 0 and tells the backward
+                col[ch] = v < 0.0f ? -0.0f : v;
             }
         }
     } else {
@@ -583,22 +584,20 @@ __device__ __forceinline__ void bwd_view_geom(const PreArgs& a, const float* dco
 
 // SH backward (backward.cu:20-139) of one Gaussian in one view: dL/dsh into
 // the LDS column s_out[f * kPitch + t], the view direction's term into dmv.
+// rgb: the forward's colour of the Gaussian in the view (a clamped channel
+// is -0 and passes no gradient).
 __device__ __forceinline__ void bwd_view_sh(const PreArgs& a, const float* campos, const float* dcolors,
-                                            float mx, float my, float mz, const float* shr,
+                                            const float* rgb, float mx, float my, float mz, const float* shr,
                                             float* dmv, float* s_out, int t)
 {
     {
-        const ShReg sh{ shr };
         const int deg = a.D;
         float d[3], dor[3];
         view_dir(campos, mx, my, mz, d, dor);
         const float x = d[0], y = d[1], z = d[2];
         float dRGB[3];
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            const float v = sh_eval(deg, sh, ch, x, y, z) + 0.5f;
-            dRGB[ch] = v < 0.f ? 0.f : dcolors[ch];
-        }
+        for (int ch = 0; ch < 3; ++ch) dRGB[ch] = signbit(rgb[ch]) ? 0.f : dcolors[ch];
         float ddx[3] = { 0.f, 0.f, 0.f }, ddy[3] = { 0.f, 0.f, 0.f }, ddz[3] = { 0.f, 0.f, 0.f };
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
@@ -668,7 +667,7 @@ __device__ __forceinline__ void preprocess_bwd_view(const PreArgs& a, const PreB
 {
     float dmv[3];
     bwd_view_geom(a, in.dL_dconic + 3 * e, in.dL_dmean2D + 3 * e, mx, my, mz, c3, dmv, dcov);
-    if (a.shs) bwd_view_sh(a, a.campos, in.dL_dcolors + 3 * e, mx, my, mz, shr, dmv, s_out, t);
+    if (a.shs) bwd_view_sh(a, a.campos, in.dL_dcolors + 3 * e, in.rgb + 3 * e, mx, my, mz, shr, dmv, s_out, t);
 #pragma unroll
     for (int k = 0; k < 3; ++k) dm[k] += dmv[k];
 }
@@ -841,10 +840,12 @@ void k_preprocess_bwd_batch_sh(PreArgs a, const BatchView* views, int V, PreBwdI
         const uint32_t P64 = batch_p64(a.P);
         uint64_t vis = batch_visible_views(in.vis_masks, P64, V, i);
         int v = vis ? __builtin_ctzll(vis) : 0;
-        float gc[3] = { 0.f, 0.f, 0.f };
+        float gc[3] = { 0.f, 0.f, 0.f }, rc[3] = { 0.f, 0.f, 0.f };
         if (vis) {
-            const float* src = in.dL_dcolors + 3 * (size_t)batch_slot(in.vis_masks, P64, v, i);
+            const size_t e = batch_slot(in.vis_masks, P64, v, i);
+            const float* src = in.dL_dcolors + 3 * e;
             gc[0] = src[0]; gc[1] = src[1]; gc[2] = src[2];
+            rc[0] = in.rgb[3 * e]; rc[1] = in.rgb[3 * e + 1]; rc[2] = in.rgb[3 * e + 2];
         }
         while (vis) {
             // Opaque to the optimiser: products of the SH row with constants
@@ -854,11 +855,14 @@ void k_preprocess_bwd_batch_sh(PreArgs a, const BatchView* views, int V, PreBwdI
             for (int f = 0; f < kMaxShFloats; ++f) asm volatile("" : "+v"(shr[f]));
             vis &= vis - 1ull;
             const int vn = vis ? __builtin_ctzll(vis) : v;
-            const float* nsrc = in.dL_dcolors + 3 * (size_t)batch_slot(in.vis_masks, P64, vn, i);
+            const size_t en = batch_slot(in.vis_masks, P64, vn, i);
+            const float* nsrc = in.dL_dcolors + 3 * en;
             const float gn[3] = { nsrc[0], nsrc[1], nsrc[2] };
-            bwd_view_sh(a, s_cam + 3 * v, gc, mx, my, mz, shr, dm, s_sh, t);
+            const float rn[3] = { in.rgb[3 * en], in.rgb[3 * en + 1], in.rgb[3 * en + 2] };
+            bwd_view_sh(a, s_cam + 3 * v, gc, rc, mx, my, mz, shr, dm, s_sh, t);
             v = vn;
             gc[0] = gn[0]; gc[1] = gn[1]; gc[2] = gn[2];
+            rc[0] = rn[0]; rc[1] = rn[1]; rc[2] = rn[2];
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) o.dL_dmeans3D[3 * i + k] = dm[k];
