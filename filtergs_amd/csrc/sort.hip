@@ -120,10 +120,14 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(Loader ld, size_t n, u
 }
 
 // Single workgroup: exclusive scan of partials[0..nb) in place, kIpt per
-// thread per step; the grand total (64-bit) goes to *total when given.
+// thread per step; the grand total (64-bit) goes to *total when given, and
+// *copy_src to *copy_dst when both are.
 __global__ __launch_bounds__(kThreads) void k_scan_partials(uint32_t* partials, size_t nb,
-                                                            unsigned long long* total_out)
+                                                            unsigned long long* total_out,
+                                                            const unsigned long long* copy_src,
+                                                            unsigned long long* copy_dst)
 {
+    if (copy_src && copy_dst && threadIdx.x == 0) *copy_dst = *copy_src;
     __shared__ uint32_t s_wave[kThreads / FGS_WAVE];
     uint32_t carry = 0;
     unsigned long long carry64 = 0;
@@ -188,17 +192,20 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(Loader ld, size_t n, con
 
 size_t scan_partials_words(size_t n) { return div_up(n > 0 ? n : 1, kChunk); }
 
+// nz_live: every live value is nonzero, so *nz_count is the live count
+// (copied, not counted).
 template <typename Loader>
 static hipError_t exclusive_scan(Loader ld, size_t n, uint32_t* out, uint32_t* partials,
                                  hipStream_t s, unsigned long long* total = nullptr,
                                  unsigned long long* nz_count = nullptr,
-                                 const unsigned long long* n_dev = nullptr)
+                                 const unsigned long long* n_dev = nullptr, bool nz_live = false)
 {
     if (n == 0) return hipSuccess;
     const size_t nb = div_up(n, kChunk);
     hipLaunchKernelGGL(k_scan_reduce<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials,
-                       nz_count, n_dev);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kThreads), 0, s, partials, nb, total);
+                       nz_live ? nullptr : nz_count, n_dev);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kThreads), 0, s, partials, nb, total,
+                       nz_live ? n_dev : nullptr, nz_live ? nz_count : nullptr);
     hipLaunchKernelGGL(k_scan_down<Loader>, dim3((unsigned)nb), dim3(kThreads), 0, s, ld, n, partials, out,
                        n_dev);
     return hipGetLastError();
@@ -714,7 +721,9 @@ hipError_t launch_scan_tiles_batch(const uint32_t* tiles_s, uint32_t* offs, size
                                    const unsigned long long* n_dev, uint32_t* partials,
                                    unsigned long long* counters, hipStream_t s)
 {
-    return exclusive_scan(PlainLoader{ tiles_s }, n, offs, partials, s, counters + 1, counters, n_dev);
+    // the slots hold the visible keys only: every one has tiles
+    return exclusive_scan(PlainLoader{ tiles_s }, n, offs, partials, s, counters + 1, counters, n_dev,
+                          n_dev != nullptr);
 }
 
 // Block-wide exclusive maximum of one u32 per thread (256 threads = 4 waves).
