@@ -186,9 +186,10 @@ Geom carve_geom(void* buf, int P, size_t* total, bool batch = false)
     g.vkeysA = c.take<uint64_t>(n);
     g.vkeysB = c.take<uint64_t>(n);
     g.offs = c.take<uint32_t>(n);
-    g.hist = c.take<uint32_t>(fgs::radix_hist_words(n));
+    // a batch's segmented first depth pass takes up to twice the columns
+    g.hist = c.take<uint32_t>(fgs::radix_hist_words(batch ? 2 * n : n));
     size_t pw = fgs::scan_partials_words(n);
-    const size_t ph = scan_words_for_hist(n);
+    const size_t ph = scan_words_for_hist(batch ? 2 * n : n);
     g.partials = c.take<uint32_t>(pw > ph ? pw : ph);
     g.seg_cnt = c.take<uint32_t>(n / FGS_WAVE + 1);
     g.nvis = c.take<unsigned long long>(1);

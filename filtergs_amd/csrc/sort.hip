@@ -276,96 +276,29 @@ __global__ __launch_bounds__(kThreads) void k_rs_count(const uint64_t* __restric
     rs_count_store(s_h, hist, nb);
 }
 
-// One step of the stable scatter: the block's 256 keys (valid where given)
-// are ranked among equal digits with wave64 ballots and an LDS prefix over
-// the four waves, then written at the digit's running offset in s_off.
-__device__ __forceinline__ void rs_scatter_step(bool valid, uint64_t key, int shift,
-                                                uint32_t (*s_cnt)[kRadix], uint32_t* s_off,
-                                                uint64_t* __restrict__ out)
-{
-    const int wave = threadIdx.x / FGS_WAVE;
-    const int lane = lane_id();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
-    const uint32_t d = (uint32_t)(key >> shift) & (kRadix - 1);
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        const bool bit = (d >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
-    }
-    const uint32_t rank = __popcll(peers & lt_mask);
-    if (valid && rank == 0) s_cnt[wave][d] = __popcll(peers);
-    __syncthreads();
-    {
-        const int t = threadIdx.x;
-        uint32_t run = s_off[t];
-#pragma unroll
-        for (int w = 0; w < kThreads / FGS_WAVE; ++w) {
-            const uint32_t c = s_cnt[w][t];
-            s_cnt[w][t] = run;
-            run += c;
-        }
-        s_off[t] = run;
-    }
-    __syncthreads();
-    if (valid) out[s_cnt[wave][d] + rank] = key;
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
-    __syncthreads();
-}
-
-__device__ __forceinline__ void rs_scatter_init(uint32_t (*s_cnt)[kRadix], uint32_t* s_off,
-                                                const uint32_t* __restrict__ offs, size_t nb)
-{
-#pragma unroll
-    for (int w = 0; w < kThreads / FGS_WAVE; ++w) s_cnt[w][threadIdx.x] = 0;
-    s_off[threadIdx.x] = offs[(size_t)threadIdx.x * nb + blockIdx.x];
-    __syncthreads();
-}
-
-// Dense pass.  The block's 4096 keys stay in registers while they are
-// ranked: wave w owns keys [w * 1024, (w + 1) * 1024) of the chunk, lane l
-// item j is key w * 1024 + j * 64 + l (coalesced loads), and items are ranked
-// in that order against per-wave digit counters in LDS, which only the owning
-// wave touches, so no barrier is needed until all 16 items are done.  The
-// keys are then placed in digit order in LDS and written out run by run, so
-// consecutive lanes store consecutive addresses.
-__global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restrict__ in,
-                                                         uint64_t* __restrict__ out, size_t n,
-                                                         int shift, const uint32_t* __restrict__ offs,
-                                                         size_t nb, const unsigned long long* n_dev)
+// The ranked write of one block of up to kChunk u64 keys: key[j] is item j
+// of the thread, block position per_wave * wave + 64 j + lane (per_wave a
+// multiple of 64, at most 1024; valid below live and inside the wave's
+// span), s_glob holds the global start of the block's run of each digit and
+// s_cnt zeros.  A wave's span is a run of positions, so ranks follow the
+// positions (the sort is stable).  s_keys may hold the items on entry: they
+// are read before it is written.
+__device__ __forceinline__ void rs_u64_rank_write(const uint64_t* key, uint32_t live, uint32_t per_wave, int shift,
+                                                  uint64_t* __restrict__ out, uint32_t (*s_cnt)[kRadix],
+                                                  uint32_t* s_start, const uint32_t* s_glob, uint32_t* s_wave,
+                                                  uint64_t* s_keys)
 {
     constexpr int kWaves = kThreads / FGS_WAVE;
-    constexpr int kPerWave = kChunk / kWaves;
-    __shared__ uint32_t s_cnt[kWaves][kRadix];   // per-wave digit counts, then local positions
-    __shared__ uint32_t s_start[kRadix];         // local start of each digit's run
-    __shared__ uint32_t s_glob[kRadix];          // global start of the block's run of each digit
-    __shared__ uint32_t s_wave[kWaves];
-    __shared__ uint64_t s_keys[kChunk];
-    n = live_count(n, n_dev);
-    const size_t base = (size_t)blockIdx.x * kChunk;
-    if (base >= n) return;
     const int t = threadIdx.x;
     const int wave = t / FGS_WAVE;
     const int lane = lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (FGS_WAVE - lane));
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
-    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
-    __syncthreads();
-    const size_t wbase = base + (size_t)wave * kPerWave;
-    uint64_t key[kIpt];
+    const uint32_t wbase = (uint32_t)wave * per_wave;
     uint32_t rank[kIpt];
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-        const size_t e = wbase + (size_t)j * FGS_WAVE + lane;
-        key[j] = e < n ? in[e] : 0ull;
-    }
-#pragma unroll
-    for (int j = 0; j < kIpt; ++j) {
-        const bool valid = wbase + (size_t)j * FGS_WAVE + lane < n;
+        if ((uint32_t)j * FGS_WAVE >= per_wave) break;   // wave-uniform
+        const bool valid = wbase + (uint32_t)j * FGS_WAVE + (uint32_t)lane < live;
         const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -399,18 +332,51 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restr
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kIpt; ++j) {
-        if (wbase + (size_t)j * FGS_WAVE + lane < n) {
+        if ((uint32_t)j * FGS_WAVE >= per_wave) break;
+        if (wbase + (uint32_t)j * FGS_WAVE + (uint32_t)lane < live) {
             const uint32_t d = (uint32_t)(key[j] >> shift) & (kRadix - 1);
             s_keys[s_cnt[wave][d] + rank[j]] = key[j];
         }
     }
     __syncthreads();
-    const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
     for (uint32_t p = t; p < live; p += kThreads) {
         const uint64_t k = s_keys[p];
         const uint32_t d = (uint32_t)(k >> shift) & (kRadix - 1);
         out[s_glob[d] + (p - s_start[d])] = k;
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rs_scatter(const uint64_t* __restrict__ in,
+                                                         uint64_t* __restrict__ out, size_t n,
+                                                         int shift, const uint32_t* __restrict__ offs,
+                                                         size_t nb, const unsigned long long* n_dev)
+{
+    constexpr int kWaves = kThreads / FGS_WAVE;
+    constexpr int kPerWave = kChunk / kWaves;
+    __shared__ uint32_t s_cnt[kWaves][kRadix];   // per-wave digit counts, then local positions
+    __shared__ uint32_t s_start[kRadix];         // local start of each digit's run
+    __shared__ uint32_t s_glob[kRadix];          // global start of the block's run of each digit
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint64_t s_keys[kChunk];
+    n = live_count(n, n_dev);
+    const size_t base = (size_t)blockIdx.x * kChunk;
+    if (base >= n) return;
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+    const int lane = lane_id();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
+    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
+    __syncthreads();
+    const size_t wbase = base + (size_t)wave * kPerWave;
+    uint64_t key[kIpt];
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const size_t e = wbase + (size_t)j * FGS_WAVE + lane;
+        key[j] = e < n ? in[e] : 0ull;
+    }
+    const uint32_t live = (uint32_t)(n - base < (size_t)kChunk ? n - base : (size_t)kChunk);
+    rs_u64_rank_write(key, live, kPerWave, shift, out, s_cnt, s_start, s_glob, s_wave, s_keys);
 }
 
 // Segmented input of the batched preprocess: segment w holds seg_cnt[w]
@@ -433,6 +399,8 @@ __global__ __launch_bounds__(kThreads) void k_rs_count_seg(const uint64_t* __res
     rs_count_store(s_h, hist, nb);
 }
 
+// The segments' keys are first copied into one dense run in LDS, then
+// ranked and written like a dense block (spb * seg_len <= kChunk).
 __global__ __launch_bounds__(kThreads) void k_rs_scatter_seg(const uint64_t* __restrict__ in,
                                                              uint64_t* __restrict__ out,
                                                              const uint32_t* __restrict__ seg_cnt,
@@ -440,20 +408,42 @@ __global__ __launch_bounds__(kThreads) void k_rs_scatter_seg(const uint64_t* __r
                                                              int shift, const uint32_t* __restrict__ offs,
                                                              size_t nb)
 {
-    __shared__ uint32_t s_cnt[kThreads / FGS_WAVE][kRadix];
-    __shared__ uint32_t s_off[kRadix];
-    rs_scatter_init(s_cnt, s_off, offs, nb);
+    constexpr int kWaves = kThreads / FGS_WAVE;
+    __shared__ uint32_t s_cnt[kWaves][kRadix];
+    __shared__ uint32_t s_start[kRadix];
+    __shared__ uint32_t s_glob[kRadix];
+    __shared__ uint32_t s_wave[kWaves];
+    __shared__ uint32_t s_pre[kThreads + 1];   // the segments' keys before each segment
+    __shared__ uint64_t s_keys[kChunk];
+    const int t = threadIdx.x;
+    const int wave = t / FGS_WAVE;
+    const int lane = lane_id();
+    const size_t w0 = (size_t)blockIdx.x * spb;
+    uint32_t c = 0u;
+    if (t < spb && w0 + t < n_seg) c = seg_cnt[w0 + t] < seg_len ? seg_cnt[w0 + t] : (uint32_t)seg_len;
+    uint32_t live;
+    const uint32_t pre = block_exclusive_scan(c, s_wave, &live);
+    s_pre[t] = pre;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) s_cnt[w][t] = 0;
+    s_glob[t] = offs[(size_t)t * nb + blockIdx.x];
+    __syncthreads();
     for (int q = 0; q < spb; ++q) {
-        const size_t w = (size_t)blockIdx.x * spb + q;
-        if (w >= n_seg) break;
-        const size_t c = seg_cnt[w] < seg_len ? seg_cnt[w] : seg_len;
-        const uint64_t* src = in + w * seg_len;
-        for (size_t j0 = 0; j0 < c; j0 += kThreads) {
-            const size_t e = j0 + threadIdx.x;
-            const bool valid = e < c;
-            rs_scatter_step(valid, valid ? src[e] : 0ull, shift, s_cnt, s_off, out);
-        }
+        const uint32_t cq = (q + 1 < spb ? s_pre[q + 1] : live) - s_pre[q];
+        const uint64_t* src = in + (w0 + q) * seg_len;
+        for (uint32_t k = t; k < cq; k += kThreads) s_keys[s_pre[q] + k] = src[k];
     }
+    __syncthreads();
+    // the waves split the live keys into four runs of whole 64-key steps
+    const uint32_t per_wave = (live + kWaves * FGS_WAVE - 1) / (kWaves * FGS_WAVE) * FGS_WAVE;
+    const uint32_t wbase = (uint32_t)wave * per_wave;
+    uint64_t key[kIpt];
+#pragma unroll
+    for (int j = 0; j < kIpt; ++j) {
+        const uint32_t off = (uint32_t)j * FGS_WAVE + (uint32_t)lane;
+        key[j] = off < per_wave && wbase + off < live ? s_keys[wbase + off] : 0ull;
+    }
+    rs_u64_rank_write(key, live, per_wave, shift, out, s_cnt, s_start, s_glob, s_wave, s_keys);
 }
 
 size_t radix_hist_words(size_t n) { return (size_t)kRadix * div_up(n > 0 ? n : 1, kChunk); }
@@ -484,9 +474,10 @@ hipError_t radix_sort_u64(uint64_t* a, uint64_t* b, size_t n, int shift_lo, int 
     return hipGetLastError();
 }
 
-// Segment length 64 * V (one preprocess wave's keys); spb segments per block
-// make a block cover at least kChunk slots, so the first pass needs no more
-// histogram columns than a dense pass over V * P keys.
+// Segment length 64 * V (one preprocess wave's keys); spb = kChunk / seg_len
+// segments per block (at least one: V <= 64), so a block's keys fit one
+// dense block; the first pass takes at most twice the histogram columns of a
+// dense pass over V * P keys.
 hipError_t radix_sort_u64_segments(uint64_t* a, uint64_t* b, const uint32_t* seg_cnt, int P, int V,
                                    int shift_lo, int n_passes, uint32_t* hist, uint32_t* partials,
                                    unsigned long long* n_dev, hipStream_t s)
@@ -495,7 +486,7 @@ hipError_t radix_sort_u64_segments(uint64_t* a, uint64_t* b, const uint32_t* seg
     if (n == 0 || n_passes < 1) return hipSuccess;
     const size_t seg_len = (size_t)FGS_WAVE * (size_t)V;
     const size_t n_seg = div_up((size_t)P, FGS_WAVE);
-    const int spb = (int)div_up((size_t)kChunk, seg_len);
+    const int spb = (int)((size_t)kChunk / seg_len);
     const size_t nb_seg = div_up(n_seg, (size_t)spb);
     hipLaunchKernelGGL(k_rs_count_seg, dim3((unsigned)nb_seg), dim3(kThreads), 0, s, a, seg_cnt, n_seg,
                        seg_len, spb, shift_lo, hist, nb_seg);
