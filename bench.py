@@ -70,7 +70,10 @@ def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
     all views.  The batched preprocess reads the Gaussians once for all views;
     the plain depth sort makes 4 passes over all P keys (culled ones sort
     last), the batched one 5 passes over the Pv visible keys only (the
-    preprocess appends just those)."""
+    preprocess appends just those).  The batched duplication is fused with
+    the tile sort's first pass: its counting kernel and its scatter each read
+    the depth-ordered records (20 B per visible pair) and the scatter writes
+    the pairs once (8 B), so its tile sort stage holds the other passes."""
     params = P * (44 + 12 * M)
     grads_out = P * (92 + 12 * M)
     sorted_keys = P if V == 1 else Pv
@@ -78,8 +81,8 @@ def stage_bytes(P, Pv, K, T, npix, M, passes_tile, V=1):
         "preprocess": params + V * P * 8 + Pv * 40,
         "depth_sort": (4 if V == 1 else 5) * 24 * sorted_keys,
         "tile_offsets": 8 * sorted_keys,
-        "duplicate": 20 * Pv + 8 * K,
-        "tile_sort": passes_tile * 24 * K,
+        "duplicate": (20 if V == 1 else 40) * Pv + 8 * K,
+        "tile_sort": (passes_tile if V == 1 else passes_tile - 1) * 24 * K,
         "ranges": 8 * K + 8 * T,
         "blend_fwd": 40 * K + 28 * npix + 4 * V * P,
         "blend_bwd": 40 * K + 20 * npix + 36 * Pv,
@@ -745,7 +748,10 @@ def main():
     if int(br.counts[1].item()) > br.K_cap:
         raise RuntimeError("kv pairs exceeded the reserved capacity during replay")
     T = int(br.sizes.tiles)
-    nbb = stage_bytes(P, Pv, K, T, npix, M, (T.bit_length() + 7) // 8, V=args.views)
+    # the batch's tile keys hold the view's own tile number: the largest view
+    # sets the tile sort's passes
+    nbb = stage_bytes(P, Pv, K, T, npix, M, (int(br.sizes.max_view_tiles).bit_length() + 7) // 8,
+                      V=args.views)
     stb = stages_of(profb, nbb)
     dominant = max((k for k in stb if not k.endswith("_kernel")), key=lambda k: stb[k]["ms"])
     # the kernel's own launch time where the stage holds more than the kernel
