@@ -198,6 +198,50 @@ __device__ __forceinline__ float wave_maxf(float xf)
     return __uint_as_float(x);
 }
 
+// Wave reductions into lane 63 only (the other lanes hold partial results):
+// a prefix inside each 16-lane row by DPP row shifts (lanes shifted in from
+// outside the row read 0, the identity of both reductions here), then the
+// row broadcasts of lane 15 into rows 1 and 3 and of lane 31 into rows 2
+// and 3.  Six DPP operations, against ten for the every-lane forms above
+// (whose two lane swaps each need a copy).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_rows_u(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_rows(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+// Maximum of a non-negative value over the wave, in lane 63.
+__device__ __forceinline__ float wave_maxf_last(float xf)
+{
+    uint32_t x = __float_as_uint(xf);
+    x = max(x, dpp_u<kDppRowShr1>(x));
+    x = max(x, dpp_u<kDppRowShr2>(x));
+    x = max(x, dpp_u<kDppRowShr4>(x));
+    x = max(x, dpp_u<kDppRowShr8>(x));
+    x = max(x, dpp_rows_u<kDppRowBcast15, 0xA>(x));
+    x = max(x, dpp_rows_u<kDppRowBcast31, 0xC>(x));
+    return __uint_as_float(x);
+}
+
+// Sum over the wave, in lane 63.
+__device__ __forceinline__ float wave_total_last(float x)
+{
+    x += dpp<kDppRowShr1>(x);
+    x += dpp<kDppRowShr2>(x);
+    x += dpp<kDppRowShr4>(x);
+    x += dpp<kDppRowShr8>(x);
+    x += dpp_rows<kDppRowBcast15, 0xA>(x);
+    x += dpp_rows<kDppRowBcast31, 0xC>(x);
+    return x;
+}
+
 // Exponent of the Gaussian in log2 units at offset (dx, dy) from its mean,
 // from the staged terms (a', b') = g0.zw and c' = g1.x:
 // dy (c' dy + b' dx) + a' dx^2, two fused steps per pixel row once the
@@ -639,7 +683,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
     asm volatile("" : "+v"(pfx));
     const float tx0 = (float)(tx * FGS_TILE), ty0 = (float)(ty * FGS_TILE);
     const uint2 range = a.ranges[tile];
-    const int n = (int)(range.y - range.x);
+    const int n = __builtin_amdgcn_readfirstlane((int)(range.y - range.x));
 
     __shared__ FwdSlot s_slot[kFwdWaveBatch];
     __shared__ uint32_t s_vidx[kFwdWaveBatch];
@@ -678,7 +722,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
 
     for (int r = 0; r * kFwdWaveBatch < n && live != 0u; ++r) {
         const int k0 = r * kFwdWaveBatch;
-        const int cnt = min(kFwdWaveBatch, n - k0);
+        const int cnt = __builtin_amdgcn_readfirstlane(min(kFwdWaveBatch, n - k0));
         uint32_t bits = 0u;
         __syncthreads();
         if (lane < cnt) {
@@ -741,7 +785,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
                 }
             }
             if (used != 0u) {   // wave-uniform
-                const float pw = wave_maxf(wl);
+                const float pw = wave_maxf_last(wl);   // in lane 63
                 if (kv_sum) {
                     const float ws = wave_total(wsum);
                     if (lane == 0) {
@@ -749,7 +793,7 @@ __device__ __forceinline__ void blend_fwd_tile_wave(const BlendArgs& a, int tile
                         sp->kc = nbl;
                     }
                 }
-                if (lane == 0) {
+                if (lane == FGS_WAVE - 1) {
                     sp->w[0] = __float_as_uint(pw);
                     sp->w[1] = used;
                 }
@@ -1498,16 +1542,16 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
         lmax = last[k] > lmax ? last[k] : lmax;
     }
     const uint32_t wmax = __builtin_amdgcn_readfirstlane(wave_max_u32(lmax));
-    const int n = (int)min(wmax, a.tile_vis[tile]);
+    const int n = __builtin_amdgcn_readfirstlane((int)min(wmax, a.tile_vis[tile]));
     const uint2 range = a.ranges[tile];
     const bool h3 = (lane >> 3) & 1;
-    const bool pub_op = lane == 1;
+    const bool pub_op = lane == FGS_WAVE - 1;   // the opacity total lands in lane 63
     const bool pub = (lane & 7) == 0 || pub_op;
     const uint32_t term = pub_op ? 8u : scatter_term(h3);
 
     for (int r = 0; r * kWaveBatch < n; ++r) {
         const int k0 = r * kWaveBatch;
-        const int cnt = min(kWaveBatch, n - k0);
+        const int cnt = __builtin_amdgcn_readfirstlane(min(kWaveBatch, n - k0));
         uint32_t bits = 0u;
         __syncthreads();
         if (lane < cnt) {
@@ -1585,7 +1629,7 @@ __device__ __forceinline__ void blend_bwd_tile_wave(const BlendArgs& a, int tile
             v[5] = v[3] * dx;
             v[6] = v[4] * dx;
             const float z = wave_reduce_scatter8(v, h3);
-            const float zop = wave_total(vsum);
+            const float zop = wave_total_last(vsum);
             if (pub) s_g[j][term] = pub_op ? zop : z;
         }
         __syncthreads();
