@@ -1858,12 +1858,7 @@ __global__ __launch_bounds__(FGS_WAVE) void k_grad_gather_elems(GradGather gg, c
         uint32_t n, g, off;
         gather_item(gg, j0 + lane, live, n, g, off);
         const uint32_t ns = n <= kGatherSolo ? n : 0u;
-        uint32_t incl = ns;
-#pragma unroll
-        for (int d = 1; d < FGS_WAVE; d <<= 1) {
-            const uint32_t u = __shfl_up(incl, d, FGS_WAVE);
-            if (lane >= d) incl += u;
-        }
+        const uint32_t incl = wave_incl_sum_u32(ns);
         const uint32_t pre = incl - ns;
         const uint32_t total = __builtin_amdgcn_readlane(incl, FGS_WAVE - 1);
         __syncthreads();   // the previous elements' totals are read
@@ -1878,12 +1873,7 @@ __global__ __launch_bounds__(FGS_WAVE) void k_grad_gather_elems(GradGather gg, c
             if (ns > 0u && pre >= r0 && pre < r0 + FGS_WAVE) s_start[pre - r0] = (uint32_t)lane + 1u;
             __syncthreads();
             const uint32_t st = s_start[lane];
-            uint32_t ow = st;
-#pragma unroll
-            for (int d = 1; d < FGS_WAVE; d <<= 1) {
-                const uint32_t u = __shfl_up(ow, d, FGS_WAVE);
-                if (lane >= d) ow = max(ow, u);
-            }
+            const uint32_t ow = wave_incl_max_u32(st);
             const uint32_t owner = ow ? ow - 1u : carry;
             carry = __builtin_amdgcn_readlane(owner, FGS_WAVE - 1);
             const uint32_t idx = r0 + (uint32_t)lane;

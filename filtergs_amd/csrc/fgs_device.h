@@ -179,6 +179,38 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
     return v;
 }
 
+// Inclusive scans of a u32 across the wave with DPP (no LDS round trips): a
+// prefix inside each 16-lane row by row shifts 1, 2, 4, 8 (lanes shifted in
+// from outside the row read 0, the identity of both scans), then lane 15's
+// value into rows 1 and 3 and lane 31's into rows 2 and 3 (the row
+// broadcasts of the wave64 DPP).  A lane reads only lanes below it, so lanes
+// that have exited must lie above every live lane.
+template <int CTRL, int ROWS, bool BOUND>
+__device__ __forceinline__ uint32_t dpp_from_below(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, BOUND);
+}
+__device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t x)
+{
+    x += dpp_from_below<0x111, 0xF, true>(x);   // row_shr:1
+    x += dpp_from_below<0x112, 0xF, true>(x);   // row_shr:2
+    x += dpp_from_below<0x114, 0xF, true>(x);   // row_shr:4
+    x += dpp_from_below<0x118, 0xF, true>(x);   // row_shr:8
+    x += dpp_from_below<0x142, 0xA, false>(x);  // row_bcast:15
+    x += dpp_from_below<0x143, 0xC, false>(x);  // row_bcast:31
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_incl_max_u32(uint32_t x)
+{
+    x = max(x, dpp_from_below<0x111, 0xF, true>(x));
+    x = max(x, dpp_from_below<0x112, 0xF, true>(x));
+    x = max(x, dpp_from_below<0x114, 0xF, true>(x));
+    x = max(x, dpp_from_below<0x118, 0xF, true>(x));
+    x = max(x, dpp_from_below<0x142, 0xA, false>(x));
+    x = max(x, dpp_from_below<0x143, 0xC, false>(x));
+    return x;
+}
+
 __device__ __forceinline__ void atomic_add_f32(float* p, float v)
 {
     unsafeAtomicAdd(p, v);

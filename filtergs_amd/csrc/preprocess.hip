@@ -456,13 +456,7 @@ __global__ __launch_bounds__(kPreThreads) void k_preprocess_fwd_batch(PreArgs a,
         // the group's tiles before this lane (the lanes past P have exited:
         // they are all above the live ones, which read only lanes below)
         uint32_t tincl = vis ? r.tiles : 0u;
-        if (m != 0u) {
-#pragma unroll
-            for (int d = 1; d < FGS_WAVE; d <<= 1) {
-                const uint32_t u = __shfl_up(tincl, d, FGS_WAVE);
-                if (lane >= d) tincl += u;
-            }
-        }
+        if (m != 0u) tincl = wave_incl_sum_u32(tincl);
         if (vis) {
             // no depth: the depth travels in the key; the record holds the
             // tile count and the element's slot offset in its group's run

@@ -37,12 +37,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 {
     const int lane = lane_id();
     const int wave = threadIdx.x / FGS_WAVE;
-    uint32_t incl = v;
-#pragma unroll
-    for (int off = 1; off < FGS_WAVE; off <<= 1) {
-        const uint32_t n = __shfl_up(incl, off, FGS_WAVE);
-        if (lane >= off) incl += n;
-    }
+    const uint32_t incl = wave_incl_sum_u32(v);
     if (lane == FGS_WAVE - 1) s_wave[wave] = incl;
     __syncthreads();
     uint32_t wave_off = 0, sum = 0;
@@ -722,12 +717,7 @@ __device__ __forceinline__ uint32_t block_exclusive_max(uint32_t v, uint32_t* s_
 {
     const int lane = lane_id();
     const int wave = threadIdx.x / FGS_WAVE;
-    uint32_t incl = v;
-#pragma unroll
-    for (int off = 1; off < FGS_WAVE; off <<= 1) {
-        const uint32_t n = __shfl_up(incl, off, FGS_WAVE);
-        if (lane >= off) incl = max(incl, n);
-    }
+    const uint32_t incl = wave_incl_max_u32(v);
     uint32_t excl = __shfl_up(incl, 1, FGS_WAVE);
     if (lane == 0) excl = 0u;
     if (lane == FGS_WAVE - 1) s_wave[wave] = incl;
