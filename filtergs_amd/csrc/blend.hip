@@ -1887,7 +1887,9 @@ __global__ __launch_bounds__(FGS_WAVE) void k_grad_gather_elems(GradGather gg, c
                 for (int e = 0; e < kGradTerms; ++e) x[e] = part[(size_t)kGradTerms * p + e];
             }
             wave_segmented_sum<kGradTerms>(x, st != 0u);
-            const bool next_head = __shfl_down(st, 1, FGS_WAVE) != 0u;
+            // lane + 1's start flag (DPP wave_shl:1; lane 63 reads 0 and is a
+            // tail anyway)
+            const bool next_head = __builtin_amdgcn_update_dpp(0, (int)st, 0x130, 0xF, 0xF, true) != 0;
             const bool tail = act && (lane == FGS_WAVE - 1 || idx + 1 == total || next_head);
             if (tail) {
 #pragma unroll
