@@ -1966,7 +1966,7 @@ hipError_t blend_bwd_and_gather(const BlendArgs& a, const GradGather& gg, float*
     if ((e = hipGetLastError()) != hipSuccess || gg.n == 0) return e;
     if (gg.slot_base) {   // batch: one wave per 64 visible elements
         size_t blocks = (gg.n + FGS_WAVE - 1) / FGS_WAVE;
-        if (blocks > 4 * kStrideBlocks) blocks = 4 * kStrideBlocks;
+        if (blocks > 16 * kStrideBlocks) blocks = 16 * kStrideBlocks;
         hipLaunchKernelGGL(k_grad_gather_elems, dim3((unsigned)blocks), dim3(FGS_WAVE), 0, s, gg, a.part, a.mark,
                            a.part_cap, dL_dmean2D, dL_dconic, dL_dopacity, dL_dcolors);
         return hipGetLastError();
