@@ -630,6 +630,7 @@ hipError_t launch_ranges(const uint64_t* sorted, size_t K, uint2* ranges, int T,
 // contiguous and the sort needs only the bits of the largest view's tile
 // count.
 // ---------------------------------------------------------------------------
+constexpr int kRecPerThread = 4;
 __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __restrict__ vkeys,
                                                               const uint4* __restrict__ recs, size_t n,
                                                               const unsigned long long* __restrict__ n_dev,
@@ -637,16 +638,33 @@ __global__ __launch_bounds__(256) void k_sorted_records_batch(const uint64_t* __
                                                               uint32_t* __restrict__ tiles_s,
                                                               uint2* __restrict__ rects_s)
 {
-    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= live_count(n, n_dev)) return;
-    // only visible (view, Gaussian) keys reach the sort
-    const uint64_t key = vkeys[s];
-    const uint4 r = recs[batch_elem(key, batch_p64(P))];
-    const uint32_t gx = (uint32_t)views[(uint32_t)(key >> kBatchViewShift)].gx;
-    const uint32_t x0 = r.x & 0xFFFFu, y0 = r.x >> 16;
-    tiles_s[s] = r.z;
-    // the rectangle as its first tile and row width | view row stride << 16
-    rects_s[s] = make_uint2(y0 * gx + x0, ((r.y & 0xFFFFu) - x0) | (gx << 16));
+    // kRecPerThread keys per thread, a block's 256 apart (coalesced key loads
+    // and stores): their record gathers are independent, so each wave has
+    // that many random loads in flight instead of one
+    const size_t live = live_count(n, n_dev);
+    const size_t s0 = (size_t)blockIdx.x * (kRecPerThread * 256) + threadIdx.x;
+    if (s0 >= live) return;
+    const uint32_t P64 = batch_p64(P);
+    uint64_t key[kRecPerThread];
+    uint4 r[kRecPerThread];
+#pragma unroll
+    for (int i = 0; i < kRecPerThread; ++i) {
+        const size_t s = s0 + (size_t)i * 256;
+        key[i] = s < live ? vkeys[s] : 0ull;   // only visible (view, Gaussian) keys reach the sort
+    }
+#pragma unroll
+    for (int i = 0; i < kRecPerThread; ++i)
+        if (s0 + (size_t)i * 256 < live) r[i] = recs[batch_elem(key[i], P64)];
+#pragma unroll
+    for (int i = 0; i < kRecPerThread; ++i) {
+        const size_t s = s0 + (size_t)i * 256;
+        if (s >= live) break;
+        const uint32_t gx = (uint32_t)views[(uint32_t)(key[i] >> kBatchViewShift)].gx;
+        const uint32_t x0 = r[i].x & 0xFFFFu, y0 = r[i].x >> 16;
+        tiles_s[s] = r[i].z;
+        // the rectangle as its first tile and row width | view row stride << 16
+        rects_s[s] = make_uint2(y0 * gx + x0, ((r[i].y & 0xFFFFu) - x0) | (gx << 16));
+    }
 }
 
 hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs, size_t n,
@@ -654,7 +672,8 @@ hipError_t launch_sorted_records_batch(const uint64_t* vkeys, const uint4* recs,
                                        uint32_t* tiles_s, uint2* rects_s, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sorted_records_batch, dim3((unsigned)div_up(n, 256)), dim3(256), 0, s, vkeys,
+    hipLaunchKernelGGL(k_sorted_records_batch, dim3((unsigned)div_up(n, (size_t)kRecPerThread * 256)), dim3(256), 0, s,
+                       vkeys,
                        recs, n, n_dev, P, views, tiles_s, rects_s);
     return hipGetLastError();
 }
