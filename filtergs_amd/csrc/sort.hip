@@ -1153,32 +1153,40 @@ hipError_t launch_duplicate_sort_batch(const uint64_t* vkeys, const uint32_t* of
 // tile keys; it is stored at the view's tile_base + tile.  Only the first
 // min(cap, *K) sorted pairs exist.  Each thread takes four consecutive keys
 // (one 16-byte load) and their two neighbours.
+constexpr int kRangeKeys = 8;   // a multiple of 4
 __global__ __launch_bounds__(256) void k_ranges_batch(const uint32_t* __restrict__ sorted, size_t cap,
                                                       const unsigned long long* __restrict__ K,
                                                       const BatchView* __restrict__ views,
                                                       uint2* __restrict__ ranges)
 {
     const size_t n = live_count(cap, K);
-    const size_t k0 = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const size_t k0 = (size_t)kRangeKeys * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (k0 >= n) return;
-    uint32_t c[4];
-    if (k0 + 4 <= n) {
-        const uint4 q = *reinterpret_cast<const uint4*>(sorted + k0);
-        c[0] = q.x;
-        c[1] = q.y;
-        c[2] = q.z;
-        c[3] = q.w;
+    // kRangeKeys keys per thread in 16-byte loads issued together (more bytes
+    // in flight per wave)
+    uint32_t c[kRangeKeys];
+    if (k0 + kRangeKeys <= n) {
+        uint4 q[kRangeKeys / 4];
+#pragma unroll
+        for (int h = 0; h < kRangeKeys / 4; ++h) q[h] = reinterpret_cast<const uint4*>(sorted + k0)[h];
+#pragma unroll
+        for (int h = 0; h < kRangeKeys / 4; ++h) {
+            c[4 * h] = q[h].x;
+            c[4 * h + 1] = q[h].y;
+            c[4 * h + 2] = q[h].z;
+            c[4 * h + 3] = q[h].w;
+        }
     } else {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) c[i] = k0 + i < n ? sorted[k0 + i] : 0u;
+        for (int i = 0; i < kRangeKeys; ++i) c[i] = k0 + i < n ? sorted[k0 + i] : 0u;
     }
     uint32_t prev = k0 > 0 ? sorted[k0 - 1] : 0u;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < kRangeKeys; ++i) {
         const size_t k = k0 + i;
         if (k >= n) break;
         const uint32_t key = c[i];
-        const uint32_t next = i < 3 ? c[i + 1] : (k + 1 < n ? sorted[k + 1] : 0u);
+        const uint32_t next = i < kRangeKeys - 1 ? c[i + 1] : (k + 1 < n ? sorted[k + 1] : 0u);
         const bool first = k == 0 || prev != key;
         const bool last = k == n - 1 || next != key;
         prev = key;
@@ -1195,7 +1203,8 @@ hipError_t launch_ranges_batch(const uint32_t* sorted_keys, size_t cap, const un
 {
     hipError_t e = launch_fill_u32(ranges, 0u, (size_t)T * 2, s);
     if (e != hipSuccess || cap == 0) return e;
-    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, 4 * 256)), dim3(256), 0, s, sorted_keys, cap, K,
+    hipLaunchKernelGGL(k_ranges_batch, dim3((unsigned)div_up(cap, (size_t)kRangeKeys * 256)), dim3(256), 0, s,
+                       sorted_keys, cap, K,
                        views, ranges);
     return hipGetLastError();
 }
